@@ -1,0 +1,1289 @@
+/*
+ * CPU oracle: a restatement of the reference substep (js/solver.js) and the
+ * marching-tetrahedra mesher (js/surface.js) of johnnybuggy/watersim-v1.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h): the checker for the HIP library,
+ * never part of it.
+ *
+ * Every stage cites the reference lines it follows. Locals are double and
+ * array stores round to OC_REAL, reproducing JavaScript's Float32Array
+ * semantics when OC_REAL is float. Sphere-world mode only.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+enum { AIR = 0, FLUID = 1, SOLID = 2 };
+
+struct oc_ctx {
+  int nx, ny, nz;
+  double dx, W, H, D, cx, cy, cz, domainR;
+  long nC, uN, vN, wN;
+  /* tunables (solver.js:56-75, 145-193) */
+  double cfl, pic, gravity, viscosity, omega, maxSpeed, vorticity;
+  int pressureIters;
+  double coreR, oceanR, atmosphereH, spacing, particleVolume;
+  double gRamp, dtLast;
+  double rainT, cloudT, rainLiftAge;
+  int ceilReflect; /* 0 linear, 1 quadratic, 2 exponential */
+  int hasSun;
+  double sunX, sunY, sunZ;
+  uint32_t cfS, bfS, evS;
+  /* terrain (solver.js:499-504) */
+  int tn;
+  double dv;
+  const uint8_t *tsolid;
+  const float *tR;
+  const uint8_t *staticTypes;
+  const oc_real *gu, *gv, *gw;
+  /* grid */
+  uint8_t *cellType;
+  oc_real *u, *v, *w, *uW, *vW, *wW, *uO, *vO, *wO;
+  uint8_t *validU, *validV, *validW;
+  oc_real *q, *rhs, *diag, *cellPhi, *dens;
+  /* particles */
+  int nP;
+  oc_real *px, *py, *pz, *pvx, *pvy, *pvz, *pT, *pAir, *pWx, *pWy, *pWz;
+  uint8_t *pflag, *pcool;
+  double *advU, *advV, *advW;
+  /* outputs */
+  int airborne, nFluid;
+  double umax;
+  /* private scratch */
+  int *fluidList, *pStart, *pAdj;
+  float *pW;
+  float *vcx, *vcy, *vcz, *omg, *omx, *omy, *omz;
+};
+
+static double clampd(double v, double a, double b) { return v < a ? a : (v > b ? b : v); }
+static int clampi(int v, int a, int b) { return v < a ? a : (v > b ? b : v); }
+
+oc_ctx *oc_create(int nx, int ny, int nz, double dx) {
+  oc_ctx *c = (oc_ctx *)calloc(1, sizeof(oc_ctx));
+  /* solver.js:50-91 */
+  c->nx = nx; c->ny = ny; c->nz = nz; c->dx = dx;
+  c->W = nx * dx; c->H = ny * dx; c->D = nz * dx;
+  c->cx = c->W / 2; c->cy = c->H / 2; c->cz = c->D / 2;
+  {
+    double m = c->W < c->H ? c->W : c->H;
+    m = m < c->D ? m : c->D;
+    c->domainR = 0.5 * m - dx;
+  }
+  c->nC = (long)nx * ny * nz;
+  c->uN = (long)(nx + 1) * ny * nz;
+  c->vN = (long)nx * (ny + 1) * nz;
+  c->wN = (long)nx * ny * (nz + 1);
+  c->cfl = 3.0; c->pic = 0.15; c->gravity = 9.81; c->viscosity = 5e-4;
+  c->pressureIters = 20; c->omega = 1.55; c->maxSpeed = 3.2;
+  c->atmosphereH = 0.35; c->spacing = dx * 0.5;
+  c->particleVolume = c->spacing * c->spacing * c->spacing;
+  c->gRamp = 1e9;
+  c->rainT = 0.22; c->cloudT = 0.34; c->rainLiftAge = 0.4;
+  c->cfS = 0x2f6e2b1u; c->bfS = 0x7f4a7c15u; c->evS = 0x51ab3c77u;
+  c->fluidList = (int *)malloc(sizeof(int) * c->nC);
+  c->pStart = (int *)malloc(sizeof(int) * (c->nC + 1));
+  c->pAdj = (int *)malloc(sizeof(int) * 6 * c->nC);
+  c->pW = (float *)malloc(sizeof(float) * c->nC);
+  c->vcx = (float *)malloc(sizeof(float) * c->nC * 7);
+  c->vcy = c->vcx + c->nC; c->vcz = c->vcy + c->nC; c->omg = c->vcz + c->nC;
+  c->omx = c->omg + c->nC; c->omy = c->omx + c->nC; c->omz = c->omy + c->nC;
+  return c;
+}
+
+void oc_destroy(oc_ctx *c) {
+  if (!c) return;
+  free(c->fluidList); free(c->pStart); free(c->pAdj); free(c->pW); free(c->vcx);
+  free(c);
+}
+
+int oc_real_size(void) { return (int)sizeof(oc_real); }
+
+#define PTR(nm, ty) if (!strcmp(name, #nm)) { c->nm = (ty)ptr; return 0; }
+int oc_set_ptr(oc_ctx *c, const char *name, void *ptr) {
+  PTR(tsolid, const uint8_t *) PTR(tR, const float *) PTR(staticTypes, const uint8_t *)
+  PTR(gu, const oc_real *) PTR(gv, const oc_real *) PTR(gw, const oc_real *)
+  PTR(cellType, uint8_t *) PTR(u, oc_real *) PTR(v, oc_real *) PTR(w, oc_real *)
+  PTR(uW, oc_real *) PTR(vW, oc_real *) PTR(wW, oc_real *)
+  PTR(uO, oc_real *) PTR(vO, oc_real *) PTR(wO, oc_real *)
+  PTR(validU, uint8_t *) PTR(validV, uint8_t *) PTR(validW, uint8_t *)
+  PTR(q, oc_real *) PTR(rhs, oc_real *) PTR(diag, oc_real *) PTR(cellPhi, oc_real *)
+  PTR(dens, oc_real *)
+  PTR(px, oc_real *) PTR(py, oc_real *) PTR(pz, oc_real *)
+  PTR(pvx, oc_real *) PTR(pvy, oc_real *) PTR(pvz, oc_real *)
+  PTR(pT, oc_real *) PTR(pAir, oc_real *)
+  PTR(pWx, oc_real *) PTR(pWy, oc_real *) PTR(pWz, oc_real *)
+  PTR(pflag, uint8_t *) PTR(pcool, uint8_t *)
+  PTR(advU, double *) PTR(advV, double *) PTR(advW, double *)
+  return -1;
+}
+#undef PTR
+
+#define FLD(nm) if (!strcmp(name, #nm)) { c->nm = v; return 0; }
+int oc_set_f(oc_ctx *c, const char *name, double v) {
+  FLD(cfl) FLD(pic) FLD(gravity) FLD(viscosity) FLD(omega) FLD(maxSpeed) FLD(vorticity)
+  FLD(coreR) FLD(oceanR) FLD(atmosphereH) FLD(spacing) FLD(particleVolume)
+  FLD(gRamp) FLD(dtLast) FLD(rainT) FLD(cloudT) FLD(rainLiftAge) FLD(dv)
+  FLD(sunX) FLD(sunY) FLD(sunZ) FLD(umax)
+  return -1;
+}
+int oc_set_i(oc_ctx *c, const char *name, long long v) {
+  FLD(pressureIters) FLD(nP) FLD(tn) FLD(ceilReflect) FLD(hasSun)
+  FLD(cfS) FLD(bfS) FLD(evS)
+  return -1;
+}
+#undef FLD
+#define GET(nm) if (!strcmp(name, #nm)) return c->nm;
+double oc_get_f(oc_ctx *c, const char *name) {
+  GET(umax) GET(gRamp) GET(dtLast) GET(domainR) GET(cx) GET(cy) GET(cz) GET(W)
+  GET(oceanR) GET(spacing) GET(particleVolume)
+  return NAN;
+}
+long long oc_get_i(oc_ctx *c, const char *name) {
+  GET(airborne) GET(nFluid) GET(nP) GET(cfS) GET(bfS) GET(evS)
+  return -1;
+}
+#undef GET
+
+/* ------------------------------------------------------------ samplers
+ * Trilinear face sampling, solver.js:750-783 (same clamps as _faceCoords
+ * :737-748 and the fused _sampleVel3 :820-854). Offsets per component:
+ * u (0,.5,.5), v (.5,0,.5), w (.5,.5,0). The i clamp is nx-1 for every
+ * component and j,k clamp at n-2, exactly as the reference writes them. */
+typedef struct { long idx, sj, sk; double s0, s1, t0, t1, r0, r1; } oc_stencil;
+
+static void face_stencil(const oc_ctx *c, int comp, double x, double y, double z, oc_stencil *st) {
+  double dx = c->dx, a, b, cc, tx, ty, tz;
+  int i0, j0, k0, nx = c->nx, ny = c->ny, nz = c->nz;
+  if (comp == 0) { a = x / dx; b = y / dx - 0.5; cc = z / dx - 0.5; }
+  else if (comp == 1) { a = x / dx - 0.5; b = y / dx; cc = z / dx - 0.5; }
+  else { a = x / dx - 0.5; b = y / dx - 0.5; cc = z / dx; }
+  i0 = (int)floor(a); j0 = (int)floor(b); k0 = (int)floor(cc);
+  if (i0 < 0) { i0 = 0; tx = 0; } else if (i0 > nx - 1) { i0 = nx - 1; tx = 1; } else tx = a - i0;
+  if (j0 < 0) { j0 = 0; ty = 0; } else if (j0 > ny - 2) { j0 = ny - 2; ty = 1; } else ty = b - j0;
+  if (k0 < 0) { k0 = 0; tz = 0; } else if (k0 > nz - 2) { k0 = nz - 2; tz = 1; } else tz = cc - k0;
+  if (comp == 0) { st->sj = nx + 1; st->sk = (long)ny * (nx + 1); st->idx = ((long)k0 * ny + j0) * (nx + 1) + i0; }
+  else if (comp == 1) { st->sj = nx; st->sk = (long)(ny + 1) * nx; st->idx = ((long)k0 * (ny + 1) + j0) * nx + i0; }
+  else { st->sj = nx; st->sk = (long)ny * nx; st->idx = ((long)k0 * ny + j0) * nx + i0; }
+  st->s0 = 1 - tx; st->s1 = tx; st->t0 = 1 - ty; st->t1 = ty; st->r0 = 1 - tz; st->r1 = tz;
+}
+
+static double stencil_eval(const oc_stencil *st, const oc_real *A) {
+  long i = st->idx, sj = st->sj, sk = st->sk;
+  double v00 = A[i] * st->s0 + A[i + 1] * st->s1;
+  double v10 = A[i + sj] * st->s0 + A[i + sj + 1] * st->s1;
+  double v01 = A[i + sk] * st->s0 + A[i + sk + 1] * st->s1;
+  double v11 = A[i + sk + sj] * st->s0 + A[i + sk + sj + 1] * st->s1;
+  return (v00 * st->t0 + v10 * st->t1) * st->r0 + (v01 * st->t0 + v11 * st->t1) * st->r1;
+}
+
+double oc_sample_face(oc_ctx *c, const oc_real *arr, int comp, double x, double y, double z) {
+  oc_stencil st;
+  face_stencil(c, comp, x, y, z, &st);
+  return stencil_eval(&st, arr);
+}
+
+void oc_sample_vel3(oc_ctx *c, double x, double y, double z, double *out) {
+  out[0] = oc_sample_face(c, c->u, 0, x, y, z);
+  out[1] = oc_sample_face(c, c->v, 1, x, y, z);
+  out[2] = oc_sample_face(c, c->w, 2, x, y, z);
+}
+
+/* ----------------------------------------------------- terrain predicates */
+/* terrainRadiusAt, solver.js:512-535: trilinear on the voxel radius lattice */
+double oc_terrain_radius_at(oc_ctx *c, double x, double y, double z) {
+  if (!c->tR) return c->coreR;
+  int n = c->tn;
+  double dv = c->dv;
+  double gx = x / dv - 0.5, gy = y / dv - 0.5, gz = z / dv - 0.5;
+  int i = (int)gx, j = (int)gy, k = (int)gz;
+  i = clampi(i, 0, n - 2); j = clampi(j, 0, n - 2); k = clampi(k, 0, n - 2);
+  double fx = clampd(gx - i, 0, 1), fy = clampd(gy - j, 0, 1), fz = clampd(gz - k, 0, 1);
+  long b = ((long)k * n + j) * n + i, s7 = (long)n * n;
+  const float *R = c->tR;
+  double v000 = R[b], v100 = R[b + 1], v010 = R[b + n], v110 = R[b + n + 1];
+  double v001 = R[b + s7], v101 = R[b + s7 + 1], v011 = R[b + s7 + n], v111 = R[b + s7 + n + 1];
+  double a0 = v000 + (v100 - v000) * fx, a1 = v010 + (v110 - v010) * fx;
+  double a2 = v001 + (v101 - v001) * fx, a3 = v011 + (v111 - v011) * fx;
+  double b0 = a0 + (a1 - a0) * fy, b1 = a2 + (a3 - a2) * fy;
+  return b0 + (b1 - b0) * fz;
+}
+
+/* _rockCellAt, solver.js:542-559. The cached static raster answers for
+ * every interior cell exactly as the voxel-at-centre rule does. */
+int oc_rock_cell_at(oc_ctx *c, double x, double y, double z) {
+  if (!c->tsolid) return 0;
+  double dx = c->dx;
+  int i = clampi((int)(x / dx), 1, c->nx - 2);
+  int j = clampi((int)(y / dx), 1, c->ny - 2);
+  int k = clampi((int)(z / dx), 1, c->nz - 2);
+  if (c->staticTypes) return c->staticTypes[((long)k * c->ny + j) * c->nx + i] == SOLID;
+  int n = c->tn;
+  int vx = clampi((int)((i + 0.5) * dx / c->dv), 0, n - 1);
+  int vy = clampi((int)((j + 0.5) * dx / c->dv), 0, n - 1);
+  int vz = clampi((int)((k + 0.5) * dx / c->dv), 0, n - 1);
+  return c->tsolid[((long)vz * n + vy) * n + vx] == 1;
+}
+
+/* _voxelRockAt, solver.js:565-574 */
+int oc_voxel_rock_at(oc_ctx *c, double x, double y, double z) {
+  if (!c->tsolid) return 0;
+  int n = c->tn;
+  int i = clampi((int)(x / c->dv), 0, n - 1);
+  int j = clampi((int)(y / c->dv), 0, n - 1);
+  int k = clampi((int)(z / c->dv), 0, n - 1);
+  return c->tsolid[((long)k * n + j) * n + i] == 1;
+}
+
+/* _onRockBelow, solver.js:581-587 */
+static int on_rock_below(oc_ctx *c, double x, double y, double z) {
+  if (!c->tsolid) return 0;
+  double ex = x - c->cx, ey = y - c->cy, ez = z - c->cz;
+  double el = sqrt(ex * ex + ey * ey + ez * ez);
+  if (!(el != 0)) el = 1e-9;
+  double s = c->dx * 0.9 / el;
+  return oc_rock_cell_at(c, x - ex * s, y - ey * s, z - ez * s);
+}
+
+/* ------------------------------------------------------------ rasterize */
+/* _rasterizeStatic, solver.js:865-916 (sphere mode, no balls): the static
+ * raster is built once per world and copied into cellType every substep. */
+void oc_rasterize_static(oc_ctx *c) {
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  if (c->staticTypes) { memcpy(c->cellType, c->staticTypes, c->nC); return; }
+  double dx = c->dx;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        int shell = (j == 0 || j == ny - 1 || k == 0 || k == nz - 1 || i == 0 || i == nx - 1);
+        int rock = 0;
+        if (c->tsolid) {
+          int n = c->tn;
+          int vx = clampi((int)((i + 0.5) * dx / c->dv), 0, n - 1);
+          int vy = clampi((int)((j + 0.5) * dx / c->dv), 0, n - 1);
+          int vz = clampi((int)((k + 0.5) * dx / c->dv), 0, n - 1);
+          rock = c->tsolid[((long)vz * n + vy) * n + vx] == 1;
+        } else {
+          double ddx = (i + 0.5) * dx - c->cx, ddy = (j + 0.5) * dx - c->cy, ddz = (k + 0.5) * dx - c->cz;
+          rock = ddx * ddx + ddy * ddy + ddz * ddz < c->coreR * c->coreR;
+        }
+        c->cellType[((long)k * ny + j) * nx + i] = (shell || rock) ? SOLID : AIR;
+      }
+}
+
+/* _markFluidCells, solver.js:945-1000 (sphere mode) */
+void oc_mark_fluid(oc_ctx *c, int p0, int p1) {
+  double dx = c->dx;
+  int nx = c->nx, ny = c->ny, nz = c->nz, jTop = ny - 2;
+  double skyAlt = 2 * dx > 0.3 ? 2 * dx : 0.3;
+  double lim = (c->oceanR + 0.3) * (c->oceanR + 0.3);
+  for (int p = p0; p < p1; p++) {
+    if (c->pflag[p] != 0) continue;
+    double x = c->px[p], y = c->py[p], z = c->pz[p];
+    double rx = x - c->cx, ry = y - c->cy, rz = z - c->cz;
+    double r2 = rx * rx + ry * ry + rz * rz;
+    if (r2 > lim && sqrt(r2) - oc_terrain_radius_at(c, x, y, z) > skyAlt && !oc_rock_cell_at(c, x, y, z)) {
+      c->pflag[p] = 1;
+      continue;
+    }
+    int i = clampi((int)floor(x / dx), 0, nx - 1);
+    int j = clampi((int)floor(y / dx), 0, ny - 1);
+    int k = clampi((int)floor(z / dx), 0, nz - 1);
+    if (j >= jTop) {
+      if (sqrt(rx * rx + ry * ry + rz * rz) - oc_terrain_radius_at(c, x, y, z) > 0.3) {
+        c->pflag[p] = 1;
+        continue;
+      }
+    }
+    long idx = ((long)k * ny + j) * nx + i;
+    if (c->cellType[idx] == AIR) c->cellType[idx] = FLUID;
+  }
+}
+
+/* ------------------------------------------------------------------ P2G */
+void oc_clear_faces(oc_ctx *c) {
+  /* solver.js:2569-2570 */
+  for (long i = 0; i < c->uN; i++) { c->u[i] = 0; c->uW[i] = 0; }
+  for (long i = 0; i < c->vN; i++) { c->v[i] = 0; c->vW[i] = 0; }
+  for (long i = 0; i < c->wN; i++) { c->w[i] = 0; c->wW[i] = 0; }
+}
+
+/* one component of _p2gInto, solver.js:1046-1104: trilinear splat of
+ * momentum and weight, accumulated sequentially in particle order */
+static void splat_comp(oc_ctx *c, int comp, double x, double y, double z, double vel, oc_real *A, oc_real *Wt) {
+  oc_stencil st;
+  face_stencil(c, comp, x, y, z, &st);
+  double wA = st.s0 * st.t0, wB = st.s1 * st.t0, wC = st.s0 * st.t1, wD = st.s1 * st.t1;
+  long i = st.idx, sj = st.sj, sk = st.sk;
+  long off[8] = { i, i + 1, i + sj, i + sj + 1, i + sk, i + sk + 1, i + sk + sj, i + sk + sj + 1 };
+  double wx[8] = { wA * st.r0, wB * st.r0, wC * st.r0, wD * st.r0, wA * st.r1, wB * st.r1, wC * st.r1, wD * st.r1 };
+  for (int e = 0; e < 8; e++) {
+    A[off[e]] = (oc_real)(A[off[e]] + wx[e] * vel);
+    Wt[off[e]] = (oc_real)(Wt[off[e]] + wx[e]);
+  }
+}
+
+/* _p2g = _p2gInto + _p2gNormalizeFace, solver.js:1030-1120 */
+void oc_p2g(oc_ctx *c) {
+  for (int p = 0; p < c->nP; p++) {
+    if (c->pflag[p] != 0) continue;
+    double x = c->px[p], y = c->py[p], z = c->pz[p];
+    splat_comp(c, 0, x, y, z, c->pvx[p], c->u, c->uW);
+    splat_comp(c, 1, x, y, z, c->pvy[p], c->v, c->vW);
+    splat_comp(c, 2, x, y, z, c->pvz[p], c->w, c->wW);
+  }
+  for (long i = 0; i < c->uN; i++) if (c->uW[i] > 1e-8) c->u[i] = (oc_real)((double)c->u[i] / c->uW[i]);
+  for (long i = 0; i < c->vN; i++) if (c->vW[i] > 1e-8) c->v[i] = (oc_real)((double)c->v[i] / c->vW[i]);
+  for (long i = 0; i < c->wN; i++) if (c->wW[i] > 1e-8) c->w[i] = (oc_real)((double)c->w[i] / c->wW[i]);
+}
+
+/* ------------------------------------------------- boundary conditions */
+/* _applyBCSlice, solver.js:1130-1171, no moving solids (sv = 0) */
+void oc_apply_bc(oc_ctx *c) {
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  const uint8_t *t = c->cellType;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++) {
+      c->u[((long)k * ny + j) * (nx + 1)] = 0;
+      c->u[((long)k * ny + j) * (nx + 1) + nx] = 0;
+    }
+  for (int k = 0; k < nz; k++)
+    for (int i = 0; i < nx; i++) {
+      c->v[((long)k * (ny + 1)) * nx + i] = 0;
+      c->v[((long)k * (ny + 1) + ny) * nx + i] = 0;
+    }
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      c->w[(long)j * nx + i] = 0;
+      c->w[((long)nz * ny + j) * nx + i] = 0;
+    }
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 1; i < nx; i++) {
+        long cl = ((long)k * ny + j) * nx + i - 1;
+        if (t[cl] == SOLID || t[cl + 1] == SOLID) c->u[((long)k * ny + j) * (nx + 1) + i] = 0;
+      }
+  for (int k = 0; k < nz; k++)
+    for (int j = 1; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        long cr = ((long)k * ny + j) * nx + i;
+        if (t[cr - nx] == SOLID || t[cr] == SOLID) c->v[((long)k * (ny + 1) + j) * nx + i] = 0;
+      }
+  for (int k = 1; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        long cr = ((long)k * ny + j) * nx + i;
+        if (t[cr - (long)nx * ny] == SOLID || t[cr] == SOLID) c->w[cr] = 0;
+      }
+}
+
+/* FLIP snapshot, solver.js:2579 */
+void oc_snapshot(oc_ctx *c) {
+  memcpy(c->uO, c->u, sizeof(oc_real) * c->uN);
+  memcpy(c->vO, c->v, sizeof(oc_real) * c->vN);
+  memcpy(c->wO, c->w, sizeof(oc_real) * c->wN);
+}
+
+/* ------------------------------------------------------------ viscosity */
+/* _viscosity, solver.js:1259-1304: in place, lexicographic, fluid-fluid
+ * faces only, u += coef (6u - sum of the six same-component neighbours),
+ * a missing neighbour reads the face itself. */
+static void visc_comp(oc_real *A, int d0, int d1, int d2, int comp, const oc_ctx *c, double coef) {
+  int nx = c->nx, ny = c->ny;
+  const uint8_t *t = c->cellType;
+  long s1 = d0, s2 = (long)d0 * d1;
+  for (int k = (comp == 2 ? 1 : 0); k < (comp == 2 ? d2 - 1 : d2); k++)
+    for (int j = (comp == 1 ? 1 : 0); j < (comp == 1 ? d1 - 1 : d1); j++)
+      for (int i = (comp == 0 ? 1 : 0); i < (comp == 0 ? d0 - 1 : d0); i++) {
+        /* the face sits between cell cr (its + side) and cl (its - side) */
+        long cr = ((long)k * ny + j) * nx + i;
+        long cl = comp == 0 ? cr - 1 : comp == 1 ? cr - nx : cr - (long)nx * ny;
+        if (t[cl] != FLUID || t[cr] != FLUID) continue;
+        long idx = ((long)k * d1 + j) * d0 + i;
+        double val = A[idx] * 6.0;
+        val -= (i - 1 >= 0) ? A[idx - 1] : A[idx];
+        val -= (i + 1 < d0) ? A[idx + 1] : A[idx];
+        val -= (j - 1 >= 0) ? A[idx - s1] : A[idx];
+        val -= (j + 1 < d1) ? A[idx + s1] : A[idx];
+        val -= (k - 1 >= 0) ? A[idx - s2] : A[idx];
+        val -= (k + 1 < d2) ? A[idx + s2] : A[idx];
+        A[idx] = (oc_real)(A[idx] + coef * val);
+      }
+}
+
+void oc_viscosity(oc_ctx *c, double dt) {
+  double nu = c->viscosity;
+  if (nu <= 1e-6) return;
+  double coef = nu * dt / (c->dx * c->dx);
+  if (coef > 0.24) coef = 0.24;
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  visc_comp(c->u, nx + 1, ny, nz, 0, c, coef);
+  visc_comp(c->v, nx, ny + 1, nz, 1, c, coef);
+  visc_comp(c->w, nx, ny, nz + 1, 2, c, coef);
+}
+
+/* -------------------------------------------------------- planet gravity */
+/* _applyPlanetGravity (the definition that wins, solver.js:2656-2662) and
+ * _applyPlanetGravitySlice, solver.js:1229-1256 */
+void oc_apply_gravity(oc_ctx *c, double dt) {
+  if (!c->gu) return;
+  double gs = c->gRamp / 1.5;
+  if (gs > 1) gs = 1;
+  c->gRamp += dt;
+  if (gs <= 0) return;
+  gs *= c->gravity;
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  const uint8_t *t = c->cellType;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 1; i < nx; i++) {
+        long cr = ((long)k * ny + j) * nx + i;
+        if (t[cr - 1] == FLUID || t[cr] == FLUID) {
+          long idx = ((long)k * ny + j) * (nx + 1) + i;
+          c->u[idx] = (oc_real)(c->u[idx] + c->gu[idx] * dt * gs);
+        }
+      }
+  for (int k = 0; k < nz; k++)
+    for (int j = 1; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        long cr = ((long)k * ny + j) * nx + i;
+        if (t[cr - nx] == FLUID || t[cr] == FLUID) {
+          long idx = ((long)k * (ny + 1) + j) * nx + i;
+          c->v[idx] = (oc_real)(c->v[idx] + c->gv[idx] * dt * gs);
+        }
+      }
+  for (int k = 1; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        long cr = ((long)k * ny + j) * nx + i;
+        if (t[cr - (long)nx * ny] == FLUID || t[cr] == FLUID)
+          c->w[cr] = (oc_real)(c->w[cr] + c->gw[cr] * dt * gs);
+      }
+}
+
+/* -------------------------------------------------------- pressure solve */
+/* _pressureSolve passes 1-2 and the CSR build, solver.js:1312-1395.
+ * Returns the number of fluid cells. */
+int oc_pressure_assemble(oc_ctx *c) {
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  long nxy = (long)nx * ny;
+  const uint8_t *t = c->cellType;
+  int nF = 0;
+  for (long cc = 0; cc < c->nC; cc++) {
+    if (t[cc] != FLUID) { c->q[cc] = 0; continue; }
+    c->fluidList[nF++] = (int)cc;
+  }
+  c->nFluid = nF;
+  int na = 0;
+  for (int fi = 0; fi < nF; fi++) {
+    long cc = c->fluidList[fi];
+    int i = (int)(cc % nx), j = (int)((cc / nx) % ny), k = (int)(cc / nxy);
+    long nb[6] = { i > 0 ? cc - 1 : -1, i < nx - 1 ? cc + 1 : -1, j > 0 ? cc - nx : -1,
+                   j < ny - 1 ? cc + nx : -1, k > 0 ? cc - nxy : -1, k < nz - 1 ? cc + nxy : -1 };
+    int Nf = 0, Na = 0;
+    c->pStart[fi] = na;
+    for (int d = 0; d < 6; d++) {
+      if (nb[d] < 0) continue;
+      if (t[nb[d]] == FLUID) { c->pAdj[na++] = (int)nb[d]; Nf++; }
+      else if (t[nb[d]] == AIR) Na++;
+    }
+    c->diag[cc] = (oc_real)(Nf + Na);
+    long iu = ((long)k * ny + j) * (nx + 1) + i;
+    long iv = ((long)k * (ny + 1) + j) * nx + i;
+    long iw = ((long)k * ny + j) * nx + i;
+    double div = ((double)c->u[iu + 1] - c->u[iu] + c->v[iv + nx] - c->v[iv] + c->w[iw + nxy] - c->w[iw]) / c->dx;
+    c->rhs[cc] = (oc_real)(-c->dx * c->dx * div);
+    double dg = c->diag[cc];
+    c->pW[fi] = (float)(dg > 0 ? c->omega / dg : 0);
+  }
+  c->pStart[nF] = na;
+  return nF;
+}
+
+/* SOR Gauss-Seidel sweeps in the reference's lexicographic order,
+ * solver.js:1397-1404 */
+void oc_sor_lex(oc_ctx *c, int iters) {
+  double om1 = 1 - c->omega;
+  for (int it = 0; it < iters; it++)
+    for (int fi = 0; fi < c->nFluid; fi++) {
+      double sum = 0;
+      for (int a = c->pStart[fi]; a < c->pStart[fi + 1]; a++) sum += c->q[c->pAdj[a]];
+      long cc = c->fluidList[fi];
+      c->q[cc] = (oc_real)(c->q[cc] * om1 + (double)c->pW[fi] * (c->rhs[cc] + sum));
+    }
+}
+
+/* The same relaxation (solver.js:1402) in red-black order: cells with
+ * (i+j+k) even first, then odd. One "iteration" relaxes both colours. */
+void oc_sor_redblack(oc_ctx *c, int iters) {
+  double om1 = 1 - c->omega;
+  int nx = c->nx, ny = c->ny;
+  long nxy = (long)nx * ny;
+  for (int it = 0; it < iters; it++)
+    for (int color = 0; color < 2; color++)
+      for (int fi = 0; fi < c->nFluid; fi++) {
+        long cc = c->fluidList[fi];
+        int par = (int)((cc % nx) + ((cc / nx) % ny) + (cc / nxy)) & 1;
+        if (par != color) continue;
+        double sum = 0;
+        for (int a = c->pStart[fi]; a < c->pStart[fi + 1]; a++) sum += c->q[c->pAdj[a]];
+        c->q[cc] = (oc_real)(c->q[cc] * om1 + (double)c->pW[fi] * (c->rhs[cc] + sum));
+      }
+}
+
+/* Relative residual ||rhs - A q||_2 / ||rhs||_2 of the assembled system
+ * (A q)_c = diag_c q_c - sum of fluid neighbours, over fluid cells, in FP64. */
+double oc_residual(oc_ctx *c, double *rhs_norm) {
+  double rr = 0, bb = 0;
+  for (int fi = 0; fi < c->nFluid; fi++) {
+    long cc = c->fluidList[fi];
+    double sum = 0;
+    for (int a = c->pStart[fi]; a < c->pStart[fi + 1]; a++) sum += c->q[c->pAdj[a]];
+    double r = (double)c->rhs[cc] - ((double)c->diag[cc] * c->q[cc] - sum);
+    rr += r * r;
+    bb += (double)c->rhs[cc] * c->rhs[cc];
+  }
+  if (rhs_norm) *rhs_norm = sqrt(bb);
+  return bb > 0 ? sqrt(rr) / sqrt(bb) : 0;
+}
+
+void oc_pressure_finish(oc_ctx *c) {
+  memcpy(c->cellPhi, c->q, sizeof(oc_real) * c->nC); /* solver.js:1409 */
+}
+
+void oc_pressure_solve(oc_ctx *c) {
+  int nF = oc_pressure_assemble(c);
+  if (nF > 0) oc_sor_lex(c, c->pressureIters);
+  oc_pressure_finish(c);
+}
+
+/* ------------------------------------------------------------- project */
+/* _projectSlice, solver.js:1417-1443 */
+void oc_project(oc_ctx *c) {
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  double inv = 1 / c->dx;
+  const uint8_t *t = c->cellType;
+  const oc_real *phi = c->cellPhi;
+  long nxy = (long)nx * ny;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 1; i < nx; i++) {
+        long cr = ((long)k * ny + j) * nx + i, cl = cr - 1;
+        if (t[cl] == SOLID || t[cr] == SOLID) continue;
+        double pL = t[cl] == FLUID ? phi[cl] : 0, pR = t[cr] == FLUID ? phi[cr] : 0;
+        long idx = ((long)k * ny + j) * (nx + 1) + i;
+        c->u[idx] = (oc_real)(c->u[idx] - (pR - pL) * inv);
+      }
+  for (int k = 0; k < nz; k++)
+    for (int j = 1; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        long cr = ((long)k * ny + j) * nx + i, cl = cr - nx;
+        if (t[cl] == SOLID || t[cr] == SOLID) continue;
+        double pL = t[cl] == FLUID ? phi[cl] : 0, pR = t[cr] == FLUID ? phi[cr] : 0;
+        long idx = ((long)k * (ny + 1) + j) * nx + i;
+        c->v[idx] = (oc_real)(c->v[idx] - (pR - pL) * inv);
+      }
+  for (int k = 1; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        long cr = ((long)k * ny + j) * nx + i, cl = cr - nxy;
+        if (t[cl] == SOLID || t[cr] == SOLID) continue;
+        double pL = t[cl] == FLUID ? phi[cl] : 0, pR = t[cr] == FLUID ? phi[cr] : 0;
+        c->w[cr] = (oc_real)(c->w[cr] - (pR - pL) * inv);
+      }
+}
+
+/* ------------------------------------------------- vorticity confinement */
+/* _vorticityConfinement, solver.js:1454-1559. Scratch is float, as the
+ * reference's Float32Array scratch is. */
+void oc_vorticity(oc_ctx *c, double dt) {
+  double eps = c->vorticity;
+  if (!(eps > 0)) return;
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  long nxy = (long)nx * ny;
+  const uint8_t *t = c->cellType;
+  float *vx = c->vcx, *vy = c->vcy, *vz = c->vcz, *om = c->omg, *ox = c->omx, *oy = c->omy, *oz = c->omz;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        long cc = ((long)k * ny + j) * nx + i;
+        long ru = ((long)k * ny + j) * (nx + 1) + i, rv = ((long)k * (ny + 1) + j) * nx + i;
+        vx[cc] = (float)(0.5 * ((double)c->u[ru] + c->u[ru + 1]));
+        vy[cc] = (float)(0.5 * ((double)c->v[rv] + c->v[rv + nx]));
+        vz[cc] = (float)(0.5 * ((double)c->w[cc] + c->w[cc + nxy]));
+      }
+  double inv2 = 0.5 / c->dx;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        long cc = ((long)k * ny + j) * nx + i;
+        long im = i > 0 ? cc - 1 : cc, ip = i < nx - 1 ? cc + 1 : cc;
+        long jm = j > 0 ? cc - nx : cc, jp = j < ny - 1 ? cc + nx : cc;
+        long km = k > 0 ? cc - nxy : cc, kp = k < nz - 1 ? cc + nxy : cc;
+        double wx = ((double)vz[jp] - vz[jm] - ((double)vy[kp] - vy[km])) * inv2;
+        double wy = ((double)vx[kp] - vx[km] - ((double)vz[ip] - vz[im])) * inv2;
+        double wz = ((double)vy[ip] - vy[im] - ((double)vx[jp] - vx[jm])) * inv2;
+        ox[cc] = (float)wx; oy[cc] = (float)wy; oz[cc] = (float)wz;
+        om[cc] = (float)sqrt(wx * wx + wy * wy + wz * wz);
+      }
+  const double fCap = 6.0, omFloor = 0.35;
+  for (int fi = 0; fi < c->nFluid; fi++) {
+    long cc = c->fluidList[fi];
+    int i = (int)(cc % nx), j = (int)((cc / nx) % ny), k = (int)(cc / nxy);
+    if (om[cc] < omFloor || (i > 0 && t[cc - 1] != FLUID) || (i < nx - 1 && t[cc + 1] != FLUID) ||
+        (j > 0 && t[cc - nx] != FLUID) || (j < ny - 1 && t[cc + nx] != FLUID) ||
+        (k > 0 && t[cc - nxy] != FLUID) || (k < nz - 1 && t[cc + nxy] != FLUID)) {
+      vx[cc] = 0; vy[cc] = 0; vz[cc] = 0;
+      continue;
+    }
+    double gx = (double)om[i < nx - 1 ? cc + 1 : cc] - om[i > 0 ? cc - 1 : cc];
+    double gy = (double)om[j < ny - 1 ? cc + nx : cc] - om[j > 0 ? cc - nx : cc];
+    double gz = (double)om[k < nz - 1 ? cc + nxy : cc] - om[k > 0 ? cc - nxy : cc];
+    double g2 = gx * gx + gy * gy + gz * gz;
+    if (g2 < 1e-12) { vx[cc] = 0; vy[cc] = 0; vz[cc] = 0; continue; }
+    double inv = 1 / sqrt(g2);
+    double Nx = gx * inv, Ny = gy * inv, Nz = gz * inv;
+    double fx = eps * (Ny * oz[cc] - Nz * oy[cc]);
+    double fy = eps * (Nz * ox[cc] - Nx * oz[cc]);
+    double fz = eps * (Nx * oy[cc] - Ny * ox[cc]);
+    double f2 = fx * fx + fy * fy + fz * fz;
+    if (f2 > fCap * fCap) { double fs = fCap / sqrt(f2); fx *= fs; fy *= fs; fz *= fs; }
+    vx[cc] = (float)(fx * dt); vy[cc] = (float)(fy * dt); vz[cc] = (float)(fz * dt);
+  }
+  for (int fi = 0; fi < c->nFluid; fi++) {
+    long cc = c->fluidList[fi];
+    int i = (int)(cc % nx), j = (int)((cc / nx) % ny), k = (int)(cc / nxy);
+    if (i < nx - 1 && t[cc + 1] == FLUID) {
+      long f = ((long)k * ny + j) * (nx + 1) + i + 1;
+      c->u[f] = (oc_real)(c->u[f] + 0.5 * ((double)vx[cc] + vx[cc + 1]));
+    }
+    if (j < ny - 1 && t[cc + nx] == FLUID) {
+      long f = ((long)k * (ny + 1) + j + 1) * nx + i;
+      c->v[f] = (oc_real)(c->v[f] + 0.5 * ((double)vy[cc] + vy[cc + nx]));
+    }
+    if (k < nz - 1 && t[cc + nxy] == FLUID) {
+      long f = ((long)(k + 1) * ny + j) * nx + i;
+      c->w[f] = (oc_real)(c->w[f] + 0.5 * ((double)vz[cc] + vz[cc + nxy]));
+    }
+  }
+}
+
+/* ---------------------------------------------------------- grid clamp */
+/* _clampUmSlice, solver.js:2629-2653 */
+static double clamp_arr(oc_real *A, long n, double vcap, double um) {
+  for (long i = 0; i < n; i++) {
+    if (A[i] > vcap) A[i] = (oc_real)vcap; else if (A[i] < -vcap) A[i] = (oc_real)(-vcap);
+    double av = A[i];
+    if (av > um) um = av; else if (-av > um) um = -av;
+  }
+  return um;
+}
+
+double oc_clamp_umax(oc_ctx *c) {
+  double vcap = c->maxSpeed * 1.1, um = 0;
+  um = clamp_arr(c->u, c->uN, vcap, um);
+  um = clamp_arr(c->v, c->vN, vcap, um);
+  um = clamp_arr(c->w, c->wN, vcap, um);
+  c->umax = um;
+  return um;
+}
+
+/* -------------------------------------------------------- extrapolation */
+/* _extrapolate, solver.js:1562-1636: valid = weighted or solid-adjacent or
+ * a domain plane; one in-place lexicographic pass where faces filled earlier
+ * in the pass (mark 2) already count as valid sources. */
+static void extrap_comp(oc_ctx *c, int comp, oc_real *A, const oc_real *Wt, uint8_t *valid, int d0, int d1, int d2) {
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  long nxy = (long)nx * ny;
+  const uint8_t *t = c->cellType;
+  for (int k = 0; k < d2; k++)
+    for (int j = 0; j < d1; j++)
+      for (int i = 0; i < d0; i++) {
+        long idx = ((long)k * d1 + j) * d0 + i;
+        long cr = ((long)k * ny + j) * nx + i;
+        int sa;
+        if (comp == 0) sa = (i == 0 || i == nx) ? 1 : (t[cr - 1] == SOLID || t[cr] == SOLID);
+        else if (comp == 1) sa = (j == 0 || j == ny) ? 1 : (t[cr - nx] == SOLID || t[cr] == SOLID);
+        else sa = (k == 0 || k == nz) ? 1 : (t[cr - nxy] == SOLID || t[cr] == SOLID);
+        valid[idx] = (Wt[idx] > 1e-8 || sa) ? 1 : 0;
+      }
+  long sK = (long)d0 * d1;
+  for (int k = 0; k < d2; k++)
+    for (int j = 0; j < d1; j++)
+      for (int i = 0; i < d0; i++) {
+        long idx = ((long)k * d1 + j) * d0 + i;
+        if (valid[idx]) continue;
+        double sum = 0;
+        int cnt = 0;
+        if (i > 0 && valid[idx - 1]) { sum += A[idx - 1]; cnt++; }
+        if (i < d0 - 1 && valid[idx + 1]) { sum += A[idx + 1]; cnt++; }
+        if (j > 0 && valid[idx - d0]) { sum += A[idx - d0]; cnt++; }
+        if (j < d1 - 1 && valid[idx + d0]) { sum += A[idx + d0]; cnt++; }
+        if (k > 0 && valid[idx - sK]) { sum += A[idx - sK]; cnt++; }
+        if (k < d2 - 1 && valid[idx + sK]) { sum += A[idx + sK]; cnt++; }
+        if (cnt > 0) { A[idx] = (oc_real)(sum / cnt); valid[idx] = 2; }
+      }
+  for (long i = 0; i < sK * d2; i++) if (valid[i] == 2) valid[i] = 1;
+}
+
+void oc_extrapolate(oc_ctx *c) {
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  extrap_comp(c, 0, c->u, c->uW, c->validU, nx + 1, ny, nz);
+  extrap_comp(c, 1, c->v, c->vW, c->validV, nx, ny + 1, nz);
+  extrap_comp(c, 2, c->w, c->wW, c->validW, nx, ny, nz + 1);
+}
+
+/* ------------------------------------------------------------------ G2P */
+/* _g2pChunk, solver.js:1646-1756. Returns the airborne count. */
+int oc_g2p(oc_ctx *c, double dt) {
+  int nx = c->nx, ny = c->ny, nz = c->nz, air = 0;
+  double dx = c->dx, pic = c->pic;
+  const uint8_t *t = c->cellType;
+  for (int p = 0; p < c->nP; p++) {
+    if (c->pflag[p] != 0) { air++; continue; }
+    if (c->pcool[p] > 0) {
+      double tmp[3];
+      c->pcool[p]--;
+      double dtt = dt != 0 ? dt : (c->dtLast != 0 ? c->dtLast : 0.01);
+      c->pvy[p] = (oc_real)(c->pvy[p] - c->gravity * dtt);
+      oc_sample_vel3(c, c->px[p], c->py[p], c->pz[p], tmp);
+      c->advU[p] = tmp[0]; c->advV[p] = tmp[1]; c->advW[p] = tmp[2];
+      continue;
+    }
+    double x = c->px[p], y = c->py[p], z = c->pz[p];
+    oc_stencil su, sv, sw;
+    face_stencil(c, 0, x, y, z, &su);
+    face_stencil(c, 1, x, y, z, &sv);
+    face_stencil(c, 2, x, y, z, &sw);
+    double ou = stencil_eval(&su, c->uO), nu = stencil_eval(&su, c->u);
+    double ov = stencil_eval(&sv, c->vO), nv = stencil_eval(&sv, c->v);
+    double ow = stencil_eval(&sw, c->wO), nw = stencil_eval(&sw, c->w);
+    c->advU[p] = nu; c->advV[p] = nv; c->advW[p] = nw;
+    c->pvx[p] = (oc_real)(((double)c->pvx[p] + nu - ou) * (1 - pic) + nu * pic);
+    c->pvy[p] = (oc_real)(((double)c->pvy[p] + nv - ov) * (1 - pic) + nv * pic);
+    c->pvz[p] = (oc_real)(((double)c->pvz[p] + nw - ow) * (1 - pic) + nw * pic);
+    double pv2 = (double)c->pvx[p] * c->pvx[p] + (double)c->pvy[p] * c->pvy[p] + (double)c->pvz[p] * c->pvz[p];
+    if (pv2 > c->maxSpeed * c->maxSpeed) {
+      double psc = c->maxSpeed / sqrt(pv2);
+      c->pvx[p] = (oc_real)(c->pvx[p] * psc);
+      c->pvy[p] = (oc_real)(c->pvy[p] * psc);
+      c->pvz[p] = (oc_real)(c->pvz[p] * psc);
+    }
+    int ci = clampi((int)floor(x / dx), 0, nx - 1);
+    int cj = clampi((int)floor(y / dx), 0, ny - 1);
+    int ck = clampi((int)floor(z / dx), 0, nz - 1);
+    long cc = ((long)ck * ny + cj) * nx + ci;
+    if (t[cc] == AIR) {
+      long nxy = (long)nx * ny;
+      int fluidNbr = (ci > 0 && t[cc - 1] == FLUID) || (ci < nx - 1 && t[cc + 1] == FLUID) ||
+                     (cj > 0 && t[cc - nx] == FLUID) || (cj < ny - 1 && t[cc + nx] == FLUID) ||
+                     (ck > 0 && t[cc - nxy] == FLUID) || (ck < nz - 1 && t[cc + nxy] == FLUID);
+      if (!fluidNbr && !on_rock_below(c, x, y, z)) { c->pflag[p] = 1; air++; }
+    }
+  }
+  c->airborne = air;
+  return air;
+}
+
+/* ------------------------------------------------------------ advection */
+static uint32_t lcg(uint32_t s) { return s * 1664525u + 1013904223u; }
+
+static double rlen(double x, double y, double z) {
+  double r = sqrt(x * x + y * y + z * z);
+  return (r != 0) ? r : 1e-9; /* "|| 1e-9" in the reference */
+}
+
+/* _advectChunk, solver.js:1769-2292, sphere mode. Positions are carried in
+ * double within a particle's update and stored to OC_REAL at the end. */
+void oc_advect(oc_ctx *c, double dt) {
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  double dx = c->dx, g = c->gravity;
+  const uint8_t *t = c->cellType;
+  double xLo = dx * 1.02, xHi = c->W - dx * 1.02, yLo = dx * 1.02, yHi = c->H - dx * 1.02;
+  double zLo = dx * 1.02, zHi = c->D - dx * 1.02;
+  double cx = c->cx, cy = c->cy, cz = c->cz;
+  double domHi = c->domainR - dx * 0.02;
+  double coreLo = c->coreR + dx * 1.02;
+  int hasTerr = c->tsolid != 0;
+  double wSX = 1, wSY = 0, wSZ = 0;
+  if (c->hasSun) {
+    wSX = c->sunX - cx; wSY = c->sunY - cy; wSZ = c->sunZ - cz;
+    double wl = sqrt(wSX * wSX + wSY * wSY + wSZ * wSZ);
+    if (!(wl != 0)) wl = 1;
+    wSX /= wl; wSY /= wl; wSZ /= wl;
+  }
+  double rainTe = c->rainT < c->cloudT - 0.02 ? c->rainT : c->cloudT - 0.02;
+  if (rainTe < 0) rainTe = 0;
+  double ms2 = c->maxSpeed * c->maxSpeed;
+
+  for (int p = 0; p < c->nP; p++) {
+    double x = c->px[p], y = c->py[p], z = c->pz[p];
+    double vx = c->pvx[p], vy = c->pvy[p], vz = c->pvz[p];
+    int fl = c->pflag[p];
+
+    if (fl == 5) { /* ice, solver.js:1797-1864 */
+      if (c->pAir[p] == 1 || !hasTerr) { c->pvx[p] = 0; c->pvy[p] = 0; c->pvz[p] = 0; continue; }
+      double dragK = 12, gI = c->gravity != 0 ? c->gravity : 9.81;
+      double exI = x - cx, eyI = y - cy, ezI = z - cz;
+      double elI = rlen(exI, eyI, ezI);
+      double axI = -gI * 0.6 * exI / elI, ayI = -gI * 0.6 * eyI / elI, azI = -gI * 0.6 * ezI / elI;
+      c->pvx[p] = (oc_real)(c->pvx[p] + (axI - dragK * c->pvx[p]) * dt);
+      c->pvy[p] = (oc_real)(c->pvy[p] + (ayI - dragK * c->pvy[p]) * dt);
+      c->pvz[p] = (oc_real)(c->pvz[p] + (azI - dragK * c->pvz[p]) * dt);
+      x += c->pvx[p] * dt; y += c->pvy[p] * dt; z += c->pvz[p] * dt;
+      double rrI = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz));
+      if (rrI > c->oceanR + c->atmosphereH) {
+        double scI = (c->oceanR + c->atmosphereH) / rrI;
+        x = cx + (x - cx) * scI; y = cy + (y - cy) * scI; z = cz + (z - cz) * scI;
+        c->pvx[p] = (oc_real)(c->pvx[p] * 0.5); c->pvy[p] = (oc_real)(c->pvy[p] * 0.5); c->pvz[p] = (oc_real)(c->pvz[p] * 0.5);
+      }
+      if (oc_rock_cell_at(c, x, y, z)) {
+        c->pAir[p] = 1; c->pvx[p] = 0; c->pvy[p] = 0; c->pvz[p] = 0;
+        c->px[p] = (oc_real)x; c->py[p] = (oc_real)y; c->pz[p] = (oc_real)z;
+        continue;
+      }
+      int ciI = (int)floor(x / dx), cjI = (int)floor(y / dx), ckI = (int)floor(z / dx);
+      if (ciI >= 0 && cjI >= 0 && ckI >= 0 && ciI < nx && cjI < ny && ckI < nz &&
+          t[((long)ckI * ny + cjI) * nx + ciI] == FLUID) {
+        double exF = x - cx, eyF = y - cy, ezF = z - cz;
+        double elF = rlen(exF, eyF, ezF);
+        double stepF = dx * 0.5;
+        for (int mf = 0; mf < 12; mf++) {
+          double qx = x + exF / elF * stepF, qy = y + eyF / elF * stepF, qz = z + ezF / elF * stepF;
+          int qi = (int)floor(qx / dx), qj = (int)floor(qy / dx), qk = (int)floor(qz / dx);
+          if (qi < 0 || qj < 0 || qk < 0 || qi >= nx || qj >= ny || qk >= nz) break;
+          if (t[((long)qk * ny + qj) * nx + qi] != FLUID) break;
+          x = qx; y = qy; z = qz;
+        }
+        c->pAir[p] = 1; c->pvx[p] = 0; c->pvy[p] = 0; c->pvz[p] = 0;
+        if (c->pWx) { c->pWx[p] = 0; c->pWy[p] = 0; c->pWz[p] = 0; }
+      }
+      c->px[p] = (oc_real)x; c->py[p] = (oc_real)y; c->pz[p] = (oc_real)z;
+      continue;
+    }
+
+    if (fl == 1 || fl == 4) { /* ballistic spray / rain, solver.js:1865-1976 */
+      double gx = x - cx, gy = y - cy, gz = z - cz;
+      double gr2 = gx * gx + gy * gy + gz * gz;
+      double ga = g * c->oceanR * c->oceanR * dt / (gr2 * sqrt(gr2));
+      c->pvx[p] = (oc_real)(vx - gx * ga); c->pvy[p] = (oc_real)(vy - gy * ga); c->pvz[p] = (oc_real)(vz - gz * ga);
+      double dr = 1 - 2.5 * dt;
+      if (dr < 0) dr = 0;
+      c->pvx[p] = (oc_real)(c->pvx[p] * dr); c->pvy[p] = (oc_real)(c->pvy[p] * dr); c->pvz[p] = (oc_real)(c->pvz[p] * dr);
+      double dsp = sqrt((double)c->pvx[p] * c->pvx[p] + (double)c->pvy[p] * c->pvy[p] + (double)c->pvz[p] * c->pvz[p]);
+      if (dsp > c->maxSpeed) {
+        double dsc = c->maxSpeed / dsp;
+        c->pvx[p] = (oc_real)(c->pvx[p] * dsc); c->pvy[p] = (oc_real)(c->pvy[p] * dsc); c->pvz[p] = (oc_real)(c->pvz[p] * dsc);
+      }
+      x += c->pvx[p] * dt; y += c->pvy[p] * dt; z += c->pvz[p] * dt;
+      double bx = x - cx, by = y - cy, bz = z - cz;
+      double br = rlen(bx, by, bz);
+      double skyHi = c->oceanR + c->atmosphereH;
+      if (br > skyHi) {
+        double sc0 = skyHi / br;
+        x = cx + bx * sc0; y = cy + by * sc0; z = cz + bz * sc0;
+        double vr0 = ((double)c->pvx[p] * bx + (double)c->pvy[p] * by + (double)c->pvz[p] * bz) / br;
+        if (vr0 > 0) {
+          c->pvx[p] = (oc_real)(c->pvx[p] - bx / br * vr0 * 1.15);
+          c->pvy[p] = (oc_real)(c->pvy[p] - by / br * vr0 * 1.15);
+          c->pvz[p] = (oc_real)(c->pvz[p] - bz / br * vr0 * 1.15);
+        }
+      } else if (br <= domHi) {
+        int tries0 = 0;
+        while (hasTerr && oc_rock_cell_at(c, x, y, z) && tries0 < 8) {
+          double rW0 = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz));
+          double stW0 = dx * 0.5 < domHi - rW0 ? dx * 0.5 : domHi - rW0;
+          if (stW0 <= 1e-5) { tries0 = 8; break; }
+          x += bx / br * stW0; y += by / br * stW0; z += bz / br * stW0;
+          tries0++;
+        }
+        if (!hasTerr && br < coreLo) {
+          double sc1 = coreLo / br;
+          x = cx + bx * sc1; y = cy + by * sc1; z = cz + bz * sc1;
+          tries0 = 1;
+        }
+        if (tries0 > 0) {
+          if (hasTerr) {
+            double exS = x - cx, eyS = y - cy, ezS = z - cz;
+            double erS = rlen(exS, eyS, ezS);
+            double wantS = oc_terrain_radius_at(c, x, y, z) + dx * 0.06;
+            if (erS > wantS) { x = cx + exS / erS * wantS; y = cy + eyS / erS * wantS; z = cz + ezS / erS * wantS; }
+            int kS = 0;
+            while (oc_rock_cell_at(c, x, y, z) && kS < 4) {
+              x += exS / erS * dx * 0.5; y += eyS / erS * dx * 0.5; z += ezS / erS * dx * 0.5;
+              kS++;
+            }
+          }
+          double vr1 = ((double)c->pvx[p] * bx + (double)c->pvy[p] * by + (double)c->pvz[p] * bz) / br;
+          c->pvx[p] = (oc_real)(c->pvx[p] - bx / br * vr1);
+          c->pvy[p] = (oc_real)(c->pvy[p] - by / br * vr1);
+          c->pvz[p] = (oc_real)(c->pvz[p] - bz / br * vr1);
+          c->pvx[p] = (oc_real)(c->pvx[p] * 0.3); c->pvy[p] = (oc_real)(c->pvy[p] * 0.3); c->pvz[p] = (oc_real)(c->pvz[p] * 0.3);
+          c->pflag[p] = 0;
+        }
+      }
+      c->px[p] = (oc_real)x; c->py[p] = (oc_real)y; c->pz[p] = (oc_real)z;
+      int ci = clampi((int)floor(x / dx), 0, nx - 1), cj = clampi((int)floor(y / dx), 0, ny - 1);
+      int ck = clampi((int)floor(z / dx), 0, nz - 1);
+      if (t[((long)ck * ny + cj) * nx + ci] == FLUID) {
+        c->pvx[p] = (oc_real)(c->pvx[p] * 0.45); c->pvy[p] = (oc_real)(c->pvy[p] * 0.45); c->pvz[p] = (oc_real)(c->pvz[p] * 0.45);
+        if (c->pvy[p] > 0.5) c->pvy[p] = (oc_real)0.5;
+        c->pcool[p] = 2;
+        c->pflag[p] = 0;
+      }
+      continue;
+    }
+
+    if (fl == 2 || fl == 3) { /* vapor / cloud, solver.js:1978-2174 */
+      double Tv = c->pT[p];
+      c->pAir[p] = (oc_real)(c->pAir[p] + dt);
+      double vEx = x - cx, vEy = y - cy, vEz = z - cz;
+      double vEr = rlen(vEx, vEy, vEz);
+      double ux = vEx / vEr, uy = vEy / vEr, uz = vEz / vEr;
+      double exposure = ux * wSX + uy * wSY + uz * wSZ;
+      double jet = c->oceanR * 0.58 < c->maxSpeed * 0.45 ? c->oceanR * 0.58 : c->maxSpeed * 0.45;
+      double ret = c->oceanR * 0.32 < c->maxSpeed * 0.25 ? c->oceanR * 0.32 : c->maxSpeed * 0.25;
+      double vr = vx * ux + vy * uy + vz * uz;
+      double radial = -c->gravity * 0.06;
+      double wx = -uz * jet - (wSX - exposure * ux) * ret;
+      double wy = -(wSY - exposure * uy) * ret;
+      double wz = ux * jet - (wSZ - exposure * uz) * ret;
+      double relax = dt * 1.3 < 1 ? dt * 1.3 : 1;
+      c->pvx[p] = (oc_real)(vx + (wx - (vx - vr * ux)) * relax + ux * radial * dt);
+      c->pvy[p] = (oc_real)(vy + (wy - (vy - vr * uy)) * relax + uy * radial * dt);
+      c->pvz[p] = (oc_real)(vz + (wz - (vz - vr * uz)) * relax + uz * radial * dt);
+      double svv = (double)c->pvx[p] * c->pvx[p] + (double)c->pvy[p] * c->pvy[p] + (double)c->pvz[p] * c->pvz[p];
+      if (svv > ms2) {
+        double svs = c->maxSpeed / sqrt(svv);
+        c->pvx[p] = (oc_real)(c->pvx[p] * svs); c->pvy[p] = (oc_real)(c->pvy[p] * svs); c->pvz[p] = (oc_real)(c->pvz[p] * svs);
+      }
+      x += c->pvx[p] * dt; y += c->pvy[p] * dt; z += c->pvz[p] * dt;
+      double vCx = x - cx, vCy = y - cy, vCz = z - cz;
+      double vCr = rlen(vCx, vCy, vCz);
+      double ceilR = c->oceanR + c->atmosphereH - 1e-3 * dx;
+      if (hasTerr) {
+        double RtE = oc_terrain_radius_at(c, x, y, z);
+        if (RtE + 0.02 > ceilR) ceilR = RtE + 0.02;
+      }
+      if (vCr > ceilR) {
+        double scE = ceilR / vCr;
+        x = cx + vCx * scE; y = cy + vCy * scE; z = cz + vCz * scE;
+        double vrE = ((double)c->pvx[p] * vCx + (double)c->pvy[p] * vCy + (double)c->pvz[p] * vCz) / vCr;
+        if (vrE > 0) {
+          c->pvx[p] = (oc_real)(c->pvx[p] - vCx / vCr * vrE * 2);
+          c->pvy[p] = (oc_real)(c->pvy[p] - vCy / vCr * vrE * 2);
+          c->pvz[p] = (oc_real)(c->pvz[p] - vCz / vCr * vrE * 2);
+        }
+      }
+      double vrC2 = ((double)c->pvx[p] * vCx + (double)c->pvy[p] * vCy + (double)c->pvz[p] * vCz) / vCr;
+      if (vrC2 > 0 && vCr > c->oceanR) {
+        double hh = (vCr - c->oceanR) / (c->atmosphereH > 1e-4 ? c->atmosphereH : 1e-4);
+        if (hh > 1) hh = 1;
+        double prf = c->ceilReflect == 1 ? hh * hh : c->ceilReflect == 2 ? (exp(3 * hh) - 1) * 0.0523957 : hh;
+        if (prf > 0) {
+          c->cfS = lcg(c->cfS);
+          if (c->cfS / 4294967296.0 < prf * dt * 4) {
+            c->pvx[p] = (oc_real)(c->pvx[p] - vCx / vCr * vrC2 * 2);
+            c->pvy[p] = (oc_real)(c->pvy[p] - vCy / vCr * vrC2 * 2);
+            c->pvz[p] = (oc_real)(c->pvz[p] - vCz / vCr * vrC2 * 2);
+          }
+        }
+      }
+      double fCx = x - cx, fCy = y - cy, fCz = z - cz;
+      double fCr = rlen(fCx, fCy, fCz);
+      double floorR = c->oceanR + 0.15 * dx;
+      if (fCr < floorR) {
+        double vrF = ((double)c->pvx[p] * fCx + (double)c->pvy[p] * fCy + (double)c->pvz[p] * fCz) / fCr;
+        if (vrF < 0) {
+          c->bfS = lcg(c->bfS);
+          double sc = (c->bfS / 4294967296.0 < 0.07) ? floorR / fCr : (floorR + 0.02 * dx) / fCr;
+          x = cx + fCx * sc; y = cy + fCy * sc; z = cz + fCz * sc;
+          c->pvx[p] = (oc_real)(c->pvx[p] - fCx / fCr * vrF * 2);
+          c->pvy[p] = (oc_real)(c->pvy[p] - fCy / fCr * vrF * 2);
+          c->pvz[p] = (oc_real)(c->pvz[p] - fCz / fCr * vrF * 2);
+        }
+      }
+      int triesE = 0;
+      double rNow3 = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz));
+      while (hasTerr && rNow3 < c->domainR && oc_rock_cell_at(c, x, y, z) && triesE < 8) {
+        double rW3 = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz));
+        double stW3 = dx * 0.5 < domHi - rW3 ? dx * 0.5 : domHi - rW3;
+        if (stW3 <= 1e-5) break;
+        x += vCx / vCr * stW3; y += vCy / vCr * stW3; z += vCz / vCr * stW3;
+        triesE++;
+      }
+      if (triesE > 0) {
+        double vrT = ((double)c->pvx[p] * vCx + (double)c->pvy[p] * vCy + (double)c->pvz[p] * vCz) / vCr;
+        if (vrT < 0) {
+          double bT = 1.5 * vrT / vCr;
+          c->pvx[p] = (oc_real)(c->pvx[p] - vCx * bT); c->pvy[p] = (oc_real)(c->pvy[p] - vCy * bT);
+          c->pvz[p] = (oc_real)(c->pvz[p] - vCz * bT);
+        }
+        c->pvx[p] = (oc_real)(c->pvx[p] * 0.92); c->pvy[p] = (oc_real)(c->pvy[p] * 0.92); c->pvz[p] = (oc_real)(c->pvz[p] * 0.92);
+      }
+      int ciV = clampi((int)floor(x / dx), 0, nx - 1), cjV = clampi((int)floor(y / dx), 0, ny - 1);
+      int ckV = clampi((int)floor(z / dx), 0, nz - 1);
+      int backToSea = t[((long)ckV * ny + cjV) * nx + ciV] == FLUID;
+      if (backToSea) {
+        double bx2 = x - cx, by2 = y - cy, bz2 = z - cz;
+        double br2 = rlen(bx2, by2, bz2);
+        if (br2 > c->oceanR + 0.15 * dx) backToSea = 0;
+        else if (((double)c->pvx[p] * bx2 + (double)c->pvy[p] * by2 + (double)c->pvz[p] * bz2) / br2 > 0.02) backToSea = 0;
+      }
+      c->px[p] = (oc_real)x; c->py[p] = (oc_real)y; c->pz[p] = (oc_real)z;
+      if (backToSea) {
+        c->pvx[p] = 0; c->pvy[p] = 0; c->pvz[p] = 0; c->pAir[p] = 0; c->pflag[p] = 0;
+      } else if (fl == 2 && ((Tv <= rainTe && c->pAir[p] >= c->rainLiftAge) || c->pAir[p] > 40)) {
+        c->pvx[p] = 0; c->pvy[p] = 0; c->pvz[p] = 0; c->pAir[p] = 0; c->pflag[p] = 4;
+      }
+      continue;
+    }
+
+    /* grid-coupled fluid: RK2/RK1 through the grid, solver.js:2176-2290 */
+    double t0x = c->advU[p], t0y = c->advV[p], t0z = c->advW[p];
+    double sp2 = t0x * t0x + t0y * t0y + t0z * t0z;
+    double v2x = t0x, v2y = t0y, v2z = t0z;
+    if (sp2 > 0.25) {
+      double mx = clampd(x + 0.5 * dt * t0x, xLo, xHi);
+      double my = clampd(y + 0.5 * dt * t0y, yLo, yHi);
+      double mz = clampd(z + 0.5 * dt * t0z, zLo, zHi);
+      double t2[3];
+      oc_sample_vel3(c, mx, my, mz, t2);
+      v2x = t2[0]; v2y = t2[1]; v2z = t2[2];
+    }
+    sp2 = v2x * v2x + v2y * v2y + v2z * v2z;
+    if (sp2 > ms2) {
+      double sc = c->maxSpeed / sqrt(sp2);
+      v2x *= sc; v2y *= sc; v2z *= sc;
+      c->pvx[p] = (oc_real)(c->pvx[p] * sc); c->pvy[p] = (oc_real)(c->pvy[p] * sc); c->pvz[p] = (oc_real)(c->pvz[p] * sc);
+    }
+    x += v2x * dt; y += v2y * dt; z += v2z * dt;
+    double nx2 = x - cx, ny2 = y - cy, nz2 = z - cz;
+    double nr = rlen(nx2, ny2, nz2);
+    if (nr > domHi) {
+      double sc2 = domHi / nr;
+      x = cx + nx2 * sc2; y = cy + ny2 * sc2; z = cz + nz2 * sc2;
+      c->pvy[p] = (oc_real)(-fabs((double)c->pvy[p]) * 0.1 - 0.3);
+      c->pflag[p] = 1;
+    }
+    if (nr < domHi) {
+      int tries = 0;
+      while (hasTerr && oc_rock_cell_at(c, x, y, z) && tries < 8) {
+        double rW2 = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz));
+        double stW2 = dx * 0.5 < domHi - rW2 ? dx * 0.5 : domHi - rW2;
+        if (stW2 <= 1e-5) { tries = 8; break; }
+        x += nx2 / nr * stW2; y += ny2 / nr * stW2; z += nz2 / nr * stW2;
+        tries++;
+      }
+      if (tries > 0) {
+        if (hasTerr) {
+          double exS2 = x - cx, eyS2 = y - cy, ezS2 = z - cz;
+          double erS2 = rlen(exS2, eyS2, ezS2);
+          double wantS2 = oc_terrain_radius_at(c, x, y, z) + dx * 0.06;
+          if (erS2 > wantS2) { x = cx + exS2 / erS2 * wantS2; y = cy + eyS2 / erS2 * wantS2; z = cz + ezS2 / erS2 * wantS2; }
+          int kS2 = 0;
+          while (oc_rock_cell_at(c, x, y, z) && kS2 < 4) {
+            x += exS2 / erS2 * dx * 0.5; y += eyS2 / erS2 * dx * 0.5; z += ezS2 / erS2 * dx * 0.5;
+            kS2++;
+          }
+        }
+        double vr2 = ((double)c->pvx[p] * nx2 + (double)c->pvy[p] * ny2 + (double)c->pvz[p] * nz2) / nr;
+        if (vr2 < 0) {
+          c->pvx[p] = (oc_real)(c->pvx[p] - nx2 / nr * vr2);
+          c->pvy[p] = (oc_real)(c->pvy[p] - ny2 / nr * vr2);
+          c->pvz[p] = (oc_real)(c->pvz[p] - nz2 / nr * vr2);
+        }
+      }
+      if (hasTerr && (tries > 0 || oc_rock_cell_at(c, x - nx2 / nr * dx, y - ny2 / nr * dx, z - nz2 / nr * dx))) {
+        double fr0 = 1 - 0.6 * dt;
+        if (fr0 < 0) fr0 = 0;
+        c->pvx[p] = (oc_real)(c->pvx[p] * fr0); c->pvy[p] = (oc_real)(c->pvy[p] * fr0); c->pvz[p] = (oc_real)(c->pvz[p] * fr0);
+      }
+    }
+    double sx = x - cx, sy = y - cy, sz = z - cz;
+    double sr = rlen(sx, sy, sz);
+    double vrad = ((double)c->pvx[p] * sx + (double)c->pvy[p] * sy + (double)c->pvz[p] * sz) / sr;
+    double da = 1.0 * dt;
+    if (da > 0.5) da = 0.5;
+    c->pvx[p] = (oc_real)(c->pvx[p] - sx / sr * vrad * da);
+    c->pvy[p] = (oc_real)(c->pvy[p] - sy / sr * vrad * da);
+    c->pvz[p] = (oc_real)(c->pvz[p] - sz / sr * vrad * da);
+    double dg = 1 - 0.42 * dt;
+    if (dg < 0) dg = 0;
+    c->pvx[p] = (oc_real)(c->pvx[p] * dg); c->pvy[p] = (oc_real)(c->pvy[p] * dg); c->pvz[p] = (oc_real)(c->pvz[p] * dg);
+    c->px[p] = (oc_real)x; c->py[p] = (oc_real)y; c->pz[p] = (oc_real)z;
+  }
+}
+
+/* --------------------------------------------------------------- substep */
+/* _substep, solver.js:2568-2624 (sphere mode, no balls) */
+void oc_substep(oc_ctx *c, double dt) {
+  oc_clear_faces(c);
+  oc_rasterize_static(c);
+  oc_mark_fluid(c, 0, c->nP);
+  oc_p2g(c);
+  oc_apply_bc(c);
+  oc_snapshot(c);
+  oc_viscosity(c, dt);
+  oc_apply_gravity(c, dt);
+  oc_apply_bc(c);
+  oc_pressure_solve(c);
+  oc_project(c);
+  if (c->vorticity > 0) oc_vorticity(c, dt);
+  oc_clamp_umax(c);
+  oc_extrapolate(c);
+  oc_g2p(c, dt);
+  oc_advect(c, dt);
+  c->dtLast = dt;
+}
+
+/* CFL law of _stepSync for one substep, solver.js:2689-2690 */
+double oc_dt_law(oc_ctx *c) {
+  double um = c->umax > 0.08 ? c->umax : 0.08;
+  double dt = c->cfl * c->dx / um;
+  if (dt > 1.0 / 100) dt = 1.0 / 100;
+  if (dt < 1e-5) dt = 1e-5;
+  return dt;
+}
+
+/* ------------------------------------------------------------- surface */
+/* splatDensity / _splatInto, solver.js:3785-3820 */
+void oc_splat_density(oc_ctx *c) {
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  double dx = c->dx, r = 1.6 * c->spacing, r2 = r * r;
+  long sj = nx + 1, sk = (long)(ny + 1) * sj, nD = sk * (nz + 1);
+  for (long i = 0; i < nD; i++) c->dens[i] = 0;
+  for (int p = 0; p < c->nP; p++) {
+    if (c->pflag[p] != 0) continue;
+    double X = c->px[p], Y = c->py[p], Z = c->pz[p];
+    int i0 = clampi((int)floor(X / dx), 0, nx - 1);
+    int j0 = clampi((int)floor(Y / dx), 0, ny - 1);
+    int k0 = clampi((int)floor(Z / dx), 0, nz - 1);
+    double fx = X - i0 * dx, fy = Y - j0 * dx, fz = Z - k0 * dx;
+    for (int kk = 0; kk < 2; kk++) {
+      double qz = kk == 0 ? fz : fz - dx;
+      for (int jj = 0; jj < 2; jj++) {
+        double qy = jj == 0 ? fy : fy - dx;
+        for (int ii = 0; ii < 2; ii++) {
+          double qx = ii == 0 ? fx : fx - dx;
+          double d2 = qx * qx + qy * qy + qz * qz;
+          if (d2 >= r2) continue;
+          double ww = 1 - d2 / r2;
+          ww = ww * ww;
+          long o = (k0 + kk) * sk + (j0 + jj) * sj + (i0 + ii);
+          c->dens[o] = (oc_real)(c->dens[o] + ww);
+        }
+      }
+    }
+  }
+}
+
+/* Marching tetrahedra, surface.js:17-46 (tables) and :66-177 (build). */
+static const int MT_CORNER[8][3] = { {0,0,0},{1,0,0},{1,1,0},{0,1,0},{0,0,1},{1,0,1},{1,1,1},{0,1,1} };
+static const int MT_TET[6][4] = { {0,5,1,6},{0,1,2,6},{0,2,3,6},{0,3,7,6},{0,7,4,6},{0,4,5,6} };
+static const int MT_EDGE[6][2] = { {0,1},{1,2},{2,0},{0,3},{1,3},{2,3} };
+/* per case (bit c = tet corner c above iso): triangle count and edge triples */
+static const int MT_NTRI[16] = { 0,1,1,2,1,2,2,1,1,2,2,1,2,1,1,0 };
+static const int MT_TRI[16][2][3] = {
+  {{0,0,0},{0,0,0}}, {{0,2,3},{0,0,0}}, {{0,1,4},{0,0,0}}, {{2,3,4},{2,4,1}},
+  {{2,1,5},{0,0,0}}, {{0,3,5},{0,5,1}}, {{0,4,5},{0,5,2}}, {{3,5,4},{0,0,0}},
+  {{3,4,5},{0,0,0}}, {{0,2,5},{0,5,4}}, {{0,1,5},{0,5,3}}, {{2,1,5},{0,0,0}},
+  {{2,1,4},{2,4,3}}, {{0,1,4},{0,0,0}}, {{0,2,3},{0,0,0}}, {{0,0,0},{0,0,0}} };
+
+long oc_mt_build(const oc_real *dens, int nx, int ny, int nz, double dx, double iso,
+                 float *pos, float *nrm, long cap, double *volume) {
+  long sj = nx + 1, sk = (long)(ny + 1) * sj;
+  long vcount = 0;
+  double vol = 0, cv[8], gx[8], gy[8], gz[8];
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        double mn = INFINITY, mxv = -INFINITY;
+        for (int cc = 0; cc < 8; cc++) {
+          double val = dens[(k + MT_CORNER[cc][2]) * sk + (j + MT_CORNER[cc][1]) * sj + i + MT_CORNER[cc][0]];
+          cv[cc] = val;
+          if (val < mn) mn = val;
+          if (val > mxv) mxv = val;
+        }
+        if (mxv < iso || mn > iso) continue;
+        for (int cc = 0; cc < 8; cc++) {
+          int ci = i + MT_CORNER[cc][0], cj = j + MT_CORNER[cc][1], ck = k + MT_CORNER[cc][2];
+          int ip = ci + 1 < nx ? ci + 1 : nx, im = ci - 1 > 0 ? ci - 1 : 0;
+          int jp = cj + 1 < ny ? cj + 1 : ny, jm = cj - 1 > 0 ? cj - 1 : 0;
+          int kp = ck + 1 < nz ? ck + 1 : nz, km = ck - 1 > 0 ? ck - 1 : 0;
+          gx[cc] = ((double)dens[ck * sk + cj * sj + ip] - dens[ck * sk + cj * sj + im]) / (2 * dx);
+          gy[cc] = ((double)dens[ck * sk + jp * sj + ci] - dens[ck * sk + jm * sj + ci]) / (2 * dx);
+          gz[cc] = ((double)dens[kp * sk + cj * sj + ci] - dens[km * sk + cj * sj + ci]) / (2 * dx);
+        }
+        for (int tt = 0; tt < 6; tt++) {
+          const int *tet = MT_TET[tt];
+          int code = 0;
+          for (int cc = 0; cc < 4; cc++) if (cv[tet[cc]] > iso) code |= 1 << cc;
+          for (int e = 0; e < MT_NTRI[code]; e++) {
+            double P[3][3], Nn[3][3];
+            for (int vi = 0; vi < 3; vi++) {
+              const int *ed = MT_EDGE[MT_TRI[code][e][vi]];
+              int a = tet[ed[0]], b = tet[ed[1]];
+              double va = cv[a], vb = cv[b], den = vb - va;
+              double tl = den != 0 ? (iso - va) / den : 0.5;
+              if (tl < 0) tl = 0; else if (tl > 1) tl = 1;
+              const int *pa = MT_CORNER[a], *pb = MT_CORNER[b];
+              P[vi][0] = ((i + pa[0]) + (pb[0] - pa[0]) * tl) * dx;
+              P[vi][1] = ((j + pa[1]) + (pb[1] - pa[1]) * tl) * dx;
+              P[vi][2] = ((k + pa[2]) + (pb[2] - pa[2]) * tl) * dx;
+              double nxx = -(gx[a] + (gx[b] - gx[a]) * tl);
+              double nyy = -(gy[a] + (gy[b] - gy[a]) * tl);
+              double nzz = -(gz[a] + (gz[b] - gz[a]) * tl);
+              double nl = sqrt(nxx * nxx + nyy * nyy + nzz * nzz);
+              if (!(nl != 0)) nl = 1;
+              Nn[vi][0] = nxx / nl; Nn[vi][1] = nyy / nl; Nn[vi][2] = nzz / nl;
+            }
+            double ux = P[1][0] - P[0][0], uy = P[1][1] - P[0][1], uz = P[1][2] - P[0][2];
+            double wx = P[2][0] - P[0][0], wy = P[2][1] - P[0][1], wz = P[2][2] - P[0][2];
+            double fnx = uy * wz - uz * wy, fny = uz * wx - ux * wz, fnz = ux * wy - uy * wx;
+            double cgx = 0, cgy = 0, cgz = 0;
+            for (int vi = 0; vi < 3; vi++) { cgx += Nn[vi][0]; cgy += Nn[vi][1]; cgz += Nn[vi][2]; }
+            if (fnx * cgx + fny * cgy + fnz * cgz < 0) {
+              for (int d = 0; d < 3; d++) {
+                double tp = P[2][d]; P[2][d] = P[1][d]; P[1][d] = tp;
+                tp = Nn[2][d]; Nn[2][d] = Nn[1][d]; Nn[1][d] = tp;
+              }
+            }
+            vol += (P[0][0] * (P[1][1] * P[2][2] - P[2][1] * P[1][2]) -
+                    P[0][1] * (P[1][0] * P[2][2] - P[2][0] * P[1][2]) +
+                    P[0][2] * (P[1][0] * P[2][1] - P[2][0] * P[1][1])) / 6;
+            if (pos && vcount + 3 <= cap) {
+              for (int vi = 0; vi < 3; vi++)
+                for (int d = 0; d < 3; d++) {
+                  pos[(vcount + vi) * 3 + d] = (float)P[vi][d];
+                  nrm[(vcount + vi) * 3 + d] = (float)Nn[vi][d];
+                }
+            }
+            vcount += 3;
+          }
+        }
+      }
+  if (volume) *volume = fabs(vol);
+  return vcount;
+}
+
+/* autotuneIso, solver.js:3838-3849 */
+double oc_autotune_iso(oc_ctx *c) {
+  double Vt = c->nP * c->particleVolume, lo = 0.3, hi = 3.0;
+  for (int it = 0; it < 8; it++) {
+    double mid = 0.5 * (lo + hi), vol;
+    long cnt = oc_mt_build(c->dens, c->nx, c->ny, c->nz, c->dx, mid, 0, 0, 0, &vol);
+    if (cnt > 900000 || vol > Vt) lo = mid; else hi = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+/* kineticEnergy, solver.js:3914-3921 */
+double oc_kinetic_energy(oc_ctx *c) {
+  double e = 0, mp = 1000 * c->particleVolume;
+  for (int p = 0; p < c->nP; p++) {
+    double v2 = (double)c->pvx[p] * c->pvx[p] + (double)c->pvy[p] * c->pvy[p] + (double)c->pvz[p] * c->pvz[p];
+    e += 0.5 * mp * v2;
+  }
+  return e;
+}

@@ -1,0 +1,216 @@
+'use strict';
+/*
+ * Golden-vector generator: runs the reference FluidSolver (js/solver.js,
+ * js/surface.js of johnnybuggy/watersim-v1) in Node on small seeded scenes and
+ * writes per-stage input/output dumps of one substep, plus trajectory
+ * summaries and a marching-tetrahedra result, into tests/golden/*.wsg.gz.
+ *
+ * Run only where the reference checkout exists:
+ *     node tests/golden/gen_golden.js /root/reference
+ * The fixtures it writes are data (inputs and expected outputs); the
+ * reference itself never travels with them.
+ *
+ * File format (.wsg.gz, gzip of): "WSG1", u32 LE header length, JSON header
+ * {meta, arrays:[{name,dtype,shape,offset,nbytes}]}, then the raw arrays
+ * (little endian) back to back.
+ */
+var path = require('path');
+var fs = require('fs');
+var zlib = require('zlib');
+
+var REF = process.argv[2] || '/root/reference';
+var OUT = path.dirname(__filename);
+
+global.MarchingTetrahedra = require(path.join(REF, 'js/surface.js'));
+var FluidSolver = require(path.join(REF, 'js/solver.js'));
+
+function Writer() { this.arrays = []; this.meta = {}; }
+Writer.prototype.add = function (name, arr) {
+  var dt;
+  if (arr instanceof Float32Array) dt = 'f4';
+  else if (arr instanceof Float64Array) dt = 'f8';
+  else if (arr instanceof Uint8Array) dt = 'u1';
+  else if (arr instanceof Int32Array) dt = 'i4';
+  else if (arr instanceof Uint32Array) dt = 'u4';
+  else throw new Error('dtype ' + name);
+  this.arrays.push({ name: name, dtype: dt, data: Buffer.from(arr.buffer.slice(arr.byteOffset, arr.byteOffset + arr.byteLength)) });
+};
+Writer.prototype.save = function (file) {
+  var off = 0, hdr = { meta: this.meta, arrays: [] };
+  this.arrays.forEach(function (a) {
+    hdr.arrays.push({ name: a.name, dtype: a.dtype, offset: off, nbytes: a.data.length });
+    off += a.data.length;
+  });
+  var hj = Buffer.from(JSON.stringify(hdr));
+  var head = Buffer.alloc(8);
+  head.write('WSG1', 0, 'ascii');
+  head.writeUInt32LE(hj.length, 4);
+  var all = Buffer.concat([head, hj].concat(this.arrays.map(function (a) { return a.data; })));
+  fs.writeFileSync(path.join(OUT, file), zlib.gzipSync(all, { level: 9 }));
+  console.log(file, (all.length / 1e6).toFixed(2), 'MB raw');
+};
+
+function copy(a, n) { var c = new a.constructor(n === undefined ? a.length : n); c.set(n === undefined ? a : a.subarray(0, n)); return c; }
+
+var PART_F = ['px', 'py', 'pz', 'pvx', 'pvy', 'pvz', 'pT', 'pAir'];
+var PART_U8 = ['pflag', 'pcool'];
+
+function dumpWorld(w, s) {
+  var t = s.terrain;
+  w.add('terrain_solid', copy(t.solid));
+  w.add('terrain_R', copy(t.R));
+  w.add('static_types', copy(s._staticTypes));
+  w.add('gu', copy(s._gu)); w.add('gv', copy(s._gv)); w.add('gw', copy(s._gw));
+  w.meta.world = {
+    nx: s.nx, ny: s.ny, nz: s.nz, dx: s.dx, W: s.W, cx: s.cx, cy: s.cy, cz: s.cz,
+    coreR: s.coreR, oceanR: s.oceanR, domainR: s.domainR, spacing: s.spacing,
+    particleVolume: s.particleVolume, tn: t.n, dv: t.dv, Rsl: t.Rsl, Rlo: t.Rlo, Rhi: t.Rhi,
+    landFrac: t.landFrac, atmosphereH: s.atmosphereH
+  };
+}
+
+function dumpParticles(w, s, tag) {
+  var n = s.nP;
+  PART_F.forEach(function (k) { w.add(tag + k, copy(s[k], n)); });
+  PART_U8.forEach(function (k) { w.add(tag + k, copy(s[k], n)); });
+}
+
+function dumpGrid(w, s, tag, names) {
+  names.forEach(function (k) { w.add(tag + k, copy(s[k])); });
+}
+
+function params(s) {
+  return { pic: s.pic, omega: s.omega, pressureIters: s.pressureIters, gravity: s.gravity,
+    maxSpeed: s.maxSpeed, viscosity: s.viscosity, vorticity: s.vorticity, cfl: s.cfl,
+    gRamp: s._gRamp, nP: s.nP, iso: s.iso };
+}
+
+function dtLaw(s) {
+  var dt = Math.min((s.cfl * s.dx) / Math.max(s.umax, 0.08), 1 / 100);
+  return dt < 1e-5 ? 1e-5 : dt;
+}
+
+// One substep, the stage order of FluidSolver.prototype._substep (sphere mode),
+// dumping what each stage writes.
+function stagedSubstep(w, s, dt) {
+  var G = ['u', 'v', 'w'];
+  w.meta.stage_dt = dt;
+  w.meta.stage_params = params(s);
+  dumpParticles(w, s, 'in_');
+  w.add('in_q', copy(s.q));
+  s.u.fill(0); s.v.fill(0); s.w.fill(0);
+  s.uW.fill(0); s.vW.fill(0); s.wW.fill(0);
+  s._rasterizeStatic();
+  s._markFluidCells(0, s.nP);
+  w.add('mark_cellType', copy(s.cellType));
+  w.add('mark_pflag', copy(s.pflag, s.nP));
+  s._pushOutOfBalls();
+  s._p2g();
+  dumpGrid(w, s, 'p2g_', ['u', 'v', 'w', 'uW', 'vW', 'wW']);
+  s._applyBC();
+  dumpGrid(w, s, 'bc_', G);
+  s.uO.set(s.u); s.vO.set(s.v); s.wO.set(s.w);
+  s._viscosity(dt);
+  dumpGrid(w, s, 'visc_', G);
+  w.meta.gRamp_before_gravity = s._gRamp;
+  s._applyPlanetGravity(dt);
+  s._applyBC();
+  dumpGrid(w, s, 'grav_', G);
+  s._pressureSolve();
+  w.add('pres_q', copy(s.q)); w.add('pres_rhs', copy(s.rhs)); w.add('pres_diag', copy(s.diag));
+  w.meta.nFluid = s.nFluid;
+  s._project();
+  dumpGrid(w, s, 'proj_', G);
+  if (s.vorticity > 0) s._vorticityConfinement(dt);
+  dumpGrid(w, s, 'vort_', G);
+  s.umax = s._clampUmSlice(0, s.uN + s.vN + s.wN);
+  w.meta.umax = s.umax;
+  dumpGrid(w, s, 'clamp_', G);
+  s._extrapolate();
+  dumpGrid(w, s, 'ext_', ['u', 'v', 'w', 'validU', 'validV', 'validW']);
+  s._g2p(dt);
+  w.meta.airborneCount = s.airborneCount;
+  dumpParticles(w, s, 'g2p_');
+  w.add('g2p_advU', copy(s._advU, s.nP)); w.add('g2p_advV', copy(s._advV, s.nP)); w.add('g2p_advW', copy(s._advW, s.nP));
+  s._advect(dt);
+  s._gridSampleValid = false;
+  s._updateBalls(dt);
+  s.dtLast = dt;
+  dumpParticles(w, s, 'adv_');
+  w.meta.rng_after = { evS: s._evS, cfS: s._cfS, bfS: s._bfS };
+}
+
+function ke(s) { return s.kineticEnergy(); }
+
+function trajectory(s, nSub, rec) {
+  for (var i = 0; i < nSub; i++) {
+    var dt = dtLaw(s);
+    s._substep(dt);
+    rec.push({ dt: dt, umax: s.umax, ke: ke(s), airborne: s.airborneCount });
+  }
+}
+
+function seededVel(s, seed, amp) {
+  var st = seed >>> 0;
+  function r() { st = (Math.imul(st, 1664525) + 1013904223) >>> 0; return st / 4294967296; }
+  for (var p = 0; p < s.nP; p++) {
+    s.pvx[p] = amp * (r() - 0.5); s.pvy[p] = amp * (r() - 0.5); s.pvz[p] = amp * (r() - 0.5);
+  }
+}
+
+// ---------------------------------------------------------------- scene A
+// Plain method, small: 32^3, 10k particles, FLIP 0.95, omega 1.7, 40 sweeps.
+(function sceneA() {
+  var n = 32, W = 5.1, dx = W / n;
+  var s = new FluidSolver({ nx: n, ny: n, nz: n, dx: dx, targetParticles: 10000, mode: 'sphere',
+    coreR: 0.25 * W, terrainSeed: 20260114 });
+  s.resetWater(0.4 * W - 0.25 * W);
+  s.pic = 0.05; s.omega = 1.7; s.pressureIters = 40; s.gravity = 9.8; s.maxSpeed = 3.2; s.vorticity = 0;
+  var w = new Writer();
+  w.meta.scene = 'cfg1_32';
+  w.meta.fill = params(s);
+  s._rasterizeStatic();
+  dumpWorld(w, s);
+  dumpParticles(w, s, 'fill_');
+  w.add('fill_dens', copy(s.dens));
+  var rec = [];
+  trajectory(s, 30, rec);
+  stagedSubstep(w, s, dtLaw(s));
+  rec.push({ dt: s.dtLast, umax: s.umax, ke: ke(s), airborne: s.airborneCount });
+  // surface on the post-substep state
+  s.splatDensity();
+  w.add('surf_dens', copy(s.dens));
+  var m = global.MarchingTetrahedra.build(s.dens, n, n, n, dx, s.iso);
+  w.meta.surf = { iso: s.iso, count: m.count, volume: m.volume };
+  w.add('surf_pos', copy(m.pos, m.count * 3)); w.add('surf_nrm', copy(m.nrm, m.count * 3));
+  w.meta.surf.autotune_iso = s.autotuneIso();
+  w.meta.surf.Vt = s.nP * s.particleVolume;
+  trajectory(s, 100 - 31, rec);
+  w.meta.trajectory = rec;
+  dumpParticles(w, s, 'end_');
+  w.save('cfg1_32.wsg.gz');
+})();
+
+// ---------------------------------------------------------------- scene B
+// Micro scene: 16^3, 2k particles, random velocities, full gravity,
+// vorticity confinement on (exercises viscosity, confinement, spray paths).
+(function sceneB() {
+  var n = 16, W = 5.1, dx = W / n;
+  var s = new FluidSolver({ nx: n, ny: n, nz: n, dx: dx, targetParticles: 2000, mode: 'sphere',
+    coreR: 0.25 * W, terrainSeed: 20260115, vorticity: 0.05 });
+  s.resetWater(0.4 * W - 0.25 * W);
+  s.pic = 0.05; s.omega = 1.7; s.pressureIters = 40; s.gravity = 9.8; s.maxSpeed = 3.2;
+  s._gRamp = 1.5;
+  seededVel(s, 99, 2.0);
+  var w = new Writer();
+  w.meta.scene = 'micro_16';
+  w.meta.fill = params(s);
+  s._rasterizeStatic();
+  dumpWorld(w, s);
+  dumpParticles(w, s, 'fill_');
+  var rec = [];
+  trajectory(s, 5, rec);
+  stagedSubstep(w, s, dtLaw(s));
+  w.meta.trajectory = rec;
+  w.save('micro_16.wsg.gz');
+})();

@@ -1,0 +1,116 @@
+"""Pins the CPU oracle against golden vectors written by the reference solver
+(tests/golden/gen_golden.js): every stage of one substep is run from the
+dumped input state and compared with the reference's output of that stage.
+
+With float storage and double locals the oracle reproduces the reference's
+JavaScript arithmetic, so the comparisons below are bit-exact.
+"""
+import numpy as np
+import pytest
+
+import golden
+from scenes import oracle_from_golden, PART_KEYS
+
+SCENES = ["cfg1_32", "micro_16"]
+
+
+def eq(a, b, what):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, what
+    bad = np.flatnonzero(a.view(np.uint8) != b.view(np.uint8)) if a.dtype == b.dtype else None
+    if bad is None or len(bad):
+        diff = np.abs(a.astype(np.float64) - b.astype(np.float64))
+        raise AssertionError("%s: %d mismatches, max |d| %g" % (what, int((diff > 0).sum()), diff.max()))
+
+
+@pytest.mark.parametrize("scene", SCENES)
+def test_substep_stages_bit_exact(scene):
+    g = golden.load(scene)
+    st = oracle_from_golden(g)
+    dt = g.meta["stage_dt"]
+    n = len(g["in_px"])
+    st.call("oc_clear_faces")
+    st.call("oc_rasterize_static")
+    st.call("oc_mark_fluid", 0, n)
+    eq(st["cellType"], g["mark_cellType"], "mark cellType")
+    eq(st["pflag"][:n], g["mark_pflag"], "mark pflag")
+    st.call("oc_p2g")
+    for k in ["u", "v", "w", "uW", "vW", "wW"]:
+        eq(st[k], g["p2g_" + k], "p2g " + k)
+    st.call("oc_apply_bc")
+    for k in "uvw":
+        eq(st[k], g["bc_" + k], "bc " + k)
+    st.call("oc_snapshot")
+    st.call("oc_viscosity", dt)
+    for k in "uvw":
+        eq(st[k], g["visc_" + k], "viscosity " + k)
+    assert st.get_f("gRamp") == g.meta["gRamp_before_gravity"]
+    st.call("oc_apply_gravity", dt)
+    st.call("oc_apply_bc")
+    for k in "uvw":
+        eq(st[k], g["grav_" + k], "gravity " + k)
+    st.call("oc_pressure_solve")
+    assert st.get_i("nFluid") == g.meta["nFluid"]
+    # rhs/diag are only written for this substep's fluid cells; the rest
+    # holds stale values from earlier substeps in the reference
+    fl = g["mark_cellType"] == 1
+    eq(st["rhs"][fl], g["pres_rhs"][fl], "rhs")
+    eq(st["diag"][fl], g["pres_diag"][fl], "diag")
+    eq(st["q"], g["pres_q"], "q after lexicographic SOR")
+    st.call("oc_project")
+    for k in "uvw":
+        eq(st[k], g["proj_" + k], "project " + k)
+    if g.meta["stage_params"]["vorticity"] > 0:
+        st.call("oc_vorticity", dt)
+    for k in "uvw":
+        eq(st[k], g["vort_" + k], "vorticity " + k)
+    um = st.call("oc_clamp_umax")
+    assert um == g.meta["umax"]
+    for k in "uvw":
+        eq(st[k], g["clamp_" + k], "clamp " + k)
+    st.call("oc_extrapolate")
+    for k in ["u", "v", "w", "validU", "validV", "validW"]:
+        eq(st[k], g["ext_" + k], "extrapolate " + k)
+    air = st.call("oc_g2p", dt)
+    assert air == g.meta["airborneCount"]
+    for k in PART_KEYS:
+        eq(st[k][:n], g["g2p_" + k], "g2p " + k)
+    for k in ["advU", "advV", "advW"]:
+        eq(st[k][:n], g["g2p_" + k], "g2p " + k)
+    st.call("oc_advect", dt)
+    for k in PART_KEYS:
+        eq(st[k][:n], g["adv_" + k], "advect " + k)
+
+
+def test_surface_bit_exact():
+    import oracle
+    g = golden.load("cfg1_32")
+    st = oracle_from_golden(g, tag="adv_")
+    st.set_f("spacing", g.meta["world"]["spacing"])
+    st.call("oc_splat_density")
+    eq(st["dens"], g["surf_dens"], "density splat")
+    w = g.meta["world"]
+    cnt, vol, pos, nrm = oracle.mt_build(st["dens"], w["nx"], w["dx"], g.meta["surf"]["iso"])
+    assert cnt == g.meta["surf"]["count"]
+    eq(pos, g["surf_pos"], "mesh positions")
+    eq(nrm, g["surf_nrm"], "mesh normals")
+    assert vol == g.meta["surf"]["volume"]
+    st.set_f("particleVolume", w["particleVolume"])
+    assert st.call("oc_autotune_iso") == g.meta["surf"]["autotune_iso"]
+
+
+def test_trajectory_replay_bit_exact():
+    """Replays the reference's first substeps from the fill state and checks
+    the per-substep dt, umax and kinetic energy it recorded."""
+    g = golden.load("cfg1_32")
+    st = oracle_from_golden(g, tag="fill_")
+    st.set_params(gRamp=g.meta["fill"]["gRamp"])
+    rec = g.meta["trajectory"]
+    st.set_f("umax", 0.0)
+    for i in range(10):
+        dt = st.call("oc_dt_law")
+        assert dt == rec[i]["dt"]
+        st.call("oc_substep", dt)
+        assert st.get_f("umax") == rec[i]["umax"], i
+        assert st.call("oc_kinetic_energy") == rec[i]["ke"], i
