@@ -1,0 +1,123 @@
+/*
+ * watersim C ABI: the FLIP substep pipeline and marching-tetrahedra surface of
+ * johnnybuggy/watersim-v1 (js/solver.js, js/surface.js) as a HIP library for
+ * MI355X (gfx950).
+ *
+ * The reference has no FFI: its boundary is the FluidSolver JavaScript object
+ * (js/solver.js:45-3924). Each entry point below replaces a method or a piece
+ * of state of that object; the citation on each names it. A host binding
+ * (N-API addon, ctypes, cgo ...) maps the object's constructor, typed-array
+ * fields and methods onto these calls.
+ *
+ * Conventions
+ *  - Plain C: opaque context, plain pointers and byte sizes, int status
+ *    (0 = ok, negative = error; ws_last_error() gives the message).
+ *  - Host buffers belong to the caller; device buffers belong to the library.
+ *  - One HIP stream per context. Calls are synchronous unless named *_async;
+ *    a context must not be used from two threads at once.
+ *  - Arrays keep the reference's layouts (solver.js:93-131): cells
+ *    (k*ny+j)*nx+i, u faces (k*ny+j)*(nx+1)+i, v faces (k*(ny+1)+j)*nx+i,
+ *    w faces (k*ny+j)*nx+i, corners (k*(ny+1)+j)*(nx+1)+i, particle SoA.
+ */
+#ifndef WATERSIM_H
+#define WATERSIM_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ws_ctx ws_ctx;
+
+/* Error codes */
+#define WS_OK 0
+#define WS_EINVAL -1   /* bad argument, unknown name or size mismatch */
+#define WS_EHIP -2     /* HIP runtime error */
+#define WS_ENODEV -3   /* no usable GPU */
+#define WS_ESTATE -4   /* call out of order (e.g. no world uploaded) */
+#define WS_ENUMERIC -5 /* non-finite values detected */
+
+/* Stage identifiers for ws_stage(): the reference's underscore methods. */
+enum ws_stage_id {
+  WS_STAGE_CLEAR = 0,      /* u..wW.fill(0), solver.js:2569-2570 */
+  WS_STAGE_RASTERIZE = 1,  /* _rasterizeStatic + _markFluidCells, :860-1000 */
+  WS_STAGE_P2G = 2,        /* _p2g, :1030-1120 */
+  WS_STAGE_BC = 3,         /* _applyBC, :1123-1171 */
+  WS_STAGE_SNAPSHOT = 4,   /* uO/vO/wO.set(u/v/w), :2579 */
+  WS_STAGE_VISCOSITY = 5,  /* _viscosity, :1259-1304 */
+  WS_STAGE_GRAVITY = 6,    /* _applyPlanetGravity + _applyBC, :2656-2662, :1229-1256 */
+  WS_STAGE_PRESSURE = 7,   /* _pressureSolve, :1312-1410 */
+  WS_STAGE_PROJECT = 8,    /* _project, :1412-1443 */
+  WS_STAGE_VORTICITY = 9,  /* _vorticityConfinement, :1454-1559 */
+  WS_STAGE_CLAMP = 10,     /* umax = _clampUmSlice, :2629-2653 */
+  WS_STAGE_EXTRAPOLATE = 11, /* _extrapolate, :1562-1636 */
+  WS_STAGE_G2P = 12,       /* _g2p, :1639-1756 */
+  WS_STAGE_ADVECT = 13,    /* _advect, :1759-2292 */
+  WS_STAGE_COUNT = 14
+};
+
+/* new FluidSolver({nx,ny,nz,dx}) plus _allocParticles(capacity),
+ * solver.js:45-231. Device memory for every field is allocated here. */
+int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity);
+/* Releases the context and its device memory. */
+void ws_destroy(ws_ctx *c);
+/* Message of the last failed call on this context (never NULL). */
+const char *ws_last_error(const ws_ctx *c);
+
+/* Scalar state by name: the tunables and world constants of FluidSolver
+ * (solver.js:56-75, 81-87, 137-193, 203-210), e.g. "pic", "omega",
+ * "pressureIters", "gravity", "maxSpeed", "viscosity", "vorticity", "cfl",
+ * "coreR", "oceanR", "spacing", "particleVolume", "atmosphereH", "gRamp",
+ * "nP", "dv", "tn", "pressureTol", "maxPressureIters", "iso".
+ * ws_get_f also reads results: "umax", "dtLast", "residual",
+ * "pressureIters_done", "airborneCount", "nFluid", "dfRefined". */
+int ws_set_f(ws_ctx *c, const char *name, double v);
+int ws_get_f(ws_ctx *c, const char *name, double *v);
+
+/* Typed-array fields by name (solver.js:93-131, 218-231, 499-504, 1181-1183):
+ * "px","py","pz","pvx","pvy","pvz","pT","pAir" (f32, nP), "pflag","pcool"
+ * (u8, nP), "cellType" (u8, C), "u","v","w","uO","vO","wO","uW","vW","wW"
+ * (f32 faces), "validU","validV","validW" (u8 faces), "q","rhs" (f32, C),
+ * "dens" (f32, corners), "advU","advV","advW" (f32, nP; the G2P samples),
+ * and the static world: "staticTypes" (u8, C), "gu","gv","gw" (f32 faces),
+ * "terrainR" (f32, tn^3). nbytes must equal the field's byte size (for
+ * particle fields: nP elements). */
+int ws_upload(ws_ctx *c, const char *name, const void *host, size_t nbytes);
+int ws_download(ws_ctx *c, const char *name, void *host, size_t nbytes);
+
+/* One operator-split substep of length dt, FluidSolver.prototype._substep,
+ * solver.js:2568-2624 (sphere world). */
+int ws_substep(ws_ctx *c, double dt);
+/* One stage of the substep, for the reference's stage methods; dt is used by
+ * the stages that take it. */
+int ws_stage(ws_ctx *c, int stage, double dt);
+/* step(dtFrame): the CFL substep loop of _stepSync, solver.js:2663-2700
+ * (dt = min(dtFrame/substeps, cfl*dx/max(umax,0.08), 1/100), >= 1e-5,
+ * at most 10 substeps). Returns the substep count through *n_sub. */
+int ws_step_frame(ws_ctx *c, double dt_frame, int *n_sub);
+/* n substeps with the CFL law evaluated on the device (no host round trip
+ * between substeps): the measurement path for substeps/s. */
+int ws_run_substeps(ws_ctx *c, int n);
+
+/* splatDensity(), solver.js:3785-3820: corner density of fluid particles. */
+int ws_splat_density(ws_ctx *c);
+/* MarchingTetrahedra.build(dens,nx,ny,nz,dx,iso), surface.js:66-177, on the
+ * context's density field (dens == NULL) or on a caller field of
+ * (nx+1)(ny+1)(nz+1) floats. Writes up to cap vertices into pos/nrm
+ * (3 floats each, may be NULL), the full vertex count into *count and the
+ * enclosed volume (FP64) into *volume. */
+int ws_mt_build(ws_ctx *c, const float *dens, int nx, int ny, int nz, double dx, double iso,
+                float *pos, float *nrm, long cap, long *count, double *volume);
+/* autotuneIso(), solver.js:3838-3849: 8 bisections of iso on the device. */
+int ws_autotune_iso(ws_ctx *c, double *iso);
+/* kineticEnergy(), solver.js:3914-3921 (FP64 on the device). */
+int ws_kinetic_energy(ws_ctx *c, double *ke);
+
+/* Waits for all work queued on the context's stream. */
+int ws_sync(ws_ctx *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

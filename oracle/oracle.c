@@ -23,6 +23,10 @@ struct oc_ctx {
   /* tunables (solver.js:56-75, 145-193) */
   double cfl, pic, gravity, viscosity, omega, maxSpeed, vorticity;
   int pressureIters;
+  /* solve variant used by the GPU build: red-black order, and optionally
+   * iterate to a relative residual checked every residualEvery sweeps */
+  int sorRedBlack, residualEvery, maxPressureIters, itersDone;
+  double pressureTol, residual;
   double coreR, oceanR, atmosphereH, spacing, particleVolume;
   double gRamp, dtLast;
   double rainT, cloudT, rainLiftAge;
@@ -124,23 +128,23 @@ int oc_set_f(oc_ctx *c, const char *name, double v) {
   FLD(cfl) FLD(pic) FLD(gravity) FLD(viscosity) FLD(omega) FLD(maxSpeed) FLD(vorticity)
   FLD(coreR) FLD(oceanR) FLD(atmosphereH) FLD(spacing) FLD(particleVolume)
   FLD(gRamp) FLD(dtLast) FLD(rainT) FLD(cloudT) FLD(rainLiftAge) FLD(dv)
-  FLD(sunX) FLD(sunY) FLD(sunZ) FLD(umax)
+  FLD(sunX) FLD(sunY) FLD(sunZ) FLD(umax) FLD(pressureTol)
   return -1;
 }
 int oc_set_i(oc_ctx *c, const char *name, long long v) {
   FLD(pressureIters) FLD(nP) FLD(tn) FLD(ceilReflect) FLD(hasSun)
-  FLD(cfS) FLD(bfS) FLD(evS)
+  FLD(cfS) FLD(bfS) FLD(evS) FLD(sorRedBlack) FLD(residualEvery) FLD(maxPressureIters)
   return -1;
 }
 #undef FLD
 #define GET(nm) if (!strcmp(name, #nm)) return c->nm;
 double oc_get_f(oc_ctx *c, const char *name) {
   GET(umax) GET(gRamp) GET(dtLast) GET(domainR) GET(cx) GET(cy) GET(cz) GET(W)
-  GET(oceanR) GET(spacing) GET(particleVolume)
+  GET(oceanR) GET(spacing) GET(particleVolume) GET(residual)
   return NAN;
 }
 long long oc_get_i(oc_ctx *c, const char *name) {
-  GET(airborne) GET(nFluid) GET(nP) GET(cfS) GET(bfS) GET(evS)
+  GET(airborne) GET(nFluid) GET(nP) GET(cfS) GET(bfS) GET(evS) GET(itersDone)
   return -1;
 }
 #undef GET
@@ -553,7 +557,24 @@ void oc_pressure_finish(oc_ctx *c) {
 
 void oc_pressure_solve(oc_ctx *c) {
   int nF = oc_pressure_assemble(c);
-  if (nF > 0) oc_sor_lex(c, c->pressureIters);
+  if (nF > 0) {
+    if (c->pressureTol > 0) {
+      int done = 0, every = c->residualEvery > 0 ? c->residualEvery : 8;
+      int mx = c->maxPressureIters > 0 ? c->maxPressureIters : 4000;
+      double res = oc_residual(c, 0);
+      while (res > c->pressureTol && done < mx) {
+        int k = done + every > mx ? mx - done : every;
+        if (c->sorRedBlack) oc_sor_redblack(c, k); else oc_sor_lex(c, k);
+        done += k;
+        res = oc_residual(c, 0);
+      }
+      c->itersDone = done;
+      c->residual = res;
+    } else {
+      if (c->sorRedBlack) oc_sor_redblack(c, c->pressureIters); else oc_sor_lex(c, c->pressureIters);
+      c->itersDone = c->pressureIters;
+    }
+  }
   oc_pressure_finish(c);
 }
 

@@ -170,7 +170,8 @@ class OracleState:
             self._bind(k, np.zeros(cap, np.float64))
 
     def set_params(self, **kw):
-        ints = {"pressureIters", "ceilReflect", "hasSun", "cfS", "bfS", "evS", "nP", "tn"}
+        ints = {"pressureIters", "ceilReflect", "hasSun", "cfS", "bfS", "evS", "nP", "tn", "sorRedBlack",
+                "residualEvery", "maxPressureIters"}
         for k, v in kw.items():
             (self.set_i if k in ints else self.set_f)(k, v)
 

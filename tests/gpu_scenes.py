@@ -1,0 +1,44 @@
+"""Loads a golden scene into a GPU solver context (HIP library through its C
+ABI), mirroring scenes.oracle_from_golden for the CPU oracle."""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "watersim-v1_amd"))
+
+import watersim_hip  # noqa: E402
+
+from scenes import PART_KEYS  # noqa: E402
+
+
+def gpu_from_golden(g, tag="in_", capacity=None):
+    w = g.meta["world"]
+    n = len(g[tag + "px"])
+    s = watersim_hip.Solver(w["nx"], w["ny"], w["nz"], w["dx"], capacity or n)
+    s.set(coreR=w["coreR"], oceanR=w["oceanR"], spacing=w["spacing"], particleVolume=w["particleVolume"],
+          atmosphereH=w["atmosphereH"], dv=w["dv"], tn=w["tn"])
+    sp = g.meta["stage_params"]
+    s.set(pic=sp["pic"], omega=sp["omega"], pressureIters=sp["pressureIters"], gravity=sp["gravity"],
+          maxSpeed=sp["maxSpeed"], viscosity=sp["viscosity"], vorticity=sp["vorticity"], cfl=sp["cfl"])
+    s.upload("staticTypes", g["static_types"])
+    s.upload("gu", g["gu"])
+    s.upload("gv", g["gv"])
+    s.upload("gw", g["gw"])
+    s.upload("terrainR", g["terrain_R"])
+    s.set(nP=n)
+    for k in PART_KEYS:
+        if tag + k in g:
+            s.upload(k, g[tag + k])
+    if tag == "in_":
+        s.upload("q", g["in_q"])
+        s.set(gRamp=sp["gRamp"])
+    return s
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = max(np.abs(b).max(), 1e-30)
+    return np.abs(a - b).max() / scale

@@ -1,0 +1,710 @@
+// C ABI of the watersim HIP library (include/watersim.h): context setup,
+// named field transfer, and the host-side orchestration of the substep
+// stages on one HIP stream.
+#include <hipcub/hipcub.hpp>
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+#include "ws_kernels.h"
+
+using namespace ws;
+
+struct ws_ctx : public ws::Ctx {};
+
+namespace {
+
+df to_df(double d) {
+  float hi = (float)d;
+  return df{hi, (float)(d - (double)hi)};
+}
+
+// thr[i] = smallest float x with floor(double(x)/dx) >= i (thr[0] = -inf):
+// the exact cell index of a float coordinate as JavaScript computes it.
+std::vector<float> floor_thresholds(double dx, int n) {
+  std::vector<float> t(n + 2);
+  t[0] = -FLT_MAX;
+  for (int i = 1; i < n + 2; i++) {
+    float hiF = (float)((i + 1) * dx);
+    uint32_t lo = 0, hi;
+    std::memcpy(&hi, &hiF, 4);
+    while (lo < hi) {
+      uint32_t mid = lo + (hi - lo) / 2;
+      float x;
+      std::memcpy(&x, &mid, 4);
+      if (std::floor((double)x / dx) >= i) hi = mid; else lo = mid + 1;
+    }
+    std::memcpy(&t[i], &lo, 4);
+  }
+  return t;
+}
+
+template <class T>
+int dalloc(ws_ctx *c, T **p, size_t n) {
+  hipError_t e = hipMalloc((void **)p, sizeof(T) * (n ? n : 1));
+  if (e != hipSuccess) {
+    c->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return WS_EHIP;
+  }
+  e = hipMemset(*p, 0, sizeof(T) * (n ? n : 1));
+  if (e != hipSuccess) {
+    c->err = std::string("hipMemset: ") + hipGetErrorString(e);
+    return WS_EHIP;
+  }
+  return WS_OK;
+}
+
+World make_world(ws_ctx *c) {
+  World w;
+  w.st = c->staticTypes;
+  w.R = c->haveTerrain ? c->terrainR : nullptr;
+  w.thr = c->thr;
+  w.tn = c->tn;
+  w.dv = (float)c->dv;
+  w.inv_dv = c->dv > 0 ? (float)(1.0 / c->dv) : 0.f;
+  w.dv_df = to_df(c->dv);
+  w.cxd = to_df(c->cx); w.cyd = to_df(c->cy); w.czd = to_df(c->cz);
+  w.oceanR = (float)c->oceanR;
+  double skyAlt = 2 * c->dx > 0.3 ? 2 * c->dx : 0.3;
+  w.skyAlt = (float)skyAlt;
+  w.skyAlt_df = to_df(skyAlt);
+  w.lim_df = to_df((c->oceanR + 0.3) * (c->oceanR + 0.3));
+  w.p3_df = to_df(0.3);
+  w.atmosphereH = (float)c->atmosphereH;
+  w.domainR = (float)c->domainR;
+  w.coreR = (float)c->coreR;
+  return w;
+}
+
+Par make_par(ws_ctx *c) {
+  Par P;
+  P.pic = (float)c->pic; P.maxSpeed = (float)c->maxSpeed; P.gravity = (float)c->gravity;
+  P.oceanR = (float)c->oceanR; P.atmosphereH = (float)c->atmosphereH; P.domainR = (float)c->domainR;
+  P.coreR = (float)c->coreR; P.W = (float)c->W; P.H = (float)c->H; P.D = (float)c->D;
+  double sx = 1, sy = 0, sz = 0;
+  if (c->hasSun) {
+    sx = c->sunX - c->cx; sy = c->sunY - c->cy; sz = c->sunZ - c->cz;
+    double l = std::sqrt(sx * sx + sy * sy + sz * sz);
+    if (!(l != 0)) l = 1;
+    sx /= l; sy /= l; sz /= l;
+  }
+  P.sunX = (float)sx; P.sunY = (float)sy; P.sunZ = (float)sz;
+  double rte = c->rainT < c->cloudT - 0.02 ? c->rainT : c->cloudT - 0.02;
+  P.rainTe = (float)(rte < 0 ? 0 : rte);
+  P.rainLiftAge = (float)c->rainLiftAge;
+  P.ceilReflect = c->ceilReflect;
+  return P;
+}
+
+int need_world(ws_ctx *c) {
+  if (!c->haveStatic) {
+    c->err = "no static world uploaded (staticTypes, gu, gv, gw)";
+    return WS_ESTATE;
+  }
+  return WS_OK;
+}
+
+__global__ void k_set_dt(DevScalars *s, float dt) {
+  if (threadIdx.x == 0) s->dt = dt;
+}
+
+__global__ void k_reset_counters(DevScalars *s) {
+  if (threadIdx.x == 0) { s->umax_bits = 0; s->airborne = 0; }
+}
+
+__global__ void k_zero_u32(unsigned *p, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// ------------------------------------------------------------------ stages
+int st_rasterize(ws_ctx *c) {
+  int nC = (int)c->nC;
+  hipLaunchKernelGGL(k_init_cells, dim3(blocks_for(nC + 1, 256)), dim3(256), 0, c->stream, c->staticTypes, c->cellType,
+                     c->cellCount, nC);
+  if (c->nP > 0)
+    hipLaunchKernelGGL(k_mark, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->geo, make_world(c), c->nP,
+                       c->px, c->py, c->pz, c->pflag, c->cellType, c->cellCount, c->pcell, c->prank, c->ds);
+  WS_LAUNCHED();
+  WS_HIP(hipcub::DeviceScan::ExclusiveSum(c->scanTmp, c->scanTmpBytes, c->cellCount, c->cellStart, nC + 1,
+                                          c->stream));
+  if (c->nP > 0)
+    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->nP, c->pcell, c->prank,
+                       c->cellStart, c->sortedIdx);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+int st_p2g(ws_ctx *c) {
+  int tx = (c->nx + P2G_T - 1) / P2G_T, ty = (c->ny + P2G_T - 1) / P2G_T, tz = (c->nz + P2G_T - 1) / P2G_T;
+  hipLaunchKernelGGL(k_p2g_tile, dim3(tx * ty * tz), dim3(256), 0, c->stream, c->geo, tx, ty, c->cellStart,
+                     c->sortedIdx, c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->vel, c->wgt);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+long nfaces(ws_ctx *c) { return c->uN + c->vN + c->wN; }
+
+int st_bc(ws_ctx *c, float *snapshot, int fuse_gravity) {
+  hipLaunchKernelGGL(k_bc_snapshot, dim3(blocks_for(nfaces(c), 256)), dim3(256), 0, c->stream, c->geo, c->cellType,
+                     c->vel, snapshot, c->wgt, c->valid, fuse_gravity, c->gu, c->ds, 1);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+int visc_iterations(double coef) {
+  // error after m iterations ~ 12 coef (3 coef)^m relative to |u|
+  int m = 1;
+  double e = 12 * coef * 3 * coef;
+  while (e > 1e-8 && m < 15) { e *= 3 * coef; m++; }
+  if (!(m & 1)) m++;
+  return m;
+}
+
+int st_viscosity(ws_ctx *c, double dt, const float *old) {
+  double nu = c->viscosity;
+  if (nu <= 1e-6) return WS_OK;
+  double coef = nu * dt / (c->dx * c->dx);
+  if (coef > 0.24) coef = 0.24;
+  int m = visc_iterations(coef);
+  long F = nfaces(c);
+  // x1 -> vel, x2 -> scr2, x3 -> vel, ... (m odd: the result lands in vel)
+  const float *cur = old;
+  for (int it = 0; it < m; it++) {
+    float *out = (it & 1) ? c->scr2 : c->vel;
+    hipLaunchKernelGGL(k_visc_iter, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->geo, c->cellType, old, cur,
+                       out, (float)coef, it == 0 ? 1 : 0);
+    cur = out;
+  }
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+int st_gravity(ws_ctx *c) {
+  hipLaunchKernelGGL(k_gravity_bc, dim3(blocks_for(nfaces(c), 256)), dim3(256), 0, c->stream, c->geo, c->cellType,
+                     c->vel, c->gu, c->ds);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+int sor_sweeps(ws_ctx *c, int n) {
+  int half = (c->nx + 1) / 2;
+  long nthr = (long)half * c->ny * c->nz;
+  float om1 = (float)(1 - c->omega);
+  for (int it = 0; it < n; it++)
+    for (int color = 0; color < 2; color++)
+      hipLaunchKernelGGL(k_sor, dim3(blocks_for(nthr, 256)), dim3(256), 0, c->stream, c->geo, c->q, c->rhs, c->pw,
+                         c->nbm, color, om1);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+int residual(ws_ctx *c, double *res) {
+  hipLaunchKernelGGL(k_residual, dim3(c->nPartials), dim3(256), 0, c->stream, c->geo, c->q, c->rhs, c->nbm, c->dg,
+                     c->partials);
+  hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, c->stream, c->partials, c->nPartials, c->ds);
+  WS_LAUNCHED();
+  WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+  WS_HIP(hipStreamSynchronize(c->stream));
+  *res = c->hs->bb > 0 ? std::sqrt(c->hs->rr) / std::sqrt(c->hs->bb) : 0.0;
+  return WS_OK;
+}
+
+int st_pressure(ws_ctx *c) {
+  hipLaunchKernelGGL(k_assemble, dim3(blocks_for(c->nC, 256)), dim3(256), 0, c->stream, c->geo, c->cellType, c->vel,
+                     c->q, c->rhs, c->pw, c->nbm, c->dg, (float)c->omega);
+  WS_LAUNCHED();
+  int r;
+  if (c->pressureTol <= 0) {
+    if ((r = sor_sweeps(c, c->pressureIters))) return r;
+    c->itersDone = c->pressureIters;
+    if (c->measureResidual && (r = residual(c, &c->residual))) return r;
+    return WS_OK;
+  }
+  // to a relative residual: sweep in batches, checking in between
+  int done = 0;
+  double res = 0;
+  if ((r = residual(c, &res))) return r;
+  while (res > c->pressureTol && done < c->maxPressureIters) {
+    int k = c->residualEvery;
+    if (done + k > c->maxPressureIters) k = c->maxPressureIters - done;
+    if ((r = sor_sweeps(c, k))) return r;
+    done += k;
+    if ((r = residual(c, &res))) return r;
+  }
+  c->itersDone = done;
+  c->residual = res;
+  return WS_OK;
+}
+
+int st_project(ws_ctx *c, int do_clamp) {
+  hipLaunchKernelGGL(k_project, dim3(blocks_for(nfaces(c), 256)), dim3(256), 0, c->stream, c->geo, c->cellType, c->q,
+                     c->vel, do_clamp, (float)(c->maxSpeed * 1.1), c->ds);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+int st_vorticity(ws_ctx *c) {
+  if (!(c->vorticity > 0)) return WS_OK;
+  int nC = (int)c->nC;
+  float *vc = c->vort, *om = c->vort + 3 * c->nC, *fc = c->vort + 7 * c->nC;
+  hipLaunchKernelGGL(k_vort_center, dim3(blocks_for(nC, 256)), dim3(256), 0, c->stream, c->geo, c->vel, vc);
+  hipLaunchKernelGGL(k_vort_omega, dim3(blocks_for(nC, 256)), dim3(256), 0, c->stream, c->geo, vc, om,
+                     (float)(0.5 / c->dx));
+  hipLaunchKernelGGL(k_vort_force, dim3(blocks_for(nC, 256)), dim3(256), 0, c->stream, c->geo, c->cellType, om, fc,
+                     (float)c->vorticity, c->ds);
+  hipLaunchKernelGGL(k_vort_splat, dim3(blocks_for(nfaces(c), 256)), dim3(256), 0, c->stream, c->geo, c->cellType,
+                     fc, c->vel);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+int st_clamp(ws_ctx *c) {
+  hipLaunchKernelGGL(k_clamp, dim3(blocks_for(nfaces(c), 256)), dim3(256), 0, c->stream, nfaces(c), c->vel,
+                     (float)(c->maxSpeed * 1.1), c->ds);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+int st_extrapolate(ws_ctx *c) {
+  for (size_t d = 0; d + 1 < c->exDiagOff.size(); d++) {
+    int n = c->exDiagOff[d + 1] - c->exDiagOff[d];
+    if (n > 0)
+      hipLaunchKernelGGL(k_extrap_diag, dim3(n), dim3(64), 0, c->stream, c->geo, c->exTiles, c->exDiagOff[d], c->vel,
+                         c->valid);
+  }
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+int st_vapor(ws_ctx *c) {
+  if (!c->hasVapor || c->nP <= 0) return WS_OK;
+  int nb = blocks_for(c->nP, 256);
+  World w = make_world(c);
+  Par P = make_par(c);
+  unsigned *needA = c->vneed, *offA = c->vneed + c->capacity, *needB = c->vneed + 2L * c->capacity,
+           *offB = c->vneed + 3L * c->capacity;
+  hipLaunchKernelGGL(k_vapor_a, dim3(nb), dim3(256), 0, c->stream, c->geo, w, P, c->nP, c->px, c->py, c->pz, c->pvx,
+                     c->pvy, c->pvz, c->pflag, c->pAir, c->vscr, needA, c->ds);
+  WS_LAUNCHED();
+  WS_HIP(hipcub::DeviceScan::ExclusiveSum(c->scanTmp, c->scanTmpBytes, needA, offA, c->nP, c->stream));
+  hipLaunchKernelGGL(k_vapor_b, dim3(nb), dim3(256), 0, c->stream, c->geo, P, c->nP, c->px, c->py, c->pz, c->pvx,
+                     c->pvy, c->pvz, c->pflag, c->vscr, needA, offA, needB, c->ds, c->rng);
+  hipLaunchKernelGGL(k_rng_advance, dim3(1), dim3(64), 0, c->stream, c->rng, 0, needA, offA, c->nP);
+  WS_LAUNCHED();
+  WS_HIP(hipcub::DeviceScan::ExclusiveSum(c->scanTmp, c->scanTmpBytes, needB, offB, c->nP, c->stream));
+  hipLaunchKernelGGL(k_vapor_c, dim3(nb), dim3(256), 0, c->stream, c->geo, w, P, c->nP, c->px, c->py, c->pz, c->pvx,
+                     c->pvy, c->pvz, c->pflag, c->pAir, c->pT, c->cellType, c->vscr, needB, offB, c->ds, c->rng);
+  hipLaunchKernelGGL(k_rng_advance, dim3(1), dim3(64), 0, c->stream, c->rng, 1, needB, offB, c->nP);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+int st_particles(ws_ctx *c, int mode) {
+  if (c->nP <= 0) return WS_OK;
+  hipLaunchKernelGGL(k_g2p_advect, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->geo, make_world(c),
+                     make_par(c), c->nP, mode, c->vel, c->velO, c->cellType, c->px, c->py, c->pz, c->pvx, c->pvy,
+                     c->pvz, c->pflag, c->pcool, c->pAir, c->advU, c->advV, c->advW, c->ds);
+  WS_LAUNCHED();
+  if (mode & 2) return st_vapor(c);
+  return WS_OK;
+}
+
+// the full substep; dt either from the host or from the CFL law on the device
+int substep(ws_ctx *c, double dt, int use_law) {
+  int r;
+  if ((r = need_world(c))) return r;
+  double base = c->dtFrameBase > 0 ? c->dtFrameBase : 0.01;
+  hipLaunchKernelGGL(k_begin_substep, dim3(1), dim3(64), 0, c->stream, c->ds, (float)dt, use_law,
+                     (float)(c->cfl * c->dx), (float)base, (float)c->gravity);
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, c->stream, c->ds);
+  WS_LAUNCHED();
+  bool visc = c->viscosity > 1e-6;
+  if ((r = st_rasterize(c))) return r;
+  if ((r = st_p2g(c))) return r;
+  if ((r = st_bc(c, c->velO, visc ? 0 : 1))) return r;
+  if (visc) {
+    if ((r = st_viscosity(c, dt, c->velO))) return r;
+    if ((r = st_gravity(c))) return r;
+  }
+  if ((r = st_pressure(c))) return r;
+  bool vort = c->vorticity > 0;
+  if ((r = st_project(c, vort ? 0 : 1))) return r;
+  if (vort) {
+    if ((r = st_vorticity(c))) return r;
+    if ((r = st_clamp(c))) return r;
+  }
+  if ((r = st_extrapolate(c))) return r;
+  if ((r = st_particles(c, 3))) return r;
+  hipLaunchKernelGGL(k_end_substep, dim3(1), dim3(64), 0, c->stream, c->ds);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+int pull_scalars(ws_ctx *c) {
+  WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+  WS_HIP(hipStreamSynchronize(c->stream));
+  c->umax = __builtin_bit_cast(float, c->hs->umax_bits);
+  c->airborne = (int)c->hs->airborne;
+  c->gRamp = c->hs->gRamp;
+  c->dtLast = c->hs->dtLast;
+  return WS_OK;
+}
+
+int push_ramp(ws_ctx *c) {
+  // gRamp lives on the device; write it without touching the other scalars
+  WS_HIP(hipMemcpyAsync(&c->ds->gRamp, &c->gRamp, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  WS_HIP(hipStreamSynchronize(c->stream));
+  return WS_OK;
+}
+
+struct Field {
+  void *dev;
+  size_t elem;
+  long count;
+  bool particle;
+};
+
+bool field_of(ws_ctx *c, const char *n, Field &f) {
+  long F = c->uN + c->vN + c->wN;
+  (void)F;
+#define PF(nm, ptr, ty) if (!strcmp(n, nm)) { f = Field{(void *)(ptr), sizeof(ty), c->nP, true}; return true; }
+#define GF(nm, ptr, ty, cnt) if (!strcmp(n, nm)) { f = Field{(void *)(ptr), sizeof(ty), (long)(cnt), false}; return true; }
+  PF("px", c->px, float) PF("py", c->py, float) PF("pz", c->pz, float)
+  PF("pvx", c->pvx, float) PF("pvy", c->pvy, float) PF("pvz", c->pvz, float)
+  PF("pT", c->pT, float) PF("pAir", c->pAir, float)
+  PF("pflag", c->pflag, uint8_t) PF("pcool", c->pcool, uint8_t)
+  PF("advU", c->advU, float) PF("advV", c->advV, float) PF("advW", c->advW, float)
+  GF("cellType", c->cellType, uint8_t, c->nC) GF("staticTypes", c->staticTypes, uint8_t, c->nC)
+  GF("q", c->q, float, c->nC) GF("rhs", c->rhs, float, c->nC)
+  GF("u", c->vel, float, c->uN) GF("v", c->vel + c->uN, float, c->vN) GF("w", c->vel + c->uN + c->vN, float, c->wN)
+  GF("uO", c->velO, float, c->uN) GF("vO", c->velO + c->uN, float, c->vN)
+  GF("wO", c->velO + c->uN + c->vN, float, c->wN)
+  GF("uW", c->wgt, float, c->uN) GF("vW", c->wgt + c->uN, float, c->vN) GF("wW", c->wgt + c->uN + c->vN, float, c->wN)
+  GF("validU", c->valid, uint8_t, c->uN) GF("validV", c->valid + c->uN, uint8_t, c->vN)
+  GF("validW", c->valid + c->uN + c->vN, uint8_t, c->wN)
+  GF("gu", c->gu, float, c->uN) GF("gv", c->gu + c->uN, float, c->vN) GF("gw", c->gu + c->uN + c->vN, float, c->wN)
+  GF("dens", c->dens, float, c->nD)
+  GF("terrainR", c->terrainR, float, (long)c->tn * c->tn * c->tn)
+#undef PF
+#undef GF
+  return false;
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+extern "C" {
+
+int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
+  if (!out) return WS_EINVAL;
+  *out = nullptr;
+  if (nx < 4 || ny < 4 || nz < 4 || !(dx > 0) || capacity < 0) return WS_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return WS_ENODEV;
+  ws_ctx *c = new ws_ctx();
+  c->nx = nx; c->ny = ny; c->nz = nz; c->dx = dx; c->capacity = capacity;
+  // solver.js:84-91
+  c->W = nx * dx; c->H = ny * dx; c->D = nz * dx;
+  c->cx = c->W / 2; c->cy = c->H / 2; c->cz = c->D / 2;
+  double m = c->W < c->H ? c->W : c->H;
+  m = m < c->D ? m : c->D;
+  c->domainR = 0.5 * m - dx;
+  c->spacing = dx * 0.5;
+  c->particleVolume = c->spacing * c->spacing * c->spacing;
+  c->nC = (long)nx * ny * nz;
+  c->uN = (long)(nx + 1) * ny * nz; c->vN = (long)nx * (ny + 1) * nz; c->wN = (long)nx * ny * (nz + 1);
+  c->nD = (long)(nx + 1) * (ny + 1) * (nz + 1);
+  Geo &g = c->geo;
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.uN = (int)c->uN; g.vN = (int)c->vN; g.wN = (int)c->wN; g.nC = (int)c->nC;
+  g.dx = (float)dx; g.inv_dx = (float)(1.0 / dx);
+  g.cx = (float)c->cx; g.cy = (float)c->cy; g.cz = (float)c->cz;
+  g.cx_lo = (float)(c->cx - g.cx); g.cy_lo = (float)(c->cy - g.cy); g.cz_lo = (float)(c->cz - g.cz);
+  long F = c->uN + c->vN + c->wN;
+  int r = WS_OK;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return WS_EHIP; }
+  int nmax = nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz);
+  std::vector<float> thr = floor_thresholds(dx, nmax);
+  long cap = capacity > 0 ? capacity : 1;
+  if ((r = dalloc(c, &c->staticTypes, c->nC)) || (r = dalloc(c, &c->gu, F)) || (r = dalloc(c, &c->thr, thr.size())) ||
+      (r = dalloc(c, &c->cellType, c->nC)) || (r = dalloc(c, &c->nbm, c->nC)) || (r = dalloc(c, &c->dg, c->nC)) ||
+      (r = dalloc(c, &c->q, c->nC)) || (r = dalloc(c, &c->rhs, c->nC)) || (r = dalloc(c, &c->pw, c->nC)) ||
+      (r = dalloc(c, &c->cellCount, c->nC + 1)) || (r = dalloc(c, &c->cellStart, c->nC + 1)) ||
+      (r = dalloc(c, &c->vort, 10 * c->nC)) || (r = dalloc(c, &c->vel, F)) || (r = dalloc(c, &c->velO, F)) ||
+      (r = dalloc(c, &c->wgt, F)) || (r = dalloc(c, &c->scr, F)) || (r = dalloc(c, &c->scr2, F)) ||
+      (r = dalloc(c, &c->valid, F)) || (r = dalloc(c, &c->dens, c->nD)) ||
+      (r = dalloc(c, &c->px, cap)) || (r = dalloc(c, &c->py, cap)) || (r = dalloc(c, &c->pz, cap)) ||
+      (r = dalloc(c, &c->pvx, cap)) || (r = dalloc(c, &c->pvy, cap)) || (r = dalloc(c, &c->pvz, cap)) ||
+      (r = dalloc(c, &c->pT, cap)) || (r = dalloc(c, &c->pAir, cap)) || (r = dalloc(c, &c->advU, cap)) ||
+      (r = dalloc(c, &c->advV, cap)) || (r = dalloc(c, &c->advW, cap)) || (r = dalloc(c, &c->pflag, cap)) ||
+      (r = dalloc(c, &c->pcool, cap)) || (r = dalloc(c, &c->pcell, cap)) || (r = dalloc(c, &c->prank, cap)) ||
+      (r = dalloc(c, &c->sortedIdx, cap)) || (r = dalloc(c, &c->vneed, 4 * cap)) ||
+      (r = dalloc(c, &c->vscr, 4 * cap)) || (r = dalloc(c, &c->rng, 4)) || (r = dalloc(c, &c->ds, 1)) ||
+      (r = dalloc(c, &c->mtCount, c->nC + 1))) {
+    ws_destroy(c);
+    return r;
+  }
+  c->gv = c->gu + c->uN;
+  c->gw = c->gu + c->uN + c->vN;
+  if (hipHostMalloc((void **)&c->hs, sizeof(DevScalars), 0) != hipSuccess) { ws_destroy(c); return WS_EHIP; }
+  c->nPartials = 512;
+  if (dalloc(c, &c->partials, 2 * c->nPartials + 2048)) { ws_destroy(c); return WS_EHIP; }
+  // scan scratch: enough for the larger of the cell and particle scans
+  size_t b1 = 0, b2 = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, b1, c->cellCount, c->cellStart, (int)c->nC + 1, c->stream);
+  hipcub::DeviceScan::ExclusiveSum(nullptr, b2, c->vneed, c->vneed, (int)cap, c->stream);
+  c->scanTmpBytes = b1 > b2 ? b1 : b2;
+  if (hipMalloc(&c->scanTmp, c->scanTmpBytes) != hipSuccess) { ws_destroy(c); return WS_EHIP; }
+  hipMemcpy(c->thr, thr.data(), thr.size() * sizeof(float), hipMemcpyHostToDevice);
+  DevScalars init{};
+  init.gRamp = c->gRamp;
+  hipMemcpy(c->ds, &init, sizeof(init), hipMemcpyHostToDevice);
+  uint32_t rng[4] = {0x2f6e2b1u, 0x7f4a7c15u, 0x51ab3c77u, 0u};  // _cfS, _bfS, _evS (solver.js:151-153)
+  hipMemcpy(c->rng, rng, sizeof(rng), hipMemcpyHostToDevice);
+  // extrapolation tile schedule, grouped by tile diagonal I+J+K
+  {
+    int dims[3][3] = {{nx + 1, ny, nz}, {nx, ny + 1, nz}, {nx, ny, nz + 1}};
+    std::vector<std::vector<int4>> byDiag;
+    for (int comp = 0; comp < 3; comp++) {
+      int ti = (dims[comp][0] + EX_TI - 1) / EX_TI, tj = (dims[comp][1] + EX_TJ - 1) / EX_TJ;
+      int tk = (dims[comp][2] + EX_TK - 1) / EX_TK;
+      for (int K = 0; K < tk; K++)
+        for (int J = 0; J < tj; J++)
+          for (int I = 0; I < ti; I++) {
+            size_t d = (size_t)(I + J + K);
+            if (byDiag.size() <= d) byDiag.resize(d + 1);
+            byDiag[d].push_back(make_int4(comp, I, J, K));
+          }
+    }
+    std::vector<int4> flat;
+    c->exDiagOff.push_back(0);
+    for (auto &v : byDiag) {
+      flat.insert(flat.end(), v.begin(), v.end());
+      c->exDiagOff.push_back((int)flat.size());
+    }
+    if (hipMalloc(&c->exTiles, flat.size() * sizeof(int4)) != hipSuccess) { ws_destroy(c); return WS_EHIP; }
+    hipMemcpy(c->exTiles, flat.data(), flat.size() * sizeof(int4), hipMemcpyHostToDevice);
+  }
+  if (hipDeviceSynchronize() != hipSuccess) { ws_destroy(c); return WS_EHIP; }
+  *out = c;
+  return WS_OK;
+}
+
+void ws_destroy(ws_ctx *c) {
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  void *ptrs[] = {c->staticTypes, c->gu, c->thr, c->terrainR, c->cellType, c->nbm, c->dg, c->q, c->rhs, c->pw,
+                  c->cellCount, c->cellStart, c->vort, c->vel, c->velO, c->wgt, c->scr, c->scr2, c->valid, c->dens,
+                  c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pT, c->pAir, c->advU, c->advV, c->advW, c->pflag,
+                  c->pcool, c->pcell, c->prank, c->sortedIdx, c->vneed, c->vscr, c->rng, c->ds, c->partials,
+                  c->scanTmp, c->exTiles, c->mtCount, c->mtPos, c->mtNrm, c->mtOff};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  if (c->hs) hipHostFree(c->hs);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *ws_last_error(const ws_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int ws_set_f(ws_ctx *c, const char *n, double v) {
+  if (!c || !n) return WS_EINVAL;
+#define SF(nm) if (!strcmp(n, #nm)) { c->nm = v; return WS_OK; }
+#define SI(nm) if (!strcmp(n, #nm)) { c->nm = (int)v; return WS_OK; }
+  SF(cfl) SF(pic) SF(gravity) SF(viscosity) SF(omega) SF(maxSpeed) SF(vorticity) SF(coreR) SF(oceanR)
+  SF(atmosphereH) SF(spacing) SF(particleVolume) SF(iso) SF(substeps) SF(rainT) SF(cloudT) SF(rainLiftAge)
+  SF(pressureTol) SF(dv) SF(sunX) SF(sunY) SF(sunZ) SF(dtFrameBase)
+  SI(pressureIters) SI(maxPressureIters) SI(residualEvery) SI(ceilReflect) SI(hasSun) SI(measureResidual)
+#undef SF
+#undef SI
+  if (!strcmp(n, "gRamp")) { c->gRamp = v; return push_ramp(c); }
+  if (!strcmp(n, "nP")) {
+    if (v < 0 || v > c->capacity) { c->err = "nP exceeds capacity"; return WS_EINVAL; }
+    c->nP = (int)v;
+    return WS_OK;
+  }
+  if (!strcmp(n, "tn")) { c->tn = (int)v; return WS_OK; }
+  if (!strcmp(n, "hasVapor")) { c->hasVapor = v != 0; return WS_OK; }
+  c->err = std::string("unknown scalar ") + n;
+  return WS_EINVAL;
+}
+
+int ws_get_f(ws_ctx *c, const char *n, double *v) {
+  if (!c || !n || !v) return WS_EINVAL;
+  int r;
+  if (!strcmp(n, "umax") || !strcmp(n, "airborneCount") || !strcmp(n, "gRamp") || !strcmp(n, "dtLast") ||
+      !strcmp(n, "dfRefined")) {
+    if ((r = pull_scalars(c))) return r;
+    if (!strcmp(n, "umax")) *v = c->umax;
+    else if (!strcmp(n, "airborneCount")) *v = c->airborne;
+    else if (!strcmp(n, "gRamp")) *v = c->gRamp;
+    else if (!strcmp(n, "dtLast")) *v = c->dtLast;
+    else *v = c->hs->df_refined;
+    return WS_OK;
+  }
+#define GF(nm) if (!strcmp(n, #nm)) { *v = c->nm; return WS_OK; }
+  GF(cfl) GF(pic) GF(gravity) GF(viscosity) GF(omega) GF(maxSpeed) GF(vorticity) GF(coreR) GF(oceanR)
+  GF(atmosphereH) GF(spacing) GF(particleVolume) GF(iso) GF(substeps) GF(pressureTol) GF(pressureIters)
+  GF(maxPressureIters) GF(nP) GF(capacity) GF(residual) GF(domainR) GF(dx) GF(tn) GF(dv)
+#undef GF
+  if (!strcmp(n, "pressureIters_done")) { *v = c->itersDone; return WS_OK; }
+  if (!strcmp(n, "rngCfS") || !strcmp(n, "rngBfS") || !strcmp(n, "rngEvS")) {
+    uint32_t rng[4];
+    WS_HIP(hipMemcpy(rng, c->rng, sizeof(rng), hipMemcpyDeviceToHost));
+    *v = n[4] == 'C' ? rng[0] : (n[4] == 'B' ? rng[1] : rng[2]);
+    return WS_OK;
+  }
+  c->err = std::string("unknown scalar ") + n;
+  return WS_EINVAL;
+}
+
+int ws_upload(ws_ctx *c, const char *name, const void *host, size_t nbytes) {
+  if (!c || !name || (!host && nbytes)) return WS_EINVAL;
+  Field f;
+  if (!strcmp(name, "terrainR")) {
+    long n = (long)c->tn * c->tn * c->tn;
+    if (n <= 0 || nbytes != (size_t)n * 4) { c->err = "terrainR: set tn first; size must be tn^3 floats"; return WS_EINVAL; }
+    if (c->terrainR) hipFree(c->terrainR);
+    c->terrainR = nullptr;
+    int r = dalloc(c, &c->terrainR, n);
+    if (r) return r;
+  }
+  if (!strcmp(name, "rng")) {
+    if (nbytes != 12) return WS_EINVAL;
+    WS_HIP(hipMemcpy(c->rng, host, 12, hipMemcpyHostToDevice));
+    return WS_OK;
+  }
+  if (!field_of(c, name, f)) { c->err = std::string("unknown field ") + name; return WS_EINVAL; }
+  if (nbytes != f.elem * (size_t)f.count) {
+    c->err = std::string("size mismatch for ") + name + ": expected " + std::to_string(f.elem * (size_t)f.count);
+    return WS_EINVAL;
+  }
+  WS_HIP(hipMemcpyAsync(f.dev, host, nbytes, hipMemcpyHostToDevice, c->stream));
+  WS_HIP(hipStreamSynchronize(c->stream));
+  if (!strcmp(name, "staticTypes")) c->haveStatic = true;
+  if (!strcmp(name, "terrainR")) c->haveTerrain = true;
+  if (!strcmp(name, "pflag")) {
+    const uint8_t *fl = (const uint8_t *)host;
+    for (long i = 0; i < f.count; i++)
+      if (fl[i] == 2 || fl[i] == 3) { c->hasVapor = true; break; }
+  }
+  return WS_OK;
+}
+
+int ws_download(ws_ctx *c, const char *name, void *host, size_t nbytes) {
+  if (!c || !name || (!host && nbytes)) return WS_EINVAL;
+  if (!strcmp(name, "rng")) {
+    if (nbytes != 12) return WS_EINVAL;
+    WS_HIP(hipMemcpy(host, c->rng, 12, hipMemcpyDeviceToHost));
+    return WS_OK;
+  }
+  Field f;
+  if (!field_of(c, name, f)) { c->err = std::string("unknown field ") + name; return WS_EINVAL; }
+  if (nbytes != f.elem * (size_t)f.count) {
+    c->err = std::string("size mismatch for ") + name + ": expected " + std::to_string(f.elem * (size_t)f.count);
+    return WS_EINVAL;
+  }
+  WS_HIP(hipMemcpyAsync(host, f.dev, nbytes, hipMemcpyDeviceToHost, c->stream));
+  WS_HIP(hipStreamSynchronize(c->stream));
+  return WS_OK;
+}
+
+int ws_substep(ws_ctx *c, double dt) {
+  if (!c || !(dt > 0)) return WS_EINVAL;
+  int r = substep(c, dt, 0);
+  if (r) return r;
+  if ((r = pull_scalars(c))) return r;
+  c->dtLast = dt;
+  return WS_OK;
+}
+
+int ws_run_substeps(ws_ctx *c, int n) {
+  if (!c || n < 0) return WS_EINVAL;
+  for (int i = 0; i < n; i++) {
+    int r = substep(c, 0.01, 1);
+    if (r) return r;
+  }
+  return WS_OK;
+}
+
+int ws_step_frame(ws_ctx *c, double dtFrame, int *n_sub) {
+  if (!c || !(dtFrame > 0)) return WS_EINVAL;
+  // _stepSync's substep loop, solver.js:2682-2695
+  int r;
+  if ((r = pull_scalars(c))) return r;
+  double remaining = dtFrame, base = dtFrame / (c->substeps > 1 ? c->substeps : 1);
+  int n = 0;
+  while (remaining > 1e-6 && n < 10) {
+    double um = c->umax > 0.08 ? c->umax : 0.08;
+    double dt = c->cfl * c->dx / um;
+    if (base < dt) dt = base;
+    if (dt > 0.01) dt = 0.01;
+    if (dt < 1e-5) dt = 1e-5;
+    if ((r = substep(c, dt, 0))) return r;
+    if ((r = pull_scalars(c))) return r;
+    remaining -= dt;
+    n++;
+  }
+  if (n_sub) *n_sub = n;
+  return WS_OK;
+}
+
+int ws_stage(ws_ctx *c, int stage, double dt) {
+  if (!c) return WS_EINVAL;
+  int r = need_world(c);
+  if (r) return r;
+  if (stage != WS_STAGE_GRAVITY)
+    hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, c->stream, c->ds, (float)dt);
+  switch (stage) {
+    case WS_STAGE_CLEAR:
+      WS_HIP(hipMemsetAsync(c->vel, 0, sizeof(float) * nfaces(c), c->stream));
+      WS_HIP(hipMemsetAsync(c->wgt, 0, sizeof(float) * nfaces(c), c->stream));
+      break;
+    case WS_STAGE_RASTERIZE: r = st_rasterize(c); break;
+    case WS_STAGE_P2G: r = st_p2g(c); break;
+    case WS_STAGE_BC: r = st_bc(c, c->scr, 0); break;
+    case WS_STAGE_SNAPSHOT:
+      WS_HIP(hipMemcpyAsync(c->velO, c->vel, sizeof(float) * nfaces(c), hipMemcpyDeviceToDevice, c->stream));
+      break;
+    case WS_STAGE_VISCOSITY:
+      WS_HIP(hipMemcpyAsync(c->scr, c->vel, sizeof(float) * nfaces(c), hipMemcpyDeviceToDevice, c->stream));
+      r = st_viscosity(c, dt, c->scr);
+      break;
+    case WS_STAGE_GRAVITY:
+      hipLaunchKernelGGL(k_begin_substep, dim3(1), dim3(64), 0, c->stream, c->ds, (float)dt, 0, 0.f, 0.f,
+                         (float)c->gravity);
+      r = st_gravity(c);
+      break;
+    case WS_STAGE_PRESSURE: r = st_pressure(c); break;
+    case WS_STAGE_PROJECT: r = st_project(c, 0); break;
+    case WS_STAGE_VORTICITY: r = st_vorticity(c); break;
+    case WS_STAGE_CLAMP:
+      hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, c->stream, c->ds);
+      r = st_clamp(c);
+      break;
+    case WS_STAGE_EXTRAPOLATE:
+      hipLaunchKernelGGL(k_valid, dim3(blocks_for(nfaces(c), 256)), dim3(256), 0, c->stream, c->geo, c->cellType,
+                         c->wgt, c->valid);
+      r = st_extrapolate(c);
+      break;
+    case WS_STAGE_G2P:
+      hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, c->stream, c->ds);
+      r = st_particles(c, 1);
+      break;
+    case WS_STAGE_ADVECT: r = st_particles(c, 2); break;
+    default: c->err = "unknown stage"; return WS_EINVAL;
+  }
+  if (r) return r;
+  WS_HIP(hipStreamSynchronize(c->stream));
+  return pull_scalars(c);
+}
+
+int ws_sync(ws_ctx *c) {
+  if (!c) return WS_EINVAL;
+  WS_HIP(hipStreamSynchronize(c->stream));
+  return WS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,667 @@
+// Grid-side kernels of the substep (js/solver.js:865-1636): rasterize + mark,
+// particle binning, deterministic P2G, boundary conditions, viscosity,
+// gravity, pressure assembly, red-black SOR, projection, vorticity
+// confinement, grid clamp and the extrapolation flood.
+#include "ws_kernels.h"
+
+namespace ws {
+
+// ---------------------------------------------------------------- scalars
+// dt and the gravity ramp of this substep (solver.js:2656-2662); mode 1 takes
+// dt from the CFL law of _stepSync (solver.js:2689-2690) on the device.
+__global__ void k_begin_substep(DevScalars *s, float dt_host, int use_law, float cfl_dx, float dt_base,
+                                float gravity) {
+  if (threadIdx.x != 0) return;
+  float dt = dt_host;
+  if (use_law) {
+    float um = __uint_as_float(s->umax_bits);
+    float law = cfl_dx / fmaxf(um, 0.08f);
+    dt = fminf(fminf(dt_base, law), 0.01f);
+    if (dt < 1e-5f) dt = 1e-5f;
+  }
+  double gsd = s->gRamp / 1.5;
+  if (gsd > 1) gsd = 1;
+  s->gRamp += (double)dt;
+  s->gs = gsd <= 0 ? 0.f : (float)(gsd * (double)gravity);
+  s->dt = dt;
+  s->simTime += dt;
+}
+
+__global__ void k_end_substep(DevScalars *s) {
+  if (threadIdx.x == 0) s->dtLast = s->dt;
+}
+
+// ------------------------------------------------------------- rasterize
+// _rasterizeStatic (cached copy, solver.js:868-870) and zeroed bin counts
+__global__ void k_init_cells(const uint8_t *__restrict__ st, uint8_t *__restrict__ ct, unsigned *__restrict__ cnt,
+                             int nC) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nC) return;
+  ct[c] = st[c];
+  cnt[c] = 0;
+}
+
+// "sea in the sky" test of _markFluidCells (solver.js:964-975) for a float
+// position: float filter, float-float when the float margin is inside its
+// error band. Returns 1 = demote.
+__device__ int sky_demote(const World &w, const Geo &g, float x, float y, float z, unsigned *refined) {
+  float rx = x - g.cx, ry = y - g.cy, rz = z - g.cz;
+  float r2 = rx * rx + ry * ry + rz * rz;
+  float lim = w.lim_df.hi;
+  int c1;
+  if (r2 > lim * (1.f + 1e-4f)) c1 = 1;
+  else if (r2 < lim * (1.f - 1e-4f)) c1 = 0;
+  else c1 = -1;
+  df r2d;
+  bool have_r2d = false;
+  if (c1 < 0) {
+    df ex = df_sub(df_make(x), w.cxd), ey = df_sub(df_make(y), w.cyd), ez = df_sub(df_make(z), w.czd);
+    r2d = df_add(df_add(df_mul(ex, ex), df_mul(ey, ey)), df_mul(ez, ez));
+    have_r2d = true;
+    c1 = df_gt0(df_sub(r2d, w.lim_df)) ? 1 : 0;
+    atomicAdd(refined, 1u);
+  }
+  if (!c1) return 0;
+  float m = sqrtf(r2) - terrain_radius(w, x, y, z) - w.skyAlt;
+  int c2;
+  if (m > 2e-5f) c2 = 1;
+  else if (m < -2e-5f) c2 = 0;
+  else {
+    if (!have_r2d) {
+      df ex = df_sub(df_make(x), w.cxd), ey = df_sub(df_make(y), w.cyd), ez = df_sub(df_make(z), w.czd);
+      r2d = df_add(df_add(df_mul(ex, ex), df_mul(ey, ey)), df_mul(ez, ez));
+    }
+    df md = df_sub(df_sub(df_sqrt(r2d), terrain_radius_df(w, x, y, z)), w.skyAlt_df);
+    c2 = df_gt0(md) ? 1 : 0;
+    atomicAdd(refined, 1u);
+  }
+  if (!c2) return 0;
+  return rock_cell(w, g, x, y, z) ? 0 : 1;
+}
+
+// ceiling guard on the planet (solver.js:979-991): airborne > 0.3 over ground
+__device__ int ceiling_demote(const World &w, const Geo &g, float x, float y, float z, unsigned *refined) {
+  float rx = x - g.cx, ry = y - g.cy, rz = z - g.cz;
+  float m = sqrtf(rx * rx + ry * ry + rz * rz) - terrain_radius(w, x, y, z) - 0.3f;
+  if (m > 2e-5f) return 1;
+  if (m < -2e-5f) return 0;
+  df ex = df_sub(df_make(x), w.cxd), ey = df_sub(df_make(y), w.cyd), ez = df_sub(df_make(z), w.czd);
+  df r2d = df_add(df_add(df_mul(ex, ex), df_mul(ey, ey)), df_mul(ez, ez));
+  df md = df_sub(df_sub(df_sqrt(r2d), terrain_radius_df(w, x, y, z)), w.p3_df);
+  atomicAdd(refined, 1u);
+  return df_gt0(md) ? 1 : 0;
+}
+
+// _markFluidCells, solver.js:945-1000, plus the counting-sort histogram of
+// the particles that stay grid-coupled (their bin is the cell they mark).
+__global__ void k_mark(Geo g, World w, int nP, const float *__restrict__ px, const float *__restrict__ py,
+                       const float *__restrict__ pz, uint8_t *__restrict__ pflag, uint8_t *__restrict__ ct,
+                       unsigned *__restrict__ cnt, int *__restrict__ pcell, unsigned *__restrict__ prank,
+                       DevScalars *s) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nP) return;
+  if (pflag[p] != 0) { pcell[p] = -1; return; }
+  float x = px[p], y = py[p], z = pz[p];
+  if (sky_demote(w, g, x, y, z, &s->df_refined)) { pflag[p] = 1; pcell[p] = -1; return; }
+  int i = clampi(cell_floor(x, g.inv_dx, w.thr, g.nx), 0, g.nx - 1);
+  int j = clampi(cell_floor(y, g.inv_dx, w.thr, g.ny), 0, g.ny - 1);
+  int k = clampi(cell_floor(z, g.inv_dx, w.thr, g.nz), 0, g.nz - 1);
+  if (j >= g.ny - 2 && ceiling_demote(w, g, x, y, z, &s->df_refined)) { pflag[p] = 1; pcell[p] = -1; return; }
+  int c = (k * g.ny + j) * g.nx + i;
+  if (w.st[c] == AIR) ct[c] = FLUID;  // idempotent byte store
+  pcell[p] = c;
+  prank[p] = atomicAdd(&cnt[c], 1u);
+}
+
+__global__ void k_scatter(int nP, const int *__restrict__ pcell, const unsigned *__restrict__ prank,
+                          const unsigned *__restrict__ start, unsigned *__restrict__ sorted) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nP) return;
+  int c = pcell[p];
+  if (c >= 0) sorted[start[c] + prank[p]] = (unsigned)p;
+}
+
+// -------------------------------------------------------------------- P2G
+// _p2gInto + _p2gNormalizeFace, solver.js:1035-1120. One workgroup per 8^3
+// cell tile owns the tile's faces; it visits the particles binned in the
+// tile and its one-cell halo (their trilinear footprint reaches one cell) and
+// accumulates momentum and weight in LDS as 64-bit fixed point. Integer
+// addition is exact and order independent, so the result is deterministic
+// without float atomics; the reference's sequential f32 sums differ from the
+// exact sums by float rounding only.
+__device__ __forceinline__ long long fix(float v) { return (long long)__builtin_rintf(v * kFixScaleF); }
+
+__global__ __launch_bounds__(256) void k_p2g_tile(Geo g, int tilesX, int tilesY, const unsigned *__restrict__ start,
+                                                  const unsigned *__restrict__ sorted, const float *__restrict__ px,
+                                                  const float *__restrict__ py, const float *__restrict__ pz,
+                                                  const float *__restrict__ pvx, const float *__restrict__ pvy,
+                                                  const float *__restrict__ pvz, float *__restrict__ vel,
+                                                  float *__restrict__ wgt) {
+  constexpr int T = P2G_T;
+  constexpr int BOX = (T + 1) * T * T;  // owned faces of one component (+1 plane for the last tile)
+  __shared__ unsigned long long accM[3][BOX];
+  __shared__ unsigned long long accW[3][BOX];
+  __shared__ unsigned rowBeg[(T + 2) * (T + 2)];
+  __shared__ unsigned rowPre[(T + 2) * (T + 2) + 1];
+
+  int b = xcd_block(blockIdx.x, gridDim.x);
+  int tx = b % tilesX, ty = (b / tilesX) % tilesY, tz = b / (tilesX * tilesY);
+  int i0 = tx * T, j0 = ty * T, k0 = tz * T;
+  for (int e = threadIdx.x; e < 3 * BOX; e += blockDim.x) {
+    (&accM[0][0])[e] = 0ull;
+    (&accW[0][0])[e] = 0ull;
+  }
+  // particle rows of the tile + halo: cells [i0-1, i0+T] x [j0-1, j0+T] x [k0-1, k0+T]
+  int ci0 = max(i0 - 1, 0), ci1 = min(i0 + T, g.nx - 1);
+  int cj0 = max(j0 - 1, 0), cj1 = min(j0 + T, g.ny - 1);
+  int ck0 = max(k0 - 1, 0), ck1 = min(k0 + T, g.nz - 1);
+  int nrj = cj1 - cj0 + 1, nrows = nrj * (ck1 - ck0 + 1);
+  if ((int)threadIdx.x < nrows) {
+    int r = threadIdx.x;
+    int cj = cj0 + r % nrj, ck = ck0 + r / nrj;
+    int row = (ck * g.ny + cj) * g.nx;
+    unsigned a = start[row + ci0], e = start[row + ci1 + 1];
+    rowBeg[r] = a;
+    rowPre[r + 1] = e - a;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned acc = 0;
+    rowPre[0] = 0;
+    for (int r = 1; r <= nrows; r++) { acc += rowPre[r]; rowPre[r] = acc; }
+  }
+  __syncthreads();
+  unsigned total = rowPre[nrows];
+  // owned face ranges per component: [lo, hi) in i, j, k
+  int iHiC = min(i0 + T, g.nx), jHiC = min(j0 + T, g.ny), kHiC = min(k0 + T, g.nz);
+  for (unsigned vI = threadIdx.x; vI < total; vI += blockDim.x) {
+    int lo = 0, hi = nrows;  // last r with rowPre[r] <= vI
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (rowPre[mid] <= vI) lo = mid; else hi = mid;
+    }
+    unsigned p = sorted[rowBeg[lo] + (vI - rowPre[lo])];
+    float x = px[p], y = py[p], z = pz[p];
+    float vc[3] = {pvx[p], pvy[p], pvz[p]};
+#pragma unroll
+    for (int comp = 0; comp < 3; comp++) {
+      float a = x * g.inv_dx, bb = y * g.inv_dx, cc = z * g.inv_dx;
+      if (comp == 0) { bb -= 0.5f; cc -= 0.5f; }
+      else if (comp == 1) { a -= 0.5f; cc -= 0.5f; }
+      else { a -= 0.5f; bb -= 0.5f; }
+      int fi = (int)floorf(a), fj = (int)floorf(bb), fk = (int)floorf(cc);
+      float txf, tyf, tzf;
+      if (fi < 0) { fi = 0; txf = 0.f; } else if (fi > g.nx - 1) { fi = g.nx - 1; txf = 1.f; } else txf = a - (float)fi;
+      if (fj < 0) { fj = 0; tyf = 0.f; } else if (fj > g.ny - 2) { fj = g.ny - 2; tyf = 1.f; } else tyf = bb - (float)fj;
+      if (fk < 0) { fk = 0; tzf = 0.f; } else if (fk > g.nz - 2) { fk = g.nz - 2; tzf = 1.f; } else tzf = cc - (float)fk;
+      float s0 = 1.f - txf, s1 = txf, t0 = 1.f - tyf, t1 = tyf, r0 = 1.f - tzf, r1 = tzf;
+      float wA = s0 * t0, wB = s1 * t0, wC = s0 * t1, wD = s1 * t1;
+      int d0 = comp == 0 ? g.nx + 1 : g.nx, d1 = comp == 1 ? g.ny + 1 : g.ny;
+      // owned ranges of this component
+      int oi1 = comp == 0 ? (i0 + T >= g.nx ? g.nx + 1 : i0 + T) : iHiC;
+      int oj1 = comp == 1 ? (j0 + T >= g.ny ? g.ny + 1 : j0 + T) : jHiC;
+      int ok1 = comp == 2 ? (k0 + T >= g.nz ? g.nz + 1 : k0 + T) : kHiC;
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        int di = e & 1, dj = (e >> 1) & 1, dk = (e >> 2) & 1;
+        float wt = (di ? (dj ? wD : wB) : (dj ? wC : wA)) * (dk ? r1 : r0);
+        int ii = fi + di, jj = fj + dj, kk = fk + dk;
+        // v/w stencils clamped at i0 = nx-1 read the next row (the
+        // reference's idx+1 at the +x edge, solver.js:765,772)
+        if (ii >= d0) { ii -= d0; jj += 1; if (jj >= d1) { jj = 0; kk += 1; } }
+        if (ii < i0 || ii >= oi1 || jj < j0 || jj >= oj1 || kk < k0 || kk >= ok1) continue;
+        int l = ((kk - k0) * (T + (comp == 1)) + (jj - j0)) * (T + (comp == 0)) + (ii - i0);
+        atomicAdd(&accM[comp][l], (unsigned long long)fix(wt * vc[comp]));
+        atomicAdd(&accW[comp][l], (unsigned long long)fix(wt));
+      }
+    }
+  }
+  __syncthreads();
+  // normalise and write the owned faces
+  for (int comp = 0; comp < 3; comp++) {
+    int d0 = comp == 0 ? g.nx + 1 : g.nx, d1 = comp == 1 ? g.ny + 1 : g.ny;
+    int oi1 = comp == 0 ? (i0 + T >= g.nx ? g.nx + 1 : i0 + T) : iHiC;
+    int oj1 = comp == 1 ? (j0 + T >= g.ny ? g.ny + 1 : j0 + T) : jHiC;
+    int ok1 = comp == 2 ? (k0 + T >= g.nz ? g.nz + 1 : k0 + T) : kHiC;
+    int ni = oi1 - i0, nj = oj1 - j0, nk = ok1 - k0;
+    long base = comp == 0 ? 0 : (comp == 1 ? g.uN : (long)g.uN + g.vN);
+    for (int e = threadIdx.x; e < ni * nj * nk; e += blockDim.x) {
+      int li = e % ni, lj = (e / ni) % nj, lk = e / (ni * nj);
+      int l = (lk * (T + (comp == 1)) + lj) * (T + (comp == 0)) + li;
+      float W = (float)(long long)accW[comp][l] * kFixInvF;
+      float M = (float)(long long)accM[comp][l] * kFixInvF;
+      long f = base + ((long)(k0 + lk) * d1 + (j0 + lj)) * d0 + (i0 + li);
+      vel[f] = W > 1e-8f ? M / W : M;
+      wgt[f] = W;
+    }
+  }
+}
+
+// ------------------------------------------------- boundary conditions
+// Face classification shared by BC, gravity, projection and extrapolation.
+struct FaceInfo {
+  int comp, i, j, k;
+  int cl, cr;  // adjacent cells (-1 if outside)
+  bool plane;  // domain boundary plane
+};
+
+__device__ __forceinline__ bool face_of(const Geo &g, long f, FaceInfo &fi) {
+  if (f < g.uN) {
+    fi.comp = 0;
+    fi.i = (int)(f % (g.nx + 1));
+    long r = f / (g.nx + 1);
+    fi.j = (int)(r % g.ny);
+    fi.k = (int)(r / g.ny);
+    fi.plane = fi.i == 0 || fi.i == g.nx;
+    int c = (fi.k * g.ny + fi.j) * g.nx + fi.i;
+    fi.cl = fi.i > 0 ? c - 1 : -1;
+    fi.cr = fi.i < g.nx ? c : -1;
+  } else if (f < (long)g.uN + g.vN) {
+    f -= g.uN;
+    fi.comp = 1;
+    fi.i = (int)(f % g.nx);
+    long r = f / g.nx;
+    fi.j = (int)(r % (g.ny + 1));
+    fi.k = (int)(r / (g.ny + 1));
+    fi.plane = fi.j == 0 || fi.j == g.ny;
+    int c = (fi.k * g.ny + fi.j) * g.nx + fi.i;
+    fi.cl = fi.j > 0 ? c - g.nx : -1;
+    fi.cr = fi.j < g.ny ? c : -1;
+  } else {
+    f -= (long)g.uN + g.vN;
+    if (f >= g.wN) return false;
+    fi.comp = 2;
+    fi.i = (int)(f % g.nx);
+    long r = f / g.nx;
+    fi.j = (int)(r % g.ny);
+    fi.k = (int)(r / g.ny);
+    fi.plane = fi.k == 0 || fi.k == g.nz;
+    int c = (fi.k * g.ny + fi.j) * g.nx + fi.i;
+    fi.cl = fi.k > 0 ? c - g.nx * g.ny : -1;
+    fi.cr = fi.k < g.nz ? c : -1;
+  }
+  return true;
+}
+
+// _applyBC (solver.js:1130-1171, static solids: sv = 0), FLIP snapshot
+// (:2579) and the extrapolation validity mark (:1578-1611: weighted, a
+// domain plane or solid-adjacent). With fuse_gravity the radial gravity of
+// _applyPlanetGravitySlice (:1229-1256) and the second _applyBC follow in
+// the same pass (used when viscosity is off).
+__global__ void k_bc_snapshot(Geo g, const uint8_t *__restrict__ ct, float *__restrict__ vel,
+                              float *__restrict__ velO, const float *__restrict__ wgt,
+                              uint8_t *__restrict__ valid, int fuse_gravity, const float *__restrict__ gvec,
+                              const DevScalars *__restrict__ s, int write_valid) {
+  long F = (long)g.uN + g.vN + g.wN;
+  long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  FaceInfo fi;
+  face_of(g, f, fi);
+  uint8_t tl = fi.cl >= 0 ? ct[fi.cl] : SOLID, tr = fi.cr >= 0 ? ct[fi.cr] : SOLID;
+  bool bc = fi.plane || tl == SOLID || tr == SOLID;
+  float u = bc ? 0.f : vel[f];
+  velO[f] = u;
+  if (write_valid) valid[f] = (wgt[f] > 1e-8f || bc) ? 1 : 0;
+  if (fuse_gravity && !bc && s->gs > 0.f && (tl == FLUID || tr == FLUID)) u = u + gvec[f] * s->dt * s->gs;
+  vel[f] = u;
+}
+
+// extrapolation validity alone (solver.js:1578-1611), for the stand-alone
+// _extrapolate stage; the substep computes it inside k_bc_snapshot
+__global__ void k_valid(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ wgt,
+                        uint8_t *__restrict__ valid) {
+  long F = (long)g.uN + g.vN + g.wN;
+  long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  FaceInfo fi;
+  face_of(g, f, fi);
+  uint8_t tl = fi.cl >= 0 ? ct[fi.cl] : SOLID, tr = fi.cr >= 0 ? ct[fi.cr] : SOLID;
+  valid[f] = (wgt[f] > 1e-8f || fi.plane || tl == SOLID || tr == SOLID) ? 1 : 0;
+}
+
+// radial gravity + _applyBC after viscosity
+__global__ void k_gravity_bc(Geo g, const uint8_t *__restrict__ ct, float *__restrict__ vel,
+                             const float *__restrict__ gvec, const DevScalars *__restrict__ s) {
+  long F = (long)g.uN + g.vN + g.wN;
+  long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  FaceInfo fi;
+  face_of(g, f, fi);
+  uint8_t tl = fi.cl >= 0 ? ct[fi.cl] : SOLID, tr = fi.cr >= 0 ? ct[fi.cr] : SOLID;
+  bool bc = fi.plane || tl == SOLID || tr == SOLID;
+  if (bc) { vel[f] = 0.f; return; }
+  if (s->gs > 0.f && (tl == FLUID || tr == FLUID)) vel[f] = vel[f] + gvec[f] * s->dt * s->gs;
+}
+
+// ------------------------------------------------------------ viscosity
+// _viscosity, solver.js:1259-1304, is one in-place lexicographic sweep over
+// the fluid-fluid faces: a face reads the already-updated -i/-j/-k
+// neighbours. That is a lower-triangular solve, evaluated here exactly (to
+// float rounding) by fixed-point iteration: x1 = Jacobi from the old field,
+// x(m+1) replaces the lower neighbours by x(m). The error shrinks by 3*coef
+// per iteration, so a few iterations reach float resolution (the host picks
+// the count from coef).
+__global__ void k_visc_iter(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ old,
+                            const float *__restrict__ cur, float *__restrict__ out, float coef, int first) {
+  long F = (long)g.uN + g.vN + g.wN;
+  long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  FaceInfo fi;
+  face_of(g, f, fi);
+  float u0 = old[f];
+  if (fi.cl < 0 || fi.cr < 0 || ct[fi.cl] != FLUID || ct[fi.cr] != FLUID) { out[f] = u0; return; }
+  int d0 = fi.comp == 0 ? g.nx + 1 : g.nx, d1 = fi.comp == 1 ? g.ny + 1 : g.ny, d2 = fi.comp == 2 ? g.nz + 1 : g.nz;
+  long s1 = d0, s2 = (long)d0 * d1;
+  int nxy = g.nx * g.ny;
+  // is the lower neighbour (same component) also a fluid-fluid face?
+  auto ffLower = [&](int dir) -> bool {
+    // the lower neighbour along the face normal is a domain plane at index 0
+    if (dir == fi.comp && (dir == 0 ? fi.i : dir == 1 ? fi.j : fi.k) - 1 == 0) return false;
+    // otherwise its cells are this face's cells shifted by -1 along dir
+    int step = dir == 0 ? 1 : (dir == 1 ? g.nx : nxy);
+    return ct[fi.cl - step] == FLUID && ct[fi.cr - step] == FLUID;
+  };
+  float val = u0 * 6.f;
+  // -i
+  if (fi.i - 1 >= 0) val -= (!first && ffLower(0)) ? cur[f - 1] : old[f - 1]; else val -= u0;
+  if (fi.i + 1 < d0) val -= old[f + 1]; else val -= u0;
+  if (fi.j - 1 >= 0) val -= (!first && ffLower(1)) ? cur[f - s1] : old[f - s1]; else val -= u0;
+  if (fi.j + 1 < d1) val -= old[f + s1]; else val -= u0;
+  if (fi.k - 1 >= 0) val -= (!first && ffLower(2)) ? cur[f - s2] : old[f - s2]; else val -= u0;
+  if (fi.k + 1 < d2) val -= old[f + s2]; else val -= u0;
+  out[f] = u0 + coef * val;
+}
+
+// ------------------------------------------------------ pressure system
+// _pressureSolve assembly, solver.js:1322-1394: rhs = -dx^2 div, diag =
+// fluid + air neighbours, relaxation weight omega/diag (stored float as the
+// reference's pW), fluid-neighbour mask; non-fluid cells hold q = 0.
+__global__ void k_assemble(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ vel,
+                           float *__restrict__ q, float *__restrict__ rhs, float *__restrict__ pw,
+                           uint8_t *__restrict__ nbm, uint8_t *__restrict__ dg, float omega) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nC) return;
+  if (ct[c] != FLUID) { q[c] = 0.f; pw[c] = 0.f; nbm[c] = 0; dg[c] = 0; return; }
+  int i = c % g.nx, j = (c / g.nx) % g.ny, k = c / (g.nx * g.ny);
+  int nxy = g.nx * g.ny;
+  int nb[6] = {i > 0 ? c - 1 : -1, i < g.nx - 1 ? c + 1 : -1, j > 0 ? c - g.nx : -1,
+               j < g.ny - 1 ? c + g.nx : -1, k > 0 ? c - nxy : -1, k < g.nz - 1 ? c + nxy : -1};
+  int Nf = 0, Na = 0;
+  unsigned m = 0x40;  // bit 6: fluid cell
+  for (int d = 0; d < 6; d++) {
+    if (nb[d] < 0) continue;
+    uint8_t t = ct[nb[d]];
+    if (t == FLUID) { Nf++; m |= 1u << d; }
+    else if (t == AIR) Na++;
+  }
+  const float *u = vel, *v = vel + g.uN, *w = vel + g.uN + g.vN;
+  int iu = (k * g.ny + j) * (g.nx + 1) + i, iv = (k * (g.ny + 1) + j) * g.nx + i;
+  float dsum = u[iu + 1] - u[iu] + v[iv + g.nx] - v[iv] + w[c + nxy] - w[c];
+  float div = dsum / g.dx;
+  rhs[c] = -g.dx * g.dx * div;
+  int diag = Nf + Na;
+  pw[c] = diag > 0 ? omega / (float)diag : 0.f;
+  nbm[c] = (uint8_t)m;
+  dg[c] = (uint8_t)diag;
+}
+
+// One colour of red-black SOR (the relaxation of solver.js:1402): cells with
+// (i+j+k)&1 == color. Each thread updates one cell of that colour.
+__global__ __launch_bounds__(256) void k_sor(Geo g, float *__restrict__ q, const float *__restrict__ rhs,
+                                             const float *__restrict__ pw, const uint8_t *__restrict__ nbm,
+                                             int color, float om1) {
+  int half = (g.nx + 1) >> 1;
+  int b = xcd_block(blockIdx.x, gridDim.x);
+  int t = b * blockDim.x + threadIdx.x;
+  int rowi = t / half, h = t % half;
+  if (rowi >= g.ny * g.nz) return;
+  int j = rowi % g.ny, k = rowi / g.ny;
+  int i = 2 * h + ((j + k + color) & 1);
+  if (i >= g.nx) return;
+  int c = rowi * g.nx + i;
+  unsigned m = nbm[c];
+  if (!(m & 0x40)) return;
+  int nxy = g.nx * g.ny;
+  float sum = 0.f;
+  if (m & 1) sum += q[c - 1];
+  if (m & 2) sum += q[c + 1];
+  if (m & 4) sum += q[c - g.nx];
+  if (m & 8) sum += q[c + g.nx];
+  if (m & 16) sum += q[c - nxy];
+  if (m & 32) sum += q[c + nxy];
+  q[c] = q[c] * om1 + pw[c] * (rhs[c] + sum);
+}
+
+// ||rhs - A q||^2 and ||rhs||^2 over fluid cells, FP64 block partials
+__global__ __launch_bounds__(256) void k_residual(Geo g, const float *__restrict__ q, const float *__restrict__ rhs,
+                                                  const uint8_t *__restrict__ nbm, const uint8_t *__restrict__ dg,
+                                                  double *__restrict__ partials) {
+  __shared__ double sr[256], sb[256];
+  double rr = 0, bb = 0;
+  int nxy = g.nx * g.ny;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.nC; c += gridDim.x * blockDim.x) {
+    unsigned m = nbm[c];
+    if (!(m & 0x40)) continue;
+    float sum = 0.f;
+    if (m & 1) sum += q[c - 1];
+    if (m & 2) sum += q[c + 1];
+    if (m & 4) sum += q[c - g.nx];
+    if (m & 8) sum += q[c + g.nx];
+    if (m & 16) sum += q[c - nxy];
+    if (m & 32) sum += q[c + nxy];
+    double r = (double)rhs[c] - ((double)dg[c] * (double)q[c] - (double)sum);
+    rr += r * r;
+    bb += (double)rhs[c] * (double)rhs[c];
+  }
+  sr[threadIdx.x] = rr;
+  sb[threadIdx.x] = bb;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) { sr[threadIdx.x] += sr[threadIdx.x + o]; sb[threadIdx.x] += sb[threadIdx.x + o]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { partials[2 * blockIdx.x] = sr[0]; partials[2 * blockIdx.x + 1] = sb[0]; }
+}
+
+// fixed-order sum of the block partials (deterministic)
+__global__ void k_residual_final(const double *__restrict__ partials, int n, DevScalars *s) {
+  __shared__ double sr[256], sb[256];
+  double rr = 0, bb = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) { rr += partials[2 * i]; bb += partials[2 * i + 1]; }
+  sr[threadIdx.x] = rr;
+  sb[threadIdx.x] = bb;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) { sr[threadIdx.x] += sr[threadIdx.x + o]; sb[threadIdx.x] += sb[threadIdx.x + o]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { s->rr = sr[0]; s->bb = sb[0]; }
+}
+
+// ---------------------------------------------------------- projection
+// _projectSlice, solver.js:1417-1443, with the grid clamp and umax of
+// _clampUmSlice (solver.js:2629-2653) fused when confinement is off.
+__device__ __forceinline__ void umax_reduce(float av, unsigned *umax_bits) {
+  // max of non-negative floats = max of their bit patterns; NaN never wins
+  unsigned bits = (av == av) ? __float_as_uint(av) : 0u;
+  for (int o = 32; o > 0; o >>= 1) bits = max(bits, (unsigned)__shfl_xor((int)bits, o));
+  if ((threadIdx.x & 63) == 0 && bits) atomicMax(umax_bits, bits);
+}
+
+__global__ void k_project(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ phi,
+                          float *__restrict__ vel, int do_clamp, float vcap, DevScalars *s) {
+  long F = (long)g.uN + g.vN + g.wN;
+  long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  float av = 0.f;
+  if (f < F) {
+    FaceInfo fi;
+    face_of(g, f, fi);
+    float u = vel[f];
+    if (!fi.plane) {
+      uint8_t tl = ct[fi.cl], tr = ct[fi.cr];
+      if (tl != SOLID && tr != SOLID) {
+        float pL = tl == FLUID ? phi[fi.cl] : 0.f, pR = tr == FLUID ? phi[fi.cr] : 0.f;
+        u = u - (pR - pL) * g.inv_dx;
+      }
+    }
+    if (do_clamp) {
+      if (u > vcap) u = vcap; else if (u < -vcap) u = -vcap;
+      av = fabsf(u);
+    }
+    vel[f] = u;
+  }
+  if (do_clamp) umax_reduce(av, &s->umax_bits);
+}
+
+__global__ void k_clamp(long F, float *__restrict__ vel, float vcap, DevScalars *s) {
+  long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  float av = 0.f;
+  if (f < F) {
+    float u = vel[f];
+    if (u > vcap) u = vcap; else if (u < -vcap) u = -vcap;
+    vel[f] = u;
+    av = fabsf(u);
+  }
+  umax_reduce(av, &s->umax_bits);
+}
+
+// ------------------------------------------------ vorticity confinement
+// _vorticityConfinement, solver.js:1454-1559, in three cell passes and one
+// face pass. scratch layout: vc[3C] centred velocity, om[4C] (x,y,z,|w|),
+// fc[3C] velocity deltas of interior fluid cells.
+__global__ void k_vort_center(Geo g, const float *__restrict__ vel, float *__restrict__ vc) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nC) return;
+  int i = c % g.nx, j = (c / g.nx) % g.ny, k = c / (g.nx * g.ny);
+  const float *u = vel, *v = vel + g.uN, *w = vel + g.uN + g.vN;
+  int ru = (k * g.ny + j) * (g.nx + 1) + i, rv = (k * (g.ny + 1) + j) * g.nx + i;
+  vc[c] = 0.5f * (u[ru] + u[ru + 1]);
+  vc[g.nC + c] = 0.5f * (v[rv] + v[rv + g.nx]);
+  vc[2 * g.nC + c] = 0.5f * (w[c] + w[c + g.nx * g.ny]);
+}
+
+__global__ void k_vort_omega(Geo g, const float *__restrict__ vc, float *__restrict__ om, float inv2) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nC) return;
+  int i = c % g.nx, j = (c / g.nx) % g.ny, k = c / (g.nx * g.ny), nxy = g.nx * g.ny;
+  int im = i > 0 ? c - 1 : c, ip = i < g.nx - 1 ? c + 1 : c;
+  int jm = j > 0 ? c - g.nx : c, jp = j < g.ny - 1 ? c + g.nx : c;
+  int km = k > 0 ? c - nxy : c, kp = k < g.nz - 1 ? c + nxy : c;
+  const float *cx = vc, *cy = vc + g.nC, *cz = vc + 2 * g.nC;
+  float wx = (cz[jp] - cz[jm] - (cy[kp] - cy[km])) * inv2;
+  float wy = (cx[kp] - cx[km] - (cz[ip] - cz[im])) * inv2;
+  float wz = (cy[ip] - cy[im] - (cx[jp] - cx[jm])) * inv2;
+  om[c] = wx;
+  om[g.nC + c] = wy;
+  om[2 * g.nC + c] = wz;
+  om[3 * g.nC + c] = sqrtf(wx * wx + wy * wy + wz * wz);
+}
+
+__global__ void k_vort_force(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ om,
+                             float *__restrict__ fc, float eps, const DevScalars *__restrict__ s) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nC) return;
+  if (ct[c] != FLUID) return;
+  int i = c % g.nx, j = (c / g.nx) % g.ny, k = c / (g.nx * g.ny), nxy = g.nx * g.ny;
+  const float *ox = om, *oy = om + g.nC, *oz = om + 2 * g.nC, *mg = om + 3 * g.nC;
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+  bool interior = !((i > 0 && ct[c - 1] != FLUID) || (i < g.nx - 1 && ct[c + 1] != FLUID) ||
+                    (j > 0 && ct[c - g.nx] != FLUID) || (j < g.ny - 1 && ct[c + g.nx] != FLUID) ||
+                    (k > 0 && ct[c - nxy] != FLUID) || (k < g.nz - 1 && ct[c + nxy] != FLUID));
+  if (mg[c] >= 0.35f && interior) {
+    float gx = mg[i < g.nx - 1 ? c + 1 : c] - mg[i > 0 ? c - 1 : c];
+    float gy = mg[j < g.ny - 1 ? c + g.nx : c] - mg[j > 0 ? c - g.nx : c];
+    float gz = mg[k < g.nz - 1 ? c + nxy : c] - mg[k > 0 ? c - nxy : c];
+    float g2 = gx * gx + gy * gy + gz * gz;
+    if (!(g2 < 1e-12f)) {
+      float inv = 1.f / sqrtf(g2);
+      float Nx = gx * inv, Ny = gy * inv, Nz = gz * inv;
+      fx = eps * (Ny * oz[c] - Nz * oy[c]);
+      fy = eps * (Nz * ox[c] - Nx * oz[c]);
+      fz = eps * (Nx * oy[c] - Ny * ox[c]);
+      float f2 = fx * fx + fy * fy + fz * fz;
+      if (f2 > 36.f) { float fs = 6.f / sqrtf(f2); fx *= fs; fy *= fs; fz *= fs; }
+      float dt = s->dt;
+      fx *= dt; fy *= dt; fz *= dt;
+    }
+  }
+  fc[c] = fx;
+  fc[g.nC + c] = fy;
+  fc[2 * g.nC + c] = fz;
+}
+
+__global__ void k_vort_splat(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ fc,
+                             float *__restrict__ vel) {
+  long F = (long)g.uN + g.vN + g.wN;
+  long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  FaceInfo fi;
+  face_of(g, f, fi);
+  if (fi.cl < 0 || fi.cr < 0 || ct[fi.cl] != FLUID || ct[fi.cr] != FLUID) return;
+  const float *fcc = fc + (long)fi.comp * g.nC;
+  vel[f] = vel[f] + 0.5f * (fcc[fi.cl] + fcc[fi.cr]);
+}
+
+// ------------------------------------------------------- extrapolation
+// _extrapolate, solver.js:1562-1636: one in-place lexicographic pass in which
+// an invalid face takes the mean of its valid neighbours, counting faces
+// filled earlier in the same pass. A face therefore depends on its final
+// -i/-j/-k neighbours and on the original state of its +i/+j/+k neighbours.
+// Tiles of 32x8x8 faces are processed one tile-diagonal (I+J+K) per launch;
+// inside a tile a hyperplane wavefront (li+lj+lk) keeps the same dependency
+// order, so every face sees exactly what the sequential pass shows it.
+__global__ __launch_bounds__(64) void k_extrap_diag(Geo g, const int4 *__restrict__ tiles, int off,
+                                                    float *__restrict__ vel, uint8_t *__restrict__ valid) {
+  constexpr int TI = EX_TI, TJ = EX_TJ, TK = EX_TK;
+  constexpr int HI = TI + 2, HJ = TJ + 2, HK = TK + 2;
+  __shared__ float sv[HK][HJ][HI];
+  __shared__ uint8_t sf[HK][HJ][HI];
+  int4 t = tiles[off + blockIdx.x];
+  int comp = t.x, I = t.y, J = t.z, K = t.w;
+  int d0 = comp == 0 ? g.nx + 1 : g.nx, d1 = comp == 1 ? g.ny + 1 : g.ny, d2 = comp == 2 ? g.nz + 1 : g.nz;
+  long base = comp == 0 ? 0 : (comp == 1 ? g.uN : (long)g.uN + g.vN);
+  float *A = vel + base;
+  uint8_t *V = valid + base;
+  int i0 = I * TI - 1, j0 = J * TJ - 1, k0 = K * TK - 1;
+  for (int e = threadIdx.x; e < HI * HJ * HK; e += blockDim.x) {
+    int li = e % HI, lj = (e / HI) % HJ, lk = e / (HI * HJ);
+    int i = i0 + li, j = j0 + lj, k = k0 + lk;
+    float val = 0.f;
+    uint8_t fl = 0;
+    if (i >= 0 && i < d0 && j >= 0 && j < d1 && k >= 0 && k < d2) {
+      long f = ((long)k * d1 + j) * d0 + i;
+      val = A[f];
+      fl = V[f] ? 1 : 0;
+    }
+    sv[lk][lj][li] = val;
+    sf[lk][lj][li] = fl;
+  }
+  __syncthreads();
+  int lj = 1 + (threadIdx.x % TJ), lk = 1 + (threadIdx.x / TJ);
+  int j = j0 + lj, k = k0 + lk;
+  for (int step = 0; step < TI + TJ + TK - 2; step++) {
+    int li = 1 + step - (lj - 1) - (lk - 1);
+    int i = i0 + li;
+    if (li >= 1 && li <= TI && i < d0 && j < d1 && k < d2 && !sf[lk][lj][li]) {
+      float sum = 0.f;
+      int cnt = 0;
+      if (i > 0 && sf[lk][lj][li - 1]) { sum += sv[lk][lj][li - 1]; cnt++; }
+      if (i < d0 - 1 && sf[lk][lj][li + 1]) { sum += sv[lk][lj][li + 1]; cnt++; }
+      if (j > 0 && sf[lk][lj - 1][li]) { sum += sv[lk][lj - 1][li]; cnt++; }
+      if (j < d1 - 1 && sf[lk][lj + 1][li]) { sum += sv[lk][lj + 1][li]; cnt++; }
+      if (k > 0 && sf[lk - 1][lj][li]) { sum += sv[lk - 1][lj][li]; cnt++; }
+      if (k < d2 - 1 && sf[lk + 1][lj][li]) { sum += sv[lk + 1][lj][li]; cnt++; }
+      if (cnt > 0) {
+        float val = sum / (float)cnt;
+        sv[lk][lj][li] = val;
+        sf[lk][lj][li] = 2;
+        long f = ((long)k * d1 + j) * d0 + i;
+        A[f] = val;
+        V[f] = 1;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace ws

@@ -1,0 +1,196 @@
+// Internal definitions of the watersim HIP library (gfx950): the context,
+// device-side scalar block, launch helpers and the device functions shared by
+// the grid, particle and surface kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/watersim.h"
+
+namespace ws {
+
+enum : uint8_t { AIR = 0, FLUID = 1, SOLID = 2 };
+
+// P2G accumulates in 64-bit fixed point (exact, order independent).
+constexpr double kFixScale = 1099511627776.0;  // 2^40
+constexpr float kFixScaleF = 1099511627776.0f;
+constexpr float kFixInvF = 1.0f / 1099511627776.0f;
+
+// Tile shapes (cells) for P2G and the extrapolation wavefront.
+constexpr int P2G_T = 8;
+constexpr int EX_TI = 32, EX_TJ = 8, EX_TK = 8;
+
+// Scalars that live on the device so that substeps can run back to back
+// without host round trips (dt law, gravity ramp, reductions).
+struct DevScalars {
+  float dt;            // current substep length
+  float gs;            // gravity scale of this substep (ramp * gravity), 0 = off
+  double gRamp;        // gravity ramp timer (solver.js:2657-2658)
+  double simTime;
+  unsigned umax_bits;  // max |face velocity| after the clamp, float bits
+  unsigned airborne;   // airborneCount of the last G2P
+  unsigned df_refined; // demotion predicates decided by the float-float path
+  unsigned nan_seen;
+  double rr, bb;       // residual^2, rhs^2 (FP64 norms)
+  double ke;
+  double vol;          // marching-tets volume (FP64)
+  unsigned long long mt_count;
+  float dtLast;
+  float pad;
+};
+
+// Geometry and per-substep constants passed by value to kernels.
+struct Geo {
+  int nx, ny, nz;
+  int uN, vN, wN, nC;
+  float dx, inv_dx;
+  float cx, cy, cz;         // domain centre (float)
+  float cx_lo, cy_lo, cz_lo;  // double centre minus its float rounding
+};
+
+struct Ctx {
+  // geometry
+  int nx, ny, nz, capacity;
+  double dx, W, H, D, cx, cy, cz, domainR;
+  long nC, uN, vN, wN, nD;
+  Geo geo;
+  // tunables (solver.js:56-75) and world constants
+  double cfl = 3.0, pic = 0.15, gravity = 9.81, viscosity = 5e-4, omega = 1.55, maxSpeed = 3.2;
+  double vorticity = 0, coreR = 0, oceanR = 0, atmosphereH = 0.35, spacing = 0, particleVolume = 0;
+  double gRamp = 1e9, dtLast = 0, iso = 1.8, substeps = 1;
+  double rainT = 0.22, cloudT = 0.34, rainLiftAge = 0.4;
+  int pressureIters = 20, maxPressureIters = 4000, nP = 0, tn = 0;
+  double pressureTol = 0, dv = 0;
+  int residualEvery = 8;
+  int measureResidual = 0, ceilReflect = 0, hasSun = 0;
+  double sunX = 0, sunY = 0, sunZ = 0, dtFrameBase = 0;
+  bool hasVapor = false;
+  // results of the last calls
+  double umax = 0, residual = 0;
+  int itersDone = 0, airborne = 0, nFluid = 0;
+  bool gRampDirty = true;
+  bool haveStatic = false, haveTerrain = false;
+  std::string err;
+
+  hipStream_t stream = nullptr;
+  // static world
+  uint8_t *staticTypes = nullptr;
+  float *gu = nullptr, *gv = nullptr, *gw = nullptr;
+  float *terrainR = nullptr;
+  float *thr = nullptr;  // exact floor(x/dx) thresholds, max(n)+2 entries
+  // cells
+  uint8_t *cellType = nullptr, *nbm = nullptr, *dg = nullptr;
+  float *q = nullptr, *rhs = nullptr, *pw = nullptr;
+  unsigned *cellCount = nullptr, *cellStart = nullptr;
+  float *vort = nullptr;  // 10*C scratch for vorticity confinement
+  // faces (u | v | w contiguous in one allocation each)
+  float *vel = nullptr, *velO = nullptr, *wgt = nullptr, *scr = nullptr, *scr2 = nullptr;
+  uint8_t *valid = nullptr;
+  float *dens = nullptr;
+  // particles
+  float *px = nullptr, *py = nullptr, *pz = nullptr, *pvx = nullptr, *pvy = nullptr, *pvz = nullptr;
+  float *pT = nullptr, *pAir = nullptr, *advU = nullptr, *advV = nullptr, *advW = nullptr;
+  uint8_t *pflag = nullptr, *pcool = nullptr;
+  int *pcell = nullptr;
+  unsigned *prank = nullptr, *sortedIdx = nullptr;
+  unsigned *vneed = nullptr;  // vapor draw flags and stream offsets, 4 x capacity
+  float *vscr = nullptr;      // vapor phase scratch, 4 x capacity
+  uint32_t *rng = nullptr;    // _cfS, _bfS, _evS stream states
+  // reductions
+  DevScalars *ds = nullptr;     // device
+  DevScalars *hs = nullptr;     // pinned host mirror
+  double *partials = nullptr;   // residual block partials
+  int nPartials = 0;
+  void *scanTmp = nullptr;
+  size_t scanTmpBytes = 0;
+  // extrapolation tile schedule
+  int4 *exTiles = nullptr;
+  std::vector<int> exDiagOff;  // tiles of diagonal d: [off[d], off[d+1])
+  // marching tets scratch
+  unsigned *mtCount = nullptr, *mtOff = nullptr;
+  float *mtPos = nullptr, *mtNrm = nullptr, *mtDens = nullptr;
+  long mtCells = 0, mtCap = 0, mtDensN = 0;
+
+  float *u() const { return vel; }
+  float *v() const { return vel + uN; }
+  float *w() const { return vel + uN + vN; }
+};
+
+#define WS_HIP(call)                                                      \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess) {                                               \
+      c->err = std::string(#call) + ": " + hipGetErrorString(e_);         \
+      return WS_EHIP;                                                     \
+    }                                                                     \
+  } while (0)
+
+#define WS_LAUNCHED()                                                     \
+  do {                                                                    \
+    hipError_t e_ = hipGetLastError();                                    \
+    if (e_ != hipSuccess) {                                               \
+      c->err = std::string("kernel launch: ") + hipGetErrorString(e_);    \
+      return WS_EHIP;                                                     \
+    }                                                                     \
+  } while (0)
+
+inline int blocks_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// ------------------------------------------------------------------ device
+// Exact floor(x / dx) of a float coordinate as the reference computes it in
+// double: thr[i] is the smallest float x with floor(double(x)/dx) >= i.
+__device__ __forceinline__ int cell_floor(float x, float inv_dx, const float *__restrict__ thr, int n) {
+  int c = (int)floorf(x * inv_dx);
+  c = c < 0 ? 0 : (c > n ? n : c);
+  if (c < n && x >= thr[c + 1]) c++;
+  else if (c > 0 && x < thr[c]) c--;
+  return c;  // in [0, n]; callers clamp to their range
+}
+
+__device__ __forceinline__ int clampi(int v, int a, int b) { return v < a ? a : (v > b ? b : v); }
+
+// Trilinear MAC stencil of one face component (solver.js:737-748).
+struct Stencil {
+  int idx, sj, sk;
+  float s0, s1, t0, t1, r0, r1;
+};
+
+__device__ __forceinline__ void face_stencil(const Geo &g, int comp, float x, float y, float z, Stencil &st) {
+  float a = x * g.inv_dx, b = y * g.inv_dx, cc = z * g.inv_dx;
+  if (comp == 0) { b -= 0.5f; cc -= 0.5f; }
+  else if (comp == 1) { a -= 0.5f; cc -= 0.5f; }
+  else { a -= 0.5f; b -= 0.5f; }
+  int i0 = (int)floorf(a), j0 = (int)floorf(b), k0 = (int)floorf(cc);
+  float tx, ty, tz;
+  if (i0 < 0) { i0 = 0; tx = 0.f; } else if (i0 > g.nx - 1) { i0 = g.nx - 1; tx = 1.f; } else tx = a - (float)i0;
+  if (j0 < 0) { j0 = 0; ty = 0.f; } else if (j0 > g.ny - 2) { j0 = g.ny - 2; ty = 1.f; } else ty = b - (float)j0;
+  if (k0 < 0) { k0 = 0; tz = 0.f; } else if (k0 > g.nz - 2) { k0 = g.nz - 2; tz = 1.f; } else tz = cc - (float)k0;
+  if (comp == 0) { st.sj = g.nx + 1; st.sk = g.ny * (g.nx + 1); st.idx = (k0 * g.ny + j0) * (g.nx + 1) + i0; }
+  else if (comp == 1) { st.sj = g.nx; st.sk = (g.ny + 1) * g.nx; st.idx = (k0 * (g.ny + 1) + j0) * g.nx + i0; }
+  else { st.sj = g.nx; st.sk = g.ny * g.nx; st.idx = (k0 * g.ny + j0) * g.nx + i0; }
+  st.s0 = 1.f - tx; st.s1 = tx; st.t0 = 1.f - ty; st.t1 = ty; st.r0 = 1.f - tz; st.r1 = tz;
+}
+
+__device__ __forceinline__ float stencil_eval(const Stencil &st, const float *__restrict__ A) {
+  int i = st.idx, sj = st.sj, sk = st.sk;
+  float v00 = A[i] * st.s0 + A[i + 1] * st.s1;
+  float v10 = A[i + sj] * st.s0 + A[i + sj + 1] * st.s1;
+  float v01 = A[i + sk] * st.s0 + A[i + sk + 1] * st.s1;
+  float v11 = A[i + sk + sj] * st.s0 + A[i + sk + sj + 1] * st.s1;
+  return (v00 * st.t0 + v10 * st.t1) * st.r0 + (v01 * st.t0 + v11 * st.t1) * st.r1;
+}
+
+// Logical block index with consecutive chunks of the grid on one XCD: the
+// hardware deals workgroups round-robin over the 8 XCDs, so block b runs on
+// XCD b%8; remapping gives each XCD (and its L2) a contiguous slab.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int X = 8;
+  int per = nb / X, rem = nb % X, xcd = b % X, slot = b / X;
+  // XCDs [0, rem) hold per+1 blocks, the others per
+  return xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
+}
+
+}  // namespace ws

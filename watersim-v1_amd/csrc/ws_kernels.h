@@ -1,0 +1,77 @@
+// Kernel declarations and the per-call parameter block of the substep.
+#pragma once
+#include "ws_internal.h"
+#include "ws_world.h"
+
+namespace ws {
+
+// particle-phase tunables (solver.js:56-75, 145-193) as floats
+struct Par {
+  float pic, maxSpeed, gravity, oceanR, atmosphereH, domainR, coreR, W, H, D;
+  float sunX, sunY, sunZ;  // unit direction to the sun, (1,0,0) without one
+  float rainTe, rainLiftAge;
+  int ceilReflect;
+};
+
+__global__ void k_begin_substep(DevScalars *s, float dt_host, int use_law, float cfl_dx, float dt_base,
+                                float gravity);
+__global__ void k_end_substep(DevScalars *s);
+__global__ void k_init_cells(const uint8_t *st, uint8_t *ct, unsigned *cnt, int nC);
+__global__ void k_mark(Geo g, World w, int nP, const float *px, const float *py, const float *pz, uint8_t *pflag,
+                       uint8_t *ct, unsigned *cnt, int *pcell, unsigned *prank, DevScalars *s);
+__global__ void k_scatter(int nP, const int *pcell, const unsigned *prank, const unsigned *start,
+                          unsigned *sorted);
+__global__ void k_p2g_tile(Geo g, int tilesX, int tilesY, const unsigned *start, const unsigned *sorted,
+                           const float *px, const float *py, const float *pz, const float *pvx, const float *pvy,
+                           const float *pvz, float *vel, float *wgt);
+__global__ void k_bc_snapshot(Geo g, const uint8_t *ct, float *vel, float *velO, const float *wgt, uint8_t *valid,
+                              int fuse_gravity, const float *gvec, const DevScalars *s, int write_valid);
+__global__ void k_valid(Geo g, const uint8_t *ct, const float *wgt, uint8_t *valid);
+__global__ void k_gravity_bc(Geo g, const uint8_t *ct, float *vel, const float *gvec, const DevScalars *s);
+__global__ void k_visc_iter(Geo g, const uint8_t *ct, const float *old, const float *cur, float *out, float coef,
+                            int first);
+__global__ void k_assemble(Geo g, const uint8_t *ct, const float *vel, float *q, float *rhs, float *pw,
+                           uint8_t *nbm, uint8_t *dg, float omega);
+__global__ void k_sor(Geo g, float *q, const float *rhs, const float *pw, const uint8_t *nbm, int color,
+                      float om1);
+__global__ void k_residual(Geo g, const float *q, const float *rhs, const uint8_t *nbm, const uint8_t *dg,
+                           double *partials);
+__global__ void k_residual_final(const double *partials, int n, DevScalars *s);
+__global__ void k_project(Geo g, const uint8_t *ct, const float *phi, float *vel, int do_clamp, float vcap,
+                          DevScalars *s);
+__global__ void k_clamp(long F, float *vel, float vcap, DevScalars *s);
+__global__ void k_vort_center(Geo g, const float *vel, float *vc);
+__global__ void k_vort_omega(Geo g, const float *vc, float *om, float inv2);
+__global__ void k_vort_force(Geo g, const uint8_t *ct, const float *om, float *fc, float eps,
+                             const DevScalars *s);
+__global__ void k_vort_splat(Geo g, const uint8_t *ct, const float *fc, float *vel);
+__global__ void k_extrap_diag(Geo g, const int4 *tiles, int off, float *vel, uint8_t *valid);
+
+__global__ void k_g2p_advect(Geo g, World w, Par P, int nP, int mode, const float *vel, const float *velO,
+                             const uint8_t *ct, float *px, float *py, float *pz, float *pvx, float *pvy, float *pvz,
+                             uint8_t *pflag, uint8_t *pcool, float *pAir, float *advU, float *advV, float *advW,
+                             DevScalars *s);
+__global__ void k_vapor_a(Geo g, World w, Par P, int nP, float *px, float *py, float *pz, float *pvx, float *pvy,
+                          float *pvz, const uint8_t *pflag, float *pAir, float *scr, unsigned *need,
+                          const DevScalars *s);
+__global__ void k_vapor_b(Geo g, Par P, int nP, const float *px, const float *py, const float *pz, float *pvx,
+                          float *pvy, float *pvz, const uint8_t *pflag, const float *scr, const unsigned *needA,
+                          const unsigned *offA, unsigned *needB, const DevScalars *s, const uint32_t *rng);
+__global__ void k_vapor_c(Geo g, World w, Par P, int nP, float *px, float *py, float *pz, float *pvx, float *pvy,
+                          float *pvz, uint8_t *pflag, float *pAir, const float *pT, const uint8_t *ct,
+                          const float *scr, const unsigned *needB, const unsigned *offB, const DevScalars *s,
+                          const uint32_t *rng);
+__global__ void k_rng_advance(uint32_t *rng, int which, const unsigned *need, const unsigned *off, int nP);
+
+// surface
+__global__ void k_splat_density(Geo g, World w, int nP, const unsigned *start, const unsigned *sorted,
+                                const float *px, const float *py, const float *pz, float *dens, float r2);
+__global__ void k_mt_count(int nx, int ny, int nz, const float *dens, float iso, unsigned *cnt);
+__global__ void k_mt_emit(int nx, int ny, int nz, float dx, const float *dens, float iso, const unsigned *off,
+                          float *pos, float *nrm, long cap, double *vpart);
+__global__ void k_mt_volume(int nx, int ny, int nz, float dx, const float *dens, float iso, double *vpart,
+                            unsigned *cnt);
+__global__ void k_sum_f64(const double *part, int n, double *out);
+__global__ void k_kinetic(int nP, const float *pvx, const float *pvy, const float *pvz, double *part);
+
+}  // namespace ws
