@@ -12,9 +12,13 @@ import watersim_hip  # noqa: E402
 
 
 def declared():
-    txt = open(os.path.join(ROOT, "include", "watersim.h")).read()
-    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(ws_[a-z_0-9]+)\s*\(", txt)))
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for h in sorted(os.listdir(inc)):
+        if h.endswith(".h"):
+            txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(inc, h)).read(), flags=re.S)
+            names |= set(re.findall(r"\b(ws_[a-z_0-9]+)\s*\(", txt))
+    return sorted(names)
 
 
 def test_header_matches_binding_list():

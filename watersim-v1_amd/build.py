@@ -5,7 +5,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = ["ws_grid.hip", "ws_particles.hip", "ws_surface.hip", "ws_api.hip", "ws_api_surface.hip"]
+SRC = ["ws_grid.hip", "ws_particles.hip", "ws_surface.hip", "ws_api.hip", "ws_api_surface.hip", "ws_world.cpp"]
 OUT = os.path.join(HERE, "_build", "libwatersim_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
@@ -18,7 +18,7 @@ def build(verbose=False):
     procs = []
     for s in SRC:
         src = os.path.join(HERE, "csrc", s)
-        obj = os.path.join(HERE, "_build", s.replace(".hip", ".o"))
+        obj = os.path.join(HERE, "_build", os.path.splitext(s)[0] + ".o")
         objs.append(obj)
         cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
@@ -30,7 +30,7 @@ def build(verbose=False):
         failed |= p.returncode != 0
     if failed:
         raise RuntimeError("hipcc failed")
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT] + objs
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", OUT] + objs
     subprocess.run(cmd, check=True)
     return OUT
 

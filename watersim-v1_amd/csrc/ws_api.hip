@@ -162,10 +162,13 @@ int visc_iterations(double coef) {
   return m;
 }
 
-int st_viscosity(ws_ctx *c, double dt, const float *old) {
+// dt_bound: the substep length, or an upper bound of it when dt is decided on
+// the device (the kernels read the actual dt; the bound only sets the
+// iteration count, and more iterations stay exact)
+int st_viscosity(ws_ctx *c, double dt_bound, const float *old) {
   double nu = c->viscosity;
   if (nu <= 1e-6) return WS_OK;
-  double coef = nu * dt / (c->dx * c->dx);
+  double coef = nu * dt_bound / (c->dx * c->dx);
   if (coef > 0.24) coef = 0.24;
   int m = visc_iterations(coef);
   long F = nfaces(c);
@@ -174,7 +177,7 @@ int st_viscosity(ws_ctx *c, double dt, const float *old) {
   for (int it = 0; it < m; it++) {
     float *out = (it & 1) ? c->scr2 : c->vel;
     hipLaunchKernelGGL(k_visc_iter, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->geo, c->cellType, old, cur,
-                       out, (float)coef, it == 0 ? 1 : 0);
+                       out, (float)(nu / (c->dx * c->dx)), c->ds, it == 0 ? 1 : 0);
     cur = out;
   }
   WS_LAUNCHED();
@@ -192,11 +195,23 @@ int sor_sweeps(ws_ctx *c, int n) {
   int half = (c->nx + 1) / 2;
   long nthr = (long)half * c->ny * c->nz;
   float om1 = (float)(1 - c->omega);
+  if (c->timing && n > 0) WS_HIP(hipEventRecord(c->ev[0], c->stream));
   for (int it = 0; it < n; it++)
     for (int color = 0; color < 2; color++)
       hipLaunchKernelGGL(k_sor, dim3(blocks_for(nthr, 256)), dim3(256), 0, c->stream, c->geo, c->q, c->rhs, c->pw,
                          c->nbm, color, om1);
   WS_LAUNCHED();
+  c->sweepsTotal += n;
+  if (c->timing && n > 0) {
+    // the batch holds only back-to-back SOR launches: its span over the
+    // launch count is the kernel's average duration (gaps included)
+    WS_HIP(hipEventRecord(c->ev[1], c->stream));
+    WS_HIP(hipEventSynchronize(c->ev[1]));
+    float ms = 0;
+    WS_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->sorMs += ms;
+    c->sorLaunches += 2L * n;
+  }
   return WS_OK;
 }
 
@@ -211,7 +226,24 @@ int residual(ws_ctx *c, double *res) {
   return WS_OK;
 }
 
+int st_pressure_solve(ws_ctx *c);
+
 int st_pressure(ws_ctx *c) {
+  if (c->timing) WS_HIP(hipEventRecord(c->ev[2], c->stream));
+  int r = st_pressure_solve(c);
+  if (r) return r;
+  if (c->timing) {
+    WS_HIP(hipEventRecord(c->ev[3], c->stream));
+    WS_HIP(hipEventSynchronize(c->ev[3]));
+    float ms = 0;
+    WS_HIP(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+    c->pressureMs += ms;
+    c->pressureSolves++;
+  }
+  return WS_OK;
+}
+
+int st_pressure_solve(ws_ctx *c) {
   hipLaunchKernelGGL(k_assemble, dim3(blocks_for(c->nC, 256)), dim3(256), 0, c->stream, c->geo, c->cellType, c->vel,
                      c->q, c->rhs, c->pw, c->nbm, c->dg, (float)c->omega);
   WS_LAUNCHED();
@@ -222,16 +254,19 @@ int st_pressure(ws_ctx *c) {
     if (c->measureResidual && (r = residual(c, &c->residual))) return r;
     return WS_OK;
   }
-  // to a relative residual: sweep in batches, checking in between
-  int done = 0;
-  double res = 0;
+  // to a relative residual: sweep in batches, checking in between; stop
+  // early if the residual no longer falls (float resolution reached)
+  int done = 0, flat = 0;
+  double res = 0, best;
   if ((r = residual(c, &res))) return r;
-  while (res > c->pressureTol && done < c->maxPressureIters) {
+  best = res;
+  while (res > c->pressureTol && done < c->maxPressureIters && flat < 12) {
     int k = c->residualEvery;
     if (done + k > c->maxPressureIters) k = c->maxPressureIters - done;
     if ((r = sor_sweeps(c, k))) return r;
     done += k;
     if ((r = residual(c, &res))) return r;
+    if (res < 0.995 * best) { best = res; flat = 0; } else flat++;
   }
   c->itersDone = done;
   c->residual = res;
@@ -325,7 +360,7 @@ int substep(ws_ctx *c, double dt, int use_law) {
   if ((r = st_p2g(c))) return r;
   if ((r = st_bc(c, c->velO, visc ? 0 : 1))) return r;
   if (visc) {
-    if ((r = st_viscosity(c, dt, c->velO))) return r;
+    if ((r = st_viscosity(c, use_law ? 0.01 : dt, c->velO))) return r;
     if ((r = st_gravity(c))) return r;
   }
   if ((r = st_pressure(c))) return r;
@@ -503,6 +538,8 @@ void ws_destroy(ws_ctx *c) {
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->hs) hipHostFree(c->hs);
+  for (auto &e : c->ev)
+    if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -527,6 +564,14 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
   }
   if (!strcmp(n, "tn")) { c->tn = (int)v; return WS_OK; }
   if (!strcmp(n, "hasVapor")) { c->hasVapor = v != 0; return WS_OK; }
+  if (!strcmp(n, "timing")) {
+    c->timing = v != 0;
+    for (int i = 0; i < 4 && c->timing; i++)
+      if (!c->ev[i]) WS_HIP(hipEventCreate(&c->ev[i]));
+    c->sorMs = c->pressureMs = 0;
+    c->sorLaunches = c->pressureSolves = c->sweepsTotal = 0;
+    return WS_OK;
+  }
   c->err = std::string("unknown scalar ") + n;
   return WS_EINVAL;
 }
@@ -548,6 +593,7 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   GF(cfl) GF(pic) GF(gravity) GF(viscosity) GF(omega) GF(maxSpeed) GF(vorticity) GF(coreR) GF(oceanR)
   GF(atmosphereH) GF(spacing) GF(particleVolume) GF(iso) GF(substeps) GF(pressureTol) GF(pressureIters)
   GF(maxPressureIters) GF(nP) GF(capacity) GF(residual) GF(domainR) GF(dx) GF(tn) GF(dv)
+  GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal)
 #undef GF
   if (!strcmp(n, "pressureIters_done")) { *v = c->itersDone; return WS_OK; }
   if (!strcmp(n, "rngCfS") || !strcmp(n, "rngBfS") || !strcmp(n, "rngEvS")) {

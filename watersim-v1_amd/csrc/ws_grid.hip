@@ -342,7 +342,9 @@ __global__ void k_gravity_bc(Geo g, const uint8_t *__restrict__ ct, float *__res
 // per iteration, so a few iterations reach float resolution (the host picks
 // the count from coef).
 __global__ void k_visc_iter(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ old,
-                            const float *__restrict__ cur, float *__restrict__ out, float coef, int first) {
+                            const float *__restrict__ cur, float *__restrict__ out, float nu_dx2,
+                            const DevScalars *__restrict__ s, int first) {
+  const float coef = fminf(nu_dx2 * s->dt, 0.24f);  // solver.js:1264
   long F = (long)g.uN + g.vN + g.wN;
   long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= F) return;

@@ -28,8 +28,8 @@ __global__ void k_bc_snapshot(Geo g, const uint8_t *ct, float *vel, float *velO,
                               int fuse_gravity, const float *gvec, const DevScalars *s, int write_valid);
 __global__ void k_valid(Geo g, const uint8_t *ct, const float *wgt, uint8_t *valid);
 __global__ void k_gravity_bc(Geo g, const uint8_t *ct, float *vel, const float *gvec, const DevScalars *s);
-__global__ void k_visc_iter(Geo g, const uint8_t *ct, const float *old, const float *cur, float *out, float coef,
-                            int first);
+__global__ void k_visc_iter(Geo g, const uint8_t *ct, const float *old, const float *cur, float *out, float nu_dx2,
+                            const DevScalars *s, int first);
 __global__ void k_assemble(Geo g, const uint8_t *ct, const float *vel, float *q, float *rhs, float *pw,
                            uint8_t *nbm, uint8_t *dg, float omega);
 __global__ void k_sor(Geo g, float *q, const float *rhs, const float *pw, const uint8_t *nbm, int color,

@@ -21,7 +21,23 @@ STAGES = {
 # exported entry points, as declared in include/watersim.h
 EXPORTS = ["ws_create", "ws_destroy", "ws_last_error", "ws_set_f", "ws_get_f", "ws_upload", "ws_download",
            "ws_substep", "ws_stage", "ws_step_frame", "ws_run_substeps", "ws_splat_density", "ws_mt_build",
-           "ws_autotune_iso", "ws_kinetic_energy", "ws_sync"]
+           "ws_autotune_iso", "ws_kinetic_energy", "ws_sync", "ws_world_build", "ws_world_free"]
+
+
+class WorldParams(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("dx", ctypes.c_double),
+                ("coreR", ctypes.c_double), ("depth", ctypes.c_double), ("bumpiness", ctypes.c_double),
+                ("seed", ctypes.c_uint32), ("targetParticles", ctypes.c_int)]
+
+
+class WorldC(ctypes.Structure):
+    _fields_ = [("tn", ctypes.c_int), ("dv", ctypes.c_double), ("solid", ctypes.POINTER(ctypes.c_uint8)),
+                ("R", ctypes.POINTER(ctypes.c_float)), ("staticTypes", ctypes.POINTER(ctypes.c_uint8)),
+                ("gravity", ctypes.POINTER(ctypes.c_float)), ("nP", ctypes.c_long), ("capacity", ctypes.c_long),
+                ("px", ctypes.POINTER(ctypes.c_float)), ("py", ctypes.POINTER(ctypes.c_float)),
+                ("pz", ctypes.POINTER(ctypes.c_float)), ("spacing", ctypes.c_double),
+                ("particleVolume", ctypes.c_double), ("oceanR", ctypes.c_double), ("coreR", ctypes.c_double),
+                ("Rlo", ctypes.c_double), ("Rhi", ctypes.c_double), ("landFrac", ctypes.c_double)]
 
 _lib = None
 
@@ -58,8 +74,11 @@ def load(path=LIB_PATH):
     L.ws_autotune_iso.argtypes = [vp, ctypes.POINTER(d)]
     L.ws_kinetic_energy.argtypes = [vp, ctypes.POINTER(d)]
     L.ws_sync.argtypes = [vp]
+    L.ws_world_build.argtypes = [ctypes.POINTER(WorldParams), ctypes.POINTER(WorldC)]
+    L.ws_world_free.argtypes = [ctypes.POINTER(WorldC)]
+    L.ws_world_free.restype = None
     for f in EXPORTS:
-        if f not in ("ws_destroy", "ws_last_error"):
+        if f not in ("ws_destroy", "ws_last_error", "ws_world_free"):
             getattr(L, f).restype = ci
     _lib = L
     return L
@@ -210,3 +229,60 @@ class Solver:
         out = ctypes.c_double()
         self._check(self.L.ws_kinetic_energy(self.ctx, ctypes.byref(out)), "kinetic_energy")
         return out.value
+
+
+def build_world(n, dx, coreR, depth, seed, target, bumpiness=1.0):
+    """Planet + ocean fill on the host (ws_world_build); returns numpy arrays.
+    Needs no GPU."""
+    L = load()
+    p = WorldParams(n, n, n, dx, coreR, depth, bumpiness, seed, target)
+    w = WorldC()
+    rc = L.ws_world_build(ctypes.byref(p), ctypes.byref(w))
+    if rc != 0:
+        raise WatersimError("ws_world_build failed (%d)" % rc)
+    try:
+        tn = w.tn
+        uN, vN = (n + 1) * n * n, n * (n + 1) * n
+        F = 3 * (n + 1) * n * n
+        grav = np.ctypeslib.as_array(w.gravity, (F,)).copy()
+        out = {
+            "tn": tn, "dv": w.dv, "nP": w.nP, "capacity": w.capacity, "spacing": w.spacing,
+            "particleVolume": w.particleVolume, "oceanR": w.oceanR, "coreR": w.coreR, "Rlo": w.Rlo,
+            "Rhi": w.Rhi, "landFrac": w.landFrac, "nx": n, "dx": dx,
+            "solid": np.ctypeslib.as_array(w.solid, (tn ** 3,)).copy(),
+            "R": np.ctypeslib.as_array(w.R, (tn ** 3,)).copy(),
+            "staticTypes": np.ctypeslib.as_array(w.staticTypes, (n ** 3,)).copy(),
+            "gu": grav[:uN], "gv": grav[uN:uN + vN], "gw": grav[uN + vN:],
+            "px": np.ctypeslib.as_array(w.px, (max(w.nP, 1),))[:w.nP].copy(),
+            "py": np.ctypeslib.as_array(w.py, (max(w.nP, 1),))[:w.nP].copy(),
+            "pz": np.ctypeslib.as_array(w.pz, (max(w.nP, 1),))[:w.nP].copy(),
+        }
+    finally:
+        L.ws_world_free(ctypes.byref(w))
+    return out
+
+
+def solver_for_world(world, capacity=None, Tamb=0.32):
+    """A GPU context holding a built world, its fill at rest and gRamp = 0,
+    as FluidSolver.resetWater leaves it (solver.js:718-730)."""
+    n, nP = world["nx"], world["nP"]
+    s = Solver(n, n, n, world["dx"], capacity or world["capacity"])
+    s.set(coreR=world["coreR"], oceanR=world["oceanR"], spacing=world["spacing"],
+          particleVolume=world["particleVolume"], dv=world["dv"], tn=world["tn"])
+    s.upload("staticTypes", world["staticTypes"])
+    for k in ("gu", "gv", "gw"):
+        s.upload(k, world[k])
+    s.upload("terrainR", world["R"])
+    s.set(nP=nP)
+    s.upload("px", world["px"])
+    s.upload("py", world["py"])
+    s.upload("pz", world["pz"])
+    z = np.zeros(nP, np.float32)
+    for k in ("pvx", "pvy", "pvz", "pAir"):
+        s.upload(k, z)
+    s.upload("pT", np.full(nP, Tamb, np.float32))
+    s.upload("pflag", np.zeros(nP, np.uint8))
+    s.upload("pcool", np.zeros(nP, np.uint8))
+    s.upload("q", np.zeros(n ** 3, np.float32))
+    s.set(gRamp=0.0)
+    return s
