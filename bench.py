@@ -19,6 +19,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "watersim-v1_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 W_DOMAIN = 5.1
 CONFIGS = {
@@ -53,7 +54,7 @@ def make_solver(cfg):
 def cpu_baseline(s, world, cfg, budget_s=12.0, max_substeps=6):
     """The oracle (single-threaded C port of the reference substep, reference
     lexicographic SOR order) on the same state, for a bounded sample."""
-    import oracle
+    import cpu_oracle as oracle
     n = cfg["n"]
     st = oracle.OracleState({"nx": n, "ny": n, "nz": n, "dx": W_DOMAIN / n, "coreR": world["coreR"],
                              "oceanR": world["oceanR"], "spacing": world["spacing"],

@@ -4,7 +4,7 @@ dumped substep."""
 import numpy as np
 
 import golden
-import oracle
+import cpu_oracle as oracle
 
 PART_KEYS = ["px", "py", "pz", "pvx", "pvy", "pvz", "pT", "pAir", "pflag", "pcool"]
 

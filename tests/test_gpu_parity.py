@@ -259,7 +259,7 @@ def test_trajectory_vs_oracle():
 
 
 def test_surface_vs_golden():
-    import oracle
+    import cpu_oracle as oracle
     g = golden.load("cfg1_32")
     from gpu_scenes import gpu_from_golden, rel_err
     s = gpu_from_golden(g, tag="adv_")

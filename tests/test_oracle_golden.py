@@ -84,7 +84,7 @@ def test_substep_stages_bit_exact(scene):
 
 
 def test_surface_bit_exact():
-    import oracle
+    import cpu_oracle as oracle
     g = golden.load("cfg1_32")
     st = oracle_from_golden(g, tag="adv_")
     st.set_f("spacing", g.meta["world"]["spacing"])

@@ -455,6 +455,7 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
   g.nx = nx; g.ny = ny; g.nz = nz;
   g.uN = (int)c->uN; g.vN = (int)c->vN; g.wN = (int)c->wN; g.nC = (int)c->nC;
   g.dx = (float)dx; g.inv_dx = (float)(1.0 / dx);
+  g.dx_lo = (float)(dx - (double)g.dx);
   g.cx = (float)c->cx; g.cy = (float)c->cy; g.cz = (float)c->cz;
   g.cx_lo = (float)(c->cx - g.cx); g.cy_lo = (float)(c->cy - g.cy); g.cz_lo = (float)(c->cz - g.cz);
   long F = c->uN + c->vN + c->wN;

@@ -185,15 +185,12 @@ __global__ __launch_bounds__(256) void k_p2g_tile(Geo g, int tilesX, int tilesY,
     float vc[3] = {pvx[p], pvy[p], pvz[p]};
 #pragma unroll
     for (int comp = 0; comp < 3; comp++) {
-      float a = x * g.inv_dx, bb = y * g.inv_dx, cc = z * g.inv_dx;
-      if (comp == 0) { bb -= 0.5f; cc -= 0.5f; }
-      else if (comp == 1) { a -= 0.5f; cc -= 0.5f; }
-      else { a -= 0.5f; bb -= 0.5f; }
-      int fi = (int)floorf(a), fj = (int)floorf(bb), fk = (int)floorf(cc);
+      float oa = comp == 0 ? 0.f : 0.5f, ob = comp == 1 ? 0.f : 0.5f, oc = comp == 2 ? 0.f : 0.5f;
+      int fi = (int)floorf(x * g.inv_dx - oa), fj = (int)floorf(y * g.inv_dx - ob), fk = (int)floorf(z * g.inv_dx - oc);
       float txf, tyf, tzf;
-      if (fi < 0) { fi = 0; txf = 0.f; } else if (fi > g.nx - 1) { fi = g.nx - 1; txf = 1.f; } else txf = a - (float)fi;
-      if (fj < 0) { fj = 0; tyf = 0.f; } else if (fj > g.ny - 2) { fj = g.ny - 2; tyf = 1.f; } else tyf = bb - (float)fj;
-      if (fk < 0) { fk = 0; tzf = 0.f; } else if (fk > g.nz - 2) { fk = g.nz - 2; tzf = 1.f; } else tzf = cc - (float)fk;
+      if (fi < 0) { fi = 0; txf = 0.f; } else if (fi > g.nx - 1) { fi = g.nx - 1; txf = 1.f; } else txf = axis_frac(g, x, oa, fi);
+      if (fj < 0) { fj = 0; tyf = 0.f; } else if (fj > g.ny - 2) { fj = g.ny - 2; tyf = 1.f; } else tyf = axis_frac(g, y, ob, fj);
+      if (fk < 0) { fk = 0; tzf = 0.f; } else if (fk > g.nz - 2) { fk = g.nz - 2; tzf = 1.f; } else tzf = axis_frac(g, z, oc, fk);
       float s0 = 1.f - txf, s1 = txf, t0 = 1.f - tyf, t1 = tyf, r0 = 1.f - tzf, r1 = tzf;
       float wA = s0 * t0, wB = s1 * t0, wC = s0 * t1, wD = s1 * t1;
       int d0 = comp == 0 ? g.nx + 1 : g.nx, d1 = comp == 1 ? g.ny + 1 : g.ny;

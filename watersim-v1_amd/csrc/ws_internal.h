@@ -49,6 +49,7 @@ struct Geo {
   int nx, ny, nz;
   int uN, vN, wN, nC;
   float dx, inv_dx;
+  float dx_lo;              // double dx minus its float rounding
   float cx, cy, cz;         // domain centre (float)
   float cx_lo, cy_lo, cz_lo;  // double centre minus its float rounding
 };
@@ -165,16 +166,24 @@ struct Stencil {
   float s0, s1, t0, t1, r0, r1;
 };
 
+// Cell-local coordinate along one axis: index i0 = floor(x/dx - off) and
+// fraction t = x/dx - off - i0. The fraction is formed as
+// (x - (i0+off)*dx)/dx with an FMA and the low part of dx, so it keeps
+// float precision relative to the cell instead of to the coordinate
+// (x/dx - i0 in float would lose ~ulp(n) ~ 1e-5 at 128 cells).
+__device__ __forceinline__ float axis_frac(const Geo &g, float x, float off, int i0) {
+  float base = (float)i0 + off;  // exact: i0 < 2^23, off in {0, 0.5}
+  float r = __builtin_fmaf(-base, g.dx, x) - base * g.dx_lo;
+  return r * g.inv_dx;
+}
+
 __device__ __forceinline__ void face_stencil(const Geo &g, int comp, float x, float y, float z, Stencil &st) {
-  float a = x * g.inv_dx, b = y * g.inv_dx, cc = z * g.inv_dx;
-  if (comp == 0) { b -= 0.5f; cc -= 0.5f; }
-  else if (comp == 1) { a -= 0.5f; cc -= 0.5f; }
-  else { a -= 0.5f; b -= 0.5f; }
-  int i0 = (int)floorf(a), j0 = (int)floorf(b), k0 = (int)floorf(cc);
+  float oa = comp == 0 ? 0.f : 0.5f, ob = comp == 1 ? 0.f : 0.5f, oc = comp == 2 ? 0.f : 0.5f;
+  int i0 = (int)floorf(x * g.inv_dx - oa), j0 = (int)floorf(y * g.inv_dx - ob), k0 = (int)floorf(z * g.inv_dx - oc);
   float tx, ty, tz;
-  if (i0 < 0) { i0 = 0; tx = 0.f; } else if (i0 > g.nx - 1) { i0 = g.nx - 1; tx = 1.f; } else tx = a - (float)i0;
-  if (j0 < 0) { j0 = 0; ty = 0.f; } else if (j0 > g.ny - 2) { j0 = g.ny - 2; ty = 1.f; } else ty = b - (float)j0;
-  if (k0 < 0) { k0 = 0; tz = 0.f; } else if (k0 > g.nz - 2) { k0 = g.nz - 2; tz = 1.f; } else tz = cc - (float)k0;
+  if (i0 < 0) { i0 = 0; tx = 0.f; } else if (i0 > g.nx - 1) { i0 = g.nx - 1; tx = 1.f; } else tx = axis_frac(g, x, oa, i0);
+  if (j0 < 0) { j0 = 0; ty = 0.f; } else if (j0 > g.ny - 2) { j0 = g.ny - 2; ty = 1.f; } else ty = axis_frac(g, y, ob, j0);
+  if (k0 < 0) { k0 = 0; tz = 0.f; } else if (k0 > g.nz - 2) { k0 = g.nz - 2; tz = 1.f; } else tz = axis_frac(g, z, oc, k0);
   if (comp == 0) { st.sj = g.nx + 1; st.sk = g.ny * (g.nx + 1); st.idx = (k0 * g.ny + j0) * (g.nx + 1) + i0; }
   else if (comp == 1) { st.sj = g.nx; st.sk = (g.ny + 1) * g.nx; st.idx = (k0 * (g.ny + 1) + j0) * g.nx + i0; }
   else { st.sj = g.nx; st.sk = g.ny * g.nx; st.idx = (k0 * g.ny + j0) * g.nx + i0; }
