@@ -196,10 +196,18 @@ int sor_sweeps(ws_ctx *c, int n) {
   long nthr = (long)half * c->ny * c->nz;
   float om1 = (float)(1 - c->omega);
   if (c->timing && n > 0) WS_HIP(hipEventRecord(c->ev[0], c->stream));
-  for (int it = 0; it < n; it++)
-    for (int color = 0; color < 2; color++)
-      hipLaunchKernelGGL(k_sor, dim3(blocks_for(nthr, 256)), dim3(256), 0, c->stream, c->geo, c->q, c->rhs, c->pw,
-                         c->nbm, color, om1);
+  int nT = c->sorTiles[0] * c->sorTiles[1] * c->sorTiles[2];
+  for (int it = 0; it < n; it++) {
+    if (c->sorTiled) {
+      hipLaunchKernelGGL(k_sor_tile, dim3(nT), dim3(256), 0, c->stream, c->geo, c->sorTiles[0], c->sorTiles[1],
+                         c->sorActive, c->q, c->q2, c->rhs, c->pw, c->nbm, om1);
+      float *t = c->q; c->q = c->q2; c->q2 = t;  // the new sweep is current
+    } else {
+      for (int color = 0; color < 2; color++)
+        hipLaunchKernelGGL(k_sor, dim3(blocks_for(nthr, 256)), dim3(256), 0, c->stream, c->geo, c->q, c->rhs, c->pw,
+                           c->nbm, color, om1);
+    }
+  }
   WS_LAUNCHED();
   c->sweepsTotal += n;
   if (c->timing && n > 0) {
@@ -210,7 +218,7 @@ int sor_sweeps(ws_ctx *c, int n) {
     float ms = 0;
     WS_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     c->sorMs += ms;
-    c->sorLaunches += 2L * n;
+    c->sorLaunches += (c->sorTiled ? 1L : 2L) * n;
   }
   return WS_OK;
 }
@@ -245,7 +253,12 @@ int st_pressure(ws_ctx *c) {
 
 int st_pressure_solve(ws_ctx *c) {
   hipLaunchKernelGGL(k_assemble, dim3(blocks_for(c->nC, 256)), dim3(256), 0, c->stream, c->geo, c->cellType, c->vel,
-                     c->q, c->rhs, c->pw, c->nbm, c->dg, (float)c->omega);
+                     c->q, c->q2, c->rhs, c->pw, c->nbm, c->dg, (float)c->omega);
+  if (c->sorTiled) {
+    int nT = c->sorTiles[0] * c->sorTiles[1] * c->sorTiles[2];
+    hipLaunchKernelGGL(k_sor_tiles_mark, dim3(nT), dim3(256), 0, c->stream, c->geo, c->sorTiles[0], c->sorTiles[1],
+                       nT, c->nbm, c->sorActive);
+  }
   WS_LAUNCHED();
   int r;
   if (c->pressureTol <= 0) {
@@ -256,25 +269,41 @@ int st_pressure_solve(ws_ctx *c) {
   }
   // to a relative residual: sweep in batches, checking in between; stop
   // early if the residual no longer falls (float resolution reached)
-  int done = 0, flat = 0;
-  double res = 0, best;
+  // The batch length follows the observed convergence rate: after each check
+  // the sweeps still needed at that rate are predicted, so only a few
+  // residual evaluations run per solve.
+  int done = 0, flat = 0, k = c->residualEvery;
+  double res = 0, best, prev;
   if ((r = residual(c, &res))) return r;
-  best = res;
+  best = prev = res;
   while (res > c->pressureTol && done < c->maxPressureIters && flat < 12) {
-    int k = c->residualEvery;
     if (done + k > c->maxPressureIters) k = c->maxPressureIters - done;
     if ((r = sor_sweeps(c, k))) return r;
     done += k;
     if ((r = residual(c, &res))) return r;
     if (res < 0.995 * best) { best = res; flat = 0; } else flat++;
+    int kn = c->residualEvery;
+    if (c->adaptiveChecks && res < prev && res > c->pressureTol) {
+      double rate = std::pow(res / prev, 1.0 / k);  // per sweep
+      double need = std::log(c->pressureTol / res) / std::log(rate);
+      kn = need < 4 ? 4 : (need > 64 ? 64 : (int)std::ceil(need));
+    }
+    prev = res;
+    k = kn;
   }
   c->itersDone = done;
   c->residual = res;
   return WS_OK;
 }
 
+// face-sweep grid with the umax reduction: a bounded grid-stride launch
+int reduce_grid(long F) {
+  int b = blocks_for(F, 256);
+  return b < 2048 ? b : 2048;
+}
+
 int st_project(ws_ctx *c, int do_clamp) {
-  hipLaunchKernelGGL(k_project, dim3(blocks_for(nfaces(c), 256)), dim3(256), 0, c->stream, c->geo, c->cellType, c->q,
+  hipLaunchKernelGGL(k_project, dim3(reduce_grid(nfaces(c))), dim3(256), 0, c->stream, c->geo, c->cellType, c->q,
                      c->vel, do_clamp, (float)(c->maxSpeed * 1.1), c->ds);
   WS_LAUNCHED();
   return WS_OK;
@@ -296,7 +325,7 @@ int st_vorticity(ws_ctx *c) {
 }
 
 int st_clamp(ws_ctx *c) {
-  hipLaunchKernelGGL(k_clamp, dim3(blocks_for(nfaces(c), 256)), dim3(256), 0, c->stream, nfaces(c), c->vel,
+  hipLaunchKernelGGL(k_clamp, dim3(reduce_grid(nfaces(c))), dim3(256), 0, c->stream, nfaces(c), c->vel,
                      (float)(c->maxSpeed * 1.1), c->ds);
   WS_LAUNCHED();
   return WS_OK;
@@ -306,8 +335,8 @@ int st_extrapolate(ws_ctx *c) {
   for (size_t d = 0; d + 1 < c->exDiagOff.size(); d++) {
     int n = c->exDiagOff[d + 1] - c->exDiagOff[d];
     if (n > 0)
-      hipLaunchKernelGGL(k_extrap_diag, dim3(n), dim3(64), 0, c->stream, c->geo, c->exTiles, c->exDiagOff[d], c->vel,
-                         c->valid);
+      hipLaunchKernelGGL(k_extrap_diag, dim3(n), dim3(EX_TJ * EX_TK), 0, c->stream, c->geo, c->exTiles, c->exDiagOff[d],
+                         c->vel, c->valid);
   }
   WS_LAUNCHED();
   return WS_OK;
@@ -466,7 +495,7 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
   long cap = capacity > 0 ? capacity : 1;
   if ((r = dalloc(c, &c->staticTypes, c->nC)) || (r = dalloc(c, &c->gu, F)) || (r = dalloc(c, &c->thr, thr.size())) ||
       (r = dalloc(c, &c->cellType, c->nC)) || (r = dalloc(c, &c->nbm, c->nC)) || (r = dalloc(c, &c->dg, c->nC)) ||
-      (r = dalloc(c, &c->q, c->nC)) || (r = dalloc(c, &c->rhs, c->nC)) || (r = dalloc(c, &c->pw, c->nC)) ||
+      (r = dalloc(c, &c->q, c->nC)) || (r = dalloc(c, &c->q2, c->nC)) || (r = dalloc(c, &c->rhs, c->nC)) || (r = dalloc(c, &c->pw, c->nC)) ||
       (r = dalloc(c, &c->cellCount, c->nC + 1)) || (r = dalloc(c, &c->cellStart, c->nC + 1)) ||
       (r = dalloc(c, &c->vort, 10 * c->nC)) || (r = dalloc(c, &c->vel, F)) || (r = dalloc(c, &c->velO, F)) ||
       (r = dalloc(c, &c->wgt, F)) || (r = dalloc(c, &c->scr, F)) || (r = dalloc(c, &c->scr2, F)) ||
@@ -482,10 +511,14 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
     ws_destroy(c);
     return r;
   }
+  c->sorTiles[0] = (nx + SOR_TX - 1) / SOR_TX;
+  c->sorTiles[1] = (ny + SOR_TY - 1) / SOR_TY;
+  c->sorTiles[2] = (nz + SOR_TZ - 1) / SOR_TZ;
+  if (dalloc(c, &c->sorActive, (size_t)c->sorTiles[0] * c->sorTiles[1] * c->sorTiles[2])) { ws_destroy(c); return WS_EHIP; }
   c->gv = c->gu + c->uN;
   c->gw = c->gu + c->uN + c->vN;
   if (hipHostMalloc((void **)&c->hs, sizeof(DevScalars), 0) != hipSuccess) { ws_destroy(c); return WS_EHIP; }
-  c->nPartials = 512;
+  c->nPartials = 1024;
   if (dalloc(c, &c->partials, 2 * c->nPartials + 2048)) { ws_destroy(c); return WS_EHIP; }
   // scan scratch: enough for the larger of the cell and particle scans
   size_t b1 = 0, b2 = 0;
@@ -531,7 +564,8 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
 void ws_destroy(ws_ctx *c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
-  void *ptrs[] = {c->staticTypes, c->gu, c->thr, c->terrainR, c->cellType, c->nbm, c->dg, c->q, c->rhs, c->pw,
+  void *ptrs[] = {c->staticTypes, c->gu, c->thr, c->terrainR, c->cellType, c->nbm, c->dg, c->q, c->q2, c->rhs, c->pw,
+                  c->sorActive,
                   c->cellCount, c->cellStart, c->vort, c->vel, c->velO, c->wgt, c->scr, c->scr2, c->valid, c->dens,
                   c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pT, c->pAir, c->advU, c->advV, c->advW, c->pflag,
                   c->pcool, c->pcell, c->prank, c->sortedIdx, c->vneed, c->vscr, c->rng, c->ds, c->partials,
@@ -555,6 +589,7 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
   SF(atmosphereH) SF(spacing) SF(particleVolume) SF(iso) SF(substeps) SF(rainT) SF(cloudT) SF(rainLiftAge)
   SF(pressureTol) SF(dv) SF(sunX) SF(sunY) SF(sunZ) SF(dtFrameBase)
   SI(pressureIters) SI(maxPressureIters) SI(residualEvery) SI(ceilReflect) SI(hasSun) SI(measureResidual)
+  SI(adaptiveChecks) SI(sorTiled)
 #undef SF
 #undef SI
   if (!strcmp(n, "gRamp")) { c->gRamp = v; return push_ramp(c); }

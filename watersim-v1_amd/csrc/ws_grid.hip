@@ -376,11 +376,12 @@ __global__ void k_visc_iter(Geo g, const uint8_t *__restrict__ ct, const float *
 // fluid + air neighbours, relaxation weight omega/diag (stored float as the
 // reference's pW), fluid-neighbour mask; non-fluid cells hold q = 0.
 __global__ void k_assemble(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ vel,
-                           float *__restrict__ q, float *__restrict__ rhs, float *__restrict__ pw,
-                           uint8_t *__restrict__ nbm, uint8_t *__restrict__ dg, float omega) {
+                           float *__restrict__ q, float *__restrict__ q2, float *__restrict__ rhs,
+                           float *__restrict__ pw, uint8_t *__restrict__ nbm, uint8_t *__restrict__ dg,
+                           float omega) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nC) return;
-  if (ct[c] != FLUID) { q[c] = 0.f; pw[c] = 0.f; nbm[c] = 0; dg[c] = 0; return; }
+  if (ct[c] != FLUID) { q[c] = 0.f; q2[c] = 0.f; pw[c] = 0.f; nbm[c] = 0; dg[c] = 0; return; }
   int i = c % g.nx, j = (c / g.nx) % g.ny, k = c / (g.nx * g.ny);
   int nxy = g.nx * g.ny;
   int nb[6] = {i > 0 ? c - 1 : -1, i < g.nx - 1 ? c + 1 : -1, j > 0 ? c - g.nx : -1,
@@ -429,6 +430,114 @@ __global__ __launch_bounds__(256) void k_sor(Geo g, float *__restrict__ q, const
   if (m & 16) sum += q[c - nxy];
   if (m & 32) sum += q[c + nxy];
   q[c] = q[c] * om1 + pw[c] * (rhs[c] + sum);
+}
+
+// One full red-black sweep per launch, tiled through LDS. A block owns a
+// 32x8x8 cell tile: it loads q with a two-cell halo and the coefficients with
+// a one-cell halo, relaxes the red cells of the tile and its one-cell ring
+// (the ring values equal what the neighbouring blocks compute for them: same
+// inputs, same arithmetic), then the black cells of the tile, and writes the
+// tile to the other q buffer. Double buffering keeps every block reading the
+// previous sweep. Tiles without a fluid cell return at once.
+__global__ __launch_bounds__(256) void k_sor_tile(Geo g, int tilesX, int tilesY, const uint8_t *__restrict__ active,
+                                                  const float *__restrict__ qin, float *__restrict__ qout,
+                                                  const float *__restrict__ rhs, const float *__restrict__ pw,
+                                                  const uint8_t *__restrict__ nbm, float om1) {
+  constexpr int TX = SOR_TX, TY = SOR_TY, TZ = SOR_TZ;
+  constexpr int QX = TX + 4, QY = TY + 4, QZ = TZ + 4, QN = QX * QY * QZ;
+  constexpr int CX = TX + 2, CY = TY + 2, CZ = TZ + 2, CN = CX * CY * CZ;
+  __shared__ float sq[QN];
+  __shared__ float srhs[CN];
+  __shared__ float spw[CN];
+  __shared__ uint8_t snb[CN];
+  int b = xcd_block(blockIdx.x, gridDim.x);
+  if (!active[b]) return;
+  int tx = b % tilesX, ty = (b / tilesX) % tilesY, tz = b / (tilesX * tilesY);
+  int i0 = tx * TX, j0 = ty * TY, k0 = tz * TZ;
+  const int nxy = g.nx * g.ny;
+  constexpr int B = 8;
+  for (int e0 = 0; e0 < QN; e0 += 256 * B) {
+    float v[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      int e = e0 + u * 256 + threadIdx.x;
+      int li = e % QX, lj = (e / QX) % QY, lk = e / (QX * QY);
+      int i = i0 - 2 + li, j = j0 - 2 + lj, k = k0 - 2 + lk;
+      bool in = e < QN && i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
+      v[u] = in ? qin[in ? k * nxy + j * g.nx + i : 0] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      int e = e0 + u * 256 + threadIdx.x;
+      if (e < QN) sq[e] = v[u];
+    }
+  }
+  for (int e0 = 0; e0 < CN; e0 += 256 * B) {
+    float r[B], w[B];
+    uint8_t m[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      int e = e0 + u * 256 + threadIdx.x;
+      int li = e % CX, lj = (e / CX) % CY, lk = e / (CX * CY);
+      int i = i0 - 1 + li, j = j0 - 1 + lj, k = k0 - 1 + lk;
+      bool in = e < CN && i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
+      int c = in ? k * nxy + j * g.nx + i : 0;
+      r[u] = in ? rhs[c] : 0.f;
+      w[u] = in ? pw[c] : 0.f;
+      m[u] = in ? nbm[c] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      int e = e0 + u * 256 + threadIdx.x;
+      if (e < CN) { srhs[e] = r[u]; spw[e] = w[u]; snb[e] = m[u]; }
+    }
+  }
+  __syncthreads();
+  // red cells ((i+j+k) even) of the tile and its ring, then black cells of the tile
+  for (int color = 0; color < 2; color++) {
+    int lo = color == 0 ? 0 : 1, hiX = color == 0 ? CX : CX - 1, hiY = color == 0 ? CY : CY - 1;
+    int hiZ = color == 0 ? CZ : CZ - 1;
+    int nX = hiX - lo, nY = hiY - lo, nZ = hiZ - lo;
+    for (int e = threadIdx.x; e < nX * nY * nZ; e += 256) {
+      int ci = lo + e % nX, cj = lo + (e / nX) % nY, ck = lo + e / (nX * nY);
+      int i = i0 - 1 + ci, j = j0 - 1 + cj, k = k0 - 1 + ck;
+      if (((i + j + k) & 1) != color) continue;
+      int cc = (ck * CY + cj) * CX + ci;
+      unsigned msk = snb[cc];
+      if (!(msk & 0x40)) continue;
+      int qi = ((ck + 1) * QY + (cj + 1)) * QX + (ci + 1);
+      float sum = 0.f;
+      if (msk & 1) sum += sq[qi - 1];
+      if (msk & 2) sum += sq[qi + 1];
+      if (msk & 4) sum += sq[qi - QX];
+      if (msk & 8) sum += sq[qi + QX];
+      if (msk & 16) sum += sq[qi - QX * QY];
+      if (msk & 32) sum += sq[qi + QX * QY];
+      sq[qi] = sq[qi] * om1 + spw[cc] * (srhs[cc] + sum);
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < TX * TY * TZ; e += 256) {
+    int li = e % TX, lj = (e / TX) % TY, lk = e / (TX * TY);
+    int i = i0 + li, j = j0 + lj, k = k0 + lk;
+    if (i < g.nx && j < g.ny && k < g.nz) qout[k * nxy + j * g.nx + i] = sq[((lk + 2) * QY + (lj + 2)) * QX + (li + 2)];
+  }
+}
+
+// per SOR tile: does it hold a fluid cell (the tile sweep skips the rest)
+__global__ void k_sor_tiles_mark(Geo g, int tilesX, int tilesY, int nTiles, const uint8_t *__restrict__ nbm,
+                                 uint8_t *__restrict__ active) {
+  int b = blockIdx.x;
+  if (b >= nTiles) return;
+  int tx = b % tilesX, ty = (b / tilesX) % tilesY, tz = b / (tilesX * tilesY);
+  int i0 = tx * SOR_TX, j0 = ty * SOR_TY, k0 = tz * SOR_TZ;
+  int any = 0;
+  for (int e = threadIdx.x; e < SOR_TX * SOR_TY * SOR_TZ; e += blockDim.x) {
+    int i = i0 + e % SOR_TX, j = j0 + (e / SOR_TX) % SOR_TY, k = k0 + e / (SOR_TX * SOR_TY);
+    if (i < g.nx && j < g.ny && k < g.nz && (nbm[(k * g.ny + j) * g.nx + i] & 0x40)) any = 1;
+  }
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) active[b] = (uint8_t)any;
 }
 
 // ||rhs - A q||^2 and ||rhs||^2 over fluid cells, FP64 block partials
@@ -480,19 +589,33 @@ __global__ void k_residual_final(const double *__restrict__ partials, int n, Dev
 // ---------------------------------------------------------- projection
 // _projectSlice, solver.js:1417-1443, with the grid clamp and umax of
 // _clampUmSlice (solver.js:2629-2653) fused when confinement is off.
-__device__ __forceinline__ void umax_reduce(float av, unsigned *umax_bits) {
-  // max of non-negative floats = max of their bit patterns; NaN never wins
-  unsigned bits = (av == av) ? __float_as_uint(av) : 0u;
+// Block max of |u| into one atomic per block (the max of non-negative
+// floats is the max of their bit patterns, so the result is exact and order
+// independent; NaN never wins). Kernels using it run a bounded grid-stride
+// loop so the number of atomics on the single counter stays small.
+__device__ __forceinline__ void umax_reduce(unsigned bits, unsigned *umax_bits) {
+  __shared__ unsigned wmax[16];
   for (int o = 32; o > 0; o >>= 1) bits = max(bits, (unsigned)__shfl_xor((int)bits, o));
-  if ((threadIdx.x & 63) == 0 && bits) atomicMax(umax_bits, bits);
+  int wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) wmax[wv] = bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned m = 0;
+    for (int i = 0; i < nw; i++) m = max(m, wmax[i]);
+    if (m) atomicMax(umax_bits, m);
+  }
 }
 
-__global__ void k_project(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ phi,
-                          float *__restrict__ vel, int do_clamp, float vcap, DevScalars *s) {
+__device__ __forceinline__ unsigned abs_bits(float u) {
+  float av = fabsf(u);
+  return (av == av) ? __float_as_uint(av) : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_project(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ phi,
+                                                 float *__restrict__ vel, int do_clamp, float vcap, DevScalars *s) {
   long F = (long)g.uN + g.vN + g.wN;
-  long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  float av = 0.f;
-  if (f < F) {
+  unsigned bits = 0;
+  for (long f = (long)blockIdx.x * blockDim.x + threadIdx.x; f < F; f += (long)gridDim.x * blockDim.x) {
     FaceInfo fi;
     face_of(g, f, fi);
     float u = vel[f];
@@ -505,23 +628,22 @@ __global__ void k_project(Geo g, const uint8_t *__restrict__ ct, const float *__
     }
     if (do_clamp) {
       if (u > vcap) u = vcap; else if (u < -vcap) u = -vcap;
-      av = fabsf(u);
+      bits = max(bits, abs_bits(u));
     }
     vel[f] = u;
   }
-  if (do_clamp) umax_reduce(av, &s->umax_bits);
+  if (do_clamp) umax_reduce(bits, &s->umax_bits);
 }
 
-__global__ void k_clamp(long F, float *__restrict__ vel, float vcap, DevScalars *s) {
-  long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  float av = 0.f;
-  if (f < F) {
+__global__ __launch_bounds__(256) void k_clamp(long F, float *__restrict__ vel, float vcap, DevScalars *s) {
+  unsigned bits = 0;
+  for (long f = (long)blockIdx.x * blockDim.x + threadIdx.x; f < F; f += (long)gridDim.x * blockDim.x) {
     float u = vel[f];
     if (u > vcap) u = vcap; else if (u < -vcap) u = -vcap;
     vel[f] = u;
-    av = fabsf(u);
+    bits = max(bits, abs_bits(u));
   }
-  umax_reduce(av, &s->umax_bits);
+  umax_reduce(bits, &s->umax_bits);
 }
 
 // ------------------------------------------------ vorticity confinement
@@ -606,13 +728,14 @@ __global__ void k_vort_splat(Geo g, const uint8_t *__restrict__ ct, const float 
 // an invalid face takes the mean of its valid neighbours, counting faces
 // filled earlier in the same pass. A face therefore depends on its final
 // -i/-j/-k neighbours and on the original state of its +i/+j/+k neighbours.
-// Tiles of 32x8x8 faces are processed one tile-diagonal (I+J+K) per launch;
-// inside a tile a hyperplane wavefront (li+lj+lk) keeps the same dependency
-// order, so every face sees exactly what the sequential pass shows it.
-__global__ __launch_bounds__(64) void k_extrap_diag(Geo g, const int4 *__restrict__ tiles, int off,
-                                                    float *__restrict__ vel, uint8_t *__restrict__ valid) {
+// Tiles of 32x16x16 faces are processed one tile-diagonal (I+J+K) per
+// launch; inside a tile a hyperplane wavefront (li+lj+lk) keeps the same
+// dependency order, so every face sees exactly what the sequential pass
+// shows it. Tiles on one diagonal touch disjoint faces.
+__global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restrict__ tiles, int off,
+                                                     float *__restrict__ vel, uint8_t *__restrict__ valid) {
   constexpr int TI = EX_TI, TJ = EX_TJ, TK = EX_TK;
-  constexpr int HI = TI + 2, HJ = TJ + 2, HK = TK + 2;
+  constexpr int HI = TI + 2, HJ = TJ + 2, HK = TK + 2, HN = HI * HJ * HK;
   __shared__ float sv[HK][HJ][HI];
   __shared__ uint8_t sf[HK][HJ][HI];
   int4 t = tiles[off + blockIdx.x];
@@ -622,18 +745,30 @@ __global__ __launch_bounds__(64) void k_extrap_diag(Geo g, const int4 *__restric
   float *A = vel + base;
   uint8_t *V = valid + base;
   int i0 = I * TI - 1, j0 = J * TJ - 1, k0 = K * TK - 1;
-  for (int e = threadIdx.x; e < HI * HJ * HK; e += blockDim.x) {
-    int li = e % HI, lj = (e / HI) % HJ, lk = e / (HI * HJ);
-    int i = i0 + li, j = j0 + lj, k = k0 + lk;
-    float val = 0.f;
-    uint8_t fl = 0;
-    if (i >= 0 && i < d0 && j >= 0 && j < d1 && k >= 0 && k < d2) {
-      long f = ((long)k * d1 + j) * d0 + i;
-      val = A[f];
-      fl = V[f] ? 1 : 0;
+  // tile + one-face halo into LDS; loads are issued in batches of 8 before
+  // any is consumed so their latencies overlap
+  constexpr int B = 8;
+  for (int e0 = 0; e0 < HN; e0 += 256 * B) {
+    float val[B];
+    uint8_t fl[B];
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      int e = e0 + b * 256 + threadIdx.x;
+      int li = e % HI, lj = (e / HI) % HJ, lk = e / (HI * HJ);
+      int i = i0 + li, j = j0 + lj, k = k0 + lk;
+      bool in = e < HN && i >= 0 && i < d0 && j >= 0 && j < d1 && k >= 0 && k < d2;
+      long f = in ? ((long)k * d1 + j) * d0 + i : 0;
+      val[b] = in ? A[f] : 0.f;
+      fl[b] = in ? V[f] : 0;
     }
-    sv[lk][lj][li] = val;
-    sf[lk][lj][li] = fl;
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      int e = e0 + b * 256 + threadIdx.x;
+      if (e < HN) {
+        (&sv[0][0][0])[e] = val[b];
+        (&sf[0][0][0])[e] = fl[b] ? 1 : 0;
+      }
+    }
   }
   __syncthreads();
   int lj = 1 + (threadIdx.x % TJ), lk = 1 + (threadIdx.x / TJ);
@@ -651,11 +786,11 @@ __global__ __launch_bounds__(64) void k_extrap_diag(Geo g, const int4 *__restric
       if (k > 0 && sf[lk - 1][lj][li]) { sum += sv[lk - 1][lj][li]; cnt++; }
       if (k < d2 - 1 && sf[lk + 1][lj][li]) { sum += sv[lk + 1][lj][li]; cnt++; }
       if (cnt > 0) {
-        float val = sum / (float)cnt;
-        sv[lk][lj][li] = val;
+        float v = sum / (float)cnt;
+        sv[lk][lj][li] = v;
         sf[lk][lj][li] = 2;
         long f = ((long)k * d1 + j) * d0 + i;
-        A[f] = val;
+        A[f] = v;
         V[f] = 1;
       }
     }

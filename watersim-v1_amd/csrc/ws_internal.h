@@ -23,7 +23,8 @@ constexpr float kFixInvF = 1.0f / 1125899906842624.0f;
 
 // Tile shapes (cells) for P2G and the extrapolation wavefront.
 constexpr int P2G_T = 8;
-constexpr int EX_TI = 32, EX_TJ = 8, EX_TK = 8;
+constexpr int EX_TI = 32, EX_TJ = 16, EX_TK = 16;
+constexpr int SOR_TX = 32, SOR_TY = 8, SOR_TZ = 8;
 
 // Scalars that live on the device so that substeps can run back to back
 // without host round trips (dt law, gravity ramp, reductions).
@@ -68,7 +69,7 @@ struct Ctx {
   int pressureIters = 20, maxPressureIters = 4000, nP = 0, tn = 0;
   double pressureTol = 0, dv = 0;
   int residualEvery = 8;
-  int measureResidual = 0, ceilReflect = 0, hasSun = 0;
+  int measureResidual = 0, ceilReflect = 0, hasSun = 0, adaptiveChecks = 1;
   double sunX = 0, sunY = 0, sunZ = 0, dtFrameBase = 0;
   bool hasVapor = false;
   // live timers (HIP events on the context stream), enabled by "timing"
@@ -91,7 +92,10 @@ struct Ctx {
   float *thr = nullptr;  // exact floor(x/dx) thresholds, max(n)+2 entries
   // cells
   uint8_t *cellType = nullptr, *nbm = nullptr, *dg = nullptr;
-  float *q = nullptr, *rhs = nullptr, *pw = nullptr;
+  float *q = nullptr, *q2 = nullptr, *rhs = nullptr, *pw = nullptr;
+  uint8_t *sorActive = nullptr;  // per SOR tile: holds a fluid cell
+  int sorTiles[3] = {0, 0, 0};
+  int sorTiled = 1;              // 1: LDS-tiled full sweeps, 0: one launch per colour
   unsigned *cellCount = nullptr, *cellStart = nullptr;
   float *vort = nullptr;  // 10*C scratch for vorticity confinement
   // faces (u | v | w contiguous in one allocation each)

@@ -30,8 +30,11 @@ __global__ void k_valid(Geo g, const uint8_t *ct, const float *wgt, uint8_t *val
 __global__ void k_gravity_bc(Geo g, const uint8_t *ct, float *vel, const float *gvec, const DevScalars *s);
 __global__ void k_visc_iter(Geo g, const uint8_t *ct, const float *old, const float *cur, float *out, float nu_dx2,
                             const DevScalars *s, int first);
-__global__ void k_assemble(Geo g, const uint8_t *ct, const float *vel, float *q, float *rhs, float *pw,
+__global__ void k_assemble(Geo g, const uint8_t *ct, const float *vel, float *q, float *q2, float *rhs, float *pw,
                            uint8_t *nbm, uint8_t *dg, float omega);
+__global__ void k_sor_tile(Geo g, int tilesX, int tilesY, const uint8_t *active, const float *qin, float *qout,
+                           const float *rhs, const float *pw, const uint8_t *nbm, float om1);
+__global__ void k_sor_tiles_mark(Geo g, int tilesX, int tilesY, int nTiles, const uint8_t *nbm, uint8_t *active);
 __global__ void k_sor(Geo g, float *q, const float *rhs, const float *pw, const uint8_t *nbm, int color,
                       float om1);
 __global__ void k_residual(Geo g, const float *q, const float *rhs, const uint8_t *nbm, const uint8_t *dg,
