@@ -54,7 +54,11 @@ enum ws_stage_id {
   WS_STAGE_EXTRAPOLATE = 11, /* _extrapolate, :1562-1636 */
   WS_STAGE_G2P = 12,       /* _g2p, :1639-1756 */
   WS_STAGE_ADVECT = 13,    /* _advect, :1759-2292 */
-  WS_STAGE_COUNT = 14
+  WS_STAGE_HEAT = 14,      /* _updateHeat(dt), :2715-2994 (dt = tick length) */
+  WS_STAGE_EVAPORATION = 15, /* _updateEvaporation(dt), :3008-3125 */
+  WS_STAGE_CURRENTS = 16,  /* _updateCurrents(dt), :3658-3782 */
+  WS_STAGE_PHASE = 17,     /* _updatePhaseChanges(dtFrame), :3152-3281 */
+  WS_STAGE_COUNT = 18
 };
 
 /* new FluidSolver({nx,ny,nz,dx}) plus _allocParticles(capacity),
@@ -92,9 +96,12 @@ int ws_substep(ws_ctx *c, double dt);
 /* One stage of the substep, for the reference's stage methods; dt is used by
  * the stages that take it. */
 int ws_stage(ws_ctx *c, int stage, double dt);
-/* step(dtFrame): the CFL substep loop of _stepSync, solver.js:2663-2700
+/* step(dtFrame): _stepSync, solver.js:2663-2700: the 25 Hz thermal,
+ * evaporation and currents tick, the CFL substep loop
  * (dt = min(dtFrame/substeps, cfl*dx/max(umax,0.08), 1/100), >= 1e-5,
- * at most 10 substeps). Returns the substep count through *n_sub. */
+ * at most 10 substeps) and the frame-end phase pass. The reference's
+ * vapor collisions, ice contacts and surface rescue passes are not part of
+ * this build. Returns the substep count through *n_sub. */
 int ws_step_frame(ws_ctx *c, double dt_frame, int *n_sub);
 /* n substeps with the CFL law evaluated on the device (no host round trip
  * between substeps): the measurement path for substeps/s. */

@@ -40,7 +40,8 @@ def lib(bits=32):
                    "oc_splat_density"]:
             getattr(L, fn).argtypes = [vp]
         L.oc_mark_fluid.argtypes = [vp, ci, ci]
-        for fn in ["oc_viscosity", "oc_apply_gravity", "oc_vorticity", "oc_advect", "oc_substep"]:
+        for fn in ["oc_viscosity", "oc_apply_gravity", "oc_vorticity", "oc_advect", "oc_substep", "oc_update_heat",
+                   "oc_update_evaporation", "oc_update_currents", "oc_update_phase"]:
             getattr(L, fn).argtypes = [vp, d]
         L.oc_g2p.argtypes = [vp, d]
         L.oc_g2p.restype = ci
@@ -76,7 +77,7 @@ def _ptr(a):
 
 
 GRID_F = ["u", "v", "w", "uW", "vW", "wW", "uO", "vO", "wO"]
-PART_F = ["px", "py", "pz", "pvx", "pvy", "pvz", "pT", "pAir", "pWx", "pWy", "pWz"]
+PART_F = ["px", "py", "pz", "pvx", "pvy", "pvz", "pT", "pAir", "pWx", "pWy", "pWz", "pLight", "pPh"]
 
 
 class OracleState:
@@ -161,7 +162,7 @@ class OracleState:
             if k in arrs:
                 a[:n] = arrs[k][:n]
             self._bind(k, a)
-        for k in ["pflag", "pcool"]:
+        for k in ["pflag", "pcool", "pDepth"]:
             a = np.zeros(cap, np.uint8)
             if k in arrs:
                 a[:n] = arrs[k][:n]
@@ -171,7 +172,7 @@ class OracleState:
 
     def set_params(self, **kw):
         ints = {"pressureIters", "ceilReflect", "hasSun", "cfS", "bfS", "evS", "nP", "tn", "sorRedBlack",
-                "residualEvery", "maxPressureIters"}
+                "residualEvery", "maxPressureIters", "spinOn"}
         for k, v in kw.items():
             (self.set_i if k in ints else self.set_f)(k, v)
 

@@ -51,6 +51,12 @@ struct oc_ctx {
   oc_real *px, *py, *pz, *pvx, *pvy, *pvz, *pT, *pAir, *pWx, *pWy, *pWz;
   uint8_t *pflag, *pcool;
   double *advU, *advV, *advW;
+  /* frame tick (solver.js:137-193, 2715-3281) */
+  double heatK, Tamb, shadeCool, sunActivity, evapIntensity, snowT, meltT, cloudP, currents;
+  double Rlo, Rhi, simTime;
+  int spinOn, evapCount;
+  oc_real *pLight, *pPh;
+  uint8_t *pDepth;
   /* outputs */
   int airborne, nFluid;
   double umax;
@@ -85,6 +91,8 @@ oc_ctx *oc_create(int nx, int ny, int nz, double dx) {
   c->gRamp = 1e9;
   c->rainT = 0.22; c->cloudT = 0.34; c->rainLiftAge = 0.4;
   c->cfS = 0x2f6e2b1u; c->bfS = 0x7f4a7c15u; c->evS = 0x51ab3c77u;
+  c->heatK = 0.6; c->Tamb = 0.32; c->shadeCool = 0.035; c->evapIntensity = 0.05;
+  c->snowT = 0.10; c->meltT = 0.105; c->cloudP = 0.35;
   c->fluidList = (int *)malloc(sizeof(int) * c->nC);
   c->pStart = (int *)malloc(sizeof(int) * (c->nC + 1));
   c->pAdj = (int *)malloc(sizeof(int) * 6 * c->nC);
@@ -119,6 +127,7 @@ int oc_set_ptr(oc_ctx *c, const char *name, void *ptr) {
   PTR(pWx, oc_real *) PTR(pWy, oc_real *) PTR(pWz, oc_real *)
   PTR(pflag, uint8_t *) PTR(pcool, uint8_t *)
   PTR(advU, double *) PTR(advV, double *) PTR(advW, double *)
+  PTR(pLight, oc_real *) PTR(pPh, oc_real *) PTR(pDepth, uint8_t *)
   return -1;
 }
 #undef PTR
@@ -129,11 +138,13 @@ int oc_set_f(oc_ctx *c, const char *name, double v) {
   FLD(coreR) FLD(oceanR) FLD(atmosphereH) FLD(spacing) FLD(particleVolume)
   FLD(gRamp) FLD(dtLast) FLD(rainT) FLD(cloudT) FLD(rainLiftAge) FLD(dv)
   FLD(sunX) FLD(sunY) FLD(sunZ) FLD(umax) FLD(pressureTol)
+  FLD(heatK) FLD(Tamb) FLD(shadeCool) FLD(sunActivity) FLD(evapIntensity) FLD(snowT) FLD(meltT) FLD(cloudP)
+  FLD(currents) FLD(Rlo) FLD(Rhi) FLD(simTime) FLD(cloudT) FLD(rainT) FLD(rainLiftAge)
   return -1;
 }
 int oc_set_i(oc_ctx *c, const char *name, long long v) {
   FLD(pressureIters) FLD(nP) FLD(tn) FLD(ceilReflect) FLD(hasSun)
-  FLD(cfS) FLD(bfS) FLD(evS) FLD(sorRedBlack) FLD(residualEvery) FLD(maxPressureIters)
+  FLD(cfS) FLD(bfS) FLD(evS) FLD(sorRedBlack) FLD(residualEvery) FLD(maxPressureIters) FLD(spinOn)
   return -1;
 }
 #undef FLD
@@ -144,7 +155,7 @@ double oc_get_f(oc_ctx *c, const char *name) {
   return NAN;
 }
 long long oc_get_i(oc_ctx *c, const char *name) {
-  GET(airborne) GET(nFluid) GET(nP) GET(cfS) GET(bfS) GET(evS) GET(itersDone)
+  GET(airborne) GET(nFluid) GET(nP) GET(cfS) GET(bfS) GET(evS) GET(itersDone) GET(evapCount)
   return -1;
 }
 #undef GET
@@ -1165,6 +1176,409 @@ double oc_dt_law(oc_ctx *c) {
   if (dt > 1.0 / 100) dt = 1.0 / 100;
   if (dt < 1e-5) dt = 1e-5;
   return dt;
+}
+
+/* ------------------------------------------------------- frame tick */
+static int trunc_clamp(double v, int n) {
+  /* (v | 0) then clamped to [0, n-1] */
+  int i = (int)v;
+  return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+/* _updateHeat, solver.js:2715-2994: exposure + relaxation per particle,
+ * conservative cell conduction on cell means (f64), radial convection */
+void oc_update_heat(oc_ctx *c, double dt) {
+  int nP = c->nP, nx = c->nx, ny = c->ny, nz = c->nz;
+  if (nP < 1) return;
+  double dx = c->dx, alpha = c->heatK * dt * 0.15;
+  if (alpha > 0.14) alpha = 0.14;
+  long nC = c->nC;
+  int *cnt = (int *)calloc(nC, sizeof(int));
+  int *cell = (int *)malloc(sizeof(int) * nP);
+  for (int p = 0; p < nP; p++) {
+    int gx = trunc_clamp(c->px[p] / dx, nx), gy = trunc_clamp(c->py[p] / dx, ny), gz = trunc_clamp(c->pz[p] / dx, nz);
+    cell[p] = (gz * ny + gy) * nx + gx;
+    cnt[cell[p]]++;
+  }
+  /* _heatShadowChunk, :2773-2918 */
+  double cx = c->cx, cy = c->cy, cz = c->cz, coreR2 = c->coreR * c->coreR;
+  long sj = nx + 1, sk = (long)(ny + 1) * sj;
+  double sunPow = c->sunActivity / 3, dissip = 0.045, half = dx, marchMax = 0.55, absK = 1.1;
+  const uint8_t *type = c->cellType;
+  for (int p = 0; p < nP; p++) {
+    double X = c->px[p], Y = c->py[p], Z = c->pz[p];
+    double lit = c->hasSun ? 1 : 0;
+    double dwx = X - cx, dwy = Y - cy, dwz = Z - cz;
+    double dwl = sqrt(dwx * dwx + dwy * dwy + dwz * dwz);
+    if (!(dwl != 0)) dwl = 1e-9;
+    double stpU = dx / dwl;
+    int above = 0;
+    for (int st = 1; st <= 4; st++) {
+      double qx = X + dwx * stpU * st, qy = Y + dwy * stpU * st, qz = Z + dwz * stpU * st;
+      int qi = (int)(qx / dx), qj = (int)(qy / dx), qk = (int)(qz / dx);
+      if (qi < 0 || qj < 0 || qk < 0 || qi >= nx || qj >= ny || qk >= nz) break;
+      if (type[((long)qk * ny + qj) * nx + qi] != FLUID) break;
+      above++;
+    }
+    if (c->pDepth) c->pDepth[p] = (uint8_t)above;
+    if (c->hasSun) {
+      if (above >= 4) lit = 0;
+      else {
+        double lx = c->sunX - X, ly = c->sunY - Y, lz = c->sunZ - Z;
+        double ld = sqrt(lx * lx + ly * ly + lz * lz);
+        if (!(ld != 0)) ld = 1e-9;
+        double dxn = lx / ld, dyn = ly / ld, dzn = lz / ld;
+        double ex = cx - X, ey = cy - Y, ez = cz - Z;
+        double b = ex * dxn + ey * dyn + ez * dzn;
+        if (c->tsolid) {
+          double d2c = ex * ex + ey * ey + ez * ez - b * b;
+          if (b > 0 && d2c < c->Rlo * c->Rlo * 0.96) lit = 0;
+          else if (b > 0 || d2c < c->Rhi * c->Rhi * 1.1) {
+            double stp = dx * 1.5, mx = X, my = Y, mz = Z, esc2 = (c->Rhi + 0.1) * (c->Rhi + 0.1);
+            int tvn = c->tn;
+            for (int st2 = 0; st2 < 26; st2++) {
+              mx += dxn * stp; my += dyn * stp; mz += dzn * stp;
+              double rex = mx - cx, rey = my - cy, rez = mz - cz;
+              if (rex * rex + rey * rey + rez * rez > esc2) break;
+              int ivx = (int)(mx / c->dv); if (ivx < 0 || ivx >= tvn) break;
+              int ivy = (int)(my / c->dv); if (ivy < 0 || ivy >= tvn) break;
+              int ivz = (int)(mz / c->dv); if (ivz < 0 || ivz >= tvn) break;
+              if (c->tsolid[((long)ivz * tvn + ivy) * tvn + ivx]) { lit = 0; break; }
+            }
+          }
+        } else if (b > 0) {
+          double d2c2 = ex * ex + ey * ey + ez * ez - b * b;
+          if (d2c2 < coreR2) lit = 0;
+        }
+        if (lit != 0) {
+          double od = 0, sx = X, sy = Y, sz = Z;
+          for (int st = 0; st < 6 && st * half < marchMax; st++) {
+            sx += dxn * half; sy += dyn * half; sz += dzn * half;
+            int i0 = (int)(sx / dx), j0 = (int)(sy / dx), k0 = (int)(sz / dx);
+            if (i0 < 0 || j0 < 0 || k0 < 0 || i0 >= nx || j0 >= ny || k0 >= nz) break;
+            od += c->dens[k0 * sk + j0 * sj + i0] * half * absK;
+            if (od > 3.5) break;
+          }
+          lit = od < 3.5 ? exp(-od) : 0;
+        }
+      }
+    }
+    if (c->pLight) c->pLight[p] = (oc_real)lit;
+    double T, Tp = c->pT[p];
+    int fl = c->pflag[p];
+    if (fl == 2) {
+      double amb = lit > 0.05 ? c->Tamb + 0.18 : 0.06, k = c->hasSun ? 0.16 + (1 - lit) * 0.20 : 1.2;
+      T = Tp + (amb - Tp) * k * dt;
+      if (lit > 0) T += sunPow * 0.6 * lit * dt;
+    } else if (fl == 3) {
+      double amb = lit > 0.05 ? c->Tamb + 0.18 : 0.06, k = c->hasSun ? 0.10 + (1 - lit) * 0.12 : 0.9;
+      T = Tp + (amb - Tp) * k * dt;
+      if (lit > 0) T += sunPow * 0.35 * lit * dt;
+    } else if (fl == 4) {
+      double amb = lit > 0.05 ? c->Tamb + 0.18 : 0.06, k = c->hasSun ? 0.14 + (1 - lit) * 0.16 : 1.0;
+      T = Tp + (amb - Tp) * k * dt;
+      if (lit > 0) T += sunPow * 0.45 * lit * dt;
+    } else {
+      double amb = lit > 0.05 ? c->Tamb : 0;
+      T = Tp + (amb - Tp) * (dissip + (1 - lit) * c->shadeCool) * dt;
+      if (lit > 0) T += sunPow * lit * dt;
+    }
+    if (T > 1.15) T = 1.15; else if (T < 0) T = 0;
+    c->pT[p] = (oc_real)T;
+  }
+  if (c->heatK > 0) {
+    /* _heatMeansSlice, _heatConductFlux, _heatApplySlice, :2921-2979 */
+    double *means = (double *)calloc(nC, sizeof(double)), *changes = (double *)calloc(nC, sizeof(double));
+    for (int p = 0; p < nP; p++) means[cell[p]] += c->pT[p];
+    for (long cc = 0; cc < nC; cc++) means[cc] = cnt[cc] ? means[cc] / cnt[cc] : 0;
+    long nxy = (long)nx * ny;
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+          long cc = ((long)k * ny + j) * nx + i;
+          if (!cnt[cc]) continue;
+          for (int axis = 0; axis < 3; axis++) {
+            if ((axis == 0 && i == nx - 1) || (axis == 1 && j == ny - 1) || (axis == 2 && k == nz - 1)) continue;
+            long nb = cc + (axis == 0 ? 1 : axis == 1 ? nx : nxy);
+            if (!cnt[nb]) continue;
+            double e = alpha * (cnt[cc] < cnt[nb] ? cnt[cc] : cnt[nb]) * (means[nb] - means[cc]);
+            changes[cc] += e;
+            changes[nb] -= e;
+          }
+        }
+    for (int p = 0; p < nP; p++) {
+      long cc = cell[p];
+      double deltaT = changes[cc] / cnt[cc];
+      c->pT[p] = (oc_real)(c->pT[p] + (alpha * (means[cc] - c->pT[p]) + deltaT));
+    }
+    free(means);
+    free(changes);
+  }
+  /* _heatConvectChunk, :2982-2994 */
+  for (int p = 0; p < nP; p++) {
+    if (c->pflag[p] != 0) continue;
+    double rx = c->px[p] - cx, ry = c->py[p] - cy, rz = c->pz[p] - cz;
+    double rl = sqrt(rx * rx + ry * ry + rz * rz);
+    if (!(rl != 0)) rl = 1e-9;
+    double ab = 3.0 * (c->pT[p] - c->Tamb) * dt / rl;
+    c->pvx[p] = (oc_real)(c->pvx[p] + rx * ab);
+    c->pvy[p] = (oc_real)(c->pvy[p] + ry * ab);
+    c->pvz[p] = (oc_real)(c->pvz[p] + rz * ab);
+  }
+  free(cnt);
+  free(cell);
+}
+
+static double ev_gauss(oc_ctx *c) {
+  /* _evGauss, solver.js:3120-3125 */
+  double u1;
+  do { c->evS = lcg(c->evS); u1 = c->evS / 4294967296.0; } while (u1 < 1e-9);
+  c->evS = lcg(c->evS);
+  return sqrt(-2 * log(u1)) * cos(6.283185306 * (c->evS / 4294967296.0));
+}
+
+/* _updateEvaporation / _evapChunk, solver.js:3008-3116 (sphere world) */
+void oc_update_evaporation(oc_ctx *c, double dt) {
+  double rate = c->sunActivity * c->evapIntensity;
+  c->evapCount = 0;
+  if (c->sunActivity <= 0 || c->evapIntensity <= 0 || rate <= 0) return;
+  double freezeT = c->meltT > c->snowT * 1.05 ? c->meltT : c->snowT * 1.05;
+  int nx = c->nx, ny = c->ny, nz = c->nz, n = 0;
+  double dx = c->dx, cx = c->cx, cy = c->cy, cz = c->cz;
+  for (int p = 0; p < c->nP; p++) {
+    if (c->pflag[p] != 0) continue;
+    double T = c->pT[p];
+    if (T <= freezeT) continue;
+    int i = (int)(c->px[p] / dx), j = (int)(c->py[p] / dx), k = (int)(c->pz[p] / dx);
+    i = clampi(i, 1, nx - 2); j = clampi(j, 1, ny - 2); k = clampi(k, 1, nz - 2);
+    long cc = ((long)k * ny + j) * nx + i;
+    if (c->cellType[cc] != FLUID) continue;
+    double ex = c->px[p] - cx, ey = c->py[p] - cy, ez = c->pz[p] - cz;
+    double el = sqrt(ex * ex + ey * ey + ez * ez);
+    if (!(el != 0)) el = 1e-9;
+    int ui = (int)((c->px[p] + ex / el * dx) / dx), uj = (int)((c->py[p] + ey / el * dx) / dx);
+    int uk = (int)((c->pz[p] + ez / el * dx) / dx);
+    if (ui < 0 || uj < 0 || uk < 0 || ui >= nx || uj >= ny || uk >= nz) continue;
+    if (c->cellType[((long)uk * ny + uj) * nx + ui] == FLUID) continue;
+    double tau = (T - freezeT) / (1.15 - freezeT);
+    if (tau < 0) tau = 0; else if (tau > 1) tau = 1;
+    double prob = rate * dt * exp(3.5 * (T - freezeT));
+    if (prob > 1) prob = 1;
+    c->evS = lcg(c->evS);
+    double rnd = c->evS / 4294967296.0;
+    if (rnd > prob) continue;
+    c->pflag[p] = 2;
+    c->pAir[p] = 0;
+    double sig = 0.30 + 0.32 * sqrt(tau);
+    double kx = sig * ev_gauss(c), ky = sig * ev_gauss(c), kz = sig * ev_gauss(c);
+    double ex2 = c->px[p] - cx, ey2 = c->py[p] - cy, ez2 = c->pz[p] - cz;
+    double el2 = ex2 * ex2 + ey2 * ey2 + ez2 * ez2;
+    double elL = sqrt(el2);
+    if (!(elL != 0)) elL = 1e-9;
+    double kUp = (kx * ex2 + ky * ey2 + kz * ez2) / el2;
+    if (kUp < 0) { kx -= 2 * kUp * ex2 / elL; ky -= 2 * kUp * ey2 / elL; kz -= 2 * kUp * ez2 / elL; }
+    double guS = c->u[((long)k * ny + j) * (nx + 1) + i];
+    double gvS = c->v[((long)k * (ny + 1) + j) * nx + i];
+    double gwS = c->w[((long)k * ny + j) * nx + i];
+    double nl2 = 1 / el2, vN = ((double)c->pvx[p] * ex2 + (double)c->pvy[p] * ey2 + (double)c->pvz[p] * ez2) * nl2;
+    double vNx = vN * ex2, vNy = vN * ey2, vNz = vN * ez2;
+    double gN = (guS * ex2 + gvS * ey2 + gwS * ez2) * nl2;
+    double vtx = guS - gN * ex2, vty = gvS - gN * ey2, vtz = gwS - gN * ez2;
+    c->pvx[p] = (oc_real)(vNx + vtx + kx);
+    c->pvy[p] = (oc_real)(vNy + vty + ky);
+    c->pvz[p] = (oc_real)(vNz + vtz + kz);
+    n++;
+  }
+  c->evapCount = n;
+}
+
+/* _updateCurrents, solver.js:3634-3782 */
+void oc_update_currents(oc_ctx *c, double dt) {
+  double strength = c->currents, shell = c->oceanR - c->coreR;
+  if (!(strength > 0) || !(shell > 0)) return;
+  double seed = 1234567;
+  double K[3] = {1.5, 4.0, 9.0}, amp3[3] = {0.10, 0.085, 0.05}, drift[3] = {0.021, 0.05, 0.11};
+  double wave[3][3][5];
+  for (int o = 0; o < 3; o++)
+    for (int cmp = 0; cmp < 3; cmp++) {
+      double r3[3];
+      for (int q = 0; q < 3; q++) { seed = fmod(seed * 1664525 + 1013904223, 4294967296.0); r3[q] = seed / 4294967296.0 * 2 - 1; }
+      double L = sqrt(r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2]);
+      if (!(L != 0)) L = 1;
+      wave[o][cmp][0] = r3[0] / L * K[o]; wave[o][cmp][1] = r3[1] / L * K[o]; wave[o][cmp][2] = r3[2] / L * K[o];
+      wave[o][cmp][3] = amp3[o] / K[o];
+      seed = fmod(seed * 1664525 + 1013904223, 4294967296.0);
+      wave[o][cmp][4] = seed / 4294967296.0 * 6.283;
+    }
+  int stride = 4, gnx = c->nx / stride + 2, gny = c->ny / stride + 2, gnz = c->nz / stride + 2;
+  long gN = (long)gnx * gny * gnz;
+  float *cf = (float *)malloc(sizeof(float) * 3 * gN);
+  double sx = c->dx * stride, t = c->simTime, amp = strength * 2;
+  for (int kk = 0; kk < gnz; kk++)
+    for (int jj = 0; jj < gny; jj++)
+      for (int ii = 0; ii < gnx; ii++) {
+        double wx2 = ii * sx, wy2 = jj * sx, wz2 = kk * sx, ux = 0, uy = 0, uz = 0;
+        for (int o = 0; o < 3; o++) {
+          double dph = drift[o] * t, th[3];
+          for (int q = 0; q < 3; q++) th[q] = wave[o][q][0] * wx2 + wave[o][q][1] * wy2 + wave[o][q][2] * wz2 + dph + wave[o][q][4];
+          ux += wave[o][2][3] * wave[o][2][1] * cos(th[2]) - wave[o][1][3] * wave[o][1][2] * cos(th[1]);
+          uy += wave[o][0][3] * wave[o][0][2] * cos(th[0]) - wave[o][2][3] * wave[o][2][0] * cos(th[2]);
+          uz += wave[o][1][3] * wave[o][1][0] * cos(th[1]) - wave[o][0][3] * wave[o][0][1] * cos(th[0]);
+        }
+        long gi = ((long)kk * gny + jj) * gnx + ii;
+        cf[gi] = (float)(ux * amp); cf[gN + gi] = (float)(uy * amp); cf[2 * gN + gi] = (float)(uz * amp);
+      }
+  double relax = 2.2 * dt;
+  if (relax > 0.35) relax = 0.35;
+  double invS = 1 / (c->dx * stride);
+  long gxy = (long)gnx * gny;
+  int kMax = c->nz / stride;
+  for (int p = 0; p < c->nP; p++) {
+    if (c->pflag[p] != 0) continue;
+    double rx = c->px[p] - c->cx, ry = c->py[p] - c->cy, rz = c->pz[p] - c->cz;
+    double rl = sqrt(rx * rx + ry * ry + rz * rz);
+    if (!(rl != 0)) rl = 1e-9;
+    double dpt = (c->oceanR - rl) / shell, wgt = (dpt - 0.06) * 4.0;
+    if (wgt <= 0) continue;
+    if (wgt > 1) wgt = 1;
+    double qa = c->px[p] * invS, qb = c->py[p] * invS, qc = c->pz[p] * invS;
+    int i0 = (int)qa, j0 = (int)qb, k0 = (int)qc;
+    if (i0 > gnx - 2) i0 = gnx - 2; else if (i0 < 0) i0 = 0;
+    if (j0 > gny - 2) j0 = gny - 2; else if (j0 < 0) j0 = 0;
+    if (k0 > kMax) k0 = kMax; else if (k0 < 0) k0 = 0;
+    double ta = clampd(qa - i0, 0, 1), tb = clampd(qb - j0, 0, 1), tc = clampd(qc - k0, 0, 1);
+    long g0 = ((long)k0 * gny + j0) * gnx + i0;
+    double a0 = 1 - ta, a1 = ta, b0 = 1 - tb, b1 = tb, c0 = 1 - tc, c1 = tc, U[3];
+    for (int d = 0; d < 3; d++) {
+      const float *f = cf + d * gN;
+      U[d] = (f[g0] * a0 + f[g0 + 1] * a1) * b0 * c0 + (f[g0 + gnx] * a0 + f[g0 + gnx + 1] * a1) * b1 * c0 +
+             (f[g0 + gxy] * a0 + f[g0 + gxy + 1] * a1) * b0 * c1 +
+             (f[g0 + gxy + gnx] * a0 + f[g0 + gxy + gnx + 1] * a1) * b1 * c1;
+    }
+    double ir = 1 / rl, Ur = (U[0] * rx + U[1] * ry + U[2] * rz) * ir;
+    double Tx = (U[0] - rx * ir * Ur) * wgt, Ty = (U[1] - ry * ir * Ur) * wgt, Tz = (U[2] - rz * ir * Ur) * wgt;
+    double vr = ((double)c->pvx[p] * rx + (double)c->pvy[p] * ry + (double)c->pvz[p] * rz) * ir;
+    double vx = c->pvx[p] - rx * ir * vr, vy = c->pvy[p] - ry * ir * vr, vz = c->pvz[p] - rz * ir * vr;
+    double ex = relax * (Tx - vx), ey = relax * (Ty - vy), ez = relax * (Tz - vz);
+    double e2 = ex * ex + ey * ey + ez * ez;
+    if (e2 > 0.15 * 0.15) { double es = 0.15 / sqrt(e2); ex *= es; ey *= es; ez *= es; }
+    c->pvx[p] = (oc_real)(c->pvx[p] + ex);
+    c->pvy[p] = (oc_real)(c->pvy[p] + ey);
+    c->pvz[p] = (oc_real)(c->pvz[p] + ez);
+  }
+  free(cf);
+}
+
+/* (a * b) >>> 0 with JavaScript's double product */
+static uint32_t js_mul(uint32_t a, uint32_t b) {
+  double pr = (double)a * (double)b;
+  return (uint32_t)fmod(pr, 4294967296.0);
+}
+
+static uint32_t spin_hash(uint32_t a, uint32_t b, uint32_t cc) {
+  /* _spinHash, solver.js:3145-3150 */
+  uint32_t s = a ^ js_mul(b, 0x9e3779b9u) ^ js_mul(cc, 0x85ebca6bu) ^ js_mul(0x51ab3c77u, 0xc2b2ae35u);
+  s = (uint32_t)fmod((double)s * 1664525 + 1013904223, 4294967296.0);
+  s ^= s >> 13;
+  s = js_mul(s, 0x5bd1e995u);
+  s ^= s >> 15;
+  return s;
+}
+
+/* _updatePhaseChanges, solver.js:3152-3281 (sphere world) */
+void oc_update_phase(oc_ctx *c, double dt) {
+  int nP = c->nP, nx = c->nx, ny = c->ny, nz = c->nz;
+  double dx = c->dx;
+  int *cnt = (int *)calloc(c->nC, sizeof(int));
+  for (int p = 0; p < nP; p++) {
+    if (c->pflag[p] != 2) continue;
+    int i = trunc_clamp(c->px[p] / dx, nx), j = trunc_clamp(c->py[p] / dx, ny), k = trunc_clamp(c->pz[p] / dx, nz);
+    cnt[((long)k * ny + j) * nx + i]++;
+  }
+  double cloudT = c->cloudT, cloudP = c->cloudP;
+  double rainT = c->rainT < cloudT - 0.02 ? c->rainT : cloudT - 0.02;
+  if (rainT < 0) rainT = 0;
+  double snowT = c->snowT < rainT - 0.02 ? c->snowT : rainT - 0.02;
+  if (snowT < 0) snowT = 0;
+  double meltT = c->meltT > snowT * 1.05 ? c->meltT : snowT * 1.05;
+  const uint8_t *type = c->cellType;
+  double cloudBase = c->oceanR + 0.3 * c->atmosphereH;
+  long nxy = (long)nx * ny;
+  for (int p = 0; p < nP; p++) {
+    int f = c->pflag[p];
+    int i = trunc_clamp(c->px[p] / dx, nx), j = trunc_clamp(c->py[p] / dx, ny), k = trunc_clamp(c->pz[p] / dx, nz);
+    long cB = ((long)k * ny + j) * nx + i;
+    if (f == 0) {
+      if (c->pT[p] <= snowT && c->pT[p] > 1e-3) {
+        int atSurf = (k > 0 && type[cB - nxy] != FLUID) || (k < nz - 1 && type[cB + nxy] != FLUID) ||
+                     (j > 0 && type[cB - nx] != FLUID) || (j < ny - 1 && type[cB + nx] != FLUID) ||
+                     (i > 0 && type[cB - 1] != FLUID) || (i < nx - 1 && type[cB + 1] != FLUID);
+        if (atSurf) { c->pflag[p] = 5; c->pvx[p] = c->pvy[p] = c->pvz[p] = 0; c->pAir[p] = 1; }
+      }
+      continue;
+    }
+    if (f == 5) {
+      if (c->pT[p] >= meltT) {
+        c->pflag[p] = type[cB] == FLUID ? 0 : 4;
+        c->pAir[p] = 0;
+        if (c->pWx) c->pWx[p] = c->pWy[p] = c->pWz[p] = 0;
+      }
+      continue;
+    }
+    double T = c->pT[p];
+    if (T <= 1e-3) continue;
+    if (f == 2) {
+      if (T <= rainT && c->pAir[p] >= c->rainLiftAge) {
+        c->pflag[p] = 4; c->pAir[p] = 0;
+        c->pvx[p] = c->pvy[p] = c->pvz[p] = 0;
+        if (c->pWx) c->pWx[p] = c->pWy[p] = c->pWz[p] = 0;
+        continue;
+      }
+      if (T > cloudT) continue;
+      double rx = c->px[p] - c->cx, ry = c->py[p] - c->cy, rz = c->pz[p] - c->cz;
+      if (sqrt(rx * rx + ry * ry + rz * rz) < cloudBase) continue;
+      int ii0 = i > 0 ? i - 1 : 0, ii1 = i < nx - 1 ? i + 1 : nx - 1;
+      int jj0 = j > 0 ? j - 1 : 0, jj1 = j < ny - 1 ? j + 1 : ny - 1;
+      int kk0 = k > 0 ? k - 1 : 0, kk1 = k < nz - 1 ? k + 1 : nz - 1;
+      double sum = 0;
+      for (int kk = kk0; kk <= kk1; kk++)
+        for (int jj = jj0; jj <= jj1; jj++)
+          for (int ii = ii0; ii <= ii1; ii++) sum += cnt[((long)kk * ny + jj) * nx + ii];
+      if (sum / 27 >= cloudP) { c->pflag[p] = 3; c->pAir[p] = 0; }
+      continue;
+    }
+    if (f == 3) {
+      if (T >= cloudT + 0.05) c->pflag[p] = 2;
+      continue;
+    }
+    if (T <= snowT) {
+      c->pflag[p] = 5; c->pvx[p] = c->pvy[p] = c->pvz[p] = 0; c->pAir[p] = 0;
+      if (c->pWx) c->pWx[p] = c->pWy[p] = c->pWz[p] = 0;
+    }
+  }
+  if (c->spinOn && c->pWx && dt > 0) {
+    uint32_t tick = (uint32_t)(int)(c->simTime * 60), sd = 0x51ab3c77u;
+    double damp = 1 - 0.12 * dt;
+    if (damp < 0) damp = 0;
+    for (int p = 0; p < nP; p++) {
+      int f = c->pflag[p];
+      if (f != 2 && f != 3) continue;
+      if (f == 2 && c->pAir[p] <= 1e-6 && c->pWx[p] == 0 && c->pWy[p] == 0 && c->pWz[p] == 0) {
+        double h1 = spin_hash(sd, (uint32_t)p, tick) / 4294967296.0;
+        double h2 = spin_hash(sd + 1, (uint32_t)p, tick) / 4294967296.0;
+        double h3 = spin_hash(sd + 2, (uint32_t)p, tick) / 4294967296.0;
+        double zz = 2 * h2 - 1, rr = sqrt(1 - zz * zz > 0 ? 1 - zz * zz : 0), th = 6.283185306 * h1;
+        double Tp = c->pT[p];
+        if (!(Tp > 0.05)) Tp = 0.05; else if (Tp > 1.15) Tp = 1.15;
+        double wSpd = 3.0 * sqrt(Tp) * (0.5 + h3);
+        c->pWx[p] = (oc_real)(rr * cos(th) * wSpd); c->pWy[p] = (oc_real)(zz * wSpd); c->pWz[p] = (oc_real)(rr * sin(th) * wSpd);
+      }
+      double wm = sqrt((double)c->pWx[p] * c->pWx[p] + (double)c->pWy[p] * c->pWy[p] + (double)c->pWz[p] * c->pWz[p]);
+      if (wm > 1e-6 && c->pPh) c->pPh[p] = (oc_real)(c->pPh[p] + wm * dt);
+      c->pWx[p] = (oc_real)(c->pWx[p] * damp); c->pWy[p] = (oc_real)(c->pWy[p] * damp); c->pWz[p] = (oc_real)(c->pWz[p] * damp);
+    }
+  }
+  free(cnt);
 }
 
 /* ------------------------------------------------------------- surface */

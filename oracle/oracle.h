@@ -64,6 +64,12 @@ void oc_advect(oc_ctx *c, double dt);
 void oc_substep(oc_ctx *c, double dt);
 double oc_dt_law(oc_ctx *c);
 
+/* frame tick */
+void oc_update_heat(oc_ctx *c, double dt);
+void oc_update_evaporation(oc_ctx *c, double dt);
+void oc_update_currents(oc_ctx *c, double dt);
+void oc_update_phase(oc_ctx *c, double dt);
+
 /* samplers and terrain predicates */
 double oc_sample_face(oc_ctx *c, const oc_real *arr, int comp, double x, double y, double z);
 void oc_sample_vel3(oc_ctx *c, double x, double y, double z, double *out);

@@ -52,8 +52,8 @@ Writer.prototype.save = function (file) {
 
 function copy(a, n) { var c = new a.constructor(n === undefined ? a.length : n); c.set(n === undefined ? a : a.subarray(0, n)); return c; }
 
-var PART_F = ['px', 'py', 'pz', 'pvx', 'pvy', 'pvz', 'pT', 'pAir'];
-var PART_U8 = ['pflag', 'pcool'];
+var PART_F = ['px', 'py', 'pz', 'pvx', 'pvy', 'pvz', 'pT', 'pAir', 'pLight', 'pWx', 'pWy', 'pWz', 'pPh'];
+var PART_U8 = ['pflag', 'pcool', 'pDepth'];
 
 function dumpWorld(w, s) {
   var t = s.terrain;
@@ -213,4 +213,55 @@ function seededVel(s, seed, amp) {
   stagedSubstep(w, s, dtLaw(s));
   w.meta.trajectory = rec;
   w.save('micro_16.wsg.gz');
+})();
+
+// ---------------------------------------------------------------- scene C
+// Full physics at 32^3: sun, evaporation, currents, vorticity, thermal tick
+// and the phase pass; temperatures from a 270-300 K latitude profile on the
+// reference's normalised scale (T = 0.105 + (T_K - 273.15) / 60).
+(function sceneC() {
+  var n = 32, W = 5.1, dx = W / n;
+  var s = new FluidSolver({ nx: n, ny: n, nz: n, dx: dx, targetParticles: 10000, mode: 'sphere',
+    coreR: 0.25 * W, terrainSeed: 20260116, sunActivity: 0.6, evapIntensity: 1.5, vorticity: 0.05,
+    currents: 0.3 });
+  s.resetWater(0.4 * W - 0.25 * W);
+  s.pic = 0.05; s.omega = 1.7; s.pressureIters = 40; s.gravity = 9.8; s.maxSpeed = 3.2;
+  s.sunPos = [9, 4, 4];
+  for (var p = 0; p < s.nP; p++) {
+    var rx = s.px[p] - s.cx, ry = s.py[p] - s.cy, rz = s.pz[p] - s.cz;
+    var r = Math.sqrt(rx * rx + ry * ry + rz * rz) || 1e-9, sl = ry / r;
+    var TK = 270 + 30 * (1 - sl * sl);
+    s.pT[p] = 0.105 + (TK - 273.15) / 60;
+  }
+  var w = new Writer();
+  w.meta.scene = 'thermal_32';
+  w.meta.fill = params(s);
+  s._rasterizeStatic();
+  dumpWorld(w, s);
+  w.add('terrain_solid_full', copy(s.terrain.solid));
+  for (var f = 0; f < 40; f++) { s.step(1 / 60); s.splatDensity(); }
+  var tick = { dtTick: 1 / 25, simTime: s._simTime, sunActivity: s.sunActivity, evapIntensity: s.evapIntensity,
+    heatK: s.heatK, Tamb: s.Tamb, shadeCool: s.shadeCool, snowT: s.snowT, meltT: s.meltT, rainT: s.rainT,
+    cloudT: s.cloudT, cloudP: s.cloudP, rainLiftAge: s.rainLiftAge, currents: s.currents, sunPos: s.sunPos,
+    Rlo: s.terrain.Rlo, Rhi: s.terrain.Rhi, evS_before: s._evS };
+  dumpParticles(w, s, 't0_');
+  w.add('t0_dens', copy(s.dens));
+  w.add('t0_cellType', copy(s.cellType));
+  dumpGrid(w, s, 't0_', ['u', 'v', 'w']);
+  s._updateHeat(1 / 25);
+  dumpParticles(w, s, 't1_');
+  s._updateEvaporation(1 / 25);
+  tick.evapCount = s.evapCount; tick.evS_after = s._evS;
+  dumpParticles(w, s, 't2_');
+  s._updateCurrents(1 / 25);
+  dumpParticles(w, s, 't3_');
+  s._updatePhaseChanges(1 / 60);
+  dumpParticles(w, s, 't4_');
+  w.meta.tick = tick;
+  var nv = 0;
+  for (p = 0; p < s.nP; p++) if (s.pflag[p] === 2 || s.pflag[p] === 3) nv++;
+  w.meta.vapor = nv;
+  w.meta.rng_before = { evS: s._evS, cfS: s._cfS, bfS: s._bfS };
+  stagedSubstep(w, s, dtLaw(s));
+  w.save('thermal_32.wsg.gz');
 })();

@@ -10,7 +10,7 @@ sys.path.insert(0, os.path.join(ROOT, "watersim-v1_amd"))
 
 import watersim_hip  # noqa: E402
 
-from scenes import PART_KEYS  # noqa: E402
+from scenes import PART_KEYS, EXTRA_KEYS, tick_params  # noqa: E402
 
 
 def gpu_from_golden(g, tag="in_", capacity=None):
@@ -28,12 +28,20 @@ def gpu_from_golden(g, tag="in_", capacity=None):
     s.upload("gw", g["gw"])
     s.upload("terrainR", g["terrain_R"])
     s.set(nP=n)
-    for k in PART_KEYS:
+    for k in PART_KEYS + EXTRA_KEYS:
         if tag + k in g:
             s.upload(k, g[tag + k])
     if tag == "in_":
         s.upload("q", g["in_q"])
         s.set(gRamp=sp["gRamp"])
+    tp = tick_params(g)
+    if tp:
+        s.upload("terrainSolid", g["terrain_solid"])
+        s.set(**tp)
+        rb = g.meta["rng_before"]
+        s.set_rng(rb["cfS"], rb["bfS"], rb["evS"])
+        if any(g[tag + "pflag"] >= 2) if tag + "pflag" in g else False:
+            s.set(hasVapor=1)
     return s
 
 

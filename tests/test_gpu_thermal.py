@@ -1,0 +1,66 @@
+"""GPU frame tick against the reference on the thermal scene: heat
+(exposure, relaxation, conduction, convection), evaporation with the
+reference's _evS stream, currents and the phase pass, each stage from the
+reference's input state. Flags, counts and stream states exact; float
+fields within 1e-5 (FP32 kernels with expf/logf/cosf vs double)."""
+import numpy as np
+import pytest
+
+import golden
+from gpu_scenes import gpu_from_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def load(tag):
+    g = golden.load("thermal_32")
+    s = gpu_from_golden(g, tag=tag)
+    s.upload("cellType", g["t0_cellType"])
+    s.upload("dens", g["t0_dens"])
+    for k in "uvw":
+        s.upload(k, g["t0_" + k])
+    s.set_rng(g.meta["rng_before"]["cfS"], g.meta["rng_before"]["bfS"], g.meta["tick"]["evS_before"])
+    return g, s
+
+
+def close(s, g, tag, keys, tol=1e-5):
+    for k in keys:
+        a = s.download(k).astype(np.float64)
+        b = g[tag + k].astype(np.float64)
+        e = np.abs(a - b).max()
+        print("  %-10s max |err| %.3e" % (tag + k, e))
+        assert e <= tol, (tag + k, e)
+
+
+def test_heat_tick():
+    g, s = load("t0_")
+    s.stage("heat", g.meta["tick"]["dtTick"])
+    assert np.array_equal(s.download("pDepth"), g["t1_pDepth"])
+    close(s, g, "t1_", ["pT", "pLight", "pvx", "pvy", "pvz"])
+    s.close()
+
+
+def test_evaporation_stream():
+    g, s = load("t1_")
+    s.stage("evaporation", g.meta["tick"]["dtTick"])
+    print("  evaporated gpu %d ref %d" % (s.get("evapCount"), g.meta["tick"]["evapCount"]))
+    assert int(s.get("evapCount")) == g.meta["tick"]["evapCount"]
+    assert int(s.get_rng()[2]) == g.meta["tick"]["evS_after"]
+    assert np.array_equal(s.download("pflag"), g["t2_pflag"])
+    close(s, g, "t2_", ["pvx", "pvy", "pvz", "pAir"])
+    s.close()
+
+
+def test_currents():
+    g, s = load("t2_")
+    s.stage("currents", g.meta["tick"]["dtTick"])
+    close(s, g, "t3_", ["pvx", "pvy", "pvz"])
+    s.close()
+
+
+def test_phase_changes():
+    g, s = load("t3_")
+    s.stage("phase", 1 / 60)
+    assert np.array_equal(s.download("pflag"), g["t4_pflag"])
+    close(s, g, "t4_", ["pvx", "pvy", "pvz", "pAir", "pWx", "pWy", "pWz", "pPh"])
+    s.close()

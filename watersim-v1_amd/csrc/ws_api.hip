@@ -7,11 +7,9 @@
 #include <cmath>
 #include <cstring>
 
-#include "ws_kernels.h"
+#include "ws_frame.h"
 
 using namespace ws;
-
-struct ws_ctx : public ws::Ctx {};
 
 namespace {
 
@@ -440,6 +438,9 @@ bool field_of(ws_ctx *c, const char *n, Field &f) {
   PF("pT", c->pT, float) PF("pAir", c->pAir, float)
   PF("pflag", c->pflag, uint8_t) PF("pcool", c->pcool, uint8_t)
   PF("advU", c->advU, float) PF("advV", c->advV, float) PF("advW", c->advW, float)
+  PF("pLight", c->pLight, float) PF("pDepth", c->pDepth, uint8_t)
+  PF("pWx", c->pWx, float) PF("pWy", c->pWy, float) PF("pWz", c->pWz, float) PF("pPh", c->pPh, float)
+  GF("terrainSolid", c->tsolid, uint8_t, (long)c->tn * c->tn * c->tn)
   GF("cellType", c->cellType, uint8_t, c->nC) GF("staticTypes", c->staticTypes, uint8_t, c->nC)
   GF("q", c->q, float, c->nC) GF("rhs", c->rhs, float, c->nC)
   GF("u", c->vel, float, c->uN) GF("v", c->vel + c->uN, float, c->vN) GF("w", c->vel + c->uN + c->vN, float, c->wN)
@@ -507,6 +508,9 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
       (r = dalloc(c, &c->pcool, cap)) || (r = dalloc(c, &c->pcell, cap)) || (r = dalloc(c, &c->prank, cap)) ||
       (r = dalloc(c, &c->sortedIdx, cap)) || (r = dalloc(c, &c->vneed, 4 * cap)) ||
       (r = dalloc(c, &c->vscr, 4 * cap)) || (r = dalloc(c, &c->rng, 4)) || (r = dalloc(c, &c->ds, 1)) ||
+      (r = dalloc(c, &c->pLight, cap)) || (r = dalloc(c, &c->pDepth, cap)) || (r = dalloc(c, &c->pWx, cap)) ||
+      (r = dalloc(c, &c->pWy, cap)) || (r = dalloc(c, &c->pWz, cap)) || (r = dalloc(c, &c->pPh, cap)) ||
+      (r = dalloc(c, &c->evBuf, 7 * cap + 4)) || (r = dalloc(c, &c->jmp, 128)) ||
       (r = dalloc(c, &c->mtCount, c->nC + 1))) {
     ws_destroy(c);
     return r;
@@ -532,6 +536,18 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
   hipMemcpy(c->ds, &init, sizeof(init), hipMemcpyHostToDevice);
   uint32_t rng[4] = {0x2f6e2b1u, 0x7f4a7c15u, 0x51ab3c77u, 0u};  // _cfS, _bfS, _evS (solver.js:151-153)
   hipMemcpy(c->rng, rng, sizeof(rng), hipMemcpyHostToDevice);
+  {
+    // affine maps of 1..64 LCG steps: s -> A s + C (mod 2^32)
+    uint32_t J[128];
+    uint32_t A = 1u, C = 0u;
+    for (int n = 0; n < 64; n++) {
+      A = A * 1664525u;
+      C = C * 1664525u + 1013904223u;
+      J[n] = A;
+      J[64 + n] = C;
+    }
+    hipMemcpy(c->jmp, J, sizeof(J), hipMemcpyHostToDevice);
+  }
   // extrapolation tile schedule, grouped by tile diagonal I+J+K
   {
     int dims[3][3] = {{nx + 1, ny, nz}, {nx, ny + 1, nz}, {nx, ny, nz + 1}};
@@ -569,7 +585,8 @@ void ws_destroy(ws_ctx *c) {
                   c->cellCount, c->cellStart, c->vort, c->vel, c->velO, c->wgt, c->scr, c->scr2, c->valid, c->dens,
                   c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pT, c->pAir, c->advU, c->advV, c->advW, c->pflag,
                   c->pcool, c->pcell, c->prank, c->sortedIdx, c->vneed, c->vscr, c->rng, c->ds, c->partials,
-                  c->scanTmp, c->exTiles, c->mtCount, c->mtPos, c->mtNrm, c->mtOff};
+                  c->scanTmp, c->exTiles, c->mtCount, c->mtPos, c->mtNrm, c->mtOff, c->tsolid, c->pDepth,
+                  c->pLight, c->pWx, c->pWy, c->pWz, c->pPh, c->evBuf, c->jmp, c->currField};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->hs) hipHostFree(c->hs);
@@ -588,8 +605,10 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
   SF(cfl) SF(pic) SF(gravity) SF(viscosity) SF(omega) SF(maxSpeed) SF(vorticity) SF(coreR) SF(oceanR)
   SF(atmosphereH) SF(spacing) SF(particleVolume) SF(iso) SF(substeps) SF(rainT) SF(cloudT) SF(rainLiftAge)
   SF(pressureTol) SF(dv) SF(sunX) SF(sunY) SF(sunZ) SF(dtFrameBase)
+  SF(heatK) SF(Tamb) SF(shadeCool) SF(sunActivity) SF(evapIntensity) SF(snowT) SF(meltT) SF(cloudP)
+  SF(currents) SF(Rlo) SF(Rhi) SF(simTime) SF(heatAcc)
   SI(pressureIters) SI(maxPressureIters) SI(residualEvery) SI(ceilReflect) SI(hasSun) SI(measureResidual)
-  SI(adaptiveChecks) SI(sorTiled)
+  SI(adaptiveChecks) SI(sorTiled) SI(spinOn)
 #undef SF
 #undef SI
   if (!strcmp(n, "gRamp")) { c->gRamp = v; return push_ramp(c); }
@@ -630,6 +649,8 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   GF(atmosphereH) GF(spacing) GF(particleVolume) GF(iso) GF(substeps) GF(pressureTol) GF(pressureIters)
   GF(maxPressureIters) GF(nP) GF(capacity) GF(residual) GF(domainR) GF(dx) GF(tn) GF(dv)
   GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal)
+  GF(heatK) GF(Tamb) GF(shadeCool) GF(sunActivity) GF(evapIntensity) GF(snowT) GF(meltT) GF(cloudP) GF(currents)
+  GF(simTime) GF(heatAcc) GF(evapCount) GF(cloudT) GF(rainT) GF(rainLiftAge)
 #undef GF
   if (!strcmp(n, "pressureIters_done")) { *v = c->itersDone; return WS_OK; }
   if (!strcmp(n, "rngCfS") || !strcmp(n, "rngBfS") || !strcmp(n, "rngEvS")) {
@@ -653,6 +674,14 @@ int ws_upload(ws_ctx *c, const char *name, const void *host, size_t nbytes) {
     int r = dalloc(c, &c->terrainR, n);
     if (r) return r;
   }
+  if (!strcmp(name, "terrainSolid")) {
+    long n = (long)c->tn * c->tn * c->tn;
+    if (n <= 0 || nbytes != (size_t)n) { c->err = "terrainSolid: set tn first; size must be tn^3 bytes"; return WS_EINVAL; }
+    if (c->tsolid) hipFree(c->tsolid);
+    c->tsolid = nullptr;
+    int r = dalloc(c, &c->tsolid, n);
+    if (r) return r;
+  }
   if (!strcmp(name, "rng")) {
     if (nbytes != 12) return WS_EINVAL;
     WS_HIP(hipMemcpy(c->rng, host, 12, hipMemcpyHostToDevice));
@@ -667,6 +696,7 @@ int ws_upload(ws_ctx *c, const char *name, const void *host, size_t nbytes) {
   WS_HIP(hipStreamSynchronize(c->stream));
   if (!strcmp(name, "staticTypes")) c->haveStatic = true;
   if (!strcmp(name, "terrainR")) c->haveTerrain = true;
+  if (!strcmp(name, "terrainSolid")) c->haveSolid = 1;
   if (!strcmp(name, "pflag")) {
     const uint8_t *fl = (const uint8_t *)host;
     for (long i = 0; i < f.count; i++)
@@ -713,8 +743,19 @@ int ws_run_substeps(ws_ctx *c, int n) {
 
 int ws_step_frame(ws_ctx *c, double dtFrame, int *n_sub) {
   if (!c || !(dtFrame > 0)) return WS_EINVAL;
-  // _stepSync's substep loop, solver.js:2682-2695
   int r;
+  if ((r = need_world(c))) return r;
+  // _stepSync, solver.js:2670-2700: the 25 Hz thermal / evaporation /
+  // currents tick, the CFL substep loop, then the frame-end phase pass
+  c->simTime += dtFrame;
+  c->heatAcc += dtFrame;
+  if (c->heatAcc >= 1.0 / 25) {
+    double dtTick = c->heatAcc < 1.0 / 12 ? c->heatAcc : 1.0 / 12;
+    if ((r = frame_heat(c, dtTick))) return r;
+    if (c->sunActivity > 0 && (r = frame_evaporation(c, dtTick))) return r;
+    if (c->currents > 0 && (r = frame_currents(c, dtTick))) return r;
+    c->heatAcc = 0;
+  }
   if ((r = pull_scalars(c))) return r;
   double remaining = dtFrame, base = dtFrame / (c->substeps > 1 ? c->substeps : 1);
   int n = 0;
@@ -729,6 +770,8 @@ int ws_step_frame(ws_ctx *c, double dtFrame, int *n_sub) {
     remaining -= dt;
     n++;
   }
+  if ((r = frame_phase(c, dtFrame))) return r;
+  WS_HIP(hipStreamSynchronize(c->stream));
   if (n_sub) *n_sub = n;
   return WS_OK;
 }
@@ -776,6 +819,10 @@ int ws_stage(ws_ctx *c, int stage, double dt) {
       r = st_particles(c, 1);
       break;
     case WS_STAGE_ADVECT: r = st_particles(c, 2); break;
+    case WS_STAGE_HEAT: r = frame_heat(c, dt); break;
+    case WS_STAGE_EVAPORATION: r = frame_evaporation(c, dt); break;
+    case WS_STAGE_CURRENTS: r = frame_currents(c, dt); break;
+    case WS_STAGE_PHASE: r = frame_phase(c, dt); break;
     default: c->err = "unknown stage"; return WS_EINVAL;
   }
   if (r) return r;

@@ -6,11 +6,9 @@
 #include <cmath>
 #include <cstring>
 
-#include "ws_kernels.h"
+#include "ws_frame.h"
 
 using namespace ws;
-
-struct ws_ctx : public ws::Ctx {};
 
 namespace ws {
 

@@ -1,0 +1,13 @@
+// The context type shared by the ABI translation units, and the frame-level
+// passes (thermal tick, evaporation, currents, phase) they call.
+#pragma once
+#include "ws_kernels.h"
+
+struct ws_ctx : public ws::Ctx {};
+
+namespace ws {
+int frame_heat(ws_ctx *c, double dt);
+int frame_evaporation(ws_ctx *c, double dt);
+int frame_currents(ws_ctx *c, double dt);
+int frame_phase(ws_ctx *c, double dt);
+}  // namespace ws

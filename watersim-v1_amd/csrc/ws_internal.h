@@ -72,6 +72,17 @@ struct Ctx {
   int measureResidual = 0, ceilReflect = 0, hasSun = 0, adaptiveChecks = 1;
   double sunX = 0, sunY = 0, sunZ = 0, dtFrameBase = 0;
   bool hasVapor = false;
+  // frame tick: thermal, evaporation, currents, phase (solver.js:137-193, 2670-2700)
+  double heatK = 0.6, Tamb = 0.32, shadeCool = 0.035, sunActivity = 0, evapIntensity = 0.05;
+  double snowT = 0.10, meltT = 0.105, cloudP = 0.35, currents = 0, Rlo = 0, Rhi = 0;
+  double simTime = 0, heatAcc = 0;
+  int spinOn = 0, evapCount = 0, haveSolid = 0;
+  uint8_t *tsolid = nullptr, *pDepth = nullptr;
+  float *pLight = nullptr, *pWx = nullptr, *pWy = nullptr, *pWz = nullptr, *pPh = nullptr;
+  unsigned *evBuf = nullptr;    // 6 x capacity: cand, off, thresh, list, thrC, evList | evState
+  uint32_t *jmp = nullptr;      // 2 x 64 jump-ahead coefficients of the LCG
+  float *currField = nullptr;   // 3 x lattice
+  long currN = 0;
   // live timers (HIP events on the context stream), enabled by "timing"
   int timing = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};

@@ -13,6 +13,27 @@ struct Par {
   int ceilReflect;
 };
 
+// thermal tick, evaporation, currents and phase changes
+struct ThermalPar {
+  float dt, sunX, sunY, sunZ, sunPow, shadeCool, Tamb, alpha;
+  float Rlo2, Rhi2, Rhi, coreR2, inv_dv;
+  int hasSun, hasTerrain, tn, conduct;
+};
+struct EvapPar {
+  float dt, rate, freezeT;
+};
+struct CurrPar {
+  int gnx, gny, gnz, kMax;
+  float sx, invS, t, amp, relax, oceanR, shell;
+  float drift[3];
+  float wave[3][3][5];  // octave, potential component: kx, ky, kz, A, phase
+};
+struct PhasePar {
+  float snowT, meltT, rainT, cloudT, cloudP, rainLiftAge, cloudBase, dt, damp;
+  uint32_t tick;
+  int spinOn;
+};
+
 __global__ void k_begin_substep(DevScalars *s, float dt_host, int use_law, float cfl_dx, float dt_base,
                                 float gravity);
 __global__ void k_end_substep(DevScalars *s);
@@ -76,5 +97,35 @@ __global__ void k_mt_volume(int nx, int ny, int nz, float dx, const float *dens,
                             unsigned *cnt);
 __global__ void k_sum_f64(const double *part, int n, double *out);
 __global__ void k_kinetic(int nP, const float *pvx, const float *pvy, const float *pvz, double *part);
+
+__global__ void k_heat_particle(Geo g, ThermalPar T, int nP, const float *px, const float *py, const float *pz,
+                                float *pT, const uint8_t *pflag, float *pLight, uint8_t *pDepth, const uint8_t *ct,
+                                const float *dens, const uint8_t *tsolid, const float *thr, unsigned long long *csum,
+                                unsigned *ccnt);
+__global__ void k_heat_cells(Geo g, float alpha, const unsigned long long *csum, const unsigned *ccnt, float *mean,
+                             float *dT);
+__global__ void k_heat_apply(Geo g, ThermalPar T, int nP, const float *px, const float *py, const float *pz,
+                             float *pT, const uint8_t *pflag, float *pvx, float *pvy, float *pvz, const float *mean,
+                             const float *dT, const float *thr);
+__global__ void k_evap_candidates(Geo g, EvapPar E, int nP, const float *px, const float *py, const float *pz,
+                                  const float *pT, const uint8_t *pflag, const uint8_t *ct, const float *thr,
+                                  unsigned *cand, unsigned *thresh);
+__global__ void k_evap_compact(int nP, const unsigned *cand, const unsigned *off, const unsigned *thresh,
+                               unsigned *list, unsigned *thrC);
+__global__ void k_evap_chooser(const unsigned *list, const unsigned *thrC, const unsigned *off, const unsigned *cand,
+                               int nP, const uint32_t *jmpA, const uint32_t *jmpC, uint32_t *rng, unsigned *evList,
+                               unsigned *evState, unsigned *evCount);
+__global__ void k_evap_apply(Geo g, EvapPar E, const unsigned *evList, const unsigned *evState,
+                             const unsigned *evCount, float *px, float *py, float *pz, float *pvx, float *pvy,
+                             float *pvz, const float *pT, uint8_t *pflag, float *pAir, const float *vel,
+                             const float *thr);
+__global__ void k_currents_lattice(CurrPar Q, float *cf);
+__global__ void k_currents_relax(Geo g, CurrPar Q, int nP, const float *px, const float *py, const float *pz,
+                                 float *pvx, float *pvy, float *pvz, const uint8_t *pflag, const float *cf);
+__global__ void k_steam_count(Geo g, int nP, const float *px, const float *py, const float *pz, const uint8_t *pflag,
+                              const float *thr, unsigned *cnt);
+__global__ void k_phase(Geo g, PhasePar P, int nP, const float *px, const float *py, const float *pz, float *pvx,
+                        float *pvy, float *pvz, const float *pT, uint8_t *pflag, float *pAir, float *pWx, float *pWy,
+                        float *pWz, float *pPh, const uint8_t *ct, const unsigned *steam, const float *thr);
 
 }  // namespace ws

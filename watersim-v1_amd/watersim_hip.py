@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "_build", "libwatersim_hip.so")
 STAGES = {
     "clear": 0, "rasterize": 1, "p2g": 2, "bc": 3, "snapshot": 4, "viscosity": 5, "gravity": 6,
     "pressure": 7, "project": 8, "vorticity": 9, "clamp": 10, "extrapolate": 11, "g2p": 12, "advect": 13,
+    "heat": 14, "evaporation": 15, "currents": 16, "phase": 17,
 }
 
 # exported entry points, as declared in include/watersim.h
@@ -84,8 +85,9 @@ def load(path=LIB_PATH):
     return L
 
 
-PART_F32 = ["px", "py", "pz", "pvx", "pvy", "pvz", "pT", "pAir", "advU", "advV", "advW"]
-PART_U8 = ["pflag", "pcool"]
+PART_F32 = ["px", "py", "pz", "pvx", "pvy", "pvz", "pT", "pAir", "advU", "advV", "advW", "pLight", "pWx", "pWy",
+            "pWz", "pPh"]
+PART_U8 = ["pflag", "pcool", "pDepth"]
 CELL_U8 = ["cellType", "staticTypes"]
 CELL_F32 = ["q", "rhs"]
 FACE_F32 = ["u", "v", "w", "uO", "vO", "wO", "uW", "vW", "wW", "gu", "gv", "gw"]
@@ -153,6 +155,9 @@ class Solver:
         if name == "terrainR":
             tn = int(self.get("tn"))
             return tn ** 3, np.float32
+        if name == "terrainSolid":
+            tn = int(self.get("tn"))
+            return tn ** 3, np.uint8
         raise KeyError(name)
 
     def upload(self, name, arr):
@@ -273,6 +278,8 @@ def solver_for_world(world, capacity=None, Tamb=0.32):
     for k in ("gu", "gv", "gw"):
         s.upload(k, world[k])
     s.upload("terrainR", world["R"])
+    s.upload("terrainSolid", world["solid"])
+    s.set(Rlo=world["Rlo"], Rhi=world["Rhi"])
     s.set(nP=nP)
     s.upload("px", world["px"])
     s.upload("py", world["py"])
