@@ -1,15 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark: FLIP substeps/s of the planet substep pipeline on one MI355X.
 
-Workload (default "cfg3"): 128^3 MAC grid, ~2.0M FLIP particles, seeded
-voxel planet (core 0.25 W, sea level 0.4 W, W = 5.1 m), FLIP 0.95,
-red-black SOR (omega 1.7) to relative residual 1e-4, vorticity confinement
-0.05, dt from the CFL law. Synthetic input built on the host by the
+Workload (default "cfg3", full physics): 128^3 MAC grid, ~2.0M FLIP
+particles, seeded voxel planet (core 0.25 W, sea level 0.4 W, W = 5.1 m),
+FLIP 0.95, red-black SOR (omega 1.7) to relative residual 1e-4, vorticity
+confinement 0.05, dt from the CFL law, driven frame by frame (1/60 s frames)
+as FluidSolver.step does: the 25 Hz thermal / evaporation tick, the substep
+loop and the frame-end phase pass. Particle temperatures start from a
+latitude profile of 270-300 K; the sun circles the planet once per 2000
+substeps at 23 degrees declination. Synthetic input built on the host by the
 library's world builder; all state is resident in HBM before timing.
 
 A "step" is one substep. Prints one JSON line (diagnostics on stderr).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -23,11 +28,22 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 W_DOMAIN = 5.1
 CONFIGS = {
-    # name: grid n, particle target, seed, vorticity, SOR mode
-    "cfg1": dict(n=32, target=10000, seed=20260114, vorticity=0.0, tol=0.0, iters=40),
-    "cfg2": dict(n=64, target=232000, seed=20260115, vorticity=0.05, tol=1e-4, iters=0),
-    "cfg3": dict(n=128, target=2117000, seed=20260116, vorticity=0.05, tol=1e-4, iters=0),
+    # name: grid n, particle target, seed, vorticity, SOR mode, frame-driven full physics
+    "cfg1": dict(n=32, target=10000, seed=20260114, vorticity=0.0, tol=0.0, iters=40, physics=False),
+    "cfg2": dict(n=64, target=232000, seed=20260115, vorticity=0.05, tol=1e-4, iters=0, physics=False),
+    "cfg3": dict(n=128, target=2117000, seed=20260116, vorticity=0.05, tol=1e-4, iters=0, physics=True),
 }
+FRAME = 1.0 / 60
+SUN_PERIOD = 2000      # substeps per revolution of the sun
+SUN_DECL = 23.0        # degrees between the sun's path and the planet's equator
+SUN_ACTIVITY = 0.6     # js/solver.js:142 "default daylight"
+
+
+def kelvin_to_model(tk):
+    """Reference temperatures are normalised (0 .. 1.15, ambient 0.32, snow
+    0.10, melt 0.105; js/solver.js:158-190). 273.15 K maps to the melt point
+    and 60 K span one unit: T = 0.105 + (T_K - 273.15) / 60."""
+    return 0.105 + (tk - 273.15) / 60.0
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 
 
@@ -44,6 +60,15 @@ def make_solver(cfg):
     log("world %d^3: %d particles, land %.3f, built in %.1fs" % (n, world["nP"], world["landFrac"], time.time() - t))
     s = watersim_hip.solver_for_world(world)
     s.set(pic=0.05, omega=1.7, gravity=9.8, maxSpeed=3.2, vorticity=cfg["vorticity"], cfl=3.0)
+    if cfg["physics"]:
+        # latitude profile 270 K (poles) .. 300 K (equator) about the y axis
+        c = 0.5 * W_DOMAIN
+        x, y, z = (s.download(k)[:world["nP"]].astype(np.float64) - c for k in ("px", "py", "pz"))
+        r = np.sqrt(x * x + y * y + z * z)
+        sin_lat = np.where(r > 0, y / np.maximum(r, 1e-12), 0.0)
+        s.upload("pT", kelvin_to_model(270.0 + 30.0 * (1.0 - sin_lat ** 2)).astype(np.float32))
+        s.set(sunActivity=SUN_ACTIVITY, hasSun=1)
+        set_sun(s, world, 0)
     if cfg["tol"] > 0:
         s.set(pressureTol=cfg["tol"], maxPressureIters=4000, residualEvery=8)
     else:
@@ -51,9 +76,47 @@ def make_solver(cfg):
     return s, world
 
 
-def cpu_baseline(s, world, cfg, budget_s=12.0, max_substeps=6):
-    """The oracle (single-threaded C port of the reference substep, reference
-    lexicographic SOR order) on the same state, for a bounded sample."""
+def set_sun(s, world, substeps_done):
+    """Sun position in the planet frame: distance 3.4 oceanR from the centre
+    (js/scene.js:1248), hour angle advancing one turn per SUN_PERIOD substeps."""
+    c = 0.5 * W_DOMAIN
+    th = 2 * np.pi * substeps_done / SUN_PERIOD
+    de = np.radians(SUN_DECL)
+    d = 3.4 * world["oceanR"]
+    s.set(sunX=c + d * np.cos(de) * np.cos(th), sunY=c + d * np.sin(de), sunZ=c + d * np.cos(de) * np.sin(th))
+
+
+def advance(s, world, cfg, n, done):
+    """n substeps: frames through the full step (tick, substeps, phase pass)
+    when the workload has the thermal cycle, else back-to-back substeps with
+    the CFL law evaluated on the device. Returns the substeps executed."""
+    if not cfg["physics"]:
+        s.run_substeps(n)
+        return n
+    k = 0
+    while k < n:
+        set_sun(s, world, done + k)
+        k += s.step(FRAME)
+    return k
+
+
+def algorithmic_bytes(cfg, n, nP, sweeps, residuals, ticks):
+    """Bytes the reference dataflow moves (each array once per stage at its
+    reference precision): plain substep 154 N + 69 F + 16 C, 13 C per SOR
+    sweep, 9 C per residual evaluation, vorticity 12 F + 82 C, and per
+    thermal tick 115 N + 77 C + 4 D plus evaporation 17 N + C."""
+    C, F, D = n ** 3, 3 * n * n * (n + 1), (n + 1) ** 3
+    b = 154 * nP + 69 * F + 16 * C
+    if cfg["vorticity"] > 0:
+        b += 12 * F + 82 * C
+    return b, 13 * C * sweeps + 9 * C * residuals + ticks * (132 * nP + 78 * C + 4 * D)
+
+
+def cpu_baseline(s, world, cfg, budget_s=15.0, max_substeps=8):
+    """The oracle (single-threaded C port of the reference: reference
+    arithmetic, lexicographic SOR) on the same state, for a bounded sample.
+    With the thermal cycle on it runs whole frames as js/solver.js:2670-2700
+    does: the tick when 1/25 s has accumulated, the substeps, the phase pass."""
     import cpu_oracle as oracle
     n = cfg["n"]
     st = oracle.OracleState({"nx": n, "ny": n, "nz": n, "dx": W_DOMAIN / n, "coreR": world["coreR"],
@@ -61,7 +124,8 @@ def cpu_baseline(s, world, cfg, budget_s=12.0, max_substeps=6):
                              "particleVolume": world["particleVolume"], "dv": world["dv"], "tn": world["tn"]})
     st.set_static(world["solid"], world["R"], world["staticTypes"], world["gu"], world["gv"], world["gw"])
     nP = int(s.get("nP"))
-    parts = {k: s.download(k) for k in ["px", "py", "pz", "pvx", "pvy", "pvz", "pT", "pAir", "pflag", "pcool"]}
+    parts = {k: s.download(k) for k in ["px", "py", "pz", "pvx", "pvy", "pvz", "pT", "pAir", "pflag", "pcool",
+                                        "pLight", "pDepth", "pWx", "pWy", "pWz", "pPh"]}
     st.set_particles(nP, **parts)
     st.arrays["q"][:] = s.download("q")
     st.set_params(pic=0.05, omega=1.7, gravity=9.8, maxSpeed=3.2, vorticity=cfg["vorticity"], cfl=3.0,
@@ -71,80 +135,134 @@ def cpu_baseline(s, world, cfg, budget_s=12.0, max_substeps=6):
     else:
         st.set_params(pressureIters=cfg["iters"])
     done, t0 = 0, time.perf_counter()
-    while done < max_substeps and time.perf_counter() - t0 < budget_s:
-        st.call("oc_substep", st.call("oc_dt_law"))
-        done += 1
+    if cfg["physics"]:
+        st.set_params(hasSun=1, sunActivity=SUN_ACTIVITY, Rlo=world["Rlo"], Rhi=world["Rhi"])
+        sim, acc = s.get("simTime"), s.get("heatAcc")
+        while done < max_substeps and time.perf_counter() - t0 < budget_s:
+            c = 0.5 * W_DOMAIN
+            th, de, d = 2 * np.pi * done / SUN_PERIOD, np.radians(SUN_DECL), 3.4 * world["oceanR"]
+            st.set_params(sunX=c + d * np.cos(de) * np.cos(th), sunY=c + d * np.sin(de),
+                          sunZ=c + d * np.cos(de) * np.sin(th))
+            sim += FRAME
+            acc += FRAME
+            st.set_params(simTime=sim)
+            if acc >= 1.0 / 25:
+                tick = min(acc, 1.0 / 12)
+                st.call("oc_update_heat", ctypes.c_double(tick))
+                st.call("oc_update_evaporation", ctypes.c_double(tick))
+                acc = 0.0
+            rem, k = FRAME, 0
+            while rem > 1e-6 and k < 10:
+                dt = min(FRAME, st.call("oc_dt_law"))
+                st.call("oc_substep", ctypes.c_double(dt))
+                rem -= dt
+                k += 1
+            done += k
+            st.call("oc_update_phase", ctypes.c_double(FRAME))
+        what = "frames (tick, substeps, phase pass)"
+    else:
+        while done < max_substeps and time.perf_counter() - t0 < budget_s:
+            st.call("oc_substep", st.call("oc_dt_law"))
+            done += 1
+        what = "substeps"
     el = time.perf_counter() - t0
     return {"value": done / el, "unit": "substeps/s", "cores": 1, "kind": "port",
-            "sample": "%d substeps of the same %d^3 / %d-particle state (oracle: reference arithmetic, "
-                      "lexicographic SOR to %g), single thread" % (done, n, nP, cfg["tol"] or cfg["iters"])}
+            "sample": "%d substeps run as %s from the same %d^3 / %d-particle state (oracle: reference "
+                      "arithmetic, lexicographic SOR %s), single thread"
+                      % (done, what, n, nP, "to residual %g" % cfg["tol"] if cfg["tol"] else "%d sweeps" % cfg["iters"])}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=1000, help="timed substeps")
+    ap.add_argument("--warmup", type=int, default=20, help="untimed substeps")
     ap.add_argument("--config", default="cfg3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB five times and exit: the "
+                         "reference kernel whose FETCH_SIZE / WRITE_SIZE counters scale the HBM traffic")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        nbytes = a.copy_calibration << 20
+        src = torch.ones(nbytes // 4, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 5}), flush=True)
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: the planet substep runs on a single MI355X; "
-              "--gpus %d is not supported (see DESIGN.md for how it would shard)" % a.gpus, file=sys.stderr)
+              "--gpus %d is not supported (a planet would shard by k-slabs, batched planets by planet)"
+              % a.gpus, file=sys.stderr)
         sys.exit(2)
     cfg = CONFIGS[a.config]
     s, world = make_solver(cfg)
     n, nP = cfg["n"], int(s.get("nP"))
     log("warmup %d substeps" % a.warmup)
-    for i in range(a.warmup):
-        s.run_substeps(1)
+    done = advance(s, world, cfg, a.warmup, 0)
     s.sync()
     s.set(timing=1)
     log("timing %d substeps" % a.steps)
     s.sync()
     t0 = time.perf_counter()
-    s.run_substeps(a.steps)
+    steps = advance(s, world, cfg, a.steps, done)
     s.sync()
     el = time.perf_counter() - t0
-    sor_ms, sor_n = s.get("sorMs"), s.get("sorLaunches")
-    p_ms, p_n, sweeps = s.get("pressureMs"), s.get("pressureSolves"), s.get("sweepsTotal")
+    g = {k: s.get(k) for k in ("sorMs", "sorLaunches", "pressureMs", "pressureSolves", "sweepsTotal",
+                               "residualsTotal", "ticksTotal", "sorTiled")}
     s.set(timing=0)
-    sps = a.steps / el
+    sps = steps / el
     C = n ** 3
-    # dominant kernel: one colour of a red-black SOR sweep; algorithmic bytes
-    # 13 B per cell per full sweep (q read+write, rhs, type), half per launch
-    sor_avg_s = (sor_ms / 1e3) / max(sor_n, 1)
-    sor_bytes = 6.5 * C
-    achieved = sor_bytes / sor_avg_s / 1e9 if sor_n else 0.0
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "sor_traffic_%s.json" % a.config)
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
+    # dominant kernel: the SOR sweep. Algorithmic bytes 13 per cell per full
+    # red+black sweep (q read + write, rhs, cell type); the tiled kernel does
+    # a full sweep per launch, the per-colour kernel half of one.
+    tiled = g["sorTiled"] != 0
+    sor_avg_s = (g["sorMs"] / 1e3) / max(g["sorLaunches"], 1)
+    sor_bytes = (13.0 if tiled else 6.5) * C
+    achieved = sor_bytes / sor_avg_s / 1e9 if g["sorLaunches"] else 0.0
+    kname = "k_sor_tile" if tiled else "k_sor"
+    traffic = read_traffic(kname, a.config)
+    b_plain, b_iter = algorithmic_bytes(cfg, n, nP, g["sweepsTotal"], g["residualsTotal"], g["ticksTotal"])
+    b_total = b_plain * steps + b_iter
     ke = s.kinetic_energy()
-    log("substep %.3f ms, pressure %.3f ms/solve, %.1f sweeps/substep, SOR launch %.2f us, ke %.4g"
-        % (1e3 * el / a.steps, p_ms / max(p_n, 1), sweeps / a.steps, 1e6 * sor_avg_s, ke))
+    log("substep %.3f ms, pressure %.3f ms/solve, %.1f sweeps/substep, %d ticks, SOR launch %.2f us, ke %.4g"
+        % (1e3 * el / steps, g["pressureMs"] / max(g["pressureSolves"], 1), g["sweepsTotal"] / steps,
+           g["ticksTotal"], 1e6 * sor_avg_s, ke))
+    sor_txt = "to residual %g" % cfg["tol"] if cfg["tol"] else "%d sweeps" % cfg["iters"]
+    if cfg["physics"]:
+        wl = ("%s: %d^3 MAC grid, %d FLIP particles, red-black SOR %s, vorticity %g, full thermal cycle "
+              "(25 Hz heat + evaporation tick, frame-end phase pass, rotating sun, 270-300 K latitude profile), "
+              "1/60 s frames" % (a.config, n, nP, sor_txt, cfg["vorticity"]))
+    else:
+        wl = ("%s: %d^3 MAC grid, %d FLIP particles, red-black SOR %s, vorticity %g, thermal off, "
+              "back-to-back substeps" % (a.config, n, nP, sor_txt, cfg["vorticity"]))
     out = {
         "metric": "FLIP substeps/s at 128^3, 2M particles",
         "value": sps,
         "unit": "substeps/s",
         "n_gpus": 1,
-        "steps": a.steps,
+        "steps": steps,
         "warmup": a.warmup,
-        "ms_per_step": 1e3 * el / a.steps,
+        "ms_per_step": 1e3 * el / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded planet built by the library's world builder)",
-        "config": {"workload": "%s: %d^3 MAC grid, %d FLIP particles, SOR %s, vorticity %g; "
-                               "thermal tick not included" % (a.config, n, nP,
-                                                              "to residual %g" % cfg["tol"] if cfg["tol"]
-                                                              else "%d sweeps" % cfg["iters"], cfg["vorticity"]),
-                   "grid": n, "particles": nP, "particle_updates_per_s": sps * nP,
-                   "pressure_ms_per_solve": p_ms / max(p_n, 1), "sor_sweeps_per_substep": sweeps / a.steps},
+        "config": {"workload": wl, "grid": n, "particles": nP, "particle_updates_per_s": sps * nP,
+                   "pressure_ms_per_solve": g["pressureMs"] / max(g["pressureSolves"], 1),
+                   "sor_sweeps_per_substep": g["sweepsTotal"] / steps,
+                   "residual_checks_per_substep": g["residualsTotal"] / steps,
+                   "thermal_ticks": int(g["ticksTotal"]),
+                   "substep_algorithmic_GBps": b_total / el / 1e9,
+                   "substep_hbm_frac": b_total / el / HBM_PEAK},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / (HBM_PEAK / 1e9), "traffic": traffic,
-                     "kernel": "k_sor (one colour of a red-black sweep)",
+                     "kernel": kname + (" (one full red-black sweep, LDS tile)" if tiled else " (one colour)"),
                      "avg_launch_us": 1e6 * sor_avg_s, "algorithmic_bytes_per_launch": sor_bytes},
     }
     if not a.no_cpu_baseline:
@@ -152,6 +270,19 @@ def main():
         out["cpu_baseline"] = cpu_baseline(s, world, cfg)
     s.close()
     print(json.dumps(out), flush=True)
+
+
+def read_traffic(kernel, config):
+    """HBM bytes per launch of the kernel from the committed counter summary
+    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, copy-kernel calibrated): the
+    line 'traffic_bytes_per_launch: <bytes>' of profiles/<kernel>_traffic_<config>.md."""
+    path = os.path.join(ROOT, "profiles", "%s_traffic_%s.md" % (kernel, config))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.strip().startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 if __name__ == "__main__":

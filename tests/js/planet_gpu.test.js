@@ -1,0 +1,237 @@
+/*
+ * The reference's ocean-planet invariants (test/solver.test.js:149-410),
+ * driven through the Node FluidSolver facade on the GPU with the same
+ * scene (44^3 grid, 38k particles, core 1.5 m, ocean depth 0.45 m) and the
+ * same pass thresholds. Sections can be picked on the command line:
+ *   node planet_gpu.test.js fill settle heat shade currents whirl long
+ */
+'use strict';
+var path = require('path');
+var FluidSolver = require(path.join(__dirname, '..', '..', 'watersim-v1_amd', 'js', 'solver.js'));
+
+var failures = 0;
+function check(name, ok, detail) {
+  console.log((ok ? '  PASS ' : '  FAIL ') + name + (detail !== undefined ? '  [' + detail + ']' : ''));
+  if (!ok) failures++;
+}
+var want = process.argv.slice(2);
+function on(section) { return want.length === 0 || want.indexOf(section) >= 0; }
+
+var CORE = 1.5, DEPTH = 0.45, SEA = CORE + DEPTH;
+var SIDE = 2 * (SEA + 0.6), N = 44, DX = SIDE / N;
+
+function planet(extra) {
+  var o = { nx: N, ny: N, nz: N, dx: DX, targetParticles: 38000, mode: 'sphere', coreR: CORE };
+  for (var k in extra || {}) o[k] = extra[k];
+  var s = new FluidSolver(o);
+  s.pressureIters = 20;
+  s.resetWater(DEPTH);
+  return s;
+}
+function frames(s, n, each) {
+  for (var f = 0; f < n; f++) {
+    s.step(1 / 60);
+    if (each) each(f);
+  }
+}
+function radius(s, p) {
+  var ex = s.px[p] - s.cx, ey = s.py[p] - s.cy, ez = s.pz[p] - s.cz;
+  return Math.sqrt(ex * ex + ey * ey + ez * ez);
+}
+function allFinite(s) {
+  var a = [s.px, s.py, s.pz, s.pvx, s.pvy, s.pvz];
+  for (var p = 0; p < s.nP; p++) for (var c = 0; c < 6; c++) if (!isFinite(a[c][p])) return false;
+  return true;
+}
+// sea-level percentile, liquid v_rms, escapes, buried particles
+function stats(s) {
+  var px = s.px, py = s.py, pz = s.pz, vx = s.pvx, vy = s.pvy, vz = s.pvz, fl = s.pflag;
+  var rs = new Float64Array(s.nP), v2 = 0, nF = 0, out = 0, inRock = 0;
+  var lim = Math.max(s.domainR, s.oceanR + (s.atmosphereH || 0.35));
+  for (var p = 0; p < s.nP; p++) {
+    var ex = px[p] - s.cx, ey = py[p] - s.cy, ez = pz[p] - s.cz, r = Math.sqrt(ex * ex + ey * ey + ez * ez);
+    rs[p] = r;
+    if (fl[p] === 0) { v2 += vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p]; nF++; }
+    if (r > lim) out++;
+    if (s._rockCellAt(px[p], py[p], pz[p]) && r < s.terrainRadiusAt(px[p], py[p], pz[p]) - 0.25 * s.dx) inRock++;
+  }
+  rs.sort();
+  return { p95: rs[Math.floor(rs.length * 0.95)], vrms: Math.sqrt(v2 / Math.max(1, nF)), out: out, inRock: inRock };
+}
+// deepest basin direction by a golden-angle scan
+function deepest(s) {
+  var best = null, bd = -1;
+  for (var i = 0; i < 400; i++) {
+    var a = i * 2.399963, u = 1 - 2 * (i + 0.5) / 400, q = Math.sqrt(1 - u * u);
+    var R = s.terrainRadiusAt(s.cx + q * Math.cos(a) * s.terrain.Rhi, s.cy + u * s.terrain.Rhi,
+      s.cz + q * Math.sin(a) * s.terrain.Rhi);
+    if (s.oceanR - R > bd) { bd = s.oceanR - R; best = [q * Math.cos(a), u, q * Math.sin(a), R]; }
+  }
+  return best;
+}
+function splash(s, d) {
+  var rr = d[3] + (s.oceanR - d[3]) * 0.5;
+  s.applyImpulseSphere(s.cx + d[0] * rr, s.cy + d[1] * rr, s.cz + d[2] * rr, 0.65, d[0] * 3.2, d[1] * 3.2, d[2] * 3.2);
+}
+
+var t0 = Date.now();
+if (on('fill') || on('settle')) {
+  console.log('planet: fill, settle, splash');
+  var sp = planet({ sunActivity: 0.6 });
+  var emb = 0, vox = 0, deep = 0, maxR = 0;
+  for (var p = 0; p < sp.nP; p++) {
+    if (sp._rockCellAt(sp.px[p], sp.py[p], sp.pz[p])) emb++;
+    if (sp._voxelRockAt(sp.px[p], sp.py[p], sp.pz[p])) vox++;
+    var r = radius(sp, p);
+    if (sp.terrainRadiusAt(sp.px[p], sp.py[p], sp.pz[p]) - r > 1.5 * sp.dx) deep++;
+    if (r > maxR) maxR = r;
+  }
+  check('fill: no particle spawns inside the planet', emb === 0 && vox === 0 && deep === 0,
+    'rockCell=' + emb + ' voxel=' + vox + ' deep=' + deep + ' of ' + sp.nP);
+  check('terrain: 25-30% land', sp.landFrac > 0.24 && sp.landFrac < 0.31, (100 * sp.landFrac).toFixed(1) + '%');
+  check('ocean fills every basin to sea level', maxR < SEA + 0.02, maxR.toFixed(3));
+  if (on('settle')) {
+    frames(sp, 360);
+    var st6 = stats(sp);
+    check('ocean settles calm (v_rms < 0.05)', st6.vrms < 0.05, st6.vrms.toFixed(4));
+    check('sea level holds (|p95 - R| < 0.06)', Math.abs(st6.p95 - SEA) < 0.06, st6.p95.toFixed(3));
+    check('nothing leaves the domain', st6.out === 0, st6.out);
+    check('no water in the rock after settling', st6.inRock === 0, st6.inRock);
+    var dir = deepest(sp);
+    splash(sp, dir);
+    frames(sp, 480);
+    var st14 = stats(sp);
+    check('splash contained, sea level re-forms (< 0.09)', Math.abs(st14.p95 - SEA) < 0.09, st14.p95.toFixed(3));
+    check('no water buried after the splash', st14.inRock === 0, st14.inRock);
+    check('finite after the splash', allFinite(sp));
+    sp.dispose();
+  }
+}
+
+if (on('heat')) {
+  console.log('planet: sun, shadow, conduction, convection');
+  var sh = planet({ sunActivity: 0.6 });
+  frames(sh, 360);
+  var d = deepest(sh);
+  sh.sunPos = [sh.cx + d[0] * 50, sh.cy + d[1] * 50, sh.cz + d[2] * 50];
+  frames(sh, 300);
+  var census = function () {
+    var dayT = 0, nD = 0, ngT = 0, nN = 0, hotR = 0, nH = 0, coldR = 0, nC = 0, mx = 0, T = sh.pT;
+    for (var p2 = 0; p2 < sh.nP; p2++) {
+      var ex = sh.px[p2] - sh.cx, ey = sh.py[p2] - sh.cy, ez = sh.pz[p2] - sh.cz;
+      var rr = Math.sqrt(ex * ex + ey * ey + ez * ez), lat = (ex * d[0] + ey * d[1] + ez * d[2]) / rr;
+      if (rr > SEA - 0.18) {
+        if (lat > 0.3) { dayT += T[p2]; nD++; }
+        if (lat < -0.3) { ngT += T[p2]; nN++; }
+      }
+      if (sh.oceanR - sh.terrainRadiusAt(sh.px[p2], sh.py[p2], sh.pz[p2]) > 0.35) {
+        if (T[p2] > 0.5) { hotR += rr; nH++; } else if (T[p2] < 0.4) { coldR += rr; nC++; }
+      }
+      if (T[p2] > mx) mx = T[p2];
+    }
+    return { day: dayT / nD, night: ngT / nN, hotR: hotR / nH, coldR: coldR / nC, nH: nH, nC: nC, max: mx };
+  };
+  var h1 = census();
+  check('sun heats the subsolar surface', h1.day > h1.night + 0.2, h1.day.toFixed(2) + ' vs ' + h1.night.toFixed(2));
+  check('planet shadow keeps the night side cold', h1.night < 0.5, h1.night.toFixed(2));
+  check('hot water rises above cold water', h1.hotR > h1.coldR + 0.03,
+    h1.hotR.toFixed(3) + ' (n=' + h1.nH + ') vs ' + h1.coldR.toFixed(3) + ' (n=' + h1.nC + ')');
+  sh.sunPos = null;
+  frames(sh, 300);
+  var h2 = census();
+  check('heat conducts and dissipates after sundown', h2.max < h1.max - 0.2, h1.max.toFixed(2) + ' -> ' + h2.max.toFixed(2));
+  sh.dispose();
+}
+
+if (on('shade')) {
+  console.log('planet: shade cooling');
+  var ss = planet({});
+  ss.sunPos = null;
+  var T0 = ss.pT;
+  for (var q = 0; q < ss.nP; q++) T0[q] = 1.0;
+  frames(ss, 180);
+  var mT = 0, T1 = ss.pT;
+  for (q = 0; q < ss.nP; q++) mT += T1[q];
+  mT /= ss.nP;
+  check('dark water cools faster (mean T 0.78 .. 0.89 after 3 s)', mT > 0.78 && mT < 0.89, mT.toFixed(3));
+  ss.dispose();
+}
+
+if (on('currents')) {
+  console.log('planet: subsurface currents');
+  var sc = planet({ vorticity: 0.55, currents: 0.8 });
+  frames(sc, 360);
+  var mid = 0, nM = 0, avr = 0, nA = 0;
+  for (var p3 = 0; p3 < sc.nP; p3++) {
+    if (sc.pflag[p3] === 1) continue;
+    var ex = sc.px[p3] - sc.cx, ey = sc.py[p3] - sc.cy, ez = sc.pz[p3] - sc.cz;
+    var r2 = Math.sqrt(ex * ex + ey * ey + ez * ez) || 1e-9;
+    var vx = sc.pvx[p3], vy = sc.pvy[p3], vz = sc.pvz[p3];
+    var vr = (vx * ex + vy * ey + vz * ez) / r2, vt2 = vx * vx + vy * vy + vz * vz - vr * vr;
+    var below = sc.oceanR - r2;
+    avr += Math.abs(vr); nA++;
+    if (below > 0.15 && below < 0.45) { mid += Math.sqrt(vt2 > 0 ? vt2 : 0); nM++; }
+  }
+  mid = nM ? mid / nM : 0; avr /= nA;
+  var stD = stats(sc);
+  check('streams flow through the ocean (mid v_tan > 1.8 cm/s)', mid > 0.018, mid.toFixed(3));
+  check('the drive is tangential', mid > avr, mid.toFixed(3) + ' vs ' + avr.toFixed(3));
+  check('currents keep the sea level and the rock dry', Math.abs(stD.p95 - SEA) < 0.08 && stD.inRock === 0,
+    'p95 ' + stD.p95.toFixed(3) + ' inRock ' + stD.inRock);
+  check('currents stay contained', stD.out === 0 && allFinite(sc), 'out ' + stD.out);
+  sc.dispose();
+}
+
+function enstrophy(s) {
+  var nx = s.nx, ny = s.ny, nz = s.nz, T = s.cellType, u = s.u, v = s.v, w = s.w, h = s.dx;
+  var sj = nx + 1, sk = ny * sj, s2 = 0;
+  for (var k = 1; k < nz - 1; k++) for (var j = 1; j < ny - 1; j++) for (var i = 1; i < nx - 1; i++) {
+    var c = k * sk + j * sj + i;
+    if (T[c] !== 1) continue;
+    var wx = (v[c + sj] - v[c - sj] - w[c + nx] + w[c - nx]) / (2 * h);
+    var wy = (w[c + 1] - w[c - 1] - u[c + sk] + u[c - sk]) / (2 * h);
+    var wz = (u[c + sj] - u[c - sj] - v[c + 1] + v[c - 1]) / (2 * h);
+    s2 += wx * wx + wy * wy + wz * wz;
+  }
+  return s2;
+}
+function maxSpeed(s) {
+  var m = 0;
+  for (var p = 0; p < s.nP; p++) {
+    var v2 = s.pvx[p] * s.pvx[p] + s.pvy[p] * s.pvy[p] + s.pvz[p] * s.pvz[p];
+    if (v2 > m) m = v2;
+  }
+  return Math.sqrt(m);
+}
+
+if (on('whirl')) {
+  console.log('planet: vorticity confinement keeps splash swirl');
+  var run = function (eps) {
+    var s = planet({ vorticity: eps, currents: 0 });
+    splash(s, deepest(s));
+    var m = {};
+    frames(s, 180, function (f) { if (f === 59) m.e1 = enstrophy(s); });
+    m.e3 = enstrophy(s);
+    m.peak = maxSpeed(s);
+    s.dispose();
+    return m;
+  };
+  var wOn = run(1.2), wOff = run(0);
+  check('confinement boosts splash vorticity (> 1.1x)', wOn.e1 > wOff.e1 * 1.1, wOn.e1.toFixed(0) + ' vs ' + wOff.e1.toFixed(0));
+  check('confined splash still decays', wOn.e3 < wOn.e1 * 0.75, (100 * wOn.e3 / wOn.e1).toFixed(0) + '%');
+  check('confined speeds bounded by the clamp', wOn.peak < 4.0, wOn.peak.toFixed(2));
+}
+
+if (on('long')) {
+  console.log('planet: shipped defaults for 20 s');
+  var sl = planet({ vorticity: 0.7, currents: 0.9 });
+  var keMax = 0;
+  frames(sl, 1200, function (f) { if (f > 120) keMax = Math.max(keMax, sl.kineticEnergy()); });
+  var keEnd = sl.kineticEnergy(), stC = stats(sl);
+  check('dynamics stay bounded for 20 s', keEnd < 7.0 && keMax < 9.0, 'KE ' + keEnd.toFixed(2) + ', max ' + keMax.toFixed(2));
+  check('the rock stays dry and nothing escapes', stC.inRock === 0 && stC.out === 0, 'inRock ' + stC.inRock + ' out ' + stC.out);
+  sl.dispose();
+}
+
+console.log((failures ? failures + ' FAILED' : 'ALL PLANET CHECKS PASSED') + ' in ' + ((Date.now() - t0) / 1000).toFixed(1) + ' s');
+process.exitCode = failures ? 1 : 0;

@@ -1,5 +1,7 @@
-"""Builds the watersim HIP library (gfx950) in-tree:
-watersim-v1_amd/_build/libwatersim_hip.so."""
+"""Builds the watersim HIP library (gfx950) in-tree,
+watersim-v1_amd/_build/libwatersim_hip.so, and the Node N-API addon over its
+C ABI, watersim-v1_amd/_build/watersim_napi.node (when Node's headers are
+present)."""
 import os
 import subprocess
 import sys
@@ -32,7 +34,25 @@ def build(verbose=False):
         raise RuntimeError("hipcc failed")
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", OUT] + objs
     subprocess.run(cmd, check=True)
+    build_addon()
     return OUT
+
+
+NODE_INC = "/usr/include/node"
+ADDON = os.path.join(HERE, "_build", "watersim_napi.node")
+
+
+def build_addon():
+    """The N-API addon: plain C against node_api.h, linked to the library
+    with an $ORIGIN rpath so the two travel together."""
+    if not os.path.exists(os.path.join(NODE_INC, "node_api.h")):
+        return None
+    cmd = ["gcc", "-O2", "-std=c11", "-shared", "-fPIC", "-Wall", "-Werror", "-DNAPI_VERSION=8",
+           "-DNODE_GYP_MODULE_NAME=watersim_napi", "-I" + NODE_INC, "-I" + os.path.join(HERE, "..", "include"),
+           os.path.join(HERE, "addon", "watersim_napi.c"), "-o", ADDON,
+           "-L" + os.path.join(HERE, "_build"), "-lwatersim_hip", "-Wl,-rpath,$ORIGIN"]
+    subprocess.run(cmd, check=True)
+    return ADDON
 
 
 if __name__ == "__main__":

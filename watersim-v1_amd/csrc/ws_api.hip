@@ -222,6 +222,7 @@ int sor_sweeps(ws_ctx *c, int n) {
 }
 
 int residual(ws_ctx *c, double *res) {
+  c->residualsTotal++;
   hipLaunchKernelGGL(k_residual, dim3(c->nPartials), dim3(256), 0, c->stream, c->geo, c->q, c->rhs, c->nbm, c->dg,
                      c->partials);
   hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, c->stream, c->partials, c->nPartials, c->ds);
@@ -624,7 +625,7 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
     for (int i = 0; i < 4 && c->timing; i++)
       if (!c->ev[i]) WS_HIP(hipEventCreate(&c->ev[i]));
     c->sorMs = c->pressureMs = 0;
-    c->sorLaunches = c->pressureSolves = c->sweepsTotal = 0;
+    c->sorLaunches = c->pressureSolves = c->sweepsTotal = c->residualsTotal = c->ticksTotal = 0;
     return WS_OK;
   }
   c->err = std::string("unknown scalar ") + n;
@@ -648,7 +649,7 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   GF(cfl) GF(pic) GF(gravity) GF(viscosity) GF(omega) GF(maxSpeed) GF(vorticity) GF(coreR) GF(oceanR)
   GF(atmosphereH) GF(spacing) GF(particleVolume) GF(iso) GF(substeps) GF(pressureTol) GF(pressureIters)
   GF(maxPressureIters) GF(nP) GF(capacity) GF(residual) GF(domainR) GF(dx) GF(tn) GF(dv)
-  GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal)
+  GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal) GF(residualsTotal) GF(ticksTotal) GF(sorTiled)
   GF(heatK) GF(Tamb) GF(shadeCool) GF(sunActivity) GF(evapIntensity) GF(snowT) GF(meltT) GF(cloudP) GF(currents)
   GF(simTime) GF(heatAcc) GF(evapCount) GF(cloudT) GF(rainT) GF(rainLiftAge)
 #undef GF
@@ -755,6 +756,7 @@ int ws_step_frame(ws_ctx *c, double dtFrame, int *n_sub) {
     if (c->sunActivity > 0 && (r = frame_evaporation(c, dtTick))) return r;
     if (c->currents > 0 && (r = frame_currents(c, dtTick))) return r;
     c->heatAcc = 0;
+    c->ticksTotal++;
   }
   if ((r = pull_scalars(c))) return r;
   double remaining = dtFrame, base = dtFrame / (c->substeps > 1 ? c->substeps : 1);

@@ -87,7 +87,7 @@ struct Ctx {
   int timing = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double sorMs = 0, pressureMs = 0;
-  long sorLaunches = 0, pressureSolves = 0, sweepsTotal = 0;
+  long sorLaunches = 0, pressureSolves = 0, sweepsTotal = 0, residualsTotal = 0, ticksTotal = 0;
   // results of the last calls
   double umax = 0, residual = 0;
   int itersDone = 0, airborne = 0, nFluid = 0;

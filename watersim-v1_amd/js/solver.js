@@ -1,0 +1,413 @@
+/*
+ * FluidSolver on MI355X: the construct / step / readback surface of the
+ * reference solver (js/solver.js:45-3924, ocean-planet mode) over the HIP
+ * library's C ABI, reached through the N-API addon in ../addon.
+ *
+ * The simulation state lives in device memory. The typed arrays callers
+ * know (px .. pPh, cellType, u/v/w, q, dens ...) are host mirrors:
+ *  - reading a field after a device call downloads it once;
+ *  - a field that was read may have been written, so it is uploaded before
+ *    the next device call;
+ *  - fields nobody touches never cross PCIe.
+ * Scalar tunables are plain properties pushed to the device before every
+ * call. There is no CPU path: without the addon or a GPU, construction of
+ * the world throws.
+ */
+'use strict';
+
+var path = require('path');
+
+var addon = null;
+function lib() {
+  if (!addon) {
+    var where = path.join(__dirname, '..', '_build', 'watersim_napi.node');
+    try {
+      addon = require(where);
+    } catch (e) {
+      throw new Error('watersim: the HIP addon is not built (' + where + '): ' + e.message);
+    }
+  }
+  return addon;
+}
+
+var AIR = 0, FLUID = 1, SOLID = 2;
+
+// ws_stage identifiers (include/watersim.h)
+var STAGE = {
+  clear: 0, rasterize: 1, p2g: 2, bc: 3, snapshot: 4, viscosity: 5, gravity: 6, pressure: 7,
+  project: 8, vorticity: 9, clamp: 10, extrapolate: 11, g2p: 12, advect: 13, heat: 14,
+  evaporation: 15, currents: 16, phase: 17
+};
+
+// scalars mirrored to the device before each call (js/solver.js:56-75, 137-193)
+var TUNABLES = ['cfl', 'pic', 'gravity', 'viscosity', 'pressureIters', 'omega', 'maxSpeed', 'vorticity',
+  'currents', 'shadeCool', 'substeps', 'heatK', 'Tamb', 'cloudT', 'cloudP', 'rainT', 'rainLiftAge', 'snowT',
+  'meltT', 'evapIntensity', 'sunActivity', 'atmosphereH', 'iso', 'pressureTol', 'maxPressureIters'];
+var CEIL = { linear: 0, quadratic: 1, exponential: 2 };
+
+// device fields: [name on the device, constructor, extent]
+var PARTICLE_FIELDS = {
+  px: Float32Array, py: Float32Array, pz: Float32Array, pvx: Float32Array, pvy: Float32Array,
+  pvz: Float32Array, pT: Float32Array, pAir: Float32Array, pLight: Float32Array, pWx: Float32Array,
+  pWy: Float32Array, pWz: Float32Array, pPh: Float32Array, pflag: Uint8Array, pcool: Uint8Array,
+  pDepth: Uint8Array
+};
+var GRID_FIELDS = {
+  cellType: [Uint8Array, 'C'], q: [Float32Array, 'C'], rhs: [Float32Array, 'C'],
+  u: [Float32Array, 'U'], v: [Float32Array, 'V'], w: [Float32Array, 'W'],
+  uW: [Float32Array, 'U'], vW: [Float32Array, 'V'], wW: [Float32Array, 'W'],
+  uO: [Float32Array, 'U'], vO: [Float32Array, 'V'], wO: [Float32Array, 'W'],
+  validU: [Uint8Array, 'U'], validV: [Uint8Array, 'V'], validW: [Uint8Array, 'W'],
+  dens: [Float32Array, 'D']
+};
+
+function FluidSolver(opts) {
+  opts = opts || {};
+  if (opts.mode !== 'sphere') {
+    throw new Error('watersim: only the ocean-planet world (mode: "sphere") is part of this build');
+  }
+  this.mode = 'sphere';
+  this.nx = opts.nx | 0; this.ny = opts.ny | 0; this.nz = opts.nz | 0;
+  this.dx = opts.dx;
+  // tunables with the reference defaults
+  this.substeps = 1; this.cfl = 3.0; this.pic = 0.15; this.gravity = 9.81; this.viscosity = 5e-4;
+  this.pressureIters = 20; this.omega = 1.55; this.maxSpeed = 3.2;
+  this.vorticity = opts.vorticity || 0;
+  this.currents = opts.currents || 0;
+  this.shadeCool = 0.035;
+  this.targetParticles = opts.targetParticles || 110000;
+  this.coreR = opts.coreR || 0;
+  this.oceanR = 0;
+  this.terrainSeed = (opts.terrainSeed || 20260114) >>> 0;
+  this.bumpiness = opts.bumpiness === undefined ? 1 : Math.max(0, Math.min(opts.bumpiness, 2.5));
+  this.sunActivity = opts.sunActivity !== undefined ? opts.sunActivity
+    : (opts.evaporation !== undefined ? opts.evaporation : 0);
+  this.atmosphereH = opts.atmosphereH || 0.35;
+  this.ceilReflect = opts.ceilReflect || 'linear';
+  this.sunPos = null;
+  this.heatK = 0.6; this.Tamb = 0.32;
+  this.cloudT = opts.cloudT !== undefined ? opts.cloudT : 0.34;
+  this.cloudP = opts.cloudP !== undefined ? opts.cloudP : 0.35;
+  this.rainT = opts.rainT !== undefined ? opts.rainT : 0.22;
+  this.rainLiftAge = opts.rainLiftAge !== undefined ? opts.rainLiftAge : 0.4;
+  this.snowT = opts.snowT !== undefined ? opts.snowT : 0.10;
+  this.meltT = opts.meltT !== undefined ? opts.meltT : this.snowT * 1.05;
+  this.evapIntensity = opts.evapIntensity !== undefined ? opts.evapIntensity : 0.05;
+  this.spinOn = false;
+  // solver-to-residual mode (0 = fixed pressureIters sweeps)
+  this.pressureTol = 0; this.maxPressureIters = 4000;
+
+  var nx = this.nx, ny = this.ny, nz = this.nz;
+  this.W = nx * this.dx; this.H = ny * this.dx; this.D = nz * this.dx;
+  this.cx = this.W / 2; this.cy = this.H / 2; this.cz = this.D / 2;
+  this.domainR = 0.5 * Math.min(this.W, this.H, this.D) - this.dx;
+  this.nCells = nx * ny * nz;
+  this.uN = (nx + 1) * ny * nz; this.vN = nx * (ny + 1) * nz; this.wN = nx * ny * (nz + 1);
+  this.spacing = this.dx * 0.5;
+  this.particleVolume = this.spacing * this.spacing * this.spacing;
+  this.iso = 1.8;
+  this.balls = [];
+  this.terrain = null;
+  this.capacity = 0;
+
+  this._h = null;                 // device context
+  this._nP = 0;
+  this._host = {};                // host mirrors
+  this._stale = {};               // device newer than the mirror
+  this._dirty = {};               // mirror handed out, upload before the next call
+  this._staticTypes = null;
+}
+
+// ---------------------------------------------------------------- mirrors
+FluidSolver.prototype._extent = function (kind) {
+  switch (kind) {
+    case 'C': return this.nCells;
+    case 'U': return this.uN;
+    case 'V': return this.vN;
+    case 'W': return this.wN;
+    default: return (this.nx + 1) * (this.ny + 1) * (this.nz + 1);
+  }
+};
+
+FluidSolver.prototype._mirror = function (name) {
+  var a = this._host[name];
+  if (!a) {
+    if (PARTICLE_FIELDS[name]) a = new PARTICLE_FIELDS[name](Math.max(1, this.capacity));
+    else a = new GRID_FIELDS[name][0](this._extent(GRID_FIELDS[name][1]));
+    this._host[name] = a;
+  }
+  return a;
+};
+
+FluidSolver.prototype._view = function (name) {
+  var a = this._mirror(name);
+  return PARTICLE_FIELDS[name] ? a.subarray(0, this._nP) : a;
+};
+
+// reading hands the caller the mirror: fetch it if the device is newer and
+// assume it may be written
+FluidSolver.prototype._touch = function (name) {
+  var a = this._mirror(name);
+  if (this._h && this._stale[name]) {
+    lib().download(this._h, name, this._view(name));
+    this._stale[name] = false;
+  }
+  if (this._h) this._dirty[name] = true;
+  return a;
+};
+
+FluidSolver.prototype._push = function () {
+  var L = lib(), h = this._h;
+  for (var i = 0; i < TUNABLES.length; i++) L.setF(h, TUNABLES[i], +this[TUNABLES[i]]);
+  L.setF(h, 'ceilReflect', CEIL[this.ceilReflect] || 0);
+  L.setF(h, 'spinOn', this.spinOn ? 1 : 0);
+  var s = this.sunPos;
+  L.setF(h, 'hasSun', s ? 1 : 0);
+  if (s) { L.setF(h, 'sunX', s[0]); L.setF(h, 'sunY', s[1]); L.setF(h, 'sunZ', s[2]); }
+  for (var name in this._dirty) {
+    if (!this._dirty[name]) continue;
+    L.upload(h, name, this._view(name));
+    this._dirty[name] = false;
+  }
+};
+
+FluidSolver.prototype._invalidate = function () {
+  for (var name in PARTICLE_FIELDS) this._stale[name] = true;
+  for (var g in GRID_FIELDS) this._stale[g] = true;
+  this._dirty = {};
+};
+
+FluidSolver.prototype._call = function (fn) {
+  if (!this._h) throw new Error('watersim: no world yet (call resetWater first)');
+  this._push();
+  var r = fn(lib(), this._h);
+  this._invalidate();
+  return r;
+};
+
+Object.keys(PARTICLE_FIELDS).concat(Object.keys(GRID_FIELDS)).forEach(function (name) {
+  Object.defineProperty(FluidSolver.prototype, name, {
+    get: function () { return this._touch(name); },
+    set: function (arr) {
+      var m = this._mirror(name);
+      m.set(arr.subarray ? arr.subarray(0, m.length) : arr);
+      if (this._h) { this._dirty[name] = true; this._stale[name] = false; }
+    }
+  });
+});
+
+// the reference caches grid samples as Float64Array (solver.js:199); the
+// device keeps them as float, so reads return float64 copies
+['U', 'V', 'W'].forEach(function (c) {
+  Object.defineProperty(FluidSolver.prototype, '_adv' + c, {
+    get: function () {
+      var f = new Float32Array(Math.max(1, this._nP));
+      if (this._h && this._nP) lib().download(this._h, 'adv' + c, f.subarray(0, this._nP));
+      return Float64Array.from(f);
+    }
+  });
+});
+
+Object.defineProperty(FluidSolver.prototype, 'nP', {
+  get: function () { return this._nP; },
+  set: function (n) {
+    n = n | 0;
+    if (n < 0 || n > this.capacity) throw new RangeError('nP exceeds the particle capacity ' + this.capacity);
+    if (this._h) {
+      // keep the fields of the particles that stay valid on the host side
+      for (var name in PARTICLE_FIELDS) if (this._stale[name]) this._touch(name);
+      lib().setF(this._h, 'nP', n);
+      for (var k in PARTICLE_FIELDS) this._dirty[k] = true;
+    }
+    this._nP = n;
+  }
+});
+
+function devScalar(name, dev) {
+  Object.defineProperty(FluidSolver.prototype, name, {
+    get: function () { return this._h ? lib().getF(this._h, dev) : 0; }
+  });
+}
+devScalar('umax', 'umax');
+devScalar('dtLast', 'dtLast');
+devScalar('airborneCount', 'airborneCount');
+devScalar('_simTime', 'simTime');
+devScalar('evapCount', 'evapCount');
+
+// ---------------------------------------------------------------- world
+// resetWater(depth), solver.js:611 / _resetWaterSphere :651-731: the voxel
+// planet, the ocean fill at rest, radial gravity and the iso calibration
+FluidSolver.prototype.resetWater = function (depth) {
+  var L = lib();
+  this.balls.length = 0;
+  var w = L.worldBuild({
+    nx: this.nx, ny: this.ny, nz: this.nz, dx: this.dx, coreR: this.coreR, depth: depth,
+    bumpiness: this.bumpiness, seed: this.terrainSeed, targetParticles: this.targetParticles
+  });
+  this.oceanR = w.oceanR;
+  this.spacing = w.spacing;
+  this.particleVolume = w.particleVolume;
+  this.landFrac = w.landFrac;
+  this.terrain = {
+    n: w.tn, dv: w.dv, solid: w.solid, R: w.R, Rlo: w.Rlo, Rhi: w.Rhi, landFrac: w.landFrac,
+    seed: this.terrainSeed
+  };
+  this._staticTypes = w.staticTypes;
+  if (this._h) { L.destroy(this._h); this._h = null; }
+  this.capacity = w.capacity;
+  this._host = {};
+  this._h = L.create(this.nx, this.ny, this.nz, this.dx, w.capacity);
+  var h = this._h;
+  L.setF(h, 'coreR', this.coreR); L.setF(h, 'oceanR', w.oceanR); L.setF(h, 'spacing', w.spacing);
+  L.setF(h, 'particleVolume', w.particleVolume); L.setF(h, 'dv', w.dv); L.setF(h, 'tn', w.tn);
+  L.setF(h, 'Rlo', w.Rlo); L.setF(h, 'Rhi', w.Rhi);
+  L.upload(h, 'staticTypes', w.staticTypes);
+  L.upload(h, 'gu', w.gravity.subarray(0, this.uN));
+  L.upload(h, 'gv', w.gravity.subarray(this.uN, this.uN + this.vN));
+  L.upload(h, 'gw', w.gravity.subarray(this.uN + this.vN));
+  L.upload(h, 'terrainR', w.R);
+  L.upload(h, 'terrainSolid', w.solid);
+  this._nP = w.nP;
+  L.setF(h, 'nP', w.nP);
+  this._stale = {}; this._dirty = {};
+  this._mirror('px').set(w.px); this._mirror('py').set(w.py); this._mirror('pz').set(w.pz);
+  this._mirror('pT').fill(this.Tamb);
+  for (var name in PARTICLE_FIELDS) L.upload(h, name, this._view(name));
+  L.setF(h, 'simTime', 0); L.setF(h, 'heatAcc', 0);
+  L.setF(h, 'gRamp', 0);                    // gravity eases in (solver.js:727)
+  this.splatDensity();
+  this.iso = this.autotuneIso();
+};
+
+// ---------------------------------------------------------------- dynamics
+FluidSolver.prototype.step = function (dtFrame) {
+  this.substepsLast = this._call(function (L, h) { return L.stepFrame(h, dtFrame); });
+};
+FluidSolver.prototype._substep = function (dt) { this._call(function (L, h) { L.substep(h, dt); }); };
+FluidSolver.prototype.runSubsteps = function (n) { this._call(function (L, h) { L.runSubsteps(h, n); }); };
+
+function stageMethod(id) {
+  return function (dt) {
+    var d = dt === undefined ? 0 : dt;
+    this._call(function (L, h) { L.stage(h, id, d); });
+  };
+}
+FluidSolver.prototype._rasterize = stageMethod(STAGE.rasterize);
+FluidSolver.prototype._p2g = stageMethod(STAGE.p2g);
+FluidSolver.prototype._applyBC = stageMethod(STAGE.bc);
+FluidSolver.prototype._viscosity = stageMethod(STAGE.viscosity);
+FluidSolver.prototype._applyPlanetGravity = stageMethod(STAGE.gravity);
+FluidSolver.prototype._pressureSolve = stageMethod(STAGE.pressure);
+FluidSolver.prototype._project = stageMethod(STAGE.project);
+FluidSolver.prototype._vorticityConfinement = stageMethod(STAGE.vorticity);
+FluidSolver.prototype._extrapolate = stageMethod(STAGE.extrapolate);
+FluidSolver.prototype._g2p = stageMethod(STAGE.g2p);
+FluidSolver.prototype._advect = stageMethod(STAGE.advect);
+FluidSolver.prototype._updateHeat = stageMethod(STAGE.heat);
+FluidSolver.prototype._updateEvaporation = stageMethod(STAGE.evaporation);
+FluidSolver.prototype._updateCurrents = stageMethod(STAGE.currents);
+FluidSolver.prototype._updatePhaseChanges = stageMethod(STAGE.phase);
+
+// ---------------------------------------------------------------- surface
+FluidSolver.prototype.splatDensity = function () {
+  this._call(function (L, h) { L.splatDensity(h); });
+};
+FluidSolver.prototype.autotuneIso = function () {
+  return this._call(function (L, h) { return L.autotuneIso(h); });
+};
+FluidSolver.prototype.kineticEnergy = function () {
+  return this._call(function (L, h) { return L.kineticEnergy(h); });
+};
+
+// ---------------------------------------------------------------- host helpers
+// terrainRadiusAt, solver.js:511-535: trilinear on the radius lattice
+FluidSolver.prototype.terrainRadiusAt = function (x, y, z) {
+  var t = this.terrain;
+  if (!t) return this.coreR;
+  var n = t.n, R = t.R;
+  var gx = x / t.dv - 0.5, gy = y / t.dv - 0.5, gz = z / t.dv - 0.5;
+  var i = Math.min(Math.max(gx | 0, 0), n - 2);
+  var j = Math.min(Math.max(gy | 0, 0), n - 2);
+  var k = Math.min(Math.max(gz | 0, 0), n - 2);
+  var fx = Math.min(Math.max(gx - i, 0), 1), fy = Math.min(Math.max(gy - j, 0), 1);
+  var fz = Math.min(Math.max(gz - k, 0), 1);
+  var b = (k * n + j) * n + i, sj = n, sk = n * n;
+  var x00 = R[b] + (R[b + 1] - R[b]) * fx;
+  var x10 = R[b + sj] + (R[b + sj + 1] - R[b + sj]) * fx;
+  var x01 = R[b + sk] + (R[b + sk + 1] - R[b + sk]) * fx;
+  var x11 = R[b + sk + sj] + (R[b + sk + sj + 1] - R[b + sk + sj]) * fx;
+  var y0 = x00 + (x10 - x00) * fy, y1 = x01 + (x11 - x01) * fy;
+  return y0 + (y1 - y0) * fz;
+};
+
+// _rockCellAt, solver.js:542-559: the static raster at the (clamped) cell
+FluidSolver.prototype._rockCellAt = function (x, y, z) {
+  if (!this._staticTypes) return false;
+  var dx = this.dx, nx = this.nx, ny = this.ny, nz = this.nz;
+  var i = Math.min(Math.max((x / dx) | 0, 1), nx - 2);
+  var j = Math.min(Math.max((y / dx) | 0, 1), ny - 2);
+  var k = Math.min(Math.max((z / dx) | 0, 1), nz - 2);
+  return this._staticTypes[(k * ny + j) * nx + i] === SOLID;
+};
+
+// _voxelRockAt, solver.js:565-574: the exact voxel
+FluidSolver.prototype._voxelRockAt = function (x, y, z) {
+  var t = this.terrain;
+  if (!t) return false;
+  var n = t.n;
+  var i = Math.min(Math.max((x / t.dv) | 0, 0), n - 1);
+  var j = Math.min(Math.max((y / t.dv) | 0, 0), n - 1);
+  var k = Math.min(Math.max((z / t.dv) | 0, 0), n - 1);
+  return t.solid[(k * n + j) * n + i] === 1;
+};
+
+// _onRockBelow, solver.js:581-587
+FluidSolver.prototype._onRockBelow = function (x, y, z) {
+  if (!this.terrain) return false;
+  var ex = x - this.cx, ey = y - this.cy, ez = z - this.cz;
+  var s = this.dx * 0.9 / (Math.sqrt(ex * ex + ey * ey + ez * ez) || 1e-9);
+  return this._rockCellAt(x - ex * s, y - ey * s, z - ez * s);
+};
+
+// applyImpulseSphere, solver.js:3852-3864: host-side edit of the particle
+// velocities (uploaded with the next device call)
+FluidSolver.prototype.applyImpulseSphere = function (cx, cy, cz, R, fx, fy, fz) {
+  var px = this.px, py = this.py, pz = this.pz, vx = this.pvx, vy = this.pvy, vz = this.pvz;
+  var fl = this.pflag, cool = this.pcool, R2 = R * R;
+  for (var p = 0; p < this._nP; p++) {
+    var ex = px[p] - cx, ey = py[p] - cy, ez = pz[p] - cz, d2 = ex * ex + ey * ey + ez * ez;
+    if (d2 > R2) continue;
+    var f = 1 - Math.sqrt(d2) / R;
+    vx[p] += fx * f; vy[p] += fy * f; vz[p] += fz * f;
+    if (fl[p] !== 0) fl[p] = 0;
+    cool[p] = 0;
+  }
+};
+
+// stirAt, solver.js:3895-3908
+FluidSolver.prototype.stirAt = function (cx, cy, cz, ux, uy, uz, R, dt) {
+  var px = this.px, py = this.py, pz = this.pz, vx = this.pvx, vy = this.pvy, vz = this.pvz;
+  var fl = this.pflag, R2 = R * R, k = Math.min(1, 14 * dt);
+  for (var p = 0; p < this._nP; p++) {
+    var ex = px[p] - cx, ey = py[p] - cy, ez = pz[p] - cz, d2 = ex * ex + ey * ey + ez * ez;
+    if (d2 > R2) continue;
+    var f = 1 - Math.sqrt(d2) / R;
+    vx[p] += (ux * 1.25 - vx[p]) * k * f;
+    vy[p] += (uy * 1.25 - vy[p]) * k * f;
+    vz[p] += (uz * 1.25 - vz[p]) * k * f;
+    fl[p] = 0;
+  }
+};
+
+FluidSolver.prototype.addBall = function () {
+  throw new Error('watersim: balls are not part of this build');
+};
+
+// releases the device context now instead of at garbage collection
+FluidSolver.prototype.dispose = function () {
+  if (this._h) { lib().destroy(this._h); this._h = null; }
+};
+
+FluidSolver.AIR = AIR; FluidSolver.FLUID = FLUID; FluidSolver.SOLID = SOLID;
+FluidSolver.STAGE = STAGE;
+module.exports = FluidSolver;
