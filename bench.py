@@ -30,7 +30,8 @@ W_DOMAIN = 5.1
 CONFIGS = {
     # name: grid n, particle target, seed, vorticity, SOR mode, frame-driven full physics
     "cfg1": dict(n=32, target=10000, seed=20260114, vorticity=0.0, tol=0.0, iters=40, physics=False),
-    "cfg2": dict(n=64, target=232000, seed=20260115, vorticity=0.05, tol=1e-4, iters=0, physics=False),
+    "cfg2": dict(n=64, target=232000, seed=20260115, vorticity=0.05, tol=1e-4, iters=0, physics=False,
+                 frames=True, currents=0.8, curl_init=0.5, surface_every=10),
     "cfg3": dict(n=128, target=2117000, seed=20260116, vorticity=0.05, tol=1e-4, iters=0, physics=True),
 }
 FRAME = 1.0 / 60
@@ -69,11 +70,36 @@ def make_solver(cfg):
         s.upload("pT", kelvin_to_model(270.0 + 30.0 * (1.0 - sin_lat ** 2)).astype(np.float32))
         s.set(sunActivity=SUN_ACTIVITY, hasSun=1)
         set_sun(s, world, 0)
+    if cfg.get("currents"):
+        s.set(currents=cfg["currents"])
+    if cfg.get("curl_init"):
+        nP = world["nP"]
+        x, y, z = (s.download(k)[:nP].astype(np.float64) for k in ("px", "py", "pz"))
+        u = curl_noise_velocity(x, y, z, cfg["seed"], cfg["curl_init"], W_DOMAIN)
+        for c, k in enumerate(("pvx", "pvy", "pvz")):
+            s.upload(k, u[c])
     if cfg["tol"] > 0:
         s.set(pressureTol=cfg["tol"], maxPressureIters=4000, residualEvery=8)
     else:
         s.set(pressureIters=cfg["iters"])
     return s, world
+
+
+def curl_noise_velocity(x, y, z, seed, rms, width, modes=24):
+    """Divergence-free random velocity: the analytic curl of a seeded sum of
+    plane-wave vector potentials, psi = sum_m a_m sin(k_m . x + phi_m), so
+    u = sum_m cos(k_m . x + phi_m) (k_m x a_m) has zero divergence exactly.
+    Wavelengths span width/4 .. width/1.5; scaled to the given RMS speed."""
+    rng = np.random.default_rng(seed)
+    u = np.zeros((3, len(x)))
+    for _ in range(modes):
+        d = rng.normal(size=3)
+        k = d / np.linalg.norm(d) * 2 * np.pi / width * rng.uniform(1.5, 4.0)
+        a = rng.normal(size=3)
+        th = k[0] * x + k[1] * y + k[2] * z + rng.uniform(0, 2 * np.pi)
+        u += np.cross(k, a)[:, None] * np.cos(th)[None, :]
+    u *= rms / max(np.sqrt((u * u).sum(0).mean()), 1e-30)
+    return u.astype(np.float32)
 
 
 def set_sun(s, world, substeps_done):
@@ -90,26 +116,33 @@ def advance(s, world, cfg, n, done):
     """n substeps: frames through the full step (tick, substeps, phase pass)
     when the workload has the thermal cycle, else back-to-back substeps with
     the CFL law evaluated on the device. Returns the substeps executed."""
-    if not cfg["physics"]:
+    if not (cfg["physics"] or cfg.get("frames")):
         s.run_substeps(n)
         return n
-    k = 0
+    k, every = 0, cfg.get("surface_every", 0)
     while k < n:
-        set_sun(s, world, done + k)
-        k += s.step(FRAME)
+        if cfg["physics"]:
+            set_sun(s, world, done + k)
+        m = s.step(FRAME)
+        if every and (done + k) // every != (done + k + m) // every:
+            # the surface, iso calibrated to the water volume (js/solver.js:3838-3849)
+            s.splat_density()
+            s.mt_build(iso=s.autotune_iso())
+        k += m
     return k
 
 
-def algorithmic_bytes(cfg, n, nP, sweeps, residuals, ticks):
+def algorithmic_bytes(cfg, n, nP, cell_sweeps, residuals, ticks):
     """Bytes the reference dataflow moves (each array once per stage at its
-    reference precision): plain substep 154 N + 69 F + 16 C, 13 C per SOR
-    sweep, 9 C per residual evaluation, vorticity 12 F + 82 C, and per
-    thermal tick 115 N + 77 C + 4 D plus evaporation 17 N + C."""
+    reference precision): plain substep 154 N + 69 F + 16 C, 13 per cell per
+    SOR sweep (the reference sweeps its fluid list, solver.js:1397-1404),
+    9 C per residual evaluation, vorticity 12 F + 82 C, and per thermal tick
+    115 N + 77 C + 4 D plus evaporation 17 N + C."""
     C, F, D = n ** 3, 3 * n * n * (n + 1), (n + 1) ** 3
     b = 154 * nP + 69 * F + 16 * C
     if cfg["vorticity"] > 0:
         b += 12 * F + 82 * C
-    return b, 13 * C * sweeps + 9 * C * residuals + ticks * (132 * nP + 78 * C + 4 * D)
+    return b, 13 * cell_sweeps + 9 * C * residuals + ticks * (132 * nP + 78 * C + 4 * D)
 
 
 def cpu_baseline(s, world, cfg, budget_s=15.0, max_substeps=8):
@@ -213,20 +246,24 @@ def main():
     s.sync()
     el = time.perf_counter() - t0
     g = {k: s.get(k) for k in ("sorMs", "sorLaunches", "pressureMs", "pressureSolves", "sweepsTotal",
-                               "residualsTotal", "ticksTotal", "sorTiled")}
+                               "residualsTotal", "ticksTotal", "sorMode", "sorCellSweeps")}
     s.set(timing=0)
     sps = steps / el
     C = n ** 3
-    # dominant kernel: the SOR sweep. Algorithmic bytes 13 per cell per full
-    # red+black sweep (q read + write, rhs, cell type); the tiled kernel does
-    # a full sweep per launch, the per-colour kernel half of one.
-    tiled = g["sorTiled"] != 0
-    sor_avg_s = (g["sorMs"] / 1e3) / max(g["sorLaunches"], 1)
-    sor_bytes = (13.0 if tiled else 6.5) * C
+    # dominant kernel: the SOR sweep. Algorithmic bytes: 13 per cell updated
+    # (q read + write, rhs, cell type). The list kernel updates the fluid
+    # cells of one colour per launch, the tiled kernel every cell of a full
+    # sweep, the grid kernel every cell of one colour.
+    mode = int(g["sorMode"])
+    launches = max(g["sorLaunches"], 1)
+    sor_avg_s = (g["sorMs"] / 1e3) / launches
+    sor_bytes = 13.0 * g["sorCellSweeps"] / launches
     achieved = sor_bytes / sor_avg_s / 1e9 if g["sorLaunches"] else 0.0
-    kname = "k_sor_tile" if tiled else "k_sor"
+    kname = {2: "k_sor_list", 1: "k_sor_tile", 0: "k_sor"}[mode]
+    kdesc = {2: " (the fluid cells of one colour)", 1: " (one full red-black sweep, LDS tile)",
+             0: " (one colour of the whole grid)"}[mode]
     traffic = read_traffic(kname, a.config)
-    b_plain, b_iter = algorithmic_bytes(cfg, n, nP, g["sweepsTotal"], g["residualsTotal"], g["ticksTotal"])
+    b_plain, b_iter = algorithmic_bytes(cfg, n, nP, g["sorCellSweeps"], g["residualsTotal"], g["ticksTotal"])
     b_total = b_plain * steps + b_iter
     ke = s.kinetic_energy()
     log("substep %.3f ms, pressure %.3f ms/solve, %.1f sweeps/substep, %d ticks, SOR launch %.2f us, ke %.4g"
@@ -237,6 +274,11 @@ def main():
         wl = ("%s: %d^3 MAC grid, %d FLIP particles, red-black SOR %s, vorticity %g, full thermal cycle "
               "(25 Hz heat + evaporation tick, frame-end phase pass, rotating sun, 270-300 K latitude profile), "
               "1/60 s frames" % (a.config, n, nP, sor_txt, cfg["vorticity"]))
+    elif cfg.get("frames"):
+        wl = ("%s: %d^3 MAC grid, %d FLIP particles, divergence-free curl-noise start (RMS %g m/s), red-black SOR "
+              "%s, vorticity %g, currents %g, marching-tets surface with iso calibration every %d substeps, "
+              "1/60 s frames" % (a.config, n, nP, cfg["curl_init"], sor_txt, cfg["vorticity"], cfg["currents"],
+                                 cfg["surface_every"]))
     else:
         wl = ("%s: %d^3 MAC grid, %d FLIP particles, red-black SOR %s, vorticity %g, thermal off, "
               "back-to-back substeps" % (a.config, n, nP, sor_txt, cfg["vorticity"]))
@@ -262,7 +304,7 @@ def main():
                    "substep_hbm_frac": b_total / el / HBM_PEAK},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / (HBM_PEAK / 1e9), "traffic": traffic,
-                     "kernel": kname + (" (one full red-black sweep, LDS tile)" if tiled else " (one colour)"),
+                     "kernel": kname + kdesc,
                      "avg_launch_us": 1e6 * sor_avg_s, "algorithmic_bytes_per_launch": sor_bytes},
     }
     if not a.no_cpu_baseline:

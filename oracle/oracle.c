@@ -1722,3 +1722,84 @@ double oc_kinetic_energy(oc_ctx *c) {
   }
   return e;
 }
+
+/* _pushSurfaceChunk, solver.js:3496-3587 (sphere world): the frame-end
+ * guarantee that no particle sits in rasterized rock or under the terrain
+ * surface, that airborne water stays under the atmosphere cap and fluid
+ * inside the domain. Per particle; walks in double like the reference and
+ * stores float. */
+void oc_push_surface(oc_ctx *c) {
+  if (!c->tsolid) return;
+  double cx = c->cx, cy = c->cy, cz = c->cz, dx = c->dx;
+  for (int p = 0; p < c->nP; p++) {
+    double x = c->px[p], y = c->py[p], z = c->pz[p];
+    double ex = x - cx, ey = y - cy, ez = z - cz;
+    double er = sqrt(ex * ex + ey * ey + ez * ez);
+    if (!(er != 0)) er = 1e-9;
+    int fixed = 0, rockFixed = 0;
+    double outer = c->domainR - dx * 0.02;
+    double skyHi = c->oceanR + c->atmosphereH - 1e-3 * dx;
+    if (er > skyHi) {
+      x = cx + ex / er * skyHi; y = cy + ey / er * skyHi; z = cz + ez / er * skyHi;
+      double outward = ((double)c->pvx[p] * ex + (double)c->pvy[p] * ey + (double)c->pvz[p] * ez) / er;
+      if (outward > 0) {
+        c->pvx[p] = (oc_real)(c->pvx[p] - ex / er * outward);
+        c->pvy[p] = (oc_real)(c->pvy[p] - ey / er * outward);
+        c->pvz[p] = (oc_real)(c->pvz[p] - ez / er * outward);
+      }
+      c->px[p] = (oc_real)x; c->py[p] = (oc_real)y; c->pz[p] = (oc_real)z;
+      ex = x - cx; ey = y - cy; ez = z - cz; er = skyHi;
+      fixed = 1;
+    }
+    if (er <= c->domainR && oc_rock_cell_at(c, x, y, z)) {
+      rockFixed = 1;
+      int tries = 0;
+      while (oc_rock_cell_at(c, x, y, z) && tries < c->nx * 2) {
+        x += ex / er * dx * 0.5; y += ey / er * dx * 0.5; z += ez / er * dx * 0.5;
+        tries++;
+      }
+      double exR = x - cx, eyR = y - cy, ezR = z - cz;
+      double erR = sqrt(exR * exR + eyR * eyR + ezR * ezR);
+      if (!(erR != 0)) erR = 1e-9;
+      double wantR = oc_terrain_radius_at(c, x, y, z) + dx * 0.06;
+      if (erR > wantR) {
+        x = cx + exR / erR * wantR; y = cy + eyR / erR * wantR; z = cz + ezR / erR * wantR;
+      }
+      int kR = 0;
+      while (oc_rock_cell_at(c, x, y, z) && kR < 4) {
+        x += exR / erR * dx * 0.5; y += eyR / erR * dx * 0.5; z += ezR / erR * dx * 0.5;
+        kR++;
+      }
+      fixed = 1;
+    } else if (c->pflag[p] == 0 && er <= c->domainR) {
+      double Rt = oc_terrain_radius_at(c, x, y, z);
+      if (er < Rt - dx * 0.5) {
+        rockFixed = 1;
+        int t2 = 0;
+        x = cx + ex / er * (Rt + dx * 0.05);
+        y = cy + ey / er * (Rt + dx * 0.05);
+        z = cz + ez / er * (Rt + dx * 0.05);
+        while (oc_rock_cell_at(c, x, y, z) && t2 < 8) {
+          x += ex / er * dx * 0.5; y += ey / er * dx * 0.5; z += ez / er * dx * 0.5;
+          t2++;
+        }
+        fixed = 1;
+      }
+    }
+    double finalR = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz));
+    double capR = c->pflag[p] == 0 ? outer : skyHi;
+    if (finalR > capR) {
+      x = cx + (x - cx) * capR / finalR; y = cy + (y - cy) * capR / finalR; z = cz + (z - cz) * capR / finalR;
+      fixed = 1;
+    }
+    if (fixed) {
+      c->px[p] = (oc_real)x; c->py[p] = (oc_real)y; c->pz[p] = (oc_real)z;
+      double vr = ((double)c->pvx[p] * ex + (double)c->pvy[p] * ey + (double)c->pvz[p] * ez) / er;
+      if (rockFixed && vr < 0) {
+        c->pvx[p] = (oc_real)(c->pvx[p] - ex / er * vr);
+        c->pvy[p] = (oc_real)(c->pvy[p] - ey / er * vr);
+        c->pvz[p] = (oc_real)(c->pvz[p] - ez / er * vr);
+      }
+    }
+  }
+}

@@ -215,6 +215,53 @@ function seededVel(s, seed, amp) {
   w.save('micro_16.wsg.gz');
 })();
 
+// The frame-end passes (_vaporCollisions, _iceContacts, _pushSurfaceParticles;
+// js/solver.js:3290-3587) on a state perturbed so that every branch runs:
+// particles pushed into the rock, above the atmosphere cap and outside the
+// domain; a tight line of vapor along the sea surface (chained contacts and
+// vapor-liquid contacts); a cluster of snow, part of it stuck.
+function frameEnd(w, s) {
+  var dx = s.dx, cx = s.cx, cy = s.cy, cz = s.cz, k = 0, p;
+  for (p = 7; p < s.nP && k < 90; p += 41, k++) {
+    var ex = s.px[p] - cx, ey = s.py[p] - cy, ez = s.pz[p] - cz;
+    var er = Math.sqrt(ex * ex + ey * ey + ez * ez) || 1e-9, r;
+    if (k % 3 === 0) { r = er - (0.5 + (k % 5) * 0.5) * dx; s.pflag[p] = 0; }
+    else if (k % 3 === 1) { r = s.oceanR + s.atmosphereH + 0.05 + 0.01 * (k % 7); s.pflag[p] = 2; }
+    else { r = s.domainR + 0.02 * (1 + (k % 4)); s.pflag[p] = 0; }
+    s.px[p] = cx + ex / er * r; s.py[p] = cy + ey / er * r; s.pz[p] = cz + ez / er * r;
+    s.pvx[p] = -ex / er * 0.3 + 0.01 * (k % 5); s.pvy[p] = -ey / er * 0.3; s.pvz[p] = -ez / er * 0.3;
+  }
+  // a line of vapor just under the sea surface, spaced 0.45 spacing apart
+  var p0 = 1500, dir = [0.6, 0.64, 0.48], r0 = s.oceanR - 0.3 * dx;
+  for (k = 0; k < 80 && p0 + k < s.nP; k++) {
+    p = p0 + k;
+    s.px[p] = cx + dir[0] * r0 + k * 0.45 * s.spacing; s.py[p] = cy + dir[1] * r0 + 0.02 * s.spacing * (k % 3);
+    s.pz[p] = cz + dir[2] * r0;
+    s.pflag[p] = k % 5 === 4 ? 3 : 2;
+    s.pvx[p] = 0.2 * ((k % 4) - 1.5); s.pvy[p] = 0.1 * ((k % 3) - 1); s.pvz[p] = 0.05 * (k % 2);
+    s.pWx[p] = 0.3 * (k % 2); s.pWy[p] = 0; s.pWz[p] = -0.2 * (k % 3 === 0 ? 1 : 0);
+  }
+  // a cluster of snow, every fourth grain stuck
+  var q0 = 3000;
+  for (k = 0; k < 60 && q0 + k < s.nP; k++) {
+    p = q0 + k;
+    s.px[p] = s.px[q0] + (k % 6) * 0.5 * s.spacing; s.py[p] = s.py[q0] + ((k / 6) | 0) * 0.5 * s.spacing;
+    s.pz[p] = s.pz[q0];
+    s.pflag[p] = 5; s.pAir[p] = k % 4 === 0 ? 1 : 0.2;
+    s.pvx[p] = 0.1 * ((k % 3) - 1); s.pvy[p] = -0.2; s.pvz[p] = 0.05 * (k % 2);
+  }
+  s.spinOn = true;
+  w.meta.frame_end = { spinOn: true, sunPos: s.sunPos, waterTopY: s.waterTopY };
+  dumpParticles(w, s, 'fe0_');
+  s._vaporCollisions();
+  w.meta.frame_end.vaporCount = s.vaporCount; w.meta.frame_end.nightVaporCount = s.nightVaporCount;
+  dumpParticles(w, s, 'fe1_');
+  s._iceContacts();
+  dumpParticles(w, s, 'fe2_');
+  s._pushSurfaceParticles();
+  dumpParticles(w, s, 'fe3_');
+}
+
 // ---------------------------------------------------------------- scene C
 // Full physics at 32^3: sun, evaporation, currents, vorticity, thermal tick
 // and the phase pass; temperatures from a 270-300 K latitude profile on the
@@ -263,5 +310,6 @@ function seededVel(s, seed, amp) {
   w.meta.vapor = nv;
   w.meta.rng_before = { evS: s._evS, cfS: s._cfS, bfS: s._bfS };
   stagedSubstep(w, s, dtLaw(s));
+  frameEnd(w, s);
   w.save('thermal_32.wsg.gz');
 })();

@@ -64,3 +64,23 @@ def test_phase_changes():
     assert np.array_equal(s.download("pflag"), g["t4_pflag"])
     close(s, g, "t4_", ["pvx", "pvy", "pvz", "pAir", "pWx", "pWy", "pWz", "pPh"])
     s.close()
+
+
+def test_push_surface():
+    """_pushSurfaceParticles on the perturbed frame-end state: every rescued
+    particle lands where the reference puts it (positions are walked in
+    float here, in double there; the rock tests agree, so the results match
+    to float rounding)."""
+    g = golden.load("thermal_32")
+    s = gpu_from_golden(g, tag="fe2_")
+    s.stage("push_surface", 0.0)
+    moved = 0
+    for k in ["px", "py", "pz", "pvx", "pvy", "pvz"]:
+        a = s.download(k).astype(np.float64)
+        b = g["fe3_" + k].astype(np.float64)
+        e = np.abs(a - b).max() / max(1.0, np.abs(b).max())
+        moved += int(np.sum(g["fe2_" + k] != g["fe3_" + k]))
+        print("  %-6s max rel err %.3e" % (k, e))
+        assert e <= 2e-5, (k, e)
+    assert moved > 100
+    s.close()

@@ -53,7 +53,7 @@ int dalloc(ws_ctx *c, T **p, size_t n) {
   return WS_OK;
 }
 
-World make_world(ws_ctx *c) {
+World world_of(ws_ctx *c) {
   World w;
   w.st = c->staticTypes;
   w.R = c->haveTerrain ? c->terrainR : nullptr;
@@ -122,7 +122,7 @@ int st_rasterize(ws_ctx *c) {
   hipLaunchKernelGGL(k_init_cells, dim3(blocks_for(nC + 1, 256)), dim3(256), 0, c->stream, c->staticTypes, c->cellType,
                      c->cellCount, nC);
   if (c->nP > 0)
-    hipLaunchKernelGGL(k_mark, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->geo, make_world(c), c->nP,
+    hipLaunchKernelGGL(k_mark, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->geo, world_of(c), c->nP,
                        c->px, c->py, c->pz, c->pflag, c->cellType, c->cellCount, c->pcell, c->prank, c->ds);
   WS_LAUNCHED();
   WS_HIP(hipcub::DeviceScan::ExclusiveSum(c->scanTmp, c->scanTmpBytes, c->cellCount, c->cellStart, nC + 1,
@@ -196,7 +196,15 @@ int sor_sweeps(ws_ctx *c, int n) {
   if (c->timing && n > 0) WS_HIP(hipEventRecord(c->ev[0], c->stream));
   int nT = c->sorTiles[0] * c->sorTiles[1] * c->sorTiles[2];
   for (int it = 0; it < n; it++) {
-    if (c->sorTiled) {
+    if (c->sorMode == 2) {
+      if (c->nRed > 0)
+        hipLaunchKernelGGL(k_sor_list, dim3(blocks_for(c->nRed, 256)), dim3(256), 0, c->stream, c->geo, (int)c->nRed,
+                           c->sorCode, c->sorRhs, c->sorPw, c->q, om1);
+      if (c->nBlack > 0)
+        hipLaunchKernelGGL(k_sor_list, dim3(blocks_for(c->nBlack, 256)), dim3(256), 0, c->stream, c->geo,
+                           (int)c->nBlack, c->sorCode + c->sorHalf, c->sorRhs + c->sorHalf, c->sorPw + c->sorHalf,
+                           c->q, om1);
+    } else if (c->sorMode == 1) {
       hipLaunchKernelGGL(k_sor_tile, dim3(nT), dim3(256), 0, c->stream, c->geo, c->sorTiles[0], c->sorTiles[1],
                          c->sorActive, c->q, c->q2, c->rhs, c->pw, c->nbm, om1);
       float *t = c->q; c->q = c->q2; c->q2 = t;  // the new sweep is current
@@ -216,19 +224,32 @@ int sor_sweeps(ws_ctx *c, int n) {
     float ms = 0;
     WS_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     c->sorMs += ms;
-    c->sorLaunches += (c->sorTiled ? 1L : 2L) * n;
+    long per = c->sorMode == 1 ? 1L : (c->sorMode == 2 ? (long)(c->nRed > 0) + (c->nBlack > 0) : 2L);
+    c->sorLaunches += per * n;
+    c->sorCellSweeps += (c->sorMode == 2 ? c->nRed + c->nBlack : c->nC) * (long)n;
   }
   return WS_OK;
 }
 
+// the list lengths, as the last device-scalar copy saw them
+void take_counts(ws_ctx *c) {
+  c->nRed = c->hs->nRed;
+  c->nBlack = c->hs->nBlack;
+}
+
 int residual(ws_ctx *c, double *res) {
   c->residualsTotal++;
-  hipLaunchKernelGGL(k_residual, dim3(c->nPartials), dim3(256), 0, c->stream, c->geo, c->q, c->rhs, c->nbm, c->dg,
-                     c->partials);
+  if (c->sorMode == 2)
+    hipLaunchKernelGGL(k_residual_list, dim3(c->nPartials), dim3(256), 0, c->stream, c->geo, c->ds, c->sorHalf,
+                       c->sorCode, c->sorRhs, c->sorDg, c->q, c->partials);
+  else
+    hipLaunchKernelGGL(k_residual, dim3(c->nPartials), dim3(256), 0, c->stream, c->geo, c->q, c->rhs, c->nbm, c->dg,
+                       c->partials);
   hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, c->stream, c->partials, c->nPartials, c->ds);
   WS_LAUNCHED();
   WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
   WS_HIP(hipStreamSynchronize(c->stream));
+  take_counts(c);
   *res = c->hs->bb > 0 ? std::sqrt(c->hs->rr) / std::sqrt(c->hs->bb) : 0.0;
   return WS_OK;
 }
@@ -253,13 +274,28 @@ int st_pressure(ws_ctx *c) {
 int st_pressure_solve(ws_ctx *c) {
   hipLaunchKernelGGL(k_assemble, dim3(blocks_for(c->nC, 256)), dim3(256), 0, c->stream, c->geo, c->cellType, c->vel,
                      c->q, c->q2, c->rhs, c->pw, c->nbm, c->dg, (float)c->omega);
-  if (c->sorTiled) {
+  if (c->sorMode == 1) {
     int nT = c->sorTiles[0] * c->sorTiles[1] * c->sorTiles[2];
     hipLaunchKernelGGL(k_sor_tiles_mark, dim3(nT), dim3(256), 0, c->stream, c->geo, c->sorTiles[0], c->sorTiles[1],
                        nT, c->nbm, c->sorActive);
   }
   WS_LAUNCHED();
   int r;
+  if (c->sorMode == 2) {
+    int nb = blocks_for(c->nC + 1, 256);
+    hipLaunchKernelGGL(k_sor_count, dim3(nb), dim3(256), 0, c->stream, c->geo, c->nbm, c->sorOff);
+    WS_LAUNCHED();
+    WS_HIP(hipcub::DeviceScan::ExclusiveSum(c->scanTmp, c->scanTmpBytes, c->sorOff, c->sorOff, (int)c->nC + 1,
+                                            c->stream));
+    hipLaunchKernelGGL(k_sor_compact, dim3(nb), dim3(256), 0, c->stream, c->geo, c->nbm, c->dg, c->rhs, c->pw,
+                       c->sorOff, c->sorHalf, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->ds);
+    WS_LAUNCHED();
+    if (c->pressureTol <= 0) {  // the to-residual path learns the counts with its first check
+      WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+      WS_HIP(hipStreamSynchronize(c->stream));
+      take_counts(c);
+    }
+  }
   if (c->pressureTol <= 0) {
     if ((r = sor_sweeps(c, c->pressureIters))) return r;
     c->itersDone = c->pressureIters;
@@ -334,7 +370,7 @@ int st_extrapolate(ws_ctx *c) {
   for (size_t d = 0; d + 1 < c->exDiagOff.size(); d++) {
     int n = c->exDiagOff[d + 1] - c->exDiagOff[d];
     if (n > 0)
-      hipLaunchKernelGGL(k_extrap_diag, dim3(n), dim3(EX_TJ * EX_TK), 0, c->stream, c->geo, c->exTiles, c->exDiagOff[d],
+      hipLaunchKernelGGL(k_extrap_diag, dim3(n), dim3(256), 0, c->stream, c->geo, c->exTiles, c->exDiagOff[d],
                          c->vel, c->valid);
   }
   WS_LAUNCHED();
@@ -344,7 +380,7 @@ int st_extrapolate(ws_ctx *c) {
 int st_vapor(ws_ctx *c) {
   if (!c->hasVapor || c->nP <= 0) return WS_OK;
   int nb = blocks_for(c->nP, 256);
-  World w = make_world(c);
+  World w = world_of(c);
   Par P = make_par(c);
   unsigned *needA = c->vneed, *offA = c->vneed + c->capacity, *needB = c->vneed + 2L * c->capacity,
            *offB = c->vneed + 3L * c->capacity;
@@ -366,7 +402,7 @@ int st_vapor(ws_ctx *c) {
 
 int st_particles(ws_ctx *c, int mode) {
   if (c->nP <= 0) return WS_OK;
-  hipLaunchKernelGGL(k_g2p_advect, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->geo, make_world(c),
+  hipLaunchKernelGGL(k_g2p_advect, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->geo, world_of(c),
                      make_par(c), c->nP, mode, c->vel, c->velO, c->cellType, c->px, c->py, c->pz, c->pvx, c->pvy,
                      c->pvz, c->pflag, c->pcool, c->pAir, c->advU, c->advV, c->advW, c->ds);
   WS_LAUNCHED();
@@ -460,6 +496,9 @@ bool field_of(ws_ctx *c, const char *n, Field &f) {
 
 }  // namespace
 
+// the terrain view the frame passes use (ws_frame.hip)
+World ws::make_world(ws_ctx *c) { return world_of(c); }
+
 // ====================================================================== ABI
 extern "C" {
 
@@ -520,6 +559,16 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
   c->sorTiles[1] = (ny + SOR_TY - 1) / SOR_TY;
   c->sorTiles[2] = (nz + SOR_TZ - 1) / SOR_TZ;
   if (dalloc(c, &c->sorActive, (size_t)c->sorTiles[0] * c->sorTiles[1] * c->sorTiles[2])) { ws_destroy(c); return WS_EHIP; }
+  // fluid-cell lists: red entries from 0, black from sorHalf; the list code
+  // holds the cell index in 26 bits
+  c->sorHalf = (long)(c->nC + 1) / 2 + 1;
+  if (c->nC >= (1L << 26)) c->sorMode = 0;
+  if ((r = dalloc(c, &c->sorOff, c->nC + 1)) || (r = dalloc(c, &c->sorCode, c->nC + 4)) ||
+      (r = dalloc(c, &c->sorRhs, c->nC + 4)) || (r = dalloc(c, &c->sorPw, c->nC + 4)) ||
+      (r = dalloc(c, &c->sorDg, c->nC + 4))) {
+    ws_destroy(c);
+    return r;
+  }
   c->gv = c->gu + c->uN;
   c->gw = c->gu + c->uN + c->vN;
   if (hipHostMalloc((void **)&c->hs, sizeof(DevScalars), 0) != hipSuccess) { ws_destroy(c); return WS_EHIP; }
@@ -529,7 +578,10 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
   size_t b1 = 0, b2 = 0;
   hipcub::DeviceScan::ExclusiveSum(nullptr, b1, c->cellCount, c->cellStart, (int)c->nC + 1, c->stream);
   hipcub::DeviceScan::ExclusiveSum(nullptr, b2, c->vneed, c->vneed, (int)cap, c->stream);
+  size_t b3 = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, b3, c->sorOff, c->sorOff, (int)c->nC + 1, c->stream);
   c->scanTmpBytes = b1 > b2 ? b1 : b2;
+  if (b3 > c->scanTmpBytes) c->scanTmpBytes = b3;
   if (hipMalloc(&c->scanTmp, c->scanTmpBytes) != hipSuccess) { ws_destroy(c); return WS_EHIP; }
   hipMemcpy(c->thr, thr.data(), thr.size() * sizeof(float), hipMemcpyHostToDevice);
   DevScalars init{};
@@ -582,7 +634,7 @@ void ws_destroy(ws_ctx *c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->staticTypes, c->gu, c->thr, c->terrainR, c->cellType, c->nbm, c->dg, c->q, c->q2, c->rhs, c->pw,
-                  c->sorActive,
+                  c->sorActive, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg,
                   c->cellCount, c->cellStart, c->vort, c->vel, c->velO, c->wgt, c->scr, c->scr2, c->valid, c->dens,
                   c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pT, c->pAir, c->advU, c->advV, c->advW, c->pflag,
                   c->pcool, c->pcell, c->prank, c->sortedIdx, c->vneed, c->vscr, c->rng, c->ds, c->partials,
@@ -609,7 +661,7 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
   SF(heatK) SF(Tamb) SF(shadeCool) SF(sunActivity) SF(evapIntensity) SF(snowT) SF(meltT) SF(cloudP)
   SF(currents) SF(Rlo) SF(Rhi) SF(simTime) SF(heatAcc)
   SI(pressureIters) SI(maxPressureIters) SI(residualEvery) SI(ceilReflect) SI(hasSun) SI(measureResidual)
-  SI(adaptiveChecks) SI(sorTiled) SI(spinOn)
+  SI(adaptiveChecks) SI(sorMode) SI(spinOn)
 #undef SF
 #undef SI
   if (!strcmp(n, "gRamp")) { c->gRamp = v; return push_ramp(c); }
@@ -626,6 +678,7 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
       if (!c->ev[i]) WS_HIP(hipEventCreate(&c->ev[i]));
     c->sorMs = c->pressureMs = 0;
     c->sorLaunches = c->pressureSolves = c->sweepsTotal = c->residualsTotal = c->ticksTotal = 0;
+    c->sorCellSweeps = 0;
     return WS_OK;
   }
   c->err = std::string("unknown scalar ") + n;
@@ -649,7 +702,7 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   GF(cfl) GF(pic) GF(gravity) GF(viscosity) GF(omega) GF(maxSpeed) GF(vorticity) GF(coreR) GF(oceanR)
   GF(atmosphereH) GF(spacing) GF(particleVolume) GF(iso) GF(substeps) GF(pressureTol) GF(pressureIters)
   GF(maxPressureIters) GF(nP) GF(capacity) GF(residual) GF(domainR) GF(dx) GF(tn) GF(dv)
-  GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal) GF(residualsTotal) GF(ticksTotal) GF(sorTiled)
+  GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal) GF(residualsTotal) GF(ticksTotal) GF(sorMode) GF(sorCellSweeps)
   GF(heatK) GF(Tamb) GF(shadeCool) GF(sunActivity) GF(evapIntensity) GF(snowT) GF(meltT) GF(cloudP) GF(currents)
   GF(simTime) GF(heatAcc) GF(evapCount) GF(cloudT) GF(rainT) GF(rainLiftAge)
 #undef GF
@@ -773,6 +826,7 @@ int ws_step_frame(ws_ctx *c, double dtFrame, int *n_sub) {
     n++;
   }
   if ((r = frame_phase(c, dtFrame))) return r;
+  if ((r = frame_push_surface(c))) return r;
   WS_HIP(hipStreamSynchronize(c->stream));
   if (n_sub) *n_sub = n;
   return WS_OK;
@@ -825,6 +879,7 @@ int ws_stage(ws_ctx *c, int stage, double dt) {
     case WS_STAGE_EVAPORATION: r = frame_evaporation(c, dt); break;
     case WS_STAGE_CURRENTS: r = frame_currents(c, dt); break;
     case WS_STAGE_PHASE: r = frame_phase(c, dt); break;
+    case WS_STAGE_PUSH_SURFACE: r = frame_push_surface(c); break;
     default: c->err = "unknown stage"; return WS_EINVAL;
   }
   if (r) return r;

@@ -10,4 +10,6 @@ int frame_heat(ws_ctx *c, double dt);
 int frame_evaporation(ws_ctx *c, double dt);
 int frame_currents(ws_ctx *c, double dt);
 int frame_phase(ws_ctx *c, double dt);
+int frame_push_surface(ws_ctx *c);
+World make_world(ws_ctx *c);
 }  // namespace ws

@@ -180,4 +180,15 @@ int frame_phase(ws_ctx *c, double dt) {
   return WS_OK;
 }
 
+// _pushSurfaceParticles, solver.js:3486-3488
+int frame_push_surface(ws_ctx *c) {
+  if (c->nP < 1 || !c->haveTerrain) return WS_OK;
+  float skyHi = (float)(c->oceanR + c->atmosphereH - 1e-3 * c->dx);
+  float outer = (float)(c->domainR - c->dx * 0.02);
+  hipLaunchKernelGGL(k_push_surface, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->geo, make_world(c),
+                     c->nP, (float)c->domainR, skyHi, outer, c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pflag);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
 }  // namespace ws

@@ -524,6 +524,110 @@ __global__ __launch_bounds__(256) void k_sor_tile(Geo g, int tilesX, int tilesY,
   }
 }
 
+// ---- fluid-cell lists for the per-colour sweeps
+// A planet's fluid is a thin shell (about an eighth of the cells at 128^3),
+// so the sweeps run over compact lists of the fluid cells of each colour
+// instead of the whole grid. Cells keep their index order within a list;
+// the relaxation of one colour reads only the other colour, so the result
+// equals the whole-grid per-colour sweep bit for bit.
+
+// scan keys: 1 for a red fluid cell, 2^32 for a black one (one 64-bit
+// exclusive scan then gives both list positions)
+__global__ void k_sor_count(Geo g, const uint8_t *__restrict__ nbm, unsigned long long *__restrict__ key) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > g.nC) return;
+  unsigned long long v = 0;
+  if (c < g.nC && (nbm[c] & 0x40)) {
+    int i = c % g.nx, j = (c / g.nx) % g.ny, k = c / (g.nx * g.ny);
+    v = ((i + j + k) & 1) ? (1ull << 32) : 1ull;
+  }
+  key[c] = v;
+}
+
+__global__ void k_sor_compact(Geo g, const uint8_t *__restrict__ nbm, const uint8_t *__restrict__ dg,
+                              const float *__restrict__ rhs, const float *__restrict__ pw,
+                              const unsigned long long *__restrict__ off, long half, unsigned *__restrict__ code,
+                              float *__restrict__ lrhs, float *__restrict__ lpw, uint8_t *__restrict__ ldg,
+                              DevScalars *ds) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > g.nC) return;
+  if (c == g.nC) {
+    unsigned long long t = off[c];
+    ds->nRed = (unsigned)(t & 0xffffffffull);
+    ds->nBlack = (unsigned)(t >> 32);
+    return;
+  }
+  unsigned m = nbm[c];
+  if (!(m & 0x40)) return;
+  int i = c % g.nx, j = (c / g.nx) % g.ny, k = c / (g.nx * g.ny);
+  unsigned long long o = off[c];
+  long pos = ((i + j + k) & 1) ? half + (long)(o >> 32) : (long)(o & 0xffffffffull);
+  code[pos] = (unsigned)c | ((m & 63u) << 26);
+  lrhs[pos] = rhs[c];
+  lpw[pos] = pw[c];
+  ldg[pos] = dg[c];
+}
+
+// one colour: q[c] = q[c](1 - omega) + omega/diag (rhs + sum of fluid
+// neighbours), the arithmetic of k_sor
+__global__ __launch_bounds__(256) void k_sor_list(Geo g, int n, const unsigned *__restrict__ code,
+                                                  const float *__restrict__ lrhs, const float *__restrict__ lpw,
+                                                  float *__restrict__ q, float om1) {
+  int b = xcd_block(blockIdx.x, gridDim.x);
+  int t = b * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  unsigned e = code[t];
+  int c = (int)(e & 0x3ffffffu);
+  unsigned m = e >> 26;
+  int nxy = g.nx * g.ny;
+  float sum = 0.f;
+  if (m & 1) sum += q[c - 1];
+  if (m & 2) sum += q[c + 1];
+  if (m & 4) sum += q[c - g.nx];
+  if (m & 8) sum += q[c + g.nx];
+  if (m & 16) sum += q[c - nxy];
+  if (m & 32) sum += q[c + nxy];
+  q[c] = q[c] * om1 + lpw[t] * (lrhs[t] + sum);
+}
+
+// ||rhs - A q||^2 and ||rhs||^2 over the two lists (red, then black), FP64
+// block partials over a fixed grid, as k_residual
+__global__ __launch_bounds__(256) void k_residual_list(Geo g, const DevScalars *__restrict__ ds, long half,
+                                                       const unsigned *__restrict__ code,
+                                                       const float *__restrict__ lrhs,
+                                                       const uint8_t *__restrict__ ldg, const float *__restrict__ q,
+                                                       double *__restrict__ partials) {
+  __shared__ double sr[256], sb[256];
+  long nR = ds->nRed, n = nR + ds->nBlack;
+  double rr = 0, bb = 0;
+  int nxy = g.nx * g.ny;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    long pos = t < nR ? t : half + (t - nR);
+    unsigned e = code[pos];
+    int c = (int)(e & 0x3ffffffu);
+    unsigned m = e >> 26;
+    float sum = 0.f;
+    if (m & 1) sum += q[c - 1];
+    if (m & 2) sum += q[c + 1];
+    if (m & 4) sum += q[c - g.nx];
+    if (m & 8) sum += q[c + g.nx];
+    if (m & 16) sum += q[c - nxy];
+    if (m & 32) sum += q[c + nxy];
+    double rh = (double)lrhs[pos];
+    double r = rh - ((double)ldg[pos] * (double)q[c] - (double)sum);
+    rr += r * r;
+    bb += rh * rh;
+  }
+  sr[threadIdx.x] = rr;
+  sb[threadIdx.x] = bb;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) { sr[threadIdx.x] += sr[threadIdx.x + o]; sb[threadIdx.x] += sb[threadIdx.x + o]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { partials[2 * blockIdx.x] = sr[0]; partials[2 * blockIdx.x + 1] = sb[0]; }
+}
+
 // per SOR tile: does it hold a fluid cell (the tile sweep skips the rest)
 __global__ void k_sor_tiles_mark(Geo g, int tilesX, int tilesY, int nTiles, const uint8_t *__restrict__ nbm,
                                  uint8_t *__restrict__ active) {
@@ -723,18 +827,21 @@ __global__ void k_vort_splat(Geo g, const uint8_t *__restrict__ ct, const float 
   vel[f] = vel[f] + 0.5f * (fcc[fi.cl] + fcc[fi.cr]);
 }
 
-// ------------------------------------------------------- extrapolation
-// _extrapolate, solver.js:1562-1636: one in-place lexicographic pass in which
-// an invalid face takes the mean of its valid neighbours, counting faces
-// filled earlier in the same pass. A face therefore depends on its final
-// -i/-j/-k neighbours and on the original state of its +i/+j/+k neighbours.
-// Tiles of 32x16x16 faces are processed one tile-diagonal (I+J+K) per
-// launch; inside a tile a hyperplane wavefront (li+lj+lk) keeps the same
-// dependency order, so every face sees exactly what the sequential pass
-// shows it. Tiles on one diagonal touch disjoint faces.
+// _extrapolate's flood (solver.js:1613-1634): in lexicographic order, an
+// invalid face takes the mean of its valid neighbours, where a face filled
+// earlier in the same pass counts as valid. A face therefore depends on the
+// final state of its -i/-j/-k neighbours and on the original state of its
+// +i/+j/+k neighbours. Tiles of 32x8x8 faces are processed one tile-diagonal
+// (I+J+K) per launch. Inside a tile one wavefront walks the hyperplanes
+// li+lj+lk: lane (lj, lk) handles face li = step - lj - lk, so the -j / -k
+// neighbours it reads were written by lanes one step earlier. The lanes of
+// one wavefront execute their LDS operations in program order, so the steps
+// need no workgroup barrier. All four waves load the tile (with a one-face
+// halo) into LDS; three of them then leave.
 __global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restrict__ tiles, int off,
                                                      float *__restrict__ vel, uint8_t *__restrict__ valid) {
   constexpr int TI = EX_TI, TJ = EX_TJ, TK = EX_TK;
+  static_assert(TJ * TK == 64, "one lane per (j, k) row of a tile");
   constexpr int HI = TI + 2, HJ = TJ + 2, HK = TK + 2, HN = HI * HJ * HK;
   __shared__ float sv[HK][HJ][HI];
   __shared__ uint8_t sf[HK][HJ][HI];
@@ -745,15 +852,13 @@ __global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restri
   float *A = vel + base;
   uint8_t *V = valid + base;
   int i0 = I * TI - 1, j0 = J * TJ - 1, k0 = K * TK - 1;
-  // tile + one-face halo into LDS; loads are issued in batches of 8 before
-  // any is consumed so their latencies overlap
-  constexpr int B = 8;
-  for (int e0 = 0; e0 < HN; e0 += 256 * B) {
+  constexpr int B = (HN + 255) / 256;
+  {
     float val[B];
     uint8_t fl[B];
 #pragma unroll
     for (int b = 0; b < B; b++) {
-      int e = e0 + b * 256 + threadIdx.x;
+      int e = b * 256 + threadIdx.x;
       int li = e % HI, lj = (e / HI) % HJ, lk = e / (HI * HJ);
       int i = i0 + li, j = j0 + lj, k = k0 + lk;
       bool in = e < HN && i >= 0 && i < d0 && j >= 0 && j < d1 && k >= 0 && k < d2;
@@ -763,7 +868,7 @@ __global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restri
     }
 #pragma unroll
     for (int b = 0; b < B; b++) {
-      int e = e0 + b * 256 + threadIdx.x;
+      int e = b * 256 + threadIdx.x;
       if (e < HN) {
         (&sv[0][0][0])[e] = val[b];
         (&sf[0][0][0])[e] = fl[b] ? 1 : 0;
@@ -771,12 +876,14 @@ __global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restri
     }
   }
   __syncthreads();
+  if (threadIdx.x >= 64) return;
   int lj = 1 + (threadIdx.x % TJ), lk = 1 + (threadIdx.x / TJ);
   int j = j0 + lj, k = k0 + lk;
+  bool row = j < d1 && k < d2;
   for (int step = 0; step < TI + TJ + TK - 2; step++) {
     int li = 1 + step - (lj - 1) - (lk - 1);
     int i = i0 + li;
-    if (li >= 1 && li <= TI && i < d0 && j < d1 && k < d2 && !sf[lk][lj][li]) {
+    if (row && li >= 1 && li <= TI && i < d0 && !sf[lk][lj][li]) {
       float sum = 0.f;
       int cnt = 0;
       if (i > 0 && sf[lk][lj][li - 1]) { sum += sv[lk][lj][li - 1]; cnt++; }
@@ -794,7 +901,9 @@ __global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restri
         V[f] = 1;
       }
     }
-    __syncthreads();
+    // keeps the compiler from moving this step's LDS stores past the next
+    // step's loads; the hardware already orders them within the wavefront
+    __builtin_amdgcn_wave_barrier();
   }
 }
 

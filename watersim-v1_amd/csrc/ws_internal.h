@@ -23,7 +23,7 @@ constexpr float kFixInvF = 1.0f / 1125899906842624.0f;
 
 // Tile shapes (cells) for P2G and the extrapolation wavefront.
 constexpr int P2G_T = 8;
-constexpr int EX_TI = 32, EX_TJ = 16, EX_TK = 16;
+constexpr int EX_TI = 32, EX_TJ = 8, EX_TK = 8;
 constexpr int SOR_TX = 32, SOR_TY = 8, SOR_TZ = 8;
 
 // Scalars that live on the device so that substeps can run back to back
@@ -37,6 +37,7 @@ struct DevScalars {
   unsigned airborne;   // airborneCount of the last G2P
   unsigned df_refined; // demotion predicates decided by the float-float path
   unsigned nan_seen;
+  unsigned nRed, nBlack;  // fluid cells of each colour (pressure lists)
   double rr, bb;       // residual^2, rhs^2 (FP64 norms)
   double ke;
   double vol;          // marching-tets volume (FP64)
@@ -88,6 +89,7 @@ struct Ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double sorMs = 0, pressureMs = 0;
   long sorLaunches = 0, pressureSolves = 0, sweepsTotal = 0, residualsTotal = 0, ticksTotal = 0;
+  long sorCellSweeps = 0;  // cells updated by the timed SOR launches
   // results of the last calls
   double umax = 0, residual = 0;
   int itersDone = 0, airborne = 0, nFluid = 0;
@@ -105,8 +107,17 @@ struct Ctx {
   uint8_t *cellType = nullptr, *nbm = nullptr, *dg = nullptr;
   float *q = nullptr, *q2 = nullptr, *rhs = nullptr, *pw = nullptr;
   uint8_t *sorActive = nullptr;  // per SOR tile: holds a fluid cell
+  // fluid cells per colour, in cell order (red from 0, black from sorHalf):
+  // code = cell | neighbour mask << 26, with rhs, omega/diag and diag per entry
+  unsigned long long *sorOff = nullptr;
+  unsigned *sorCode = nullptr;
+  float *sorRhs = nullptr, *sorPw = nullptr;
+  uint8_t *sorDg = nullptr;
+  long sorHalf = 0, nRed = 0, nBlack = 0;
   int sorTiles[3] = {0, 0, 0};
-  int sorTiled = 1;              // 1: LDS-tiled full sweeps, 0: one launch per colour
+  // 2: one launch per colour over the fluid-cell lists, 1: LDS-tiled full
+  // sweeps, 0: one launch per colour over the whole grid
+  int sorMode = 2;
   unsigned *cellCount = nullptr, *cellStart = nullptr;
   float *vort = nullptr;  // 10*C scratch for vorticity confinement
   // faces (u | v | w contiguous in one allocation each)

@@ -60,6 +60,14 @@ __global__ void k_sor(Geo g, float *q, const float *rhs, const float *pw, const 
                       float om1);
 __global__ void k_residual(Geo g, const float *q, const float *rhs, const uint8_t *nbm, const uint8_t *dg,
                            double *partials);
+__global__ void k_sor_count(Geo g, const uint8_t *nbm, unsigned long long *key);
+__global__ void k_sor_compact(Geo g, const uint8_t *nbm, const uint8_t *dg, const float *rhs, const float *pw,
+                              const unsigned long long *off, long half, unsigned *code, float *lrhs, float *lpw,
+                              uint8_t *ldg, DevScalars *ds);
+__global__ void k_sor_list(Geo g, int n, const unsigned *code, const float *lrhs, const float *lpw, float *q,
+                           float om1);
+__global__ void k_residual_list(Geo g, const DevScalars *ds, long half, const unsigned *code, const float *lrhs,
+                                const uint8_t *ldg, const float *q, double *partials);
 __global__ void k_residual_final(const double *partials, int n, DevScalars *s);
 __global__ void k_project(Geo g, const uint8_t *ct, const float *phi, float *vel, int do_clamp, float vcap,
                           DevScalars *s);
@@ -124,6 +132,8 @@ __global__ void k_currents_relax(Geo g, CurrPar Q, int nP, const float *px, cons
                                  float *pvx, float *pvy, float *pvz, const uint8_t *pflag, const float *cf);
 __global__ void k_steam_count(Geo g, int nP, const float *px, const float *py, const float *pz, const uint8_t *pflag,
                               const float *thr, unsigned *cnt);
+__global__ void k_push_surface(Geo g, World w, int nP, float domainR, float skyHi, float outer, float *px, float *py,
+                               float *pz, float *pvx, float *pvy, float *pvz, const uint8_t *pflag);
 __global__ void k_phase(Geo g, PhasePar P, int nP, const float *px, const float *py, const float *pz, float *pvx,
                         float *pvy, float *pvz, const float *pT, uint8_t *pflag, float *pAir, float *pWx, float *pWy,
                         float *pWz, float *pPh, const uint8_t *ct, const unsigned *steam, const float *thr);

@@ -522,4 +522,72 @@ __global__ void k_phase(Geo g, PhasePar P, int nP, const float *__restrict__ px,
   }
 }
 
+// ------------------------------------------------------- surface rescue
+// _pushSurfaceChunk, solver.js:3496-3587: airborne water capped under the
+// atmosphere top, particles in rasterized rock walked out radially and
+// parked on the smooth surface, grid-coupled water buried under the surface
+// lifted back, fluid kept inside the domain; the inward velocity of a
+// rescued particle is removed. Per particle, float arithmetic.
+__global__ void k_push_surface(Geo g, World w, int nP, float domainR, float skyHi, float outer,
+                               float *__restrict__ px, float *__restrict__ py, float *__restrict__ pz,
+                               float *__restrict__ pvx, float *__restrict__ pvy, float *__restrict__ pvz,
+                               const uint8_t *__restrict__ pflag) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nP) return;
+  const float dx = g.dx, cx = g.cx, cy = g.cy, cz = g.cz;
+  float x = px[p], y = py[p], z = pz[p];
+  float ex = x - cx, ey = y - cy, ez = z - cz;
+  float er = sqrtf(ex * ex + ey * ey + ez * ez);
+  if (!(er != 0.f)) er = 1e-9f;
+  bool fixed = false, rockFixed = false;
+  uint8_t fl = pflag[p];
+  if (er > skyHi) {
+    x = cx + ex / er * skyHi; y = cy + ey / er * skyHi; z = cz + ez / er * skyHi;
+    float vx = pvx[p], vy = pvy[p], vz = pvz[p];
+    float outward = (vx * ex + vy * ey + vz * ez) / er;
+    if (outward > 0.f) {
+      pvx[p] = vx - ex / er * outward; pvy[p] = vy - ey / er * outward; pvz[p] = vz - ez / er * outward;
+    }
+    px[p] = x; py[p] = y; pz[p] = z;
+    ex = x - cx; ey = y - cy; ez = z - cz; er = skyHi;
+    fixed = true;
+  }
+  if (er <= domainR && rock_cell(w, g, x, y, z)) {
+    rockFixed = true;
+    float sx = ex / er * dx * 0.5f, sy = ey / er * dx * 0.5f, sz = ez / er * dx * 0.5f;
+    for (int tries = 0; tries < 2 * g.nx && rock_cell(w, g, x, y, z); tries++) { x += sx; y += sy; z += sz; }
+    float exR = x - cx, eyR = y - cy, ezR = z - cz;
+    float erR = sqrtf(exR * exR + eyR * eyR + ezR * ezR);
+    if (!(erR != 0.f)) erR = 1e-9f;
+    float wantR = terrain_radius(w, x, y, z) + dx * 0.06f;
+    if (erR > wantR) { x = cx + exR / erR * wantR; y = cy + eyR / erR * wantR; z = cz + ezR / erR * wantR; }
+    float tx = exR / erR * dx * 0.5f, ty = eyR / erR * dx * 0.5f, tz = ezR / erR * dx * 0.5f;
+    for (int kR = 0; kR < 4 && rock_cell(w, g, x, y, z); kR++) { x += tx; y += ty; z += tz; }
+    fixed = true;
+  } else if (fl == 0 && er <= domainR) {
+    float Rt = terrain_radius(w, x, y, z);
+    if (er < Rt - dx * 0.5f) {
+      rockFixed = true;
+      float lift = Rt + dx * 0.05f;
+      x = cx + ex / er * lift; y = cy + ey / er * lift; z = cz + ez / er * lift;
+      float sx = ex / er * dx * 0.5f, sy = ey / er * dx * 0.5f, sz = ez / er * dx * 0.5f;
+      for (int t2 = 0; t2 < 8 && rock_cell(w, g, x, y, z); t2++) { x += sx; y += sy; z += sz; }
+      fixed = true;
+    }
+  }
+  float fx = x - cx, fy = y - cy, fz = z - cz;
+  float finalR = sqrtf(fx * fx + fy * fy + fz * fz);
+  float capR = fl == 0 ? outer : skyHi;
+  if (finalR > capR) {
+    x = cx + fx * capR / finalR; y = cy + fy * capR / finalR; z = cz + fz * capR / finalR;
+    fixed = true;
+  }
+  if (fixed) {
+    px[p] = x; py[p] = y; pz[p] = z;
+    float vx = pvx[p], vy = pvy[p], vz = pvz[p];
+    float vr = (vx * ex + vy * ey + vz * ez) / er;
+    if (rockFixed && vr < 0.f) { pvx[p] = vx - ex / er * vr; pvy[p] = vy - ey / er * vr; pvz[p] = vz - ez / er * vr; }
+  }
+}
+
 }  // namespace ws

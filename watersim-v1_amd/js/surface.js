@@ -19,7 +19,7 @@ function handle() {
       throw new Error('watersim: the HIP addon is not built (' + where + '): ' + e.message);
     }
     // a minimal context: the field and its dimensions come with each call
-    ctx = addon.create(2, 2, 2, 1.0, 1);
+    ctx = addon.create(4, 4, 4, 1.0, 1);
   }
   return ctx;
 }

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "_build", "libwatersim_hip.so")
 STAGES = {
     "clear": 0, "rasterize": 1, "p2g": 2, "bc": 3, "snapshot": 4, "viscosity": 5, "gravity": 6,
     "pressure": 7, "project": 8, "vorticity": 9, "clamp": 10, "extrapolate": 11, "g2p": 12, "advect": 13,
-    "heat": 14, "evaporation": 15, "currents": 16, "phase": 17,
+    "heat": 14, "evaporation": 15, "currents": 16, "phase": 17, "push_surface": 18,
 }
 
 # exported entry points, as declared in include/watersim.h
@@ -108,6 +108,7 @@ class Solver:
         self.sizes = {"u": (nx + 1) * ny * nz, "v": nx * (ny + 1) * nz, "w": nx * ny * (nz + 1)}
         self.nD = (nx + 1) * (ny + 1) * (nz + 1)
         self.nP = 0
+        self._mt_pos = self._mt_nrm = None
 
     def close(self):
         if self.ctx:
@@ -205,6 +206,9 @@ class Solver:
         self._check(self.L.ws_splat_density(self.ctx), "splat_density")
 
     def mt_build(self, dens=None, dims=None, dx=None, iso=1.8, want_mesh=True):
+        """MarchingTetrahedra.build on the context field (dens None) or a host
+        field; the mesh lands in grow-only host buffers reused across calls
+        (capacity doubling from 32768 vertices, as js/surface.js:53-64)."""
         nx, ny, nz = dims if dims else (self.nx, self.ny, self.nz)
         dx = self.dx if dx is None else dx
         cnt = ctypes.c_long()
@@ -213,17 +217,25 @@ class Solver:
         if dens is not None:
             dens = np.ascontiguousarray(dens, np.float32)
             dptr = dens.ctypes.data_as(ctypes.c_void_p)
-        self._check(self.L.ws_mt_build(self.ctx, dptr, nx, ny, nz, dx, iso, None, None, 0, ctypes.byref(cnt),
-                                       ctypes.byref(vol)), "mt_build")
         if not want_mesh:
+            self._check(self.L.ws_mt_build(self.ctx, dptr, nx, ny, nz, dx, iso, None, None, 0, ctypes.byref(cnt),
+                                           ctypes.byref(vol)), "mt_build")
             return cnt.value, vol.value, None, None
-        n = cnt.value
-        pos = np.zeros(max(n, 1) * 3, np.float32)
-        nrm = np.zeros(max(n, 1) * 3, np.float32)
-        self._check(self.L.ws_mt_build(self.ctx, dptr, nx, ny, nz, dx, iso, pos.ctypes.data_as(ctypes.c_void_p),
-                                       nrm.ctypes.data_as(ctypes.c_void_p), n, ctypes.byref(cnt),
-                                       ctypes.byref(vol)), "mt_build")
-        return cnt.value, vol.value, pos[:n * 3], nrm[:n * 3]
+        for _ in range(2):
+            cap = 0 if self._mt_pos is None else len(self._mt_pos) // 3
+            pp = self._mt_pos.ctypes.data_as(ctypes.c_void_p) if cap else None
+            pn = self._mt_nrm.ctypes.data_as(ctypes.c_void_p) if cap else None
+            self._check(self.L.ws_mt_build(self.ctx, dptr, nx, ny, nz, dx, iso, pp, pn, cap, ctypes.byref(cnt),
+                                           ctypes.byref(vol)), "mt_build")
+            n = cnt.value
+            if n <= cap:
+                return n, vol.value, self._mt_pos[:n * 3], self._mt_nrm[:n * 3]
+            c = max(cap, 32768)
+            while c < n:
+                c *= 2
+            self._mt_pos = np.zeros(c * 3, np.float32)
+            self._mt_nrm = np.zeros(c * 3, np.float32)
+        raise RuntimeError("mt_build: vertex count changed between calls")
 
     def autotune_iso(self):
         out = ctypes.c_double()
