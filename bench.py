@@ -259,8 +259,9 @@ def main():
     sor_avg_s = (g["sorMs"] / 1e3) / launches
     sor_bytes = 13.0 * g["sorCellSweeps"] / launches
     achieved = sor_bytes / sor_avg_s / 1e9 if g["sorLaunches"] else 0.0
-    kname = {2: "k_sor_list", 1: "k_sor_tile", 0: "k_sor"}[mode]
-    kdesc = {2: " (the fluid cells of one colour)", 1: " (one full red-black sweep, LDS tile)",
+    kname = {3: "k_sor_tb", 2: "k_sor_list", 1: "k_sor_tile", 0: "k_sor"}[mode]
+    kdesc = {3: " (two red-black sweeps of the fluid cells per launch, LDS tiles)",
+             2: " (the fluid cells of one colour)", 1: " (one full red-black sweep, LDS tile)",
              0: " (one colour of the whole grid)"}[mode]
     traffic = read_traffic(kname, a.config)
     b_plain, b_iter = algorithmic_bytes(cfg, n, nP, g["sorCellSweeps"], g["residualsTotal"], g["ticksTotal"])

@@ -190,13 +190,26 @@ int st_gravity(ws_ctx *c) {
 }
 
 int sor_sweeps(ws_ctx *c, int n) {
+  c->tbLaunchesPending = 0;
   int half = (c->nx + 1) / 2;
   long nthr = (long)half * c->ny * c->nz;
   float om1 = (float)(1 - c->omega);
   if (c->timing && n > 0) WS_HIP(hipEventRecord(c->ev[0], c->stream));
   int nT = c->sorTiles[0] * c->sorTiles[1] * c->sorTiles[2];
   for (int it = 0; it < n; it++) {
-    if (c->sorMode == 2) {
+    if (c->sorMode == 3) {
+      int nTB = c->tbTiles[0] * c->tbTiles[1] * c->tbTiles[2];
+      if (it + 1 < n) {
+        hipLaunchKernelGGL(k_sor_tb<2>, dim3(nTB), dim3(256), 0, c->stream, c->geo, c->tbTiles[0], c->tbTiles[1],
+                           c->tbActive, c->q, c->q2, c->rhs, c->pw, c->nbm, om1);
+        it++;
+      } else {
+        hipLaunchKernelGGL(k_sor_tb<1>, dim3(nTB), dim3(256), 0, c->stream, c->geo, c->tbTiles[0], c->tbTiles[1],
+                           c->tbActive, c->q, c->q2, c->rhs, c->pw, c->nbm, om1);
+      }
+      float *t = c->q; c->q = c->q2; c->q2 = t;
+      c->tbLaunchesPending++;
+    } else if (c->sorMode == 2) {
       if (c->nRed > 0)
         hipLaunchKernelGGL(k_sor_list, dim3(blocks_for(c->nRed, 256)), dim3(256), 0, c->stream, c->geo, (int)c->nRed,
                            c->sorCode, c->sorRhs, c->sorPw, c->q, om1);
@@ -224,9 +237,13 @@ int sor_sweeps(ws_ctx *c, int n) {
     float ms = 0;
     WS_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     c->sorMs += ms;
-    long per = c->sorMode == 1 ? 1L : (c->sorMode == 2 ? (long)(c->nRed > 0) + (c->nBlack > 0) : 2L);
-    c->sorLaunches += per * n;
-    c->sorCellSweeps += (c->sorMode == 2 ? c->nRed + c->nBlack : c->nC) * (long)n;
+    if (c->sorMode == 3) {
+      c->sorLaunches += c->tbLaunchesPending;
+    } else {
+      long per = c->sorMode == 1 ? 1L : (c->sorMode == 2 ? (long)(c->nRed > 0) + (c->nBlack > 0) : 2L);
+      c->sorLaunches += per * n;
+    }
+    c->sorCellSweeps += (c->sorMode >= 2 ? c->nRed + c->nBlack : c->nC) * (long)n;
   }
   return WS_OK;
 }
@@ -239,7 +256,7 @@ void take_counts(ws_ctx *c) {
 
 int residual(ws_ctx *c, double *res) {
   c->residualsTotal++;
-  if (c->sorMode == 2)
+  if (c->sorMode >= 2)
     hipLaunchKernelGGL(k_residual_list, dim3(c->nPartials), dim3(256), 0, c->stream, c->geo, c->ds, c->sorHalf,
                        c->sorCode, c->sorRhs, c->sorDg, c->q, c->partials);
   else
@@ -279,9 +296,14 @@ int st_pressure_solve(ws_ctx *c) {
     hipLaunchKernelGGL(k_sor_tiles_mark, dim3(nT), dim3(256), 0, c->stream, c->geo, c->sorTiles[0], c->sorTiles[1],
                        nT, c->nbm, c->sorActive);
   }
+  if (c->sorMode == 3) {
+    int nTB = c->tbTiles[0] * c->tbTiles[1] * c->tbTiles[2];
+    hipLaunchKernelGGL(k_sor_tb_mark, dim3(nTB), dim3(256), 0, c->stream, c->geo, c->tbTiles[0], c->tbTiles[1], nTB,
+                       c->nbm, c->tbActive);
+  }
   WS_LAUNCHED();
   int r;
-  if (c->sorMode == 2) {
+  if (c->sorMode >= 2) {
     int nb = blocks_for(c->nC + 1, 256);
     hipLaunchKernelGGL(k_sor_count, dim3(nb), dim3(256), 0, c->stream, c->geo, c->nbm, c->sorOff);
     WS_LAUNCHED();
@@ -563,6 +585,13 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
   // holds the cell index in 26 bits
   c->sorHalf = (long)(c->nC + 1) / 2 + 1;
   if (c->nC >= (1L << 26)) c->sorMode = 0;
+  c->tbTiles[0] = (nx + SOR_TB_X - 1) / SOR_TB_X;
+  c->tbTiles[1] = (ny + SOR_TB_Y - 1) / SOR_TB_Y;
+  c->tbTiles[2] = (nz + SOR_TB_Z - 1) / SOR_TB_Z;
+  if ((r = dalloc(c, &c->tbActive, (size_t)c->tbTiles[0] * c->tbTiles[1] * c->tbTiles[2]))) {
+    ws_destroy(c);
+    return r;
+  }
   if ((r = dalloc(c, &c->sorOff, c->nC + 1)) || (r = dalloc(c, &c->sorCode, c->nC + 4)) ||
       (r = dalloc(c, &c->sorRhs, c->nC + 4)) || (r = dalloc(c, &c->sorPw, c->nC + 4)) ||
       (r = dalloc(c, &c->sorDg, c->nC + 4))) {
@@ -634,7 +663,7 @@ void ws_destroy(ws_ctx *c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->staticTypes, c->gu, c->thr, c->terrainR, c->cellType, c->nbm, c->dg, c->q, c->q2, c->rhs, c->pw,
-                  c->sorActive, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg,
+                  c->sorActive, c->tbActive, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg,
                   c->cellCount, c->cellStart, c->vort, c->vel, c->velO, c->wgt, c->scr, c->scr2, c->valid, c->dens,
                   c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pT, c->pAir, c->advU, c->advV, c->advW, c->pflag,
                   c->pcool, c->pcell, c->prank, c->sortedIdx, c->vneed, c->vscr, c->rng, c->ds, c->partials,

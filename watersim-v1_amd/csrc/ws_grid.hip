@@ -405,6 +405,13 @@ __global__ void k_assemble(Geo g, const uint8_t *__restrict__ ct, const float *_
   dg[c] = (uint8_t)diag;
 }
 
+// The relaxation of solver.js:1402, q(1 - omega) + omega/diag (rhs + sum),
+// with every rounding explicit so that all SOR kernels produce the same
+// bits for the same inputs (no contraction into FMAs).
+__device__ __forceinline__ float sor_relax(float q, float sum, float rhs, float pw, float om1) {
+  return __fadd_rn(__fmul_rn(q, om1), __fmul_rn(pw, __fadd_rn(rhs, sum)));
+}
+
 // One colour of red-black SOR (the relaxation of solver.js:1402): cells with
 // (i+j+k)&1 == color. Each thread updates one cell of that colour.
 __global__ __launch_bounds__(256) void k_sor(Geo g, float *__restrict__ q, const float *__restrict__ rhs,
@@ -429,7 +436,7 @@ __global__ __launch_bounds__(256) void k_sor(Geo g, float *__restrict__ q, const
   if (m & 8) sum += q[c + g.nx];
   if (m & 16) sum += q[c - nxy];
   if (m & 32) sum += q[c + nxy];
-  q[c] = q[c] * om1 + pw[c] * (rhs[c] + sum);
+  q[c] = sor_relax(q[c], sum, rhs[c], pw[c], om1);
 }
 
 // One full red-black sweep per launch, tiled through LDS. A block owns a
@@ -513,7 +520,7 @@ __global__ __launch_bounds__(256) void k_sor_tile(Geo g, int tilesX, int tilesY,
       if (msk & 8) sum += sq[qi + QX];
       if (msk & 16) sum += sq[qi - QX * QY];
       if (msk & 32) sum += sq[qi + QX * QY];
-      sq[qi] = sq[qi] * om1 + spw[cc] * (srhs[cc] + sum);
+      sq[qi] = sor_relax(sq[qi], sum, srhs[cc], spw[cc], om1);
     }
     __syncthreads();
   }
@@ -587,7 +594,7 @@ __global__ __launch_bounds__(256) void k_sor_list(Geo g, int n, const unsigned *
   if (m & 8) sum += q[c + g.nx];
   if (m & 16) sum += q[c - nxy];
   if (m & 32) sum += q[c + nxy];
-  q[c] = q[c] * om1 + lpw[t] * (lrhs[t] + sum);
+  q[c] = sor_relax(q[c], sum, lrhs[t], lpw[t], om1);
 }
 
 // ||rhs - A q||^2 and ||rhs||^2 over the two lists (red, then black), FP64
@@ -638,6 +645,127 @@ __global__ void k_sor_tiles_mark(Geo g, int tilesX, int tilesY, int nTiles, cons
   int any = 0;
   for (int e = threadIdx.x; e < SOR_TX * SOR_TY * SOR_TZ; e += blockDim.x) {
     int i = i0 + e % SOR_TX, j = j0 + (e / SOR_TX) % SOR_TY, k = k0 + e / (SOR_TX * SOR_TY);
+    if (i < g.nx && j < g.ny && k < g.nz && (nbm[(k * g.ny + j) * g.nx + i] & 0x40)) any = 1;
+  }
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) active[b] = (uint8_t)any;
+}
+
+// S red-black sweeps per launch over a 16x8x8 cell tile (temporal
+// blocking). The block loads q with a halo of 2S cells and the coefficients
+// with a halo of 2S-1 into LDS, then runs the 2S colour phases on shrinking
+// regions: the red phase of sweep t covers the tile plus a ring of
+// 2(S-1-t)+1 cells, the black phase a ring of 2(S-1-t). Every ring cell is
+// recomputed from exactly the inputs its owning tile uses, so the tile ends
+// with the values of S global sweeps, bit for bit. The tile is written to
+// the other q buffer (every workgroup reads the previous state). Tiles
+// without a fluid cell return at once.
+template <int S>
+__global__ __launch_bounds__(256) void k_sor_tb(Geo g, int tilesX, int tilesY, const uint8_t *__restrict__ active,
+                                                const float *__restrict__ qin, float *__restrict__ qout,
+                                                const float *__restrict__ rhs, const float *__restrict__ pw,
+                                                const uint8_t *__restrict__ nbm, float om1) {
+  constexpr int TX = SOR_TB_X, TY = SOR_TB_Y, TZ = SOR_TB_Z, H = 2 * S, HC = 2 * S - 1;
+  constexpr int QX = TX + 2 * H, QY = TY + 2 * H, QZ = TZ + 2 * H, QN = QX * QY * QZ;
+  constexpr int CX = TX + 2 * HC, CY = TY + 2 * HC, CZ = TZ + 2 * HC, CN = CX * CY * CZ;
+  __shared__ float sq[QN];
+  __shared__ float srhs[CN];
+  __shared__ float spw[CN];
+  __shared__ uint8_t snb[CN];
+  int b = xcd_block(blockIdx.x, gridDim.x);
+  if (!active[b]) return;
+  int tx = b % tilesX, ty = (b / tilesX) % tilesY, tz = b / (tilesX * tilesY);
+  int i0 = tx * TX, j0 = ty * TY, k0 = tz * TZ;
+  const int nxy = g.nx * g.ny;
+  constexpr int BQ = (QN + 255) / 256, BC = (CN + 255) / 256;
+  {
+    float v[BQ];
+#pragma unroll
+    for (int u = 0; u < BQ; u++) {
+      int e = u * 256 + threadIdx.x;
+      int li = e % QX, lj = (e / QX) % QY, lk = e / (QX * QY);
+      int i = i0 - H + li, j = j0 - H + lj, k = k0 - H + lk;
+      bool in = e < QN && i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
+      v[u] = in ? qin[k * nxy + j * g.nx + i] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < BQ; u++) {
+      int e = u * 256 + threadIdx.x;
+      if (e < QN) sq[e] = v[u];
+    }
+  }
+  {
+    float r[BC], w[BC];
+    uint8_t m[BC];
+#pragma unroll
+    for (int u = 0; u < BC; u++) {
+      int e = u * 256 + threadIdx.x;
+      int li = e % CX, lj = (e / CX) % CY, lk = e / (CX * CY);
+      int i = i0 - HC + li, j = j0 - HC + lj, k = k0 - HC + lk;
+      bool in = e < CN && i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
+      int c = in ? k * nxy + j * g.nx + i : 0;
+      m[u] = in ? nbm[c] : 0;
+      r[u] = in ? rhs[c] : 0.f;
+      w[u] = in ? pw[c] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < BC; u++) {
+      int e = u * 256 + threadIdx.x;
+      if (e < CN) { srhs[e] = r[u]; spw[e] = w[u]; snb[e] = m[u]; }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int ph = 0; ph < 2 * S; ph++) {
+    int color = ph & 1;                     // red (even i+j+k) first
+    int ring = 2 * S - 1 - ph;              // cells around the tile updated in this phase
+    int lo = HC - ring;                     // region start in coefficient coordinates
+    int nX = TX + 2 * ring, nY = TY + 2 * ring, nZ = TZ + 2 * ring;
+    int hx = (nX + 1) >> 1;                 // cells of one colour per row (upper bound)
+    for (int e = threadIdx.x; e < hx * nY * nZ; e += 256) {
+      int cj = lo + (e / hx) % nY, ck = lo + e / (hx * nY);
+      // the first column of this row with the phase's colour
+      int gi0 = i0 - HC + lo, gj = j0 - HC + cj, gk = k0 - HC + ck;
+      int first = ((gi0 + gj + gk) & 1) == color ? 0 : 1;
+      int ci = lo + first + 2 * (e % hx);
+      if (ci >= lo + nX) continue;
+      int cc = (ck * CY + cj) * CX + ci;
+      unsigned msk = snb[cc];
+      if (!(msk & 0x40)) continue;
+      int qi = ((ck + 1) * QY + (cj + 1)) * QX + (ci + 1);
+      float sum = 0.f;
+      if (msk & 1) sum += sq[qi - 1];
+      if (msk & 2) sum += sq[qi + 1];
+      if (msk & 4) sum += sq[qi - QX];
+      if (msk & 8) sum += sq[qi + QX];
+      if (msk & 16) sum += sq[qi - QX * QY];
+      if (msk & 32) sum += sq[qi + QX * QY];
+      sq[qi] = sor_relax(sq[qi], sum, srhs[cc], spw[cc], om1);
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < TX * TY * TZ; e += 256) {
+    int li = e % TX, lj = (e / TX) % TY, lk = e / (TX * TY);
+    int i = i0 + li, j = j0 + lj, k = k0 + lk;
+    if (i < g.nx && j < g.ny && k < g.nz)
+      qout[k * nxy + j * g.nx + i] = sq[((lk + H) * QY + (lj + H)) * QX + (li + H)];
+  }
+}
+template __global__ void k_sor_tb<1>(Geo, int, int, const uint8_t *, const float *, float *, const float *,
+                                     const float *, const uint8_t *, float);
+template __global__ void k_sor_tb<2>(Geo, int, int, const uint8_t *, const float *, float *, const float *,
+                                     const float *, const uint8_t *, float);
+
+// per temporal-blocking tile: does it hold a fluid cell
+__global__ void k_sor_tb_mark(Geo g, int tilesX, int tilesY, int nTiles, const uint8_t *__restrict__ nbm,
+                              uint8_t *__restrict__ active) {
+  int b = blockIdx.x;
+  if (b >= nTiles) return;
+  int tx = b % tilesX, ty = (b / tilesX) % tilesY, tz = b / (tilesX * tilesY);
+  int i0 = tx * SOR_TB_X, j0 = ty * SOR_TB_Y, k0 = tz * SOR_TB_Z;
+  int any = 0;
+  for (int e = threadIdx.x; e < SOR_TB_X * SOR_TB_Y * SOR_TB_Z; e += blockDim.x) {
+    int i = i0 + e % SOR_TB_X, j = j0 + (e / SOR_TB_X) % SOR_TB_Y, k = k0 + e / (SOR_TB_X * SOR_TB_Y);
     if (i < g.nx && j < g.ny && k < g.nz && (nbm[(k * g.ny + j) * g.nx + i] & 0x40)) any = 1;
   }
   any = __syncthreads_or(any);
@@ -883,16 +1011,25 @@ __global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restri
   for (int step = 0; step < TI + TJ + TK - 2; step++) {
     int li = 1 + step - (lj - 1) - (lk - 1);
     int i = i0 + li;
-    if (row && li >= 1 && li <= TI && i < d0 && !sf[lk][lj][li]) {
+    if (row && li >= 1 && li <= TI && i < d0) {
+      // all flags and values of the face and its six neighbours are read at
+      // once (independent LDS loads), then combined in the reference's order
+      uint8_t f0 = sf[lk][lj][li];
+      uint8_t fa = sf[lk][lj][li - 1], fb = sf[lk][lj][li + 1], fc = sf[lk][lj - 1][li];
+      uint8_t fd = sf[lk][lj + 1][li], fe = sf[lk - 1][lj][li], ff = sf[lk + 1][lj][li];
+      float va = sv[lk][lj][li - 1], vb = sv[lk][lj][li + 1], vc = sv[lk][lj - 1][li];
+      float vd = sv[lk][lj + 1][li], ve = sv[lk - 1][lj][li], vf = sv[lk + 1][lj][li];
+      bool ua = i > 0 && fa, ub = i < d0 - 1 && fb, uc = j > 0 && fc;
+      bool ud = j < d1 - 1 && fd, ue = k > 0 && fe, uf = k < d2 - 1 && ff;
       float sum = 0.f;
-      int cnt = 0;
-      if (i > 0 && sf[lk][lj][li - 1]) { sum += sv[lk][lj][li - 1]; cnt++; }
-      if (i < d0 - 1 && sf[lk][lj][li + 1]) { sum += sv[lk][lj][li + 1]; cnt++; }
-      if (j > 0 && sf[lk][lj - 1][li]) { sum += sv[lk][lj - 1][li]; cnt++; }
-      if (j < d1 - 1 && sf[lk][lj + 1][li]) { sum += sv[lk][lj + 1][li]; cnt++; }
-      if (k > 0 && sf[lk - 1][lj][li]) { sum += sv[lk - 1][lj][li]; cnt++; }
-      if (k < d2 - 1 && sf[lk + 1][lj][li]) { sum += sv[lk + 1][lj][li]; cnt++; }
-      if (cnt > 0) {
+      if (ua) sum += va;
+      if (ub) sum += vb;
+      if (uc) sum += vc;
+      if (ud) sum += vd;
+      if (ue) sum += ve;
+      if (uf) sum += vf;
+      int cnt = (int)ua + (int)ub + (int)uc + (int)ud + (int)ue + (int)uf;
+      if (!f0 && cnt > 0) {
         float v = sum / (float)cnt;
         sv[lk][lj][li] = v;
         sf[lk][lj][li] = 2;

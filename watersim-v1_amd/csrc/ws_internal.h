@@ -25,6 +25,7 @@ constexpr float kFixInvF = 1.0f / 1125899906842624.0f;
 constexpr int P2G_T = 8;
 constexpr int EX_TI = 32, EX_TJ = 8, EX_TK = 8;
 constexpr int SOR_TX = 32, SOR_TY = 8, SOR_TZ = 8;
+constexpr int SOR_TB_X = 16, SOR_TB_Y = 8, SOR_TB_Z = 8;  // temporal-blocking tile
 
 // Scalars that live on the device so that substeps can run back to back
 // without host round trips (dt law, gravity ramp, reductions).
@@ -107,6 +108,8 @@ struct Ctx {
   uint8_t *cellType = nullptr, *nbm = nullptr, *dg = nullptr;
   float *q = nullptr, *q2 = nullptr, *rhs = nullptr, *pw = nullptr;
   uint8_t *sorActive = nullptr;  // per SOR tile: holds a fluid cell
+  uint8_t *tbActive = nullptr;   // per temporal-blocking tile: holds a fluid cell
+  int tbTiles[3] = {0, 0, 0};
   // fluid cells per colour, in cell order (red from 0, black from sorHalf):
   // code = cell | neighbour mask << 26, with rhs, omega/diag and diag per entry
   unsigned long long *sorOff = nullptr;
@@ -114,10 +117,12 @@ struct Ctx {
   float *sorRhs = nullptr, *sorPw = nullptr;
   uint8_t *sorDg = nullptr;
   long sorHalf = 0, nRed = 0, nBlack = 0;
+  long tbLaunchesPending = 0;
   int sorTiles[3] = {0, 0, 0};
-  // 2: one launch per colour over the fluid-cell lists, 1: LDS-tiled full
-  // sweeps, 0: one launch per colour over the whole grid
-  int sorMode = 2;
+  // 3: two sweeps per launch over LDS tiles (temporal blocking), 2: one
+  // launch per colour over the fluid-cell lists, 1: one LDS-tiled sweep per
+  // launch, 0: one launch per colour over the whole grid
+  int sorMode = 3;
   unsigned *cellCount = nullptr, *cellStart = nullptr;
   float *vort = nullptr;  // 10*C scratch for vorticity confinement
   // faces (u | v | w contiguous in one allocation each)

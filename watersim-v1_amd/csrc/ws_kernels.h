@@ -68,6 +68,10 @@ __global__ void k_sor_list(Geo g, int n, const unsigned *code, const float *lrhs
                            float om1);
 __global__ void k_residual_list(Geo g, const DevScalars *ds, long half, const unsigned *code, const float *lrhs,
                                 const uint8_t *ldg, const float *q, double *partials);
+template <int S>
+__global__ void k_sor_tb(Geo g, int tilesX, int tilesY, const uint8_t *active, const float *qin, float *qout,
+                         const float *rhs, const float *pw, const uint8_t *nbm, float om1);
+__global__ void k_sor_tb_mark(Geo g, int tilesX, int tilesY, int nTiles, const uint8_t *nbm, uint8_t *active);
 __global__ void k_residual_final(const double *partials, int n, DevScalars *s);
 __global__ void k_project(Geo g, const uint8_t *ct, const float *phi, float *vel, int do_clamp, float vcap,
                           DevScalars *s);
