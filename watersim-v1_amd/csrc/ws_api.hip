@@ -8,6 +8,7 @@
 #include <cstring>
 
 #include "ws_frame.h"
+#include "ws_learn.h"
 
 using namespace ws;
 
@@ -318,10 +319,15 @@ int st_pressure_solve(ws_ctx *c) {
       take_counts(c);
     }
   }
+  // the learned warm start: record this solve's inputs for training and/or
+  // start the sweeps from the network's prediction
+  if (c->recordSamples && (r = net_record_inputs(c))) return r;
+  if (c->warmStart && (r = net_warm_start(c))) return r;
   if (c->pressureTol <= 0) {
     if ((r = sor_sweeps(c, c->pressureIters))) return r;
     c->itersDone = c->pressureIters;
     if (c->measureResidual && (r = residual(c, &c->residual))) return r;
+    if (c->recordSamples && (r = net_record_target(c))) return r;
     return WS_OK;
   }
   // to a relative residual: sweep in batches, checking in between; stop
@@ -350,6 +356,7 @@ int st_pressure_solve(ws_ctx *c) {
   }
   c->itersDone = done;
   c->residual = res;
+  if (c->recordSamples && (r = net_record_target(c))) return r;
   return WS_OK;
 }
 
@@ -662,6 +669,7 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
 void ws_destroy(ws_ctx *c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
+  net_free(c);
   void *ptrs[] = {c->staticTypes, c->gu, c->thr, c->terrainR, c->cellType, c->nbm, c->dg, c->q, c->q2, c->rhs, c->pw,
                   c->sorActive, c->tbActive, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg,
                   c->cellCount, c->cellStart, c->vort, c->vel, c->velO, c->wgt, c->scr, c->scr2, c->valid, c->dens,
@@ -690,7 +698,7 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
   SF(heatK) SF(Tamb) SF(shadeCool) SF(sunActivity) SF(evapIntensity) SF(snowT) SF(meltT) SF(cloudP)
   SF(currents) SF(Rlo) SF(Rhi) SF(simTime) SF(heatAcc)
   SI(pressureIters) SI(maxPressureIters) SI(residualEvery) SI(ceilReflect) SI(hasSun) SI(measureResidual)
-  SI(adaptiveChecks) SI(sorMode) SI(spinOn)
+  SI(adaptiveChecks) SI(sorMode) SI(spinOn) SI(warmStart) SI(recordSamples)
 #undef SF
 #undef SI
   if (!strcmp(n, "gRamp")) { c->gRamp = v; return push_ramp(c); }

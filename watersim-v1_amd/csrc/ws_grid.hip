@@ -1030,7 +1030,10 @@ __global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restri
       if (uf) sum += vf;
       int cnt = (int)ua + (int)ub + (int)uc + (int)ud + (int)ue + (int)uf;
       if (!f0 && cnt > 0) {
-        float v = sum / (float)cnt;
+        // the mean through a reciprocal (within 1.5 ulp of the quotient; the
+        // reference divides in double and rounds, so neither is exact here)
+        const float rcp[7] = {0.f, 1.f, 0.5f, 1.f / 3.f, 0.25f, 0.2f, 1.f / 6.f};
+        float v = sum * rcp[cnt];
         sv[lk][lj][li] = v;
         sf[lk][lj][li] = 2;
         long f = ((long)k * d1 + j) * d0 + i;

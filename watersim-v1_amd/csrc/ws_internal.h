@@ -57,6 +57,8 @@ struct Geo {
   float cx_lo, cy_lo, cz_lo;  // double centre minus its float rounding
 };
 
+struct Net;  // the learned warm start (ws_learn.h)
+
 struct Ctx {
   // geometry
   int nx, ny, nz, capacity;
@@ -97,6 +99,10 @@ struct Ctx {
   bool gRampDirty = true;
   bool haveStatic = false, haveTerrain = false;
   std::string err;
+  // learned pressure warm start: the network, whether the pressure solve
+  // starts from its prediction, and whether solves record training samples
+  Net *net = nullptr;
+  int warmStart = 0, recordSamples = 0;
 
   hipStream_t stream = nullptr;
   // static world
@@ -119,10 +125,11 @@ struct Ctx {
   long sorHalf = 0, nRed = 0, nBlack = 0;
   long tbLaunchesPending = 0;
   int sorTiles[3] = {0, 0, 0};
-  // 3: two sweeps per launch over LDS tiles (temporal blocking), 2: one
-  // launch per colour over the fluid-cell lists, 1: one LDS-tiled sweep per
-  // launch, 0: one launch per colour over the whole grid
-  int sorMode = 3;
+  // 2: one launch per colour over the fluid-cell lists (the fastest
+  // measured), 3: two sweeps per launch over LDS tiles (temporal blocking),
+  // 1: one LDS-tiled sweep per launch, 0: one launch per colour over the
+  // whole grid. All four give the same bits.
+  int sorMode = 2;
   unsigned *cellCount = nullptr, *cellStart = nullptr;
   float *vort = nullptr;  // 10*C scratch for vorticity confinement
   // faces (u | v | w contiguous in one allocation each)

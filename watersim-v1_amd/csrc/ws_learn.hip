@@ -1,0 +1,978 @@
+// Kernels of the learned pressure warm start: a 3-level 3-D U-Net on
+// channels-last bf16 grids, trained and run on the device. Convolutions are
+// implicit GEMMs on the gfx950 matrix cores (v_mfma_f32_16x16x32_bf16,
+// FP32 accumulation): one wavefront computes 16 voxels x all output
+// channels, the K dimension runs over (tap, input channel) in chunks of 32.
+// Weight gradients are the same product with voxels as the K dimension,
+// split over voxel chunks and summed in a fixed order (deterministic).
+#include "ws_learn.h"
+#include "../../include/watersim_learn.h"
+
+namespace ws {
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16_t f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (bf16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float((unsigned)h << 16); }
+
+__device__ __forceinline__ void tap_offset(int tap, int flip, int &dz, int &dy, int &dx) {
+  dz = tap / 9 - 1;
+  dy = (tap / 3) % 3 - 1;
+  dx = tap % 3 - 1;
+  if (flip) { dz = -dz; dy = -dy; dx = -dx; }
+}
+
+// out[v][n] = sum_k A[v][k] W[n][k] (+ bias), A[v][tap*Cin + ci] =
+// in[v + off(tap)][ci] (zero outside the grid; off negated when flip).
+// OUT 0: bf16 with ReLU, 1: bf16, 2: f32. Cin is a multiple of 8 so the
+// eight K values of a lane lie in one tap; K is a multiple of 32.
+template <int NT, int OUT>
+__global__ __launch_bounds__(256) void k_conv3(NetDims d, int Cin, int K, int flip, const bf16_t *__restrict__ in,
+                                               const bf16_t *__restrict__ W, const float *__restrict__ bias,
+                                               void *__restrict__ out) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long V = (long)d.Z * d.Y * d.X;
+  const long v0 = ((long)blockIdx.x * 4 + wv) * 16;
+  const long va = v0 + (lane & 15);
+  const bool okA = va < V;
+  const int x = (int)(va % d.X), y = (int)((va / d.X) % d.Y), z = (int)(va / ((long)d.X * d.Y));
+  const int kq = 8 * (lane >> 4);
+  f4 acc[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; nt++) acc[nt] = f4{0.f, 0.f, 0.f, 0.f};
+  const bf16_t *wrow = W + (long)(lane & 15) * K;
+  for (int kc = 0; kc < K; kc += 32) {
+    const int kk = kc + kq;
+    const int tap = kk / Cin, ci = kk - tap * Cin;
+    bf8 a = {};
+    if (okA && tap < 27) {
+      int dz, dy, dx;
+      tap_offset(tap, flip, dz, dy, dx);
+      const int zz = z + dz, yy = y + dy, xx = x + dx;
+      if (zz >= 0 && zz < d.Z && yy >= 0 && yy < d.Y && xx >= 0 && xx < d.X)
+        a = *(const bf8 *)(in + (((long)zz * d.Y + yy) * d.X + xx) * Cin + ci);
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) {
+      const bf8 b = *(const bf8 *)(wrow + (long)nt * 16 * K + kk);
+      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nt], 0, 0, 0);
+    }
+  }
+  constexpr int Cout = NT * 16;
+#pragma unroll
+  for (int nt = 0; nt < NT; nt++) {
+    const int n = nt * 16 + (lane & 15);
+    const float bn = bias ? bias[n] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const long v = v0 + 4 * (lane >> 4) + i;
+      if (v >= V) continue;
+      float val = acc[nt][i] + bn;
+      if (OUT == 0) val = fmaxf(val, 0.f);
+      if (OUT == 2) ((float *)out)[v * Cout + n] = val;
+      else ((bf16_t *)out)[v * Cout + n] = f2bf(val);
+    }
+  }
+}
+
+// Weight gradient partials: part[chunk][co][k] = sum over the chunk's
+// voxels v of gz[v][co] * x[v + off(tap)][ci], k = tap*Cin + ci. The
+// product has M = output channels, N = k, and voxels as its K dimension;
+// the four waves of a block take interleaved 32-voxel slices of the chunk
+// and are summed through LDS in a fixed order.
+template <int MT>
+__global__ __launch_bounds__(256) void k_wgrad(NetDims d, int Cin, int K, int vchunk, const bf16_t *__restrict__ x,
+                                               const bf16_t *__restrict__ gz, float *__restrict__ part) {
+  constexpr int Cout = MT * 16;
+  __shared__ float red[4][MT][4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = blockIdx.x * 16 + (lane & 15);
+  const int tap = col / Cin, ci = col - tap * Cin;
+  int dz = 0, dy = 0, dx = 0;
+  const bool colOk = tap < 27;
+  if (colOk) tap_offset(tap, 0, dz, dy, dx);
+  const long V = (long)d.Z * d.Y * d.X;
+  const long vbeg = (long)blockIdx.y * vchunk;
+  const long vend = vbeg + vchunk < V ? vbeg + vchunk : V;
+  f4 acc[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++) acc[mt] = f4{0.f, 0.f, 0.f, 0.f};
+  for (long v0 = vbeg + 32 * wv; v0 < vend; v0 += 128) {
+    bf8 b;
+    bf8 a[MT];
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      const long v = v0 + 8 * (lane >> 4) + t;
+      bf16_t xv = 0;
+      if (v < vend && colOk) {
+        const int xx = (int)(v % d.X) + dx, yy = (int)((v / d.X) % d.Y) + dy, zz = (int)(v / ((long)d.X * d.Y)) + dz;
+        if (zz >= 0 && zz < d.Z && yy >= 0 && yy < d.Y && xx >= 0 && xx < d.X)
+          xv = x[(((long)zz * d.Y + yy) * d.X + xx) * Cin + ci];
+      }
+      b[t] = __builtin_bit_cast(__bf16, xv);
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++) {
+        const bf16_t gv = v < vend ? gz[v * Cout + mt * 16 + (lane & 15)] : (bf16_t)0;
+        a[mt][t] = __builtin_bit_cast(__bf16, gv);
+      }
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b, acc[mt], 0, 0, 0);
+  }
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) red[wv][mt][i][lane] = acc[mt][i];
+  __syncthreads();
+  if (wv == 0) {
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const float s = red[0][mt][i][lane] + red[1][mt][i][lane] + red[2][mt][i][lane] + red[3][mt][i][lane];
+        const int co = mt * 16 + 4 * (lane >> 4) + i;
+        part[((long)blockIdx.y * Cout + co) * K + col] = s;
+      }
+  }
+}
+
+// grad[W(co, tap, ci)] (+)= sum over chunks of part[chunk][co][tap*Cin_p + ci]
+// for ci < Cin (the real input channels), tap < taps; fixed chunk order.
+__global__ void k_wgrad_reduce(int nchunk, int Cout, int K, int Cin_p, int Cin, int taps, const float *__restrict__ part,
+                               float *__restrict__ grad, int accumulate) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long n = (long)Cout * taps * Cin;
+  if (e >= n) return;
+  int ci = (int)(e % Cin), tap = (int)((e / Cin) % taps), co = (int)(e / ((long)Cin * taps));
+  int k = tap * Cin_p + ci;
+  float s = 0.f;
+  for (int c = 0; c < nchunk; c++) s += part[((long)c * Cout + co) * K + k];
+  grad[e] = accumulate ? grad[e] + s : s;
+}
+
+// bias gradient: gb[c] (+)= sum_v g[v][c], per-block partials then a fixed-order sum
+__global__ void k_colsum_part(long V, int C, const bf16_t *__restrict__ g, float *__restrict__ part) {
+  __shared__ float s[256];
+  int c = blockIdx.y;
+  float acc = 0.f;
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x)
+    acc += bf2f(g[v * C + c]);
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(long)c * gridDim.x + blockIdx.x] = s[0];
+}
+__global__ void k_colsum_final(int C, int nb, const float *__restrict__ part, float *__restrict__ gb, int accumulate) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  float s = 0.f;
+  for (int b = 0; b < nb; b++) s += part[(long)c * nb + b];
+  gb[c] = accumulate ? gb[c] + s : s;
+}
+
+// weights: master FP32 W[co][tap][ci] -> forward bf16 Wf[co][K] (k = tap*Cin_p
+// + ci) and input-gradient bf16 Wd[ci][Kd] (k = tap*Cout + co), zero padding
+__global__ void k_pack_weights(int Cout, int Cin, int Cin_p, int K, int Kd, const float *__restrict__ w,
+                               bf16_t *__restrict__ wf, bf16_t *__restrict__ wd) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long nf = (long)Cout * K;
+  if (e < nf) {
+    int co = (int)(e / K), k = (int)(e % K);
+    int tap = k / Cin_p, ci = k - tap * Cin_p;
+    float v = (tap < 27 && ci < Cin) ? w[((long)co * 27 + tap) * Cin + ci] : 0.f;
+    wf[e] = f2bf(v);
+  }
+  if (wd) {
+    long nd = (long)Cin * Kd;
+    if (e < nd) {
+      int ci = (int)(e / Kd), k = (int)(e % Kd);
+      int tap = k / Cout, co = k - tap * Cout;
+      float v = tap < 27 ? w[((long)co * 27 + tap) * Cin + ci] : 0.f;
+      wd[e] = f2bf(v);
+    }
+  }
+}
+
+// 2x2x2 average pooling (average in f32, stored bf16)
+__global__ void k_pool(NetDims dc, int C, const bf16_t *__restrict__ fine, bf16_t *__restrict__ coarse) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long Vc = (long)dc.Z * dc.Y * dc.X;
+  if (e >= Vc * C) return;
+  int c = (int)(e % C);
+  long v = e / C;
+  int x = (int)(v % dc.X), y = (int)((v / dc.X) % dc.Y), z = (int)(v / ((long)dc.X * dc.Y));
+  int FX = 2 * dc.X, FY = 2 * dc.Y;
+  float s = 0.f;
+  for (int q = 0; q < 8; q++) {
+    int zz = 2 * z + (q >> 2), yy = 2 * y + ((q >> 1) & 1), xx = 2 * x + (q & 1);
+    s += bf2f(fine[(((long)zz * FY + yy) * FX + xx) * C + c]);
+  }
+  coarse[e] = f2bf(s * 0.125f);
+}
+
+// nearest upsampling x2 of the coarse tensor concatenated with the skip:
+// out[v][0..Cu) = coarse[parent(v)], out[v][Cu..Cu+Cs) = skip[v]
+__global__ void k_up_concat(NetDims df, int Cu, int Cs, const bf16_t *__restrict__ coarse,
+                            const bf16_t *__restrict__ skip, bf16_t *__restrict__ out) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long Vf = (long)df.Z * df.Y * df.X;
+  int C = Cu + Cs;
+  if (e >= Vf * C) return;
+  int c = (int)(e % C);
+  long v = e / C;
+  if (c >= Cu) { out[e] = skip[v * Cs + (c - Cu)]; return; }
+  int x = (int)(v % df.X), y = (int)((v / df.X) % df.Y), z = (int)(v / ((long)df.X * df.Y));
+  long p = ((long)(z >> 1) * (df.Y >> 1) + (y >> 1)) * (df.X >> 1) + (x >> 1);
+  out[e] = coarse[p * Cu + c];
+}
+
+// gz = bf16(g * [y > 0]), g a channel slice [off, off + C) of a row of Cs floats
+__global__ void k_relu_back(long V, int C, int Cs, int off, const float *__restrict__ g, const bf16_t *__restrict__ y,
+                            bf16_t *__restrict__ gz) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= V * C) return;
+  long v = e / C;
+  int c = (int)(e % C);
+  gz[e] = bf2f(y[e]) > 0.f ? f2bf(g[v * Cs + off + c]) : (bf16_t)0;
+}
+
+// gradient through nearest upsampling: the sum over the eight children of
+// channels [0, C) of a fine gradient row of Cs floats, then ReLU of yc
+__global__ void k_upback_relu(NetDims dc, int C, int Cs, const float *__restrict__ gf, const bf16_t *__restrict__ yc,
+                              bf16_t *__restrict__ gz) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long Vc = (long)dc.Z * dc.Y * dc.X;
+  if (e >= Vc * C) return;
+  int c = (int)(e % C);
+  long v = e / C;
+  int x = (int)(v % dc.X), y = (int)((v / dc.X) % dc.Y), z = (int)(v / ((long)dc.X * dc.Y));
+  int FX = 2 * dc.X, FY = 2 * dc.Y;
+  float s = 0.f;
+  for (int q = 0; q < 8; q++) {
+    int zz = 2 * z + (q >> 2), yy = 2 * y + ((q >> 1) & 1), xx = 2 * x + (q & 1);
+    s += gf[(((long)zz * FY + yy) * FX + xx) * Cs + c];
+  }
+  gz[e] = bf2f(yc[e]) > 0.f ? f2bf(s) : (bf16_t)0;
+}
+
+// gradient into a pooled layer's output: its pooled child's gradient / 8
+// plus the skip connection's gradient (a channel slice of a row of Cs),
+// then ReLU of y
+__global__ void k_poolback_add_relu(NetDims df, int C, const float *__restrict__ gc, int Cs, int off,
+                                    const float *__restrict__ gskip, const bf16_t *__restrict__ y,
+                                    bf16_t *__restrict__ gz) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long Vf = (long)df.Z * df.Y * df.X;
+  if (e >= Vf * C) return;
+  int c = (int)(e % C);
+  long v = e / C;
+  int x = (int)(v % df.X), yy = (int)((v / df.X) % df.Y), z = (int)(v / ((long)df.X * df.Y));
+  long p = ((long)(z >> 1) * (df.Y >> 1) + (yy >> 1)) * (df.X >> 1) + (x >> 1);
+  float g = gc[p * C + c] * 0.125f + gskip[v * Cs + off + c];
+  gz[e] = bf2f(y[e]) > 0.f ? f2bf(g) : (bf16_t)0;
+}
+
+// 1x1x1 head: out[v] = sum_c w[c] a[v][c] + b
+__global__ void k_head_fwd(long V, const bf16_t *__restrict__ a, const float *__restrict__ w, float *__restrict__ out) {
+  long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  float s = 0.f;
+  for (int c = 0; c < 16; c++) s += bf2f(f2bf(w[c])) * bf2f(a[v * 16 + c]);
+  out[v] = s + w[16];
+}
+
+// head backward: gz9[v][c] = bf16(gout[v] w[c] [a9 > 0]); per-block partial
+// sums of gout * a9 (weights) and gout (bias)
+__global__ void k_head_back(long V, const bf16_t *__restrict__ a, const float *__restrict__ w,
+                            const float *__restrict__ gout, bf16_t *__restrict__ gz, float *__restrict__ part) {
+  __shared__ float s[17][256];
+  float acc[17];
+  for (int c = 0; c < 17; c++) acc[c] = 0.f;
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
+    float go = gout[v];
+    for (int c = 0; c < 16; c++) {
+      float av = bf2f(a[v * 16 + c]);
+      acc[c] += go * av;
+      gz[v * 16 + c] = av > 0.f ? f2bf(go * bf2f(f2bf(w[c]))) : (bf16_t)0;
+    }
+    acc[16] += go;
+  }
+  for (int c = 0; c < 17; c++) s[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int c = 0; c < 17; c++) s[c][threadIdx.x] += s[c][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 17) part[(long)blockIdx.x * 17 + threadIdx.x] = s[threadIdx.x][0];
+}
+__global__ void k_head_back_final(int nb, const float *__restrict__ part, float *__restrict__ g, int accumulate) {
+  int c = threadIdx.x;
+  if (c >= 17) return;
+  float s = 0.f;
+  for (int b = 0; b < nb; b++) s += part[(long)b * 17 + c];
+  g[c] = accumulate ? g[c] + s : s;
+}
+
+// masked squared error normalised by the target energy: block partials of
+// sum m (y - t)^2 and sum m t^2
+__global__ void k_loss_part(long V, const float *__restrict__ y, const float *__restrict__ t,
+                            const uint8_t *__restrict__ m, double *__restrict__ part) {
+  __shared__ double se[256], st[256];
+  double a = 0, b = 0;
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
+    if (!m[v]) continue;
+    double d = (double)y[v] - t[v];
+    a += d * d;
+    b += (double)t[v] * t[v];
+  }
+  se[threadIdx.x] = a;
+  st[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) { se[threadIdx.x] += se[threadIdx.x + o]; st[threadIdx.x] += st[threadIdx.x + o]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = se[0]; part[2 * blockIdx.x + 1] = st[0]; }
+}
+__global__ void k_loss_final(int nb, const double *__restrict__ part, double *__restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double a = 0, b = 0;
+  for (int i = 0; i < nb; i++) { a += part[2 * i]; b += part[2 * i + 1]; }
+  out[0] = a;
+  out[1] = b;
+}
+// d loss / d y = 2 m (y - t) / sum m t^2
+__global__ void k_loss_grad(long V, const float *__restrict__ y, const float *__restrict__ t,
+                            const uint8_t *__restrict__ m, const double *__restrict__ sums, float *__restrict__ g) {
+  long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  double den = sums[1] > 1e-30 ? sums[1] : 1e-30;
+  g[v] = m[v] ? (float)(2.0 * ((double)y[v] - t[v]) / den) : 0.f;
+}
+
+// Adam on the flat parameter vector (FP32 master weights)
+__global__ void k_adam(long n, float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m1,
+                       float *__restrict__ m2, float lr, float b1, float b2, float eps, float c1, float c2,
+                       float gscale) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  float gv = g[e] * gscale;
+  float a = b1 * m1[e] + (1.f - b1) * gv;
+  float b = b2 * m2[e] + (1.f - b2) * gv * gv;
+  m1[e] = a;
+  m2[e] = b;
+  w[e] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+// host-supplied input channels (f32 [V][5]) -> bf16 [V][8]
+__global__ void k_pack_input(long V, const float *__restrict__ in5, bf16_t *__restrict__ x) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= V * 8) return;
+  long v = e >> 3;
+  int c = (int)(e & 7);
+  x[e] = c < 5 ? f2bf(in5[v * 5 + c]) : (bf16_t)0;
+}
+
+// a crop of a bank sample: x [S^3][8], target, mask (fluid channel)
+__global__ void k_crop(NetDims full, NetDims cd, int oz, int oy, int ox, const bf16_t *__restrict__ sx,
+                       const float *__restrict__ st, bf16_t *__restrict__ x, float *__restrict__ t,
+                       uint8_t *__restrict__ m) {
+  long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long Vc = (long)cd.Z * cd.Y * cd.X;
+  if (v >= Vc) return;
+  int xx = (int)(v % cd.X) + ox, yy = (int)((v / cd.X) % cd.Y) + oy, zz = (int)(v / ((long)cd.X * cd.Y)) + oz;
+  long f = ((long)zz * full.Y + yy) * full.X + xx;
+  const uint4 *src = (const uint4 *)(sx + f * 8);
+  *(uint4 *)(x + v * 8) = *src;
+  t[v] = st[f];
+  m[v] = bf2f(sx[f * 8 + 2]) > 0.5f ? 1 : 0;
+}
+
+// ---- the solver side: inputs from the pressure system, the correction
+// initial residual r0 = rhs - A q over fluid cells and FP64 sums for the
+// two scales (sum r0^2, sum q^2, fluid count)
+__global__ void k_ws_residual(NetDims d, const uint8_t *__restrict__ ct, const uint8_t *__restrict__ nbm,
+                              const uint8_t *__restrict__ dg, const float *__restrict__ rhs,
+                              const float *__restrict__ q, float *__restrict__ r0, double *__restrict__ part) {
+  __shared__ double sa[256], sb[256], sc[256];
+  double a = 0, b = 0, n = 0;
+  long C = (long)d.Z * d.Y * d.X;
+  int nx = d.X, nxy = d.X * d.Y;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < C; c += (long)gridDim.x * blockDim.x) {
+    unsigned mk = nbm[c];
+    if (!(mk & 0x40)) { r0[c] = 0.f; continue; }
+    float sum = 0.f;
+    if (mk & 1) sum += q[c - 1];
+    if (mk & 2) sum += q[c + 1];
+    if (mk & 4) sum += q[c - nx];
+    if (mk & 8) sum += q[c + nx];
+    if (mk & 16) sum += q[c - nxy];
+    if (mk & 32) sum += q[c + nxy];
+    float r = rhs[c] - ((float)dg[c] * q[c] - sum);
+    r0[c] = r;
+    a += (double)r * r;
+    b += (double)q[c] * q[c];
+    n += 1;
+  }
+  sa[threadIdx.x] = a; sb[threadIdx.x] = b; sc[threadIdx.x] = n;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      sa[threadIdx.x] += sa[threadIdx.x + o]; sb[threadIdx.x] += sb[threadIdx.x + o]; sc[threadIdx.x] += sc[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { part[3 * blockIdx.x] = sa[0]; part[3 * blockIdx.x + 1] = sb[0]; part[3 * blockIdx.x + 2] = sc[0]; }
+}
+__global__ void k_ws_scales(int nb, const double *__restrict__ part, float *__restrict__ scales) {
+  if (threadIdx.x != 0) return;
+  double a = 0, b = 0, n = 0;
+  for (int i = 0; i < nb; i++) { a += part[3 * i]; b += part[3 * i + 1]; n += part[3 * i + 2]; }
+  double s = n > 0 ? sqrt(a / n) : 1.0, sq = n > 0 ? sqrt(b / n) : 1.0;
+  scales[0] = s > 0 ? (float)s : 1.f;
+  scales[1] = sq > 0 ? (float)sq : 1.f;
+}
+// the five input channels [r0/s, air, fluid, solid, q/sq] as bf16 [V][8]
+__global__ void k_ws_inputs(long C, const uint8_t *__restrict__ ct, const float *__restrict__ r0,
+                            const float *__restrict__ q, const float *__restrict__ scales, bf16_t *__restrict__ x) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  uint8_t t = ct[c];
+  float s = scales[0], sq = scales[1];
+  bool fl = t == FLUID;
+  bf16_t v[8] = {fl ? f2bf(r0[c] / s) : (bf16_t)0, f2bf(t == AIR ? 1.f : 0.f), f2bf(fl ? 1.f : 0.f),
+                 f2bf(t == SOLID ? 1.f : 0.f), fl ? f2bf(q[c] / sq) : (bf16_t)0, 0, 0, 0};
+  for (int i = 0; i < 8; i++) x[c * 8 + i] = v[i];
+}
+// q += s y on fluid cells (the prediction of the correction)
+__global__ void k_ws_apply(long C, const uint8_t *__restrict__ ct, const float *__restrict__ y,
+                           const float *__restrict__ scales, float *__restrict__ q) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  if (ct[c] == FLUID) q[c] = q[c] + scales[0] * y[c];
+}
+// training target of a recorded sample: (q* - q_prev) / s on fluid cells
+__global__ void k_ws_target(long C, const uint8_t *__restrict__ ct, const float *__restrict__ q,
+                            const float *__restrict__ qprev, const float *__restrict__ scales,
+                            float *__restrict__ t) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  t[c] = ct[c] == FLUID ? (q[c] - qprev[c]) / scales[0] : 0.f;
+}
+
+// explicit instantiations
+#define CONV(NT, OUT)                                                                                     \
+  template __global__ void k_conv3<NT, OUT>(NetDims, int, int, int, const bf16_t *, const bf16_t *, const float *, \
+                                            void *);
+CONV(1, 0) CONV(2, 0) CONV(4, 0) CONV(1, 2) CONV(2, 2) CONV(3, 2) CONV(4, 2) CONV(6, 2)
+#undef CONV
+template __global__ void k_wgrad<1>(NetDims, int, int, int, const bf16_t *, const bf16_t *, float *);
+template __global__ void k_wgrad<2>(NetDims, int, int, int, const bf16_t *, const bf16_t *, float *);
+template __global__ void k_wgrad<4>(NetDims, int, int, int, const bf16_t *, const bf16_t *, float *);
+
+// ====================================================================== host
+namespace {
+
+const int kLayerDef[kNetLayers][4] = {  // cin, cinp, cout, taps (oracle/unet_ref.py LAYERS)
+    {5, 8, 16, 27},   {16, 16, 16, 27}, {16, 16, 32, 27}, {32, 32, 32, 27}, {32, 32, 64, 27}, {64, 64, 64, 27},
+    {96, 96, 32, 27}, {32, 32, 32, 27}, {48, 48, 16, 27}, {16, 16, 16, 27}, {16, 16, 1, 1}};
+
+int round32(int k) { return (k + 31) / 32 * 32; }
+
+template <typename T>
+int nalloc(ws_ctx *c, Net *n, T **p, size_t count) {
+  hipError_t e = hipMalloc((void **)p, sizeof(T) * (count ? count : 1));
+  if (e != hipSuccess) {
+    c->err = std::string("hipMalloc (warm-start network): ") + hipGetErrorString(e);
+    return WS_EHIP;
+  }
+  n->owned.push_back(*p);
+  return WS_OK;
+}
+
+void release(Net *n, void *p) {
+  if (!p) return;
+  (void)hipFree(p);
+  for (auto &q : n->owned)
+    if (q == p) q = nullptr;
+}
+
+long vox(NetDims d) { return (long)d.Z * d.Y * d.X; }
+NetDims half(NetDims d) { return NetDims{d.Z / 2, d.Y / 2, d.X / 2}; }
+
+template <int NT, int OUT>
+void conv(ws_ctx *c, NetDims d, int cin, int K, int flip, const bf16_t *in, const bf16_t *W, const float *b, void *out) {
+  long V = vox(d);
+  hipLaunchKernelGGL((k_conv3<NT, OUT>), dim3(blocks_for(V, 64)), dim3(256), 0, c->stream, d, cin, K, flip, in, W, b,
+                     out);
+}
+
+// forward conv of layer l (bf16 + ReLU)
+void conv_fwd(ws_ctx *c, int l, NetDims d, const bf16_t *in, bf16_t *out) {
+  Net *n = c->net;
+  const NetLayer &L = n->L[l];
+  const float *b = n->w + L.boff;
+  switch (L.cout / 16) {
+    case 1: conv<1, 0>(c, d, L.cinp, L.K, 0, in, L.wf, b, out); break;
+    case 2: conv<2, 0>(c, d, L.cinp, L.K, 0, in, L.wf, b, out); break;
+    default: conv<4, 0>(c, d, L.cinp, L.K, 0, in, L.wf, b, out); break;
+  }
+}
+
+// input gradient of layer l: f32 [V][cinp] from gz [V][cout]
+void conv_dgrad(ws_ctx *c, int l, NetDims d, const bf16_t *gz, float *gx) {
+  Net *n = c->net;
+  const NetLayer &L = n->L[l];
+  switch (L.cinp / 16) {
+    case 1: conv<1, 2>(c, d, L.cout, L.Kd, 1, gz, L.wd, nullptr, gx); break;
+    case 2: conv<2, 2>(c, d, L.cout, L.Kd, 1, gz, L.wd, nullptr, gx); break;
+    case 3: conv<3, 2>(c, d, L.cout, L.Kd, 1, gz, L.wd, nullptr, gx); break;
+    case 4: conv<4, 2>(c, d, L.cout, L.Kd, 1, gz, L.wd, nullptr, gx); break;
+    default: conv<6, 2>(c, d, L.cout, L.Kd, 1, gz, L.wd, nullptr, gx); break;
+  }
+}
+
+// weight and bias gradients of layer l from its input x and gz
+int conv_wgrad(ws_ctx *c, int l, NetDims d, const bf16_t *x, const bf16_t *gz, int accumulate) {
+  Net *n = c->net;
+  const NetLayer &L = n->L[l];
+  long V = vox(d);
+  int vchunk = 4096;
+  while ((V + vchunk - 1) / vchunk > 64) vchunk *= 2;
+  int nchunk = (int)((V + vchunk - 1) / vchunk);
+  if ((long)nchunk * L.cout * L.K > n->partN) {
+    c->err = "warm-start network: gradient scratch too small";
+    return WS_EINVAL;
+  }
+  dim3 grid(L.K / 16, nchunk);
+  switch (L.cout / 16) {
+    case 1: hipLaunchKernelGGL(k_wgrad<1>, grid, dim3(256), 0, c->stream, d, L.cinp, L.K, vchunk, x, gz, n->part); break;
+    case 2: hipLaunchKernelGGL(k_wgrad<2>, grid, dim3(256), 0, c->stream, d, L.cinp, L.K, vchunk, x, gz, n->part); break;
+    default: hipLaunchKernelGGL(k_wgrad<4>, grid, dim3(256), 0, c->stream, d, L.cinp, L.K, vchunk, x, gz, n->part); break;
+  }
+  long nw = (long)L.cout * L.taps * L.cin;
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks_for(nw, 256)), dim3(256), 0, c->stream, nchunk, L.cout, L.K, L.cinp,
+                     L.cin, L.taps, n->part, n->g + L.woff, accumulate);
+  // bias: column sums of gz (the scratch after the weight partials)
+  float *bpart = n->part + (long)nchunk * L.cout * L.K;
+  hipLaunchKernelGGL(k_colsum_part, dim3(64, L.cout), dim3(256), 0, c->stream, V, L.cout, gz, bpart);
+  hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(64), 0, c->stream, L.cout, 64, bpart, n->g + L.boff, accumulate);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+}  // namespace
+
+long net_param_count() {
+  long p = 0;
+  for (int l = 0; l < kNetLayers; l++) p += (long)kLayerDef[l][2] * kLayerDef[l][3] * kLayerDef[l][0] + kLayerDef[l][2];
+  return p;
+}
+
+int net_ensure(ws_ctx *c) {
+  if (c->net) return WS_OK;
+  Net *n = new Net();
+  c->net = n;
+  long off = 0;
+  for (int l = 0; l < kNetLayers; l++) {
+    NetLayer &L = n->L[l];
+    L.cin = kLayerDef[l][0]; L.cinp = kLayerDef[l][1]; L.cout = kLayerDef[l][2]; L.taps = kLayerDef[l][3];
+    L.woff = off;
+    off += (long)L.cout * L.taps * L.cin;
+    L.boff = off;
+    off += L.cout;
+    L.K = round32(27 * L.cinp);
+    L.Kd = round32(27 * L.cout);
+  }
+  n->nparam = off;
+  int r;
+  if ((r = nalloc(c, n, &n->w, off)) || (r = nalloc(c, n, &n->g, off)) || (r = nalloc(c, n, &n->m1, off)) ||
+      (r = nalloc(c, n, &n->m2, off)) || (r = nalloc(c, n, &n->sums, 4096)) || (r = nalloc(c, n, &n->scales, 4)) ||
+      (r = nalloc(c, n, &n->rpart, 3 * 1024)))
+    return r;
+  WS_HIP(hipMemset(n->w, 0, sizeof(float) * off));
+  WS_HIP(hipMemset(n->m1, 0, sizeof(float) * off));
+  WS_HIP(hipMemset(n->m2, 0, sizeof(float) * off));
+  for (int l = 0; l < kNetLayers - 1; l++) {
+    NetLayer &L = n->L[l];
+    if ((r = nalloc(c, n, &L.wf, (size_t)L.cout * L.K))) return r;
+    if (l > 0 && (r = nalloc(c, n, &L.wd, (size_t)L.cinp * L.Kd))) return r;
+  }
+  return WS_OK;
+}
+
+int net_pack(ws_ctx *c) {
+  Net *n = c->net;
+  for (int l = 0; l < kNetLayers - 1; l++) {
+    NetLayer &L = n->L[l];
+    long cnt = (long)L.cout * L.K;
+    long cd = (long)L.cinp * L.Kd;
+    long m = cnt > cd ? cnt : cd;
+    hipLaunchKernelGGL(k_pack_weights, dim3(blocks_for(m, 256)), dim3(256), 0, c->stream, L.cout, L.cin, L.cinp, L.K,
+                       L.Kd, n->w + L.woff, L.wf, l > 0 ? L.wd : nullptr);
+  }
+  WS_LAUNCHED();
+  n->packed = true;
+  return WS_OK;
+}
+
+// activation buffers for grids of up to V voxels (three levels)
+static int net_buffers(ws_ctx *c, long V) {
+  Net *n = c->net;
+  if (V <= n->capV) return WS_OK;
+  bf16_t **bb[] = {&n->x0, &n->a0, &n->s1, &n->p1, &n->a2, &n->s2, &n->p2, &n->a4, &n->a5, &n->u1,
+                   &n->a6, &n->a7, &n->u2, &n->a8, &n->a9, &n->gz};
+  for (auto p : bb) { release(n, *p); *p = nullptr; }
+  float **ff[] = {&n->out, &n->gout, &n->tgt, &n->gA, &n->gB, &n->gC, &n->gD, &n->gE, &n->gF, &n->gG, &n->part};
+  for (auto p : ff) { release(n, *p); *p = nullptr; }
+  release(n, n->mask);
+  n->mask = nullptr;
+  long h = V / 8, q = V / 64;
+  int r;
+  if ((r = nalloc(c, n, &n->x0, V * 8)) || (r = nalloc(c, n, &n->a0, V * 16)) || (r = nalloc(c, n, &n->s1, V * 16)) ||
+      (r = nalloc(c, n, &n->p1, h * 16)) || (r = nalloc(c, n, &n->a2, h * 32)) || (r = nalloc(c, n, &n->s2, h * 32)) ||
+      (r = nalloc(c, n, &n->p2, q * 32)) || (r = nalloc(c, n, &n->a4, q * 64)) || (r = nalloc(c, n, &n->a5, q * 64)) ||
+      (r = nalloc(c, n, &n->u1, h * 96)) || (r = nalloc(c, n, &n->a6, h * 32)) || (r = nalloc(c, n, &n->a7, h * 32)) ||
+      (r = nalloc(c, n, &n->u2, V * 48)) || (r = nalloc(c, n, &n->a8, V * 16)) || (r = nalloc(c, n, &n->a9, V * 16)) ||
+      (r = nalloc(c, n, &n->gz, V * 16)) || (r = nalloc(c, n, &n->out, V)) || (r = nalloc(c, n, &n->gout, V)) ||
+      (r = nalloc(c, n, &n->tgt, V)) || (r = nalloc(c, n, &n->mask, V)) || (r = nalloc(c, n, &n->gA, V * 48)) ||
+      (r = nalloc(c, n, &n->gB, V * 16)) || (r = nalloc(c, n, &n->gC, h * 32)) || (r = nalloc(c, n, &n->gD, h * 96)) ||
+      (r = nalloc(c, n, &n->gE, q * 64)) || (r = nalloc(c, n, &n->gF, q * 32)) || (r = nalloc(c, n, &n->gG, h * 16)))
+    return r;
+  // weight-gradient partials: at most 64 chunks of the widest product, plus
+  // the bias partials
+  long pw = 0;
+  for (int l = 0; l < kNetLayers - 1; l++) {
+    long x = 64L * n->L[l].cout * n->L[l].K + 64L * n->L[l].cout;
+    pw = x > pw ? x : pw;
+  }
+  if ((r = nalloc(c, n, &n->part, pw))) return r;
+  n->partN = pw;
+  n->capV = V;
+  return WS_OK;
+}
+
+int net_forward(ws_ctx *c, NetDims d) {
+  Net *n = c->net;
+  if (!n->packed) { int r = net_pack(c); if (r) return r; }
+  NetDims h = half(d), q = half(h);
+  long V = vox(d), Vh = vox(h), Vq = vox(q);
+  conv_fwd(c, 0, d, n->x0, n->a0);
+  conv_fwd(c, 1, d, n->a0, n->s1);
+  hipLaunchKernelGGL(k_pool, dim3(blocks_for(Vh * 16, 256)), dim3(256), 0, c->stream, h, 16, n->s1, n->p1);
+  conv_fwd(c, 2, h, n->p1, n->a2);
+  conv_fwd(c, 3, h, n->a2, n->s2);
+  hipLaunchKernelGGL(k_pool, dim3(blocks_for(Vq * 32, 256)), dim3(256), 0, c->stream, q, 32, n->s2, n->p2);
+  conv_fwd(c, 4, q, n->p2, n->a4);
+  conv_fwd(c, 5, q, n->a4, n->a5);
+  hipLaunchKernelGGL(k_up_concat, dim3(blocks_for(Vh * 96, 256)), dim3(256), 0, c->stream, h, 64, 32, n->a5, n->s2,
+                     n->u1);
+  conv_fwd(c, 6, h, n->u1, n->a6);
+  conv_fwd(c, 7, h, n->a6, n->a7);
+  hipLaunchKernelGGL(k_up_concat, dim3(blocks_for(V * 48, 256)), dim3(256), 0, c->stream, d, 32, 16, n->a7, n->s1,
+                     n->u2);
+  conv_fwd(c, 8, d, n->u2, n->a8);
+  conv_fwd(c, 9, d, n->a8, n->a9);
+  hipLaunchKernelGGL(k_head_fwd, dim3(blocks_for(V, 256)), dim3(256), 0, c->stream, V, n->a9, n->w + n->L[10].woff,
+                     n->out);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+// gradients of the loss whose d/d out is in gout (accumulated into g when
+// accumulate); the layer inputs and outputs of the last forward are used
+int net_backward(ws_ctx *c, NetDims d, int acc) {
+  Net *n = c->net;
+  NetDims h = half(d), q = half(h);
+  long V = vox(d), Vh = vox(h), Vq = vox(q);
+  int r;
+  // head, and c9's output gradient through its ReLU
+  hipLaunchKernelGGL(k_head_back, dim3(64), dim3(256), 0, c->stream, V, n->a9, n->w + n->L[10].woff, n->gout, n->gz,
+                     n->part);
+  hipLaunchKernelGGL(k_head_back_final, dim3(1), dim3(64), 0, c->stream, 64, n->part, n->g + n->L[10].woff, acc);
+  WS_LAUNCHED();
+  // c9 (input a8)
+  if ((r = conv_wgrad(c, 9, d, n->a8, n->gz, acc))) return r;
+  conv_dgrad(c, 9, d, n->gz, n->gB);
+  hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(V * 16, 256)), dim3(256), 0, c->stream, V, 16, 16, 0, n->gB, n->a8,
+                     n->gz);
+  // c8 (input u2 = [up(a7) 32, s1 16])
+  if ((r = conv_wgrad(c, 8, d, n->u2, n->gz, acc))) return r;
+  conv_dgrad(c, 8, d, n->gz, n->gA);
+  hipLaunchKernelGGL(k_upback_relu, dim3(blocks_for(Vh * 32, 256)), dim3(256), 0, c->stream, h, 32, 48, n->gA, n->a7,
+                     n->gz);
+  // c7 (input a6)
+  if ((r = conv_wgrad(c, 7, h, n->a6, n->gz, acc))) return r;
+  conv_dgrad(c, 7, h, n->gz, n->gC);
+  hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(Vh * 32, 256)), dim3(256), 0, c->stream, Vh, 32, 32, 0, n->gC, n->a6,
+                     n->gz);
+  // c6 (input u1 = [up(a5) 64, s2 32])
+  if ((r = conv_wgrad(c, 6, h, n->u1, n->gz, acc))) return r;
+  conv_dgrad(c, 6, h, n->gz, n->gD);
+  hipLaunchKernelGGL(k_upback_relu, dim3(blocks_for(Vq * 64, 256)), dim3(256), 0, c->stream, q, 64, 96, n->gD, n->a5,
+                     n->gz);
+  // c5 (input a4)
+  if ((r = conv_wgrad(c, 5, q, n->a4, n->gz, acc))) return r;
+  conv_dgrad(c, 5, q, n->gz, n->gE);
+  hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(Vq * 64, 256)), dim3(256), 0, c->stream, Vq, 64, 64, 0, n->gE, n->a4,
+                     n->gz);
+  // c4 (input p2)
+  if ((r = conv_wgrad(c, 4, q, n->p2, n->gz, acc))) return r;
+  conv_dgrad(c, 4, q, n->gz, n->gF);
+  // c3: pooled gradient / 8 plus the skip slice [64, 96) of gD, ReLU of s2
+  hipLaunchKernelGGL(k_poolback_add_relu, dim3(blocks_for(Vh * 32, 256)), dim3(256), 0, c->stream, h, 32, n->gF, 96,
+                     64, n->gD, n->s2, n->gz);
+  if ((r = conv_wgrad(c, 3, h, n->a2, n->gz, acc))) return r;
+  conv_dgrad(c, 3, h, n->gz, n->gC);
+  hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(Vh * 32, 256)), dim3(256), 0, c->stream, Vh, 32, 32, 0, n->gC, n->a2,
+                     n->gz);
+  // c2 (input p1)
+  if ((r = conv_wgrad(c, 2, h, n->p1, n->gz, acc))) return r;
+  conv_dgrad(c, 2, h, n->gz, n->gG);
+  // c1: pooled gradient / 8 plus the skip slice [32, 48) of gA, ReLU of s1
+  hipLaunchKernelGGL(k_poolback_add_relu, dim3(blocks_for(V * 16, 256)), dim3(256), 0, c->stream, d, 16, n->gG, 48, 32,
+                     n->gA, n->s1, n->gz);
+  if ((r = conv_wgrad(c, 1, d, n->a0, n->gz, acc))) return r;
+  conv_dgrad(c, 1, d, n->gz, n->gB);
+  hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(V * 16, 256)), dim3(256), 0, c->stream, V, 16, 16, 0, n->gB, n->a0,
+                     n->gz);
+  // c0 (input x0): weights only
+  if ((r = conv_wgrad(c, 0, d, n->x0, n->gz, acc))) return r;
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+static int dims_ok(ws_ctx *c, NetDims d) {
+  if (d.Z < 4 || d.Y < 4 || d.X < 4 || (d.Z & 3) || (d.Y & 3) || (d.X & 3)) {
+    c->err = "warm-start network: grid sides must be multiples of 4";
+    return WS_EINVAL;
+  }
+  return WS_OK;
+}
+
+// loss of the last forward against tgt/mask; writes gout, returns the loss
+static int net_loss(ws_ctx *c, NetDims d, double *loss) {
+  Net *n = c->net;
+  long V = vox(d);
+  hipLaunchKernelGGL(k_loss_part, dim3(256), dim3(256), 0, c->stream, V, n->out, n->tgt, n->mask, n->sums);
+  hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(64), 0, c->stream, 256, n->sums, n->sums + 1024);
+  hipLaunchKernelGGL(k_loss_grad, dim3(blocks_for(V, 256)), dim3(256), 0, c->stream, V, n->out, n->tgt, n->mask,
+                     n->sums + 1024, n->gout);
+  WS_LAUNCHED();
+  if (loss) {
+    double h[2];
+    WS_HIP(hipMemcpyAsync(h, n->sums + 1024, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    WS_HIP(hipStreamSynchronize(c->stream));
+    *loss = h[1] > 1e-30 ? h[0] / h[1] : h[0] / 1e-30;
+  }
+  return WS_OK;
+}
+
+// ---- the solver side
+static NetDims grid_dims(ws_ctx *c) { return NetDims{c->nz, c->ny, c->nx}; }
+
+static int solver_inputs(ws_ctx *c) {
+  Net *n = c->net;
+  NetDims d = grid_dims(c);
+  long C = c->nC;
+  hipLaunchKernelGGL(k_ws_residual, dim3(1024), dim3(256), 0, c->stream, d, c->cellType, c->nbm, c->dg, c->rhs, c->q,
+                     n->r0, n->rpart);
+  hipLaunchKernelGGL(k_ws_scales, dim3(1), dim3(64), 0, c->stream, 1024, n->rpart, n->scales);
+  hipLaunchKernelGGL(k_ws_inputs, dim3(blocks_for(C, 256)), dim3(256), 0, c->stream, C, c->cellType, n->r0, c->q,
+                     n->scales, n->x0);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+static int solver_side(ws_ctx *c) {
+  int r;
+  if ((r = net_ensure(c)) || (r = dims_ok(c, grid_dims(c))) || (r = net_buffers(c, c->nC))) return r;
+  Net *n = c->net;
+  if (!n->r0 && ((r = nalloc(c, n, &n->r0, c->nC)) || (r = nalloc(c, n, &n->qprev, c->nC)))) return r;
+  return WS_OK;
+}
+
+int net_warm_start(ws_ctx *c) {
+  int r;
+  if ((r = solver_side(c)) || (r = solver_inputs(c)) || (r = net_forward(c, grid_dims(c)))) return r;
+  Net *n = c->net;
+  hipLaunchKernelGGL(k_ws_apply, dim3(blocks_for(c->nC, 256)), dim3(256), 0, c->stream, c->nC, c->cellType, n->out,
+                     n->scales, c->q);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+int net_record_inputs(ws_ctx *c) {
+  int r;
+  if ((r = solver_side(c))) return r;
+  Net *n = c->net;
+  if (n->bankCap < 1) { c->err = "recordSamples: no sample bank (ws_net_bank)"; return WS_ESTATE; }
+  if ((r = solver_inputs(c))) return r;
+  long C = c->nC;
+  WS_HIP(hipMemcpyAsync(n->bankX + (long)n->bankNext * C * 8, n->x0, sizeof(bf16_t) * C * 8, hipMemcpyDeviceToDevice,
+                        c->stream));
+  WS_HIP(hipMemcpyAsync(n->qprev, c->q, sizeof(float) * C, hipMemcpyDeviceToDevice, c->stream));
+  return WS_OK;
+}
+
+int net_record_target(ws_ctx *c) {
+  Net *n = c->net;
+  long C = c->nC;
+  hipLaunchKernelGGL(k_ws_target, dim3(blocks_for(C, 256)), dim3(256), 0, c->stream, C, c->cellType, c->q, n->qprev,
+                     n->scales, n->bankT + (long)n->bankNext * C);
+  WS_LAUNCHED();
+  n->bankNext = (n->bankNext + 1) % n->bankCap;
+  if (n->bankCount < n->bankCap) n->bankCount++;
+  return WS_OK;
+}
+
+void net_free(ws_ctx *c) {
+  Net *n = c->net;
+  if (!n) return;
+  for (void *p : n->owned)
+    if (p) (void)hipFree(p);
+  delete n;
+  c->net = nullptr;
+}
+
+}  // namespace ws
+
+// ======================================================================= ABI
+using namespace ws;
+
+extern "C" {
+
+long ws_net_param_count(void) { return net_param_count(); }
+
+int ws_net_set_params(ws_ctx *c, const float *w, long n) {
+  if (!c || !w) return WS_EINVAL;
+  int r = net_ensure(c);
+  if (r) return r;
+  if (n != c->net->nparam) { c->err = "ws_net_set_params: wrong parameter count"; return WS_EINVAL; }
+  WS_HIP(hipMemcpy(c->net->w, w, sizeof(float) * n, hipMemcpyHostToDevice));
+  c->net->packed = false;
+  return WS_OK;
+}
+
+int ws_net_get_params(ws_ctx *c, float *w, long n) {
+  if (!c || !w) return WS_EINVAL;
+  int r = net_ensure(c);
+  if (r) return r;
+  if (n != c->net->nparam) { c->err = "ws_net_get_params: wrong parameter count"; return WS_EINVAL; }
+  WS_HIP(hipMemcpy(w, c->net->w, sizeof(float) * n, hipMemcpyDeviceToHost));
+  return WS_OK;
+}
+
+int ws_net_forward(ws_ctx *c, int nz, int ny, int nx, const float *in5, float *out) {
+  if (!c || !in5 || !out) return WS_EINVAL;
+  NetDims d{nz, ny, nx};
+  int r;
+  if ((r = net_ensure(c)) || (r = dims_ok(c, d)) || (r = net_buffers(c, vox(d)))) return r;
+  Net *n = c->net;
+  long V = vox(d);
+  float *tmp = nullptr;
+  WS_HIP(hipMallocAsync((void **)&tmp, sizeof(float) * V * 5, c->stream));
+  WS_HIP(hipMemcpyAsync(tmp, in5, sizeof(float) * V * 5, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_pack_input, dim3(blocks_for(V * 8, 256)), dim3(256), 0, c->stream, V, tmp, n->x0);
+  WS_LAUNCHED();
+  if ((r = net_forward(c, d))) return r;
+  WS_HIP(hipMemcpyAsync(out, n->out, sizeof(float) * V, hipMemcpyDeviceToHost, c->stream));
+  WS_HIP(hipFreeAsync(tmp, c->stream));
+  WS_HIP(hipStreamSynchronize(c->stream));
+  return WS_OK;
+}
+
+int ws_net_grad(ws_ctx *c, int nz, int ny, int nx, const float *in5, const float *target, const uint8_t *mask,
+                float *grad, double *loss) {
+  if (!c || !in5 || !target || !mask || !grad) return WS_EINVAL;
+  NetDims d{nz, ny, nx};
+  int r;
+  if ((r = net_ensure(c)) || (r = dims_ok(c, d)) || (r = net_buffers(c, vox(d)))) return r;
+  Net *n = c->net;
+  long V = vox(d);
+  float *tmp = nullptr;
+  WS_HIP(hipMallocAsync((void **)&tmp, sizeof(float) * V * 5, c->stream));
+  WS_HIP(hipMemcpyAsync(tmp, in5, sizeof(float) * V * 5, hipMemcpyHostToDevice, c->stream));
+  WS_HIP(hipMemcpyAsync(n->tgt, target, sizeof(float) * V, hipMemcpyHostToDevice, c->stream));
+  WS_HIP(hipMemcpyAsync(n->mask, mask, V, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_pack_input, dim3(blocks_for(V * 8, 256)), dim3(256), 0, c->stream, V, tmp, n->x0);
+  WS_LAUNCHED();
+  if ((r = net_forward(c, d)) || (r = net_loss(c, d, loss)) || (r = net_backward(c, d, 0))) return r;
+  WS_HIP(hipMemcpyAsync(grad, n->g, sizeof(float) * n->nparam, hipMemcpyDeviceToHost, c->stream));
+  WS_HIP(hipFreeAsync(tmp, c->stream));
+  WS_HIP(hipStreamSynchronize(c->stream));
+  return WS_OK;
+}
+
+int ws_net_bank(ws_ctx *c, int capacity) {
+  if (!c || capacity < 1) return WS_EINVAL;
+  int r;
+  if ((r = solver_side(c))) return r;
+  Net *n = c->net;
+  release(n, n->bankX);
+  release(n, n->bankT);
+  n->bankX = nullptr;
+  n->bankT = nullptr;
+  n->bankCap = n->bankCount = n->bankNext = 0;
+  if ((r = nalloc(c, n, &n->bankX, (size_t)capacity * c->nC * 8)) ||
+      (r = nalloc(c, n, &n->bankT, (size_t)capacity * c->nC)))
+    return r;
+  n->bankCap = capacity;
+  return WS_OK;
+}
+
+int ws_net_bank_count(ws_ctx *c, int *count) {
+  if (!c || !count) return WS_EINVAL;
+  *count = c->net ? c->net->bankCount : 0;
+  return WS_OK;
+}
+
+int ws_net_train(ws_ctx *c, int steps, int batch, int crop, double lr, unsigned seed, double *loss) {
+  if (!c || steps < 0 || batch < 1 || crop < 4 || (crop & 3)) return WS_EINVAL;
+  Net *n = c->net;
+  if (!n || n->bankCount < 1) { c->err = "ws_net_train: no recorded samples"; return WS_ESTATE; }
+  if (crop > c->nx || crop > c->ny || crop > c->nz) { c->err = "ws_net_train: crop larger than the grid"; return WS_EINVAL; }
+  NetDims cd{crop, crop, crop}, full = grid_dims(c);
+  int r;
+  if ((r = net_buffers(c, c->nC > vox(cd) ? c->nC : vox(cd)))) return r;
+  long Vc = vox(cd);
+  uint32_t s = seed * 2654435761u + 12345u;
+  auto rnd = [&](int m) { s = s * 1664525u + 1013904223u; return (int)((s >> 8) % (uint32_t)m); };
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  double last = 0;
+  for (int it = 0; it < steps; it++) {
+    double lsum = 0;
+    for (int bi = 0; bi < batch; bi++) {
+      int smp = rnd(n->bankCount);
+      int oz = 4 * rnd((full.Z - crop) / 4 + 1), oy = 4 * rnd((full.Y - crop) / 4 + 1);
+      int ox = 4 * rnd((full.X - crop) / 4 + 1);
+      hipLaunchKernelGGL(k_crop, dim3(blocks_for(Vc, 256)), dim3(256), 0, c->stream, full, cd, oz, oy, ox,
+                         n->bankX + (long)smp * c->nC * 8, n->bankT + (long)smp * c->nC, n->x0, n->tgt, n->mask);
+      WS_LAUNCHED();
+      double l = 0;
+      if ((r = net_forward(c, cd)) || (r = net_loss(c, cd, &l)) || (r = net_backward(c, cd, bi > 0))) return r;
+      lsum += l;
+    }
+    n->adamStep++;
+    float c1 = 1.f - std::pow(b1, (float)n->adamStep), c2 = 1.f - std::pow(b2, (float)n->adamStep);
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for(n->nparam, 256)), dim3(256), 0, c->stream, n->nparam, n->w, n->g, n->m1,
+                       n->m2, (float)lr, b1, b2, eps, c1, c2, 1.f / batch);
+    WS_LAUNCHED();
+    if ((r = net_pack(c))) return r;
+    last = lsum / batch;
+  }
+  WS_HIP(hipStreamSynchronize(c->stream));
+  if (loss) *loss = last;
+  return WS_OK;
+}
+
+}  // extern "C"

@@ -19,10 +19,13 @@ STAGES = {
     "heat": 14, "evaporation": 15, "currents": 16, "phase": 17, "push_surface": 18,
 }
 
-# exported entry points, as declared in include/watersim.h
+# exported entry points, as declared in include/watersim.h, watersim_world.h
+# and watersim_learn.h
 EXPORTS = ["ws_create", "ws_destroy", "ws_last_error", "ws_set_f", "ws_get_f", "ws_upload", "ws_download",
            "ws_substep", "ws_stage", "ws_step_frame", "ws_run_substeps", "ws_splat_density", "ws_mt_build",
-           "ws_autotune_iso", "ws_kinetic_energy", "ws_sync", "ws_world_build", "ws_world_free"]
+           "ws_autotune_iso", "ws_kinetic_energy", "ws_sync", "ws_world_build", "ws_world_free",
+           "ws_net_param_count", "ws_net_set_params", "ws_net_get_params", "ws_net_forward", "ws_net_grad",
+           "ws_net_bank", "ws_net_bank_count", "ws_net_train"]
 
 
 class WorldParams(ctypes.Structure):
@@ -78,9 +81,19 @@ def load(path=LIB_PATH):
     L.ws_world_build.argtypes = [ctypes.POINTER(WorldParams), ctypes.POINTER(WorldC)]
     L.ws_world_free.argtypes = [ctypes.POINTER(WorldC)]
     L.ws_world_free.restype = None
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.ws_net_param_count.argtypes = []
+    L.ws_net_set_params.argtypes = [vp, fp, ctypes.c_long]
+    L.ws_net_get_params.argtypes = [vp, fp, ctypes.c_long]
+    L.ws_net_forward.argtypes = [vp, ci, ci, ci, fp, fp]
+    L.ws_net_grad.argtypes = [vp, ci, ci, ci, fp, fp, ctypes.POINTER(ctypes.c_uint8), fp, ctypes.POINTER(d)]
+    L.ws_net_bank.argtypes = [vp, ci]
+    L.ws_net_bank_count.argtypes = [vp, ctypes.POINTER(ci)]
+    L.ws_net_train.argtypes = [vp, ci, ci, ci, d, ctypes.c_uint, ctypes.POINTER(d)]
     for f in EXPORTS:
         if f not in ("ws_destroy", "ws_last_error", "ws_world_free"):
             getattr(L, f).restype = ci
+    L.ws_net_param_count.restype = ctypes.c_long
     _lib = L
     return L
 
@@ -201,6 +214,55 @@ class Solver:
 
     def sync(self):
         self._check(self.L.ws_sync(self.ctx), "sync")
+
+    # ---- the learned pressure warm start (include/watersim_learn.h)
+    def net_params(self):
+        n = self.L.ws_net_param_count()
+        w = np.zeros(n, np.float32)
+        self._check(self.L.ws_net_get_params(self.ctx, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n),
+                    "net_get_params")
+        return w
+
+    def set_net_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._check(self.L.ws_net_set_params(self.ctx, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(w)),
+                    "net_set_params")
+
+    def net_forward(self, inp):
+        """inp [Z, Y, X, 5] -> out [Z, Y, X]"""
+        inp = np.ascontiguousarray(inp, np.float32)
+        Z, Y, X = inp.shape[:3]
+        out = np.zeros((Z, Y, X), np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        self._check(self.L.ws_net_forward(self.ctx, Z, Y, X, inp.ctypes.data_as(fp), out.ctypes.data_as(fp)),
+                    "net_forward")
+        return out
+
+    def net_grad(self, inp, target, mask):
+        inp = np.ascontiguousarray(inp, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        mask = np.ascontiguousarray(mask, np.uint8)
+        Z, Y, X = inp.shape[:3]
+        g = np.zeros(self.L.ws_net_param_count(), np.float32)
+        loss = ctypes.c_double()
+        fp = ctypes.POINTER(ctypes.c_float)
+        self._check(self.L.ws_net_grad(self.ctx, Z, Y, X, inp.ctypes.data_as(fp), target.ctypes.data_as(fp),
+                                       mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), g.ctypes.data_as(fp),
+                                       ctypes.byref(loss)), "net_grad")
+        return loss.value, g
+
+    def net_bank(self, capacity):
+        self._check(self.L.ws_net_bank(self.ctx, capacity), "net_bank")
+
+    def net_bank_count(self):
+        n = ctypes.c_int()
+        self._check(self.L.ws_net_bank_count(self.ctx, ctypes.byref(n)), "net_bank_count")
+        return n.value
+
+    def net_train(self, steps, batch, crop, lr, seed=1):
+        loss = ctypes.c_double()
+        self._check(self.L.ws_net_train(self.ctx, steps, batch, crop, lr, seed, ctypes.byref(loss)), "net_train")
+        return loss.value
 
     def splat_density(self):
         self._check(self.L.ws_splat_density(self.ctx), "splat_density")
