@@ -33,7 +33,23 @@ CONFIGS = {
     "cfg2": dict(n=64, target=232000, seed=20260115, vorticity=0.05, tol=1e-4, iters=0, physics=False,
                  frames=True, currents=0.8, curl_init=0.5, surface_every=10),
     "cfg3": dict(n=128, target=2117000, seed=20260116, vorticity=0.05, tol=1e-4, iters=0, physics=True),
+    # the learned warm start on the cfg3 scene (a seed held out of training)
+    "cfg4": dict(n=128, target=2117000, seed=20260116, vorticity=0.05, tol=1e-4, iters=0, physics=True,
+                 learned=True),
 }
+WEIGHTS = os.path.join(ROOT, "watersim-v1_amd", "weights", "warmstart_unet.f32")
+BF16_PEAK = 2.5e15  # MI355X dense BF16 matrix FLOP/s
+
+
+def net_flops(n):
+    """Multiply-adds x 2 of one inference of the warm-start network on an
+    n^3 grid: 3x3x3 convolutions 5->16->16 (n^3), 16->32->32 ((n/2)^3),
+    32->64->64 ((n/4)^3), 96->32->32 ((n/2)^3), 48->16->16 (n^3), head."""
+    V = n ** 3
+    full = 16 * 5 + 16 * 16 + 48 * 16 + 16 * 16
+    half = 16 * 32 + 32 * 32 + 96 * 32 + 32 * 32
+    quarter = 32 * 64 + 64 * 64
+    return 2 * 27 * (V * full + V / 8 * half + V / 64 * quarter) + 2 * 16 * V
 FRAME = 1.0 / 60
 SUN_PERIOD = 2000      # substeps per revolution of the sun
 SUN_DECL = 23.0        # degrees between the sun's path and the planet's equator
@@ -212,6 +228,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20, help="untimed substeps")
     ap.add_argument("--config", default="cfg3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--weights", default=WEIGHTS, help="warm-start network weights (cfg4)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times and exit: the "
                          "reference kernel whose FETCH_SIZE / WRITE_SIZE counters scale the HBM traffic")
@@ -233,6 +250,8 @@ def main():
               % a.gpus, file=sys.stderr)
         sys.exit(2)
     cfg = CONFIGS[a.config]
+    if cfg.get("learned"):
+        return learned_main(a, cfg)
     s, world = make_solver(cfg)
     n, nP = cfg["n"], int(s.get("nP"))
     log("warmup %d substeps" % a.warmup)
@@ -311,6 +330,55 @@ def main():
     if not a.no_cpu_baseline:
         log("cpu baseline (oracle, 1 thread)")
         out["cpu_baseline"] = cpu_baseline(s, world, cfg)
+    s.close()
+    print(json.dumps(out), flush=True)
+
+
+def learned_main(a, cfg):
+    """Pressure-solve time to relative residual 1e-4 with the learned warm
+    start against plain SOR, both from the same assembled system every
+    substep (the library's compare mode), on the full-physics 128^3 scene."""
+    if not os.path.exists(a.weights):
+        print("no trained weights at %s (run watersim-v1_amd/train_warmstart.py)" % a.weights, file=sys.stderr)
+        sys.exit(2)
+    s, world = make_solver(cfg)
+    n, nP = cfg["n"], int(s.get("nP"))
+    s.set_net_params(np.fromfile(a.weights, dtype=np.float32))
+    done = advance(s, world, cfg, a.warmup, 0)
+    s.sync()
+    s.set(timing=1, pressureCompare=1)
+    t0 = time.perf_counter()
+    steps = advance(s, world, cfg, a.steps, done)
+    s.sync()
+    el = time.perf_counter() - t0
+    g = {k: s.get(k) for k in ("cmpMsPlain", "cmpMsLearned", "cmpSweepsPlain", "cmpSweepsLearned", "cmpSolves",
+                               "netMs", "netCalls")}
+    s.set(timing=0, pressureCompare=0)
+    ns = max(g["cmpSolves"], 1)
+    plain, learned = g["cmpMsPlain"] / ns, g["cmpMsLearned"] / ns
+    infer = g["netMs"] / max(g["netCalls"], 1)
+    flops = net_flops(n)
+    achieved = flops / (infer / 1e3) / 1e12 if infer > 0 else 0.0
+    log("plain %.3f ms (%.1f sweeps), learned %.3f ms (%.1f sweeps, inference %.3f ms), %d solves"
+        % (plain, g["cmpSweepsPlain"] / ns, learned, g["cmpSweepsLearned"] / ns, infer, ns))
+    out = {
+        "metric": "pressure-solve speed-up to residual 1e-4, learned warm start vs plain SOR, 128^3 / 2M particles",
+        "value": plain / learned if learned > 0 else 0.0,
+        "unit": "x",
+        "n_gpus": 1, "steps": steps, "warmup": a.warmup, "ms_per_step": 1e3 * el / max(steps, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic (seeded planet built by the library's world builder; world seed held out of training)",
+        "config": {"workload": "cfg4: the cfg3 full-physics scene, %d^3, %d particles; every pressure solve run "
+                               "twice from the same system: plain red-black SOR from the previous pressure, and "
+                               "SOR from the network's prediction, both to relative residual 1e-4" % (n, nP),
+                   "pressure_ms_plain": plain, "pressure_ms_learned": learned,
+                   "sweeps_plain": g["cmpSweepsPlain"] / ns, "sweeps_learned": g["cmpSweepsLearned"] / ns,
+                   "inference_ms": infer, "solves": int(ns)},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / (BF16_PEAK / 1e12), "traffic": None,
+                     "kernel": "warm-start network inference (k_conv3 BF16 MFMA implicit GEMMs + elementwise)",
+                     "avg_launch_us": 1e3 * infer, "algorithmic_flops_per_launch": flops},
+    }
     s.close()
     print(json.dumps(out), flush=True)
 

@@ -274,7 +274,34 @@ int residual(ws_ctx *c, double *res) {
 
 int st_pressure_solve(ws_ctx *c);
 
+// Measurement of the learned warm start: both solves on the same assembled
+// system, plain SOR from the previous pressure and then, from the same
+// previous pressure, the network's prediction followed by SOR to the same
+// residual. Times (HIP events on the solver stream) and sweep counts are
+// accumulated per method; the warm-started result is kept.
+int st_pressure_compare(ws_ctx *c) {
+  int r, keep = c->warmStart;
+  WS_HIP(hipMemcpyAsync(c->scr2, c->q, sizeof(float) * c->nC, hipMemcpyDeviceToDevice, c->stream));
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) WS_HIP(hipMemcpyAsync(c->q, c->scr2, sizeof(float) * c->nC, hipMemcpyDeviceToDevice, c->stream));
+    c->warmStart = pass;
+    WS_HIP(hipEventRecord(c->ev[2], c->stream));
+    r = st_pressure_solve(c);
+    if (r) { c->warmStart = keep; return r; }
+    WS_HIP(hipEventRecord(c->ev[3], c->stream));
+    WS_HIP(hipEventSynchronize(c->ev[3]));
+    float ms = 0;
+    WS_HIP(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+    if (pass == 0) { c->cmpMsPlain += ms; c->cmpSweepsPlain += c->itersDone; }
+    else { c->cmpMsLearned += ms; c->cmpSweepsLearned += c->itersDone; }
+  }
+  c->cmpSolves++;
+  c->warmStart = keep;
+  return WS_OK;
+}
+
 int st_pressure(ws_ctx *c) {
+  if (c->pressureCompare && c->timing) return st_pressure_compare(c);
   if (c->timing) WS_HIP(hipEventRecord(c->ev[2], c->stream));
   int r = st_pressure_solve(c);
   if (r) return r;
@@ -322,7 +349,18 @@ int st_pressure_solve(ws_ctx *c) {
   // the learned warm start: record this solve's inputs for training and/or
   // start the sweeps from the network's prediction
   if (c->recordSamples && (r = net_record_inputs(c))) return r;
-  if (c->warmStart && (r = net_warm_start(c))) return r;
+  if (c->warmStart) {
+    if (c->timing) WS_HIP(hipEventRecord(c->ev[0], c->stream));
+    if ((r = net_warm_start(c))) return r;
+    if (c->timing) {
+      WS_HIP(hipEventRecord(c->ev[1], c->stream));
+      WS_HIP(hipEventSynchronize(c->ev[1]));
+      float ms = 0;
+      WS_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+      c->netMs += ms;
+      c->netCalls++;
+    }
+  }
   if (c->pressureTol <= 0) {
     if ((r = sor_sweeps(c, c->pressureIters))) return r;
     c->itersDone = c->pressureIters;
@@ -698,7 +736,7 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
   SF(heatK) SF(Tamb) SF(shadeCool) SF(sunActivity) SF(evapIntensity) SF(snowT) SF(meltT) SF(cloudP)
   SF(currents) SF(Rlo) SF(Rhi) SF(simTime) SF(heatAcc)
   SI(pressureIters) SI(maxPressureIters) SI(residualEvery) SI(ceilReflect) SI(hasSun) SI(measureResidual)
-  SI(adaptiveChecks) SI(sorMode) SI(spinOn) SI(warmStart) SI(recordSamples)
+  SI(adaptiveChecks) SI(sorMode) SI(spinOn) SI(warmStart) SI(recordSamples) SI(pressureCompare)
 #undef SF
 #undef SI
   if (!strcmp(n, "gRamp")) { c->gRamp = v; return push_ramp(c); }
@@ -716,6 +754,9 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
     c->sorMs = c->pressureMs = 0;
     c->sorLaunches = c->pressureSolves = c->sweepsTotal = c->residualsTotal = c->ticksTotal = 0;
     c->sorCellSweeps = 0;
+    c->cmpMsPlain = c->cmpMsLearned = c->netMs = 0;
+    c->netCalls = 0;
+    c->cmpSweepsPlain = c->cmpSweepsLearned = c->cmpSolves = 0;
     return WS_OK;
   }
   c->err = std::string("unknown scalar ") + n;
@@ -740,6 +781,7 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   GF(atmosphereH) GF(spacing) GF(particleVolume) GF(iso) GF(substeps) GF(pressureTol) GF(pressureIters)
   GF(maxPressureIters) GF(nP) GF(capacity) GF(residual) GF(domainR) GF(dx) GF(tn) GF(dv)
   GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal) GF(residualsTotal) GF(ticksTotal) GF(sorMode) GF(sorCellSweeps)
+  GF(cmpMsPlain) GF(cmpMsLearned) GF(cmpSweepsPlain) GF(cmpSweepsLearned) GF(cmpSolves) GF(warmStart) GF(netMs) GF(netCalls)
   GF(heatK) GF(Tamb) GF(shadeCool) GF(sunActivity) GF(evapIntensity) GF(snowT) GF(meltT) GF(cloudP) GF(currents)
   GF(simTime) GF(heatAcc) GF(evapCount) GF(cloudT) GF(rainT) GF(rainLiftAge)
 #undef GF

@@ -103,6 +103,12 @@ struct Ctx {
   // starts from its prediction, and whether solves record training samples
   Net *net = nullptr;
   int warmStart = 0, recordSamples = 0;
+  // both solves per pressure stage, timed separately (measurement)
+  int pressureCompare = 0;
+  double cmpMsPlain = 0, cmpMsLearned = 0;
+  long cmpSweepsPlain = 0, cmpSweepsLearned = 0, cmpSolves = 0;
+  double netMs = 0;  // timed network inferences inside pressure solves
+  long netCalls = 0;
 
   hipStream_t stream = nullptr;
   // static world

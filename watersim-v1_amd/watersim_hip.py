@@ -344,8 +344,17 @@ def build_world(n, dx, coreR, depth, seed, target, bumpiness=1.0):
 def solver_for_world(world, capacity=None, Tamb=0.32):
     """A GPU context holding a built world, its fill at rest and gRamp = 0,
     as FluidSolver.resetWater leaves it (solver.js:718-730)."""
-    n, nP = world["nx"], world["nP"]
+    n = world["nx"]
     s = Solver(n, n, n, world["dx"], capacity or world["capacity"])
+    load_world(s, world, Tamb)
+    return s
+
+
+def load_world(s, world, Tamb=0.32):
+    """Replaces the world and fill of an existing context of the same grid."""
+    n, nP = world["nx"], world["nP"]
+    if nP > s.capacity:
+        raise WatersimError("world has %d particles, the context holds %d" % (nP, s.capacity))
     s.set(coreR=world["coreR"], oceanR=world["oceanR"], spacing=world["spacing"],
           particleVolume=world["particleVolume"], dv=world["dv"], tn=world["tn"])
     s.upload("staticTypes", world["staticTypes"])
@@ -365,5 +374,4 @@ def solver_for_world(world, capacity=None, Tamb=0.32):
     s.upload("pflag", np.zeros(nP, np.uint8))
     s.upload("pcool", np.zeros(nP, np.uint8))
     s.upload("q", np.zeros(n ** 3, np.float32))
-    s.set(gRamp=0.0)
-    return s
+    s.set(gRamp=0.0, simTime=0.0, heatAcc=0.0)
