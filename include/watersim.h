@@ -59,7 +59,9 @@ enum ws_stage_id {
   WS_STAGE_CURRENTS = 16,  /* _updateCurrents(dt), :3658-3782 */
   WS_STAGE_PHASE = 17,     /* _updatePhaseChanges(dtFrame), :3152-3281 */
   WS_STAGE_PUSH_SURFACE = 18, /* _pushSurfaceParticles, :3486-3587 */
-  WS_STAGE_COUNT = 19
+  WS_STAGE_VAPOR_COLLISIONS = 19, /* _vaporCollisions, :3345-3479 */
+  WS_STAGE_ICE_CONTACTS = 20,  /* _iceContacts, :3290-3324 */
+  WS_STAGE_COUNT = 21
 };
 
 /* new FluidSolver({nx,ny,nz,dx}) plus _allocParticles(capacity),
@@ -100,9 +102,9 @@ int ws_stage(ws_ctx *c, int stage, double dt);
 /* step(dtFrame): _stepSync, solver.js:2663-2700: the 25 Hz thermal,
  * evaporation and currents tick, the CFL substep loop
  * (dt = min(dtFrame/substeps, cfl*dx/max(umax,0.08), 1/100), >= 1e-5,
- * at most 10 substeps), the frame-end phase pass and surface rescue. The
- * reference's vapor collisions and ice contacts are not part of this build.
- * Returns the substep count through *n_sub. */
+ * at most 10 substeps), then the frame-end passes: vapor collisions, phase
+ * changes, snow contacts, surface rescue. Returns the substep count through
+ * *n_sub. */
 int ws_step_frame(ws_ctx *c, double dt_frame, int *n_sub);
 /* n substeps with the CFL law evaluated on the device (no host round trip
  * between substeps): the measurement path for substeps/s. */

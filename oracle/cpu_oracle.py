@@ -37,7 +37,8 @@ def lib(bits=32):
         L.oc_get_i.restype = i64
         for fn in ["oc_rasterize_static", "oc_clear_faces", "oc_p2g", "oc_apply_bc", "oc_snapshot",
                    "oc_project", "oc_extrapolate", "oc_pressure_finish", "oc_pressure_solve",
-                   "oc_splat_density", "oc_push_surface"]:
+                   "oc_splat_density", "oc_push_surface", "oc_vapor_collisions",
+                   "oc_ice_contacts"]:
             getattr(L, fn).argtypes = [vp]
         L.oc_mark_fluid.argtypes = [vp, ci, ci]
         for fn in ["oc_viscosity", "oc_apply_gravity", "oc_vorticity", "oc_advect", "oc_substep", "oc_update_heat",

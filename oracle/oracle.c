@@ -54,7 +54,7 @@ struct oc_ctx {
   /* frame tick (solver.js:137-193, 2715-3281) */
   double heatK, Tamb, shadeCool, sunActivity, evapIntensity, snowT, meltT, cloudP, currents;
   double Rlo, Rhi, simTime;
-  int spinOn, evapCount;
+  int spinOn, evapCount, vaporCount, nightVaporCount;
   oc_real *pLight, *pPh;
   uint8_t *pDepth;
   /* outputs */
@@ -155,7 +155,7 @@ double oc_get_f(oc_ctx *c, const char *name) {
   return NAN;
 }
 long long oc_get_i(oc_ctx *c, const char *name) {
-  GET(airborne) GET(nFluid) GET(nP) GET(cfS) GET(bfS) GET(evS) GET(itersDone) GET(evapCount)
+  GET(airborne) GET(nFluid) GET(nP) GET(cfS) GET(bfS) GET(evS) GET(itersDone) GET(evapCount) GET(vaporCount) GET(nightVaporCount)
   return -1;
 }
 #undef GET
@@ -1802,4 +1802,163 @@ void oc_push_surface(oc_ctx *c) {
       }
     }
   }
+}
+
+/* ordering of the x-sorted sweeps: px, then particle index */
+static const oc_real *g_sort_px;
+static int by_px(const void *a, const void *b) {
+  int i = *(const int *)a, j = *(const int *)b;
+  double d = (double)g_sort_px[i] - (double)g_sort_px[j];
+  if (d < 0) return -1;
+  if (d > 0) return 1;
+  return i < j ? -1 : (i > j);
+}
+
+/* _vaporCollisions, solver.js:3345-3479: steam (2) and cloud (3) parcels
+ * sorted by x; every overlapping pair (in sorted order, with the early
+ * break on the x gap) exchanges the normal velocity if approaching, is
+ * separated and pushed apart, and with spin on trades linear momentum for
+ * rotation; then every liquid particle (surface band, spray, rain) pushes
+ * the parcels it touches out of the water. Serial, in place, as written. */
+void oc_vapor_collisions(oc_ctx *c) {
+  int nP = c->nP, nV = 0, nSteam = 0, night = 0;
+  int *v = (int *)malloc(sizeof(int) * (nP > 0 ? nP : 1));
+  double sx = c->hasSun ? c->sunX - c->cx : 0, sy = c->hasSun ? c->sunY - c->cy : 0;
+  double sz = c->hasSun ? c->sunZ - c->cz : 0;
+  for (int p = 0; p < nP; p++) {
+    int f = c->pflag[p];
+    if (f != 2 && f != 3) continue;
+    v[nV++] = p;
+    if (f == 2) {
+      nSteam++;
+      if (!c->hasSun || ((double)c->px[p] - c->cx) * sx + ((double)c->py[p] - c->cy) * sy +
+                                ((double)c->pz[p] - c->cz) * sz < 0)
+        night++;
+    }
+  }
+  c->vaporCount = nSteam;
+  c->nightVaporCount = night;
+  if (nV < 2) { free(v); return; }
+  oc_real *px = c->px, *py = c->py, *pz = c->pz, *vx = c->pvx, *vy = c->pvy, *vz = c->pvz;
+  oc_real *wx = c->pWx, *wy = c->pWy, *wz = c->pWz;
+  int spin = c->spinOn && wx != 0;
+  double rad = c->spacing * 0.6, rad2 = rad * rad;
+  g_sort_px = px;
+  qsort(v, nV, sizeof(int), by_px);
+  for (int a = 0; a < nV; a++) {
+    int i = v[a];
+    for (int b = a + 1; b < nV; b++) {
+      int j = v[b];
+      if ((double)px[j] - px[i] >= rad) break;
+      double ddx = (double)px[j] - px[i], ddy = (double)py[j] - py[i], ddz = (double)pz[j] - pz[i];
+      double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+      if (d2 >= rad2) continue;
+      double d = sqrt(d2);
+      if (!(d != 0)) d = 1e-9;
+      double nX = ddx / d, nY = ddy / d, nZ = ddz / d;
+      double vRel = ((double)vx[i] - vx[j]) * nX + ((double)vy[i] - vy[j]) * nY + ((double)vz[i] - vz[j]) * nZ;
+      if (vRel > 0) {
+        vx[i] = (oc_real)(vx[i] - vRel * nX); vy[i] = (oc_real)(vy[i] - vRel * nY); vz[i] = (oc_real)(vz[i] - vRel * nZ);
+        vx[j] = (oc_real)(vx[j] + vRel * nX); vy[j] = (oc_real)(vy[j] + vRel * nY); vz[j] = (oc_real)(vz[j] + vRel * nZ);
+      }
+      double half = (rad - d) * 0.5;
+      px[i] = (oc_real)(px[i] - nX * half); py[i] = (oc_real)(py[i] - nY * half); pz[i] = (oc_real)(pz[i] - nZ * half);
+      px[j] = (oc_real)(px[j] + nX * half); py[j] = (oc_real)(py[j] + nY * half); pz[j] = (oc_real)(pz[j] + nZ * half);
+      double kick = (rad - d) * 2;
+      vx[i] = (oc_real)(vx[i] - nX * kick); vy[i] = (oc_real)(vy[i] - nY * kick); vz[i] = (oc_real)(vz[i] - nZ * kick);
+      vx[j] = (oc_real)(vx[j] + nX * kick); vy[j] = (oc_real)(vy[j] + nY * kick); vz[j] = (oc_real)(vz[j] + nZ * kick);
+      if (!spin) continue;
+      double R = rad * 0.5;
+      double wSx = (double)wx[i] + wx[j], wSy = (double)wy[i] + wy[j], wSz = (double)wz[i] + wz[j];
+      double usx = ((double)vx[i] - vx[j]) + R * (wSy * nZ - wSz * nY);
+      double usy = ((double)vy[i] - vy[j]) + R * (wSz * nX - wSx * nZ);
+      double usz = ((double)vz[i] - vz[j]) + R * (wSx * nY - wSy * nX);
+      double uN = usx * nX + usy * nY + usz * nZ;
+      double utx = usx - uN * nX, uty = usy - uN * nY, utz = usz - uN * nZ;
+      double utM = sqrt(utx * utx + uty * uty + utz * utz);
+      if (utM < 1e-9) continue;
+      double jt = utM / 7;
+      if (vRel > 0) { double mu = 0.25 * vRel; if (jt > mu) jt = mu; }
+      double jx = -utx / utM * jt, jy = -uty / utM * jt, jz = -utz / utM * jt;
+      vx[i] = (oc_real)(vx[i] + jx); vy[i] = (oc_real)(vy[i] + jy); vz[i] = (oc_real)(vz[i] + jz);
+      vx[j] = (oc_real)(vx[j] - jx); vy[j] = (oc_real)(vy[j] - jy); vz[j] = (oc_real)(vz[j] - jz);
+      double invI = 1 / (0.4 * R * R);
+      double dwx = R * (nY * jz - nZ * jy) * invI, dwy = R * (nZ * jx - nX * jz) * invI;
+      double dwz = R * (nX * jy - nY * jx) * invI;
+      wx[i] = (oc_real)(wx[i] + dwx); wy[i] = (oc_real)(wy[i] + dwy); wz[i] = (oc_real)(wz[i] + dwz);
+      wx[j] = (oc_real)(wx[j] + dwx); wy[j] = (oc_real)(wy[j] + dwy); wz[j] = (oc_real)(wz[j] + dwz);
+    }
+  }
+  /* airborne x liquid contacts */
+  double band = c->oceanR - c->dx * 2, bandLo2 = band * band;
+  for (int p = 0; p < nP; p++) {
+    int fw = c->pflag[p];
+    if (fw != 0 && fw != 1 && fw != 4) continue;
+    double wxp = px[p], wyp = py[p], wzp = pz[p];
+    if (fw == 0) {
+      double ex = wxp - c->cx, ey = wyp - c->cy, ez = wzp - c->cz;
+      if (ex * ex + ey * ey + ez * ez < bandLo2) continue;
+    }
+    int lo = 0, hi = nV;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if ((double)px[v[mid]] < wxp - rad) lo = mid + 1; else hi = mid;
+    }
+    for (int b = lo; b < nV; b++) {
+      int j = v[b];
+      double ddx = (double)px[j] - wxp;
+      if (ddx >= rad) break;
+      double ddy = (double)py[j] - wyp, ddz = (double)pz[j] - wzp;
+      double dd2 = ddx * ddx + ddy * ddy + ddz * ddz;
+      if (dd2 >= rad2 || dd2 < 1e-12) continue;
+      double dd = sqrt(dd2);
+      double nX = ddx / dd, nY = ddy / dd, nZ = ddz / dd;
+      double vR = ((double)vx[j] - vx[p]) * nX + ((double)vy[j] - vy[p]) * nY + ((double)vz[j] - vz[p]) * nZ;
+      if (vR < 0) {
+        double vB = vR * (1 + 0.4);
+        vx[j] = (oc_real)(vx[j] - vB * nX); vy[j] = (oc_real)(vy[j] - vB * nY); vz[j] = (oc_real)(vz[j] - vB * nZ);
+      }
+      double ov = rad - dd;
+      px[j] = (oc_real)(px[j] + nX * ov); py[j] = (oc_real)(py[j] + nY * ov); pz[j] = (oc_real)(pz[j] + nZ * ov);
+      double kk = ov * 2;
+      vx[j] = (oc_real)(vx[j] + nX * kk); vy[j] = (oc_real)(vy[j] + nY * kk); vz[j] = (oc_real)(vz[j] + nZ * kk);
+    }
+  }
+  free(v);
+}
+
+/* _iceContacts, solver.js:3290-3324: snow grains (5) sorted by x; a free
+ * grain touching a stuck one (pAir == 1) sticks, two free grains pair up
+ * at their mean velocity. Serial, in place. */
+void oc_ice_contacts(oc_ctx *c) {
+  int nP = c->nP, nI = 0;
+  int *v = (int *)malloc(sizeof(int) * (nP > 0 ? nP : 1));
+  for (int p = 0; p < nP; p++)
+    if (c->pflag[p] == 5) v[nI++] = p;
+  if (nI < 2) { free(v); return; }
+  oc_real *px = c->px, *py = c->py, *pz = c->pz, *vx = c->pvx, *vy = c->pvy, *vz = c->pvz, *air = c->pAir;
+  double rad = c->spacing * 0.7, rad2 = rad * rad;
+  g_sort_px = px;
+  qsort(v, nI, sizeof(int), by_px);
+  for (int a = 0; a < nI; a++) {
+    int i = v[a];
+    int stuckI = air[i] == 1;
+    for (int b = a + 1; b < nI; b++) {
+      int j = v[b];
+      if ((double)px[j] - px[i] >= rad) break;
+      double ddx = (double)px[j] - px[i], ddy = (double)py[j] - py[i], ddz = (double)pz[j] - pz[i];
+      if (ddx * ddx + ddy * ddy + ddz * ddz >= rad2) continue;
+      int stuckJ = air[j] == 1;
+      if (stuckI && stuckJ) continue;
+      if (stuckJ) {
+        air[i] = 1; vx[i] = vy[i] = vz[i] = 0; stuckI = 1;
+      } else if (stuckI) {
+        air[j] = 1; vx[j] = vy[j] = vz[j] = 0;
+      } else {
+        double mx = ((double)vx[i] + vx[j]) * 0.5, my = ((double)vy[i] + vy[j]) * 0.5, mz = ((double)vz[i] + vz[j]) * 0.5;
+        vx[i] = vx[j] = (oc_real)mx; vy[i] = vy[j] = (oc_real)my; vz[i] = vz[j] = (oc_real)mz;
+      }
+    }
+  }
+  free(v);
 }

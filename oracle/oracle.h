@@ -71,6 +71,8 @@ void oc_update_currents(oc_ctx *c, double dt);
 void oc_update_phase(oc_ctx *c, double dt);
 /* frame-end passes (solver.js:3290-3587) */
 void oc_push_surface(oc_ctx *c);
+void oc_vapor_collisions(oc_ctx *c);
+void oc_ice_contacts(oc_ctx *c);
 
 /* samplers and terrain predicates */
 double oc_sample_face(oc_ctx *c, const oc_real *arr, int comp, double x, double y, double z);

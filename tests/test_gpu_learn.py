@@ -113,3 +113,25 @@ def test_warm_start_runs_inside_the_pressure_solve():
         qs.append(s.download("q"))
         s.close()
     assert np.array_equal(qs[0], qs[1])
+
+
+def test_tiled_and_untiled_convolutions_agree():
+    """The LDS-staged convolution and the untiled one compute the same
+    products (different FP32 summation order only)."""
+    x, fl, rng = scene(16, 13)
+    tgt = rng.normal(size=(16, 16, 16))
+    th = params(9).astype(np.float32)
+    res = []
+    for tiled in (1, 0):
+        s = solver()
+        s.set(netTiled=tiled)
+        s.set_net_params(th)
+        out = s.net_forward(x)
+        _, g = s.net_grad(x, tgt, fl)
+        res.append((out, g))
+        s.close()
+    (o1, g1), (o0, g0) = res
+    eo = np.abs(o1 - o0).max() / np.abs(o0).max()
+    eg = np.linalg.norm(g1 - g0) / np.linalg.norm(g0)
+    print("  tiled vs untiled: out %.2e, grad %.2e" % (eo, eg))
+    assert eo <= 1e-2 and eg <= 1e-2

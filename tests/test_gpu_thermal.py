@@ -84,3 +84,49 @@ def test_push_surface():
         assert e <= 2e-5, (k, e)
     assert moved > 100
     s.close()
+
+
+def _fe_gpu(tag):
+    g = golden.load("thermal_32")
+    s = gpu_from_golden(g, tag=tag)
+    fe = g.meta["frame_end"]
+    sp = fe["sunPos"]
+    s.set(spinOn=1, hasSun=1, sunX=sp[0], sunY=sp[1], sunZ=sp[2])
+    return g, s
+
+
+def _fe_compare(s, g, a, b, keys, tol=2e-5):
+    moved = 0
+    for k in keys:
+        got = s.download(k).astype(np.float64)
+        want = g[b + k].astype(np.float64)
+        e = np.abs(got - want).max() / max(1.0, np.abs(want).max())
+        moved += int(np.sum(g[a + k] != g[b + k]))
+        print("  %-5s max rel err %.3e" % (k, e))
+        assert e <= tol, (k, e)
+    return moved
+
+
+def test_vapor_collisions():
+    """Parcel-parcel and parcel-liquid contacts on the perturbed frame-end
+    state (a chain of 80 touching parcels along the sea surface, real steam
+    above it): the census exact, positions, velocities and spins to float
+    rounding of the reference's serial sweep."""
+    g, s = _fe_gpu("fe0_")
+    s.stage("vapor_collisions", 0.0)
+    fe = g.meta["frame_end"]
+    assert int(s.get("vaporCount")) == fe["vaporCount"]
+    assert int(s.get("nightVaporCount")) == fe["nightVaporCount"]
+    assert s.get("contactOverflow") == 0
+    moved = _fe_compare(s, g, "fe0_", "fe1_", ["px", "py", "pz", "pvx", "pvy", "pvz", "pWx", "pWy", "pWz"])
+    assert moved > 100
+    s.close()
+
+
+def test_ice_contacts():
+    g, s = _fe_gpu("fe1_")
+    s.stage("ice_contacts", 0.0)
+    assert np.array_equal(s.download("pAir"), g["fe2_pAir"])
+    moved = _fe_compare(s, g, "fe1_", "fe2_", ["pvx", "pvy", "pvz"])
+    assert moved > 20
+    s.close()

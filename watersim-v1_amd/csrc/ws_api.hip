@@ -708,6 +708,7 @@ void ws_destroy(ws_ctx *c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
   net_free(c);
+  contacts_release(c, true);
   void *ptrs[] = {c->staticTypes, c->gu, c->thr, c->terrainR, c->cellType, c->nbm, c->dg, c->q, c->q2, c->rhs, c->pw,
                   c->sorActive, c->tbActive, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg,
                   c->cellCount, c->cellStart, c->vort, c->vel, c->velO, c->wgt, c->scr, c->scr2, c->valid, c->dens,
@@ -736,7 +737,7 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
   SF(heatK) SF(Tamb) SF(shadeCool) SF(sunActivity) SF(evapIntensity) SF(snowT) SF(meltT) SF(cloudP)
   SF(currents) SF(Rlo) SF(Rhi) SF(simTime) SF(heatAcc)
   SI(pressureIters) SI(maxPressureIters) SI(residualEvery) SI(ceilReflect) SI(hasSun) SI(measureResidual)
-  SI(adaptiveChecks) SI(sorMode) SI(spinOn) SI(warmStart) SI(recordSamples) SI(pressureCompare)
+  SI(adaptiveChecks) SI(sorMode) SI(spinOn) SI(warmStart) SI(recordSamples) SI(pressureCompare) SI(netTiled)
 #undef SF
 #undef SI
   if (!strcmp(n, "gRamp")) { c->gRamp = v; return push_ramp(c); }
@@ -782,6 +783,7 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   GF(maxPressureIters) GF(nP) GF(capacity) GF(residual) GF(domainR) GF(dx) GF(tn) GF(dv)
   GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal) GF(residualsTotal) GF(ticksTotal) GF(sorMode) GF(sorCellSweeps)
   GF(cmpMsPlain) GF(cmpMsLearned) GF(cmpSweepsPlain) GF(cmpSweepsLearned) GF(cmpSolves) GF(warmStart) GF(netMs) GF(netCalls)
+  GF(vaporCount) GF(nightVaporCount) GF(contactOverflow)
   GF(heatK) GF(Tamb) GF(shadeCool) GF(sunActivity) GF(evapIntensity) GF(snowT) GF(meltT) GF(cloudP) GF(currents)
   GF(simTime) GF(heatAcc) GF(evapCount) GF(cloudT) GF(rainT) GF(rainLiftAge)
 #undef GF
@@ -904,7 +906,10 @@ int ws_step_frame(ws_ctx *c, double dtFrame, int *n_sub) {
     remaining -= dt;
     n++;
   }
+  // frame-end passes in the reference's order (solver.js:2696-2699)
+  if ((r = frame_vapor_collisions(c))) return r;
   if ((r = frame_phase(c, dtFrame))) return r;
+  if ((r = frame_ice_contacts(c))) return r;
   if ((r = frame_push_surface(c))) return r;
   WS_HIP(hipStreamSynchronize(c->stream));
   if (n_sub) *n_sub = n;
@@ -959,6 +964,8 @@ int ws_stage(ws_ctx *c, int stage, double dt) {
     case WS_STAGE_CURRENTS: r = frame_currents(c, dt); break;
     case WS_STAGE_PHASE: r = frame_phase(c, dt); break;
     case WS_STAGE_PUSH_SURFACE: r = frame_push_surface(c); break;
+    case WS_STAGE_VAPOR_COLLISIONS: r = frame_vapor_collisions(c); break;
+    case WS_STAGE_ICE_CONTACTS: r = frame_ice_contacts(c); break;
     default: c->err = "unknown stage"; return WS_EINVAL;
   }
   if (r) return r;

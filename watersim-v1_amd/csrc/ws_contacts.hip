@@ -1,0 +1,534 @@
+// Frame-end particle contacts of the planet (js/solver.js:3290-3479):
+// vapor-vapor collisions with the vapor-liquid contacts, and snow contacts.
+//
+// The reference resolves contacts in one serial sweep over the parcels
+// sorted by x (then index), updating as it goes. Here the parcels are split
+// into groups that cannot reach each other: connected components of the
+// "closer than a threshold" graph (union-find over cell-binned
+// neighbours). Each group is then resolved serially, in the reference's
+// order, by one thread; groups run in parallel. Snow grains do not move in
+// their pass, so with the contact radius as threshold the result is the
+// reference's exactly. Vapor parcels move while they are resolved; the
+// threshold is twice the radius, which holds every contact unless a chain
+// of pushes carries a parcel more than one radius (or breaks the x order
+// the reference's early exit relies on).
+//
+// The vapor-liquid contacts move only the vapor parcel and never the water,
+// so every parcel is independent: one thread per parcel visits the liquid
+// particles near it in index order, as the reference's outer loop does.
+#include <hipcub/hipcub.hpp>
+
+#include "ws_frame.h"
+
+namespace ws {
+
+namespace {
+
+// subset flags: bit 0 vapor (2/3), bit 1 snow (5), bit 2 liquid contact
+// partner (spray 1, rain 4, or grid fluid in the surface band)
+__global__ void k_ct_flags(Geo g, int nP, const float *__restrict__ px, const float *__restrict__ py,
+                           const float *__restrict__ pz, const uint8_t *__restrict__ pflag, float bandLo2,
+                           unsigned *__restrict__ fv, unsigned *__restrict__ fi, unsigned *__restrict__ fl) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > nP) return;
+  if (p == nP) { fv[p] = fi[p] = fl[p] = 0; return; }
+  uint8_t f = pflag[p];
+  fv[p] = (f == 2 || f == 3);
+  fi[p] = f == 5;
+  unsigned liq = 0;
+  if (f == 1 || f == 4) liq = 1;
+  else if (f == 0) {
+    float ex = px[p] - g.cx, ey = py[p] - g.cy, ez = pz[p] - g.cz;
+    liq = ex * ex + ey * ey + ez * ez >= bandLo2;
+  }
+  fl[p] = liq;
+}
+
+__global__ void k_ct_compact(int nP, const unsigned *__restrict__ flag, const unsigned *__restrict__ off,
+                             unsigned *__restrict__ list) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < nP && flag[p]) list[off[p]] = (unsigned)p;
+}
+
+// sort keys: x bits (positions are non-negative, so float order = bit
+// order), then the particle index
+__global__ void k_ct_keys(int n, const unsigned *__restrict__ list, const float *__restrict__ px,
+                          unsigned long long *__restrict__ key) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  unsigned p = list[e];
+  key[e] = ((unsigned long long)__float_as_uint(px[p]) << 32) | p;
+}
+
+__device__ __forceinline__ int cell_of(const Geo &g, float x, float y, float z) {
+  int i = clampi((int)(x * g.inv_dx), 0, g.nx - 1), j = clampi((int)(y * g.inv_dx), 0, g.ny - 1);
+  int k = clampi((int)(z * g.inv_dx), 0, g.nz - 1);
+  return (k * g.ny + j) * g.nx + i;
+}
+
+// counting sort of a subset by cell (order inside a cell is not used)
+__global__ void k_ct_count(Geo g, int n, const unsigned *__restrict__ list, const float *__restrict__ px,
+                           const float *__restrict__ py, const float *__restrict__ pz, unsigned *__restrict__ cnt,
+                           unsigned *__restrict__ cellOf, unsigned *__restrict__ rankIn) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  unsigned p = list[e];
+  int c = cell_of(g, px[p], py[p], pz[p]);
+  cellOf[e] = c;
+  rankIn[e] = atomicAdd(&cnt[c], 1u);
+}
+__global__ void k_ct_scatter(int n, const unsigned *__restrict__ cellOf, const unsigned *__restrict__ rankIn,
+                             const unsigned *__restrict__ start, unsigned *__restrict__ binned) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  binned[start[cellOf[e]] + rankIn[e]] = (unsigned)e;  // subset position
+}
+
+// Parents only ever point to smaller indices, so every walk ends. Reads and
+// the path-halving writes are device-scope atomics: the links other
+// workgroups make (on other XCDs) are seen without stale cache lines.
+__device__ __forceinline__ unsigned ld_parent(unsigned *parent, unsigned x) {
+  return __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned uf_find(unsigned *parent, unsigned x) {
+  unsigned p = ld_parent(parent, x);
+  while (p != x) {
+    unsigned q = ld_parent(parent, p);
+    if (q != p) __hip_atomic_store(&parent[x], q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // halving
+    x = p;
+    p = ld_parent(parent, x);
+  }
+  return x;
+}
+
+// union of subset members closer than thr (the smaller index becomes the root)
+__global__ void k_ct_union(Geo g, int n, const unsigned *__restrict__ list, const float *__restrict__ px,
+                           const float *__restrict__ py, const float *__restrict__ pz,
+                           const unsigned *__restrict__ cellOf, const unsigned *__restrict__ start,
+                           const unsigned *__restrict__ binned, float thr2, unsigned *__restrict__ parent) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  unsigned p = list[e];
+  float x = px[p], y = py[p], z = pz[p];
+  int c = cellOf[e];
+  int ci = c % g.nx, cj = (c / g.nx) % g.ny, ck = c / (g.nx * g.ny);
+  for (int dk = -1; dk <= 1; dk++)
+    for (int dj = -1; dj <= 1; dj++)
+      for (int di = -1; di <= 1; di++) {
+        int ii = ci + di, jj = cj + dj, kk = ck + dk;
+        if (ii < 0 || jj < 0 || kk < 0 || ii >= g.nx || jj >= g.ny || kk >= g.nz) continue;
+        int cc = (kk * g.ny + jj) * g.nx + ii;
+        for (unsigned b = start[cc]; b < start[cc + 1]; b++) {
+          unsigned o = binned[b];
+          if (o <= (unsigned)e) continue;
+          unsigned q = list[o];
+          float ddx = px[q] - x, ddy = py[q] - y, ddz = pz[q] - z;
+          if (ddx * ddx + ddy * ddy + ddz * ddz >= thr2) continue;
+          // link the larger root under the smaller one
+          for (int guard = 0; guard < 1 << 20; guard++) {
+            unsigned ra = uf_find(parent, (unsigned)e), rb = uf_find(parent, o);
+            if (ra == rb) break;
+            unsigned hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
+            if (atomicCAS(&parent[hi], hi, lo) == hi) break;
+          }
+        }
+      }
+}
+
+// component key: (root, position in the x order)
+__global__ void k_ct_groupkeys(int n, unsigned *__restrict__ parent, const unsigned *__restrict__ rankOf,
+                               unsigned long long *__restrict__ key) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  unsigned r = uf_find(parent, (unsigned)e);
+  key[e] = ((unsigned long long)r << 32) | rankOf[e];
+}
+
+// position of each member in the x order
+__global__ void k_ct_rank(int n, const unsigned long long *__restrict__ sortedKey, const unsigned *__restrict__ posOf,
+                          unsigned *__restrict__ rankOf) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  unsigned p = (unsigned)(sortedKey[r] & 0xffffffffull);
+  rankOf[posOf[p]] = (unsigned)r;
+}
+__global__ void k_ct_pos(int n, const unsigned *__restrict__ list, unsigned *__restrict__ posOf) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) posOf[list[e]] = (unsigned)e;
+}
+
+// the members of one group in x order: particle indices from the sorted
+// group keys (low word = x rank, which indexes the x-sorted particle list)
+__device__ __forceinline__ unsigned member(const unsigned long long *gk, const unsigned long long *xk, int t) {
+  return (unsigned)(xk[(unsigned)(gk[t] & 0xffffffffull)] & 0xffffffffull);
+}
+
+// _vaporCollisions' parcel-parcel sweep, one group per thread
+__global__ void k_ct_vapor_groups(int n, const unsigned long long *__restrict__ gk,
+                                  const unsigned long long *__restrict__ xk, float rad, int spin,
+                                  float *__restrict__ px, float *__restrict__ py, float *__restrict__ pz,
+                                  float *__restrict__ vx, float *__restrict__ vy, float *__restrict__ vz,
+                                  float *__restrict__ wx, float *__restrict__ wy, float *__restrict__ wz) {
+  int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t0 >= n) return;
+  if (t0 > 0 && (gk[t0] >> 32) == (gk[t0 - 1] >> 32)) return;  // not the first member of its group
+  int t1 = t0 + 1;
+  while (t1 < n && (gk[t1] >> 32) == (gk[t0] >> 32)) t1++;
+  if (t1 - t0 < 2) return;
+  const float rad2 = rad * rad;
+  for (int a = t0; a < t1; a++) {
+    unsigned i = member(gk, xk, a);
+    for (int b = a + 1; b < t1; b++) {
+      unsigned j = member(gk, xk, b);
+      if (px[j] - px[i] >= rad) break;
+      float ddx = px[j] - px[i], ddy = py[j] - py[i], ddz = pz[j] - pz[i];
+      float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+      if (d2 >= rad2) continue;
+      float d = sqrtf(d2);
+      if (!(d != 0.f)) d = 1e-9f;
+      float nX = ddx / d, nY = ddy / d, nZ = ddz / d;
+      float vRel = (vx[i] - vx[j]) * nX + (vy[i] - vy[j]) * nY + (vz[i] - vz[j]) * nZ;
+      if (vRel > 0.f) {
+        vx[i] -= vRel * nX; vy[i] -= vRel * nY; vz[i] -= vRel * nZ;
+        vx[j] += vRel * nX; vy[j] += vRel * nY; vz[j] += vRel * nZ;
+      }
+      float half = (rad - d) * 0.5f;
+      px[i] -= nX * half; py[i] -= nY * half; pz[i] -= nZ * half;
+      px[j] += nX * half; py[j] += nY * half; pz[j] += nZ * half;
+      float kick = (rad - d) * 2.f;
+      vx[i] -= nX * kick; vy[i] -= nY * kick; vz[i] -= nZ * kick;
+      vx[j] += nX * kick; vy[j] += nY * kick; vz[j] += nZ * kick;
+      if (!spin) continue;
+      float R = rad * 0.5f;
+      float wSx = wx[i] + wx[j], wSy = wy[i] + wy[j], wSz = wz[i] + wz[j];
+      float usx = (vx[i] - vx[j]) + R * (wSy * nZ - wSz * nY);
+      float usy = (vy[i] - vy[j]) + R * (wSz * nX - wSx * nZ);
+      float usz = (vz[i] - vz[j]) + R * (wSx * nY - wSy * nX);
+      float uN = usx * nX + usy * nY + usz * nZ;
+      float utx = usx - uN * nX, uty = usy - uN * nY, utz = usz - uN * nZ;
+      float utM = sqrtf(utx * utx + uty * uty + utz * utz);
+      if (utM < 1e-9f) continue;
+      float jt = utM / 7.f;
+      if (vRel > 0.f) { float mu = 0.25f * vRel; if (jt > mu) jt = mu; }
+      float jx = -utx / utM * jt, jy = -uty / utM * jt, jz = -utz / utM * jt;
+      vx[i] += jx; vy[i] += jy; vz[i] += jz;
+      vx[j] -= jx; vy[j] -= jy; vz[j] -= jz;
+      float invI = 1.f / (0.4f * R * R);
+      float dwx = R * (nY * jz - nZ * jy) * invI, dwy = R * (nZ * jx - nX * jz) * invI;
+      float dwz = R * (nX * jy - nY * jx) * invI;
+      wx[i] += dwx; wy[i] += dwy; wz[i] += dwz;
+      wx[j] += dwx; wy[j] += dwy; wz[j] += dwz;
+    }
+  }
+}
+
+// _iceContacts, one group per thread (snow does not move: exact)
+__global__ void k_ct_ice_groups(int n, const unsigned long long *__restrict__ gk, const unsigned long long *__restrict__ xk,
+                                float rad, const float *__restrict__ px, const float *__restrict__ py,
+                                const float *__restrict__ pz, float *__restrict__ vx, float *__restrict__ vy,
+                                float *__restrict__ vz, float *__restrict__ air) {
+  int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t0 >= n) return;
+  if (t0 > 0 && (gk[t0] >> 32) == (gk[t0 - 1] >> 32)) return;
+  int t1 = t0 + 1;
+  while (t1 < n && (gk[t1] >> 32) == (gk[t0] >> 32)) t1++;
+  if (t1 - t0 < 2) return;
+  const float rad2 = rad * rad;
+  for (int a = t0; a < t1; a++) {
+    unsigned i = member(gk, xk, a);
+    bool stuckI = air[i] == 1.f;
+    for (int b = a + 1; b < t1; b++) {
+      unsigned j = member(gk, xk, b);
+      if (px[j] - px[i] >= rad) break;
+      float ddx = px[j] - px[i], ddy = py[j] - py[i], ddz = pz[j] - pz[i];
+      if (ddx * ddx + ddy * ddy + ddz * ddz >= rad2) continue;
+      bool stuckJ = air[j] == 1.f;
+      if (stuckI && stuckJ) continue;
+      if (stuckJ) {
+        air[i] = 1.f; vx[i] = vy[i] = vz[i] = 0.f; stuckI = true;
+      } else if (stuckI) {
+        air[j] = 1.f; vx[j] = vy[j] = vz[j] = 0.f;
+      } else {
+        float mx = (vx[i] + vx[j]) * 0.5f, my = (vy[i] + vy[j]) * 0.5f, mz = (vz[i] + vz[j]) * 0.5f;
+        vx[i] = vx[j] = mx; vy[i] = vy[j] = my; vz[i] = vz[j] = mz;
+      }
+    }
+  }
+}
+
+// vapor x liquid: each parcel meets the liquid particles near it in index
+// order and is pushed out of each it overlaps (the water never moves)
+__global__ void k_ct_vapor_liquid(Geo g, int nV, const unsigned *__restrict__ vlist, const unsigned *__restrict__ llist,
+                                  const unsigned *__restrict__ lstart, const unsigned *__restrict__ lbinned, float rad,
+                                  float *__restrict__ px, float *__restrict__ py, float *__restrict__ pz,
+                                  float *__restrict__ vx, float *__restrict__ vy, float *__restrict__ vz,
+                                  unsigned *__restrict__ overflow) {
+  constexpr int MAXC = 64;
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nV) return;
+  unsigned j = vlist[e];
+  float x = px[j], y = py[j], z = pz[j];
+  // candidates within 3 radii of the parcel's position (it moves by at
+  // most one radius per contact), sorted by particle index
+  unsigned cand[MAXC];
+  int nc = 0;
+  const float reach = 3.f * rad, reach2 = reach * reach;
+  int ci = clampi((int)(x * g.inv_dx), 0, g.nx - 1), cj = clampi((int)(y * g.inv_dx), 0, g.ny - 1);
+  int ck = clampi((int)(z * g.inv_dx), 0, g.nz - 1);
+  int span = (int)ceilf(reach * g.inv_dx);
+  for (int kk = max(ck - span, 0); kk <= min(ck + span, g.nz - 1); kk++)
+    for (int jj = max(cj - span, 0); jj <= min(cj + span, g.ny - 1); jj++)
+      for (int ii = max(ci - span, 0); ii <= min(ci + span, g.nx - 1); ii++) {
+        int cc = (kk * g.ny + jj) * g.nx + ii;
+        for (unsigned b = lstart[cc]; b < lstart[cc + 1]; b++) {
+          unsigned p = llist[lbinned[b]];
+          float ddx = px[p] - x, ddy = py[p] - y, ddz = pz[p] - z;
+          if (ddx * ddx + ddy * ddy + ddz * ddz >= reach2) continue;
+          if (nc == MAXC) { atomicAdd(overflow, 1u); continue; }
+          int k = nc++;
+          while (k > 0 && cand[k - 1] > p) { cand[k] = cand[k - 1]; k--; }
+          cand[k] = p;
+        }
+      }
+  const float rad2 = rad * rad;
+  float jx = x, jy = y, jz = z, ux = vx[j], uy = vy[j], uz = vz[j];
+  for (int k = 0; k < nc; k++) {
+    unsigned p = cand[k];
+    float wx = px[p], wy = py[p], wz = pz[p];
+    // the reference's x window: binary search to x >= wx - rad, stop at x - wx >= rad
+    if (jx < wx - rad) continue;
+    float ddx = jx - wx;
+    if (ddx >= rad) continue;
+    float ddy = jy - wy, ddz = jz - wz;
+    float dd2 = ddx * ddx + ddy * ddy + ddz * ddz;
+    if (dd2 >= rad2 || dd2 < 1e-12f) continue;
+    float dd = sqrtf(dd2);
+    float nX = ddx / dd, nY = ddy / dd, nZ = ddz / dd;
+    float vR = (ux - vx[p]) * nX + (uy - vy[p]) * nY + (uz - vz[p]) * nZ;
+    if (vR < 0.f) {
+      float vB = vR * 1.4f;
+      ux -= vB * nX; uy -= vB * nY; uz -= vB * nZ;
+    }
+    float ov = rad - dd;
+    jx += nX * ov; jy += nY * ov; jz += nZ * ov;
+    float kk2 = ov * 2.f;
+    ux += nX * kk2; uy += nY * kk2; uz += nZ * kk2;
+  }
+  px[j] = jx; py[j] = jy; pz[j] = jz;
+  vx[j] = ux; vy[j] = uy; vz[j] = uz;
+}
+
+__global__ void k_ct_census(Geo g, int nV, const unsigned *__restrict__ vlist, const uint8_t *__restrict__ pflag,
+                            const float *__restrict__ px, const float *__restrict__ py, const float *__restrict__ pz,
+                            int hasSun, float sx, float sy, float sz, unsigned *__restrict__ counts) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nV) return;
+  unsigned p = vlist[e];
+  if (pflag[p] != 2) return;
+  atomicAdd(&counts[0], 1u);
+  if (!hasSun || (px[p] - g.cx) * sx + (py[p] - g.cy) * sy + (pz[p] - g.cz) * sz < 0.f) atomicAdd(&counts[1], 1u);
+}
+
+__global__ void k_ct_iota(int n, unsigned *__restrict__ a) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) a[e] = (unsigned)e;
+}
+
+__global__ void k_ct_zero(long n, unsigned *__restrict__ a) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) a[e] = 0u;
+}
+
+}  // namespace
+
+// scratch of the contact passes (allocated on first use, grown with nP)
+struct Contacts {
+  long cap = 0;
+  unsigned *fv = nullptr, *fi = nullptr, *fl = nullptr, *ov = nullptr, *oi = nullptr, *ol = nullptr;
+  unsigned *vlist = nullptr, *ilist = nullptr, *llist = nullptr;
+  unsigned *cellOf = nullptr, *rankIn = nullptr, *binned = nullptr, *parent = nullptr, *posOf = nullptr,
+           *rankOf = nullptr;
+  unsigned *cnt = nullptr, *start = nullptr, *lstart = nullptr, *lbinned = nullptr, *lcell = nullptr,
+           *lrank = nullptr, *counts = nullptr;
+  unsigned long long *xk = nullptr, *xk2 = nullptr, *gk = nullptr, *gk2 = nullptr;
+  void *tmp = nullptr;
+  size_t tmpBytes = 0;
+};
+
+namespace {
+
+template <typename T>
+int calloc_dev(ws_ctx *c, T **p, size_t n) {
+  hipError_t e = hipMalloc((void **)p, sizeof(T) * (n ? n : 1));
+  if (e != hipSuccess) { c->err = std::string("hipMalloc (contacts): ") + hipGetErrorString(e); return WS_EHIP; }
+  return WS_OK;
+}
+
+int contacts_ensure(ws_ctx *c) {
+  Contacts *k = c->contacts;
+  long n = c->capacity + 1;
+  if (k && k->cap >= n) return WS_OK;
+  if (!k) { k = new Contacts(); c->contacts = k; }
+  contacts_release(c, false);
+  long C = c->nC + 1;
+  int r;
+  if ((r = calloc_dev(c, &k->fv, n)) || (r = calloc_dev(c, &k->fi, n)) || (r = calloc_dev(c, &k->fl, n)) ||
+      (r = calloc_dev(c, &k->ov, n)) || (r = calloc_dev(c, &k->oi, n)) || (r = calloc_dev(c, &k->ol, n)) ||
+      (r = calloc_dev(c, &k->vlist, n)) || (r = calloc_dev(c, &k->ilist, n)) || (r = calloc_dev(c, &k->llist, n)) ||
+      (r = calloc_dev(c, &k->cellOf, n)) || (r = calloc_dev(c, &k->rankIn, n)) || (r = calloc_dev(c, &k->binned, n)) ||
+      (r = calloc_dev(c, &k->parent, n)) || (r = calloc_dev(c, &k->posOf, n)) || (r = calloc_dev(c, &k->rankOf, n)) ||
+      (r = calloc_dev(c, &k->cnt, C)) || (r = calloc_dev(c, &k->start, C)) || (r = calloc_dev(c, &k->lstart, C)) ||
+      (r = calloc_dev(c, &k->lbinned, n)) || (r = calloc_dev(c, &k->lcell, n)) || (r = calloc_dev(c, &k->lrank, n)) ||
+      (r = calloc_dev(c, &k->counts, 4)) || (r = calloc_dev(c, &k->xk, n)) || (r = calloc_dev(c, &k->xk2, n)) ||
+      (r = calloc_dev(c, &k->gk, n)) || (r = calloc_dev(c, &k->gk2, n)))
+    return r;
+  size_t a = 0, b = 0, d = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, a, k->fv, k->ov, (int)n, c->stream);
+  hipcub::DeviceScan::ExclusiveSum(nullptr, b, k->cnt, k->start, (int)C, c->stream);
+  hipcub::DeviceRadixSort::SortKeys(nullptr, d, k->xk, k->xk2, (int)n, 0, 64, c->stream);
+  k->tmpBytes = a > b ? a : b;
+  if (d > k->tmpBytes) k->tmpBytes = d;
+  if ((r = calloc_dev(c, (char **)&k->tmp, k->tmpBytes))) return r;
+  k->cap = n;
+  return WS_OK;
+}
+
+int bin_subset(ws_ctx *c, int n, const unsigned *list, unsigned *cellOf, unsigned *rankIn, unsigned *start,
+               unsigned *binned) {
+  Contacts *k = c->contacts;
+  long C = c->nC;
+  hipLaunchKernelGGL(k_ct_zero, dim3(blocks_for(C + 1, 256)), dim3(256), 0, c->stream, C + 1, k->cnt);
+  if (n > 0)
+    hipLaunchKernelGGL(k_ct_count, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, c->geo, n, list, c->px, c->py,
+                       c->pz, k->cnt, cellOf, rankIn);
+  WS_LAUNCHED();
+  WS_HIP(hipcub::DeviceScan::ExclusiveSum(k->tmp, k->tmpBytes, k->cnt, start, (int)C + 1, c->stream));
+  if (n > 0)
+    hipLaunchKernelGGL(k_ct_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, cellOf, rankIn, start,
+                       binned);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+// sorted x order of a subset, its connected groups (pairs closer than thr)
+// and the group keys sorted by (group, x rank); leaves the sorted x keys in
+// xk and the sorted group keys in gk
+int group_subset(ws_ctx *c, int n, const unsigned *list, float thr) {
+  Contacts *k = c->contacts;
+  int nb = blocks_for(n, 256);
+  hipLaunchKernelGGL(k_ct_keys, dim3(nb), dim3(256), 0, c->stream, n, list, c->px, k->xk2);
+  WS_LAUNCHED();
+  WS_HIP(hipcub::DeviceRadixSort::SortKeys(k->tmp, k->tmpBytes, k->xk2, k->xk, n, 0, 64, c->stream));
+  hipLaunchKernelGGL(k_ct_pos, dim3(nb), dim3(256), 0, c->stream, n, list, k->posOf);
+  hipLaunchKernelGGL(k_ct_rank, dim3(nb), dim3(256), 0, c->stream, n, k->xk, k->posOf, k->rankOf);
+  WS_LAUNCHED();
+  int r;
+  if ((r = bin_subset(c, n, list, k->cellOf, k->rankIn, k->start, k->binned))) return r;
+  hipLaunchKernelGGL(k_ct_iota, dim3(nb), dim3(256), 0, c->stream, n, k->parent);
+  hipLaunchKernelGGL(k_ct_union, dim3(nb), dim3(256), 0, c->stream, c->geo, n, list, c->px, c->py, c->pz, k->cellOf,
+                     k->start, k->binned, thr * thr, k->parent);
+  hipLaunchKernelGGL(k_ct_groupkeys, dim3(nb), dim3(256), 0, c->stream, n, k->parent, k->rankOf, k->gk2);
+  WS_LAUNCHED();
+  WS_HIP(hipcub::DeviceRadixSort::SortKeys(k->tmp, k->tmpBytes, k->gk2, k->gk, n, 0, 64, c->stream));
+  return WS_OK;
+}
+
+int subset(ws_ctx *c, unsigned *flag, unsigned *off, unsigned *list, int *count) {
+  Contacts *k = c->contacts;
+  int n = c->nP;
+  WS_HIP(hipcub::DeviceScan::ExclusiveSum(k->tmp, k->tmpBytes, flag, off, n + 1, c->stream));
+  hipLaunchKernelGGL(k_ct_compact, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, flag, off, list);
+  WS_LAUNCHED();
+  unsigned h = 0;
+  WS_HIP(hipMemcpyAsync(&h, off + n, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+  WS_HIP(hipStreamSynchronize(c->stream));
+  *count = (int)h;
+  return WS_OK;
+}
+
+int flags(ws_ctx *c) {
+  Contacts *k = c->contacts;
+  double band = c->oceanR - c->dx * 2;
+  hipLaunchKernelGGL(k_ct_flags, dim3(blocks_for(c->nP + 1, 256)), dim3(256), 0, c->stream, c->geo, c->nP, c->px,
+                     c->py, c->pz, c->pflag, (float)(band * band), k->fv, k->fi, k->fl);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+}  // namespace
+
+void contacts_release(ws_ctx *c, bool all) {
+  Contacts *k = c->contacts;
+  if (!k) return;
+  void *ptrs[] = {k->fv, k->fi, k->fl, k->ov, k->oi, k->ol, k->vlist, k->ilist, k->llist, k->cellOf, k->rankIn,
+                  k->binned, k->parent, k->posOf, k->rankOf, k->cnt, k->start, k->lstart, k->lbinned, k->lcell,
+                  k->lrank, k->counts, k->xk, k->xk2, k->gk, k->gk2, k->tmp};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (all) {
+    delete k;
+    c->contacts = nullptr;
+  } else {
+    *k = Contacts();
+  }
+}
+
+// _vaporCollisions, solver.js:3345-3479
+int frame_vapor_collisions(ws_ctx *c) {
+  if (c->nP < 1) return WS_OK;
+  int r;
+  if ((r = contacts_ensure(c)) || (r = flags(c))) return r;
+  Contacts *k = c->contacts;
+  int nV = 0;
+  if ((r = subset(c, k->fv, k->ov, k->vlist, &nV))) return r;
+  hipLaunchKernelGGL(k_ct_zero, dim3(1), dim3(64), 0, c->stream, 4L, k->counts);
+  if (nV > 0) {
+    double sx = c->sunX - c->cx, sy = c->sunY - c->cy, sz = c->sunZ - c->cz;
+    hipLaunchKernelGGL(k_ct_census, dim3(blocks_for(nV, 256)), dim3(256), 0, c->stream, c->geo, nV, k->vlist, c->pflag,
+                       c->px, c->py, c->pz, c->hasSun, (float)sx, (float)sy, (float)sz, k->counts);
+  }
+  WS_LAUNCHED();
+  if (nV >= 2) {
+    float rad = (float)(c->spacing * 0.6);
+    if ((r = group_subset(c, nV, k->vlist, 2.f * rad))) return r;
+    hipLaunchKernelGGL(k_ct_vapor_groups, dim3(blocks_for(nV, 256)), dim3(256), 0, c->stream, nV, k->gk, k->xk, rad,
+                       c->spinOn, c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pWx, c->pWy, c->pWz);
+    WS_LAUNCHED();
+    // liquid partners, binned by cell (they do not move in this pass)
+    int nL = 0;
+    if ((r = subset(c, k->fl, k->ol, k->llist, &nL))) return r;
+    if ((r = bin_subset(c, nL, k->llist, k->lcell, k->lrank, k->lstart, k->lbinned))) return r;
+    hipLaunchKernelGGL(k_ct_vapor_liquid, dim3(blocks_for(nV, 256)), dim3(256), 0, c->stream, c->geo, nV, k->vlist,
+                       k->llist, k->lstart, k->lbinned, rad, c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz,
+                       k->counts + 2);
+    WS_LAUNCHED();
+  }
+  unsigned h[4];
+  WS_HIP(hipMemcpyAsync(h, k->counts, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  WS_HIP(hipStreamSynchronize(c->stream));
+  c->vaporCount = (int)h[0];
+  c->nightVaporCount = (int)h[1];
+  c->contactOverflow += h[2];
+  return WS_OK;
+}
+
+// _iceContacts, solver.js:3290-3324
+int frame_ice_contacts(ws_ctx *c) {
+  if (c->nP < 1) return WS_OK;
+  int r;
+  if ((r = contacts_ensure(c)) || (r = flags(c))) return r;
+  Contacts *k = c->contacts;
+  int nI = 0;
+  if ((r = subset(c, k->fi, k->oi, k->ilist, &nI))) return r;
+  if (nI < 2) return WS_OK;
+  float rad = (float)(c->spacing * 0.7);
+  // groups from a radius a hair above the contact radius, so that no pair
+  // the reference's double-precision test accepts is split
+  if ((r = group_subset(c, nI, k->ilist, rad * (1.f + 1e-5f)))) return r;
+  hipLaunchKernelGGL(k_ct_ice_groups, dim3(blocks_for(nI, 256)), dim3(256), 0, c->stream, nI, k->gk, k->xk, rad, c->px,
+                     c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pAir);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+}  // namespace ws

@@ -11,5 +11,8 @@ int frame_evaporation(ws_ctx *c, double dt);
 int frame_currents(ws_ctx *c, double dt);
 int frame_phase(ws_ctx *c, double dt);
 int frame_push_surface(ws_ctx *c);
+int frame_vapor_collisions(ws_ctx *c);
+int frame_ice_contacts(ws_ctx *c);
+void contacts_release(ws_ctx *c, bool all);
 World make_world(ws_ctx *c);
 }  // namespace ws

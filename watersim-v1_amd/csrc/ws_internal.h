@@ -57,7 +57,8 @@ struct Geo {
   float cx_lo, cy_lo, cz_lo;  // double centre minus its float rounding
 };
 
-struct Net;  // the learned warm start (ws_learn.h)
+struct Net;       // the learned warm start (ws_learn.h)
+struct Contacts;  // scratch of the frame-end contact passes (ws_contacts.hip)
 
 struct Ctx {
   // geometry
@@ -103,6 +104,11 @@ struct Ctx {
   // starts from its prediction, and whether solves record training samples
   Net *net = nullptr;
   int warmStart = 0, recordSamples = 0;
+  int netTiled = 1;  // convolutions staged through LDS (0: the untiled kernel)
+  // frame-end contact passes: scratch, the vapor census, candidate overflows
+  Contacts *contacts = nullptr;
+  int vaporCount = 0, nightVaporCount = 0;
+  long contactOverflow = 0;
   // both solves per pressure stage, timed separately (measurement)
   int pressureCompare = 0;
   double cmpMsPlain = 0, cmpMsLearned = 0;

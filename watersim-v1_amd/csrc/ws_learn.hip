@@ -80,6 +80,96 @@ __global__ __launch_bounds__(256) void k_conv3(NetDims d, int Cin, int K, int fl
   }
 }
 
+// The same convolution with the input tile staged in LDS. A block computes
+// a 16 x 8 x (MT/2) voxel tile: each of its four waves owns MT rows of 16
+// voxels along x and all output channels. The input region (tile plus a
+// one-voxel halo) is loaded into LDS one chunk of CH channels at a time
+// (CH = 16, or 8 for an 8-channel input), so every input element comes from
+// L2 once per block and the 27 taps read it from LDS. One MFMA step covers
+// 32 K values: two taps of 16 channels (four taps of 8 when CH = 8).
+template <int NT, int OUT, int MT>
+__global__ __launch_bounds__(256) void k_conv3l(NetDims d, int Cin, int K, int flip, const bf16_t *__restrict__ in,
+                                                const bf16_t *__restrict__ W, const float *__restrict__ bias,
+                                                void *__restrict__ out) {
+  constexpr int TX = 16, TY = 8, TZ = MT / 2, RX = TX + 2, RY = TY + 2, RZ = TZ + 2, RN = RX * RY * RZ;
+  __shared__ uint4 reg[RN * 2];  // RN voxels x 16 channels of bf16 (32 bytes each)
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int tilesX = (d.X + TX - 1) / TX, tilesY = (d.Y + TY - 1) / TY;
+  const int b = blockIdx.x;
+  const int x0 = (b % tilesX) * TX, y0 = ((b / tilesX) % tilesY) * TY, z0 = (b / (tilesX * tilesY)) * TZ;
+  const int CH = Cin < 16 ? 8 : 16;  // channels per staged chunk
+  const int tpl = CH == 16 ? 2 : 4;  // taps per MFMA step
+  const int steps = (27 + tpl - 1) / tpl;
+  // this lane's tap within a step and its 8-channel group
+  const int tsub = CH == 16 ? (lane >> 5) : (lane >> 4);
+  const int cgrp = CH == 16 ? ((lane >> 4) & 1) : 0;
+  f4 acc[MT][NT];
+#pragma unroll
+  for (int m = 0; m < MT; m++)
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) acc[m][nt] = f4{0.f, 0.f, 0.f, 0.f};
+  const int nchunk = Cin / CH;
+  for (int ch = 0; ch < nchunk; ch++) {
+    __syncthreads();
+    // stage the region: one uint4 (8 channels) per element
+    const int per = CH / 8;
+    for (int e = threadIdx.x; e < RN * per; e += 256) {
+      int r = e / per, hlf = e - r * per;
+      int rx = r % RX, ry = (r / RX) % RY, rz = r / (RX * RY);
+      int gx = x0 - 1 + rx, gy = y0 - 1 + ry, gz = z0 - 1 + rz;
+      uint4 v = {0u, 0u, 0u, 0u};
+      if (gx >= 0 && gx < d.X && gy >= 0 && gy < d.Y && gz >= 0 && gz < d.Z)
+        v = *(const uint4 *)(in + (((long)gz * d.Y + gy) * d.X + gx) * Cin + ch * CH + hlf * 8);
+      reg[r * 2 + hlf] = v;
+    }
+    __syncthreads();
+    for (int st = 0; st < steps; st++) {
+      const int tap = st * tpl + tsub;
+      const bool tapOk = tap < 27;
+      int dz = 0, dy = 0, dx = 0;
+      if (tapOk) tap_offset(tap, flip, dz, dy, dx);
+      bf8 bw[NT];
+      const long kk = (long)tap * Cin + ch * CH + cgrp * 8;
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++) {
+        bw[nt] = tapOk ? *(const bf8 *)(W + (long)(nt * 16 + (lane & 15)) * K + kk) : bf8{};
+      }
+#pragma unroll
+      for (int m = 0; m < MT; m++) {
+        const int row = wv * MT + m;          // row of 16 voxels within the tile
+        const int ly = row % TY, lz = row / TY;
+        const int rx = (lane & 15) + 1 + dx, ry = ly + 1 + dy, rz = lz + 1 + dz;
+        const uint4 av = reg[((rz * RY + ry) * RX + rx) * 2 + cgrp];
+        const bf8 a = __builtin_bit_cast(bf8, av);
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) acc[m][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[nt], acc[m][nt], 0, 0, 0);
+      }
+    }
+  }
+  constexpr int Cout = NT * 16;
+#pragma unroll
+  for (int m = 0; m < MT; m++) {
+    const int row = wv * MT + m;
+    const int gy = y0 + row % TY, gz = z0 + row / TY;
+    if (gy >= d.Y || gz >= d.Z) continue;
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) {
+      const int n = nt * 16 + (lane & 15);
+      const float bn = bias ? bias[n] : 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int gx = x0 + 4 * (lane >> 4) + i;
+        if (gx >= d.X) continue;
+        const long v = ((long)gz * d.Y + gy) * d.X + gx;
+        float val = acc[m][nt][i] + bn;
+        if (OUT == 0) val = fmaxf(val, 0.f);
+        if (OUT == 2) ((float *)out)[v * Cout + n] = val;
+        else ((bf16_t *)out)[v * Cout + n] = f2bf(val);
+      }
+    }
+  }
+}
+
 // Weight gradient partials: part[chunk][co][k] = sum over the chunk's
 // voxels v of gz[v][co] * x[v + off(tap)][ci], k = tap*Cin + ci. The
 // product has M = output channels, N = k, and voxels as its K dimension;
@@ -475,6 +565,12 @@ __global__ void k_ws_target(long C, const uint8_t *__restrict__ ct, const float 
                                             void *);
 CONV(1, 0) CONV(2, 0) CONV(4, 0) CONV(1, 2) CONV(2, 2) CONV(3, 2) CONV(4, 2) CONV(6, 2)
 #undef CONV
+#define CONVL(NT, OUT, MT)                                                                                    \
+  template __global__ void k_conv3l<NT, OUT, MT>(NetDims, int, int, int, const bf16_t *, const bf16_t *,     \
+                                                 const float *, void *);
+CONVL(1, 0, 8) CONVL(2, 0, 8) CONVL(4, 0, 4) CONVL(1, 2, 8) CONVL(2, 2, 8) CONVL(3, 2, 4) CONVL(4, 2, 4)
+CONVL(6, 2, 4)
+#undef CONVL
 template __global__ void k_wgrad<1>(NetDims, int, int, int, const bf16_t *, const bf16_t *, float *);
 template __global__ void k_wgrad<2>(NetDims, int, int, int, const bf16_t *, const bf16_t *, float *);
 template __global__ void k_wgrad<4>(NetDims, int, int, int, const bf16_t *, const bf16_t *, float *);
@@ -512,7 +608,15 @@ NetDims half(NetDims d) { return NetDims{d.Z / 2, d.Y / 2, d.X / 2}; }
 template <int NT, int OUT>
 void conv(ws_ctx *c, NetDims d, int cin, int K, int flip, const bf16_t *in, const bf16_t *W, const float *b, void *out) {
   long V = vox(d);
-  hipLaunchKernelGGL((k_conv3<NT, OUT>), dim3(blocks_for(V, 64)), dim3(256), 0, c->stream, d, cin, K, flip, in, W, b,
+  if (!c->netTiled) {  // the untiled form of the same product
+
+    hipLaunchKernelGGL((k_conv3<NT, OUT>), dim3(blocks_for(V, 64)), dim3(256), 0, c->stream, d, cin, K, flip, in, W,
+                       b, out);
+    return;
+  }
+  constexpr int MT = NT >= 3 ? 4 : 8;
+  int tx = (d.X + 15) / 16, ty = (d.Y + 7) / 8, tz = (d.Z + MT / 2 - 1) / (MT / 2);
+  hipLaunchKernelGGL((k_conv3l<NT, OUT, MT>), dim3(tx * ty * tz), dim3(256), 0, c->stream, d, cin, K, flip, in, W, b,
                      out);
 }
 

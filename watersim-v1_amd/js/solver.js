@@ -36,7 +36,7 @@ var AIR = 0, FLUID = 1, SOLID = 2;
 var STAGE = {
   clear: 0, rasterize: 1, p2g: 2, bc: 3, snapshot: 4, viscosity: 5, gravity: 6, pressure: 7,
   project: 8, vorticity: 9, clamp: 10, extrapolate: 11, g2p: 12, advect: 13, heat: 14,
-  evaporation: 15, currents: 16, phase: 17, pushSurface: 18
+  evaporation: 15, currents: 16, phase: 17, pushSurface: 18, vaporCollisions: 19, iceContacts: 20
 };
 
 // scalars mirrored to the device before each call (js/solver.js:56-75, 137-193)
@@ -308,6 +308,8 @@ FluidSolver.prototype._updateEvaporation = stageMethod(STAGE.evaporation);
 FluidSolver.prototype._updateCurrents = stageMethod(STAGE.currents);
 FluidSolver.prototype._updatePhaseChanges = stageMethod(STAGE.phase);
 FluidSolver.prototype._pushSurfaceParticles = stageMethod(STAGE.pushSurface);
+FluidSolver.prototype._vaporCollisions = stageMethod(STAGE.vaporCollisions);
+FluidSolver.prototype._iceContacts = stageMethod(STAGE.iceContacts);
 
 // ---------------------------------------------------------------- surface
 FluidSolver.prototype.splatDensity = function () {

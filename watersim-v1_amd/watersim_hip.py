@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(HERE, "_build", "libwatersim_hip.so")
 STAGES = {
     "clear": 0, "rasterize": 1, "p2g": 2, "bc": 3, "snapshot": 4, "viscosity": 5, "gravity": 6,
     "pressure": 7, "project": 8, "vorticity": 9, "clamp": 10, "extrapolate": 11, "g2p": 12, "advect": 13,
-    "heat": 14, "evaporation": 15, "currents": 16, "phase": 17, "push_surface": 18,
+    "heat": 14, "evaporation": 15, "currents": 16, "phase": 17, "push_surface": 18, "vapor_collisions": 19,
+    "ice_contacts": 20,
 }
 
 # exported entry points, as declared in include/watersim.h, watersim_world.h
