@@ -735,7 +735,7 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
   SF(atmosphereH) SF(spacing) SF(particleVolume) SF(iso) SF(substeps) SF(rainT) SF(cloudT) SF(rainLiftAge)
   SF(pressureTol) SF(dv) SF(sunX) SF(sunY) SF(sunZ) SF(dtFrameBase)
   SF(heatK) SF(Tamb) SF(shadeCool) SF(sunActivity) SF(evapIntensity) SF(snowT) SF(meltT) SF(cloudP)
-  SF(currents) SF(Rlo) SF(Rhi) SF(simTime) SF(heatAcc)
+  SF(currents) SF(Rlo) SF(Rhi) SF(simTime) SF(heatAcc) SF(netLossCoarse)
   SI(pressureIters) SI(maxPressureIters) SI(residualEvery) SI(ceilReflect) SI(hasSun) SI(measureResidual)
   SI(adaptiveChecks) SI(sorMode) SI(spinOn) SI(warmStart) SI(recordSamples) SI(pressureCompare) SI(netTiled)
 #undef SF

@@ -962,10 +962,9 @@ __global__ void k_vort_splat(Geo g, const uint8_t *__restrict__ ct, const float 
 // +i/+j/+k neighbours. Tiles of 32x8x8 faces are processed one tile-diagonal
 // (I+J+K) per launch. Inside a tile one wavefront walks the hyperplanes
 // li+lj+lk: lane (lj, lk) handles face li = step - lj - lk, so the -j / -k
-// neighbours it reads were written by lanes one step earlier. The lanes of
-// one wavefront execute their LDS operations in program order, so the steps
-// need no workgroup barrier. All four waves load the tile (with a one-face
-// halo) into LDS; three of them then leave.
+// neighbours it needs were decided by lanes one step earlier and arrive by
+// lane shuffles; the steps need no barrier. All four waves load the tile
+// (with a one-face halo) into LDS; three of them then leave.
 __global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restrict__ tiles, int off,
                                                      float *__restrict__ vel, uint8_t *__restrict__ valid) {
   constexpr int TI = EX_TI, TJ = EX_TJ, TK = EX_TK;
@@ -1008,17 +1007,28 @@ __global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restri
   int lj = 1 + (threadIdx.x % TJ), lk = 1 + (threadIdx.x / TJ);
   int j = j0 + lj, k = k0 + lk;
   bool row = j < d1 && k < d2;
+  // the LDS copy stays the original state; a face's final state travels in
+  // registers: a lane keeps what it decided at the previous step (its -i
+  // neighbour now) and reads the -j / -k neighbours from the lanes one row
+  // and one plane back, which decided the same li one step earlier
+  float lastV = 0.f;
+  int lastF = 0;
+  const float rcp[7] = {0.f, 1.f, 0.5f, 1.f / 3.f, 0.25f, 0.2f, 1.f / 6.f};
   for (int step = 0; step < TI + TJ + TK - 2; step++) {
     int li = 1 + step - (lj - 1) - (lk - 1);
     int i = i0 + li;
+    float vjm = __shfl_up(lastV, 1), vkm = __shfl_up(lastV, TJ);
+    int fjm = __shfl_up(lastF, 1), fkm = __shfl_up(lastF, TJ);
     if (row && li >= 1 && li <= TI && i < d0) {
-      // all flags and values of the face and its six neighbours are read at
-      // once (independent LDS loads), then combined in the reference's order
       uint8_t f0 = sf[lk][lj][li];
-      uint8_t fa = sf[lk][lj][li - 1], fb = sf[lk][lj][li + 1], fc = sf[lk][lj - 1][li];
-      uint8_t fd = sf[lk][lj + 1][li], fe = sf[lk - 1][lj][li], ff = sf[lk + 1][lj][li];
-      float va = sv[lk][lj][li - 1], vb = sv[lk][lj][li + 1], vc = sv[lk][lj - 1][li];
-      float vd = sv[lk][lj + 1][li], ve = sv[lk - 1][lj][li], vf = sv[lk + 1][lj][li];
+      uint8_t fb = sf[lk][lj][li + 1], fd = sf[lk][lj + 1][li], ff = sf[lk + 1][lj][li];
+      float v0 = sv[lk][lj][li], vb = sv[lk][lj][li + 1], vd = sv[lk][lj + 1][li], vf = sv[lk + 1][lj][li];
+      float va = li > 1 ? lastV : sv[lk][lj][0];
+      int fa = li > 1 ? lastF : sf[lk][lj][0];
+      float vc = lj > 1 ? vjm : sv[lk][0][li];
+      int fc = lj > 1 ? fjm : sf[lk][0][li];
+      float ve = lk > 1 ? vkm : sv[0][lj][li];
+      int fe = lk > 1 ? fkm : sf[0][lj][li];
       bool ua = i > 0 && fa, ub = i < d0 - 1 && fb, uc = j > 0 && fc;
       bool ud = j < d1 - 1 && fd, ue = k > 0 && fe, uf = k < d2 - 1 && ff;
       float sum = 0.f;
@@ -1032,18 +1042,17 @@ __global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restri
       if (!f0 && cnt > 0) {
         // the mean through a reciprocal (within 1.5 ulp of the quotient; the
         // reference divides in double and rounds, so neither is exact here)
-        const float rcp[7] = {0.f, 1.f, 0.5f, 1.f / 3.f, 0.25f, 0.2f, 1.f / 6.f};
         float v = sum * rcp[cnt];
-        sv[lk][lj][li] = v;
-        sf[lk][lj][li] = 2;
         long f = ((long)k * d1 + j) * d0 + i;
         A[f] = v;
         V[f] = 1;
+        lastV = v;
+        lastF = 1;
+      } else {
+        lastV = v0;
+        lastF = f0 ? 1 : 0;
       }
     }
-    // keeps the compiler from moving this step's LDS stores past the next
-    // step's loads; the hardware already orders them within the wavefront
-    __builtin_amdgcn_wave_barrier();
   }
 }
 

@@ -105,6 +105,7 @@ struct Ctx {
   Net *net = nullptr;
   int warmStart = 0, recordSamples = 0;
   int netTiled = 1;  // convolutions staged through LDS (0: the untiled kernel)
+  double netLossCoarse = 0;  // weight of the 4x4x4-averaged error term of the training loss
   // frame-end contact passes: scratch, the vapor census, candidate overflows
   Contacts *contacts = nullptr;
   int vaporCount = 0, nightVaporCount = 0;

@@ -449,6 +449,49 @@ __global__ void k_loss_grad(long V, const float *__restrict__ y, const float *__
   g[v] = m[v] ? (float)(2.0 * ((double)y[v] - t[v]) / den) : 0.f;
 }
 
+// coarse-scale term of the loss: the masked error and target averaged over
+// 4x4x4 blocks (the smooth part SOR removes slowest), block partials of
+// sum e4^2 and sum t4^2
+__global__ void k_loss4_part(NetDims d, const float *__restrict__ y, const float *__restrict__ t,
+                             const uint8_t *__restrict__ m, float *__restrict__ e4, double *__restrict__ part) {
+  __shared__ double se[256], st[256];
+  NetDims c4{d.Z / 4, d.Y / 4, d.X / 4};
+  long V4 = (long)c4.Z * c4.Y * c4.X;
+  double a = 0, b = 0;
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V4; v += (long)gridDim.x * blockDim.x) {
+    int x = (int)(v % c4.X), yy = (int)((v / c4.X) % c4.Y), z = (int)(v / ((long)c4.X * c4.Y));
+    float se4 = 0.f, st4 = 0.f;
+    for (int q = 0; q < 64; q++) {
+      long f = ((long)(4 * z + (q >> 4)) * d.Y + (4 * yy + ((q >> 2) & 3))) * d.X + (4 * x + (q & 3));
+      if (m[f]) { se4 += y[f] - t[f]; st4 += t[f]; }
+    }
+    se4 *= (1.f / 64.f);
+    st4 *= (1.f / 64.f);
+    e4[v] = se4;
+    a += (double)se4 * se4;
+    b += (double)st4 * st4;
+  }
+  se[threadIdx.x] = a;
+  st[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) { se[threadIdx.x] += se[threadIdx.x + o]; st[threadIdx.x] += st[threadIdx.x + o]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = se[0]; part[2 * blockIdx.x + 1] = st[0]; }
+}
+// gout += w * d(sum e4^2 / sum t4^2) / dy = w * m * 2 e4(parent) / 64 / sum t4^2
+__global__ void k_loss4_grad(NetDims d, const uint8_t *__restrict__ m, const float *__restrict__ e4,
+                             const double *__restrict__ sums, float w, float *__restrict__ g) {
+  long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long V = (long)d.Z * d.Y * d.X;
+  if (v >= V || !m[v]) return;
+  int x = (int)(v % d.X), y = (int)((v / d.X) % d.Y), z = (int)(v / ((long)d.X * d.Y));
+  long p = ((long)(z >> 2) * (d.Y >> 2) + (y >> 2)) * (d.X >> 2) + (x >> 2);
+  double den = sums[1] > 1e-30 ? sums[1] : 1e-30;
+  g[v] += (float)(w * 2.0 * e4[p] / 64.0 / den);
+}
+
 // Adam on the flat parameter vector (FP32 master weights)
 __global__ void k_adam(long n, float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m1,
                        float *__restrict__ m2, float lr, float b1, float b2, float eps, float c1, float c2,
@@ -872,11 +915,21 @@ static int net_loss(ws_ctx *c, NetDims d, double *loss) {
   hipLaunchKernelGGL(k_loss_grad, dim3(blocks_for(V, 256)), dim3(256), 0, c->stream, V, n->out, n->tgt, n->mask,
                      n->sums + 1024, n->gout);
   WS_LAUNCHED();
+  float wc = (float)c->netLossCoarse;
+  if (wc > 0) {
+    // the coarse error goes into the f32 gradient scratch gB (free before the backward pass)
+    hipLaunchKernelGGL(k_loss4_part, dim3(256), dim3(256), 0, c->stream, d, n->out, n->tgt, n->mask, n->gB, n->sums);
+    hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(64), 0, c->stream, 256, n->sums, n->sums + 1026);
+    hipLaunchKernelGGL(k_loss4_grad, dim3(blocks_for(V, 256)), dim3(256), 0, c->stream, d, n->mask, n->gB,
+                       n->sums + 1026, wc, n->gout);
+    WS_LAUNCHED();
+  }
   if (loss) {
-    double h[2];
+    double h[4];
     WS_HIP(hipMemcpyAsync(h, n->sums + 1024, sizeof h, hipMemcpyDeviceToHost, c->stream));
     WS_HIP(hipStreamSynchronize(c->stream));
     *loss = h[1] > 1e-30 ? h[0] / h[1] : h[0] / 1e-30;
+    if (wc > 0) *loss += wc * (h[3] > 1e-30 ? h[2] / h[3] : 0.0);
   }
   return WS_OK;
 }

@@ -47,6 +47,8 @@ def main():
     ap.add_argument("--train-every", type=int, default=24, help="recorded substeps between training bursts")
     ap.add_argument("--burst", type=int, default=24, help="updates per training burst")
     ap.add_argument("--final", type=int, default=600, help="updates after the last seed")
+    ap.add_argument("--coarse", type=float, default=1.0,
+                    help="weight of the loss term on the 4x4x4-averaged error (the smooth modes SOR is slow on)")
     ap.add_argument("--init", default=None, help="start from these weights")
     ap.add_argument("--out", required=True)
     a = ap.parse_args()
@@ -67,6 +69,7 @@ def main():
                 np.random.default_rng(a.seed0), len(s.net_params()))
             s.set_net_params(w0)
             s.net_bank(a.bank)
+            s.set(netLossCoarse=a.coarse)
         watersim_hip.load_world(s, world)
         s.set(pic=0.05, omega=1.7, gravity=9.8, maxSpeed=3.2, vorticity=0.05, cfl=3.0)
         s.set(pressureTol=1e-4, maxPressureIters=4000, residualEvery=8, recordSamples=0)
