@@ -140,6 +140,8 @@ def advance(s, world, cfg, n, done):
         if cfg["physics"]:
             set_sun(s, world, done + k)
         m = s.step(FRAME)
+        if (k + m) // 250 != k // 250:
+            log("  %d substeps" % (k + m))
         if every and (done + k) // every != (done + k + m) // every:
             # the surface, iso calibrated to the water volume (js/solver.js:3838-3849)
             s.splat_density()
@@ -292,7 +294,8 @@ def main():
     sor_txt = "to residual %g" % cfg["tol"] if cfg["tol"] else "%d sweeps" % cfg["iters"]
     if cfg["physics"]:
         wl = ("%s: %d^3 MAC grid, %d FLIP particles, red-black SOR %s, vorticity %g, full thermal cycle "
-              "(25 Hz heat + evaporation tick, frame-end phase pass, rotating sun, 270-300 K latitude profile), "
+              "(25 Hz heat + evaporation tick, frame-end vapor collisions, phase pass, snow contacts and surface "
+              "rescue, rotating sun, 270-300 K latitude profile), "
               "1/60 s frames" % (a.config, n, nP, sor_txt, cfg["vorticity"]))
     elif cfg.get("frames"):
         wl = ("%s: %d^3 MAC grid, %d FLIP particles, divergence-free curl-noise start (RMS %g m/s), red-black SOR "

@@ -61,8 +61,13 @@ enum ws_stage_id {
   WS_STAGE_PUSH_SURFACE = 18, /* _pushSurfaceParticles, :3486-3587 */
   WS_STAGE_VAPOR_COLLISIONS = 19, /* _vaporCollisions, :3345-3479 */
   WS_STAGE_ICE_CONTACTS = 20,  /* _iceContacts, :3290-3324 */
-  WS_STAGE_COUNT = 21
+  WS_STAGE_PUSH_BALLS = 21,  /* _pushOutOfBalls, :1005-1026 (then the particle bins are rebuilt) */
+  WS_STAGE_UPDATE_BALLS = 22, /* _updateBalls(dt), :2344-2564 */
+  WS_STAGE_COUNT = 23
 };
+
+/* Most balls a context holds. */
+#define WS_MAX_BALLS 64
 
 /* new FluidSolver({nx,ny,nz,dx}) plus _allocParticles(capacity),
  * solver.js:45-231. Device memory for every field is allocated here. */
@@ -88,8 +93,11 @@ int ws_get_f(ws_ctx *c, const char *name, double *v);
  * (f32 faces), "validU","validV","validW" (u8 faces), "q","rhs" (f32, C),
  * "dens" (f32, corners), "advU","advV","advW" (f32, nP; the G2P samples),
  * and the static world: "staticTypes" (u8, C), "gu","gv","gw" (f32 faces),
- * "terrainR" (f32, tn^3). nbytes must equal the field's byte size (for
- * particle fields: nP elements). */
+ * "terrainR" (f32, tn^3); with balls also "svx","svy","svz" (f32, C, the
+ * solid cell velocities), "ballHist" (u32, 25 per ball: the waterline
+ * histogram and its count) and "ballSurf" (f32 per ball, the waterline).
+ * nbytes must equal the field's byte size (for particle fields: nP
+ * elements). */
 int ws_upload(ws_ctx *c, const char *name, const void *host, size_t nbytes);
 int ws_download(ws_ctx *c, const char *name, void *host, size_t nbytes);
 
@@ -123,6 +131,16 @@ int ws_mt_build(ws_ctx *c, const float *dens, int nx, int ny, int nz, double dx,
 int ws_autotune_iso(ws_ctx *c, double *iso);
 /* kineticEnergy(), solver.js:3914-3921 (FP64 on the device). */
 int ws_kinetic_energy(ws_ctx *c, double *ke);
+
+/* Balls (moving solids): the `balls` array of FluidSolver (solver.js:115)
+ * and addBall (solver.js:3910-3912). State is 9 doubles per ball, in order
+ * x y z vx vy vz r rho submerged, for n <= WS_MAX_BALLS balls. ws_set_balls
+ * replaces the context's balls (n = 0 removes them); the substep rasterizes
+ * them as moving solids (:918-938), pushes particles out of them after the
+ * marking pass and after advection (:1005-1026) and integrates them at its
+ * end (:2344-2564). ws_get_balls reads the current state back. */
+int ws_set_balls(ws_ctx *c, int n, const double *state);
+int ws_get_balls(ws_ctx *c, int n, double *state);
 
 /* Waits for all work queued on the context's stream. */
 int ws_sync(ws_ctx *c);

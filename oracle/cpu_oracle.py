@@ -69,6 +69,16 @@ def lib(bits=32):
         L.oc_autotune_iso.restype = d
         L.oc_kinetic_energy.argtypes = [vp]
         L.oc_kinetic_energy.restype = d
+        L.oc_set_balls.argtypes = [vp, ci, vp]
+        L.oc_set_balls.restype = ci
+        L.oc_get_balls.argtypes = [vp, ci, vp]
+        L.oc_get_balls.restype = ci
+        L.oc_push_balls.argtypes = [vp]
+        L.oc_update_balls.argtypes = [vp, d]
+        L.oc_ball_hist.argtypes = [vp]
+        L.oc_ball_hist.restype = ctypes.POINTER(ctypes.c_float)
+        L.oc_ball_surf.argtypes = [vp]
+        L.oc_ball_surf.restype = ctypes.POINTER(ctypes.c_float)
         _LIBS[bits] = L
     return _LIBS[bits]
 
@@ -179,6 +189,28 @@ class OracleState:
 
     def call(self, fn, *args):
         return getattr(self.L, fn)(self.ctx, *args)
+
+    def set_balls(self, balls):
+        """balls: (n, 9) doubles x y z vx vy vz r rho submerged; binds the
+        solid-velocity fields on first use"""
+        b = np.ascontiguousarray(np.asarray(balls, np.float64).reshape(-1, 9))
+        if "svx" not in self.arrays:
+            for k in ["svx", "svy", "svz"]:
+                self._bind(k, np.zeros(self.nC, self.real))
+        if self.L.oc_set_balls(self.ctx, len(b), _ptr(b)) != 0:
+            raise ValueError("too many balls")
+
+    def balls(self):
+        n = self.get_i("nBalls")
+        out = np.zeros((n, 9), np.float64)
+        self.L.oc_get_balls(self.ctx, n, _ptr(out))
+        return out
+
+    def ball_hist(self):
+        n = self.get_i("nBalls")
+        h = np.ctypeslib.as_array(self.L.oc_ball_hist(self.ctx), shape=(max(n, 1) * 24,))[:n * 24].copy()
+        sy = np.ctypeslib.as_array(self.L.oc_ball_surf(self.ctx), shape=(max(n, 1),))[:n].copy()
+        return h.reshape(n, 24), sy
 
 
 def mt_build(dens, n, dx, iso, cap=None, bits=32):

@@ -16,6 +16,9 @@
 
 enum { AIR = 0, FLUID = 1, SOLID = 2 };
 
+/* Math.PI */
+static const double JS_PI = 3.141592653589793;
+
 struct oc_ctx {
   int nx, ny, nz;
   double dx, W, H, D, cx, cy, cz, domainR;
@@ -60,6 +63,13 @@ struct oc_ctx {
   /* outputs */
   int airborne, nFluid;
   double umax;
+  /* balls (solver.js:115, 3910-3912): the reference keeps them as objects
+   * of doubles; sv* are the moving-solid cell velocities (solver.js:95-97) */
+  int nBalls, hadSolidVelocity;
+  oc_ball balls[OC_MAX_BALLS];
+  oc_real *svx, *svy, *svz;
+  float surfY[OC_MAX_BALLS], hist[OC_MAX_BALLS * 24];
+  int histC[OC_MAX_BALLS];
   /* private scratch */
   int *fluidList, *pStart, *pAdj;
   float *pW;
@@ -128,6 +138,7 @@ int oc_set_ptr(oc_ctx *c, const char *name, void *ptr) {
   PTR(pflag, uint8_t *) PTR(pcool, uint8_t *)
   PTR(advU, double *) PTR(advV, double *) PTR(advW, double *)
   PTR(pLight, oc_real *) PTR(pPh, oc_real *) PTR(pDepth, uint8_t *)
+  PTR(svx, oc_real *) PTR(svy, oc_real *) PTR(svz, oc_real *)
   return -1;
 }
 #undef PTR
@@ -145,6 +156,7 @@ int oc_set_f(oc_ctx *c, const char *name, double v) {
 int oc_set_i(oc_ctx *c, const char *name, long long v) {
   FLD(pressureIters) FLD(nP) FLD(tn) FLD(ceilReflect) FLD(hasSun)
   FLD(cfS) FLD(bfS) FLD(evS) FLD(sorRedBlack) FLD(residualEvery) FLD(maxPressureIters) FLD(spinOn)
+  FLD(hadSolidVelocity)
   return -1;
 }
 #undef FLD
@@ -156,6 +168,7 @@ double oc_get_f(oc_ctx *c, const char *name) {
 }
 long long oc_get_i(oc_ctx *c, const char *name) {
   GET(airborne) GET(nFluid) GET(nP) GET(cfS) GET(bfS) GET(evS) GET(itersDone) GET(evapCount) GET(vaporCount) GET(nightVaporCount)
+  GET(nBalls) GET(hadSolidVelocity)
   return -1;
 }
 #undef GET
@@ -261,11 +274,14 @@ static int on_rock_below(oc_ctx *c, double x, double y, double z) {
 }
 
 /* ------------------------------------------------------------ rasterize */
-/* _rasterizeStatic, solver.js:865-916 (sphere mode, no balls): the static
- * raster is built once per world and copied into cellType every substep. */
+static void ball_raster(oc_ctx *c);
+
+/* _rasterizeStatic, solver.js:865-939 (sphere mode): the static raster is
+ * built once per world and copied into cellType every substep; balls are
+ * then rasterized as moving solids */
 void oc_rasterize_static(oc_ctx *c) {
   int nx = c->nx, ny = c->ny, nz = c->nz;
-  if (c->staticTypes) { memcpy(c->cellType, c->staticTypes, c->nC); return; }
+  if (c->staticTypes) { memcpy(c->cellType, c->staticTypes, c->nC); ball_raster(c); return; }
   double dx = c->dx;
   for (int k = 0; k < nz; k++)
     for (int j = 0; j < ny; j++)
@@ -284,6 +300,67 @@ void oc_rasterize_static(oc_ctx *c) {
         }
         c->cellType[((long)k * ny + j) * nx + i] = (shell || rock) ? SOLID : AIR;
       }
+  ball_raster(c);
+}
+
+/* moving solids, solver.js:918-938: the solid velocities are cleared while
+ * balls exist (and once after the last one is gone); each ball turns the
+ * cells whose centres lie within r + 0.4 dx SOLID, a later ball overwriting
+ * an earlier one's velocity */
+static void ball_raster(oc_ctx *c) {
+  int nx = c->nx, ny = c->ny, nz = c->nz;
+  if (c->nBalls || c->hadSolidVelocity) {
+    if (c->svx)
+      for (long i = 0; i < c->nC; i++) c->svx[i] = c->svy[i] = c->svz[i] = 0;
+  }
+  c->hadSolidVelocity = c->nBalls > 0;
+  double dx = c->dx;
+  for (int n = 0; n < c->nBalls; n++) {
+    const oc_ball *b = &c->balls[n];
+    double r = b->r, rr = (r + 0.4 * dx) * (r + 0.4 * dx);
+    int i0 = clampi((int)floor((b->x - r) / dx), 1, nx - 2), i1 = clampi((int)ceil((b->x + r) / dx), 1, nx - 2);
+    int j0 = clampi((int)floor((b->y - r) / dx), 1, ny - 2), j1 = clampi((int)ceil((b->y + r) / dx), 1, ny - 2);
+    int k0 = clampi((int)floor((b->z - r) / dx), 1, nz - 2), k1 = clampi((int)ceil((b->z + r) / dx), 1, nz - 2);
+    for (int k = k0; k <= k1; k++)
+      for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++) {
+          double ex = (i + 0.5) * dx - b->x, ey = (j + 0.5) * dx - b->y, ez = (k + 0.5) * dx - b->z;
+          if (ex * ex + ey * ey + ez * ez <= rr) {
+            long ci = ((long)k * ny + j) * nx + i;
+            c->cellType[ci] = SOLID;
+            c->svx[ci] = (oc_real)b->vx; c->svy[ci] = (oc_real)b->vy; c->svz[ci] = (oc_real)b->vz;
+          }
+        }
+  }
+}
+
+/* _pushOutOfBalls / _pushBallsChunk, solver.js:1005-1026: every particle
+ * (any flag) inside r + 0.002 of a ball moves to that distance along the
+ * ball normal and loses its inward relative normal velocity; balls in order */
+void oc_push_balls(oc_ctx *c) {
+  for (int n = 0; n < c->nBalls; n++) {
+    const oc_ball *b = &c->balls[n];
+    double rr = b->r + 0.002;
+    for (int p = 0; p < c->nP; p++) {
+      double ex = c->px[p] - b->x, ey = c->py[p] - b->y, ez = c->pz[p] - b->z;
+      double d2 = ex * ex + ey * ey + ez * ez;
+      if (d2 < rr * rr) {
+        double d = sqrt(d2);
+        if (d == 0) d = 1e-9;
+        double inv = 1 / d;
+        double nxn = ex * inv, nyn = ey * inv, nzn = ez * inv;
+        c->px[p] = (oc_real)(b->x + nxn * rr); c->py[p] = (oc_real)(b->y + nyn * rr);
+        c->pz[p] = (oc_real)(b->z + nzn * rr);
+        double rvx = c->pvx[p] - b->vx, rvy = c->pvy[p] - b->vy, rvz = c->pvz[p] - b->vz;
+        double vn = rvx * nxn + rvy * nyn + rvz * nzn;
+        if (vn < 0) {
+          c->pvx[p] = (oc_real)(c->pvx[p] - nxn * vn);
+          c->pvy[p] = (oc_real)(c->pvy[p] - nyn * vn);
+          c->pvz[p] = (oc_real)(c->pvz[p] - nzn * vn);
+        }
+      }
+    }
+  }
 }
 
 /* _markFluidCells, solver.js:945-1000 (sphere mode) */
@@ -353,9 +430,12 @@ void oc_p2g(oc_ctx *c) {
 }
 
 /* ------------------------------------------------- boundary conditions */
-/* _applyBCSlice, solver.js:1130-1171, no moving solids (sv = 0) */
+/* _applyBCSlice, solver.js:1130-1171: domain planes 0, a face next to a
+ * SOLID cell takes that cell's velocity (the left/lower one first; 0 for
+ * rock, the ball's velocity for a ball cell) */
 void oc_apply_bc(oc_ctx *c) {
   int nx = c->nx, ny = c->ny, nz = c->nz;
+#define SV(a, i) (c->a ? c->a[i] : (oc_real)0)
   const uint8_t *t = c->cellType;
   for (int k = 0; k < nz; k++)
     for (int j = 0; j < ny; j++) {
@@ -376,20 +456,24 @@ void oc_apply_bc(oc_ctx *c) {
     for (int j = 0; j < ny; j++)
       for (int i = 1; i < nx; i++) {
         long cl = ((long)k * ny + j) * nx + i - 1;
-        if (t[cl] == SOLID || t[cl + 1] == SOLID) c->u[((long)k * ny + j) * (nx + 1) + i] = 0;
+        if (t[cl] == SOLID) c->u[((long)k * ny + j) * (nx + 1) + i] = SV(svx, cl);
+        else if (t[cl + 1] == SOLID) c->u[((long)k * ny + j) * (nx + 1) + i] = SV(svx, cl + 1);
       }
   for (int k = 0; k < nz; k++)
     for (int j = 1; j < ny; j++)
       for (int i = 0; i < nx; i++) {
         long cr = ((long)k * ny + j) * nx + i;
-        if (t[cr - nx] == SOLID || t[cr] == SOLID) c->v[((long)k * (ny + 1) + j) * nx + i] = 0;
+        if (t[cr - nx] == SOLID) c->v[((long)k * (ny + 1) + j) * nx + i] = SV(svy, cr - nx);
+        else if (t[cr] == SOLID) c->v[((long)k * (ny + 1) + j) * nx + i] = SV(svy, cr);
       }
   for (int k = 1; k < nz; k++)
     for (int j = 0; j < ny; j++)
       for (int i = 0; i < nx; i++) {
         long cr = ((long)k * ny + j) * nx + i;
-        if (t[cr - (long)nx * ny] == SOLID || t[cr] == SOLID) c->w[cr] = 0;
+        if (t[cr - (long)nx * ny] == SOLID) c->w[cr] = SV(svz, cr - (long)nx * ny);
+        else if (t[cr] == SOLID) c->w[cr] = SV(svz, cr);
       }
+#undef SV
 }
 
 /* FLIP snapshot, solver.js:2579 */
@@ -1145,14 +1229,242 @@ void oc_advect(oc_ctx *c, double dt) {
     c->pvx[p] = (oc_real)(c->pvx[p] * dg); c->pvy[p] = (oc_real)(c->pvy[p] * dg); c->pvz[p] = (oc_real)(c->pvz[p] * dg);
     c->px[p] = (oc_real)x; c->py[p] = (oc_real)y; c->pz[p] = (oc_real)z;
   }
+  /* ball collisions after advection, solver.js:1762-1763 */
+  oc_push_balls(c);
 }
 
 /* --------------------------------------------------------------- substep */
 /* _substep, solver.js:2568-2624 (sphere mode, no balls) */
+/* ------------------------------------------------------------- balls */
+int oc_set_balls(oc_ctx *c, int n, const double *s9) {
+  if (n < 0 || n > OC_MAX_BALLS) return -1;
+  c->nBalls = n;
+  for (int i = 0; i < n; i++) {
+    oc_ball *b = &c->balls[i];
+    const double *q = s9 + 9 * i;
+    b->x = q[0]; b->y = q[1]; b->z = q[2]; b->vx = q[3]; b->vy = q[4]; b->vz = q[5];
+    b->r = q[6]; b->rho = q[7]; b->submerged = q[8];
+  }
+  return 0;
+}
+
+int oc_get_balls(oc_ctx *c, int n, double *s9) {
+  if (n < 0 || n > c->nBalls) return -1;
+  for (int i = 0; i < n; i++) {
+    const oc_ball *b = &c->balls[i];
+    double *q = s9 + 9 * i;
+    q[0] = b->x; q[1] = b->y; q[2] = b->z; q[3] = b->vx; q[4] = b->vy; q[5] = b->vz;
+    q[6] = b->r; q[7] = b->rho; q[8] = b->submerged;
+  }
+  return 0;
+}
+
+const float *oc_ball_hist(oc_ctx *c) { return c->hist; }
+const float *oc_ball_surf(oc_ctx *c) { return c->surfY; }
+
+/* _ballVsTerrain, solver.js:2299-2342: the deepest penetration of the ball
+ * into a solid voxel of the terrain lattice (first voxel in k, j, i order
+ * among equals) pushes the ball out along the voxel's closest-point normal
+ * (radially when the centre is inside the voxel) */
+static void ball_vs_terrain(oc_ctx *c, oc_ball *b) {
+  int n = c->tn;
+  double dv = c->dv;
+  const uint8_t *solid = c->tsolid;
+  int i0 = (int)((b->x - b->r) / dv), i1 = (int)((b->x + b->r) / dv);
+  int j0 = (int)((b->y - b->r) / dv), j1 = (int)((b->y + b->r) / dv);
+  int k0 = (int)((b->z - b->r) / dv), k1 = (int)((b->z + b->r) / dv);
+  if (i0 < 0) i0 = 0;
+  if (j0 < 0) j0 = 0;
+  if (k0 < 0) k0 = 0;
+  if (i1 > n - 1) i1 = n - 1;
+  if (j1 > n - 1) j1 = n - 1;
+  if (k1 > n - 1) k1 = n - 1;
+  double bestPen = 0, bpx = 0, bpy = 0, bpz = 0;
+  for (int k = k0; k <= k1; k++)
+    for (int j = j0; j <= j1; j++) {
+      long rowB = ((long)k * n + j) * n;
+      for (int i = i0; i <= i1; i++) {
+        if (!solid[rowB + i]) continue;
+        double qx = b->x, qy = b->y, qz = b->z;
+        double vx0 = i * dv, vy0 = j * dv, vz0 = k * dv;
+        if (qx < vx0) qx = vx0; else if (qx > vx0 + dv) qx = vx0 + dv;
+        if (qy < vy0) qy = vy0; else if (qy > vy0 + dv) qy = vy0 + dv;
+        if (qz < vz0) qz = vz0; else if (qz > vz0 + dv) qz = vz0 + dv;
+        double ddx = b->x - qx, ddy = b->y - qy, ddz = b->z - qz;
+        double dd = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+        double pen = b->r - dd;
+        if (pen > bestPen) {
+          if (dd > 1e-6) {
+            bestPen = pen; bpx = ddx / dd; bpy = ddy / dd; bpz = ddz / dd;
+          } else {
+            double rx = b->x - c->cx, ry = b->y - c->cy, rz = b->z - c->cz;
+            double rl = sqrt(rx * rx + ry * ry + rz * rz);
+            if (rl == 0) rl = 1e-9;
+            bestPen = pen; bpx = rx / rl; bpy = ry / rl; bpz = rz / rl;
+          }
+        }
+      }
+    }
+  if (bestPen > 0) {
+    b->x += bpx * (bestPen + 0.002); b->y += bpy * (bestPen + 0.002); b->z += bpz * (bestPen + 0.002);
+    double vn = b->vx * bpx + b->vy * bpy + b->vz * bpz;
+    if (vn < 0) {
+      b->vx -= bpx * vn; b->vy -= bpy * vn; b->vz -= bpz * vn;
+      b->vx *= 0.985; b->vy *= 0.985; b->vz *= 0.985;
+    }
+  }
+}
+
+/* _updateBalls, solver.js:2344-2564 (sphere mode): the local sea level of
+ * each ball from a 24-bucket radial histogram of the liquid particles in a
+ * shell 2.2r..3.2r around it (90th percentile, fallbacks to oceanR); then
+ * per ball Archimedes buoyancy of the submerged cap, weight under the
+ * inverse-square law, quadratic drag against the grid velocity at the centre
+ * (clamped to 0.8 m g), added mass, damping, a 4 m/s clamp, explicit
+ * integration, terrain (or core) contact and the domain sphere; finally the
+ * ball-ball contacts in pair order */
+void oc_update_balls(oc_ctx *c, double dt) {
+  const double rhoF = 1000, Cd = 0.5;
+  const int HIST = 24;
+  int nb = c->nBalls;
+  if (nb > 0) {
+    double bcx = c->cx, bcy = c->cy, bcz = c->cz;
+    double hLo = c->coreR, hSpan = (c->oceanR - c->coreR) * 1.5;
+    for (int i = 0; i < nb * HIST; i++) c->hist[i] = 0;
+    for (int i = 0; i < nb; i++) c->histC[i] = 0;
+    for (int p = 0; p < c->nP; p++) {
+      if (c->pflag[p] != 0) continue;
+      double ppx = c->px[p], ppy = c->py[p], ppz = c->pz[p];
+      double ldx = ppx - bcx, ldy = ppy - bcy, ldz = ppz - bcz;
+      double level = sqrt(ldx * ldx + ldy * ldy + ldz * ldz);
+      for (int sb = 0; sb < nb; sb++) {
+        const oc_ball *bb = &c->balls[sb];
+        double bdx = ppx - bb->x, bdy = ppy - bb->y, bdz = ppz - bb->z;
+        double dh2 = bdx * bdx + bdy * bdy + bdz * bdz;
+        double r1 = 2.2 * bb->r, r2 = 3.2 * bb->r;
+        if (dh2 < r1 * r1 || dh2 > r2 * r2) continue;
+        int bucket = (int)floor(((level - hLo) / hSpan) * HIST);
+        if (bucket < 0) bucket = 0; else if (bucket >= HIST) bucket = HIST - 1;
+        c->hist[sb * HIST + bucket] += 1.0f;
+        c->histC[sb]++;
+      }
+    }
+    for (int sb = 0; sb < nb; sb++) {
+      int total = c->histC[sb];
+      double wl = -INFINITY;
+      if (total > 20) {
+        double acc = 0, target = total * 0.10;
+        for (int bkt = HIST - 1; bkt >= 0; bkt--) {
+          acc += c->hist[sb * HIST + bkt];
+          if (acc >= target) { wl = hLo + (bkt + 0.5) / HIST * hSpan; break; }
+        }
+        if (wl == -INFINITY || total < 140) wl = c->oceanR;
+        double wlMax = c->oceanR + (c->oceanR - c->coreR) * 0.3;
+        if (wl > wlMax) wl = wlMax;
+        if (wl < c->coreR) wl = c->coreR;
+      } else {
+        wl = c->oceanR;
+      }
+      c->surfY[sb] = (float)wl;
+    }
+  }
+  for (int n = 0; n < nb; n++) {
+    oc_ball *b = &c->balls[n];
+    double m = b->rho * 4 / 3 * JS_PI * b->r * b->r * b->r;
+    double area = JS_PI * b->r * b->r;
+    double surf = c->surfY[n];
+    double bR = sqrt((b->x - c->cx) * (b->x - c->cx) + (b->y - c->cy) * (b->y - c->cy) + (b->z - c->cz) * (b->z - c->cz));
+    double inner = bR - b->r;
+    double h = surf - inner;
+    if (!(h > 0)) h = 0;
+    if (h > 2 * b->r) h = 2 * b->r;
+    double Vs = JS_PI * h * h * (3 * b->r - h) / 3;
+    b->submerged = Vs / (4.0 / 3 * JS_PI * b->r * b->r * b->r);
+    double upx = 0, upy = 1, upz = 0, gEff = c->gravity;
+    if (bR > 1e-6) {
+      upx = (b->x - c->cx) / bR; upy = (b->y - c->cy) / bR; upz = (b->z - c->cz) / bR;
+      gEff = c->gravity * (c->oceanR * c->oceanR) / (bR * bR);
+    }
+    double Fb = (rhoF * Vs - m) * gEff;
+    double tmp[3];
+    oc_sample_vel3(c, b->x, b->y, b->z, tmp);
+    double rvx = b->vx - tmp[0], rvy = b->vy - tmp[1], rvz = b->vz - tmp[2];
+    double vr = sqrt(rvx * rvx + rvy * rvy + rvz * rvz);
+    double kd = 0.5 * rhoF * Cd * area * vr;
+    double Fx = -kd * rvx + upx * Fb, Fz = -kd * rvz + upz * Fb, Fyd = -kd * rvy + upy * Fb;
+    {
+      double fdx = Fx - upx * Fb, fdy = Fyd - upy * Fb, fdz = Fz - upz * Fb;
+      double fdm = sqrt(fdx * fdx + fdy * fdy + fdz * fdz);
+      double fdmMax = 0.8 * m * c->gravity;
+      if (fdm > fdmMax) {
+        double fdc = fdmMax / fdm;
+        Fx = upx * Fb + fdx * fdc; Fyd = upy * Fb + fdy * fdc; Fz = upz * Fb + fdz * fdc;
+      }
+    }
+    double mEff = m + 0.5 * rhoF * Vs;
+    b->vx += (Fx / mEff) * dt;
+    b->vz += (Fz / mEff) * dt;
+    b->vy += (Fyd / mEff) * dt;
+    b->vx *= 0.999; b->vy *= 0.999; b->vz *= 0.999;
+    const double bvmax = 4;
+    if (b->vx > bvmax) b->vx = bvmax; else if (b->vx < -bvmax) b->vx = -bvmax;
+    if (b->vz > bvmax) b->vz = bvmax; else if (b->vz < -bvmax) b->vz = -bvmax;
+    if (b->vy > bvmax) b->vy = bvmax; else if (b->vy < -bvmax) b->vy = -bvmax;
+    b->x += b->vx * dt; b->y += b->vy * dt; b->z += b->vz * dt;
+    if (c->tsolid) {
+      ball_vs_terrain(c, b);
+    } else {
+      double cbx = b->x - c->cx, cby = b->y - c->cy, cbz = b->z - c->cz;
+      double cbr = sqrt(cbx * cbx + cby * cby + cbz * cbz);
+      if (cbr == 0) cbr = 1e-9;
+      if (cbr < c->coreR + b->r) {
+        double rnx0 = cbx / cbr, rny0 = cby / cbr, rnz0 = cbz / cbr;
+        double rr0 = c->coreR + b->r;
+        b->x = c->cx + rnx0 * rr0; b->y = c->cy + rny0 * rr0; b->z = c->cz + rnz0 * rr0;
+        double vn0 = b->vx * rnx0 + b->vy * rny0 + b->vz * rnz0;
+        if (vn0 < 0) {
+          b->vx -= rnx0 * vn0 * 1.3; b->vy -= rny0 * vn0 * 1.3; b->vz -= rnz0 * vn0 * 1.3;
+          b->vx *= 0.98; b->vy *= 0.98; b->vz *= 0.98;
+        }
+      }
+    }
+    double sbx = b->x - c->cx, sby = b->y - c->cy, sbz = b->z - c->cz;
+    double sbr = sqrt(sbx * sbx + sby * sby + sbz * sbz);
+    if (sbr == 0) sbr = 1e-9;
+    double rnx = sbx / sbr, rny = sby / sbr, rnz = sbz / sbr;
+    if (sbr > c->domainR - b->r) {
+      double rr1 = c->domainR - b->r;
+      b->x = c->cx + rnx * rr1; b->y = c->cy + rny * rr1; b->z = c->cz + rnz * rr1;
+      double vn1 = b->vx * rnx + b->vy * rny + b->vz * rnz;
+      if (vn1 > 0) { b->vx -= rnx * vn1 * 1.3; b->vy -= rny * vn1 * 1.3; b->vz -= rnz * vn1 * 1.3; }
+    }
+  }
+  for (int a = 0; a < nb; a++)
+    for (int b2 = a + 1; b2 < nb; b2++) {
+      oc_ball *A = &c->balls[a], *B = &c->balls[b2];
+      double ex = B->x - A->x, ey = B->y - A->y, ez = B->z - A->z;
+      double d2 = ex * ex + ey * ey + ez * ez, rSum = A->r + B->r;
+      if (d2 > rSum * rSum || d2 < 1e-12) continue;
+      double d = sqrt(d2), nxn = ex / d, nyn = ey / d, nzn = ez / d;
+      double overlap = rSum - d;
+      double mA = A->rho * A->r * A->r * A->r, mB = B->rho * B->r * B->r * B->r;
+      double tA = mB / (mA + mB), tB = mA / (mA + mB);
+      A->x -= nxn * overlap * tA; A->y -= nyn * overlap * tA; A->z -= nzn * overlap * tA;
+      B->x += nxn * overlap * tB; B->y += nyn * overlap * tB; B->z += nzn * overlap * tB;
+      double rvn = (B->vx - A->vx) * nxn + (B->vy - A->vy) * nyn + (B->vz - A->vz) * nzn;
+      if (rvn < 0) {
+        double jimp = -(1 + 0.3) * rvn / (1 / mA + 1 / mB);
+        A->vx -= nxn * jimp / mA; A->vy -= nyn * jimp / mA; A->vz -= nzn * jimp / mA;
+        B->vx += nxn * jimp / mB; B->vy += nyn * jimp / mB; B->vz += nzn * jimp / mB;
+      }
+    }
+}
+
 void oc_substep(oc_ctx *c, double dt) {
   oc_clear_faces(c);
   oc_rasterize_static(c);
   oc_mark_fluid(c, 0, c->nP);
+  oc_push_balls(c);
   oc_p2g(c);
   oc_apply_bc(c);
   oc_snapshot(c);
@@ -1166,6 +1478,7 @@ void oc_substep(oc_ctx *c, double dt) {
   oc_extrapolate(c);
   oc_g2p(c, dt);
   oc_advect(c, dt);
+  oc_update_balls(c, dt);
   c->dtLast = dt;
 }
 

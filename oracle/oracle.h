@@ -30,6 +30,10 @@ extern "C" {
 
 typedef struct oc_ctx oc_ctx;
 
+/* a ball (solver.js:3910-3912): JavaScript numbers, so doubles */
+#define OC_MAX_BALLS 64
+typedef struct { double x, y, z, vx, vy, vz, r, rho, submerged; } oc_ball;
+
 oc_ctx *oc_create(int nx, int ny, int nz, double dx);
 void oc_destroy(oc_ctx *c);
 /* name-based binding keeps the Python mirror free of struct layouts */
@@ -73,6 +77,15 @@ void oc_update_phase(oc_ctx *c, double dt);
 void oc_push_surface(oc_ctx *c);
 void oc_vapor_collisions(oc_ctx *c);
 void oc_ice_contacts(oc_ctx *c);
+
+/* balls (solver.js:918-938, 1005-1026, 2294-2564); state as 9 doubles per
+ * ball: x y z vx vy vz r rho submerged */
+int oc_set_balls(oc_ctx *c, int n, const double *s9);
+int oc_get_balls(oc_ctx *c, int n, double *s9);
+void oc_push_balls(oc_ctx *c);
+void oc_update_balls(oc_ctx *c, double dt);
+const float *oc_ball_hist(oc_ctx *c);
+const float *oc_ball_surf(oc_ctx *c);
 
 /* samplers and terrain predicates */
 double oc_sample_face(oc_ctx *c, const oc_real *arr, int comp, double x, double y, double z);

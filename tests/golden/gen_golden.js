@@ -19,6 +19,7 @@ var fs = require('fs');
 var zlib = require('zlib');
 
 var REF = process.argv[2] || '/root/reference';
+var ONLY = process.argv[3] || '';   // optional: write only this scene
 var OUT = path.dirname(__filename);
 
 global.MarchingTetrahedra = require(path.join(REF, 'js/surface.js'));
@@ -98,13 +99,17 @@ function stagedSubstep(w, s, dt) {
   w.meta.stage_params = params(s);
   dumpParticles(w, s, 'in_');
   w.add('in_q', copy(s.q));
+  var nb = s.balls.length;
+  if (nb) w.add('in_balls', ballState(s));
   s.u.fill(0); s.v.fill(0); s.w.fill(0);
   s.uW.fill(0); s.vW.fill(0); s.wW.fill(0);
   s._rasterizeStatic();
+  if (nb) dumpGrid(w, s, 'rast_', ['cellType', 'svx', 'svy', 'svz']);
   s._markFluidCells(0, s.nP);
   w.add('mark_cellType', copy(s.cellType));
   w.add('mark_pflag', copy(s.pflag, s.nP));
   s._pushOutOfBalls();
+  if (nb) dumpParticles(w, s, 'pushb_');
   s._p2g();
   dumpGrid(w, s, 'p2g_', ['u', 'v', 'w', 'uW', 'vW', 'wW']);
   s._applyBC();
@@ -137,10 +142,23 @@ function stagedSubstep(w, s, dt) {
   s._updateBalls(dt);
   s.dtLast = dt;
   dumpParticles(w, s, 'adv_');
+  if (nb) {
+    w.add('out_balls', ballState(s));
+    w.add('ball_hist', copy(s._hist)); w.add('ball_surf', copy(s._surfY));
+  }
   w.meta.rng_after = { evS: s._evS, cfS: s._cfS, bfS: s._bfS };
 }
 
 function ke(s) { return s.kineticEnergy(); }
+
+// ball objects as 9 doubles each: x y z vx vy vz r rho submerged
+function ballState(s) {
+  var a = new Float64Array(9 * s.balls.length);
+  s.balls.forEach(function (b, i) {
+    a.set([b.x, b.y, b.z, b.vx, b.vy, b.vz, b.r, b.rho, b.submerged], 9 * i);
+  });
+  return a;
+}
 
 function trajectory(s, nSub, rec) {
   for (var i = 0; i < nSub; i++) {
@@ -161,6 +179,7 @@ function seededVel(s, seed, amp) {
 // ---------------------------------------------------------------- scene A
 // Plain method, small: 32^3, 10k particles, FLIP 0.95, omega 1.7, 40 sweeps.
 (function sceneA() {
+  if (ONLY && ONLY !== 'cfg1_32') return;
   var n = 32, W = 5.1, dx = W / n;
   var s = new FluidSolver({ nx: n, ny: n, nz: n, dx: dx, targetParticles: 10000, mode: 'sphere',
     coreR: 0.25 * W, terrainSeed: 20260114 });
@@ -195,6 +214,7 @@ function seededVel(s, seed, amp) {
 // Micro scene: 16^3, 2k particles, random velocities, full gravity,
 // vorticity confinement on (exercises viscosity, confinement, spray paths).
 (function sceneB() {
+  if (ONLY && ONLY !== 'micro_16') return;
   var n = 16, W = 5.1, dx = W / n;
   var s = new FluidSolver({ nx: n, ny: n, nz: n, dx: dx, targetParticles: 2000, mode: 'sphere',
     coreR: 0.25 * W, terrainSeed: 20260115, vorticity: 0.05 });
@@ -267,6 +287,7 @@ function frameEnd(w, s) {
 // and the phase pass; temperatures from a 270-300 K latitude profile on the
 // reference's normalised scale (T = 0.105 + (T_K - 273.15) / 60).
 (function sceneC() {
+  if (ONLY && ONLY !== 'thermal_32') return;
   var n = 32, W = 5.1, dx = W / n;
   var s = new FluidSolver({ nx: n, ny: n, nz: n, dx: dx, targetParticles: 10000, mode: 'sphere',
     coreR: 0.25 * W, terrainSeed: 20260116, sunActivity: 0.6, evapIntensity: 1.5, vorticity: 0.05,
@@ -312,4 +333,63 @@ function frameEnd(w, s) {
   stagedSubstep(w, s, dtLaw(s));
   frameEnd(w, s);
   w.save('thermal_32.wsg.gz');
+})();
+
+// ---------------------------------------------------------------- scene D
+// Balls on the 32^3 planet (solver.js:918-938, 1005-1026, 2294-2564): one
+// floating at sea level, one dropped from above the sea, an overlapping pair
+// in the water (ball-ball contact) and one sunk into the rock of a continent
+// (terrain contact).
+(function sceneD() {
+  if (ONLY && ONLY !== 'balls_32') return;
+  var n = 32, W = 5.1, dx = W / n;
+  var s = new FluidSolver({ nx: n, ny: n, nz: n, dx: dx, targetParticles: 10000, mode: 'sphere',
+    coreR: 0.25 * W, terrainSeed: 20260117 });
+  s.resetWater(0.4 * W - 0.25 * W);
+  s.pic = 0.05; s.omega = 1.7; s.pressureIters = 40; s.gravity = 9.8; s.maxSpeed = 3.2; s.vorticity = 0;
+  // candidate directions on a Fibonacci sphere, split into deep sea and land
+  var sea = [], land = [], ga = Math.PI * (3 - Math.sqrt(5));
+  for (var i = 0; i < 300; i++) {
+    var y = 1 - 2 * (i + 0.5) / 300, rr = Math.sqrt(1 - y * y), th = ga * i;
+    var d = [Math.cos(th) * rr, y * 0.9, Math.sin(th) * rr];
+    var dl = Math.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    d = [d[0] / dl, d[1] / dl, d[2] / dl];
+    var Rt = s.terrainRadiusAt(s.cx + d[0] * s.oceanR, s.cy + d[1] * s.oceanR, s.cz + d[2] * s.oceanR);
+    if (Rt < s.oceanR - 0.35) sea.push(d); else if (Rt > s.oceanR + 0.1) land.push([d, Rt]);
+  }
+  function at(d, r) { return [s.cx + d[0] * r, s.cy + d[1] * r, s.cz + d[2] * r]; }
+  var p0 = at(sea[3], s.oceanR);
+  s.addBall(p0[0], p0[1], p0[2], 0.22, 500);
+  var p1 = at(sea[Math.floor(sea.length / 2)], s.oceanR + 0.45);
+  s.addBall(p1[0], p1[1], p1[2], 0.2, 900);
+  var d1 = sea[Math.floor(sea.length / 2)];
+  s.balls[1].vx = -d1[0]; s.balls[1].vy = -d1[1]; s.balls[1].vz = -d1[2];
+  var d2 = sea[sea.length - 4], t2 = [d2[1], -d2[0], 0], tl = Math.sqrt(t2[0] * t2[0] + t2[1] * t2[1]) || 1;
+  var c2 = at(d2, s.oceanR - 0.12);
+  s.addBall(c2[0] + 0.15 * t2[0] / tl, c2[1] + 0.15 * t2[1] / tl, c2[2], 0.18, 1200);
+  s.addBall(c2[0] - 0.15 * t2[0] / tl, c2[1] - 0.15 * t2[1] / tl, c2[2], 0.18, 700);
+  var L = land[Math.floor(land.length / 3)], p4 = at(L[0], L[1] + 0.05);
+  s.addBall(p4[0], p4[1], p4[2], 0.2, 800);
+  var w = new Writer();
+  w.meta.scene = 'balls_32';
+  w.meta.fill = params(s);
+  s._rasterizeStatic();
+  dumpWorld(w, s);
+  dumpParticles(w, s, 'fill_');
+  w.add('fill_balls', ballState(s));
+  w.add('fill_q', copy(s.q));
+  var rec = [], tb = [];
+  for (var k = 0; k < 20; k++) {
+    var dt = dtLaw(s);
+    s._substep(dt);
+    rec.push({ dt: dt, umax: s.umax, ke: ke(s), airborne: s.airborneCount });
+    tb.push(ballState(s));
+  }
+  var all = new Float64Array(20 * 9 * s.balls.length);
+  tb.forEach(function (a, k) { all.set(a, k * a.length); });
+  w.add('traj_balls', all);
+  w.meta.trajectory = rec;
+  dumpParticles(w, s, 'traj_end_');
+  stagedSubstep(w, s, dtLaw(s));
+  w.save('balls_32.wsg.gz');
 })();

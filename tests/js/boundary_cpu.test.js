@@ -19,7 +19,8 @@ function throws(fn, re) {
 
 var addon = require(path.join(root, '_build', 'watersim_napi.node'));
 var want = ['create', 'destroy', 'setF', 'getF', 'upload', 'download', 'substep', 'stage', 'stepFrame',
-  'runSubsteps', 'splatDensity', 'mtBuild', 'autotuneIso', 'kineticEnergy', 'sync', 'worldBuild'];
+  'runSubsteps', 'splatDensity', 'mtBuild', 'autotuneIso', 'kineticEnergy', 'sync', 'worldBuild', 'setBalls',
+  'getBalls'];
 check('addon exports the C ABI surface', want.every(function (k) { return typeof addon[k] === 'function'; }),
   want.filter(function (k) { return typeof addon[k] !== 'function'; }).join(','));
 
@@ -30,8 +31,11 @@ var s = new FluidSolver({ nx: 24, ny: 24, nz: 24, dx: 5.1 / 24, targetParticles:
   coreR: 1.275 });
 check('reference defaults', s.pic === 0.15 && s.omega === 1.55 && s.pressureIters === 20 && s.cfl === 3 &&
   s.gravity === 9.81 && s.maxSpeed === 3.2 && s.Tamb === 0.32 && Math.abs(s.meltT - 0.105) < 1e-12, 'pic ' + s.pic);
-m = throws(function () { s.addBall(1, 1, 1, 0.3, 350); }, /balls/);
-check('balls are refused with a clear error', m && !/wrong/.test(m), m);
+s.addBall(1, 1, 1, 0.3, 350);
+var b = s.balls[0];
+check('addBall keeps the reference ball object (solver.js:3910-3912)', s.balls.length === 1 && b.x === 1 &&
+  b.vx === 0 && b.r === 0.3 && b.rho === 350 && b.submerged === 0);
+s.balls.length = 0;
 m = throws(function () { s.step(1 / 60); }, /resetWater/);
 check('stepping before a world exists is an error', m && !/wrong/.test(m), m);
 if (!process.env.WATERSIM_EXPECT_GPU) {

@@ -3,7 +3,7 @@
  * driven through the Node FluidSolver facade on the GPU with the same
  * scene (44^3 grid, 38k particles, core 1.5 m, ocean depth 0.45 m) and the
  * same pass thresholds. Sections can be picked on the command line:
- *   node planet_gpu.test.js fill settle heat shade currents whirl long
+ *   node planet_gpu.test.js fill settle heat shade currents whirl long balls
  */
 'use strict';
 var path = require('path');
@@ -231,6 +231,41 @@ if (on('long')) {
   check('dynamics stay bounded for 20 s', keEnd < 7.0 && keMax < 9.0, 'KE ' + keEnd.toFixed(2) + ', max ' + keMax.toFixed(2));
   check('the rock stays dry and nothing escapes', stC.inRock === 0 && stC.out === 0, 'inRock ' + stC.inRock + ' out ' + stC.out);
   sl.dispose();
+}
+
+if (on('balls')) {
+  // addBall (solver.js:3910-3912) and the ball physics of _updateBalls
+  // (:2344-2564) through the facade: a light ball dropped over deep sea
+  // comes to float at the sea surface, a dense one sinks onto the rock,
+  // nothing leaves the domain and the water stays out of the rock
+  console.log('planet: balls');
+  var sb = planet();
+  frames(sb, 30);
+  var sea = null, best = -1;
+  for (var q = 0; q < 200; q++) {
+    var yy = 1 - 2 * (q + 0.5) / 200, rq = Math.sqrt(1 - yy * yy), th = 2.399963 * q;
+    var d = [Math.cos(th) * rq, yy, Math.sin(th) * rq];
+    var depth = sb.oceanR - sb.terrainRadiusAt(sb.cx + d[0] * sb.oceanR, sb.cy + d[1] * sb.oceanR, sb.cz + d[2] * sb.oceanR);
+    if (depth > best) { best = depth; sea = d; }
+  }
+  var up = sb.oceanR + 0.3;
+  sb.addBall(sb.cx + sea[0] * up, sb.cy + sea[1] * up, sb.cz + sea[2] * up, 0.12, 400);
+  var t = [sea[1], -sea[0], 0], tl = Math.sqrt(t[0] * t[0] + t[1] * t[1]) || 1;
+  sb.addBall(sb.cx + sea[0] * up + 0.5 * t[0] / tl, sb.cy + sea[1] * up + 0.5 * t[1] / tl, sb.cz + sea[2] * up, 0.12, 3000);
+  frames(sb, 240);
+  var b0 = sb.balls[0], b1 = sb.balls[1];
+  function rad(b) { return Math.sqrt((b.x - sb.cx) * (b.x - sb.cx) + (b.y - sb.cy) * (b.y - sb.cy) + (b.z - sb.cz) * (b.z - sb.cz)); }
+  var r0 = rad(b0), r1 = rad(b1), stB = stats(sb);
+  check('the light ball floats at the sea surface', b0.submerged > 0.05 && b0.submerged < 0.95 && Math.abs(r0 - sb.oceanR) < 0.25,
+    'submerged ' + b0.submerged.toFixed(2) + ', r - sea ' + (r0 - sb.oceanR).toFixed(3));
+  check('the dense ball sinks below the surface', r1 < sb.oceanR - 0.1 && b1.submerged > 0.95,
+    'r - sea ' + (r1 - sb.oceanR).toFixed(3) + ', submerged ' + b1.submerged.toFixed(2));
+  check('ball state stays finite and in the domain', [b0, b1].every(function (b) {
+    return isFinite(b.x + b.y + b.z + b.vx + b.vy + b.vz) && rad(b) < sb.domainR;
+  }));
+  check('water stays out of the rock, nothing escapes, particles finite', stB.inRock === 0 && stB.out === 0 && allFinite(sb),
+    'inRock ' + stB.inRock + ' out ' + stB.out);
+  sb.dispose();
 }
 
 console.log((failures ? failures + ' FAILED' : 'ALL PLANET CHECKS PASSED') + ' in ' + ((Date.now() - t0) / 1000).toFixed(1) + ' s');

@@ -40,6 +40,6 @@ def test_marching_tets_through_node():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("section", ["settle", "heat", "shade", "currents", "whirl", "long"])
+@pytest.mark.parametrize("section", ["settle", "heat", "shade", "currents", "whirl", "long", "balls"])
 def test_planet_invariants_through_node(section):
     run_node("planet_gpu.test.js", section, timeout=110, gpu=True)

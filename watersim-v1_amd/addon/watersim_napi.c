@@ -296,6 +296,26 @@ static napi_value js_sync(napi_env env, napi_callback_info info) {
   return rc ? fail(env, c, rc, "ws_sync") : NULL;
 }
 
+/* setBalls(h, Float64Array of 9 per ball) / getBalls(h, Float64Array):
+ * the balls array of FluidSolver (solver.js:115, 3910-3912) */
+static napi_value balls_transfer(napi_env env, napi_callback_info info, int down) {
+  napi_value a[2];
+  if (!get_args(env, info, 2, a)) return NULL;
+  ws_ctx *c = get_ctx(env, a[0]);
+  void *p = NULL;
+  size_t nb = 0;
+  if (!c || !get_bytes(env, a[1], &p, &nb)) return NULL;
+  if (nb % (9 * sizeof(double))) {
+    napi_throw_error(env, NULL, "watersim: ball state is 9 doubles per ball");
+    return NULL;
+  }
+  int n = (int)(nb / (9 * sizeof(double)));
+  int rc = down ? ws_get_balls(c, n, (double *)p) : ws_set_balls(c, n, (const double *)p);
+  return rc ? fail(env, c, rc, down ? "ws_get_balls" : "ws_set_balls") : NULL;
+}
+static napi_value js_set_balls(napi_env env, napi_callback_info info) { return balls_transfer(env, info, 0); }
+static napi_value js_get_balls(napi_env env, napi_callback_info info) { return balls_transfer(env, info, 1); }
+
 /* Copies n bytes into a new ArrayBuffer and wraps it in a typed array. */
 static napi_value typed_copy(napi_env env, napi_typedarray_type t, const void *src, size_t n_elem, size_t elem) {
   void *dst = NULL;
@@ -380,6 +400,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"autotuneIso", 0, js_autotune, 0, 0, 0, napi_default, 0},
       {"kineticEnergy", 0, js_kinetic, 0, 0, 0, napi_default, 0},
       {"sync", 0, js_sync, 0, 0, 0, napi_default, 0},
+      {"setBalls", 0, js_set_balls, 0, 0, 0, napi_default, 0},
+      {"getBalls", 0, js_get_balls, 0, 0, 0, napi_default, 0},
       {"worldBuild", 0, js_world_build, 0, 0, 0, napi_default, 0},
   };
   if (napi_define_properties(env, exports, sizeof d / sizeof d[0], d) != napi_ok) return NULL;

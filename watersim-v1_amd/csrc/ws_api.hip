@@ -104,8 +104,8 @@ int need_world(ws_ctx *c) {
   return WS_OK;
 }
 
-__global__ void k_set_dt(DevScalars *s, float dt) {
-  if (threadIdx.x == 0) s->dt = dt;
+__global__ void k_set_dt(DevScalars *s, double dt) {
+  if (threadIdx.x == 0) { s->dt = (float)dt; s->dtD = dt; }
 }
 
 __global__ void k_reset_counters(DevScalars *s) {
@@ -122,6 +122,8 @@ int st_rasterize(ws_ctx *c) {
   int nC = (int)c->nC;
   hipLaunchKernelGGL(k_init_cells, dim3(blocks_for(nC + 1, 256)), dim3(256), 0, c->stream, c->staticTypes, c->cellType,
                      c->cellCount, nC);
+  int r = balls_raster(c);
+  if (r) return r;
   if (c->nP > 0)
     hipLaunchKernelGGL(k_mark, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->geo, world_of(c), c->nP,
                        c->px, c->py, c->pz, c->pflag, c->cellType, c->cellCount, c->pcell, c->prank, c->ds);
@@ -147,7 +149,7 @@ long nfaces(ws_ctx *c) { return c->uN + c->vN + c->wN; }
 
 int st_bc(ws_ctx *c, float *snapshot, int fuse_gravity) {
   hipLaunchKernelGGL(k_bc_snapshot, dim3(blocks_for(nfaces(c), 256)), dim3(256), 0, c->stream, c->geo, c->cellType,
-                     c->vel, snapshot, c->wgt, c->valid, fuse_gravity, c->gu, c->ds, 1);
+                     c->vel, snapshot, c->wgt, c->valid, fuse_gravity, c->gu, c->ds, 1, c->nBalls ? c->sv : nullptr);
   WS_LAUNCHED();
   return WS_OK;
 }
@@ -185,7 +187,7 @@ int st_viscosity(ws_ctx *c, double dt_bound, const float *old) {
 
 int st_gravity(ws_ctx *c) {
   hipLaunchKernelGGL(k_gravity_bc, dim3(blocks_for(nfaces(c), 256)), dim3(256), 0, c->stream, c->geo, c->cellType,
-                     c->vel, c->gu, c->ds);
+                     c->vel, c->gu, c->ds, c->nBalls ? c->sv : nullptr);
   WS_LAUNCHED();
   return WS_OK;
 }
@@ -473,7 +475,12 @@ int st_particles(ws_ctx *c, int mode) {
                      make_par(c), c->nP, mode, c->vel, c->velO, c->cellType, c->px, c->py, c->pz, c->pvx, c->pvy,
                      c->pvz, c->pflag, c->pcool, c->pAir, c->advU, c->advV, c->advW, c->ds);
   WS_LAUNCHED();
-  if (mode & 2) return st_vapor(c);
+  if (mode & 2) {
+    int r = st_vapor(c);
+    if (r) return r;
+    // ball collisions after advection (solver.js:1762-1763)
+    return balls_push(c, false);
+  }
   return WS_OK;
 }
 
@@ -482,12 +489,13 @@ int substep(ws_ctx *c, double dt, int use_law) {
   int r;
   if ((r = need_world(c))) return r;
   double base = c->dtFrameBase > 0 ? c->dtFrameBase : 0.01;
-  hipLaunchKernelGGL(k_begin_substep, dim3(1), dim3(64), 0, c->stream, c->ds, (float)dt, use_law,
-                     (float)(c->cfl * c->dx), (float)base, (float)c->gravity);
+  hipLaunchKernelGGL(k_begin_substep, dim3(1), dim3(64), 0, c->stream, c->ds, dt, use_law, c->cfl * c->dx, base,
+                     (float)c->gravity);
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, c->stream, c->ds);
   WS_LAUNCHED();
   bool visc = c->viscosity > 1e-6;
   if ((r = st_rasterize(c))) return r;
+  if ((r = balls_push(c, true))) return r;
   if ((r = st_p2g(c))) return r;
   if ((r = st_bc(c, c->velO, visc ? 0 : 1))) return r;
   if (visc) {
@@ -503,6 +511,7 @@ int substep(ws_ctx *c, double dt, int use_law) {
   }
   if ((r = st_extrapolate(c))) return r;
   if ((r = st_particles(c, 3))) return r;
+  if ((r = balls_update(c))) return r;
   hipLaunchKernelGGL(k_end_substep, dim3(1), dim3(64), 0, c->stream, c->ds);
   WS_LAUNCHED();
   return WS_OK;
@@ -556,6 +565,10 @@ bool field_of(ws_ctx *c, const char *n, Field &f) {
   GF("gu", c->gu, float, c->uN) GF("gv", c->gu + c->uN, float, c->vN) GF("gw", c->gu + c->uN + c->vN, float, c->wN)
   GF("dens", c->dens, float, c->nD)
   GF("terrainR", c->terrainR, float, (long)c->tn * c->tn * c->tn)
+  if (c->sv) {
+    GF("svx", c->sv, float, c->nC) GF("svy", c->sv + c->nC, float, c->nC) GF("svz", c->sv + 2 * c->nC, float, c->nC)
+    GF("ballHist", c->ballHist, unsigned, (long)c->nBalls * 25) GF("ballSurf", c->ballSurf, float, c->nBalls)
+  }
 #undef PF
 #undef GF
   return false;
@@ -709,6 +722,7 @@ void ws_destroy(ws_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   net_free(c);
   contacts_release(c, true);
+  balls_release(c);
   void *ptrs[] = {c->staticTypes, c->gu, c->thr, c->terrainR, c->cellType, c->nbm, c->dg, c->q, c->q2, c->rhs, c->pw,
                   c->sorActive, c->tbActive, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg,
                   c->cellCount, c->cellStart, c->vort, c->vel, c->velO, c->wgt, c->scr, c->scr2, c->valid, c->dens,
@@ -783,7 +797,7 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   GF(maxPressureIters) GF(nP) GF(capacity) GF(residual) GF(domainR) GF(dx) GF(tn) GF(dv)
   GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal) GF(residualsTotal) GF(ticksTotal) GF(sorMode) GF(sorCellSweeps)
   GF(cmpMsPlain) GF(cmpMsLearned) GF(cmpSweepsPlain) GF(cmpSweepsLearned) GF(cmpSolves) GF(warmStart) GF(netMs) GF(netCalls)
-  GF(vaporCount) GF(nightVaporCount) GF(contactOverflow)
+  GF(vaporCount) GF(nightVaporCount) GF(contactOverflow) GF(nBalls)
   GF(heatK) GF(Tamb) GF(shadeCool) GF(sunActivity) GF(evapIntensity) GF(snowT) GF(meltT) GF(cloudP) GF(currents)
   GF(simTime) GF(heatAcc) GF(evapCount) GF(cloudT) GF(rainT) GF(rainLiftAge)
 #undef GF
@@ -921,7 +935,7 @@ int ws_stage(ws_ctx *c, int stage, double dt) {
   int r = need_world(c);
   if (r) return r;
   if (stage != WS_STAGE_GRAVITY)
-    hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, c->stream, c->ds, (float)dt);
+    hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, c->stream, c->ds, dt);
   switch (stage) {
     case WS_STAGE_CLEAR:
       WS_HIP(hipMemsetAsync(c->vel, 0, sizeof(float) * nfaces(c), c->stream));
@@ -938,7 +952,7 @@ int ws_stage(ws_ctx *c, int stage, double dt) {
       r = st_viscosity(c, dt, c->scr);
       break;
     case WS_STAGE_GRAVITY:
-      hipLaunchKernelGGL(k_begin_substep, dim3(1), dim3(64), 0, c->stream, c->ds, (float)dt, 0, 0.f, 0.f,
+      hipLaunchKernelGGL(k_begin_substep, dim3(1), dim3(64), 0, c->stream, c->ds, dt, 0, 0.0, 0.0,
                          (float)c->gravity);
       r = st_gravity(c);
       break;
@@ -966,6 +980,8 @@ int ws_stage(ws_ctx *c, int stage, double dt) {
     case WS_STAGE_PUSH_SURFACE: r = frame_push_surface(c); break;
     case WS_STAGE_VAPOR_COLLISIONS: r = frame_vapor_collisions(c); break;
     case WS_STAGE_ICE_CONTACTS: r = frame_ice_contacts(c); break;
+    case WS_STAGE_PUSH_BALLS: r = balls_push(c, true); break;
+    case WS_STAGE_UPDATE_BALLS: r = balls_update(c); break;
     default: c->err = "unknown stage"; return WS_EINVAL;
   }
   if (r) return r;

@@ -101,26 +101,30 @@ __device__ __forceinline__ unsigned uf_find(unsigned *parent, unsigned x) {
   return x;
 }
 
-// union of subset members closer than thr (the smaller index becomes the root)
+// union of subset members closer than thr (the smaller index becomes the
+// root); with a mask only masked members take part, the others stay alone
 __global__ void k_ct_union(Geo g, int n, const unsigned *__restrict__ list, const float *__restrict__ px,
                            const float *__restrict__ py, const float *__restrict__ pz,
                            const unsigned *__restrict__ cellOf, const unsigned *__restrict__ start,
-                           const unsigned *__restrict__ binned, float thr2, unsigned *__restrict__ parent) {
+                           const unsigned *__restrict__ binned, float thr2, int span,
+                           const unsigned *__restrict__ mask, unsigned *__restrict__ parent) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
+  if (mask && !mask[e]) return;
   unsigned p = list[e];
   float x = px[p], y = py[p], z = pz[p];
   int c = cellOf[e];
   int ci = c % g.nx, cj = (c / g.nx) % g.ny, ck = c / (g.nx * g.ny);
-  for (int dk = -1; dk <= 1; dk++)
-    for (int dj = -1; dj <= 1; dj++)
-      for (int di = -1; di <= 1; di++) {
+  for (int dk = -span; dk <= span; dk++)
+    for (int dj = -span; dj <= span; dj++)
+      for (int di = -span; di <= span; di++) {
         int ii = ci + di, jj = cj + dj, kk = ck + dk;
         if (ii < 0 || jj < 0 || kk < 0 || ii >= g.nx || jj >= g.ny || kk >= g.nz) continue;
         int cc = (kk * g.ny + jj) * g.nx + ii;
         for (unsigned b = start[cc]; b < start[cc + 1]; b++) {
           unsigned o = binned[b];
           if (o <= (unsigned)e) continue;
+          if (mask && !mask[o]) continue;
           unsigned q = list[o];
           float ddx = px[q] - x, ddy = py[q] - y, ddz = pz[q] - z;
           if (ddx * ddx + ddy * ddy + ddz * ddz >= thr2) continue;
@@ -135,11 +139,13 @@ __global__ void k_ct_union(Geo g, int n, const unsigned *__restrict__ list, cons
       }
 }
 
-// component key: (root, position in the x order)
+// component key: (root, position in the x order); members outside the
+// mask get the all-ones key and sort after every group
 __global__ void k_ct_groupkeys(int n, unsigned *__restrict__ parent, const unsigned *__restrict__ rankOf,
-                               unsigned long long *__restrict__ key) {
+                               const unsigned *__restrict__ mask, unsigned long long *__restrict__ key) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
+  if (mask && !mask[e]) { key[e] = ~0ull; return; }
   unsigned r = uf_find(parent, (unsigned)e);
   key[e] = ((unsigned long long)r << 32) | rankOf[e];
 }
@@ -222,37 +228,113 @@ __global__ void k_ct_vapor_groups(int n, const unsigned long long *__restrict__ 
   }
 }
 
-// _iceContacts, one group per thread (snow does not move: exact)
-__global__ void k_ct_ice_groups(int n, const unsigned long long *__restrict__ gk, const unsigned long long *__restrict__ xk,
-                                float rad, const float *__restrict__ px, const float *__restrict__ py,
+// _iceContacts (solver.js:3290-3324). Snow does not move in this pass, so
+// every pair test is fixed in advance; only the stuck flags and velocities
+// evolve. Stuck grains are never written: a pair of two stuck grains does
+// nothing and a stuck grain only turns its free partner stuck. Free grains
+// are therefore grouped by their free-free contacts only, and each free
+// grain carries the sweep position of its first contact with a stuck grain
+// (the pair key (earlier x rank, later x rank)). One thread runs a group's
+// free-free pairs in the reference's order and applies a member's first
+// stuck contact just before the first pair that comes after it (it changes
+// nothing but that member, so applying it late is the same), or at the end.
+
+// which members of the ice subset are free (pAir != 1)
+__global__ void k_ct_ice_free(int n, const unsigned *__restrict__ list, const float *__restrict__ air,
+                              unsigned *__restrict__ freeMask) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) freeMask[e] = air[list[e]] != 1.f;
+}
+
+// the reference's pair test in double: the later grain (in x order) lies
+// less than rad ahead in x, and the squared distance is below rad^2
+__device__ __forceinline__ bool ice_pair(const float *px, const float *py, const float *pz, unsigned a, unsigned b,
+                                         double rad, double rad2) {
+  double ddx = (double)px[b] - (double)px[a];
+  if (ddx >= rad) return false;
+  double ddy = (double)py[b] - (double)py[a], ddz = (double)pz[b] - (double)pz[a];
+  return ddx * ddx + ddy * ddy + ddz * ddz < rad2;
+}
+
+// per free grain: the key of its first contact with a stuck grain (all ones
+// when it has none)
+__global__ void k_ct_ice_stuckkey(Geo g, int n, const unsigned *__restrict__ list, const float *__restrict__ px,
+                                  const float *__restrict__ py, const float *__restrict__ pz,
+                                  const unsigned *__restrict__ cellOf, const unsigned *__restrict__ start,
+                                  const unsigned *__restrict__ binned, int span, const unsigned *__restrict__ freeMask,
+                                  const unsigned *__restrict__ rankOf, double rad, double rad2,
+                                  unsigned long long *__restrict__ skey) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n || !freeMask[e]) return;
+  unsigned p = list[e], re = rankOf[e];
+  int c = cellOf[e];
+  int ci = c % g.nx, cj = (c / g.nx) % g.ny, ck = c / (g.nx * g.ny);
+  unsigned long long best = ~0ull;
+  for (int dk = -span; dk <= span; dk++)
+    for (int dj = -span; dj <= span; dj++)
+      for (int di = -span; di <= span; di++) {
+        int ii = ci + di, jj = cj + dj, kk = ck + dk;
+        if (ii < 0 || jj < 0 || kk < 0 || ii >= g.nx || jj >= g.ny || kk >= g.nz) continue;
+        int cc = (kk * g.ny + jj) * g.nx + ii;
+        for (unsigned b = start[cc]; b < start[cc + 1]; b++) {
+          unsigned o = binned[b];
+          if (freeMask[o]) continue;
+          unsigned q = list[o], ro = rankOf[o];
+          bool hit = re < ro ? ice_pair(px, py, pz, p, q, rad, rad2) : ice_pair(px, py, pz, q, p, rad, rad2);
+          if (!hit) continue;
+          unsigned long long k = re < ro ? ((unsigned long long)re << 32 | ro) : ((unsigned long long)ro << 32 | re);
+          if (k < best) best = k;
+        }
+      }
+  skey[e] = best;
+}
+
+__device__ __forceinline__ void ice_stick(float *vx, float *vy, float *vz, float *air, unsigned m) {
+  air[m] = 1.f;
+  vx[m] = vy[m] = vz[m] = 0.f;
+}
+
+// one group of free grains per thread; stuck members (key all ones) skip
+__global__ void k_ct_ice_groups(int n, const unsigned long long *__restrict__ gk,
+                                const unsigned long long *__restrict__ xk, const unsigned *__restrict__ posOf,
+                                const unsigned long long *__restrict__ skey, double rad, double rad2,
+                                const float *__restrict__ px, const float *__restrict__ py,
                                 const float *__restrict__ pz, float *__restrict__ vx, float *__restrict__ vy,
                                 float *__restrict__ vz, float *__restrict__ air) {
   int t0 = blockIdx.x * blockDim.x + threadIdx.x;
   if (t0 >= n) return;
-  if (t0 > 0 && (gk[t0] >> 32) == (gk[t0 - 1] >> 32)) return;
+  if (gk[t0] == ~0ull) return;
+  if (t0 > 0 && (gk[t0] >> 32) == (gk[t0 - 1] >> 32)) return;  // not the first member of its group
   int t1 = t0 + 1;
-  while (t1 < n && (gk[t1] >> 32) == (gk[t0] >> 32)) t1++;
-  if (t1 - t0 < 2) return;
-  const float rad2 = rad * rad;
+  while (t1 < n && gk[t1] != ~0ull && (gk[t1] >> 32) == (gk[t0] >> 32)) t1++;
   for (int a = t0; a < t1; a++) {
     unsigned i = member(gk, xk, a);
-    bool stuckI = air[i] == 1.f;
+    unsigned ri = (unsigned)(gk[a] & 0xffffffffull);
+    unsigned long long ski = skey[posOf[i]];
     for (int b = a + 1; b < t1; b++) {
       unsigned j = member(gk, xk, b);
-      if (px[j] - px[i] >= rad) break;
-      float ddx = px[j] - px[i], ddy = py[j] - py[i], ddz = pz[j] - pz[i];
-      if (ddx * ddx + ddy * ddy + ddz * ddz >= rad2) continue;
-      bool stuckJ = air[j] == 1.f;
+      if ((double)px[j] - (double)px[i] >= rad) break;
+      if (!ice_pair(px, py, pz, i, j, rad, rad2)) continue;
+      unsigned long long K = ((unsigned long long)ri << 32) | (unsigned)(gk[b] & 0xffffffffull);
+      // first stuck contacts that come before this pair
+      if (air[i] != 1.f && ski < K) ice_stick(vx, vy, vz, air, i);
+      if (air[j] != 1.f && skey[posOf[j]] < K) ice_stick(vx, vy, vz, air, j);
+      bool stuckI = air[i] == 1.f, stuckJ = air[j] == 1.f;
       if (stuckI && stuckJ) continue;
       if (stuckJ) {
-        air[i] = 1.f; vx[i] = vy[i] = vz[i] = 0.f; stuckI = true;
+        ice_stick(vx, vy, vz, air, i);
       } else if (stuckI) {
-        air[j] = 1.f; vx[j] = vy[j] = vz[j] = 0.f;
+        ice_stick(vx, vy, vz, air, j);
       } else {
         float mx = (vx[i] + vx[j]) * 0.5f, my = (vy[i] + vy[j]) * 0.5f, mz = (vz[i] + vz[j]) * 0.5f;
         vx[i] = vx[j] = mx; vy[i] = vy[j] = my; vz[i] = vz[j] = mz;
       }
     }
+  }
+  // first stuck contacts after the group's last pair
+  for (int a = t0; a < t1; a++) {
+    unsigned i = member(gk, xk, a);
+    if (air[i] != 1.f && skey[posOf[i]] != ~0ull) ice_stick(vx, vy, vz, air, i);
   }
 }
 
@@ -410,10 +492,16 @@ int bin_subset(ws_ctx *c, int n, const unsigned *list, unsigned *cellOf, unsigne
   return WS_OK;
 }
 
+// neighbour cells to visit for a search radius
+int cell_span(ws_ctx *c, double r) {
+  int s = (int)ceil(r / c->dx);
+  return s < 1 ? 1 : s;
+}
+
 // sorted x order of a subset, its connected groups (pairs closer than thr)
 // and the group keys sorted by (group, x rank); leaves the sorted x keys in
 // xk and the sorted group keys in gk
-int group_subset(ws_ctx *c, int n, const unsigned *list, float thr) {
+int group_subset(ws_ctx *c, int n, const unsigned *list, float thr, const unsigned *mask) {
   Contacts *k = c->contacts;
   int nb = blocks_for(n, 256);
   hipLaunchKernelGGL(k_ct_keys, dim3(nb), dim3(256), 0, c->stream, n, list, c->px, k->xk2);
@@ -426,8 +514,8 @@ int group_subset(ws_ctx *c, int n, const unsigned *list, float thr) {
   if ((r = bin_subset(c, n, list, k->cellOf, k->rankIn, k->start, k->binned))) return r;
   hipLaunchKernelGGL(k_ct_iota, dim3(nb), dim3(256), 0, c->stream, n, k->parent);
   hipLaunchKernelGGL(k_ct_union, dim3(nb), dim3(256), 0, c->stream, c->geo, n, list, c->px, c->py, c->pz, k->cellOf,
-                     k->start, k->binned, thr * thr, k->parent);
-  hipLaunchKernelGGL(k_ct_groupkeys, dim3(nb), dim3(256), 0, c->stream, n, k->parent, k->rankOf, k->gk2);
+                     k->start, k->binned, thr * thr, cell_span(c, thr), mask, k->parent);
+  hipLaunchKernelGGL(k_ct_groupkeys, dim3(nb), dim3(256), 0, c->stream, n, k->parent, k->rankOf, mask, k->gk2);
   WS_LAUNCHED();
   WS_HIP(hipcub::DeviceRadixSort::SortKeys(k->tmp, k->tmpBytes, k->gk2, k->gk, n, 0, 64, c->stream));
   return WS_OK;
@@ -490,7 +578,7 @@ int frame_vapor_collisions(ws_ctx *c) {
   WS_LAUNCHED();
   if (nV >= 2) {
     float rad = (float)(c->spacing * 0.6);
-    if ((r = group_subset(c, nV, k->vlist, 2.f * rad))) return r;
+    if ((r = group_subset(c, nV, k->vlist, 2.f * rad, nullptr))) return r;
     hipLaunchKernelGGL(k_ct_vapor_groups, dim3(blocks_for(nV, 256)), dim3(256), 0, c->stream, nV, k->gk, k->xk, rad,
                        c->spinOn, c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pWx, c->pWy, c->pWz);
     WS_LAUNCHED();
@@ -521,12 +609,17 @@ int frame_ice_contacts(ws_ctx *c) {
   int nI = 0;
   if ((r = subset(c, k->fi, k->oi, k->ilist, &nI))) return r;
   if (nI < 2) return WS_OK;
-  float rad = (float)(c->spacing * 0.7);
-  // groups from a radius a hair above the contact radius, so that no pair
-  // the reference's double-precision test accepts is split
-  if ((r = group_subset(c, nI, k->ilist, rad * (1.f + 1e-5f)))) return r;
-  hipLaunchKernelGGL(k_ct_ice_groups, dim3(blocks_for(nI, 256)), dim3(256), 0, c->stream, nI, k->gk, k->xk, rad, c->px,
-                     c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pAir);
+  double rad = c->spacing * 0.7, rad2 = rad * rad;
+  int nb = blocks_for(nI, 256);
+  // free grains, grouped by free-free contacts from a radius a hair above
+  // the contact radius, so that no pair the double test accepts is split
+  hipLaunchKernelGGL(k_ct_ice_free, dim3(nb), dim3(256), 0, c->stream, nI, k->ilist, c->pAir, k->fi);
+  WS_LAUNCHED();
+  if ((r = group_subset(c, nI, k->ilist, (float)rad * (1.f + 1e-5f), k->fi))) return r;
+  hipLaunchKernelGGL(k_ct_ice_stuckkey, dim3(nb), dim3(256), 0, c->stream, c->geo, nI, k->ilist, c->px, c->py, c->pz,
+                     k->cellOf, k->start, k->binned, cell_span(c, rad), k->fi, k->rankOf, rad, rad2, k->gk2);
+  hipLaunchKernelGGL(k_ct_ice_groups, dim3(nb), dim3(256), 0, c->stream, nI, k->gk, k->xk, k->posOf, k->gk2, rad,
+                     rad2, c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pAir);
   WS_LAUNCHED();
   return WS_OK;
 }

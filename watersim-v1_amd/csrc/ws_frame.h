@@ -14,5 +14,10 @@ int frame_push_surface(ws_ctx *c);
 int frame_vapor_collisions(ws_ctx *c);
 int frame_ice_contacts(ws_ctx *c);
 void contacts_release(ws_ctx *c, bool all);
+// balls (ws_balls.hip)
+int balls_raster(ws_ctx *c);
+int balls_push(ws_ctx *c, bool rebin);
+int balls_update(ws_ctx *c);
+void balls_release(ws_ctx *c);
 World make_world(ws_ctx *c);
 }  // namespace ws

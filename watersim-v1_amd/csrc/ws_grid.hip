@@ -9,21 +9,25 @@ namespace ws {
 // ---------------------------------------------------------------- scalars
 // dt and the gravity ramp of this substep (solver.js:2656-2662); mode 1 takes
 // dt from the CFL law of _stepSync (solver.js:2689-2690) on the device.
-__global__ void k_begin_substep(DevScalars *s, float dt_host, int use_law, float cfl_dx, float dt_base,
+__global__ void k_begin_substep(DevScalars *s, double dt_host, int use_law, double cfl_dx, double dt_base,
                                 float gravity) {
   if (threadIdx.x != 0) return;
-  float dt = dt_host;
+  // the length is a double, as in the reference; the float copy feeds the
+  // float kernels
+  double dt = dt_host;
   if (use_law) {
-    float um = __uint_as_float(s->umax_bits);
-    float law = cfl_dx / fmaxf(um, 0.08f);
-    dt = fminf(fminf(dt_base, law), 0.01f);
-    if (dt < 1e-5f) dt = 1e-5f;
+    double um = (double)__uint_as_float(s->umax_bits);
+    double law = cfl_dx / (um > 0.08 ? um : 0.08);
+    dt = law < dt_base ? law : dt_base;
+    if (dt > 0.01) dt = 0.01;
+    if (dt < 1e-5) dt = 1e-5;
   }
   double gsd = s->gRamp / 1.5;
   if (gsd > 1) gsd = 1;
-  s->gRamp += (double)dt;
+  s->gRamp += dt;
   s->gs = gsd <= 0 ? 0.f : (float)(gsd * (double)gravity);
-  s->dt = dt;
+  s->dt = (float)dt;
+  s->dtD = dt;
   s->simTime += dt;
 }
 
@@ -280,15 +284,25 @@ __device__ __forceinline__ bool face_of(const Geo &g, long f, FaceInfo &fi) {
   return true;
 }
 
-// _applyBC (solver.js:1130-1171, static solids: sv = 0), FLIP snapshot
+// _applyBC (solver.js:1130-1171; solid faces take the solid's velocity), FLIP snapshot
 // (:2579) and the extrapolation validity mark (:1578-1611: weighted, a
 // domain plane or solid-adjacent). With fuse_gravity the radial gravity of
 // _applyPlanetGravitySlice (:1229-1256) and the second _applyBC follow in
 // the same pass (used when viscosity is off).
+// the boundary value of a face next to a solid: the solid cell's velocity
+// component (the left / lower cell first), 0 on the domain planes and when
+// no moving solid exists (sv == nullptr)
+__device__ __forceinline__ float solid_face_value(const Geo &g, const FaceInfo &fi, uint8_t tl,
+                                                  const float *__restrict__ sv) {
+  if (fi.plane || !sv) return 0.f;
+  const float *a = sv + (long)fi.comp * g.nC;
+  return tl == SOLID ? a[fi.cl] : a[fi.cr];
+}
+
 __global__ void k_bc_snapshot(Geo g, const uint8_t *__restrict__ ct, float *__restrict__ vel,
                               float *__restrict__ velO, const float *__restrict__ wgt,
                               uint8_t *__restrict__ valid, int fuse_gravity, const float *__restrict__ gvec,
-                              const DevScalars *__restrict__ s, int write_valid) {
+                              const DevScalars *__restrict__ s, int write_valid, const float *__restrict__ sv) {
   long F = (long)g.uN + g.vN + g.wN;
   long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= F) return;
@@ -296,7 +310,7 @@ __global__ void k_bc_snapshot(Geo g, const uint8_t *__restrict__ ct, float *__re
   face_of(g, f, fi);
   uint8_t tl = fi.cl >= 0 ? ct[fi.cl] : SOLID, tr = fi.cr >= 0 ? ct[fi.cr] : SOLID;
   bool bc = fi.plane || tl == SOLID || tr == SOLID;
-  float u = bc ? 0.f : vel[f];
+  float u = bc ? solid_face_value(g, fi, tl, sv) : vel[f];
   velO[f] = u;
   if (write_valid) valid[f] = (wgt[f] > 1e-8f || bc) ? 1 : 0;
   if (fuse_gravity && !bc && s->gs > 0.f && (tl == FLUID || tr == FLUID)) u = u + gvec[f] * s->dt * s->gs;
@@ -318,7 +332,8 @@ __global__ void k_valid(Geo g, const uint8_t *__restrict__ ct, const float *__re
 
 // radial gravity + _applyBC after viscosity
 __global__ void k_gravity_bc(Geo g, const uint8_t *__restrict__ ct, float *__restrict__ vel,
-                             const float *__restrict__ gvec, const DevScalars *__restrict__ s) {
+                             const float *__restrict__ gvec, const DevScalars *__restrict__ s,
+                             const float *__restrict__ sv) {
   long F = (long)g.uN + g.vN + g.wN;
   long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= F) return;
@@ -326,7 +341,7 @@ __global__ void k_gravity_bc(Geo g, const uint8_t *__restrict__ ct, float *__res
   face_of(g, f, fi);
   uint8_t tl = fi.cl >= 0 ? ct[fi.cl] : SOLID, tr = fi.cr >= 0 ? ct[fi.cr] : SOLID;
   bool bc = fi.plane || tl == SOLID || tr == SOLID;
-  if (bc) { vel[f] = 0.f; return; }
+  if (bc) { vel[f] = solid_face_value(g, fi, tl, sv); return; }
   if (s->gs > 0.f && (tl == FLUID || tr == FLUID)) vel[f] = vel[f] + gvec[f] * s->dt * s->gs;
 }
 

@@ -45,6 +45,7 @@ struct DevScalars {
   unsigned long long mt_count;
   float dtLast;
   float pad;
+  double dtD;          // the substep length as the reference holds it (double)
 };
 
 // Geometry and per-substep constants passed by value to kernels.
@@ -106,6 +107,15 @@ struct Ctx {
   int warmStart = 0, recordSamples = 0;
   int netTiled = 1;  // convolutions staged through LDS (0: the untiled kernel)
   double netLossCoarse = 0;  // weight of the 4x4x4-averaged error term of the training loss
+  // balls (moving solids, solver.js:115, 918-938, 1005-1026, 2294-2564):
+  // nine doubles per ball on the device (x y z vx vy vz r rho submerged),
+  // the solid cell velocities (svx | svy | svz, 3 x nC, allocated with the
+  // first ball), the waterline histogram (24 buckets + count per ball) and
+  // the waterline levels of the last update
+  int nBalls = 0, hadSolidVelocity = 0;
+  double *balls = nullptr;
+  float *sv = nullptr, *ballSurf = nullptr;
+  unsigned *ballHist = nullptr;
   // frame-end contact passes: scratch, the vapor census, candidate overflows
   Contacts *contacts = nullptr;
   int vaporCount = 0, nightVaporCount = 0;

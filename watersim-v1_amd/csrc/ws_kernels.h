@@ -34,7 +34,7 @@ struct PhasePar {
   int spinOn;
 };
 
-__global__ void k_begin_substep(DevScalars *s, float dt_host, int use_law, float cfl_dx, float dt_base,
+__global__ void k_begin_substep(DevScalars *s, double dt_host, int use_law, double cfl_dx, double dt_base,
                                 float gravity);
 __global__ void k_end_substep(DevScalars *s);
 __global__ void k_init_cells(const uint8_t *st, uint8_t *ct, unsigned *cnt, int nC);
@@ -46,9 +46,11 @@ __global__ void k_p2g_tile(Geo g, int tilesX, int tilesY, const unsigned *start,
                            const float *px, const float *py, const float *pz, const float *pvx, const float *pvy,
                            const float *pvz, float *vel, float *wgt);
 __global__ void k_bc_snapshot(Geo g, const uint8_t *ct, float *vel, float *velO, const float *wgt, uint8_t *valid,
-                              int fuse_gravity, const float *gvec, const DevScalars *s, int write_valid);
+                              int fuse_gravity, const float *gvec, const DevScalars *s, int write_valid,
+                              const float *sv);
 __global__ void k_valid(Geo g, const uint8_t *ct, const float *wgt, uint8_t *valid);
-__global__ void k_gravity_bc(Geo g, const uint8_t *ct, float *vel, const float *gvec, const DevScalars *s);
+__global__ void k_gravity_bc(Geo g, const uint8_t *ct, float *vel, const float *gvec, const DevScalars *s,
+                             const float *sv);
 __global__ void k_visc_iter(Geo g, const uint8_t *ct, const float *old, const float *cur, float *out, float nu_dx2,
                             const DevScalars *s, int first);
 __global__ void k_assemble(Geo g, const uint8_t *ct, const float *vel, float *q, float *q2, float *rhs, float *pw,

@@ -36,7 +36,8 @@ var AIR = 0, FLUID = 1, SOLID = 2;
 var STAGE = {
   clear: 0, rasterize: 1, p2g: 2, bc: 3, snapshot: 4, viscosity: 5, gravity: 6, pressure: 7,
   project: 8, vorticity: 9, clamp: 10, extrapolate: 11, g2p: 12, advect: 13, heat: 14,
-  evaporation: 15, currents: 16, phase: 17, pushSurface: 18, vaporCollisions: 19, iceContacts: 20
+  evaporation: 15, currents: 16, phase: 17, pushSurface: 18, vaporCollisions: 19, iceContacts: 20,
+  pushBalls: 21, updateBalls: 22
 };
 
 // scalars mirrored to the device before each call (js/solver.js:56-75, 137-193)
@@ -180,7 +181,29 @@ FluidSolver.prototype._invalidate = function () {
 FluidSolver.prototype._call = function (fn) {
   if (!this._h) throw new Error('watersim: no world yet (call resetWater first)');
   this._push();
-  var r = fn(lib(), this._h);
+  var L = lib();
+  var nb = this.balls.length;
+  // the ball objects are the state callers read and write (solver.js:115);
+  // the device works on a copy taken before and returned after each call
+  if (nb || this._ballsOnDevice) {
+    var st = new Float64Array(9 * nb);
+    for (var i = 0; i < nb; i++) {
+      var b = this.balls[i];
+      st.set([b.x, b.y, b.z, b.vx, b.vy, b.vz, b.r, b.rho, b.submerged], 9 * i);
+    }
+    L.setBalls(this._h, st);
+    this._ballsOnDevice = nb > 0;
+  }
+  var r = fn(L, this._h);
+  if (nb) {
+    var out = new Float64Array(9 * nb);
+    L.getBalls(this._h, out);
+    for (var j = 0; j < nb; j++) {
+      var o = this.balls[j], q = 9 * j;
+      o.x = out[q]; o.y = out[q + 1]; o.z = out[q + 2]; o.vx = out[q + 3]; o.vy = out[q + 4]; o.vz = out[q + 5];
+      o.submerged = out[q + 8];
+    }
+  }
   this._invalidate();
   return r;
 };
@@ -257,6 +280,7 @@ FluidSolver.prototype.resetWater = function (depth) {
   this.capacity = w.capacity;
   this._host = {};
   this._h = L.create(this.nx, this.ny, this.nz, this.dx, w.capacity);
+  this._ballsOnDevice = false;
   var h = this._h;
   L.setF(h, 'coreR', this.coreR); L.setF(h, 'oceanR', w.oceanR); L.setF(h, 'spacing', w.spacing);
   L.setF(h, 'particleVolume', w.particleVolume); L.setF(h, 'dv', w.dv); L.setF(h, 'tn', w.tn);
@@ -310,6 +334,8 @@ FluidSolver.prototype._updatePhaseChanges = stageMethod(STAGE.phase);
 FluidSolver.prototype._pushSurfaceParticles = stageMethod(STAGE.pushSurface);
 FluidSolver.prototype._vaporCollisions = stageMethod(STAGE.vaporCollisions);
 FluidSolver.prototype._iceContacts = stageMethod(STAGE.iceContacts);
+FluidSolver.prototype._pushOutOfBalls = stageMethod(STAGE.pushBalls);
+FluidSolver.prototype._updateBalls = stageMethod(STAGE.updateBalls);
 
 // ---------------------------------------------------------------- surface
 FluidSolver.prototype.splatDensity = function () {
@@ -402,8 +428,9 @@ FluidSolver.prototype.stirAt = function (cx, cy, cz, ux, uy, uz, R, dt) {
   }
 };
 
-FluidSolver.prototype.addBall = function () {
-  throw new Error('watersim: balls are not part of this build');
+// addBall, solver.js:3910-3912
+FluidSolver.prototype.addBall = function (x, y, z, r, rho) {
+  this.balls.push({ x: x, y: y, z: z, vx: 0, vy: 0, vz: 0, r: r, rho: rho, submerged: 0 });
 };
 
 // releases the device context now instead of at garbage collection

@@ -17,7 +17,7 @@ STAGES = {
     "clear": 0, "rasterize": 1, "p2g": 2, "bc": 3, "snapshot": 4, "viscosity": 5, "gravity": 6,
     "pressure": 7, "project": 8, "vorticity": 9, "clamp": 10, "extrapolate": 11, "g2p": 12, "advect": 13,
     "heat": 14, "evaporation": 15, "currents": 16, "phase": 17, "push_surface": 18, "vapor_collisions": 19,
-    "ice_contacts": 20,
+    "ice_contacts": 20, "push_balls": 21, "update_balls": 22,
 }
 
 # exported entry points, as declared in include/watersim.h, watersim_world.h
@@ -26,7 +26,7 @@ EXPORTS = ["ws_create", "ws_destroy", "ws_last_error", "ws_set_f", "ws_get_f", "
            "ws_substep", "ws_stage", "ws_step_frame", "ws_run_substeps", "ws_splat_density", "ws_mt_build",
            "ws_autotune_iso", "ws_kinetic_energy", "ws_sync", "ws_world_build", "ws_world_free",
            "ws_net_param_count", "ws_net_set_params", "ws_net_get_params", "ws_net_forward", "ws_net_grad",
-           "ws_net_bank", "ws_net_bank_count", "ws_net_train"]
+           "ws_net_bank", "ws_net_bank_count", "ws_net_train", "ws_set_balls", "ws_get_balls"]
 
 
 class WorldParams(ctypes.Structure):
@@ -91,6 +91,8 @@ def load(path=LIB_PATH):
     L.ws_net_bank.argtypes = [vp, ci]
     L.ws_net_bank_count.argtypes = [vp, ctypes.POINTER(ci)]
     L.ws_net_train.argtypes = [vp, ci, ci, ci, d, ctypes.c_uint, ctypes.POINTER(d)]
+    L.ws_set_balls.argtypes = [vp, ci, vp]
+    L.ws_get_balls.argtypes = [vp, ci, vp]
     for f in EXPORTS:
         if f not in ("ws_destroy", "ws_last_error", "ws_world_free"):
             getattr(L, f).restype = ci
@@ -103,7 +105,7 @@ PART_F32 = ["px", "py", "pz", "pvx", "pvy", "pvz", "pT", "pAir", "advU", "advV",
             "pWz", "pPh"]
 PART_U8 = ["pflag", "pcool", "pDepth"]
 CELL_U8 = ["cellType", "staticTypes"]
-CELL_F32 = ["q", "rhs"]
+CELL_F32 = ["q", "rhs", "svx", "svy", "svz"]
 FACE_F32 = ["u", "v", "w", "uO", "vO", "wO", "uW", "vW", "wW", "gu", "gv", "gw"]
 FACE_U8 = ["validU", "validV", "validW"]
 
@@ -173,6 +175,10 @@ class Solver:
         if name == "terrainSolid":
             tn = int(self.get("tn"))
             return tn ** 3, np.uint8
+        if name == "ballHist":
+            return 25 * int(self.get("nBalls")), np.uint32
+        if name == "ballSurf":
+            return int(self.get("nBalls")), np.float32
         raise KeyError(name)
 
     def upload(self, name, arr):
@@ -215,6 +221,17 @@ class Solver:
 
     def sync(self):
         self._check(self.L.ws_sync(self.ctx), "sync")
+
+    # ---- balls (moving solids): (n, 9) doubles x y z vx vy vz r rho submerged
+    def set_balls(self, balls):
+        b = np.ascontiguousarray(np.asarray(balls, np.float64).reshape(-1, 9))
+        self._check(self.L.ws_set_balls(self.ctx, len(b), b.ctypes.data_as(ctypes.c_void_p)), "set_balls")
+
+    def balls(self):
+        n = int(self.get("nBalls"))
+        out = np.zeros((n, 9), np.float64)
+        self._check(self.L.ws_get_balls(self.ctx, n, out.ctypes.data_as(ctypes.c_void_p)), "get_balls")
+        return out
 
     # ---- the learned pressure warm start (include/watersim_learn.h)
     def net_params(self):
