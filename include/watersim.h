@@ -95,7 +95,8 @@ int ws_get_f(ws_ctx *c, const char *name, double *v);
  * and the static world: "staticTypes" (u8, C), "gu","gv","gw" (f32 faces),
  * "terrainR" (f32, tn^3); with balls also "svx","svy","svz" (f32, C, the
  * solid cell velocities), "ballHist" (u32, 25 per ball: the waterline
- * histogram and its count) and "ballSurf" (f32 per ball, the waterline).
+ * histogram and its count) and "ballSurf" (f32 per ball, the waterline);
+ * after ws_metaballs_build "metaballField" (f32, (m+1)^3).
  * nbytes must equal the field's byte size (for particle fields: nP
  * elements). */
 int ws_upload(ws_ctx *c, const char *name, const void *host, size_t nbytes);
@@ -129,6 +130,16 @@ int ws_mt_build(ws_ctx *c, const float *dens, int nx, int ny, int nz, double dx,
                 float *pos, float *nrm, long cap, long *count, double *volume);
 /* autotuneIso(), solver.js:3838-3849: 8 bisections of iso on the device. */
 int ws_autotune_iso(ws_ctx *c, double *iso);
+/* Metaballs.build(solver, tension, detail), surface.js:178-284: the skin of
+ * the liquid particles (flag 0) on its own cubic corner lattice of
+ * m = max(6, round(nx * clamp(detail, 0.5, 1.6))) cells a side (detail NaN
+ * or <= 0: the solver lattice), kernel radius (1.6 + 2.2 t) spacings,
+ * contoured at Metaballs.iso(t) = 1.5 ((1.6 + 2.2 t) / 1.6)^1.5 with
+ * t = clamp(tension, 0, 1) (NaN: 0.5). Mesh output as ws_mt_build; the
+ * contour level and lattice size go to *iso and *lattice (either may be
+ * NULL). The field stays readable as "metaballField". */
+int ws_metaballs_build(ws_ctx *c, double tension, double detail, float *pos, float *nrm, long cap, long *count,
+                       double *volume, double *iso, int *lattice);
 /* kineticEnergy(), solver.js:3914-3921 (FP64 on the device). */
 int ws_kinetic_energy(ws_ctx *c, double *ke);
 

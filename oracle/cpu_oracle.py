@@ -77,6 +77,9 @@ def lib(bits=32):
         L.oc_update_balls.argtypes = [vp, d]
         L.oc_ball_hist.argtypes = [vp]
         L.oc_ball_hist.restype = ctypes.POINTER(ctypes.c_float)
+        L.oc_metaball_params.argtypes = [ci, d, d, d, d, ctypes.POINTER(ci), ctypes.POINTER(d), ctypes.POINTER(d),
+                                         ctypes.POINTER(ci), ctypes.POINTER(d)]
+        L.oc_metaball_field.argtypes = [vp, vp, vp, vp, ci, ci, d, d, ci, vp]
         L.oc_ball_surf.argtypes = [vp]
         L.oc_ball_surf.restype = ctypes.POINTER(ctypes.c_float)
         _LIBS[bits] = L
@@ -226,3 +229,24 @@ def mt_build(dens, n, dx, iso, cap=None, bits=32):
     nrm = np.zeros(max(cap, 1) * 3, np.float32)
     cnt = L.oc_mt_build(_ptr(dens), nx, ny, nz, dx, iso, _ptr(pos), _ptr(nrm), cap, ctypes.byref(vol))
     return cnt, vol.value, pos[:cnt * 3], nrm[:cnt * 3]
+
+
+def metaball_params(nx, dx, spacing, tension, detail=float("nan")):
+    """(m, mdx, R, rad, iso) of Metaballs.build (surface.js:178-250)."""
+    L = lib(32)
+    m, rad = ctypes.c_int(), ctypes.c_int()
+    mdx, R, iso = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    L.oc_metaball_params(nx, dx, spacing, tension, detail, ctypes.byref(m), ctypes.byref(mdx), ctypes.byref(R),
+                         ctypes.byref(rad), ctypes.byref(iso))
+    return m.value, mdx.value, R.value, rad.value, iso.value
+
+
+def metaball_field(px, py, pz, pflag, m, mdx, R, rad, bits=32):
+    """The metaball corner field, (m+1)^3 values."""
+    L = lib(bits)
+    real = np.float32 if bits == 32 else np.float64
+    px, py, pz = (np.ascontiguousarray(a, dtype=real) for a in (px, py, pz))
+    fl = np.ascontiguousarray(pflag, dtype=np.uint8)
+    field = np.zeros((m + 1) ** 3, real)
+    L.oc_metaball_field(_ptr(px), _ptr(py), _ptr(pz), _ptr(fl), len(px), m, mdx, R, rad, _ptr(field))
+    return field

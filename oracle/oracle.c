@@ -2015,6 +2015,74 @@ long oc_mt_build(const oc_real *dens, int nx, int ny, int nz, double dx, double 
   return vcount;
 }
 
+/* Math.round: nearest integer, halves toward +infinity (x - floor(x) is
+ * exact, unlike floor(x + 0.5)) */
+static double js_round(double x) {
+  double r = floor(x);
+  return x - r >= 0.5 ? r + 1 : r;
+}
+
+/* ---------------------------------------------------------- metaballs
+ * Metaballs.build, surface.js:178-284: the kernel radius in lattice cells,
+ * the contour level and the field lattice of a tension / detail setting,
+ * then the summed compact kernels of the liquid particles. */
+void oc_metaball_params(int nx, double dx, double spacing, double tension, double detail, int *m_out,
+                        double *mdx_out, double *R_out, int *rad_out, double *iso_out) {
+  const double R_MIN = 1.6, R_SPAN = 2.2, ISO0 = 1.5, ALPHA = 1.5, D_MIN = 0.5, D_MAX = 1.6;
+  double f = 1;
+  if (isfinite(detail) && detail > 0) f = detail < D_MIN ? D_MIN : (detail > D_MAX ? D_MAX : detail);
+  /* Math.max(6, Math.round(nx * f)); Math.round rounds halves up */
+  int m = (int)js_round(nx * f);
+  if (m < 6) m = 6;
+  double W = nx * dx;
+  double mdx = m == nx ? dx : W / m;
+  double t = isfinite(tension) ? tension : 0.5;
+  t = t < 0 ? 0 : (t > 1 ? 1 : t);
+  double cell = mdx > 0 ? mdx : dx;
+  double R = (R_MIN + R_SPAN * t) * spacing / cell;
+  double rc = ceil(R);
+  *m_out = m;
+  *mdx_out = mdx;
+  *R_out = R;
+  *rad_out = (int)(3 < rc ? 3 : rc);
+  *iso_out = ISO0 * pow((R_MIN + R_SPAN * t) / R_MIN, ALPHA);
+}
+
+/* the field, surface.js:256-281: every liquid particle adds (1 - d^2/R^2)^2
+ * to the lattice corners within `rad` of its nearest corner and within R,
+ * in particle order, each sum rounded to the field type */
+void oc_metaball_field(const oc_real *px, const oc_real *py, const oc_real *pz, const uint8_t *pflag, int nP, int m,
+                       double mdx, double R, int rad, oc_real *field) {
+  long sj = m + 1, sk = (long)(m + 1) * (m + 1), len = sk * (m + 1);
+  for (long i = 0; i < len; i++) field[i] = 0;
+  double R2 = R * R;
+  for (int p = 0; p < nP; p++) {
+    if (pflag[p] != 0) continue;
+    double gx = px[p] / mdx, gy = py[p] / mdx, gz = pz[p] / mdx;
+    int i0 = (int)js_round(gx), j0 = (int)js_round(gy), k0 = (int)js_round(gz);
+    int ia = i0 - rad > 0 ? i0 - rad : 0, ib = i0 + rad < m ? i0 + rad : m;
+    int ja = j0 - rad > 0 ? j0 - rad : 0, jb = j0 + rad < m ? j0 + rad : m;
+    int ka = k0 - rad > 0 ? k0 - rad : 0, kb = k0 + rad < m ? k0 + rad : m;
+    for (int k = ka; k <= kb; k++) {
+      double ez = k - gz, dz2 = ez * ez;
+      if (dz2 >= R2) continue;
+      long row = k * sk;
+      for (int j = ja; j <= jb; j++) {
+        double ey = j - gy, dy2 = dz2 + ey * ey;
+        if (dy2 >= R2) continue;
+        long base = row + j * sj;
+        for (int i = ia; i <= ib; i++) {
+          double ex = i - gx;
+          double d2 = dy2 + ex * ex;
+          if (d2 >= R2) continue;
+          double u = 1 - d2 / R2;
+          field[base + i] = (oc_real)(field[base + i] + u * u);
+        }
+      }
+    }
+  }
+}
+
 /* autotuneIso, solver.js:3838-3849 */
 double oc_autotune_iso(oc_ctx *c) {
   double Vt = c->nP * c->particleVolume, lo = 0.3, hi = 3.0;

@@ -100,6 +100,12 @@ void oc_splat_density(oc_ctx *c);
 long oc_mt_build(const oc_real *dens, int nx, int ny, int nz, double dx, double iso,
                  float *pos, float *nrm, long cap, double *volume);
 double oc_autotune_iso(oc_ctx *c);
+/* Metaballs.build (surface.js:178-284): lattice, kernel radius, stencil
+ * reach and contour level of a tension / detail setting, and the field */
+void oc_metaball_params(int nx, double dx, double spacing, double tension, double detail, int *m, double *mdx,
+                        double *R, int *rad, double *iso);
+void oc_metaball_field(const oc_real *px, const oc_real *py, const oc_real *pz, const uint8_t *pflag, int nP, int m,
+                       double mdx, double R, int rad, oc_real *field);
 double oc_kinetic_energy(oc_ctx *c);
 
 #ifdef __cplusplus

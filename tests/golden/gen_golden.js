@@ -393,3 +393,49 @@ function frameEnd(w, s) {
   stagedSubstep(w, s, dtLaw(s));
   w.save('balls_32.wsg.gz');
 })();
+
+// ---------------------------------------------------------------- scene E
+// Metaballs.build (js/surface.js:178-284) on a settled 32^3 planet for a
+// sweep of tension and detail settings, and on the 7^3 particle block of
+// the reference's own metaball test (test/metaball.test.js:31-46).
+(function sceneE() {
+  if (ONLY && ONLY !== 'metaballs') return;
+  var MB = global.MarchingTetrahedra.Metaballs;
+  var w = new Writer();
+  w.meta.scene = 'metaballs';
+  var n = 32, W = 5.1, dx = W / n;
+  var s = new FluidSolver({ nx: n, ny: n, nz: n, dx: dx, targetParticles: 10000, mode: 'sphere',
+    coreR: 0.25 * W, terrainSeed: 20260118 });
+  s.resetWater(0.4 * W - 0.25 * W);
+  s.pic = 0.05; s.omega = 1.7; s.pressureIters = 40; s.gravity = 9.8; s.maxSpeed = 3.2; s.vorticity = 0;
+  for (var k = 0; k < 40; k++) s._substep(dtLaw(s));
+  w.meta.planet = { nx: n, dx: dx, W: s.W, spacing: s.spacing, nP: s.nP };
+  w.add('planet_px', copy(s.px, s.nP)); w.add('planet_py', copy(s.py, s.nP)); w.add('planet_pz', copy(s.pz, s.nP));
+  w.add('planet_pflag', copy(s.pflag, s.nP));
+  var cases = [[0, null], [0.5, null], [1, null], [0.5, 0.6], [0.5, 1.4], [0.25, 0.3], [0.75, 2.5]];
+  w.meta.planet_cases = [];
+  cases.forEach(function (c, ci) {
+    var m = c[1] === null ? MB.build(s, c[0]) : MB.build(s, c[0], c[1]);
+    w.meta.planet_cases.push({ tension: c[0], detail: c[1], count: m.count, volume: m.volume, iso: MB.iso(c[0]) });
+    if (m.count < 70000) { w.add('planet_pos_' + ci, copy(m.pos, m.count * 3)); w.add('planet_nrm_' + ci, copy(m.nrm, m.count * 3)); }
+  });
+  // the reference test's stub: 16^3 lattice, spacing dx/4, a 7^3 block
+  var sn = 16, sdx = 1 / sn, sp = sdx * 0.25, part = [];
+  for (var kk = 0; kk < 7; kk++) for (var jj = 0; jj < 7; jj++) for (var ii = 0; ii < 7; ii++)
+    part.push((0.5 - 3 * sp) + ii * sp, (0.5 - 3 * sp) + jj * sp, (0.5 - 3 * sp) + kk * sp);
+  var np = part.length / 3, spx = new Float32Array(np), spy = new Float32Array(np), spz = new Float32Array(np);
+  for (var q = 0; q < np; q++) { spx[q] = part[3 * q]; spy[q] = part[3 * q + 1]; spz[q] = part[3 * q + 2]; }
+  var stub = { nx: sn, ny: sn, nz: sn, dx: sdx, W: sn * sdx, H: sn * sdx, D: sn * sdx, spacing: sp, nP: np,
+    px: spx, py: spy, pz: spz, pflag: new Int8Array(np) };
+  w.meta.stub = { nx: sn, dx: sdx, spacing: sp, nP: np, cases: [] };
+  w.add('stub_px', spx); w.add('stub_py', spy); w.add('stub_pz', spz);
+  [0, 0.25, 0.5, 0.75, 1].forEach(function (t) {
+    var m = MB.build(stub, t);
+    w.meta.stub.cases.push({ tension: t, detail: null, count: m.count, volume: m.volume });
+  });
+  [0.6, 1.4, 0.1, 9].forEach(function (d) {
+    var m = MB.build(stub, 0.5, d);
+    w.meta.stub.cases.push({ tension: 0.5, detail: d, count: m.count, volume: m.volume });
+  });
+  w.save('metaballs.wsg.gz');
+})();

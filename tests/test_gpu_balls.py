@@ -21,19 +21,28 @@ pytestmark = pytest.mark.gpu
 PART = ["px", "py", "pz", "pvx", "pvy", "pvz"]
 
 
-def ball_scene(tag="in_"):
+def ball_scene(tag="in_", balls="in_balls"):
     g = golden.load("balls_32")
     s = gpu_from_golden(g, tag=tag)
     s.upload("terrainSolid", g["terrain_solid"])
-    s.set_balls(g[tag + "balls"])
+    s.set_balls(g[balls])
     return g, s
+
+
+def report_cells(got, want, g):
+    d = np.nonzero(got != want)[0]
+    ball = (g["rast_cellType"] == 2) & (g["static_types"] != 2)
+    for c in d[:8]:
+        print("   cell %d: got %d want %d static %d ball-cell %d" % (c, got[c], want[c], g["static_types"][c], ball[c]))
+    return len(d)
 
 
 def test_ball_raster_bit_exact():
     g, s = ball_scene()
     s.stage("clear")
     s.stage("rasterize")
-    assert np.array_equal(s.download("cellType"), g["mark_cellType"])
+    got = s.download("cellType")
+    assert report_cells(got, g["mark_cellType"], g) == 0
     for k in ["svx", "svy", "svz"]:
         assert np.array_equal(s.download(k), g["rast_" + k]), k
     print("  ball cells %d, moving-solid velocities bit-exact"
@@ -58,8 +67,7 @@ def test_ball_push_out_bit_exact():
 def test_ball_integrator_bit_exact():
     """_updateBalls on the reference's own post-advection particles and
     extrapolated grid."""
-    g, s = ball_scene(tag="adv_")
-    s.set_balls(g["in_balls"])
+    g, s = ball_scene(tag="adv_", balls="in_balls")
     for k in "uvw":
         s.upload(k, g["ext_" + k])
     s.stage("update_balls", g.meta["stage_dt"])
@@ -97,7 +105,7 @@ def test_substep_with_balls_against_oracle():
 def test_ball_trajectory_follows_reference():
     """20 substeps from the fill state with the dt law on the host, ball
     states against the reference's recorded trajectory."""
-    g, s = ball_scene(tag="fill_")
+    g, s = ball_scene(tag="fill_", balls="fill_balls")
     s.upload("q", g["fill_q"])
     s.set(gRamp=g.meta["fill"]["gRamp"])
     rec = g.meta["trajectory"]

@@ -107,12 +107,16 @@ def _fe_compare(s, g, a, b, keys, tol=2e-5):
     return moved
 
 
-def test_vapor_collisions():
+@pytest.mark.parametrize("serial_max", [48, 0, 1 << 30])
+def test_vapor_collisions(serial_max):
     """Parcel-parcel and parcel-liquid contacts on the perturbed frame-end
     state (a chain of 80 touching parcels along the sea surface, real steam
     above it): the census exact, positions, velocities and spins to float
-    rounding of the reference's serial sweep."""
+    rounding of the reference's serial sweep. serial_max picks the path of
+    the contact groups: 0 sends every group through the dependency-ordered
+    parallel sweep, 2^30 every group through the one-thread serial sweep."""
     g, s = _fe_gpu("fe0_")
+    s.set(vaporSerialMax=serial_max)
     s.stage("vapor_collisions", 0.0)
     fe = g.meta["frame_end"]
     assert int(s.get("vaporCount")) == fe["vaporCount"]

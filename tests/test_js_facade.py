@@ -40,6 +40,11 @@ def test_marching_tets_through_node():
 
 
 @pytest.mark.gpu
+def test_metaballs_through_node():
+    run_node("metaball_gpu.test.js", timeout=120, gpu=True)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("section", ["settle", "heat", "shade", "currents", "whirl", "long", "balls"])
 def test_planet_invariants_through_node(section):
     run_node("planet_gpu.test.js", section, timeout=110, gpu=True)

@@ -275,6 +275,35 @@ static napi_value js_mt_build(napi_env env, napi_callback_info info) {
   return o;
 }
 
+/* metaballs(h, tension, detail, pos, nrm) -> {count, volume, iso, lattice} */
+static napi_value js_metaballs(napi_env env, napi_callback_info info) {
+  napi_value a[5];
+  if (!get_args(env, info, 5, a)) return NULL;
+  ws_ctx *c = get_ctx(env, a[0]);
+  if (!c) return NULL;
+  double t, d;
+  void *pos, *nrm;
+  size_t nbP, nbN;
+  if (!get_double(env, a[1], &t) || !get_double(env, a[2], &d)) return NULL;
+  if (!get_bytes(env, a[3], &pos, &nbP) || !get_bytes(env, a[4], &nrm, &nbN)) return NULL;
+  long cap = (long)(nbP / (3 * sizeof(float)));
+  if ((long)(nbN / (3 * sizeof(float))) < cap) cap = (long)(nbN / (3 * sizeof(float)));
+  if (!pos || !nrm) cap = 0;
+  long count = 0;
+  double volume = 0, iso = 0;
+  int lattice = 0;
+  int rc = ws_metaballs_build(c, t, d, cap ? (float *)pos : NULL, cap ? (float *)nrm : NULL, cap, &count, &volume,
+                              &iso, &lattice);
+  if (rc) return fail(env, c, rc, "ws_metaballs_build");
+  napi_value o;
+  NAPI_OK(napi_create_object(env, &o));
+  NAPI_OK(napi_set_named_property(env, o, "count", num(env, (double)count)));
+  NAPI_OK(napi_set_named_property(env, o, "volume", num(env, volume)));
+  NAPI_OK(napi_set_named_property(env, o, "iso", num(env, iso)));
+  NAPI_OK(napi_set_named_property(env, o, "lattice", num(env, (double)lattice)));
+  return o;
+}
+
 static napi_value scalar_call(napi_env env, napi_callback_info info, int which) {
   napi_value a[1];
   if (!get_args(env, info, 1, a)) return NULL;
@@ -400,6 +429,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"autotuneIso", 0, js_autotune, 0, 0, 0, napi_default, 0},
       {"kineticEnergy", 0, js_kinetic, 0, 0, 0, napi_default, 0},
       {"sync", 0, js_sync, 0, 0, 0, napi_default, 0},
+      {"metaballs", 0, js_metaballs, 0, 0, 0, napi_default, 0},
       {"setBalls", 0, js_set_balls, 0, 0, 0, napi_default, 0},
       {"getBalls", 0, js_get_balls, 0, 0, 0, napi_default, 0},
       {"worldBuild", 0, js_world_build, 0, 0, 0, napi_default, 0},

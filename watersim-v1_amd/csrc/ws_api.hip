@@ -569,6 +569,7 @@ bool field_of(ws_ctx *c, const char *n, Field &f) {
     GF("svx", c->sv, float, c->nC) GF("svy", c->sv + c->nC, float, c->nC) GF("svz", c->sv + 2 * c->nC, float, c->nC)
     GF("ballHist", c->ballHist, unsigned, (long)c->nBalls * 25) GF("ballSurf", c->ballSurf, float, c->nBalls)
   }
+  if (c->mbField) GF("metaballField", c->mbField, float, (long)(c->mbM + 1) * (c->mbM + 1) * (c->mbM + 1))
 #undef PF
 #undef GF
   return false;
@@ -723,6 +724,7 @@ void ws_destroy(ws_ctx *c) {
   net_free(c);
   contacts_release(c, true);
   balls_release(c);
+  metaballs_release(c);
   void *ptrs[] = {c->staticTypes, c->gu, c->thr, c->terrainR, c->cellType, c->nbm, c->dg, c->q, c->q2, c->rhs, c->pw,
                   c->sorActive, c->tbActive, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg,
                   c->cellCount, c->cellStart, c->vort, c->vel, c->velO, c->wgt, c->scr, c->scr2, c->valid, c->dens,
@@ -751,7 +753,7 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
   SF(heatK) SF(Tamb) SF(shadeCool) SF(sunActivity) SF(evapIntensity) SF(snowT) SF(meltT) SF(cloudP)
   SF(currents) SF(Rlo) SF(Rhi) SF(simTime) SF(heatAcc) SF(netLossCoarse)
   SI(pressureIters) SI(maxPressureIters) SI(residualEvery) SI(ceilReflect) SI(hasSun) SI(measureResidual)
-  SI(adaptiveChecks) SI(sorMode) SI(spinOn) SI(warmStart) SI(recordSamples) SI(pressureCompare) SI(netTiled)
+  SI(adaptiveChecks) SI(sorMode) SI(spinOn) SI(warmStart) SI(recordSamples) SI(pressureCompare) SI(netTiled) SI(vaporSerialMax)
 #undef SF
 #undef SI
   if (!strcmp(n, "gRamp")) { c->gRamp = v; return push_ramp(c); }
@@ -797,7 +799,7 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   GF(maxPressureIters) GF(nP) GF(capacity) GF(residual) GF(domainR) GF(dx) GF(tn) GF(dv)
   GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal) GF(residualsTotal) GF(ticksTotal) GF(sorMode) GF(sorCellSweeps)
   GF(cmpMsPlain) GF(cmpMsLearned) GF(cmpSweepsPlain) GF(cmpSweepsLearned) GF(cmpSolves) GF(warmStart) GF(netMs) GF(netCalls)
-  GF(vaporCount) GF(nightVaporCount) GF(contactOverflow) GF(nBalls)
+  GF(vaporCount) GF(nightVaporCount) GF(contactOverflow) GF(nBalls) GF(mbM)
   GF(heatK) GF(Tamb) GF(shadeCool) GF(sunActivity) GF(evapIntensity) GF(snowT) GF(meltT) GF(cloudP) GF(currents)
   GF(simTime) GF(heatAcc) GF(evapCount) GF(cloudT) GF(rainT) GF(rainLiftAge)
 #undef GF

@@ -73,45 +73,14 @@ int ensure_scan(ws_ctx *c, long n) {
 
 }  // namespace
 
-extern "C" {
+namespace ws {
 
-int ws_splat_density(ws_ctx *c) {
-  if (!c) return WS_EINVAL;
-  int nC = (int)c->nC;
-  hipLaunchKernelGGL(k_zero_counts, dim3(blocks_for(nC + 1, 256)), dim3(256), 0, c->stream, c->cellCount,
-                     (long)nC + 1);
-  if (c->nP > 0)
-    hipLaunchKernelGGL(k_bin_particles, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->geo, c->thr, c->nP,
-                       c->px, c->py, c->pz, c->pflag, c->cellCount, c->pcell, c->prank);
-  WS_LAUNCHED();
-  WS_HIP(hipcub::DeviceScan::ExclusiveSum(c->scanTmp, c->scanTmpBytes, c->cellCount, c->cellStart, nC + 1,
-                                          c->stream));
-  if (c->nP > 0)
-    hipLaunchKernelGGL(k_bin_scatter, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->nP, c->pcell,
-                       c->prank, c->cellStart, c->sortedIdx);
-  double r = 1.6 * c->spacing;
-  World w{};
-  hipLaunchKernelGGL(k_splat_density, dim3(blocks_for(c->nD, 256)), dim3(256), 0, c->stream, c->geo, w, c->nP,
-                     c->cellStart, c->sortedIdx, c->px, c->py, c->pz, c->dens, (float)(r * r));
-  WS_LAUNCHED();
-  WS_HIP(hipStreamSynchronize(c->stream));
-  return WS_OK;
-}
-
-int ws_mt_build(ws_ctx *c, const float *dens, int nx, int ny, int nz, double dx, double iso, float *pos, float *nrm,
-                long cap, long *count, double *volume) {
-  if (!c || nx < 1 || ny < 1 || nz < 1 || !(dx > 0)) return WS_EINVAL;
-  long nC = (long)nx * ny * nz, nD = (long)(nx + 1) * (ny + 1) * (nz + 1);
-  const float *dd = c->dens;
+// marching tetrahedra over a device corner field of (nx+1)(ny+1)(nz+1)
+// values: count, scan, emit with the FP64 volume
+int mt_build_dev(ws_ctx *c, const float *dd, int nx, int ny, int nz, double dx, double iso, float *pos, float *nrm,
+                 long cap, long *count, double *volume) {
+  long nC = (long)nx * ny * nz;
   int r;
-  if (dens) {
-    if ((r = grow(c, &c->mtDens, c->mtDensN, nD))) return r;
-    WS_HIP(hipMemcpyAsync(c->mtDens, dens, sizeof(float) * nD, hipMemcpyHostToDevice, c->stream));
-    dd = c->mtDens;
-  } else if (nx != c->nx || ny != c->ny || nz != c->nz) {
-    c->err = "ws_mt_build on the context field needs the context dimensions";
-    return WS_EINVAL;
-  }
   long have = c->mtCells;
   if ((r = grow(c, &c->mtOff, have, nC + 1))) return r;
   have = c->mtCells;
@@ -147,15 +116,60 @@ int ws_mt_build(ws_ctx *c, const float *dens, int nx, int ny, int nz, double dx,
   double vol = 0;
   WS_HIP(hipMemcpyAsync(&vol, vpart + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   long ncopy = need < cap ? need : cap;
-  if (pos && ncopy > 0) {
+  if (pos && ncopy > 0)
     WS_HIP(hipMemcpyAsync(pos, c->mtPos, sizeof(float) * 3 * ncopy, hipMemcpyDeviceToHost, c->stream));
+  if (pos && nrm && ncopy > 0)
     WS_HIP(hipMemcpyAsync(nrm, c->mtNrm, sizeof(float) * 3 * ncopy, hipMemcpyDeviceToHost, c->stream));
-  }
   WS_HIP(hipFreeAsync(vpart, c->stream));
   WS_HIP(hipStreamSynchronize(c->stream));
   if (count) *count = (long)total;
   if (volume) *volume = std::fabs(vol);
   return WS_OK;
+}
+
+
+}  // namespace ws
+
+extern "C" {
+
+int ws_splat_density(ws_ctx *c) {
+  if (!c) return WS_EINVAL;
+  int nC = (int)c->nC;
+  hipLaunchKernelGGL(k_zero_counts, dim3(blocks_for(nC + 1, 256)), dim3(256), 0, c->stream, c->cellCount,
+                     (long)nC + 1);
+  if (c->nP > 0)
+    hipLaunchKernelGGL(k_bin_particles, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->geo, c->thr, c->nP,
+                       c->px, c->py, c->pz, c->pflag, c->cellCount, c->pcell, c->prank);
+  WS_LAUNCHED();
+  WS_HIP(hipcub::DeviceScan::ExclusiveSum(c->scanTmp, c->scanTmpBytes, c->cellCount, c->cellStart, nC + 1,
+                                          c->stream));
+  if (c->nP > 0)
+    hipLaunchKernelGGL(k_bin_scatter, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->nP, c->pcell,
+                       c->prank, c->cellStart, c->sortedIdx);
+  double r = 1.6 * c->spacing;
+  World w{};
+  hipLaunchKernelGGL(k_splat_density, dim3(blocks_for(c->nD, 256)), dim3(256), 0, c->stream, c->geo, w, c->nP,
+                     c->cellStart, c->sortedIdx, c->px, c->py, c->pz, c->dens, (float)(r * r));
+  WS_LAUNCHED();
+  WS_HIP(hipStreamSynchronize(c->stream));
+  return WS_OK;
+}
+
+int ws_mt_build(ws_ctx *c, const float *dens, int nx, int ny, int nz, double dx, double iso, float *pos, float *nrm,
+                long cap, long *count, double *volume) {
+  if (!c || nx < 1 || ny < 1 || nz < 1 || !(dx > 0)) return WS_EINVAL;
+  long nD = (long)(nx + 1) * (ny + 1) * (nz + 1);
+  const float *dd = c->dens;
+  int r;
+  if (dens) {
+    if ((r = grow(c, &c->mtDens, c->mtDensN, nD))) return r;
+    WS_HIP(hipMemcpyAsync(c->mtDens, dens, sizeof(float) * nD, hipMemcpyHostToDevice, c->stream));
+    dd = c->mtDens;
+  } else if (nx != c->nx || ny != c->ny || nz != c->nz) {
+    c->err = "ws_mt_build on the context field needs the context dimensions";
+    return WS_EINVAL;
+  }
+  return mt_build_dev(c, dd, nx, ny, nz, dx, iso, pos, nrm, cap, count, volume);
 }
 
 int ws_autotune_iso(ws_ctx *c, double *iso) {

@@ -169,62 +169,227 @@ __device__ __forceinline__ unsigned member(const unsigned long long *gk, const u
   return (unsigned)(xk[(unsigned)(gk[t] & 0xffffffffull)] & 0xffffffffull);
 }
 
-// _vaporCollisions' parcel-parcel sweep, one group per thread
+// One pair of _vaporCollisions' parcel-parcel sweep (solver.js:3380-3470):
+// i is the earlier parcel in x order. Returns false where the x window of i
+// ends (the sweep moves on to the next parcel).
+struct VaporState {
+  float *px, *py, *pz, *vx, *vy, *vz, *wx, *wy, *wz;
+};
+
+__device__ __forceinline__ bool vapor_pair(const VaporState &P, unsigned i, unsigned j, float rad, float rad2,
+                                           int spin) {
+  float *px = P.px, *py = P.py, *pz = P.pz, *vx = P.vx, *vy = P.vy, *vz = P.vz;
+  if (px[j] - px[i] >= rad) return false;
+  float ddx = px[j] - px[i], ddy = py[j] - py[i], ddz = pz[j] - pz[i];
+  float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+  if (d2 >= rad2) return true;
+  float d = sqrtf(d2);
+  if (!(d != 0.f)) d = 1e-9f;
+  float nX = ddx / d, nY = ddy / d, nZ = ddz / d;
+  float vRel = (vx[i] - vx[j]) * nX + (vy[i] - vy[j]) * nY + (vz[i] - vz[j]) * nZ;
+  if (vRel > 0.f) {
+    vx[i] -= vRel * nX; vy[i] -= vRel * nY; vz[i] -= vRel * nZ;
+    vx[j] += vRel * nX; vy[j] += vRel * nY; vz[j] += vRel * nZ;
+  }
+  float half = (rad - d) * 0.5f;
+  px[i] -= nX * half; py[i] -= nY * half; pz[i] -= nZ * half;
+  px[j] += nX * half; py[j] += nY * half; pz[j] += nZ * half;
+  float kick = (rad - d) * 2.f;
+  vx[i] -= nX * kick; vy[i] -= nY * kick; vz[i] -= nZ * kick;
+  vx[j] += nX * kick; vy[j] += nY * kick; vz[j] += nZ * kick;
+  if (!spin) return true;
+  float *wx = P.wx, *wy = P.wy, *wz = P.wz;
+  float R = rad * 0.5f;
+  float wSx = wx[i] + wx[j], wSy = wy[i] + wy[j], wSz = wz[i] + wz[j];
+  float usx = (vx[i] - vx[j]) + R * (wSy * nZ - wSz * nY);
+  float usy = (vy[i] - vy[j]) + R * (wSz * nX - wSx * nZ);
+  float usz = (vz[i] - vz[j]) + R * (wSx * nY - wSy * nX);
+  float uN = usx * nX + usy * nY + usz * nZ;
+  float utx = usx - uN * nX, uty = usy - uN * nY, utz = usz - uN * nZ;
+  float utM = sqrtf(utx * utx + uty * uty + utz * utz);
+  if (utM < 1e-9f) return true;
+  float jt = utM / 7.f;
+  if (vRel > 0.f) { float mu = 0.25f * vRel; if (jt > mu) jt = mu; }
+  float jx = -utx / utM * jt, jy = -uty / utM * jt, jz = -utz / utM * jt;
+  vx[i] += jx; vy[i] += jy; vz[i] += jz;
+  vx[j] -= jx; vy[j] -= jy; vz[j] -= jz;
+  float invI = 1.f / (0.4f * R * R);
+  float dwx = R * (nY * jz - nZ * jy) * invI, dwy = R * (nZ * jx - nX * jz) * invI;
+  float dwz = R * (nX * jy - nY * jx) * invI;
+  wx[i] += dwx; wy[i] += dwy; wz[i] += dwz;
+  wx[j] += dwx; wy[j] += dwy; wz[j] += dwz;
+  return true;
+}
+
+// the sweep of a small group, one group per thread; groups larger than
+// serialMax (and not marked for the serial path) go to k_vc_sweep
 __global__ void k_ct_vapor_groups(int n, const unsigned long long *__restrict__ gk,
-                                  const unsigned long long *__restrict__ xk, float rad, int spin,
-                                  float *__restrict__ px, float *__restrict__ py, float *__restrict__ pz,
-                                  float *__restrict__ vx, float *__restrict__ vy, float *__restrict__ vz,
-                                  float *__restrict__ wx, float *__restrict__ wy, float *__restrict__ wz) {
+                                  const unsigned long long *__restrict__ xk, float rad, int spin, VaporState P,
+                                  int serialMax, const unsigned *__restrict__ serialOnly) {
   int t0 = blockIdx.x * blockDim.x + threadIdx.x;
   if (t0 >= n) return;
   if (t0 > 0 && (gk[t0] >> 32) == (gk[t0 - 1] >> 32)) return;  // not the first member of its group
   int t1 = t0 + 1;
   while (t1 < n && (gk[t1] >> 32) == (gk[t0] >> 32)) t1++;
   if (t1 - t0 < 2) return;
+  if (t1 - t0 > serialMax && !serialOnly[gk[t0] >> 32]) return;
   const float rad2 = rad * rad;
   for (int a = t0; a < t1; a++) {
     unsigned i = member(gk, xk, a);
-    for (int b = a + 1; b < t1; b++) {
-      unsigned j = member(gk, xk, b);
-      if (px[j] - px[i] >= rad) break;
-      float ddx = px[j] - px[i], ddy = py[j] - py[i], ddz = pz[j] - pz[i];
-      float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-      if (d2 >= rad2) continue;
-      float d = sqrtf(d2);
-      if (!(d != 0.f)) d = 1e-9f;
-      float nX = ddx / d, nY = ddy / d, nZ = ddz / d;
-      float vRel = (vx[i] - vx[j]) * nX + (vy[i] - vy[j]) * nY + (vz[i] - vz[j]) * nZ;
-      if (vRel > 0.f) {
-        vx[i] -= vRel * nX; vy[i] -= vRel * nY; vz[i] -= vRel * nZ;
-        vx[j] += vRel * nX; vy[j] += vRel * nY; vz[j] += vRel * nZ;
+    for (int b = a + 1; b < t1; b++)
+      if (!vapor_pair(P, i, member(gk, xk, b), rad, rad2, spin)) break;
+  }
+}
+
+// ---- large vapor groups: the same sweep, run in dependency order.
+// Parcel a's step of the sweep reads and writes a and its later partners,
+// the parcels after it in x order within twice the contact radius (the
+// group threshold); the others it only passes over. Two steps that share a
+// parcel must run in sweep order; steps that share none commute. Each
+// parcel x keeps the sweep-ordered list of the steps that touch it (its
+// own last), and a counter of how many of them are done; step a may run
+// when, for each parcel it touches, the counter equals a's position in that
+// parcel's list. One workgroup sweeps one group (the steps of a group only
+// touch its members), each thread taking every blockDim-th step in order
+// and polling for readiness a bounded number of times; the earliest pending
+// step is always ready, so every poll round completes at least one step.
+constexpr int kVcK = 32;  // partners kept per parcel (more: the group is swept serially)
+
+// later partners of each parcel: same group, later x rank, closer than thr
+// at the start of the pass, sorted by rank
+__global__ void k_vc_cand(Geo g, int n, const unsigned *__restrict__ list, const float *__restrict__ px,
+                          const float *__restrict__ py, const float *__restrict__ pz,
+                          const unsigned *__restrict__ cellOf, const unsigned *__restrict__ start,
+                          const unsigned *__restrict__ binned, int span, const unsigned *__restrict__ rankOf,
+                          const unsigned long long *__restrict__ key, float thr2, unsigned *__restrict__ cand,
+                          unsigned *__restrict__ ncand, unsigned *__restrict__ serialOnly) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  unsigned p = list[e], re = rankOf[e], root = (unsigned)(key[e] >> 32);
+  float x = px[p], y = py[p], z = pz[p];
+  int c = cellOf[e];
+  int ci = c % g.nx, cj = (c / g.nx) % g.ny, ck = c / (g.nx * g.ny);
+  unsigned *mine = cand + (long)e * kVcK;
+  unsigned rk[kVcK];
+  int m = 0;
+  bool over = false;
+  for (int dk = -span; dk <= span; dk++)
+    for (int dj = -span; dj <= span; dj++)
+      for (int di = -span; di <= span; di++) {
+        int ii = ci + di, jj = cj + dj, kk = ck + dk;
+        if (ii < 0 || jj < 0 || kk < 0 || ii >= g.nx || jj >= g.ny || kk >= g.nz) continue;
+        int cc = (kk * g.ny + jj) * g.nx + ii;
+        for (unsigned b = start[cc]; b < start[cc + 1]; b++) {
+          unsigned o = binned[b];
+          unsigned ro = rankOf[o];
+          if (ro <= re || (unsigned)(key[o] >> 32) != root) continue;
+          unsigned q = list[o];
+          float ddx = px[q] - x, ddy = py[q] - y, ddz = pz[q] - z;
+          if (ddx * ddx + ddy * ddy + ddz * ddz >= thr2) continue;
+          if (m == kVcK) { over = true; continue; }
+          int k = m++;
+          while (k > 0 && rk[k - 1] > ro) { rk[k] = rk[k - 1]; mine[k] = mine[k - 1]; k--; }
+          rk[k] = ro;
+          mine[k] = o;
+        }
       }
-      float half = (rad - d) * 0.5f;
-      px[i] -= nX * half; py[i] -= nY * half; pz[i] -= nZ * half;
-      px[j] += nX * half; py[j] += nY * half; pz[j] += nZ * half;
-      float kick = (rad - d) * 2.f;
-      vx[i] -= nX * kick; vy[i] -= nY * kick; vz[i] -= nZ * kick;
-      vx[j] += nX * kick; vy[j] += nY * kick; vz[j] += nZ * kick;
-      if (!spin) continue;
-      float R = rad * 0.5f;
-      float wSx = wx[i] + wx[j], wSy = wy[i] + wy[j], wSz = wz[i] + wz[j];
-      float usx = (vx[i] - vx[j]) + R * (wSy * nZ - wSz * nY);
-      float usy = (vy[i] - vy[j]) + R * (wSz * nX - wSx * nZ);
-      float usz = (vz[i] - vz[j]) + R * (wSx * nY - wSy * nX);
-      float uN = usx * nX + usy * nY + usz * nZ;
-      float utx = usx - uN * nX, uty = usy - uN * nY, utz = usz - uN * nZ;
-      float utM = sqrtf(utx * utx + uty * uty + utz * utz);
-      if (utM < 1e-9f) continue;
-      float jt = utM / 7.f;
-      if (vRel > 0.f) { float mu = 0.25f * vRel; if (jt > mu) jt = mu; }
-      float jx = -utx / utM * jt, jy = -uty / utM * jt, jz = -utz / utM * jt;
-      vx[i] += jx; vy[i] += jy; vz[i] += jz;
-      vx[j] -= jx; vy[j] -= jy; vz[j] -= jz;
-      float invI = 1.f / (0.4f * R * R);
-      float dwx = R * (nY * jz - nZ * jy) * invI, dwy = R * (nZ * jx - nX * jz) * invI;
-      float dwz = R * (nX * jy - nY * jx) * invI;
-      wx[i] += dwx; wy[i] += dwy; wz[i] += dwz;
-      wx[j] += dwx; wy[j] += dwy; wz[j] += dwz;
+  ncand[e] = (unsigned)m;
+  if (over) __hip_atomic_store(&serialOnly[root], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_vc_pair_count(int n, const unsigned *__restrict__ ncand, unsigned *__restrict__ cnt) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) cnt[e] = 1 + ncand[e];
+  if (e == n) cnt[e] = 0;
+}
+
+// (touched parcel, step rank, slot): slot 0 is the step's own parcel,
+// slot k its k-th partner
+__global__ void k_vc_pair_emit(int n, const unsigned *__restrict__ rankOf, const unsigned *__restrict__ cand,
+                               const unsigned *__restrict__ ncand, const unsigned *__restrict__ off,
+                               unsigned long long *__restrict__ keys) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  unsigned long long r = (unsigned long long)rankOf[e] << 6;
+  unsigned long long *o = keys + off[e];
+  o[0] = ((unsigned long long)e << 32) | r;
+  for (unsigned k = 0; k < ncand[e]; k++) o[1 + k] = ((unsigned long long)cand[(long)e * kVcK + k] << 32) | r | (k + 1);
+}
+
+__global__ void k_vc_chain_heads(int np, const unsigned long long *__restrict__ keys, unsigned *__restrict__ chainStart) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= np) return;
+  unsigned x = (unsigned)(keys[s] >> 32);
+  if (s == 0 || (unsigned)(keys[s - 1] >> 32) != x) chainStart[x] = (unsigned)s;
+}
+
+// the position of each step in the list of every parcel it touches
+__global__ void k_vc_need(int np, const unsigned long long *__restrict__ keys, const unsigned *__restrict__ chainStart,
+                          const unsigned long long *__restrict__ xk, const unsigned *__restrict__ posOf,
+                          unsigned *__restrict__ need) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= np) return;
+  unsigned long long k = keys[s];
+  unsigned x = (unsigned)(k >> 32), rank = (unsigned)((k >> 6) & 0x3ffffffull), slot = (unsigned)(k & 63ull);
+  unsigned e = posOf[(unsigned)(xk[rank] & 0xffffffffull)];
+  need[(long)e * (kVcK + 1) + slot] = (unsigned)s - chainStart[x];
+}
+
+// the groups for the dependency-ordered sweep, as [t0, t1) ranges of the
+// sorted group keys
+__global__ void k_vc_big(int n, const unsigned long long *__restrict__ gk, int serialMax,
+                         const unsigned *__restrict__ serialOnly, unsigned *__restrict__ big,
+                         unsigned *__restrict__ nBig) {
+  int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t0 >= n) return;
+  if (t0 > 0 && (gk[t0] >> 32) == (gk[t0 - 1] >> 32)) return;
+  int t1 = t0 + 1;
+  while (t1 < n && (gk[t1] >> 32) == (gk[t0] >> 32)) t1++;
+  if (t1 - t0 <= serialMax || serialOnly[gk[t0] >> 32]) return;
+  unsigned slot = atomicAdd(nBig, 1u);
+  big[2 * slot] = (unsigned)t0;
+  big[2 * slot + 1] = (unsigned)t1;
+}
+
+__device__ __forceinline__ unsigned ld_wg(const unsigned *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__global__ __launch_bounds__(256) void k_vc_sweep(const unsigned *__restrict__ big, const unsigned *__restrict__ nBig,
+                                                  const unsigned long long *__restrict__ gk,
+                                                  const unsigned long long *__restrict__ xk,
+                                                  const unsigned *__restrict__ posOf, const unsigned *__restrict__ list,
+                                                  const unsigned *__restrict__ cand, const unsigned *__restrict__ ncand,
+                                                  const unsigned *__restrict__ need, unsigned *__restrict__ done,
+                                                  float rad, int spin, VaporState P, unsigned *__restrict__ unfinished) {
+  const float rad2 = rad * rad;
+  unsigned nb = *nBig;
+  for (unsigned gi = blockIdx.x; gi < nb; gi += gridDim.x) {
+    int t0 = (int)big[2 * gi], t1 = (int)big[2 * gi + 1];
+    int cur = t0 + (int)threadIdx.x;
+    long limit = 64L * (t1 - t0) + 4096;
+    for (long it = 0; cur < t1 && it < limit; it++) {
+      unsigned r = (unsigned)(gk[cur] & 0xffffffffull);
+      unsigned pi = (unsigned)(xk[r] & 0xffffffffull);
+      unsigned e = posOf[pi];
+      unsigned nc = ncand[e];
+      const unsigned *ne = need + (long)e * (kVcK + 1);
+      const unsigned *ce = cand + (long)e * kVcK;
+      bool ready = ld_wg(&done[e]) == ne[0];
+      for (unsigned k = 0; ready && k < nc; k++) ready = ld_wg(&done[ce[k]]) == ne[1 + k];
+      if (!ready) {
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      for (unsigned k = 0; k < nc; k++)
+        if (!vapor_pair(P, pi, list[ce[k]], rad, rad2, spin)) break;
+      __hip_atomic_store(&done[e], ne[0] + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      for (unsigned k = 0; k < nc; k++)
+        __hip_atomic_store(&done[ce[k]], ne[1 + k] + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      cur += blockDim.x;
     }
+    if (cur < t1) atomicAdd(unfinished, 1u);
+    __syncthreads();
   }
 }
 
@@ -435,6 +600,13 @@ struct Contacts {
   unsigned long long *xk = nullptr, *xk2 = nullptr, *gk = nullptr, *gk2 = nullptr;
   void *tmp = nullptr;
   size_t tmpBytes = 0;
+  // dependency-ordered sweep of large vapor groups, sized by the vapor count
+  long vcap = 0;
+  unsigned *cand = nullptr, *ncand = nullptr, *pairOff = nullptr, *need = nullptr, *done = nullptr,
+           *serialOnly = nullptr, *big = nullptr, *vcCounters = nullptr, *chainStart = nullptr;
+  unsigned long long *pairKeys = nullptr, *pairKeys2 = nullptr;
+  void *vtmp = nullptr;
+  size_t vtmpBytes = 0;
 };
 
 namespace {
@@ -521,6 +693,34 @@ int group_subset(ws_ctx *c, int n, const unsigned *list, float thr, const unsign
   return WS_OK;
 }
 
+// scratch of the dependency-ordered vapor sweep for nV parcels (grow-only)
+int vapor_ensure(ws_ctx *c, long nV) {
+  Contacts *k = c->contacts;
+  if (k->vcap >= nV) return WS_OK;
+  long n = nV + nV / 4 + 1024;
+  void *old[] = {k->cand, k->ncand, k->pairOff, k->need, k->done, k->serialOnly, k->big, k->vcCounters,
+                 k->chainStart, k->pairKeys, k->pairKeys2, k->vtmp};
+  for (void *p : old)
+    if (p) (void)hipFree(p);
+  k->vcap = 0;
+  long np = n * (kVcK + 1);
+  int r;
+  if ((r = calloc_dev(c, &k->cand, n * kVcK)) || (r = calloc_dev(c, &k->ncand, n + 1)) ||
+      (r = calloc_dev(c, &k->pairOff, n + 1)) || (r = calloc_dev(c, &k->need, np)) ||
+      (r = calloc_dev(c, &k->done, n)) || (r = calloc_dev(c, &k->serialOnly, n)) ||
+      (r = calloc_dev(c, &k->big, 2 * n)) || (r = calloc_dev(c, &k->vcCounters, 4)) ||
+      (r = calloc_dev(c, &k->chainStart, n)) || (r = calloc_dev(c, &k->pairKeys, np)) ||
+      (r = calloc_dev(c, &k->pairKeys2, np)))
+    return r;
+  size_t a = 0, b = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, a, k->ncand, k->pairOff, (int)(n + 1), c->stream);
+  hipcub::DeviceRadixSort::SortKeys(nullptr, b, k->pairKeys2, k->pairKeys, (int)np, 0, 64, c->stream);
+  k->vtmpBytes = a > b ? a : b;
+  if ((r = calloc_dev(c, (char **)&k->vtmp, k->vtmpBytes))) return r;
+  k->vcap = n;
+  return WS_OK;
+}
+
 int subset(ws_ctx *c, unsigned *flag, unsigned *off, unsigned *list, int *count) {
   Contacts *k = c->contacts;
   int n = c->nP;
@@ -550,7 +750,9 @@ void contacts_release(ws_ctx *c, bool all) {
   if (!k) return;
   void *ptrs[] = {k->fv, k->fi, k->fl, k->ov, k->oi, k->ol, k->vlist, k->ilist, k->llist, k->cellOf, k->rankIn,
                   k->binned, k->parent, k->posOf, k->rankOf, k->cnt, k->start, k->lstart, k->lbinned, k->lcell,
-                  k->lrank, k->counts, k->xk, k->xk2, k->gk, k->gk2, k->tmp};
+                  k->lrank, k->counts, k->xk, k->xk2, k->gk, k->gk2, k->tmp, k->cand, k->ncand, k->pairOff,
+                  k->need, k->done, k->serialOnly, k->big, k->vcCounters, k->chainStart, k->pairKeys,
+                  k->pairKeys2, k->vtmp};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (all) {
@@ -579,9 +781,65 @@ int frame_vapor_collisions(ws_ctx *c) {
   if (nV >= 2) {
     float rad = (float)(c->spacing * 0.6);
     if ((r = group_subset(c, nV, k->vlist, 2.f * rad, nullptr))) return r;
-    hipLaunchKernelGGL(k_ct_vapor_groups, dim3(blocks_for(nV, 256)), dim3(256), 0, c->stream, nV, k->gk, k->xk, rad,
-                       c->spinOn, c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pWx, c->pWy, c->pWz);
+    if ((r = vapor_ensure(c, nV))) return r;
+    VaporState P{c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pWx, c->pWy, c->pWz};
+    int nb = blocks_for(nV, 256);
+    int serialMax = c->vaporSerialMax;
+    // groups too large for one thread's serial sweep
+    hipLaunchKernelGGL(k_ct_zero, dim3(blocks_for(nV, 256)), dim3(256), 0, c->stream, (long)nV, k->serialOnly);
+    hipLaunchKernelGGL(k_ct_zero, dim3(1), dim3(64), 0, c->stream, 4L, k->vcCounters);
+    hipLaunchKernelGGL(k_vc_big, dim3(nb), dim3(256), 0, c->stream, nV, k->gk, serialMax, k->serialOnly, k->big,
+                       k->vcCounters);
     WS_LAUNCHED();
+    unsigned nBig = 0;
+    WS_HIP(hipMemcpyAsync(&nBig, k->vcCounters, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    WS_HIP(hipStreamSynchronize(c->stream));
+    if (nBig > 0) {
+      // partners, the steps touching each parcel in sweep order, positions
+      hipLaunchKernelGGL(k_vc_cand, dim3(nb), dim3(256), 0, c->stream, c->geo, nV, k->vlist, c->px, c->py, c->pz,
+                         k->cellOf, k->start, k->binned, cell_span(c, 2.0 * rad), k->rankOf, k->gk2,
+                         4.f * rad * rad, k->cand, k->ncand, k->serialOnly);
+      hipLaunchKernelGGL(k_vc_pair_count, dim3(blocks_for(nV + 1, 256)), dim3(256), 0, c->stream, nV, k->ncand,
+                         k->pairOff);
+      WS_LAUNCHED();
+      WS_HIP(hipcub::DeviceScan::ExclusiveSum(k->vtmp, k->vtmpBytes, k->pairOff, k->pairOff, nV + 1, c->stream));
+      unsigned np = 0;
+      WS_HIP(hipMemcpyAsync(&np, k->pairOff + nV, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+      hipLaunchKernelGGL(k_vc_pair_emit, dim3(nb), dim3(256), 0, c->stream, nV, k->rankOf, k->cand, k->ncand,
+                         k->pairOff, k->pairKeys2);
+      WS_LAUNCHED();
+      WS_HIP(hipStreamSynchronize(c->stream));
+      WS_HIP(hipcub::DeviceRadixSort::SortKeys(k->vtmp, k->vtmpBytes, k->pairKeys2, k->pairKeys, (int)np, 0, 64,
+                                               c->stream));
+      hipLaunchKernelGGL(k_vc_chain_heads, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, (int)np, k->pairKeys,
+                         k->chainStart);
+      hipLaunchKernelGGL(k_vc_need, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, (int)np, k->pairKeys,
+                         k->chainStart, k->xk, k->posOf, k->need);
+      hipLaunchKernelGGL(k_ct_zero, dim3(nb), dim3(256), 0, c->stream, (long)nV, k->done);
+      // the large groups again, now without those marked for the serial sweep
+      hipLaunchKernelGGL(k_ct_zero, dim3(1), dim3(64), 0, c->stream, 4L, k->vcCounters);
+      hipLaunchKernelGGL(k_vc_big, dim3(nb), dim3(256), 0, c->stream, nV, k->gk, serialMax, k->serialOnly, k->big,
+                         k->vcCounters);
+      WS_LAUNCHED();
+    }
+    hipLaunchKernelGGL(k_ct_vapor_groups, dim3(nb), dim3(256), 0, c->stream, nV, k->gk, k->xk, rad, c->spinOn, P,
+                       serialMax, k->serialOnly);
+    if (nBig > 0) {
+      int grid = (int)(nBig < 2048 ? nBig : 2048);
+      hipLaunchKernelGGL(k_vc_sweep, dim3(grid), dim3(256), 0, c->stream, k->big, k->vcCounters, k->gk, k->xk,
+                         k->posOf, k->vlist, k->cand, k->ncand, k->need, k->done, rad, c->spinOn, P,
+                         k->vcCounters + 1);
+    }
+    WS_LAUNCHED();
+    if (nBig > 0) {
+      unsigned unfinished = 0;
+      WS_HIP(hipMemcpyAsync(&unfinished, k->vcCounters + 1, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+      WS_HIP(hipStreamSynchronize(c->stream));
+      if (unfinished) {
+        c->err = "vapor contact sweep: a group did not finish within its step bound";
+        return WS_ESTATE;
+      }
+    }
     // liquid partners, binned by cell (they do not move in this pass)
     int nL = 0;
     if ((r = subset(c, k->fl, k->ol, k->llist, &nL))) return r;

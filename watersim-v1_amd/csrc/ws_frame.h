@@ -19,5 +19,9 @@ int balls_raster(ws_ctx *c);
 int balls_push(ws_ctx *c, bool rebin);
 int balls_update(ws_ctx *c);
 void balls_release(ws_ctx *c);
+// marching tetrahedra over a device corner field (ws_api_surface.hip)
+int mt_build_dev(ws_ctx *c, const float *dd, int nx, int ny, int nz, double dx, double iso, float *pos, float *nrm,
+                 long cap, long *count, double *volume);
+void metaballs_release(ws_ctx *c);
 World make_world(ws_ctx *c);
 }  // namespace ws

@@ -112,7 +112,9 @@ __global__ void k_mark(Geo g, World w, int nP, const float *__restrict__ px, con
   int k = clampi(cell_floor(z, g.inv_dx, w.thr, g.nz), 0, g.nz - 1);
   if (j >= g.ny - 2 && ceiling_demote(w, g, x, y, z, &s->df_refined)) { pflag[p] = 1; pcell[p] = -1; return; }
   int c = (k * g.ny + j) * g.nx + i;
-  if (w.st[c] == AIR) ct[c] = FLUID;  // idempotent byte store
+  // AIR -> FLUID on the current types (static rock and ball cells stay
+  // SOLID); concurrent markers of one cell all store the same byte
+  if (ct[c] == AIR) ct[c] = FLUID;
   pcell[p] = c;
   prank[p] = atomicAdd(&cnt[c], 1u);
 }

@@ -119,6 +119,7 @@ struct Ctx {
   // frame-end contact passes: scratch, the vapor census, candidate overflows
   Contacts *contacts = nullptr;
   int vaporCount = 0, nightVaporCount = 0;
+  int vaporSerialMax = 48;  // larger vapor groups run the dependency-ordered sweep
   long contactOverflow = 0;
   // both solves per pressure stage, timed separately (measurement)
   int pressureCompare = 0;
@@ -182,6 +183,12 @@ struct Ctx {
   unsigned *mtCount = nullptr, *mtOff = nullptr;
   float *mtPos = nullptr, *mtNrm = nullptr, *mtDens = nullptr;
   long mtCells = 0, mtCap = 0, mtDensN = 0;
+  // metaball field (its own corner lattice of mbM cells a side) and its
+  // fixed-point accumulator
+  float *mbField = nullptr;
+  unsigned long long *mbAcc = nullptr;
+  long mbCap = 0;
+  int mbM = 0;
 
   float *u() const { return vel; }
   float *v() const { return vel + uN; }

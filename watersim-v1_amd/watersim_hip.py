@@ -26,7 +26,7 @@ EXPORTS = ["ws_create", "ws_destroy", "ws_last_error", "ws_set_f", "ws_get_f", "
            "ws_substep", "ws_stage", "ws_step_frame", "ws_run_substeps", "ws_splat_density", "ws_mt_build",
            "ws_autotune_iso", "ws_kinetic_energy", "ws_sync", "ws_world_build", "ws_world_free",
            "ws_net_param_count", "ws_net_set_params", "ws_net_get_params", "ws_net_forward", "ws_net_grad",
-           "ws_net_bank", "ws_net_bank_count", "ws_net_train", "ws_set_balls", "ws_get_balls"]
+           "ws_net_bank", "ws_net_bank_count", "ws_net_train", "ws_set_balls", "ws_get_balls", "ws_metaballs_build"]
 
 
 class WorldParams(ctypes.Structure):
@@ -91,6 +91,8 @@ def load(path=LIB_PATH):
     L.ws_net_bank.argtypes = [vp, ci]
     L.ws_net_bank_count.argtypes = [vp, ctypes.POINTER(ci)]
     L.ws_net_train.argtypes = [vp, ci, ci, ci, d, ctypes.c_uint, ctypes.POINTER(d)]
+    L.ws_metaballs_build.argtypes = [vp, d, d, vp, vp, ctypes.c_long, ctypes.POINTER(ctypes.c_long),
+                                     ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(ci)]
     L.ws_set_balls.argtypes = [vp, ci, vp]
     L.ws_get_balls.argtypes = [vp, ci, vp]
     for f in EXPORTS:
@@ -179,6 +181,8 @@ class Solver:
             return 25 * int(self.get("nBalls")), np.uint32
         if name == "ballSurf":
             return int(self.get("nBalls")), np.float32
+        if name == "metaballField":
+            return (int(self.get("mbM")) + 1) ** 3, np.float32
         raise KeyError(name)
 
     def upload(self, name, arr):
@@ -284,6 +288,21 @@ class Solver:
 
     def splat_density(self):
         self._check(self.L.ws_splat_density(self.ctx), "splat_density")
+
+    def metaballs_build(self, tension, detail=float("nan")):
+        """Metaballs.build(solver, tension, detail): (count, volume, pos, nrm,
+        iso, lattice cells)."""
+        cnt, vol = ctypes.c_long(), ctypes.c_double()
+        iso, m = ctypes.c_double(), ctypes.c_int()
+        self._check(self.L.ws_metaballs_build(self.ctx, tension, detail, None, None, 0, ctypes.byref(cnt),
+                                              ctypes.byref(vol), ctypes.byref(iso), ctypes.byref(m)), "metaballs")
+        n = cnt.value
+        pos = np.zeros(max(n, 1) * 3, np.float32)
+        nrm = np.zeros(max(n, 1) * 3, np.float32)
+        self._check(self.L.ws_metaballs_build(self.ctx, tension, detail, pos.ctypes.data_as(ctypes.c_void_p),
+                                              nrm.ctypes.data_as(ctypes.c_void_p), n, ctypes.byref(cnt),
+                                              ctypes.byref(vol), ctypes.byref(iso), ctypes.byref(m)), "metaballs")
+        return cnt.value, vol.value, pos[:n * 3], nrm[:n * 3], iso.value, m.value
 
     def mt_build(self, dens=None, dims=None, dx=None, iso=1.8, want_mesh=True):
         """MarchingTetrahedra.build on the context field (dens None) or a host
