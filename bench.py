@@ -288,6 +288,13 @@ def main():
     b_plain, b_iter = algorithmic_bytes(cfg, n, nP, g["sorCellSweeps"], g["residualsTotal"], g["ticksTotal"])
     b_total = b_plain * steps + b_iter
     ke = s.kinetic_energy()
+    # outside the timed region: one render readback of the fields a renderer
+    # reads each frame (PCIe-inclusive; never part of `value`)
+    s.read_render()
+    t1 = time.perf_counter()
+    for _ in range(3):
+        s.read_render()
+    readback_ms = 1e3 * (time.perf_counter() - t1) / 3
     log("substep %.3f ms, pressure %.3f ms/solve, %.1f sweeps/substep, %d ticks, SOR launch %.2f us, ke %.4g"
         % (1e3 * el / steps, g["pressureMs"] / max(g["pressureSolves"], 1), g["sweepsTotal"] / steps,
            g["ticksTotal"], 1e6 * sor_avg_s, ke))
@@ -324,7 +331,8 @@ def main():
                    "residual_checks_per_substep": g["residualsTotal"] / steps,
                    "thermal_ticks": int(g["ticksTotal"]),
                    "substep_algorithmic_GBps": b_total / el / 1e9,
-                   "substep_hbm_frac": b_total / el / HBM_PEAK},
+                   "substep_hbm_frac": b_total / el / HBM_PEAK,
+                   "render_readback_ms": readback_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / (HBM_PEAK / 1e9), "traffic": traffic,
                      "kernel": kname + kdesc,

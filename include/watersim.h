@@ -153,6 +153,26 @@ int ws_kinetic_energy(ws_ctx *c, double *ke);
 int ws_set_balls(ws_ctx *c, int n, const double *state);
 int ws_get_balls(ws_ctx *c, int n, double *state);
 
+/* Render readback: the particle fields a renderer reads every frame
+ * (scene.updateParticles, js/scene.js:1595-1700: px, py, pz, pflag, pT,
+ * pLight, pDepth, pPh and, for the spin colours, pWx, pWy, pWz). The fields
+ * picked by `mask` (WS_RF_* bits) are copied with back-to-back DMA
+ * transfers into pinned host memory owned by the context, with one wait;
+ * `view` receives pointers to them (NULL for fields not picked) and the
+ * particle count. The memory stays valid until the next ws_read_render or
+ * ws_destroy on the context; callers read it in place (no further copy). */
+enum {
+  WS_RF_PX = 1 << 0, WS_RF_PY = 1 << 1, WS_RF_PZ = 1 << 2, WS_RF_PFLAG = 1 << 3, WS_RF_PT = 1 << 4,
+  WS_RF_PLIGHT = 1 << 5, WS_RF_PDEPTH = 1 << 6, WS_RF_PPH = 1 << 7, WS_RF_PWX = 1 << 8, WS_RF_PWY = 1 << 9,
+  WS_RF_PWZ = 1 << 10, WS_RF_ALL = (1 << 11) - 1
+};
+typedef struct {
+  int nP;
+  const float *px, *py, *pz, *pT, *pLight, *pPh, *pWx, *pWy, *pWz;
+  const uint8_t *pflag, *pDepth;
+} ws_render_view;
+int ws_read_render(ws_ctx *c, unsigned mask, ws_render_view *view);
+
 /* Waits for all work queued on the context's stream. */
 int ws_sync(ws_ctx *c);
 

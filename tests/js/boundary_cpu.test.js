@@ -20,7 +20,7 @@ function throws(fn, re) {
 var addon = require(path.join(root, '_build', 'watersim_napi.node'));
 var want = ['create', 'destroy', 'setF', 'getF', 'upload', 'download', 'substep', 'stage', 'stepFrame',
   'runSubsteps', 'splatDensity', 'mtBuild', 'autotuneIso', 'kineticEnergy', 'sync', 'worldBuild', 'setBalls',
-  'getBalls'];
+  'getBalls', 'metaballs', 'readRender'];
 check('addon exports the C ABI surface', want.every(function (k) { return typeof addon[k] === 'function'; }),
   want.filter(function (k) { return typeof addon[k] !== 'function'; }).join(','));
 

@@ -265,6 +265,12 @@ if (on('balls')) {
   }));
   check('water stays out of the rock, nothing escapes, particles finite', stB.inRock === 0 && stB.out === 0 && allFinite(sb),
     'inRock ' + stB.inRock + ' out ' + stB.out);
+  var rr = sb.readRender(), same = rr.nP === sb.nP;
+  ['px', 'py', 'pz', 'pflag', 'pT', 'pLight', 'pDepth', 'pPh'].forEach(function (k) {
+    var m = sb[k];
+    for (var q = 0; same && q < sb.nP; q += 97) same = rr[k][q] === m[q];
+  });
+  check('the batched render readback equals the field mirrors', same);
   sb.dispose();
 }
 

@@ -275,6 +275,41 @@ static napi_value js_mt_build(napi_env env, napi_callback_info info) {
   return o;
 }
 
+/* readRender(h, mask) -> {nP, px, py, ...}: typed arrays over the
+ * context's pinned readback memory (no copy; valid until the next call) */
+static napi_value render_array(napi_env env, const void *p, size_t n, size_t elem) {
+  napi_value ab, arr;
+  if (napi_create_external_arraybuffer(env, (void *)p, n * elem, NULL, NULL, &ab) != napi_ok) return NULL;
+  if (napi_create_typedarray(env, elem == 4 ? napi_float32_array : napi_uint8_array, n, ab, 0, &arr) != napi_ok)
+    return NULL;
+  return arr;
+}
+
+static napi_value js_read_render(napi_env env, napi_callback_info info) {
+  napi_value a[2];
+  if (!get_args(env, info, 2, a)) return NULL;
+  ws_ctx *c = get_ctx(env, a[0]);
+  double m;
+  if (!c || !get_double(env, a[1], &m)) return NULL;
+  ws_render_view v;
+  int rc = ws_read_render(c, (unsigned)m, &v);
+  if (rc) return fail(env, c, rc, "ws_read_render");
+  napi_value o;
+  NAPI_OK(napi_create_object(env, &o));
+  NAPI_OK(napi_set_named_property(env, o, "nP", num(env, (double)v.nP)));
+  const struct { const char *name; const void *p; size_t elem; } f[] = {
+      {"px", v.px, 4}, {"py", v.py, 4}, {"pz", v.pz, 4}, {"pflag", v.pflag, 1}, {"pT", v.pT, 4},
+      {"pLight", v.pLight, 4}, {"pDepth", v.pDepth, 1}, {"pPh", v.pPh, 4}, {"pWx", v.pWx, 4},
+      {"pWy", v.pWy, 4}, {"pWz", v.pWz, 4}};
+  for (size_t i = 0; i < sizeof f / sizeof f[0]; i++) {
+    if (!f[i].p || v.nP <= 0) continue;
+    napi_value arr = render_array(env, f[i].p, (size_t)v.nP, f[i].elem);
+    if (!arr) return NULL;
+    NAPI_OK(napi_set_named_property(env, o, f[i].name, arr));
+  }
+  return o;
+}
+
 /* metaballs(h, tension, detail, pos, nrm) -> {count, volume, iso, lattice} */
 static napi_value js_metaballs(napi_env env, napi_callback_info info) {
   napi_value a[5];
@@ -430,6 +465,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"kineticEnergy", 0, js_kinetic, 0, 0, 0, napi_default, 0},
       {"sync", 0, js_sync, 0, 0, 0, napi_default, 0},
       {"metaballs", 0, js_metaballs, 0, 0, 0, napi_default, 0},
+      {"readRender", 0, js_read_render, 0, 0, 0, napi_default, 0},
       {"setBalls", 0, js_set_balls, 0, 0, 0, napi_default, 0},
       {"getBalls", 0, js_get_balls, 0, 0, 0, napi_default, 0},
       {"worldBuild", 0, js_world_build, 0, 0, 0, napi_default, 0},

@@ -735,6 +735,7 @@ void ws_destroy(ws_ctx *c) {
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->hs) hipHostFree(c->hs);
+  if (c->renderHost) hipHostFree(c->renderHost);
   for (auto &e : c->ev)
     if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -989,6 +990,46 @@ int ws_stage(ws_ctx *c, int stage, double dt) {
   if (r) return r;
   WS_HIP(hipStreamSynchronize(c->stream));
   return pull_scalars(c);
+}
+
+int ws_read_render(ws_ctx *c, unsigned mask, ws_render_view *view) {
+  if (!c || !view) return WS_EINVAL;
+  struct Src {
+    unsigned bit;
+    const void *dev;
+    size_t elem;
+    const void **dst;
+  };
+  const Src src[] = {
+      {WS_RF_PX, c->px, 4, (const void **)&view->px},       {WS_RF_PY, c->py, 4, (const void **)&view->py},
+      {WS_RF_PZ, c->pz, 4, (const void **)&view->pz},       {WS_RF_PFLAG, c->pflag, 1, (const void **)&view->pflag},
+      {WS_RF_PT, c->pT, 4, (const void **)&view->pT},       {WS_RF_PLIGHT, c->pLight, 4, (const void **)&view->pLight},
+      {WS_RF_PDEPTH, c->pDepth, 1, (const void **)&view->pDepth}, {WS_RF_PPH, c->pPh, 4, (const void **)&view->pPh},
+      {WS_RF_PWX, c->pWx, 4, (const void **)&view->pWx},    {WS_RF_PWY, c->pWy, 4, (const void **)&view->pWy},
+      {WS_RF_PWZ, c->pWz, 4, (const void **)&view->pWz}};
+  size_t n = (size_t)(c->nP > 0 ? c->nP : 0), need = 0;
+  for (const Src &f : src)
+    if (mask & f.bit) need += (f.elem * n + 255) & ~(size_t)255;
+  if (need > c->renderCap) {
+    if (c->renderHost) (void)hipHostFree(c->renderHost);
+    c->renderHost = nullptr;
+    c->renderCap = 0;
+    size_t cap = need + need / 4;
+    WS_HIP(hipHostMalloc((void **)&c->renderHost, cap, 0));
+    c->renderCap = cap;
+  }
+  size_t off = 0;
+  view->nP = (int)n;
+  for (const Src &f : src) {
+    *f.dst = nullptr;
+    if (!(mask & f.bit)) continue;
+    if (!f.dev) { c->err = "render readback: field not allocated"; return WS_ESTATE; }
+    if (n) WS_HIP(hipMemcpyAsync(c->renderHost + off, f.dev, f.elem * n, hipMemcpyDeviceToHost, c->stream));
+    *f.dst = c->renderHost + off;
+    off += (f.elem * n + 255) & ~(size_t)255;
+  }
+  WS_HIP(hipStreamSynchronize(c->stream));
+  return WS_OK;
 }
 
 int ws_sync(ws_ctx *c) {

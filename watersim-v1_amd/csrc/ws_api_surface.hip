@@ -88,7 +88,7 @@ int mt_build_dev(ws_ctx *c, const float *dd, int nx, int ny, int nz, double dx, 
   c->mtCells = have;
   if ((r = ensure_scan(c, nC + 1))) return r;
   hipLaunchKernelGGL(k_zero_counts, dim3(blocks_for(nC + 1, 256)), dim3(256), 0, c->stream, c->mtCount, nC + 1);
-  hipLaunchKernelGGL(k_mt_count, dim3(blocks_for(nC, 256)), dim3(256), 0, c->stream, nx, ny, nz, dd, (float)iso,
+  hipLaunchKernelGGL(k_mt_count, dim3(blocks_for(nC, 256)), dim3(256), 0, c->stream, nx, ny, nz, dd, iso,
                      c->mtCount);
   WS_LAUNCHED();
   WS_HIP(hipcub::DeviceScan::ExclusiveSum(c->scanTmp, c->scanTmpBytes, c->mtCount, c->mtOff, (int)(nC + 1),
@@ -109,7 +109,7 @@ int mt_build_dev(ws_ctx *c, const float *dd, int nx, int ny, int nz, double dx, 
   int nb = blocks_for(nC, 256);
   double *vpart = nullptr;
   WS_HIP(hipMallocAsync((void **)&vpart, sizeof(double) * (nb + 1), c->stream));
-  hipLaunchKernelGGL(k_mt_emit, dim3(nb), dim3(256), 0, c->stream, nx, ny, nz, (float)dx, dd, (float)iso, c->mtOff,
+  hipLaunchKernelGGL(k_mt_emit, dim3(nb), dim3(256), 0, c->stream, nx, ny, nz, dx, dd, iso, c->mtOff,
                      need ? c->mtPos : nullptr, need ? c->mtNrm : nullptr, need, vpart);
   hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(256), 0, c->stream, vpart, nb, vpart + nb);
   WS_LAUNCHED();
@@ -186,8 +186,8 @@ int ws_autotune_iso(ws_ctx *c, double *iso) {
   for (int it = 0; it < 8; it++) {
     double mid = 0.5 * (lo + hi);
     hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, c->stream, cnt, 1L);
-    hipLaunchKernelGGL(k_mt_volume, dim3(nb), dim3(256), 0, c->stream, c->nx, c->ny, c->nz, (float)c->dx, c->dens,
-                       (float)mid, vpart, cnt);
+    hipLaunchKernelGGL(k_mt_volume, dim3(nb), dim3(256), 0, c->stream, c->nx, c->ny, c->nz, c->dx, c->dens,
+                       mid, vpart, cnt);
     hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(256), 0, c->stream, vpart, nb, vpart + nb);
     WS_LAUNCHED();
     double vol = 0;

@@ -183,6 +183,9 @@ struct Ctx {
   unsigned *mtCount = nullptr, *mtOff = nullptr;
   float *mtPos = nullptr, *mtNrm = nullptr, *mtDens = nullptr;
   long mtCells = 0, mtCap = 0, mtDensN = 0;
+  // pinned host memory of the render readback
+  uint8_t *renderHost = nullptr;
+  size_t renderCap = 0;
   // metaball field (its own corner lattice of mbM cells a side) and its
   // fixed-point accumulator
   float *mbField = nullptr;

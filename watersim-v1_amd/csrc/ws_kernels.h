@@ -104,10 +104,10 @@ __global__ void k_rng_advance(uint32_t *rng, int which, const unsigned *need, co
 // surface
 __global__ void k_splat_density(Geo g, World w, int nP, const unsigned *start, const unsigned *sorted,
                                 const float *px, const float *py, const float *pz, float *dens, float r2);
-__global__ void k_mt_count(int nx, int ny, int nz, const float *dens, float iso, unsigned *cnt);
-__global__ void k_mt_emit(int nx, int ny, int nz, float dx, const float *dens, float iso, const unsigned *off,
+__global__ void k_mt_count(int nx, int ny, int nz, const float *dens, double iso, unsigned *cnt);
+__global__ void k_mt_emit(int nx, int ny, int nz, double dx, const float *dens, double iso, const unsigned *off,
                           float *pos, float *nrm, long cap, double *vpart);
-__global__ void k_mt_volume(int nx, int ny, int nz, float dx, const float *dens, float iso, double *vpart,
+__global__ void k_mt_volume(int nx, int ny, int nz, double dx, const float *dens, double iso, double *vpart,
                             unsigned *cnt);
 __global__ void k_sum_f64(const double *part, int n, double *out);
 __global__ void k_kinetic(int nP, const float *pvx, const float *pvy, const float *pvz, double *part);

@@ -58,8 +58,13 @@ __constant__ int MT_TRI[16][2][3] = {
     {{3, 4, 5}, {0, 0, 0}}, {{0, 2, 5}, {0, 5, 4}}, {{0, 1, 5}, {0, 5, 3}}, {{2, 1, 5}, {0, 0, 0}},
     {{2, 1, 4}, {2, 4, 3}}, {{0, 1, 4}, {0, 0, 0}}, {{0, 2, 3}, {0, 0, 0}}, {{0, 0, 0}, {0, 0, 0}}};
 
+// The corner values of a cell and its triangle count. The reference
+// compares the float field values with a double contour level, and does
+// all vertex, normal, orientation and volume arithmetic in double; so does
+// this code (the file is compiled without FMA contraction), storing only
+// the final positions and normals as float.
 __device__ __forceinline__ int mt_load(int nx, int ny, const float *__restrict__ dens, int i, int j, int k,
-                                       float iso, float cv[8]) {
+                                       double iso, float cv[8]) {
   long sj = nx + 1, sk = (long)(ny + 1) * sj;
   float mn = INFINITY, mx = -INFINITY;
 #pragma unroll
@@ -69,19 +74,19 @@ __device__ __forceinline__ int mt_load(int nx, int ny, const float *__restrict__
     mn = fminf(mn, v);
     mx = fmaxf(mx, v);
   }
-  if (mx < iso || mn > iso) return 0;
+  if ((double)mx < iso || (double)mn > iso) return 0;
   int tris = 0;
 #pragma unroll
   for (int t = 0; t < 6; t++) {
     int code = 0;
     for (int c = 0; c < 4; c++)
-      if (cv[MT_TET[t][c]] > iso) code |= 1 << c;
+      if ((double)cv[MT_TET[t][c]] > iso) code |= 1 << c;
     tris += MT_NTRI[code];
   }
   return tris;
 }
 
-__global__ void k_mt_count(int nx, int ny, int nz, const float *__restrict__ dens, float iso, unsigned *cnt) {
+__global__ void k_mt_count(int nx, int ny, int nz, const float *__restrict__ dens, double iso, unsigned *cnt) {
   long nC = (long)nx * ny * nz;
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nC) return;
@@ -91,62 +96,63 @@ __global__ void k_mt_count(int nx, int ny, int nz, const float *__restrict__ den
 }
 
 // Triangles of one cell in the reference's order (tets 0..5, case rows),
-// surface.js:102-170. Emits when pos != NULL; returns the signed volume.
-__device__ double mt_cell(int nx, int ny, int nz, float dx, const float *__restrict__ dens, int i, int j, int k,
-                          float iso, const float cv[8], float *pos, float *nrm, long o, long cap) {
+// surface.js:89-170. Emits when pos != NULL; returns the signed volume.
+__device__ double mt_cell(int nx, int ny, int nz, double dx, const float *__restrict__ dens, int i, int j, int k,
+                          double iso, const float cv[8], float *pos, float *nrm, long o, long cap) {
   long sj = nx + 1, sk = (long)(ny + 1) * sj;
-  float gxv[8], gyv[8], gzv[8];
-  float inv2dx = 1.f / (2.f * dx);
+  double gxv[8], gyv[8], gzv[8];
+  double dx2 = 2 * dx;
 #pragma unroll
   for (int c = 0; c < 8; c++) {
     int ci = i + MT_CORNER[c][0], cj = j + MT_CORNER[c][1], ck = k + MT_CORNER[c][2];
     int ip = min(ci + 1, nx), im = max(ci - 1, 0), jp = min(cj + 1, ny), jm = max(cj - 1, 0);
     int kp = min(ck + 1, nz), km = max(ck - 1, 0);
-    gxv[c] = (dens[ck * sk + cj * sj + ip] - dens[ck * sk + cj * sj + im]) * inv2dx;
-    gyv[c] = (dens[ck * sk + jp * sj + ci] - dens[ck * sk + jm * sj + ci]) * inv2dx;
-    gzv[c] = (dens[kp * sk + cj * sj + ci] - dens[km * sk + cj * sj + ci]) * inv2dx;
+    gxv[c] = ((double)dens[ck * sk + cj * sj + ip] - (double)dens[ck * sk + cj * sj + im]) / dx2;
+    gyv[c] = ((double)dens[ck * sk + jp * sj + ci] - (double)dens[ck * sk + jm * sj + ci]) / dx2;
+    gzv[c] = ((double)dens[kp * sk + cj * sj + ci] - (double)dens[km * sk + cj * sj + ci]) / dx2;
   }
   double vol = 0;
   for (int t = 0; t < 6; t++) {
     int code = 0;
     for (int c = 0; c < 4; c++)
-      if (cv[MT_TET[t][c]] > iso) code |= 1 << c;
+      if ((double)cv[MT_TET[t][c]] > iso) code |= 1 << c;
     for (int e = 0; e < MT_NTRI[code]; e++) {
-      float P[3][3], N[3][3];
+      double P[3][3], N[3][3];
       for (int vi = 0; vi < 3; vi++) {
         int ed = MT_TRI[code][e][vi];
         int a = MT_TET[t][MT_EDGE[ed][0]], b = MT_TET[t][MT_EDGE[ed][1]];
-        float va = cv[a], vb = cv[b], den = vb - va;
-        float tl = den != 0.f ? (iso - va) / den : 0.5f;
-        tl = fminf(fmaxf(tl, 0.f), 1.f);
-        P[vi][0] = ((float)(i + MT_CORNER[a][0]) + (float)(MT_CORNER[b][0] - MT_CORNER[a][0]) * tl) * dx;
-        P[vi][1] = ((float)(j + MT_CORNER[a][1]) + (float)(MT_CORNER[b][1] - MT_CORNER[a][1]) * tl) * dx;
-        P[vi][2] = ((float)(k + MT_CORNER[a][2]) + (float)(MT_CORNER[b][2] - MT_CORNER[a][2]) * tl) * dx;
-        float nxx = -(gxv[a] + (gxv[b] - gxv[a]) * tl);
-        float nyy = -(gyv[a] + (gyv[b] - gyv[a]) * tl);
-        float nzz = -(gzv[a] + (gzv[b] - gzv[a]) * tl);
-        float nl = sqrtf(nxx * nxx + nyy * nyy + nzz * nzz);
-        if (!(nl != 0.f)) nl = 1.f;
+        double va = cv[a], vb = cv[b], den = vb - va;
+        double tl = den != 0 ? (iso - va) / den : 0.5;
+        if (tl < 0) tl = 0; else if (tl > 1) tl = 1;
+        double wxx = (double)(i + MT_CORNER[a][0]) + (double)(MT_CORNER[b][0] - MT_CORNER[a][0]) * tl;
+        double wyy = (double)(j + MT_CORNER[a][1]) + (double)(MT_CORNER[b][1] - MT_CORNER[a][1]) * tl;
+        double wzz = (double)(k + MT_CORNER[a][2]) + (double)(MT_CORNER[b][2] - MT_CORNER[a][2]) * tl;
+        P[vi][0] = wxx * dx; P[vi][1] = wyy * dx; P[vi][2] = wzz * dx;
+        double nxx = -(gxv[a] + (gxv[b] - gxv[a]) * tl);
+        double nyy = -(gyv[a] + (gyv[b] - gyv[a]) * tl);
+        double nzz = -(gzv[a] + (gzv[b] - gzv[a]) * tl);
+        double nl = sqrt(nxx * nxx + nyy * nyy + nzz * nzz);
+        if (!(nl != 0)) nl = 1;
         N[vi][0] = nxx / nl; N[vi][1] = nyy / nl; N[vi][2] = nzz / nl;
       }
-      float ux = P[1][0] - P[0][0], uy = P[1][1] - P[0][1], uz = P[1][2] - P[0][2];
-      float wx = P[2][0] - P[0][0], wy = P[2][1] - P[0][1], wz = P[2][2] - P[0][2];
-      float fnx = uy * wz - uz * wy, fny = uz * wx - ux * wz, fnz = ux * wy - uy * wx;
-      float cgx = N[0][0] + N[1][0] + N[2][0], cgy = N[0][1] + N[1][1] + N[2][1], cgz = N[0][2] + N[1][2] + N[2][2];
-      if (fnx * cgx + fny * cgy + fnz * cgz < 0.f) {
+      double ux = P[1][0] - P[0][0], uy = P[1][1] - P[0][1], uz = P[1][2] - P[0][2];
+      double wx = P[2][0] - P[0][0], wy = P[2][1] - P[0][1], wz = P[2][2] - P[0][2];
+      double fnx = uy * wz - uz * wy, fny = uz * wx - ux * wz, fnz = ux * wy - uy * wx;
+      double cgx = 0, cgy = 0, cgz = 0;
+      for (int vi = 0; vi < 3; vi++) { cgx += N[vi][0]; cgy += N[vi][1]; cgz += N[vi][2]; }
+      if (fnx * cgx + fny * cgy + fnz * cgz < 0) {
         for (int d = 0; d < 3; d++) {
-          float tp = P[2][d]; P[2][d] = P[1][d]; P[1][d] = tp;
+          double tp = P[2][d]; P[2][d] = P[1][d]; P[1][d] = tp;
           tp = N[2][d]; N[2][d] = N[1][d]; N[1][d] = tp;
         }
       }
-      double p00 = P[0][0], p01 = P[0][1], p02 = P[0][2], p10 = P[1][0], p11 = P[1][1], p12 = P[1][2];
-      double p20 = P[2][0], p21 = P[2][1], p22 = P[2][2];
-      vol += (p00 * (p11 * p22 - p21 * p12) - p01 * (p10 * p22 - p20 * p12) + p02 * (p10 * p21 - p20 * p11)) / 6.0;
+      vol += (P[0][0] * (P[1][1] * P[2][2] - P[2][1] * P[1][2]) - P[0][1] * (P[1][0] * P[2][2] - P[2][0] * P[1][2]) +
+              P[0][2] * (P[1][0] * P[2][1] - P[2][0] * P[1][1])) / 6;
       if (pos && o + 3 <= cap) {
         for (int vi = 0; vi < 3; vi++)
           for (int d = 0; d < 3; d++) {
-            pos[(o + vi) * 3 + d] = P[vi][d];
-            nrm[(o + vi) * 3 + d] = N[vi][d];
+            pos[(o + vi) * 3 + d] = (float)P[vi][d];
+            nrm[(o + vi) * 3 + d] = (float)N[vi][d];
           }
       }
       o += 3;
@@ -166,8 +172,8 @@ __device__ __forceinline__ void block_sum_f64(double v, double *out) {
   if (threadIdx.x == 0) *out = sh[0];
 }
 
-__global__ __launch_bounds__(256) void k_mt_emit(int nx, int ny, int nz, float dx, const float *__restrict__ dens,
-                                                 float iso, const unsigned *__restrict__ off, float *pos, float *nrm,
+__global__ __launch_bounds__(256) void k_mt_emit(int nx, int ny, int nz, double dx, const float *__restrict__ dens,
+                                                 double iso, const unsigned *__restrict__ off, float *pos, float *nrm,
                                                  long cap, double *vpart) {
   long nC = (long)nx * ny * nz;
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -181,8 +187,8 @@ __global__ __launch_bounds__(256) void k_mt_emit(int nx, int ny, int nz, float d
 }
 
 // volume and vertex count only (the bisection of autotuneIso)
-__global__ __launch_bounds__(256) void k_mt_volume(int nx, int ny, int nz, float dx, const float *__restrict__ dens,
-                                                   float iso, double *vpart, unsigned *cnt) {
+__global__ __launch_bounds__(256) void k_mt_volume(int nx, int ny, int nz, double dx, const float *__restrict__ dens,
+                                                   double iso, double *vpart, unsigned *cnt) {
   long nC = (long)nx * ny * nz;
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double vol = 0;

@@ -428,6 +428,18 @@ FluidSolver.prototype.stirAt = function (cx, cy, cz, ux, uy, uz, R, dt) {
   }
 };
 
+// The particle fields a renderer reads each frame (scene.updateParticles,
+// js/scene.js:1595-1700) in one batched readback: typed arrays over pinned
+// host memory, valid until the next readRender call. `fields` lists names
+// (default: all of px py pz pflag pT pLight pDepth pPh pWx pWy pWz).
+var RENDER_BITS = { px: 1, py: 2, pz: 4, pflag: 8, pT: 16, pLight: 32, pDepth: 64, pPh: 128, pWx: 256,
+  pWy: 512, pWz: 1024 };
+FluidSolver.prototype.readRender = function (fields) {
+  var mask = 0;
+  (fields || Object.keys(RENDER_BITS)).forEach(function (k) { mask |= RENDER_BITS[k] || 0; });
+  return this._call(function (L, h) { return L.readRender(h, mask); });
+};
+
 // addBall, solver.js:3910-3912
 FluidSolver.prototype.addBall = function (x, y, z, r, rho) {
   this.balls.push({ x: x, y: y, z: z, vx: 0, vy: 0, vz: 0, r: r, rho: rho, submerged: 0 });

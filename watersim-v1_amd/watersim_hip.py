@@ -26,7 +26,17 @@ EXPORTS = ["ws_create", "ws_destroy", "ws_last_error", "ws_set_f", "ws_get_f", "
            "ws_substep", "ws_stage", "ws_step_frame", "ws_run_substeps", "ws_splat_density", "ws_mt_build",
            "ws_autotune_iso", "ws_kinetic_energy", "ws_sync", "ws_world_build", "ws_world_free",
            "ws_net_param_count", "ws_net_set_params", "ws_net_get_params", "ws_net_forward", "ws_net_grad",
-           "ws_net_bank", "ws_net_bank_count", "ws_net_train", "ws_set_balls", "ws_get_balls", "ws_metaballs_build"]
+           "ws_net_bank", "ws_net_bank_count", "ws_net_train", "ws_set_balls", "ws_get_balls", "ws_metaballs_build", "ws_read_render"]
+
+
+class RenderView(ctypes.Structure):
+    _fields_ = [("nP", ctypes.c_int)] + [(k, ctypes.POINTER(ctypes.c_float)) for k in
+                                          ("px", "py", "pz", "pT", "pLight", "pPh", "pWx", "pWy", "pWz")] + \
+               [(k, ctypes.POINTER(ctypes.c_uint8)) for k in ("pflag", "pDepth")]
+
+
+RENDER_BITS = {"px": 1, "py": 2, "pz": 4, "pflag": 8, "pT": 16, "pLight": 32, "pDepth": 64, "pPh": 128, "pWx": 256,
+               "pWy": 512, "pWz": 1024}
 
 
 class WorldParams(ctypes.Structure):
@@ -93,6 +103,7 @@ def load(path=LIB_PATH):
     L.ws_net_train.argtypes = [vp, ci, ci, ci, d, ctypes.c_uint, ctypes.POINTER(d)]
     L.ws_metaballs_build.argtypes = [vp, d, d, vp, vp, ctypes.c_long, ctypes.POINTER(ctypes.c_long),
                                      ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(ci)]
+    L.ws_read_render.argtypes = [vp, ctypes.c_uint, ctypes.POINTER(RenderView)]
     L.ws_set_balls.argtypes = [vp, ci, vp]
     L.ws_get_balls.argtypes = [vp, ci, vp]
     for f in EXPORTS:
@@ -225,6 +236,20 @@ class Solver:
 
     def sync(self):
         self._check(self.L.ws_sync(self.ctx), "sync")
+
+    def read_render(self, names=tuple(RENDER_BITS)):
+        """The render fields in one batched readback; returns numpy views of
+        the context's pinned host memory (valid until the next call)."""
+        mask = 0
+        for k in names:
+            mask |= RENDER_BITS[k]
+        v = RenderView()
+        self._check(self.L.ws_read_render(self.ctx, mask, ctypes.byref(v)), "read_render")
+        out = {}
+        for k in names:
+            ptr = getattr(v, k)
+            out[k] = np.ctypeslib.as_array(ptr, (v.nP,)) if v.nP and ptr else np.zeros(0)
+        return out
 
     # ---- balls (moving solids): (n, 9) doubles x y z vx vy vz r rho submerged
     def set_balls(self, balls):
