@@ -173,6 +173,15 @@ typedef struct {
 } ws_render_view;
 int ws_read_render(ws_ctx *c, unsigned mask, ws_render_view *view);
 
+/* Autotuner (no counterpart in the reference, which has one SOR loop): on
+ * the context's current state it times `sweeps` SOR sweeps of each SOR
+ * kernel (sorMode 0 per-colour grid, 1 LDS tile, 2 fluid list, 3 two sweeps
+ * per LDS launch; all give the same bits) and, with the warm start on, the
+ * untiled and LDS-tiled convolutions, and keeps the fastest ("sorMode",
+ * "netTiled"). The pressure and all counters are left as they were.
+ * `report` (may be NULL) receives one line per candidate and the choice. */
+int ws_tune(ws_ctx *c, int sweeps, char *report, size_t len);
+
 /* Waits for all work queued on the context's stream. */
 int ws_sync(ws_ctx *c);
 

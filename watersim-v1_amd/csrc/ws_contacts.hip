@@ -169,41 +169,68 @@ __device__ __forceinline__ unsigned member(const unsigned long long *gk, const u
   return (unsigned)(xk[(unsigned)(gk[t] & 0xffffffffull)] & 0xffffffffull);
 }
 
-// One pair of _vaporCollisions' parcel-parcel sweep (solver.js:3380-3470):
-// i is the earlier parcel in x order. Returns false where the x window of i
-// ends (the sweep moves on to the next parcel).
+// one past the last position of every group in the sorted group keys,
+// indexed by the group's root (all-ones keys belong to no group)
+__global__ void k_ct_group_ends(int n, const unsigned long long *__restrict__ gk, unsigned *__restrict__ ends) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || gk[t] == ~0ull) return;
+  unsigned root = (unsigned)(gk[t] >> 32);
+  if (t == n - 1 || gk[t + 1] == ~0ull || (unsigned)(gk[t + 1] >> 32) != root) ends[root] = (unsigned)(t + 1);
+}
+
+// One pair of _vaporCollisions' parcel-parcel sweep (solver.js:3380-3470)
+// on two parcels held in registers: I is the earlier parcel in x order.
+// Returns false where the x window of I ends (the sweep moves on to the
+// next parcel); *touched tells whether J changed.
 struct VaporState {
   float *px, *py, *pz, *vx, *vy, *vz, *wx, *wy, *wz;
 };
+struct Parcel {
+  float x, y, z, vx, vy, vz, wx, wy, wz;
+};
 
-__device__ __forceinline__ bool vapor_pair(const VaporState &P, unsigned i, unsigned j, float rad, float rad2,
-                                           int spin) {
-  float *px = P.px, *py = P.py, *pz = P.pz, *vx = P.vx, *vy = P.vy, *vz = P.vz;
-  if (px[j] - px[i] >= rad) return false;
-  float ddx = px[j] - px[i], ddy = py[j] - py[i], ddz = pz[j] - pz[i];
+__device__ __forceinline__ Parcel parcel_load(const VaporState &P, unsigned i, int spin) {
+  Parcel a;
+  a.x = P.px[i]; a.y = P.py[i]; a.z = P.pz[i];
+  a.vx = P.vx[i]; a.vy = P.vy[i]; a.vz = P.vz[i];
+  a.wx = a.wy = a.wz = 0.f;
+  if (spin) { a.wx = P.wx[i]; a.wy = P.wy[i]; a.wz = P.wz[i]; }
+  return a;
+}
+
+__device__ __forceinline__ void parcel_store(const VaporState &P, unsigned i, const Parcel &a, int spin) {
+  P.px[i] = a.x; P.py[i] = a.y; P.pz[i] = a.z;
+  P.vx[i] = a.vx; P.vy[i] = a.vy; P.vz[i] = a.vz;
+  if (spin) { P.wx[i] = a.wx; P.wy[i] = a.wy; P.wz[i] = a.wz; }
+}
+
+__device__ __forceinline__ bool vapor_pair(Parcel &I, Parcel &J, float rad, float rad2, int spin, bool *touched) {
+  *touched = false;
+  if (J.x - I.x >= rad) return false;
+  float ddx = J.x - I.x, ddy = J.y - I.y, ddz = J.z - I.z;
   float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
   if (d2 >= rad2) return true;
+  *touched = true;
   float d = sqrtf(d2);
   if (!(d != 0.f)) d = 1e-9f;
   float nX = ddx / d, nY = ddy / d, nZ = ddz / d;
-  float vRel = (vx[i] - vx[j]) * nX + (vy[i] - vy[j]) * nY + (vz[i] - vz[j]) * nZ;
+  float vRel = (I.vx - J.vx) * nX + (I.vy - J.vy) * nY + (I.vz - J.vz) * nZ;
   if (vRel > 0.f) {
-    vx[i] -= vRel * nX; vy[i] -= vRel * nY; vz[i] -= vRel * nZ;
-    vx[j] += vRel * nX; vy[j] += vRel * nY; vz[j] += vRel * nZ;
+    I.vx -= vRel * nX; I.vy -= vRel * nY; I.vz -= vRel * nZ;
+    J.vx += vRel * nX; J.vy += vRel * nY; J.vz += vRel * nZ;
   }
   float half = (rad - d) * 0.5f;
-  px[i] -= nX * half; py[i] -= nY * half; pz[i] -= nZ * half;
-  px[j] += nX * half; py[j] += nY * half; pz[j] += nZ * half;
+  I.x -= nX * half; I.y -= nY * half; I.z -= nZ * half;
+  J.x += nX * half; J.y += nY * half; J.z += nZ * half;
   float kick = (rad - d) * 2.f;
-  vx[i] -= nX * kick; vy[i] -= nY * kick; vz[i] -= nZ * kick;
-  vx[j] += nX * kick; vy[j] += nY * kick; vz[j] += nZ * kick;
+  I.vx -= nX * kick; I.vy -= nY * kick; I.vz -= nZ * kick;
+  J.vx += nX * kick; J.vy += nY * kick; J.vz += nZ * kick;
   if (!spin) return true;
-  float *wx = P.wx, *wy = P.wy, *wz = P.wz;
   float R = rad * 0.5f;
-  float wSx = wx[i] + wx[j], wSy = wy[i] + wy[j], wSz = wz[i] + wz[j];
-  float usx = (vx[i] - vx[j]) + R * (wSy * nZ - wSz * nY);
-  float usy = (vy[i] - vy[j]) + R * (wSz * nX - wSx * nZ);
-  float usz = (vz[i] - vz[j]) + R * (wSx * nY - wSy * nX);
+  float wSx = I.wx + J.wx, wSy = I.wy + J.wy, wSz = I.wz + J.wz;
+  float usx = (I.vx - J.vx) + R * (wSy * nZ - wSz * nY);
+  float usy = (I.vy - J.vy) + R * (wSz * nX - wSx * nZ);
+  float usz = (I.vz - J.vz) + R * (wSx * nY - wSy * nX);
   float uN = usx * nX + usy * nY + usz * nZ;
   float utx = usx - uN * nX, uty = usy - uN * nY, utz = usz - uN * nZ;
   float utM = sqrtf(utx * utx + uty * uty + utz * utz);
@@ -211,33 +238,49 @@ __device__ __forceinline__ bool vapor_pair(const VaporState &P, unsigned i, unsi
   float jt = utM / 7.f;
   if (vRel > 0.f) { float mu = 0.25f * vRel; if (jt > mu) jt = mu; }
   float jx = -utx / utM * jt, jy = -uty / utM * jt, jz = -utz / utM * jt;
-  vx[i] += jx; vy[i] += jy; vz[i] += jz;
-  vx[j] -= jx; vy[j] -= jy; vz[j] -= jz;
+  I.vx += jx; I.vy += jy; I.vz += jz;
+  J.vx -= jx; J.vy -= jy; J.vz -= jz;
   float invI = 1.f / (0.4f * R * R);
   float dwx = R * (nY * jz - nZ * jy) * invI, dwy = R * (nZ * jx - nX * jz) * invI;
   float dwz = R * (nX * jy - nY * jx) * invI;
-  wx[i] += dwx; wy[i] += dwy; wz[i] += dwz;
-  wx[j] += dwx; wy[j] += dwy; wz[j] += dwz;
+  I.wx += dwx; I.wy += dwy; I.wz += dwz;
+  J.wx += dwx; J.wy += dwy; J.wz += dwz;
   return true;
 }
 
 // the sweep of a small group, one group per thread; groups larger than
 // serialMax (and not marked for the serial path) go to k_vc_sweep
 __global__ void k_ct_vapor_groups(int n, const unsigned long long *__restrict__ gk,
-                                  const unsigned long long *__restrict__ xk, float rad, int spin, VaporState P,
-                                  int serialMax, const unsigned *__restrict__ serialOnly) {
+                                  const unsigned long long *__restrict__ xk, const unsigned *__restrict__ ends,
+                                  float rad, int spin, VaporState P, int serialMax,
+                                  const unsigned *__restrict__ serialOnly) {
   int t0 = blockIdx.x * blockDim.x + threadIdx.x;
   if (t0 >= n) return;
   if (t0 > 0 && (gk[t0] >> 32) == (gk[t0 - 1] >> 32)) return;  // not the first member of its group
-  int t1 = t0 + 1;
-  while (t1 < n && (gk[t1] >> 32) == (gk[t0] >> 32)) t1++;
+  unsigned root = (unsigned)(gk[t0] >> 32);
+  int t1 = (int)ends[root];
   if (t1 - t0 < 2) return;
-  if (t1 - t0 > serialMax && !serialOnly[gk[t0] >> 32]) return;
+  if (t1 - t0 > serialMax && !serialOnly[root]) return;
   const float rad2 = rad * rad;
   for (int a = t0; a < t1; a++) {
     unsigned i = member(gk, xk, a);
-    for (int b = a + 1; b < t1; b++)
-      if (!vapor_pair(P, i, member(gk, xk, b), rad, rad2, spin)) break;
+    Parcel I = parcel_load(P, i, spin);
+    unsigned jn = a + 1 < t1 ? member(gk, xk, a + 1) : 0;
+    Parcel Jn = a + 1 < t1 ? parcel_load(P, jn, spin) : I;
+    for (int b = a + 1; b < t1; b++) {
+      // the next member's state is fetched while this pair is resolved
+      unsigned j = jn;
+      Parcel J = Jn;
+      if (b + 1 < t1) {
+        jn = member(gk, xk, b + 1);
+        Jn = parcel_load(P, jn, spin);
+      }
+      bool touched;
+      bool go = vapor_pair(I, J, rad, rad2, spin, &touched);
+      if (touched) parcel_store(P, j, J, spin);
+      if (!go) break;
+    }
+    parcel_store(P, i, I, spin);
   }
 }
 
@@ -337,59 +380,332 @@ __global__ void k_vc_need(int np, const unsigned long long *__restrict__ keys, c
 
 // the groups for the dependency-ordered sweep, as [t0, t1) ranges of the
 // sorted group keys
-__global__ void k_vc_big(int n, const unsigned long long *__restrict__ gk, int serialMax,
-                         const unsigned *__restrict__ serialOnly, unsigned *__restrict__ big,
+__global__ void k_vc_big(int n, const unsigned long long *__restrict__ gk, const unsigned *__restrict__ ends,
+                         int serialMax, const unsigned *__restrict__ serialOnly, unsigned *__restrict__ big,
                          unsigned *__restrict__ nBig) {
   int t0 = blockIdx.x * blockDim.x + threadIdx.x;
   if (t0 >= n) return;
   if (t0 > 0 && (gk[t0] >> 32) == (gk[t0 - 1] >> 32)) return;
-  int t1 = t0 + 1;
-  while (t1 < n && (gk[t1] >> 32) == (gk[t0] >> 32)) t1++;
-  if (t1 - t0 <= serialMax || serialOnly[gk[t0] >> 32]) return;
+  unsigned root = (unsigned)(gk[t0] >> 32);
+  int t1 = (int)ends[root];
+  if (t1 - t0 <= serialMax || serialOnly[root]) return;
+  atomicMax(&nBig[2], (unsigned)(t1 - t0));   // largest group
+  atomicAdd(&nBig[3], (unsigned)(t1 - t0));   // parcels in these groups
   unsigned slot = atomicAdd(nBig, 1u);
   big[2 * slot] = (unsigned)t0;
   big[2 * slot + 1] = (unsigned)t1;
 }
 
-__device__ __forceinline__ unsigned ld_wg(const unsigned *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+__device__ __forceinline__ unsigned ld_wg_relaxed(const unsigned *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// one group's steps in dependency order; the per-parcel counts live in LDS
+// (indexed by group position) when the group fits, else in global memory
+// (indexed by subset element)
+constexpr int kVcLds = 16384;
+
+template <bool LDS>
+struct VcStep {
+  unsigned pi, e, nc, own;
+  unsigned x[4], need[4], p[4];  // slots 0..3: counter index, position in its list, partner particle
+  const unsigned *ne, *ce, *cp;  // the full lists (slots beyond 3)
+};
+
+// the metadata of step `cur` (its parcel, partners, list positions); the
+// first four slots are kept in registers, the rest read from memory
+template <bool LDS>
+__device__ __forceinline__ void vc_step_load(VcStep<LDS> &S, int cur, int t0, const unsigned long long *__restrict__ gk,
+                                             const unsigned long long *__restrict__ xk,
+                                             const unsigned *__restrict__ posOf, const unsigned *__restrict__ cand,
+                                             const unsigned *__restrict__ candT, const unsigned *__restrict__ candP,
+                                             const unsigned *__restrict__ ncand, const unsigned *__restrict__ need) {
+  unsigned r = (unsigned)(gk[cur] & 0xffffffffull);
+  S.pi = (unsigned)(xk[r] & 0xffffffffull);
+  S.e = posOf[S.pi];
+  S.nc = ncand[S.e];
+  S.ne = need + (long)S.e * (kVcK + 1);
+  S.ce = LDS ? candT + (long)cur * kVcK : cand + (long)S.e * kVcK;
+  S.cp = candP + (long)cur * kVcK;
+  S.own = LDS ? (unsigned)(cur - t0) : S.e;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    bool in = (unsigned)k <= S.nc;
+    S.need[k] = in ? S.ne[k] : 0;
+    S.x[k] = k == 0 ? S.own : (in ? (LDS ? S.ce[k - 1] - (unsigned)t0 : S.ce[k - 1]) : 0);
+    S.p[k] = (k > 0 && in) ? S.cp[k - 1] : S.pi;
+  }
+}
+
+// register "arrays" indexed at run time through selects (an indexed
+// register array would live in scratch memory)
+__device__ __forceinline__ unsigned sel4(const unsigned (&a)[4], unsigned k) {
+  return k == 0 ? a[0] : (k == 1 ? a[1] : (k == 2 ? a[2] : a[3]));
+}
+
+template <bool LDS>
+__device__ __forceinline__ unsigned vc_slot_x(const VcStep<LDS> &S, unsigned k, int t0) {
+  if (k < 4) return sel4(S.x, k);
+  return LDS ? S.ce[k - 1] - (unsigned)t0 : S.ce[k - 1];
+}
+
+template <bool LDS>
+__device__ __forceinline__ unsigned vc_slot_need(const VcStep<LDS> &S, unsigned k) {
+  return k < 4 ? sel4(S.need, k) : S.ne[k];
+}
+
+template <bool LDS>
+__device__ __forceinline__ unsigned vc_slot_p(const VcStep<LDS> &S, unsigned k) {
+  return k < 4 ? sel4(S.p, k) : S.cp[k - 1];
+}
+
+template <bool LDS>
+__device__ void vc_sweep_group(int t0, int t1, unsigned *cnt, const unsigned long long *__restrict__ gk,
+                               const unsigned long long *__restrict__ xk, const unsigned *__restrict__ posOf,
+                               const unsigned *__restrict__ cand, const unsigned *__restrict__ candT,
+                               const unsigned *__restrict__ candP, const unsigned *__restrict__ ncand,
+                               const unsigned *__restrict__ need, float rad, float rad2, int spin, VaporState P,
+                               unsigned *__restrict__ unfinished) {
+  int cur = t0 + (int)threadIdx.x;
+  long limit = 64L * (t1 - t0) + 4096;
+  // this thread's steps are cur, cur + blockDim, ...
+  VcStep<LDS> S;
+  if (cur < t1) vc_step_load<LDS>(S, cur, t0, gk, xk, posOf, cand, candT, candP, ncand, need);
+  unsigned blocker = 0;
+  for (long it = 0; cur < t1 && it < limit; it++) {
+    // readiness: every touched parcel's count at this step's position (the
+    // counts are read without ordering; one acquire fence then orders the
+    // parcel reads after them). A slot once seen ready stays ready.
+    bool ready = true;
+    for (unsigned k = blocker; k <= S.nc; k++) {
+      unsigned want = vc_slot_need<LDS>(S, k);
+      if (__hip_atomic_load(&cnt[vc_slot_x<LDS>(S, k, t0)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) !=
+          want) {
+        ready = false;
+        blocker = k;
+        break;
+      }
+    }
+    if (!ready) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    Parcel I = parcel_load(P, S.pi, spin);
+    Parcel Jn = S.nc ? parcel_load(P, S.p[1], spin) : I;
+    for (unsigned k = 0; k < S.nc; k++) {
+      // the next partner's state is fetched while this pair is resolved
+      unsigned j = vc_slot_p<LDS>(S, k + 1);
+      Parcel J = Jn;
+      if (k + 1 < S.nc) Jn = parcel_load(P, vc_slot_p<LDS>(S, k + 2), spin);
+      bool touched;
+      bool go = vapor_pair(I, J, rad, rad2, spin, &touched);
+      if (touched) parcel_store(P, j, J, spin);
+      if (!go) break;
+    }
+    parcel_store(P, S.pi, I, spin);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    for (unsigned k = 0; k <= S.nc; k++)
+      __hip_atomic_store(&cnt[vc_slot_x<LDS>(S, k, t0)], vc_slot_need<LDS>(S, k) + 1, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WORKGROUP);
+    // the next step's metadata, after the counts are published (its loads
+    // stay off the path of the steps waiting on this one)
+    cur += (int)blockDim.x;
+    if (cur < t1) vc_step_load<LDS>(S, cur, t0, gk, xk, posOf, cand, candT, candP, ncand, need);
+    blocker = 0;
+  }
+  if (cur < t1) atomicAdd(unfinished, 1u);
+}
+
+// Large groups that fit in LDS run from a ready queue instead: each step's
+// count of unfinished predecessors lives in LDS; a finished step releases
+// its writes, decrements its successors and queues those that reach zero.
+// A wavefront runs one step at a time: lane k fetches partner k (all
+// partners at once), the pairs are resolved in order on values broadcast
+// from the partners' lanes (every lane computes the same numbers), and the
+// lanes of the touched partners write them back.
+constexpr int kVcQ = 16000;  // steps of a group kept in LDS
+constexpr unsigned short kVcEmpty = 0xffffu;
+constexpr int kVcIdle = 1 << 22;  // polls without progress before a group is given up
+
+__device__ __forceinline__ float bcast(float v, unsigned k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)k));
+}
+
+__global__ __launch_bounds__(1024) void k_vc_sweepq(const unsigned *__restrict__ big, const unsigned *__restrict__ nBig,
+                                                    const unsigned *__restrict__ stepP,
+                                                    const unsigned *__restrict__ stepN,
+                                                    const unsigned *__restrict__ candP,
+                                                    const unsigned *__restrict__ succ,
+                                                    const unsigned *__restrict__ depc, float rad, int spin,
+                                                    VaporState P, unsigned *__restrict__ unfinished) {
+  __shared__ unsigned sdep[kVcQ / 2];  // two 16-bit counts per word
+  __shared__ unsigned short sq[kVcQ];
+  __shared__ unsigned shead, stail, sdone;
+  const float rad2 = rad * rad;
+  const unsigned lane = threadIdx.x & 63;
+  unsigned nb = *nBig;
+  for (unsigned gi = blockIdx.x; gi < nb; gi += gridDim.x) {
+    int t0 = (int)big[2 * gi], t1 = (int)big[2 * gi + 1], m = t1 - t0;
+    if (m > kVcQ) continue;  // left to the polling sweep
+    if (threadIdx.x == 0) { shead = 0; stail = 0; sdone = 0; }
+    for (int q = threadIdx.x; q < m; q += blockDim.x) sq[q] = kVcEmpty;
+    for (int w = threadIdx.x; 2 * w < m; w += blockDim.x) {
+      unsigned lo = depc[t0 + 2 * w];
+      unsigned hi = 2 * w + 1 < m ? depc[t0 + 2 * w + 1] : 0u;
+      sdep[w] = lo | (hi << 16);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < m; q += blockDim.x)
+      if (((sdep[q >> 1] >> (16 * (q & 1))) & 0xffffu) == 0) sq[atomicAdd(&stail, 1u)] = (unsigned short)q;
+    __syncthreads();
+    unsigned lastDone = 0;
+    int idle = 0, status = 0;  // 0 a step popped, 1 nothing ready, 2 all done, 3 given up
+    while (true) {
+      unsigned q = 0;
+      if (lane == 0) {
+        unsigned done = __hip_atomic_load(&sdone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (done != lastDone) { lastDone = done; idle = 0; }
+        status = 1;
+        if (done >= (unsigned)m) {
+          status = 2;
+        } else if (++idle > kVcIdle) {
+          status = 3;
+        } else {
+          unsigned h = __hip_atomic_load(&shead, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          unsigned tl = __hip_atomic_load(&stail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (h < tl && atomicCAS(&shead, h, h + 1) == h) {
+            // slot h is claimed by a push whose entry may still be on its way
+            unsigned short v = kVcEmpty;
+            for (int w = 0; w < kVcIdle && v == kVcEmpty; w++)
+              v = __hip_atomic_load(&sq[h], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            status = v == kVcEmpty ? 3 : 0;
+            q = v;
+          }
+        }
+      }
+      status = __builtin_amdgcn_readfirstlane(status);
+      if (status >= 2) break;
+      if (status == 1) {
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      q = __builtin_amdgcn_readfirstlane(q);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      const long t = t0 + (long)q;
+      unsigned pi = __builtin_amdgcn_readfirstlane(stepP[t]);
+      unsigned nc = __builtin_amdgcn_readfirstlane(stepN[t]);
+      bool mine = lane < nc;
+      unsigned jp = mine ? candP[t * kVcK + lane] : 0u;
+      unsigned sT = lane <= nc ? succ[t * (kVcK + 1) + lane] : 0xffffffffu;
+      Parcel I = parcel_load(P, pi, spin);
+      Parcel J{};
+      if (mine) J = parcel_load(P, jp, spin);
+      bool wrote = false;
+      for (unsigned k = 0; k < nc; k++) {
+        Parcel K;
+        K.x = bcast(J.x, k); K.y = bcast(J.y, k); K.z = bcast(J.z, k);
+        K.vx = bcast(J.vx, k); K.vy = bcast(J.vy, k); K.vz = bcast(J.vz, k);
+        K.wx = bcast(J.wx, k); K.wy = bcast(J.wy, k); K.wz = bcast(J.wz, k);
+        bool touched;
+        bool go = vapor_pair(I, K, rad, rad2, spin, &touched);
+        if (touched && lane == k) {
+          J = K;
+          wrote = true;
+        }
+        if (!go) break;
+      }
+      if (wrote) parcel_store(P, jp, J, spin);
+      if (lane == 0) parcel_store(P, pi, I, spin);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      if (sT != 0xffffffffu) {
+        unsigned ls = sT - (unsigned)t0, sh = 16 * (ls & 1);
+        unsigned old = __hip_atomic_fetch_add(&sdep[ls >> 1], 0u - (1u << sh), __ATOMIC_ACQ_REL,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (((old >> sh) & 0xffffu) == 1) {
+          unsigned at = atomicAdd(&stail, 1u);
+          __hip_atomic_store(&sq[at], (unsigned short)ls, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
+      if (lane == 0) atomicAdd(&sdone, 1u);
+    }
+    if (lane == 0 && status == 3) atomicAdd(unfinished, 1u);
+    __syncthreads();
+  }
 }
 
 __global__ __launch_bounds__(256) void k_vc_sweep(const unsigned *__restrict__ big, const unsigned *__restrict__ nBig,
                                                   const unsigned long long *__restrict__ gk,
                                                   const unsigned long long *__restrict__ xk,
-                                                  const unsigned *__restrict__ posOf, const unsigned *__restrict__ list,
-                                                  const unsigned *__restrict__ cand, const unsigned *__restrict__ ncand,
-                                                  const unsigned *__restrict__ need, unsigned *__restrict__ done,
-                                                  float rad, int spin, VaporState P, unsigned *__restrict__ unfinished) {
+                                                  const unsigned *__restrict__ posOf, const unsigned *__restrict__ cand,
+                                                  const unsigned *__restrict__ candT,
+                                                  const unsigned *__restrict__ candP,
+                                                  const unsigned *__restrict__ ncand, const unsigned *__restrict__ need,
+                                                  unsigned *__restrict__ done, float rad, int spin, VaporState P,
+                                                  unsigned *__restrict__ unfinished) {
+  __shared__ unsigned sdone[kVcLds];
   const float rad2 = rad * rad;
   unsigned nb = *nBig;
   for (unsigned gi = blockIdx.x; gi < nb; gi += gridDim.x) {
     int t0 = (int)big[2 * gi], t1 = (int)big[2 * gi + 1];
-    int cur = t0 + (int)threadIdx.x;
-    long limit = 64L * (t1 - t0) + 4096;
-    for (long it = 0; cur < t1 && it < limit; it++) {
-      unsigned r = (unsigned)(gk[cur] & 0xffffffffull);
-      unsigned pi = (unsigned)(xk[r] & 0xffffffffull);
-      unsigned e = posOf[pi];
-      unsigned nc = ncand[e];
-      const unsigned *ne = need + (long)e * (kVcK + 1);
-      const unsigned *ce = cand + (long)e * kVcK;
-      bool ready = ld_wg(&done[e]) == ne[0];
-      for (unsigned k = 0; ready && k < nc; k++) ready = ld_wg(&done[ce[k]]) == ne[1 + k];
-      if (!ready) {
-        __builtin_amdgcn_s_sleep(1);
-        continue;
-      }
-      for (unsigned k = 0; k < nc; k++)
-        if (!vapor_pair(P, pi, list[ce[k]], rad, rad2, spin)) break;
-      __hip_atomic_store(&done[e], ne[0] + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-      for (unsigned k = 0; k < nc; k++)
-        __hip_atomic_store(&done[ce[k]], ne[1 + k] + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-      cur += blockDim.x;
+    if (t1 - t0 <= kVcQ) continue;  // run from the ready queue (k_vc_sweepq)
+    if (t1 - t0 <= kVcLds) {
+      for (int q = threadIdx.x; q < t1 - t0; q += blockDim.x) sdone[q] = 0;
+      __syncthreads();
+      vc_sweep_group<true>(t0, t1, sdone, gk, xk, posOf, cand, candT, candP, ncand, need, rad, rad2, spin, P,
+                           unfinished);
+    } else {
+      vc_sweep_group<false>(t0, t1, done, gk, xk, posOf, cand, candT, candP, ncand, need, rad, rad2, spin, P,
+                            unfinished);
     }
-    if (cur < t1) atomicAdd(unfinished, 1u);
     __syncthreads();
+  }
+}
+
+// each parcel's group position (step index), the particle of each step and
+// its partner count
+__global__ void k_vc_tpos(int n, const unsigned long long *__restrict__ gk, const unsigned long long *__restrict__ xk,
+                          const unsigned *__restrict__ posOf, const unsigned *__restrict__ ncand,
+                          unsigned *__restrict__ tpos, unsigned *__restrict__ stepP, unsigned *__restrict__ stepN) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  unsigned p = member(gk, xk, t), e = posOf[p];
+  tpos[e] = (unsigned)t;
+  stepP[t] = p;
+  stepN[t] = ncand[e];
+}
+
+// the successor of every step in each list it belongs to (the next step
+// touching that parcel; all ones: none) and each step's number of
+// predecessors, by step
+__global__ void k_vc_succ(int np, const unsigned long long *__restrict__ keys, const unsigned *__restrict__ chainStart,
+                          const unsigned long long *__restrict__ xk, const unsigned *__restrict__ posOf,
+                          const unsigned *__restrict__ tpos, unsigned *__restrict__ succ, unsigned *__restrict__ depc) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= np) return;
+  unsigned long long k = keys[s];
+  unsigned x = (unsigned)(k >> 32), rank = (unsigned)((k >> 6) & 0x3ffffffull), slot = (unsigned)(k & 63ull);
+  unsigned e = posOf[(unsigned)(xk[rank] & 0xffffffffull)];
+  unsigned nxt = 0xffffffffu;
+  if (s + 1 < np && (unsigned)(keys[s + 1] >> 32) == x) {
+    unsigned rb = (unsigned)((keys[s + 1] >> 6) & 0x3ffffffull);
+    nxt = tpos[posOf[(unsigned)(xk[rb] & 0xffffffffull)]];
+  }
+  unsigned t = tpos[e];
+  succ[(long)t * (kVcK + 1) + slot] = nxt;
+  if ((unsigned)s != chainStart[x]) atomicAdd(&depc[t], 1u);
+}
+
+__global__ void k_vc_cand_maps(int n, const unsigned *__restrict__ list, const unsigned *__restrict__ tpos,
+                               const unsigned *__restrict__ cand, const unsigned *__restrict__ ncand,
+                               unsigned *__restrict__ candT, unsigned *__restrict__ candP) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  long t = tpos[e];
+  for (unsigned k = 0; k < ncand[e]; k++) {
+    unsigned o = cand[(long)e * kVcK + k];
+    candT[t * kVcK + k] = tpos[o];
+    candP[t * kVcK + k] = list[o];
   }
 }
 
@@ -461,7 +777,8 @@ __device__ __forceinline__ void ice_stick(float *vx, float *vy, float *vz, float
 
 // one group of free grains per thread; stuck members (key all ones) skip
 __global__ void k_ct_ice_groups(int n, const unsigned long long *__restrict__ gk,
-                                const unsigned long long *__restrict__ xk, const unsigned *__restrict__ posOf,
+                                const unsigned long long *__restrict__ xk, const unsigned *__restrict__ ends,
+                                const unsigned *__restrict__ posOf,
                                 const unsigned long long *__restrict__ skey, double rad, double rad2,
                                 const float *__restrict__ px, const float *__restrict__ py,
                                 const float *__restrict__ pz, float *__restrict__ vx, float *__restrict__ vy,
@@ -470,8 +787,7 @@ __global__ void k_ct_ice_groups(int n, const unsigned long long *__restrict__ gk
   if (t0 >= n) return;
   if (gk[t0] == ~0ull) return;
   if (t0 > 0 && (gk[t0] >> 32) == (gk[t0 - 1] >> 32)) return;  // not the first member of its group
-  int t1 = t0 + 1;
-  while (t1 < n && gk[t1] != ~0ull && (gk[t1] >> 32) == (gk[t0] >> 32)) t1++;
+  int t1 = (int)ends[gk[t0] >> 32];
   for (int a = t0; a < t1; a++) {
     unsigned i = member(gk, xk, a);
     unsigned ri = (unsigned)(gk[a] & 0xffffffffull);
@@ -594,7 +910,7 @@ struct Contacts {
   unsigned *fv = nullptr, *fi = nullptr, *fl = nullptr, *ov = nullptr, *oi = nullptr, *ol = nullptr;
   unsigned *vlist = nullptr, *ilist = nullptr, *llist = nullptr;
   unsigned *cellOf = nullptr, *rankIn = nullptr, *binned = nullptr, *parent = nullptr, *posOf = nullptr,
-           *rankOf = nullptr;
+           *rankOf = nullptr, *gend = nullptr;
   unsigned *cnt = nullptr, *start = nullptr, *lstart = nullptr, *lbinned = nullptr, *lcell = nullptr,
            *lrank = nullptr, *counts = nullptr;
   unsigned long long *xk = nullptr, *xk2 = nullptr, *gk = nullptr, *gk2 = nullptr;
@@ -603,7 +919,8 @@ struct Contacts {
   // dependency-ordered sweep of large vapor groups, sized by the vapor count
   long vcap = 0;
   unsigned *cand = nullptr, *ncand = nullptr, *pairOff = nullptr, *need = nullptr, *done = nullptr,
-           *serialOnly = nullptr, *big = nullptr, *vcCounters = nullptr, *chainStart = nullptr;
+           *serialOnly = nullptr, *big = nullptr, *vcCounters = nullptr, *chainStart = nullptr, *candT = nullptr,
+           *candP = nullptr, *tpos = nullptr, *stepP = nullptr, *stepN = nullptr, *succ = nullptr, *depc = nullptr;
   unsigned long long *pairKeys = nullptr, *pairKeys2 = nullptr;
   void *vtmp = nullptr;
   size_t vtmpBytes = 0;
@@ -631,6 +948,7 @@ int contacts_ensure(ws_ctx *c) {
       (r = calloc_dev(c, &k->vlist, n)) || (r = calloc_dev(c, &k->ilist, n)) || (r = calloc_dev(c, &k->llist, n)) ||
       (r = calloc_dev(c, &k->cellOf, n)) || (r = calloc_dev(c, &k->rankIn, n)) || (r = calloc_dev(c, &k->binned, n)) ||
       (r = calloc_dev(c, &k->parent, n)) || (r = calloc_dev(c, &k->posOf, n)) || (r = calloc_dev(c, &k->rankOf, n)) ||
+      (r = calloc_dev(c, &k->gend, n)) ||
       (r = calloc_dev(c, &k->cnt, C)) || (r = calloc_dev(c, &k->start, C)) || (r = calloc_dev(c, &k->lstart, C)) ||
       (r = calloc_dev(c, &k->lbinned, n)) || (r = calloc_dev(c, &k->lcell, n)) || (r = calloc_dev(c, &k->lrank, n)) ||
       (r = calloc_dev(c, &k->counts, 4)) || (r = calloc_dev(c, &k->xk, n)) || (r = calloc_dev(c, &k->xk2, n)) ||
@@ -690,6 +1008,8 @@ int group_subset(ws_ctx *c, int n, const unsigned *list, float thr, const unsign
   hipLaunchKernelGGL(k_ct_groupkeys, dim3(nb), dim3(256), 0, c->stream, n, k->parent, k->rankOf, mask, k->gk2);
   WS_LAUNCHED();
   WS_HIP(hipcub::DeviceRadixSort::SortKeys(k->tmp, k->tmpBytes, k->gk2, k->gk, n, 0, 64, c->stream));
+  hipLaunchKernelGGL(k_ct_group_ends, dim3(nb), dim3(256), 0, c->stream, n, k->gk, k->gend);
+  WS_LAUNCHED();
   return WS_OK;
 }
 
@@ -699,7 +1019,8 @@ int vapor_ensure(ws_ctx *c, long nV) {
   if (k->vcap >= nV) return WS_OK;
   long n = nV + nV / 4 + 1024;
   void *old[] = {k->cand, k->ncand, k->pairOff, k->need, k->done, k->serialOnly, k->big, k->vcCounters,
-                 k->chainStart, k->pairKeys, k->pairKeys2, k->vtmp};
+                 k->chainStart, k->pairKeys, k->pairKeys2, k->vtmp, k->candT, k->candP, k->tpos, k->stepP,
+                 k->stepN, k->succ, k->depc};
   for (void *p : old)
     if (p) (void)hipFree(p);
   k->vcap = 0;
@@ -710,7 +1031,9 @@ int vapor_ensure(ws_ctx *c, long nV) {
       (r = calloc_dev(c, &k->done, n)) || (r = calloc_dev(c, &k->serialOnly, n)) ||
       (r = calloc_dev(c, &k->big, 2 * n)) || (r = calloc_dev(c, &k->vcCounters, 4)) ||
       (r = calloc_dev(c, &k->chainStart, n)) || (r = calloc_dev(c, &k->pairKeys, np)) ||
-      (r = calloc_dev(c, &k->pairKeys2, np)))
+      (r = calloc_dev(c, &k->pairKeys2, np)) || (r = calloc_dev(c, &k->candT, n * kVcK)) ||
+      (r = calloc_dev(c, &k->candP, n * kVcK)) || (r = calloc_dev(c, &k->tpos, n)) ||
+      (r = calloc_dev(c, &k->stepP, n)) || (r = calloc_dev(c, &k->stepN, n)) || (r = calloc_dev(c, &k->succ, np)) || (r = calloc_dev(c, &k->depc, n)))
     return r;
   size_t a = 0, b = 0;
   hipcub::DeviceScan::ExclusiveSum(nullptr, a, k->ncand, k->pairOff, (int)(n + 1), c->stream);
@@ -749,10 +1072,11 @@ void contacts_release(ws_ctx *c, bool all) {
   Contacts *k = c->contacts;
   if (!k) return;
   void *ptrs[] = {k->fv, k->fi, k->fl, k->ov, k->oi, k->ol, k->vlist, k->ilist, k->llist, k->cellOf, k->rankIn,
-                  k->binned, k->parent, k->posOf, k->rankOf, k->cnt, k->start, k->lstart, k->lbinned, k->lcell,
+                  k->binned, k->parent, k->posOf, k->rankOf, k->gend, k->cnt, k->start, k->lstart, k->lbinned, k->lcell,
                   k->lrank, k->counts, k->xk, k->xk2, k->gk, k->gk2, k->tmp, k->cand, k->ncand, k->pairOff,
                   k->need, k->done, k->serialOnly, k->big, k->vcCounters, k->chainStart, k->pairKeys,
-                  k->pairKeys2, k->vtmp};
+                  k->pairKeys2, k->vtmp, k->candT, k->candP, k->tpos, k->stepP, k->stepN, k->succ,
+                  k->depc};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (all) {
@@ -788,7 +1112,7 @@ int frame_vapor_collisions(ws_ctx *c) {
     // groups too large for one thread's serial sweep
     hipLaunchKernelGGL(k_ct_zero, dim3(blocks_for(nV, 256)), dim3(256), 0, c->stream, (long)nV, k->serialOnly);
     hipLaunchKernelGGL(k_ct_zero, dim3(1), dim3(64), 0, c->stream, 4L, k->vcCounters);
-    hipLaunchKernelGGL(k_vc_big, dim3(nb), dim3(256), 0, c->stream, nV, k->gk, serialMax, k->serialOnly, k->big,
+    hipLaunchKernelGGL(k_vc_big, dim3(nb), dim3(256), 0, c->stream, nV, k->gk, k->gend, serialMax, k->serialOnly, k->big,
                        k->vcCounters);
     WS_LAUNCHED();
     unsigned nBig = 0;
@@ -816,25 +1140,38 @@ int frame_vapor_collisions(ws_ctx *c) {
       hipLaunchKernelGGL(k_vc_need, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, (int)np, k->pairKeys,
                          k->chainStart, k->xk, k->posOf, k->need);
       hipLaunchKernelGGL(k_ct_zero, dim3(nb), dim3(256), 0, c->stream, (long)nV, k->done);
+      hipLaunchKernelGGL(k_vc_tpos, dim3(nb), dim3(256), 0, c->stream, nV, k->gk, k->xk, k->posOf, k->ncand, k->tpos,
+                         k->stepP, k->stepN);
+      hipLaunchKernelGGL(k_ct_zero, dim3(nb), dim3(256), 0, c->stream, (long)nV, k->depc);
+      hipLaunchKernelGGL(k_vc_succ, dim3(blocks_for(np, 256)), dim3(256), 0, c->stream, (int)np, k->pairKeys,
+                         k->chainStart, k->xk, k->posOf, k->tpos, k->succ, k->depc);
+      hipLaunchKernelGGL(k_vc_cand_maps, dim3(nb), dim3(256), 0, c->stream, nV, k->vlist, k->tpos, k->cand, k->ncand,
+                         k->candT, k->candP);
       // the large groups again, now without those marked for the serial sweep
       hipLaunchKernelGGL(k_ct_zero, dim3(1), dim3(64), 0, c->stream, 4L, k->vcCounters);
-      hipLaunchKernelGGL(k_vc_big, dim3(nb), dim3(256), 0, c->stream, nV, k->gk, serialMax, k->serialOnly, k->big,
+      hipLaunchKernelGGL(k_vc_big, dim3(nb), dim3(256), 0, c->stream, nV, k->gk, k->gend, serialMax, k->serialOnly, k->big,
                          k->vcCounters);
       WS_LAUNCHED();
     }
-    hipLaunchKernelGGL(k_ct_vapor_groups, dim3(nb), dim3(256), 0, c->stream, nV, k->gk, k->xk, rad, c->spinOn, P,
+    hipLaunchKernelGGL(k_ct_vapor_groups, dim3(nb), dim3(256), 0, c->stream, nV, k->gk, k->xk, k->gend, rad, c->spinOn, P,
                        serialMax, k->serialOnly);
     if (nBig > 0) {
       int grid = (int)(nBig < 2048 ? nBig : 2048);
+      hipLaunchKernelGGL(k_vc_sweepq, dim3(grid), dim3(1024), 0, c->stream, k->big, k->vcCounters, k->stepP, k->stepN,
+                         k->candP, k->succ, k->depc, rad, c->spinOn, P, k->vcCounters + 1);
       hipLaunchKernelGGL(k_vc_sweep, dim3(grid), dim3(256), 0, c->stream, k->big, k->vcCounters, k->gk, k->xk,
-                         k->posOf, k->vlist, k->cand, k->ncand, k->need, k->done, rad, c->spinOn, P,
+                         k->posOf, k->cand, k->candT, k->candP, k->ncand, k->need, k->done, rad, c->spinOn, P,
                          k->vcCounters + 1);
     }
     WS_LAUNCHED();
+    c->vaporBigGroups = (long)nBig;
     if (nBig > 0) {
-      unsigned unfinished = 0;
-      WS_HIP(hipMemcpyAsync(&unfinished, k->vcCounters + 1, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+      unsigned vc[4];
+      WS_HIP(hipMemcpyAsync(vc, k->vcCounters, sizeof vc, hipMemcpyDeviceToHost, c->stream));
       WS_HIP(hipStreamSynchronize(c->stream));
+      unsigned unfinished = vc[1];
+      c->vaporGroupMax = (long)vc[2];
+      c->vaporBigParcels = (long)vc[3];
       if (unfinished) {
         c->err = "vapor contact sweep: a group did not finish within its step bound";
         return WS_ESTATE;
@@ -876,7 +1213,7 @@ int frame_ice_contacts(ws_ctx *c) {
   if ((r = group_subset(c, nI, k->ilist, (float)rad * (1.f + 1e-5f), k->fi))) return r;
   hipLaunchKernelGGL(k_ct_ice_stuckkey, dim3(nb), dim3(256), 0, c->stream, c->geo, nI, k->ilist, c->px, c->py, c->pz,
                      k->cellOf, k->start, k->binned, cell_span(c, rad), k->fi, k->rankOf, rad, rad2, k->gk2);
-  hipLaunchKernelGGL(k_ct_ice_groups, dim3(nb), dim3(256), 0, c->stream, nI, k->gk, k->xk, k->posOf, k->gk2, rad,
+  hipLaunchKernelGGL(k_ct_ice_groups, dim3(nb), dim3(256), 0, c->stream, nI, k->gk, k->xk, k->gend, k->posOf, k->gk2, rad,
                      rad2, c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pAir);
   WS_LAUNCHED();
   return WS_OK;

@@ -120,6 +120,7 @@ struct Ctx {
   Contacts *contacts = nullptr;
   int vaporCount = 0, nightVaporCount = 0;
   int vaporSerialMax = 48;  // larger vapor groups run the dependency-ordered sweep
+  long vaporBigGroups = 0, vaporGroupMax = 0, vaporBigParcels = 0;  // of the last vapor pass
   long contactOverflow = 0;
   // both solves per pressure stage, timed separately (measurement)
   int pressureCompare = 0;

@@ -26,7 +26,7 @@ EXPORTS = ["ws_create", "ws_destroy", "ws_last_error", "ws_set_f", "ws_get_f", "
            "ws_substep", "ws_stage", "ws_step_frame", "ws_run_substeps", "ws_splat_density", "ws_mt_build",
            "ws_autotune_iso", "ws_kinetic_energy", "ws_sync", "ws_world_build", "ws_world_free",
            "ws_net_param_count", "ws_net_set_params", "ws_net_get_params", "ws_net_forward", "ws_net_grad",
-           "ws_net_bank", "ws_net_bank_count", "ws_net_train", "ws_set_balls", "ws_get_balls", "ws_metaballs_build", "ws_read_render"]
+           "ws_net_bank", "ws_net_bank_count", "ws_net_train", "ws_set_balls", "ws_get_balls", "ws_metaballs_build", "ws_read_render", "ws_tune"]
 
 
 class RenderView(ctypes.Structure):
@@ -104,6 +104,7 @@ def load(path=LIB_PATH):
     L.ws_metaballs_build.argtypes = [vp, d, d, vp, vp, ctypes.c_long, ctypes.POINTER(ctypes.c_long),
                                      ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(ci)]
     L.ws_read_render.argtypes = [vp, ctypes.c_uint, ctypes.POINTER(RenderView)]
+    L.ws_tune.argtypes = [vp, ci, ctypes.c_char_p, sz]
     L.ws_set_balls.argtypes = [vp, ci, vp]
     L.ws_get_balls.argtypes = [vp, ci, vp]
     for f in EXPORTS:
@@ -236,6 +237,13 @@ class Solver:
 
     def sync(self):
         self._check(self.L.ws_sync(self.ctx), "sync")
+
+    def tune(self, sweeps=64):
+        """The autotuner: fastest SOR kernel (and convolution tiling with the
+        warm start on) on the current state; returns its report."""
+        buf = ctypes.create_string_buffer(1024)
+        self._check(self.L.ws_tune(self.ctx, sweeps, buf, len(buf)), "tune")
+        return buf.value.decode()
 
     def read_render(self, names=tuple(RENDER_BITS)):
         """The render fields in one batched readback; returns numpy views of
