@@ -801,7 +801,7 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal) GF(residualsTotal) GF(ticksTotal) GF(sorMode) GF(sorCellSweeps)
   GF(cmpMsPlain) GF(cmpMsLearned) GF(cmpSweepsPlain) GF(cmpSweepsLearned) GF(cmpSolves) GF(warmStart) GF(netMs) GF(netCalls)
   GF(vaporCount) GF(nightVaporCount) GF(contactOverflow) GF(nBalls) GF(mbM) GF(vaporBigGroups) GF(vaporGroupMax)
-  GF(vaporBigParcels)
+  GF(vaporBigParcels) GF(netTiled)
   GF(heatK) GF(Tamb) GF(shadeCool) GF(sunActivity) GF(evapIntensity) GF(snowT) GF(meltT) GF(cloudP) GF(currents)
   GF(simTime) GF(heatAcc) GF(evapCount) GF(cloudT) GF(rainT) GF(rainLiftAge)
 #undef GF
@@ -889,6 +889,8 @@ int ws_tune(ws_ctx *c, int sweeps, char *report, size_t len) {
   if (!c || sweeps < 1) return WS_EINVAL;
   int r;
   if ((r = need_world(c))) return r;
+  for (int i = 0; i < 4; i++)
+    if (!c->ev[i]) WS_HIP(hipEventCreate(&c->ev[i]));
   // the state the trial solves must leave untouched
   const int keepMode = c->sorMode, keepIters = c->pressureIters, keepRec = c->recordSamples, keepWarm = c->warmStart;
   const int keepTiled = c->netTiled, keepTiming = c->timing;
