@@ -400,13 +400,14 @@ def learned_main(a, cfg):
 
 
 def batched_main(a, cfg):
-    """Batched planets: independent planets of mixed sizes, each its own
-    context and stream, driven by host threads (one call per substep each;
-    the GPU runs the planets' kernels side by side). Each planet uses the
-    learned warm start plus SOR to 1e-4; the autotuner picks the SOR kernel
-    and the convolution tiling per grid size on one planet of that size.
+    """Batched planets: independent planets of mixed sizes advanced together
+    by the library's batch (ws_batch_*): each planet's stages on its own
+    stream, all pressure solves (learned warm start, then SOR to 1e-4 with
+    the residual checks on the device) in one launch (a few workgroups per
+    planet, by size), and the whole batched substep captured into a HIP
+    graph and replayed. The autotuner picks the convolution tiling per grid size (on
+    one planet of each size) and the solve's workgroup size for the batch.
     value = particle updates per second summed over the planets."""
-    import threading
     from concurrent.futures import ThreadPoolExecutor
     import watersim_hip
     rng = np.random.default_rng(cfg["seed"])
@@ -414,6 +415,8 @@ def batched_main(a, cfg):
     sizes = [int(x) for x in rng.choice(cfg["sizes"], size=P)]
     gravs = [float(9.8 * rng.uniform(0.8, 1.2)) for _ in range(P)]
     weights = np.fromfile(a.weights, dtype=np.float32) if os.path.exists(a.weights) else None
+    if weights is None:
+        raise SystemExit("cfg5 needs the trained warm-start weights (%s)" % a.weights)
     t = time.time()
 
     def build(i):
@@ -428,65 +431,60 @@ def batched_main(a, cfg):
         s = watersim_hip.solver_for_world(w)
         s.set(pic=0.05, omega=1.7, gravity=gravs[i], maxSpeed=3.2, vorticity=cfg["vorticity"], cfl=3.0,
               pressureTol=cfg["tol"], maxPressureIters=4000, residualEvery=8)
-        if weights is not None:
-            s.set_net_params(weights)
-            s.set(warmStart=1)
+        s.set_net_params(weights)
+        s.set(warmStart=1)
         planets.append(s)
+    del worlds
     nPs = [int(s.get("nP")) for s in planets]
     log("%d planets, %d particles (%d .. %d)" % (P, sum(nPs), min(nPs), max(nPs)))
-    # warm up, then tune one planet per size and apply its choice to the size
-    for s in planets:
-        s.run_substeps(a.warmup)
-    choice = {}
+    batch = watersim_hip.Batch(planets, graph=True, threads=1024)
+    batch.run(a.warmup)
+    # autotuner: convolution tiling per grid size (one planet of each size),
+    # then the solve's workgroup size for the whole batch
+    tiling = {}
     for n in sorted(set(sizes)):
         s = planets[sizes.index(n)]
         rep = s.tune(64)
-        choice[n] = (int(s.get("sorMode")), int(s.get("netTiled")))
+        tiling[n] = int(s.get("netTiled"))
         log("tune %d^3: %s" % (n, rep.replace("\n", "; ")))
     for i, s in enumerate(planets):
-        s.set(sorMode=choice[sizes[i]][0], netTiled=choice[sizes[i]][1])
-    for s in planets:
-        s.sync()
-    nthreads = 8
-    lat = [0.0] * P
-    lock = threading.Lock()
-    progress = [0]
-
-    def drive(k):
-        mine = list(range(k, P, nthreads))
-        for step in range(a.steps):
-            for i in mine:
-                t1 = time.perf_counter()
-                planets[i].run_substeps(1)
-                planets[i].sync()
-                lat[i] += time.perf_counter() - t1
-            with lock:
-                progress[0] += 1
-                if progress[0] % (nthreads * 100) == 0:
-                    log("  %d substeps per planet" % (progress[0] // nthreads))
-    log("timing %d substeps per planet, %d host threads" % (a.steps, nthreads))
+        s.set(netTiled=tiling[sizes[i]])
+    trial = {}
+    for th in (256, 512, 1024):
+        batch.set(threads=th)
+        t1 = time.perf_counter()
+        batch.run(10)
+        trial[th] = (time.perf_counter() - t1) / 10
+        log("batch with %d-thread solves: %.3f ms per batched substep" % (th, 1e3 * trial[th]))
+    threads = min(trial, key=trial.get)
+    batch.set(threads=threads)
+    log("timing %d batched substeps (HIP graph), %d-thread solves" % (a.steps, threads))
     t0 = time.perf_counter()
-    with ThreadPoolExecutor(nthreads) as ex:
-        list(ex.map(drive, range(nthreads)))
+    done = 0
+    while done < a.steps:
+        k = min(100, a.steps - done)
+        batch.run(k)
+        done += k
+        log("  %d substeps per planet" % done)
     el = time.perf_counter() - t0
     updates = sum(nPs) * a.steps
+    log("aggregate %.3g particle-updates/s, wall %.1f s" % (updates / el, el))
+    # the dominant kernel, the joint solve, timed with HIP events on the batch
+    # stream over a few eager substeps
+    batch.set(timing=1)
+    batch.run(4)
+    g = {k: batch.get(k) for k in ("solveMs", "solveLaunches", "cellSweeps", "sweeps")}
+    batch.set(timing=0)
+    launches = max(g["solveLaunches"], 1)
+    solve_s = g["solveMs"] / 1e3 / launches
+    solve_b = 13.0 * g["cellSweeps"] / launches
+    achieved = solve_b / solve_s / 1e9 if g["solveMs"] > 0 else 0.0
+    sweeps_per = g["sweeps"] / launches / P
     per_size = {}
     for n in sorted(set(sizes)):
         idx = [i for i in range(P) if sizes[i] == n]
         per_size["%d^3" % n] = {"planets": len(idx), "particles_mean": float(np.mean([nPs[i] for i in idx])),
-                                "substep_ms_mean": 1e3 * float(np.mean([lat[i] for i in idx])) / a.steps,
-                                "sorMode": choice[n][0], "netTiled": choice[n][1]}
-    log("aggregate %.3g particle-updates/s, wall %.1f s" % (updates / el, el))
-    # the dominant kernel: the list SOR of the largest size, timed on its own
-    big = planets[sizes.index(max(sizes))]
-    big.set(timing=1, sorMode=2)
-    big.run_substeps(4)
-    g = {k: big.get(k) for k in ("sorMs", "sorLaunches", "sorCellSweeps")}
-    big.set(timing=0, sorMode=choice[max(sizes)][0])
-    launches = max(g["sorLaunches"], 1)
-    sor_s = g["sorMs"] / 1e3 / launches
-    sor_b = 13.0 * g["sorCellSweeps"] / launches
-    achieved = sor_b / sor_s / 1e9 if g["sorLaunches"] else 0.0
+                                "netTiled": tiling[n]}
     out = {
         "metric": "aggregate particle-updates/s, %d batched planets (32^3..96^3, learned warm start + SOR to 1e-4)" % P,
         "value": updates / el,
@@ -497,16 +495,21 @@ def batched_main(a, cfg):
                                                                                             cfg["seed"] + P),
         "config": {"workload": "cfg5: %d independent planets, grid sizes drawn from %s, about 8 particles per "
                                "liquid cell, own terrain seed and gravity each, learned warm start + red-black SOR "
-                               "to 1e-4, vorticity %g; one context and stream per planet, %d host threads, "
-                               "autotuned SOR kernel and convolution tiling per size"
-                               % (P, list(cfg["sizes"]), cfg["vorticity"], nthreads),
+                               "to 1e-4, vorticity %g; one batch: every planet's stages on its own stream, all "
+                               "pressure solves in one launch (workgroups per planet by size), the batched substep "
+                               "replayed from a HIP graph; autotuned convolution tiling per size and solve "
+                               "workgroup size" % (P, list(cfg["sizes"]), cfg["vorticity"]),
                    "particles_total": sum(nPs), "per_size": per_size,
-                   "planet_substep_ms_mean": 1e3 * float(np.mean(lat)) / a.steps},
+                   "planet_substep_ms": 1e3 * el / a.steps,
+                   "solve_threads": threads,
+                   "batch_trial_ms": {str(k): 1e3 * v for k, v in trial.items()},
+                   "joint_solve_ms": 1e3 * solve_s, "sweeps_per_planet_substep": sweeps_per},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / (HBM_PEAK / 1e9), "traffic": None,
-                     "kernel": "k_sor_list on a %d^3 planet (the fluid cells of one colour)" % max(sizes),
-                     "avg_launch_us": 1e6 * sor_s, "algorithmic_bytes_per_launch": sor_b},
+                     "kernel": "k_sor_solve_batch (every planet's whole pressure solve, workgroups per planet by size)",
+                     "avg_launch_us": 1e6 * solve_s, "algorithmic_bytes_per_launch": solve_b},
     }
+    batch.close()
     for s in planets:
         s.close()
     print(json.dumps(out), flush=True)

@@ -176,11 +176,45 @@ int ws_read_render(ws_ctx *c, unsigned mask, ws_render_view *view);
 /* Autotuner (no counterpart in the reference, which has one SOR loop): on
  * the context's current state it times `sweeps` SOR sweeps of each SOR
  * kernel (sorMode 0 per-colour grid, 1 LDS tile, 2 fluid list, 3 two sweeps
- * per LDS launch; all give the same bits) and, with the warm start on, the
- * untiled and LDS-tiled convolutions, and keeps the fastest ("sorMode",
- * "netTiled"). The pressure and all counters are left as they were.
- * `report` (may be NULL) receives one line per candidate and the choice. */
+ * per LDS launch, 4 the whole solve in one workgroup of 256, 512 or 1024
+ * threads; all give the same bits for the same sweeps) and, with the warm
+ * start on, the untiled and LDS-tiled convolutions, and keeps the fastest
+ * ("sorMode", "sorThreads", "netTiled"). The pressure and all counters are
+ * left as they were. `report` (may be NULL) receives one line per candidate
+ * and the choice. */
 int ws_tune(ws_ctx *c, int sweeps, char *report, size_t len);
+
+/* Batched planets (no counterpart in the reference, which runs one solver
+ * per page). A batch advances n contexts together, each by whole substeps
+ * with the device dt law, as ws_run_substeps does for one context: every
+ * planet's stages are queued on its own stream, the pressure solves of all
+ * planets run in one launch (one workgroup per planet, residual checks on
+ * the device, sorMode 4 arithmetic), and with use_graph the batched substep
+ * is captured once per ws_batch_run call into a HIP graph that is replayed
+ * for each substep. Each planet's results equal those of ws_run_substeps
+ * with sorMode 4 and the same sorThreads (the batch's); a planet's solve may
+ * span several workgroups, which changes no bits. Between calls the
+ * contexts may be read, written and reconfigured as usual; while
+ * ws_batch_run runs they must not be used from another thread. Contexts in
+ * a batch may not have "timing", "recordSamples" or "pressureCompare" on.
+ * `threads` is the workgroup size of the solves (256, 512 or 1024). The
+ * contexts stay owned by the caller and must outlive the batch. */
+typedef struct ws_batch ws_batch;
+int ws_batch_create(ws_batch **out, ws_ctx *const *ctx, int n, int use_graph, int threads);
+int ws_batch_run(ws_batch *b, int substeps);
+/* Settings: "graph" (0/1), "threads" (256/512/1024), "cellsPerGroup" (a
+ * planet's solve gets one workgroup per that many cells, 1 to 32),
+ * "maxGroups" (1..256: workgroups of the joint solve in all; they must all be
+ * resident at once, so the launch runs alone on the GPU between the planets'
+ * stages), "timing" (0/1: the following runs go eagerly, timing the joint
+ * solve launch with HIP events; setting it resets the totals). Values: those,
+ * "groups" (workgroups of the last joint solve), and the totals of the timed
+ * runs: "solveMs", "solveLaunches", "sweeps" (summed over planets),
+ * "cellSweeps" (fluid cells relaxed, summed over planets and sweeps). */
+int ws_batch_set(ws_batch *b, const char *name, double v);
+int ws_batch_get(ws_batch *b, const char *name, double *v);
+const char *ws_batch_last_error(const ws_batch *b);
+void ws_batch_destroy(ws_batch *b);
 
 /* Waits for all work queued on the context's stream. */
 int ws_sync(ws_ctx *c);

@@ -251,6 +251,75 @@ int sor_sweeps(ws_ctx *c, int n) {
   return WS_OK;
 }
 
+int fold_dev_stats(ws_ctx *c);
+void take_counts(ws_ctx *c);
+
+// the on-device solve of one context (sorMode 4)
+SorJob sor_job(ws_ctx *c) {
+  SorJob J;
+  J.g = c->geo;
+  J.ds = c->ds;
+  J.code = c->sorCode;
+  J.lrhs = c->sorRhs;
+  J.lpw = c->sorPw;
+  J.ldg = c->sorDg;
+  J.q = c->q;
+  J.half = c->sorHalf;
+  J.om1 = (float)(1 - c->omega);
+  J.every = c->residualEvery > 0 ? c->residualEvery : 1;
+  J.maxIters = c->maxPressureIters;
+  J.adaptive = c->adaptiveChecks;
+  J.fixedIters = c->pressureIters;
+  J.measure = c->measureResidual;
+  J.tol = c->pressureTol;
+  J.chunk = c->sorChunk;
+  J.arrive = nullptr;
+  J.G = 1;
+  return J;
+}
+
+int sor_solve_dev(ws_ctx *c) {
+  SorJob J = sor_job(c);
+  if (c->timing) WS_HIP(hipEventRecord(c->ev[0], c->stream));
+  if (c->sorThreads == 256)
+    hipLaunchKernelGGL(k_sor_solve<256>, dim3(1), dim3(256), 0, c->stream, J);
+  else if (c->sorThreads == 512)
+    hipLaunchKernelGGL(k_sor_solve<512>, dim3(1), dim3(512), 0, c->stream, J);
+  else
+    hipLaunchKernelGGL(k_sor_solve<1024>, dim3(1), dim3(1024), 0, c->stream, J);
+  WS_LAUNCHED();
+  c->devStatsPending = true;
+  if (c->timing) {
+    WS_HIP(hipEventRecord(c->ev[1], c->stream));
+    WS_HIP(hipEventSynchronize(c->ev[1]));
+    float ms = 0;
+    WS_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->sorMs += ms;
+    c->sorLaunches += 1;
+    int r = fold_dev_stats(c);
+    if (r) return r;
+    c->sorCellSweeps += (c->nRed + c->nBlack) * (long)c->itersDone;
+  }
+  return WS_OK;
+}
+
+// the sweep and residual counts of on-device solves, folded into the host
+// counters (and the last solve's sweeps and residual)
+int fold_dev_stats(ws_ctx *c) {
+  if (!c->devStatsPending) return WS_OK;
+  WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+  WS_HIP(hipStreamSynchronize(c->stream));
+  c->devStatsPending = false;
+  c->itersDone = (int)c->hs->sorIters;
+  c->residual = c->hs->sorRes;
+  c->sweepsTotal += (long)(c->hs->sorSweeps - c->sorSweepsSeen);
+  c->residualsTotal += (long)(c->hs->sorChecks - c->sorChecksSeen);
+  c->sorSweepsSeen = c->hs->sorSweeps;
+  c->sorChecksSeen = c->hs->sorChecks;
+  take_counts(c);
+  return WS_OK;
+}
+
 // the list lengths, as the last device-scalar copy saw them
 void take_counts(ws_ctx *c) {
   c->nRed = c->hs->nRed;
@@ -318,7 +387,16 @@ int st_pressure(ws_ctx *c) {
   return WS_OK;
 }
 
+int pressure_setup(ws_ctx *c);
+int pressure_iterate(ws_ctx *c);
+
 int st_pressure_solve(ws_ctx *c) {
+  int r = pressure_setup(c);
+  return r ? r : pressure_iterate(c);
+}
+
+// the system, its fluid lists and the starting guess (warm start)
+int pressure_setup(ws_ctx *c) {
   hipLaunchKernelGGL(k_assemble, dim3(blocks_for(c->nC, 256)), dim3(256), 0, c->stream, c->geo, c->cellType, c->vel,
                      c->q, c->q2, c->rhs, c->pw, c->nbm, c->dg, (float)c->omega);
   if (c->sorMode == 1) {
@@ -342,7 +420,7 @@ int st_pressure_solve(ws_ctx *c) {
     hipLaunchKernelGGL(k_sor_compact, dim3(nb), dim3(256), 0, c->stream, c->geo, c->nbm, c->dg, c->rhs, c->pw,
                        c->sorOff, c->sorHalf, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->ds);
     WS_LAUNCHED();
-    if (c->pressureTol <= 0) {  // the to-residual path learns the counts with its first check
+    if (c->pressureTol <= 0 && c->sorMode != 4) {  // the to-residual path learns the counts with its first check
       WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
       WS_HIP(hipStreamSynchronize(c->stream));
       take_counts(c);
@@ -362,6 +440,17 @@ int st_pressure_solve(ws_ctx *c) {
       c->netMs += ms;
       c->netCalls++;
     }
+  }
+  return WS_OK;
+}
+
+// the sweeps, to the tolerance or the fixed count
+int pressure_iterate(ws_ctx *c) {
+  int r;
+  if (c->sorMode == 4) {
+    if ((r = sor_solve_dev(c))) return r;
+    if (c->recordSamples && (r = net_record_target(c))) return r;
+    return WS_OK;
   }
   if (c->pressureTol <= 0) {
     if ((r = sor_sweeps(c, c->pressureIters))) return r;
@@ -484,8 +573,19 @@ int st_particles(ws_ctx *c, int mode) {
   return WS_OK;
 }
 
+int substep_pre(ws_ctx *c, double dt, int use_law);
+int substep_post(ws_ctx *c);
+
 // the full substep; dt either from the host or from the CFL law on the device
 int substep(ws_ctx *c, double dt, int use_law) {
+  int r;
+  if ((r = substep_pre(c, dt, use_law)) || (r = st_pressure(c)) || (r = substep_post(c))) return r;
+  return WS_OK;
+}
+
+// the substep up to the pressure solve: dt, rasterization, P2G, boundary
+// conditions, gravity and viscosity
+int substep_pre(ws_ctx *c, double dt, int use_law) {
   int r;
   if ((r = need_world(c))) return r;
   double base = c->dtFrameBase > 0 ? c->dtFrameBase : 0.01;
@@ -502,7 +602,13 @@ int substep(ws_ctx *c, double dt, int use_law) {
     if ((r = st_viscosity(c, use_law ? 0.01 : dt, c->velO))) return r;
     if ((r = st_gravity(c))) return r;
   }
-  if ((r = st_pressure(c))) return r;
+  return WS_OK;
+}
+
+// the substep after the pressure solve: projection, confinement, clamp,
+// extrapolation, G2P and advection, balls
+int substep_post(ws_ctx *c) {
+  int r;
   bool vort = c->vorticity > 0;
   if ((r = st_project(c, vort ? 0 : 1))) return r;
   if (vort) {
@@ -576,6 +682,18 @@ bool field_of(ws_ctx *c, const char *n, Field &f) {
 }
 
 }  // namespace
+
+// the halves of a device-dt-law substep around the pressure sweeps, for the
+// batched driver (ws_batch.hip)
+namespace ws {
+int batch_substep_pre(ws_ctx *c) {
+  int r = substep_pre(c, 0.01, 1);
+  return r ? r : pressure_setup(c);
+}
+int batch_substep_post(ws_ctx *c) { return substep_post(c); }
+SorJob batch_sor_job(ws_ctx *c) { return sor_job(c); }
+void batch_solved(ws_ctx *c) { c->devStatsPending = true; }
+}  // namespace ws
 
 // the terrain view the frame passes use (ws_frame.hip)
 World ws::make_world(ws_ctx *c) { return world_of(c); }
@@ -653,7 +771,7 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
   }
   if ((r = dalloc(c, &c->sorOff, c->nC + 1)) || (r = dalloc(c, &c->sorCode, c->nC + 4)) ||
       (r = dalloc(c, &c->sorRhs, c->nC + 4)) || (r = dalloc(c, &c->sorPw, c->nC + 4)) ||
-      (r = dalloc(c, &c->sorDg, c->nC + 4))) {
+      (r = dalloc(c, &c->sorDg, c->nC + 4)) || (r = dalloc(c, &c->sorChunk, 2 * (c->nC / 2048 + 2)))) {
     ws_destroy(c);
     return r;
   }
@@ -726,7 +844,7 @@ void ws_destroy(ws_ctx *c) {
   balls_release(c);
   metaballs_release(c);
   void *ptrs[] = {c->staticTypes, c->gu, c->thr, c->terrainR, c->cellType, c->nbm, c->dg, c->q, c->q2, c->rhs, c->pw,
-                  c->sorActive, c->tbActive, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg,
+                  c->sorActive, c->tbActive, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->sorChunk,
                   c->cellCount, c->cellStart, c->vort, c->vel, c->velO, c->wgt, c->scr, c->scr2, c->valid, c->dens,
                   c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pT, c->pAir, c->advU, c->advV, c->advW, c->pflag,
                   c->pcool, c->pcell, c->prank, c->sortedIdx, c->vneed, c->vscr, c->rng, c->ds, c->partials,
@@ -754,9 +872,21 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
   SF(heatK) SF(Tamb) SF(shadeCool) SF(sunActivity) SF(evapIntensity) SF(snowT) SF(meltT) SF(cloudP)
   SF(currents) SF(Rlo) SF(Rhi) SF(simTime) SF(heatAcc) SF(netLossCoarse)
   SI(pressureIters) SI(maxPressureIters) SI(residualEvery) SI(ceilReflect) SI(hasSun) SI(measureResidual)
-  SI(adaptiveChecks) SI(sorMode) SI(spinOn) SI(warmStart) SI(recordSamples) SI(pressureCompare) SI(netTiled) SI(vaporSerialMax)
+  SI(adaptiveChecks) SI(spinOn) SI(warmStart) SI(recordSamples) SI(pressureCompare) SI(netTiled) SI(vaporSerialMax)
 #undef SF
 #undef SI
+  if (!strcmp(n, "sorMode")) {
+    int m = (int)v;
+    if (m < 0 || m > 4 || m != v) { c->err = "sorMode must be 0..4"; return WS_EINVAL; }
+    if (m >= 2 && c->nC >= (1L << 26)) { c->err = "sorMode 2..4 need fewer than 2^26 cells"; return WS_EINVAL; }
+    c->sorMode = m;
+    return WS_OK;
+  }
+  if (!strcmp(n, "sorThreads")) {
+    if (v != 256 && v != 512 && v != 1024) { c->err = "sorThreads must be 256, 512 or 1024"; return WS_EINVAL; }
+    c->sorThreads = (int)v;
+    return WS_OK;
+  }
   if (!strcmp(n, "gRamp")) { c->gRamp = v; return push_ramp(c); }
   if (!strcmp(n, "nP")) {
     if (v < 0 || v > c->capacity) { c->err = "nP exceeds capacity"; return WS_EINVAL; }
@@ -766,6 +896,8 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
   if (!strcmp(n, "tn")) { c->tn = (int)v; return WS_OK; }
   if (!strcmp(n, "hasVapor")) { c->hasVapor = v != 0; return WS_OK; }
   if (!strcmp(n, "timing")) {
+    int r = fold_dev_stats(c);  // solves before the reset stay out of the new totals
+    if (r) return r;
     c->timing = v != 0;
     for (int i = 0; i < 4 && c->timing; i++)
       if (!c->ev[i]) WS_HIP(hipEventCreate(&c->ev[i]));
@@ -784,6 +916,9 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
 int ws_get_f(ws_ctx *c, const char *n, double *v) {
   if (!c || !n || !v) return WS_EINVAL;
   int r;
+  if (!strcmp(n, "residual") || !strcmp(n, "sweepsTotal") || !strcmp(n, "residualsTotal") ||
+      !strcmp(n, "pressureIters_done"))
+    if ((r = fold_dev_stats(c))) return r;
   if (!strcmp(n, "umax") || !strcmp(n, "airborneCount") || !strcmp(n, "gRamp") || !strcmp(n, "dtLast") ||
       !strcmp(n, "dfRefined")) {
     if ((r = pull_scalars(c))) return r;
@@ -801,7 +936,7 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal) GF(residualsTotal) GF(ticksTotal) GF(sorMode) GF(sorCellSweeps)
   GF(cmpMsPlain) GF(cmpMsLearned) GF(cmpSweepsPlain) GF(cmpSweepsLearned) GF(cmpSolves) GF(warmStart) GF(netMs) GF(netCalls)
   GF(vaporCount) GF(nightVaporCount) GF(contactOverflow) GF(nBalls) GF(mbM) GF(vaporBigGroups) GF(vaporGroupMax)
-  GF(vaporBigParcels) GF(netTiled)
+  GF(vaporBigParcels) GF(netTiled) GF(sorThreads)
   GF(heatK) GF(Tamb) GF(shadeCool) GF(sunActivity) GF(evapIntensity) GF(snowT) GF(meltT) GF(cloudP) GF(currents)
   GF(simTime) GF(heatAcc) GF(evapCount) GF(cloudT) GF(rainT) GF(rainLiftAge)
 #undef GF
@@ -891,7 +1026,9 @@ int ws_tune(ws_ctx *c, int sweeps, char *report, size_t len) {
   if ((r = need_world(c))) return r;
   for (int i = 0; i < 4; i++)
     if (!c->ev[i]) WS_HIP(hipEventCreate(&c->ev[i]));
+  if ((r = fold_dev_stats(c))) return r;
   // the state the trial solves must leave untouched
+  const double keepRes = c->residual;
   const int keepMode = c->sorMode, keepIters = c->pressureIters, keepRec = c->recordSamples, keepWarm = c->warmStart;
   const int keepTiled = c->netTiled, keepTiming = c->timing;
   const double keepTol = c->pressureTol;
@@ -916,21 +1053,31 @@ int ws_tune(ws_ctx *c, int sweeps, char *report, size_t len) {
     WS_HIP(hipEventElapsedTime(ms, c->ev[2], c->ev[3]));
     return WS_OK;
   };
-  // SOR kernels: per-colour grid, LDS tile, fluid list, two sweeps per launch
+  // SOR kernels: per-colour grid, LDS tile, fluid list, two sweeps per
+  // launch, and the one-workgroup solve at three workgroup sizes
   c->warmStart = 0;
-  int best = keepMode;
+  const int keepThreads = c->sorThreads;
+  int best = keepMode, bestThreads = keepThreads;
   float bestMs = 1e30f;
-  for (int mode = 0; mode <= 3 && !r; mode++) {
-    c->sorMode = mode;
+  const int cand[][2] = {{0, 0}, {1, 0}, {2, 0}, {3, 0}, {4, 256}, {4, 512}, {4, 1024}};
+  for (const auto &cm : cand) {
+    if (r) break;
+    if (cm[0] >= 2 && c->nC >= (1L << 26)) continue;
+    c->sorMode = cm[0];
+    if (cm[1]) c->sorThreads = cm[1];
     float ms = 0;
     for (int rep2 = 0; rep2 < 2 && !r; rep2++) r = trial(&ms);  // the second run is timed
     if (r) break;
     char line[128];
-    snprintf(line, sizeof line, "sorMode %d: %d sweeps in %.3f ms\n", mode, sweeps, ms);
+    if (cm[1])
+      snprintf(line, sizeof line, "sorMode %d (%d threads): %d sweeps in %.3f ms\n", cm[0], cm[1], sweeps, ms);
+    else
+      snprintf(line, sizeof line, "sorMode %d: %d sweeps in %.3f ms\n", cm[0], sweeps, ms);
     rep += line;
-    if (ms < bestMs) { bestMs = ms; best = mode; }
+    if (ms < bestMs) { bestMs = ms; best = cm[0]; bestThreads = cm[1] ? cm[1] : keepThreads; }
   }
   c->sorMode = best;
+  c->sorThreads = bestThreads;
   // the warm start's convolutions: untiled or LDS-tiled
   int bestTiled = keepTiled;
   if (!r && keepWarm && c->net) {
@@ -952,6 +1099,9 @@ int ws_tune(ws_ctx *c, int sweeps, char *report, size_t len) {
   hipError_t e1 = hipMemcpyAsync(c->q, saved, sizeof(float) * c->nC, hipMemcpyDeviceToDevice, c->stream);
   hipError_t e2 = hipStreamSynchronize(c->stream);
   (void)hipFree(saved);
+  int rf = fold_dev_stats(c);  // the trial solves' device counts, absorbed and then undone below
+  if (!r) r = rf;
+  c->residual = keepRes;
   c->recordSamples = keepRec;
   c->warmStart = keepWarm;
   c->timing = keepTiming;
@@ -963,7 +1113,8 @@ int ws_tune(ws_ctx *c, int sweeps, char *report, size_t len) {
   if (r) return r;
   if (e1 != hipSuccess || e2 != hipSuccess) { c->err = "ws_tune: restoring the pressure failed"; return WS_EHIP; }
   if (report && len) {
-    snprintf(report, len, "%schosen: sorMode %d, netTiled %d", rep.c_str(), c->sorMode, c->netTiled);
+    snprintf(report, len, "%schosen: sorMode %d, sorThreads %d, netTiled %d", rep.c_str(), c->sorMode, c->sorThreads,
+             c->netTiled);
   }
   return WS_OK;
 }

@@ -1,0 +1,355 @@
+// Batched planets: many contexts advanced together by device-dt-law
+// substeps. Each planet's stages before and after the pressure sweeps are
+// queued on its own stream; between them one launch of k_sor_solve_batch
+// runs every planet's whole pressure solve (one workgroup per planet, the
+// residual checks on the device), so a substep costs no host round trip and
+// the solves of all planets run side by side. With a graph, the batched
+// substep (fork to the planets' streams, the joint solve, join) is captured
+// once per run and replayed for every substep.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ws_frame.h"
+
+using namespace ws;
+
+struct ws_batch {
+  std::vector<ws_ctx *> ctx;
+  int useGraph = 0, threads = 1024;
+  bool warm = false;  // one eager substep has run (lazy allocations are done)
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, solved = nullptr;
+  std::vector<hipEvent_t> pre, post;
+  SorJob *jobs = nullptr;  // device copy of the planets' solve descriptions
+  std::vector<SorJob> hjobs;
+  // workgroups of the joint solve: G per planet (about one per
+  // cellsPerGroup cells, at most maxGroups in all, so that every workgroup
+  // is resident while the launch runs alone on the GPU), block -> (planet,
+  // workgroup), and per planet a barrier counter and a timed-out flag
+  double cellsPerGroup = 110000;
+  int maxGroups = 256, nBlocks = 0;
+  std::vector<int> G;
+  int2 *blk = nullptr;
+  unsigned *arrive = nullptr;
+  // measurement: with timing on, substeps run eagerly with HIP events around
+  // the joint solve launch and a synchronization after each
+  int timing = 0;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  double solveMs = 0, cellSweeps = 0, sweeps = 0;
+  long solveLaunches = 0;
+  std::vector<DevScalars> hs;  // the planets' device scalars after a timed substep
+  std::string err;
+};
+
+namespace {
+
+int fail(ws_batch *b, const std::string &m, int code) {
+  b->err = m;
+  return code;
+}
+
+#define BH(x)                                                                               \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess) return fail(b, std::string(#x) + ": " + hipGetErrorString(e_), WS_EHIP); \
+  } while (0)
+
+int planet_fail(ws_batch *b, int i, int r) {
+  return fail(b, "planet " + std::to_string(i) + ": " + b->ctx[i]->err, r);
+}
+
+int check_planets(ws_batch *b) {
+  for (size_t i = 0; i < b->ctx.size(); i++) {
+    ws_ctx *c = b->ctx[i];
+    if (c->timing || c->recordSamples || c->pressureCompare)
+      return fail(b, "planet " + std::to_string(i) + ": timing, recordSamples and pressureCompare must be off",
+                  WS_ESTATE);
+    if (c->nC >= (1L << 26)) return fail(b, "planet " + std::to_string(i) + ": 2^26 cells or more", WS_EINVAL);
+  }
+  return WS_OK;
+}
+
+__global__ void k_batch_arm(int n, unsigned *__restrict__ arrive) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) arrive[2 * p] = 0u;
+}
+
+void launch_solves(ws_batch *b) {
+  int n = (int)b->ctx.size();
+  hipLaunchKernelGGL(k_batch_arm, dim3((n + 255) / 256), dim3(256), 0, b->stream, n, b->arrive);
+  if (b->threads == 256)
+    hipLaunchKernelGGL(k_sor_solve_batch<256>, dim3(b->nBlocks), dim3(256), 0, b->stream, b->jobs, b->blk);
+  else if (b->threads == 512)
+    hipLaunchKernelGGL(k_sor_solve_batch<512>, dim3(b->nBlocks), dim3(512), 0, b->stream, b->jobs, b->blk);
+  else
+    hipLaunchKernelGGL(k_sor_solve_batch<1024>, dim3(b->nBlocks), dim3(1024), 0, b->stream, b->jobs, b->blk);
+}
+
+// workgroups per planet from its cell count, the block table, the jobs
+int plan_solves(ws_batch *b) {
+  int n = (int)b->ctx.size();
+  if (n > b->maxGroups) return fail(b, "more planets than workgroups allowed for the joint solve", WS_EINVAL);
+  b->G.assign(n, 1);
+  int total = 0;
+  for (int i = 0; i < n; i++) {
+    double g = std::ceil((double)b->ctx[i]->nC / b->cellsPerGroup);
+    b->G[i] = g < 1 ? 1 : (g > 32 ? 32 : (int)g);
+    total += b->G[i];
+  }
+  while (total > b->maxGroups) {  // take from the planets with the most
+    int m = 0;
+    for (int i = 1; i < n; i++)
+      if (b->G[i] > b->G[m]) m = i;
+    b->G[m]--;
+    total--;
+  }
+  std::vector<int2> hb;
+  for (int i = 0; i < n; i++)
+    for (int w = 0; w < b->G[i]; w++) hb.push_back(make_int2(i, w));
+  if ((int)hb.size() > b->nBlocks) {
+    if (b->blk) (void)hipFree(b->blk);
+    b->blk = nullptr;
+    BH(hipMalloc((void **)&b->blk, sizeof(int2) * b->maxGroups));
+  }
+  b->nBlocks = (int)hb.size();
+  BH(hipMemcpy(b->blk, hb.data(), sizeof(int2) * hb.size(), hipMemcpyHostToDevice));
+  for (int i = 0; i < n; i++) {
+    b->hjobs[i] = batch_sor_job(b->ctx[i]);
+    b->hjobs[i].G = b->G[i];
+    b->hjobs[i].arrive = b->arrive + 2 * i;
+  }
+  BH(hipMemcpy(b->jobs, b->hjobs.data(), sizeof(SorJob) * n, hipMemcpyHostToDevice));
+  return WS_OK;
+}
+
+// one batched substep
+int enqueue_substep(ws_batch *b) {
+  int n = (int)b->ctx.size(), r;
+  BH(hipEventRecord(b->fork, b->stream));
+  for (int i = 0; i < n; i++) {
+    ws_ctx *c = b->ctx[i];
+    BH(hipStreamWaitEvent(c->stream, b->fork, 0));
+    if ((r = batch_substep_pre(c))) return planet_fail(b, i, r);
+    BH(hipEventRecord(b->pre[i], c->stream));
+    BH(hipStreamWaitEvent(b->stream, b->pre[i], 0));
+  }
+  if (b->timing) BH(hipEventRecord(b->t0, b->stream));
+  launch_solves(b);
+  BH(hipGetLastError());
+  if (b->timing) BH(hipEventRecord(b->t1, b->stream));
+  BH(hipEventRecord(b->solved, b->stream));
+  for (int i = 0; i < n; i++) {
+    ws_ctx *c = b->ctx[i];
+    BH(hipStreamWaitEvent(c->stream, b->solved, 0));
+    if ((r = batch_substep_post(c))) return planet_fail(b, i, r);
+    BH(hipEventRecord(b->post[i], c->stream));
+    BH(hipStreamWaitEvent(b->stream, b->post[i], 0));
+  }
+  return WS_OK;
+}
+
+// after a timed substep: the solve's duration and the cells it relaxed
+int account(ws_batch *b) {
+  BH(hipStreamSynchronize(b->stream));
+  float ms = 0;
+  BH(hipEventElapsedTime(&ms, b->t0, b->t1));
+  b->solveMs += ms;
+  b->solveLaunches++;
+  for (size_t i = 0; i < b->ctx.size(); i++) {
+    BH(hipMemcpy(&b->hs[i], b->ctx[i]->ds, sizeof(DevScalars), hipMemcpyDeviceToHost));
+    b->cellSweeps += (double)(b->hs[i].nRed + b->hs[i].nBlack) * b->hs[i].sorIters;
+    b->sweeps += b->hs[i].sorIters;
+  }
+  return WS_OK;
+}
+
+int finish(ws_batch *b) {
+  BH(hipStreamSynchronize(b->stream));
+  for (ws_ctx *c : b->ctx) batch_solved(c);
+  // a planet whose solve barrier timed out (its workgroups were not all
+  // resident): the flag is cleared and the run reported as failed
+  int n = (int)b->ctx.size();
+  std::vector<unsigned> st(2 * n);
+  BH(hipMemcpy(st.data(), b->arrive, sizeof(unsigned) * 2 * n, hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; i++)
+    if (st[2 * i + 1]) {
+      BH(hipMemset(b->arrive, 0, sizeof(unsigned) * 2 * n));
+      return fail(b, "planet " + std::to_string(i) + ": the joint pressure solve's barrier timed out", WS_ESTATE);
+    }
+  return WS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ws_batch_create(ws_batch **out, ws_ctx *const *ctx, int n, int use_graph, int threads) {
+  if (!out || !ctx || n < 1 || (threads != 256 && threads != 512 && threads != 1024)) return WS_EINVAL;
+  *out = nullptr;
+  for (int i = 0; i < n; i++)
+    if (!ctx[i]) return WS_EINVAL;
+  ws_batch *b = new ws_batch();
+  b->ctx.assign(ctx, ctx + n);
+  b->useGraph = use_graph != 0;
+  b->threads = threads;
+  b->pre.assign(n, nullptr);
+  b->post.assign(n, nullptr);
+  b->hjobs.resize(n);
+  b->hs.resize(n);
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->solved, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreate(&b->t0);
+  if (e == hipSuccess) e = hipEventCreate(&b->t1);
+  for (int i = 0; i < n && e == hipSuccess; i++) {
+    e = hipEventCreateWithFlags(&b->pre[i], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->post[i], hipEventDisableTiming);
+  }
+  // one resident 1024-thread workgroup per compute unit
+  int dev = 0, cus = 0;
+  if (e == hipSuccess) e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess && cus > 0 && cus < b->maxGroups) b->maxGroups = cus;
+  if (e == hipSuccess) e = hipMalloc((void **)&b->jobs, sizeof(SorJob) * n);
+  if (e == hipSuccess) e = hipMalloc((void **)&b->arrive, sizeof(unsigned) * 2 * n);
+  if (e == hipSuccess) e = hipMemset(b->arrive, 0, sizeof(unsigned) * 2 * n);
+  if (e != hipSuccess) {
+    ws_batch_destroy(b);
+    return WS_EHIP;
+  }
+  *out = b;
+  return WS_OK;
+}
+
+int ws_batch_run(ws_batch *b, int substeps) {
+  if (!b || substeps < 0) return WS_EINVAL;
+  int r;
+  if ((r = check_planets(b))) return r;
+  // the solves' descriptions as the contexts stand now (tolerances, caps,
+  // omega); every planet runs the one-workgroup solve
+  for (size_t i = 0; i < b->ctx.size(); i++) {
+    b->ctx[i]->sorMode = 4;
+    b->ctx[i]->sorThreads = b->threads;
+  }
+  if ((r = plan_solves(b))) return r;
+  int done = 0;
+  if (b->timing) {
+    for (; done < substeps; done++)
+      if ((r = enqueue_substep(b)) || (r = account(b))) return r;
+    b->warm = true;
+    return finish(b);
+  }
+  if (!b->useGraph) {
+    for (; done < substeps; done++)
+      if ((r = enqueue_substep(b))) return r;
+    if ((r = finish(b))) return r;
+    b->warm = b->warm || substeps > 0;
+    return WS_OK;
+  }
+  // the first substep of a batch runs eagerly: the contexts allocate their
+  // lazily created buffers (network activations) outside the capture
+  if (!b->warm && substeps > 0) {
+    if ((r = enqueue_substep(b)) || (r = finish(b))) return r;
+    b->warm = true;
+    done = 1;
+  }
+  if (done < substeps) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    BH(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    r = enqueue_substep(b);
+    hipError_t ec = hipStreamEndCapture(b->stream, &g);
+    if (r || ec != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      if (r) return r;
+      return fail(b, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec), WS_EHIP);
+    }
+    hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      return fail(b, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei), WS_EHIP);
+    }
+    hipError_t el = hipSuccess;
+    for (; done < substeps && el == hipSuccess; done++) el = hipGraphLaunch(x, b->stream);
+    hipError_t es = hipStreamSynchronize(b->stream);
+    (void)hipGraphExecDestroy(x);
+    (void)hipGraphDestroy(g);
+    if (el != hipSuccess) return fail(b, std::string("hipGraphLaunch: ") + hipGetErrorString(el), WS_EHIP);
+    if (es != hipSuccess) return fail(b, std::string("hipStreamSynchronize: ") + hipGetErrorString(es), WS_EHIP);
+  }
+  return finish(b);
+}
+
+int ws_batch_set(ws_batch *b, const char *name, double v) {
+  if (!b || !name) return WS_EINVAL;
+  if (!strcmp(name, "timing")) {
+    b->timing = v != 0;
+    b->solveMs = b->cellSweeps = b->sweeps = 0;
+    b->solveLaunches = 0;
+    return WS_OK;
+  }
+  if (!strcmp(name, "graph")) {
+    b->useGraph = v != 0;
+    return WS_OK;
+  }
+  if (!strcmp(name, "threads")) {
+    if (v != 256 && v != 512 && v != 1024) return fail(b, "threads must be 256, 512 or 1024", WS_EINVAL);
+    b->threads = (int)v;
+    return WS_OK;
+  }
+  if (!strcmp(name, "cellsPerGroup")) {
+    if (!(v >= 1)) return fail(b, "cellsPerGroup must be at least 1", WS_EINVAL);
+    b->cellsPerGroup = v;
+    return WS_OK;
+  }
+  if (!strcmp(name, "maxGroups")) {
+    if (!(v >= 1 && v <= 256)) return fail(b, "maxGroups must be 1..256", WS_EINVAL);
+    int dev = 0, cus = 0;
+    BH(hipGetDevice(&dev));
+    BH(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (v > cus) return fail(b, "maxGroups exceeds the compute units", WS_EINVAL);
+    b->maxGroups = (int)v;
+    return WS_OK;
+  }
+  return fail(b, std::string("unknown batch setting ") + name, WS_EINVAL);
+}
+
+int ws_batch_get(ws_batch *b, const char *name, double *v) {
+  if (!b || !name || !v) return WS_EINVAL;
+  if (!strcmp(name, "solveMs")) *v = b->solveMs;
+  else if (!strcmp(name, "solveLaunches")) *v = (double)b->solveLaunches;
+  else if (!strcmp(name, "cellSweeps")) *v = b->cellSweeps;
+  else if (!strcmp(name, "sweeps")) *v = b->sweeps;
+  else if (!strcmp(name, "threads")) *v = b->threads;
+  else if (!strcmp(name, "graph")) *v = b->useGraph;
+  else if (!strcmp(name, "timing")) *v = b->timing;
+  else if (!strcmp(name, "cellsPerGroup")) *v = b->cellsPerGroup;
+  else if (!strcmp(name, "maxGroups")) *v = b->maxGroups;
+  else if (!strcmp(name, "groups")) *v = b->nBlocks;
+  else return fail(b, std::string("unknown batch value ") + name, WS_EINVAL);
+  return WS_OK;
+}
+
+const char *ws_batch_last_error(const ws_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+void ws_batch_destroy(ws_batch *b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (hipEvent_t e : b->pre)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : b->post)
+    if (e) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->solved) (void)hipEventDestroy(b->solved);
+  if (b->t0) (void)hipEventDestroy(b->t0);
+  if (b->t1) (void)hipEventDestroy(b->t1);
+  if (b->jobs) (void)hipFree(b->jobs);
+  if (b->blk) (void)hipFree(b->blk);
+  if (b->arrive) (void)hipFree(b->arrive);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+}  // extern "C"

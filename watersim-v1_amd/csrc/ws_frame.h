@@ -24,4 +24,9 @@ int mt_build_dev(ws_ctx *c, const float *dd, int nx, int ny, int nz, double dx, 
                  long cap, long *count, double *volume);
 void metaballs_release(ws_ctx *c);
 World make_world(ws_ctx *c);
+// a device-dt-law substep split around its pressure sweeps (ws_api.hip)
+int batch_substep_pre(ws_ctx *c);
+int batch_substep_post(ws_ctx *c);
+SorJob batch_sor_job(ws_ctx *c);
+void batch_solved(ws_ctx *c);
 }  // namespace ws

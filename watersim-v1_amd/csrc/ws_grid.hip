@@ -424,9 +424,15 @@ __global__ void k_assemble(Geo g, const uint8_t *__restrict__ ct, const float *_
 
 // The relaxation of solver.js:1402, q(1 - omega) + omega/diag (rhs + sum),
 // with every rounding explicit so that all SOR kernels produce the same
-// bits for the same inputs (no contraction into FMAs).
+// bits for the same inputs. The operators are written here under the
+// pragma (HIP's __fmul_rn / __fadd_rn are plain operators defined elsewhere,
+// which the compiler may contract into an FMA differently in each kernel).
 __device__ __forceinline__ float sor_relax(float q, float sum, float rhs, float pw, float om1) {
-  return __fadd_rn(__fmul_rn(q, om1), __fmul_rn(pw, __fadd_rn(rhs, sum)));
+#pragma clang fp contract(off)
+  float keep = q * om1;
+  float src = rhs + sum;
+  float upd = pw * src;
+  return keep + upd;
 }
 
 // One colour of red-black SOR (the relaxation of solver.js:1402): cells with
@@ -834,6 +840,223 @@ __global__ void k_residual_final(const double *__restrict__ partials, int n, Dev
   }
   if (threadIdx.x == 0) { s->rr = sr[0]; s->bb = sb[0]; }
 }
+
+// ------------------------------------------------ on-device pressure solve
+// The whole to-residual solve of st_pressure_solve on the device: the
+// red-black list sweeps (the arithmetic of k_sor_list), the FP64 residual
+// of k_residual_list, and the host loop's stopping rule (checks every
+// `every` sweeps, adaptive check spacing from the observed rate, stop on
+// the tolerance, the sweep cap or twelve checks without a 0.5 % gain). A
+// solve takes G workgroups (G = 1: k_sor_solve, one context; k_sor_solve_batch
+// runs many planets' solves in one launch, G per planet). Workgroup w of G
+// relaxes the w-th slice of each colour list; the G workgroups of a planet
+// meet at a counter barrier after every half-sweep (agent-scope release /
+// acquire, a bounded number of polls; a planet whose barrier times out is
+// flagged and its solve abandoned).
+// Cells of one colour never neighbour each other, so a thread loads the
+// inputs of kSorU cells before it writes any of them; the values are those
+// k_sor_list computes, whatever G and the workgroup size.
+// The residual is summed in a fixed order that depends on neither: chunks of
+// kSorChunk list entries, each summed by 256 lanes (lane l takes entries l,
+// l + 256, ...; a wavefront butterfly, then the four wavefronts in order),
+// the chunk sums added in chunk order.
+constexpr int kSorU = 8;
+constexpr int kSorChunk = 2048;
+constexpr int kSorPolls = 1 << 20;
+
+// the G workgroups of a planet wait for each other (barrier number k)
+__device__ __forceinline__ bool sor_barrier(const SorJob &J, unsigned k) {
+  __syncthreads();
+  __shared__ int ok;
+  if (threadIdx.x == 0) {
+    ok = 1;
+    if (J.G > 1) {
+      __hip_atomic_fetch_add(J.arrive, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned want = k * (unsigned)J.G;
+      int polls = 0;
+      while (__hip_atomic_load(J.arrive, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
+        if (__hip_atomic_load(J.arrive + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++polls > kSorPolls) {
+          __hip_atomic_store(J.arrive + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+template <int NT>
+__device__ __forceinline__ void sor_half(const SorJob &J, long base, long lo, long hi, int nxy) {
+  const int nx = J.g.nx;
+  for (long t0 = lo + threadIdx.x; t0 < hi; t0 += (long)NT * kSorU) {
+    unsigned e[kSorU];
+    float rh[kSorU], pw[kSorU], sum[kSorU], qc[kSorU];
+#pragma unroll
+    for (int u = 0; u < kSorU; u++) {
+      long t = t0 + (long)u * NT;
+      bool in = t < hi;
+      e[u] = in ? J.code[base + t] : 0u;
+      rh[u] = in ? J.lrhs[base + t] : 0.f;
+      pw[u] = in ? J.lpw[base + t] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < kSorU; u++) {
+      int c = (int)(e[u] & 0x3ffffffu);
+      unsigned m = e[u] >> 26;
+      float a = 0.f;
+      if (m & 1) a += J.q[c - 1];
+      if (m & 2) a += J.q[c + 1];
+      if (m & 4) a += J.q[c - nx];
+      if (m & 8) a += J.q[c + nx];
+      if (m & 16) a += J.q[c - nxy];
+      if (m & 32) a += J.q[c + nxy];
+      sum[u] = a;
+      qc[u] = J.q[c];
+    }
+#pragma unroll
+    for (int u = 0; u < kSorU; u++)
+      if (t0 + (long)u * NT < hi) J.q[e[u] & 0x3ffffffu] = sor_relax(qc[u], sum[u], rh[u], pw[u], J.om1);
+  }
+}
+
+// one sweep (red, then black) by workgroup w of the planet's G
+template <int NT>
+__device__ __forceinline__ bool sor_sweep(const SorJob &J, int w, long nR, long nB, int nxy, unsigned &bar) {
+  sor_half<NT>(J, 0, nR * w / J.G, nR * (w + 1) / J.G, nxy);
+  if (!sor_barrier(J, ++bar)) return false;
+  sor_half<NT>(J, J.half, nB * w / J.G, nB * (w + 1) / J.G, nxy);
+  return sor_barrier(J, ++bar);
+}
+
+// sqrt(rr) / sqrt(bb) over both lists; every thread of every workgroup of
+// the planet returns the same value (false: a barrier timed out)
+template <int NT>
+__device__ bool sor_residual(const SorJob &J, int w, long nR, long nB, int nxy, unsigned &bar, double *res) {
+  __shared__ double wr[NT / 64], wb[NT / 64], tot;
+  const int nx = J.g.nx;
+  const long n = nR + nB;
+  const long nChunks = (n + kSorChunk - 1) / kSorChunk;
+  const int grp = threadIdx.x >> 8, lane = threadIdx.x & 255, wave = threadIdx.x >> 6;
+  for (long ch0 = (long)w * (NT / 256); ch0 < nChunks; ch0 += (long)J.G * (NT / 256)) {
+    const long ch = ch0 + grp;
+    double rr = 0, bb = 0;
+    if (ch < nChunks) {
+      const long end = min(n, (ch + 1) * kSorChunk);
+      for (long t = ch * kSorChunk + lane; t < end; t += 256) {
+        long pos = t < nR ? t : J.half + (t - nR);
+        unsigned e = J.code[pos];
+        int c = (int)(e & 0x3ffffffu);
+        unsigned m = e >> 26;
+        float a = 0.f;
+        if (m & 1) a += J.q[c - 1];
+        if (m & 2) a += J.q[c + 1];
+        if (m & 4) a += J.q[c - nx];
+        if (m & 8) a += J.q[c + nx];
+        if (m & 16) a += J.q[c - nxy];
+        if (m & 32) a += J.q[c + nxy];
+        double rh = (double)J.lrhs[pos];
+        double r = rh - ((double)J.ldg[pos] * (double)J.q[c] - (double)a);
+        rr += r * r;
+        bb += rh * rh;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      rr += __shfl_xor(rr, o);
+      bb += __shfl_xor(bb, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      wr[wave] = rr;
+      wb[wave] = bb;
+    }
+    __syncthreads();
+    if (lane == 0 && ch < nChunks) {
+      const int w0 = grp * 4;
+      J.chunk[2 * ch] = ((wr[w0] + wr[w0 + 1]) + wr[w0 + 2]) + wr[w0 + 3];
+      J.chunk[2 * ch + 1] = ((wb[w0] + wb[w0 + 1]) + wb[w0 + 2]) + wb[w0 + 3];
+    }
+    __syncthreads();
+  }
+  if (!sor_barrier(J, ++bar)) return false;
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0;
+    for (long ch = 0; ch < nChunks; ch++) {
+      a += __hip_atomic_load(J.chunk + 2 * ch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      b += __hip_atomic_load(J.chunk + 2 * ch + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    tot = b > 0 ? sqrt(a) / sqrt(b) : 0.0;
+  }
+  __syncthreads();
+  *res = tot;
+  // the chunk sums are rewritten by the next evaluation only after every
+  // workgroup has read them
+  return sor_barrier(J, ++bar);
+}
+
+template <int NT>
+__device__ void sor_solve_part(const SorJob &J, int w) {
+  const long nR = J.ds->nRed, nB = J.ds->nBlack;
+  const int nxy = J.g.nx * J.g.ny;
+  unsigned done = 0, checks = 0, bar = 0;
+  double res = 0;
+  bool ok = true;
+  if (J.tol <= 0) {
+    for (int s = 0; s < J.fixedIters && ok; s++) ok = sor_sweep<NT>(J, w, nR, nB, nxy, bar);
+    done = (unsigned)max(J.fixedIters, 0);
+    if (ok && J.measure) { ok = sor_residual<NT>(J, w, nR, nB, nxy, bar, &res); checks++; }
+  } else {
+    ok = sor_residual<NT>(J, w, nR, nB, nxy, bar, &res);
+    checks++;
+    double best = res, prev = res;
+    int flat = 0, k = J.every;
+    while (ok && res > J.tol && (int)done < J.maxIters && flat < 12) {
+      if ((int)done + k > J.maxIters) k = J.maxIters - (int)done;
+      for (int s = 0; s < k && ok; s++) ok = sor_sweep<NT>(J, w, nR, nB, nxy, bar);
+      done += (unsigned)k;
+      if (!ok) break;
+      ok = sor_residual<NT>(J, w, nR, nB, nxy, bar, &res);
+      checks++;
+      if (res < 0.995 * best) { best = res; flat = 0; } else flat++;
+      int kn = J.every;
+      if (J.adaptive && res < prev && res > J.tol) {
+        double rate = pow(res / prev, 1.0 / k);  // per sweep
+        double need = log(J.tol / res) / log(rate);
+        kn = need < 4 ? 4 : (need > 64 ? 64 : (int)ceil(need));
+      }
+      prev = res;
+      k = kn;
+    }
+  }
+  if (w == 0 && threadIdx.x == 0) {
+    J.ds->sorIters = done;
+    J.ds->sorChecksLast = checks;
+    J.ds->sorRes = ok ? res : -1.0;
+    J.ds->sorSweeps += done;
+    J.ds->sorChecks += checks;
+  }
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_sor_solve(SorJob J) {
+  sor_solve_part<NT>(J, 0);
+}
+
+// block b works on planet blk[b].x as its workgroup blk[b].y
+template <int NT>
+__global__ __launch_bounds__(NT) void k_sor_solve_batch(const SorJob *__restrict__ jobs, const int2 *__restrict__ blk) {
+  const int2 pw = blk[blockIdx.x];
+  const SorJob J = jobs[pw.x];
+  sor_solve_part<NT>(J, pw.y);
+}
+
+template __global__ void k_sor_solve<256>(SorJob);
+template __global__ void k_sor_solve<512>(SorJob);
+template __global__ void k_sor_solve<1024>(SorJob);
+template __global__ void k_sor_solve_batch<256>(const SorJob *, const int2 *);
+template __global__ void k_sor_solve_batch<512>(const SorJob *, const int2 *);
+template __global__ void k_sor_solve_batch<1024>(const SorJob *, const int2 *);
 
 // ---------------------------------------------------------- projection
 // _projectSlice, solver.js:1417-1443, with the grid clamp and umax of

@@ -46,6 +46,11 @@ struct DevScalars {
   float dtLast;
   float pad;
   double dtD;          // the substep length as the reference holds it (double)
+  // the on-device solve (k_sor_solve): sweeps and residual of the last
+  // solve, running totals of sweeps and residual evaluations
+  unsigned sorIters, sorChecksLast;
+  double sorRes;
+  unsigned long long sorSweeps, sorChecks;
 };
 
 // Geometry and per-substep constants passed by value to kernels.
@@ -56,6 +61,24 @@ struct Geo {
   float dx_lo;              // double dx minus its float rounding
   float cx, cy, cz;         // domain centre (float)
   float cx_lo, cy_lo, cz_lo;  // double centre minus its float rounding
+};
+
+// One pressure solve run to its end by one workgroup (k_sor_solve): the
+// fluid lists of a context, its q, and the stopping rule of the host loop
+struct SorJob {
+  Geo g;
+  DevScalars *ds;
+  const unsigned *code;
+  const float *lrhs, *lpw;
+  const uint8_t *ldg;
+  float *q;
+  long half;
+  float om1;
+  int every, maxIters, adaptive, fixedIters, measure;
+  double tol;        // <= 0: exactly fixedIters sweeps
+  double *chunk;     // residual chunk sums, 2 x ceil(nC / 2048)
+  unsigned *arrive;  // G > 1: barrier arrivals, then a timed-out flag
+  int G;             // workgroups sharing this solve
 };
 
 struct Net;       // the learned warm start (ws_learn.h)
@@ -151,10 +174,16 @@ struct Ctx {
   long tbLaunchesPending = 0;
   int sorTiles[3] = {0, 0, 0};
   // 2: one launch per colour over the fluid-cell lists (the fastest
-  // measured), 3: two sweeps per launch over LDS tiles (temporal blocking),
-  // 1: one LDS-tiled sweep per launch, 0: one launch per colour over the
-  // whole grid. All four give the same bits.
+  // measured at 128^3), 3: two sweeps per launch over LDS tiles (temporal
+  // blocking), 1: one LDS-tiled sweep per launch, 0: one launch per colour
+  // over the whole grid. All give the same bits for the same sweeps.
+  // 4: the whole solve in one workgroup (k_sor_solve, sorThreads threads),
+  // residual checks on the device, no host round trips
   int sorMode = 2;
+  int sorThreads = 1024;
+  double *sorChunk = nullptr;  // residual chunk sums of the on-device solve
+  unsigned long long sorSweepsSeen = 0, sorChecksSeen = 0;  // device totals folded into the host counters
+  bool devStatsPending = false;
   unsigned *cellCount = nullptr, *cellStart = nullptr;
   float *vort = nullptr;  // 10*C scratch for vorticity confinement
   // faces (u | v | w contiguous in one allocation each)

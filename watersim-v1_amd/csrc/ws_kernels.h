@@ -75,6 +75,10 @@ __global__ void k_sor_tb(Geo g, int tilesX, int tilesY, const uint8_t *active, c
                          const float *rhs, const float *pw, const uint8_t *nbm, float om1);
 __global__ void k_sor_tb_mark(Geo g, int tilesX, int tilesY, int nTiles, const uint8_t *nbm, uint8_t *active);
 __global__ void k_residual_final(const double *partials, int n, DevScalars *s);
+template <int NT>
+__global__ void k_sor_solve(SorJob J);
+template <int NT>
+__global__ void k_sor_solve_batch(const SorJob *jobs, const int2 *blk);
 __global__ void k_project(Geo g, const uint8_t *ct, const float *phi, float *vel, int do_clamp, float vcap,
                           DevScalars *s);
 __global__ void k_clamp(long F, float *vel, float vcap, DevScalars *s);

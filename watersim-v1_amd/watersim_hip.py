@@ -26,7 +26,9 @@ EXPORTS = ["ws_create", "ws_destroy", "ws_last_error", "ws_set_f", "ws_get_f", "
            "ws_substep", "ws_stage", "ws_step_frame", "ws_run_substeps", "ws_splat_density", "ws_mt_build",
            "ws_autotune_iso", "ws_kinetic_energy", "ws_sync", "ws_world_build", "ws_world_free",
            "ws_net_param_count", "ws_net_set_params", "ws_net_get_params", "ws_net_forward", "ws_net_grad",
-           "ws_net_bank", "ws_net_bank_count", "ws_net_train", "ws_set_balls", "ws_get_balls", "ws_metaballs_build", "ws_read_render", "ws_tune"]
+           "ws_net_bank", "ws_net_bank_count", "ws_net_train", "ws_set_balls", "ws_get_balls", "ws_metaballs_build", "ws_read_render", "ws_tune",
+           "ws_batch_create", "ws_batch_run", "ws_batch_set", "ws_batch_get", "ws_batch_last_error",
+           "ws_batch_destroy"]
 
 
 class RenderView(ctypes.Structure):
@@ -107,8 +109,16 @@ def load(path=LIB_PATH):
     L.ws_tune.argtypes = [vp, ci, ctypes.c_char_p, sz]
     L.ws_set_balls.argtypes = [vp, ci, vp]
     L.ws_get_balls.argtypes = [vp, ci, vp]
+    L.ws_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), ci, ci, ci]
+    L.ws_batch_run.argtypes = [vp, ci]
+    L.ws_batch_set.argtypes = [vp, cs, d]
+    L.ws_batch_get.argtypes = [vp, cs, ctypes.POINTER(d)]
+    L.ws_batch_last_error.argtypes = [vp]
+    L.ws_batch_last_error.restype = cs
+    L.ws_batch_destroy.argtypes = [vp]
+    L.ws_batch_destroy.restype = None
     for f in EXPORTS:
-        if f not in ("ws_destroy", "ws_last_error", "ws_world_free"):
+        if f not in ("ws_destroy", "ws_last_error", "ws_world_free", "ws_batch_last_error", "ws_batch_destroy"):
             getattr(L, f).restype = ci
     L.ws_net_param_count.restype = ctypes.c_long
     _lib = L
@@ -378,6 +388,49 @@ class Solver:
         out = ctypes.c_double()
         self._check(self.L.ws_kinetic_energy(self.ctx, ctypes.byref(out)), "kinetic_energy")
         return out.value
+
+
+class Batch:
+    """Planets advanced together (ws_batch_*): each planet's stages on its
+    own stream, all pressure solves in one launch, optionally replayed from
+    a HIP graph. The solvers stay owned by the caller."""
+
+    def __init__(self, solvers, graph=True, threads=1024):
+        self.L = load()
+        self.solvers = list(solvers)
+        arr = (ctypes.c_void_p * len(self.solvers))(*[s.ctx.value for s in self.solvers])
+        self.b = ctypes.c_void_p()
+        rc = self.L.ws_batch_create(ctypes.byref(self.b), arr, len(self.solvers), int(bool(graph)), int(threads))
+        if rc != 0:
+            raise WatersimError("ws_batch_create failed (%d)" % rc)
+
+    def run(self, substeps):
+        self._check(self.L.ws_batch_run(self.b, int(substeps)), "batch run")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.L.ws_batch_last_error(self.b)
+            raise WatersimError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def set(self, **kw):
+        for k, v in kw.items():
+            self._check(self.L.ws_batch_set(self.b, k.encode(), float(v)), "batch set " + k)
+
+    def get(self, k):
+        v = ctypes.c_double()
+        self._check(self.L.ws_batch_get(self.b, k.encode(), ctypes.byref(v)), "batch get " + k)
+        return v.value
+
+    def close(self):
+        if self.b:
+            self.L.ws_batch_destroy(self.b)
+            self.b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def build_world(n, dx, coreR, depth, seed, target, bumpiness=1.0):
