@@ -1253,7 +1253,11 @@ __global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restri
   // and one plane back, which decided the same li one step earlier
   float lastV = 0.f;
   int lastF = 0;
-  const float rcp[7] = {0.f, 1.f, 0.5f, 1.f / 3.f, 0.25f, 0.2f, 1.f / 6.f};
+  // 1/cnt from registers (an indexed constant array would be a global
+  // load on every step of the dependency chain)
+  auto rcp = [](int n) -> float {
+    return n == 1 ? 1.f : (n == 2 ? 0.5f : (n == 3 ? 1.f / 3.f : (n == 4 ? 0.25f : (n == 5 ? 0.2f : 1.f / 6.f))));
+  };
   for (int step = 0; step < TI + TJ + TK - 2; step++) {
     int li = 1 + step - (lj - 1) - (lk - 1);
     int i = i0 + li;
@@ -1282,7 +1286,7 @@ __global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restri
       if (!f0 && cnt > 0) {
         // the mean through a reciprocal (within 1.5 ulp of the quotient; the
         // reference divides in double and rounds, so neither is exact here)
-        float v = sum * rcp[cnt];
+        float v = sum * rcp(cnt);
         long f = ((long)k * d1 + j) * d0 + i;
         A[f] = v;
         V[f] = 1;

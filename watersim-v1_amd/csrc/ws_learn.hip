@@ -566,10 +566,23 @@ __global__ void k_ws_residual(NetDims d, const uint8_t *__restrict__ ct, const u
   }
   if (threadIdx.x == 0) { part[3 * blockIdx.x] = sa[0]; part[3 * blockIdx.x + 1] = sb[0]; part[3 * blockIdx.x + 2] = sc[0]; }
 }
-__global__ void k_ws_scales(int nb, const double *__restrict__ part, float *__restrict__ scales) {
-  if (threadIdx.x != 0) return;
+// the block partials summed in a fixed order by 256 threads (thread t takes
+// partials t, t + 256, ... in turn, then a pairwise tree); nb <= 1024
+__global__ __launch_bounds__(256) void k_ws_scales(int nb, const double *__restrict__ part,
+                                                   float *__restrict__ scales) {
+  __shared__ double sa[256], sb[256], sn[256];
   double a = 0, b = 0, n = 0;
-  for (int i = 0; i < nb; i++) { a += part[3 * i]; b += part[3 * i + 1]; n += part[3 * i + 2]; }
+  for (int i = threadIdx.x; i < nb; i += 256) { a += part[3 * i]; b += part[3 * i + 1]; n += part[3 * i + 2]; }
+  sa[threadIdx.x] = a; sb[threadIdx.x] = b; sn[threadIdx.x] = n;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      sa[threadIdx.x] += sa[threadIdx.x + o]; sb[threadIdx.x] += sb[threadIdx.x + o]; sn[threadIdx.x] += sn[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  a = sa[0]; b = sb[0]; n = sn[0];
   double s = n > 0 ? sqrt(a / n) : 1.0, sq = n > 0 ? sqrt(b / n) : 1.0;
   scales[0] = s > 0 ? (float)s : 1.f;
   scales[1] = sq > 0 ? (float)sq : 1.f;
@@ -943,7 +956,7 @@ static int solver_inputs(ws_ctx *c) {
   long C = c->nC;
   hipLaunchKernelGGL(k_ws_residual, dim3(1024), dim3(256), 0, c->stream, d, c->cellType, c->nbm, c->dg, c->rhs, c->q,
                      n->r0, n->rpart);
-  hipLaunchKernelGGL(k_ws_scales, dim3(1), dim3(64), 0, c->stream, 1024, n->rpart, n->scales);
+  hipLaunchKernelGGL(k_ws_scales, dim3(1), dim3(256), 0, c->stream, 1024, n->rpart, n->scales);
   hipLaunchKernelGGL(k_ws_inputs, dim3(blocks_for(C, 256)), dim3(256), 0, c->stream, C, c->cellType, n->r0, c->q,
                      n->scales, n->x0);
   WS_LAUNCHED();
