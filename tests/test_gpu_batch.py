@@ -67,6 +67,22 @@ def test_device_solve_equals_list_sweeps(adaptive):
     b.close()
 
 
+@pytest.mark.parametrize("groups", [4, 16])
+def test_device_solve_split_over_workgroups(groups):
+    """One context's solve over several workgroups meeting at barriers gives
+    the bits of the list sweeps (and of the one-workgroup solve)."""
+    a, b = planet(48, 4), planet(48, 4)
+    a.set(sorMode=2)
+    b.set(sorMode=4, sorGroups=groups)
+    for step in range(3):
+        a.run_substeps(1)
+        b.run_substeps(1)
+        assert a.get("pressureIters_done") == b.get("pressureIters_done")
+        assert_same(a, b, "substep %d" % step)
+    a.close()
+    b.close()
+
+
 def test_device_solve_fixed_sweeps():
     a, b = planet(32, 2), planet(32, 2)
     for s, mode in ((a, 2), (b, 4)):

@@ -281,7 +281,23 @@ SorJob sor_job(ws_ctx *c) {
 int sor_solve_dev(ws_ctx *c) {
   SorJob J = sor_job(c);
   if (c->timing) WS_HIP(hipEventRecord(c->ev[0], c->stream));
-  if (c->sorThreads == 256)
+  if (c->sorGroups > 1) {
+    // several workgroups meeting at a barrier: all must be resident, so at
+    // most one per compute unit (a barrier that times out fails the solve)
+    if (!c->sorArrive) {
+      WS_HIP(hipMalloc((void **)&c->sorArrive, 2 * sizeof(unsigned)));
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->sorArrive, 2L);
+    }
+    J.G = c->sorGroups;
+    J.arrive = c->sorArrive;
+    hipLaunchKernelGGL(k_sor_arm, dim3(1), dim3(64), 0, c->stream, 1, c->sorArrive);
+    if (c->sorThreads == 256)
+      hipLaunchKernelGGL(k_sor_solve_multi<256>, dim3(J.G), dim3(256), 0, c->stream, J);
+    else if (c->sorThreads == 512)
+      hipLaunchKernelGGL(k_sor_solve_multi<512>, dim3(J.G), dim3(512), 0, c->stream, J);
+    else
+      hipLaunchKernelGGL(k_sor_solve_multi<1024>, dim3(J.G), dim3(1024), 0, c->stream, J);
+  } else if (c->sorThreads == 256)
     hipLaunchKernelGGL(k_sor_solve<256>, dim3(1), dim3(256), 0, c->stream, J);
   else if (c->sorThreads == 512)
     hipLaunchKernelGGL(k_sor_solve<512>, dim3(1), dim3(512), 0, c->stream, J);
@@ -310,6 +326,10 @@ int fold_dev_stats(ws_ctx *c) {
   WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
   WS_HIP(hipStreamSynchronize(c->stream));
   c->devStatsPending = false;
+  if (c->hs->sorRes < 0) {
+    c->err = "on-device pressure solve: its workgroups' barrier timed out (not all resident)";
+    return WS_ESTATE;
+  }
   c->itersDone = (int)c->hs->sorIters;
   c->residual = c->hs->sorRes;
   c->sweepsTotal += (long)(c->hs->sorSweeps - c->sorSweepsSeen);
@@ -844,7 +864,7 @@ void ws_destroy(ws_ctx *c) {
   balls_release(c);
   metaballs_release(c);
   void *ptrs[] = {c->staticTypes, c->gu, c->thr, c->terrainR, c->cellType, c->nbm, c->dg, c->q, c->q2, c->rhs, c->pw,
-                  c->sorActive, c->tbActive, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->sorChunk,
+                  c->sorActive, c->tbActive, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->sorChunk, c->sorArrive,
                   c->cellCount, c->cellStart, c->vort, c->vel, c->velO, c->wgt, c->scr, c->scr2, c->valid, c->dens,
                   c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pT, c->pAir, c->advU, c->advV, c->advW, c->pflag,
                   c->pcool, c->pcell, c->prank, c->sortedIdx, c->vneed, c->vscr, c->rng, c->ds, c->partials,
@@ -880,6 +900,14 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
     if (m < 0 || m > 4 || m != v) { c->err = "sorMode must be 0..4"; return WS_EINVAL; }
     if (m >= 2 && c->nC >= (1L << 26)) { c->err = "sorMode 2..4 need fewer than 2^26 cells"; return WS_EINVAL; }
     c->sorMode = m;
+    return WS_OK;
+  }
+  if (!strcmp(n, "sorGroups")) {
+    int dev = 0, cus = 0;
+    WS_HIP(hipGetDevice(&dev));
+    WS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (!(v >= 1 && v <= cus) || v != (int)v) { c->err = "sorGroups must be 1 .. the compute units"; return WS_EINVAL; }
+    c->sorGroups = (int)v;
     return WS_OK;
   }
   if (!strcmp(n, "sorThreads")) {
@@ -936,7 +964,7 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal) GF(residualsTotal) GF(ticksTotal) GF(sorMode) GF(sorCellSweeps)
   GF(cmpMsPlain) GF(cmpMsLearned) GF(cmpSweepsPlain) GF(cmpSweepsLearned) GF(cmpSolves) GF(warmStart) GF(netMs) GF(netCalls)
   GF(vaporCount) GF(nightVaporCount) GF(contactOverflow) GF(nBalls) GF(mbM) GF(vaporBigGroups) GF(vaporGroupMax)
-  GF(vaporBigParcels) GF(netTiled) GF(sorThreads)
+  GF(vaporBigParcels) GF(netTiled) GF(sorThreads) GF(sorGroups)
   GF(heatK) GF(Tamb) GF(shadeCool) GF(sunActivity) GF(evapIntensity) GF(snowT) GF(meltT) GF(cloudP) GF(currents)
   GF(simTime) GF(heatAcc) GF(evapCount) GF(cloudT) GF(rainT) GF(rainLiftAge)
 #undef GF

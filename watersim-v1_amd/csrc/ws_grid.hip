@@ -1051,6 +1051,21 @@ __global__ __launch_bounds__(NT) void k_sor_solve_batch(const SorJob *__restrict
   sor_solve_part<NT>(J, pw.y);
 }
 
+// one context's solve over gridDim.x workgroups (J.G == gridDim.x)
+template <int NT>
+__global__ __launch_bounds__(NT) void k_sor_solve_multi(SorJob J) {
+  sor_solve_part<NT>(J, blockIdx.x);
+}
+
+// the barrier counters of solves about to start (flags stay set)
+__global__ void k_sor_arm(int n, unsigned *__restrict__ arrive) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) arrive[2 * p] = 0u;
+}
+
+template __global__ void k_sor_solve_multi<256>(SorJob);
+template __global__ void k_sor_solve_multi<512>(SorJob);
+template __global__ void k_sor_solve_multi<1024>(SorJob);
 template __global__ void k_sor_solve<256>(SorJob);
 template __global__ void k_sor_solve<512>(SorJob);
 template __global__ void k_sor_solve<1024>(SorJob);
@@ -1244,58 +1259,69 @@ __global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restri
   }
   __syncthreads();
   if (threadIdx.x >= 64) return;
-  int lj = 1 + (threadIdx.x % TJ), lk = 1 + (threadIdx.x / TJ);
+  const int lane = threadIdx.x;
+  int lj = 1 + (lane % TJ), lk = 1 + (lane / TJ);
   int j = j0 + lj, k = k0 + lk;
   bool row = j < d1 && k < d2;
   // the LDS copy stays the original state; a face's final state travels in
   // registers: a lane keeps what it decided at the previous step (its -i
-  // neighbour now) and reads the -j / -k neighbours from the lanes one row
-  // and one plane back, which decided the same li one step earlier
+  // neighbour now) and takes the -j / -k neighbours from the lanes one row
+  // and one plane back, which decided the same li one step earlier (-j by a
+  // DPP row shift, -k by a lane permute, the flags of all lanes by a ballot).
+  // Every LDS read is unconditional (clamped index), so a step has no
+  // branches but the store.
   float lastV = 0.f;
   int lastF = 0;
-  // 1/cnt from registers (an indexed constant array would be a global
-  // load on every step of the dependency chain)
-  auto rcp = [](int n) -> float {
-    return n == 1 ? 1.f : (n == 2 ? 0.5f : (n == 3 ? 1.f / 3.f : (n == 4 ? 0.25f : (n == 5 ? 0.2f : 1.f / 6.f))));
-  };
+  const float vaH = sv[lk][lj][0];
+  const int faH = sf[lk][lj][0];
+  const bool hasJ = lj > 1, hasK = lk > 1;
+  const bool jlo = j > 0, jhi = j < d1 - 1, klo = k > 0, khi = k < d2 - 1;
+  const long rowBase = ((long)k * d1 + j) * d0;
   for (int step = 0; step < TI + TJ + TK - 2; step++) {
     int li = 1 + step - (lj - 1) - (lk - 1);
     int i = i0 + li;
-    float vjm = __shfl_up(lastV, 1), vkm = __shfl_up(lastV, TJ);
-    int fjm = __shfl_up(lastF, 1), fkm = __shfl_up(lastF, TJ);
-    if (row && li >= 1 && li <= TI && i < d0) {
-      uint8_t f0 = sf[lk][lj][li];
-      uint8_t fb = sf[lk][lj][li + 1], fd = sf[lk][lj + 1][li], ff = sf[lk + 1][lj][li];
-      float v0 = sv[lk][lj][li], vb = sv[lk][lj][li + 1], vd = sv[lk][lj + 1][li], vf = sv[lk + 1][lj][li];
-      float va = li > 1 ? lastV : sv[lk][lj][0];
-      int fa = li > 1 ? lastF : sf[lk][lj][0];
-      float vc = lj > 1 ? vjm : sv[lk][0][li];
-      int fc = lj > 1 ? fjm : sf[lk][0][li];
-      float ve = lk > 1 ? vkm : sv[0][lj][li];
-      int fe = lk > 1 ? fkm : sf[0][lj][li];
-      bool ua = i > 0 && fa, ub = i < d0 - 1 && fb, uc = j > 0 && fc;
-      bool ud = j < d1 - 1 && fd, ue = k > 0 && fe, uf = k < d2 - 1 && ff;
-      float sum = 0.f;
-      if (ua) sum += va;
-      if (ub) sum += vb;
-      if (uc) sum += vc;
-      if (ud) sum += vd;
-      if (ue) sum += ve;
-      if (uf) sum += vf;
-      int cnt = (int)ua + (int)ub + (int)uc + (int)ud + (int)ue + (int)uf;
-      if (!f0 && cnt > 0) {
-        // the mean through a reciprocal (within 1.5 ulp of the quotient; the
-        // reference divides in double and rounds, so neither is exact here)
-        float v = sum * rcp(cnt);
-        long f = ((long)k * d1 + j) * d0 + i;
-        A[f] = v;
-        V[f] = 1;
-        lastV = v;
-        lastF = 1;
-      } else {
-        lastV = v0;
-        lastF = f0 ? 1 : 0;
-      }
+    // lane - 1 (same 16-lane row whenever lj > 1) and lane - TJ
+    float vjm = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(lastV), 0x111, 0xf, 0xf, false));
+    float vkm = __shfl_up(lastV, TJ);
+    unsigned long long fm = __ballot(lastF);
+    int fjm = (int)((fm >> (lane > 0 ? lane - 1 : 0)) & 1ull);
+    int fkm = (int)((fm >> (lane >= TJ ? lane - TJ : 0)) & 1ull);
+    bool act = row && li >= 1 && li <= TI && i < d0;
+    int lc = li < 1 ? 1 : (li > TI ? TI : li);
+    int f0 = sf[lk][lj][lc], fb = sf[lk][lj][lc + 1], fd = sf[lk][lj + 1][lc], ff = sf[lk + 1][lj][lc];
+    float v0 = sv[lk][lj][lc], vb = sv[lk][lj][lc + 1], vd = sv[lk][lj + 1][lc], vf = sv[lk + 1][lj][lc];
+    float vcH = sv[lk][0][lc], veH = sv[0][lj][lc];
+    int fcH = sf[lk][0][lc], feH = sf[0][lj][lc];
+    float va = li > 1 ? lastV : vaH;
+    int fa = li > 1 ? lastF : faH;
+    float vc = hasJ ? vjm : vcH;
+    int fc = hasJ ? fjm : fcH;
+    float ve = hasK ? vkm : veH;
+    int fe = hasK ? fkm : feH;
+    bool ua = i > 0 && fa, ub = i < d0 - 1 && fb, uc = jlo && fc;
+    bool ud = jhi && fd, ue = klo && fe, uf = khi && ff;
+    float sum = 0.f;
+    sum = ua ? sum + va : sum;
+    sum = ub ? sum + vb : sum;
+    sum = uc ? sum + vc : sum;
+    sum = ud ? sum + vd : sum;
+    sum = ue ? sum + ve : sum;
+    sum = uf ? sum + vf : sum;
+    int cnt = (int)ua + (int)ub + (int)uc + (int)ud + (int)ue + (int)uf;
+    bool fill = act && !f0 && cnt > 0;
+    // the mean through the correctly rounded reciprocal of the count (the
+    // product is within 1.5 ulp of the quotient; the reference divides in
+    // double and rounds, so neither is exact here). The reciprocal is
+    // computed, not looked up: an indexed constant array would be a global
+    // load on every step of the dependency chain.
+    float v = sum * (1.f / (float)(cnt > 0 ? cnt : 1));
+    if (fill) {
+      A[rowBase + i] = v;
+      V[rowBase + i] = 1;
+    }
+    if (act) {
+      lastV = fill ? v : v0;
+      lastF = fill ? 1 : (f0 ? 1 : 0);
     }
   }
 }

@@ -182,6 +182,8 @@ struct Ctx {
   int sorMode = 2;
   int sorThreads = 1024;
   double *sorChunk = nullptr;  // residual chunk sums of the on-device solve
+  int sorGroups = 1;           // workgroups of one context's on-device solve
+  unsigned *sorArrive = nullptr;  // their barrier counter and timed-out flag
   unsigned long long sorSweepsSeen = 0, sorChecksSeen = 0;  // device totals folded into the host counters
   bool devStatsPending = false;
   unsigned *cellCount = nullptr, *cellStart = nullptr;
