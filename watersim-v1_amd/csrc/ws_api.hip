@@ -627,7 +627,17 @@ int substep_pre(ws_ctx *c, double dt, int use_law) {
 
 // the substep after the pressure solve: projection, confinement, clamp,
 // extrapolation, G2P and advection, balls
+int substep_post1(ws_ctx *c);
+int substep_post2(ws_ctx *c);
+
 int substep_post(ws_ctx *c) {
+  int r;
+  if ((r = substep_post1(c)) || (r = st_extrapolate(c)) || (r = substep_post2(c))) return r;
+  return WS_OK;
+}
+
+// projection, confinement and clamp
+int substep_post1(ws_ctx *c) {
   int r;
   bool vort = c->vorticity > 0;
   if ((r = st_project(c, vort ? 0 : 1))) return r;
@@ -635,7 +645,12 @@ int substep_post(ws_ctx *c) {
     if ((r = st_vorticity(c))) return r;
     if ((r = st_clamp(c))) return r;
   }
-  if ((r = st_extrapolate(c))) return r;
+  return WS_OK;
+}
+
+// after the extrapolation: G2P and advection, balls, the substep's end
+int substep_post2(ws_ctx *c) {
+  int r;
   if ((r = st_particles(c, 3))) return r;
   if ((r = balls_update(c))) return r;
   hipLaunchKernelGGL(k_end_substep, dim3(1), dim3(64), 0, c->stream, c->ds);
@@ -710,7 +725,9 @@ int batch_substep_pre(ws_ctx *c) {
   int r = substep_pre(c, 0.01, 1);
   return r ? r : pressure_setup(c);
 }
-int batch_substep_post(ws_ctx *c) { return substep_post(c); }
+int batch_substep_post1(ws_ctx *c) { return substep_post1(c); }
+int batch_substep_post2(ws_ctx *c) { return substep_post2(c); }
+ExPlanet batch_ex_planet(ws_ctx *c) { return ExPlanet{c->geo, c->vel, c->valid}; }
 SorJob batch_sor_job(ws_ctx *c) { return sor_job(c); }
 void batch_solved(ws_ctx *c) { c->devStatsPending = true; }
 }  // namespace ws
@@ -850,6 +867,7 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
     }
     if (hipMalloc(&c->exTiles, flat.size() * sizeof(int4)) != hipSuccess) { ws_destroy(c); return WS_EHIP; }
     hipMemcpy(c->exTiles, flat.data(), flat.size() * sizeof(int4), hipMemcpyHostToDevice);
+    c->exTilesHost = flat;
   }
   if (hipDeviceSynchronize() != hipSuccess) { ws_destroy(c); return WS_EHIP; }
   *out = c;

@@ -1,9 +1,10 @@
 // Batched planets: many contexts advanced together by device-dt-law
-// substeps. Each planet's stages before and after the pressure sweeps are
-// queued on its own stream; between them one launch of k_sor_solve_batch
-// runs every planet's whole pressure solve (one workgroup per planet, the
-// residual checks on the device), so a substep costs no host round trip and
-// the solves of all planets run side by side. With a graph, the batched
+// substeps. Each planet's stages are queued on its own stream, except two
+// that run for all planets at once on the batch stream: the pressure solve
+// (one launch of k_sor_solve_batch: every planet's whole solve, a few
+// workgroups each, the residual checks on the device) and the extrapolation
+// wavefront (one launch per tile diagonal over every planet's tiles of that
+// diagonal). A substep costs no host round trip. With a graph, the batched
 // substep (fork to the planets' streams, the joint solve, join) is captured
 // once per run and replayed for every substep.
 #include <cmath>
@@ -33,6 +34,14 @@ struct ws_batch {
   std::vector<int> G;
   int2 *blk = nullptr;
   unsigned *arrive = nullptr;
+  // the planets' extrapolation schedules merged by tile diagonal: one launch
+  // per diagonal covers every planet's tiles of that diagonal
+  ExPlanet *exPlanets = nullptr;
+  int4 *exTiles = nullptr;
+  int *exPlanetOf = nullptr;
+  std::vector<int> exDiagOff;
+  std::vector<hipEvent_t> mid;
+  hipEvent_t extrapolated = nullptr;
   // measurement: with timing on, substeps run eagerly with HIP events around
   // the joint solve launch and a synchronization after each
   int timing = 0;
@@ -143,9 +152,59 @@ int enqueue_substep(ws_batch *b) {
   for (int i = 0; i < n; i++) {
     ws_ctx *c = b->ctx[i];
     BH(hipStreamWaitEvent(c->stream, b->solved, 0));
-    if ((r = batch_substep_post(c))) return planet_fail(b, i, r);
+    if ((r = batch_substep_post1(c))) return planet_fail(b, i, r);
+    BH(hipEventRecord(b->mid[i], c->stream));
+    BH(hipStreamWaitEvent(b->stream, b->mid[i], 0));
+  }
+  for (size_t d = 0; d + 1 < b->exDiagOff.size(); d++) {
+    int nt = b->exDiagOff[d + 1] - b->exDiagOff[d];
+    if (nt > 0)
+      hipLaunchKernelGGL(k_extrap_diag_batch, dim3(nt), dim3(256), 0, b->stream, b->exPlanets, b->exTiles,
+                         b->exPlanetOf, b->exDiagOff[d]);
+  }
+  BH(hipGetLastError());
+  BH(hipEventRecord(b->extrapolated, b->stream));
+  for (int i = 0; i < n; i++) {
+    ws_ctx *c = b->ctx[i];
+    BH(hipStreamWaitEvent(c->stream, b->extrapolated, 0));
+    if ((r = batch_substep_post2(c))) return planet_fail(b, i, r);
     BH(hipEventRecord(b->post[i], c->stream));
     BH(hipStreamWaitEvent(b->stream, b->post[i], 0));
+  }
+  return WS_OK;
+}
+
+// the merged extrapolation schedule (the planets' grids and face buffers do
+// not change over a batch's life)
+int plan_extrapolation(ws_batch *b) {
+  int n = (int)b->ctx.size();
+  std::vector<ExPlanet> pl(n);
+  size_t nd = 0;
+  for (int i = 0; i < n; i++) {
+    pl[i] = batch_ex_planet(b->ctx[i]);
+    if (b->ctx[i]->exDiagOff.size() > nd) nd = b->ctx[i]->exDiagOff.size();
+  }
+  std::vector<int4> tiles;
+  std::vector<int> owner;
+  b->exDiagOff.assign(1, 0);
+  for (size_t d = 0; d + 1 < nd; d++) {
+    for (int i = 0; i < n; i++) {
+      const ws_ctx *c = b->ctx[i];
+      if (d + 1 >= c->exDiagOff.size()) continue;
+      for (int t = c->exDiagOff[d]; t < c->exDiagOff[d + 1]; t++) {
+        tiles.push_back(c->exTilesHost[t]);
+        owner.push_back(i);
+      }
+    }
+    b->exDiagOff.push_back((int)tiles.size());
+  }
+  BH(hipMalloc((void **)&b->exPlanets, sizeof(ExPlanet) * n));
+  BH(hipMalloc((void **)&b->exTiles, sizeof(int4) * (tiles.size() + 1)));
+  BH(hipMalloc((void **)&b->exPlanetOf, sizeof(int) * (owner.size() + 1)));
+  BH(hipMemcpy(b->exPlanets, pl.data(), sizeof(ExPlanet) * n, hipMemcpyHostToDevice));
+  if (!tiles.empty()) {
+    BH(hipMemcpy(b->exTiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));
+    BH(hipMemcpy(b->exPlanetOf, owner.data(), sizeof(int) * owner.size(), hipMemcpyHostToDevice));
   }
   return WS_OK;
 }
@@ -196,6 +255,7 @@ int ws_batch_create(ws_batch **out, ws_ctx *const *ctx, int n, int use_graph, in
   b->threads = threads;
   b->pre.assign(n, nullptr);
   b->post.assign(n, nullptr);
+  b->mid.assign(n, nullptr);
   b->hjobs.resize(n);
   b->hs.resize(n);
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
@@ -203,9 +263,11 @@ int ws_batch_create(ws_batch **out, ws_ctx *const *ctx, int n, int use_graph, in
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->solved, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreate(&b->t0);
   if (e == hipSuccess) e = hipEventCreate(&b->t1);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->extrapolated, hipEventDisableTiming);
   for (int i = 0; i < n && e == hipSuccess; i++) {
     e = hipEventCreateWithFlags(&b->pre[i], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->post[i], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->mid[i], hipEventDisableTiming);
   }
   // one resident 1024-thread workgroup per compute unit
   int dev = 0, cus = 0;
@@ -215,7 +277,7 @@ int ws_batch_create(ws_batch **out, ws_ctx *const *ctx, int n, int use_graph, in
   if (e == hipSuccess) e = hipMalloc((void **)&b->jobs, sizeof(SorJob) * n);
   if (e == hipSuccess) e = hipMalloc((void **)&b->arrive, sizeof(unsigned) * 2 * n);
   if (e == hipSuccess) e = hipMemset(b->arrive, 0, sizeof(unsigned) * 2 * n);
-  if (e != hipSuccess) {
+  if (e != hipSuccess || plan_extrapolation(b) != WS_OK) {
     ws_batch_destroy(b);
     return WS_EHIP;
   }
@@ -341,6 +403,12 @@ void ws_batch_destroy(ws_batch *b) {
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : b->post)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : b->mid)
+    if (e) (void)hipEventDestroy(e);
+  if (b->extrapolated) (void)hipEventDestroy(b->extrapolated);
+  if (b->exPlanets) (void)hipFree(b->exPlanets);
+  if (b->exTiles) (void)hipFree(b->exTiles);
+  if (b->exPlanetOf) (void)hipFree(b->exPlanetOf);
   if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->solved) (void)hipEventDestroy(b->solved);
   if (b->t0) (void)hipEventDestroy(b->t0);

@@ -26,7 +26,9 @@ void metaballs_release(ws_ctx *c);
 World make_world(ws_ctx *c);
 // a device-dt-law substep split around its pressure sweeps (ws_api.hip)
 int batch_substep_pre(ws_ctx *c);
-int batch_substep_post(ws_ctx *c);
+int batch_substep_post1(ws_ctx *c);
+int batch_substep_post2(ws_ctx *c);
+ExPlanet batch_ex_planet(ws_ctx *c);
 SorJob batch_sor_job(ws_ctx *c);
 void batch_solved(ws_ctx *c);
 }  // namespace ws

@@ -1220,14 +1220,13 @@ __global__ void k_vort_splat(Geo g, const uint8_t *__restrict__ ct, const float 
 // neighbours it needs were decided by lanes one step earlier and arrive by
 // lane shuffles; the steps need no barrier. All four waves load the tile
 // (with a one-face halo) into LDS; three of them then leave.
-__global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restrict__ tiles, int off,
-                                                     float *__restrict__ vel, uint8_t *__restrict__ valid) {
+__device__ __forceinline__ void extrap_tile(const Geo &g, int4 t, float *__restrict__ vel,
+                                            uint8_t *__restrict__ valid) {
   constexpr int TI = EX_TI, TJ = EX_TJ, TK = EX_TK;
   static_assert(TJ * TK == 64, "one lane per (j, k) row of a tile");
   constexpr int HI = TI + 2, HJ = TJ + 2, HK = TK + 2, HN = HI * HJ * HK;
   __shared__ float sv[HK][HJ][HI];
   __shared__ uint8_t sf[HK][HJ][HI];
-  int4 t = tiles[off + blockIdx.x];
   int comp = t.x, I = t.y, J = t.z, K = t.w;
   int d0 = comp == 0 ? g.nx + 1 : g.nx, d1 = comp == 1 ? g.ny + 1 : g.ny, d2 = comp == 2 ? g.nz + 1 : g.nz;
   long base = comp == 0 ? 0 : (comp == 1 ? g.uN : (long)g.uN + g.vN);
@@ -1324,6 +1323,23 @@ __global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restri
       lastF = fill ? 1 : (f0 ? 1 : 0);
     }
   }
+}
+
+// one launch per tile diagonal of one context
+__global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restrict__ tiles, int off,
+                                                     float *__restrict__ vel, uint8_t *__restrict__ valid) {
+  extrap_tile(g, tiles[off + blockIdx.x], vel, valid);
+}
+
+// one launch per tile diagonal of many planets: block b takes tile b of the
+// merged schedule, on planet planetOf[b] (planets are independent, and a
+// planet's tiles of one diagonal depend only on its earlier diagonals)
+__global__ __launch_bounds__(256) void k_extrap_diag_batch(const ExPlanet *__restrict__ planets,
+                                                           const int4 *__restrict__ tiles,
+                                                           const int *__restrict__ planetOf, int off) {
+  const int b = off + blockIdx.x;
+  const ExPlanet P = planets[planetOf[b]];
+  extrap_tile(P.g, tiles[b], P.vel, P.valid);
 }
 
 }  // namespace ws

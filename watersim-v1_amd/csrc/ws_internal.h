@@ -81,6 +81,13 @@ struct SorJob {
   int G;             // workgroups sharing this solve
 };
 
+// a planet's face fields for the batched extrapolation wavefront
+struct ExPlanet {
+  Geo g;
+  float *vel;
+  uint8_t *valid;
+};
+
 struct Net;       // the learned warm start (ws_learn.h)
 struct Contacts;  // scratch of the frame-end contact passes (ws_contacts.hip)
 
@@ -210,6 +217,7 @@ struct Ctx {
   size_t scanTmpBytes = 0;
   // extrapolation tile schedule
   int4 *exTiles = nullptr;
+  std::vector<int4> exTilesHost;  // the same schedule on the host (batches merge the planets')
   std::vector<int> exDiagOff;  // tiles of diagonal d: [off[d], off[d+1])
   // marching tets scratch
   unsigned *mtCount = nullptr, *mtOff = nullptr;
