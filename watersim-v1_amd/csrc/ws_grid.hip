@@ -1247,16 +1247,28 @@ __device__ __forceinline__ void extrap_tile(const Geo &g, int4 t, float *__restr
       val[b] = in ? A[f] : 0.f;
       fl[b] = in ? V[f] : 0;
     }
+    int valid = 0, open = 0;
 #pragma unroll
     for (int b = 0; b < B; b++) {
       int e = b * 256 + threadIdx.x;
       if (e < HN) {
         (&sv[0][0][0])[e] = val[b];
         (&sf[0][0][0])[e] = fl[b] ? 1 : 0;
+        int li = e % HI, lj = (e / HI) % HJ, lk = e / (HI * HJ);
+        int i = i0 + li, j = j0 + lj, k = k0 + lk;
+        bool in = i >= 0 && i < d0 && j >= 0 && j < d1 && k >= 0 && k < d2;
+        bool own = li >= 1 && li <= TI && lj >= 1 && lj <= TJ && lk >= 1 && lk <= TK;
+        valid |= (in && fl[b]) ? 1 : 0;
+        open |= (in && own && !fl[b]) ? 1 : 0;
       }
     }
+    // nothing to fill (every owned face valid), or nothing to fill from (no
+    // valid face in the tile or its halo, whose -i/-j/-k side already holds
+    // the earlier diagonals' results): the flood leaves the tile as it is
+    valid = __syncthreads_or(valid);
+    open = __syncthreads_or(open);
+    if (!valid || !open) return;
   }
-  __syncthreads();
   if (threadIdx.x >= 64) return;
   const int lane = threadIdx.x;
   int lj = 1 + (lane % TJ), lk = 1 + (lane / TJ);
