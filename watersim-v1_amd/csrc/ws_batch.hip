@@ -19,7 +19,6 @@ using namespace ws;
 struct ws_batch {
   std::vector<ws_ctx *> ctx;
   int useGraph = 0, threads = 1024;
-  bool warm = false;  // one eager substep has run (lazy allocations are done)
   hipStream_t stream = nullptr;
   hipEvent_t fork = nullptr, solved = nullptr;
   std::vector<hipEvent_t> pre, post;
@@ -300,21 +299,18 @@ int ws_batch_run(ws_batch *b, int substeps) {
   if (b->timing) {
     for (; done < substeps; done++)
       if ((r = enqueue_substep(b)) || (r = account(b))) return r;
-    b->warm = true;
     return finish(b);
   }
   if (!b->useGraph) {
     for (; done < substeps; done++)
       if ((r = enqueue_substep(b))) return r;
-    if ((r = finish(b))) return r;
-    b->warm = b->warm || substeps > 0;
-    return WS_OK;
+    return finish(b);
   }
-  // the first substep of a batch runs eagerly: the contexts allocate their
-  // lazily created buffers (network activations) outside the capture
-  if (!b->warm && substeps > 0) {
+  // the first substep of every run goes eagerly: buffers the contexts create
+  // on first use (network activations, after a change of settings) are
+  // allocated outside the capture
+  if (substeps > 0) {
     if ((r = enqueue_substep(b)) || (r = finish(b))) return r;
-    b->warm = true;
     done = 1;
   }
   if (done < substeps) {
