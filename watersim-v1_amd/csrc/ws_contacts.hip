@@ -820,43 +820,75 @@ __global__ void k_ct_ice_groups(int n, const unsigned long long *__restrict__ gk
 }
 
 // vapor x liquid: each parcel meets the liquid particles near it in index
-// order and is pushed out of each it overlaps (the water never moves)
-__global__ void k_ct_vapor_liquid(Geo g, int nV, const unsigned *__restrict__ vlist, const unsigned *__restrict__ llist,
-                                  const unsigned *__restrict__ lstart, const unsigned *__restrict__ lbinned, float rad,
-                                  float *__restrict__ px, float *__restrict__ py, float *__restrict__ pz,
-                                  float *__restrict__ vx, float *__restrict__ vy, float *__restrict__ vz,
-                                  unsigned *__restrict__ overflow) {
-  constexpr int MAXC = 64;
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nV) return;
-  unsigned j = vlist[e];
-  float x = px[j], y = py[j], z = pz[j];
-  // candidates within 3 radii of the parcel's position (it moves by at
-  // most one radius per contact), sorted by particle index
-  unsigned cand[MAXC];
-  int nc = 0;
+// order and is pushed out of each it overlaps (the water never moves). One
+// wavefront per parcel: the lanes share the cells of the search box and
+// gather the liquid particles within three radii (the parcel moves by at
+// most one radius per contact) into LDS, the list is sorted by particle
+// index (rank of each entry), and the contacts run in that order. Beyond
+// kVlMax candidates (three radii hold about 25 particles at 8 per cell) the
+// overflow counter is raised and the parcel meets only the kVlMax that
+// arrived first: such a pass is reported, not exact.
+constexpr int kVlMax = 64;
+constexpr int kVlWaves = 4;  // parcels per 256-thread workgroup
+
+__global__ __launch_bounds__(256) void k_ct_vapor_liquid(Geo g, int nV, const unsigned *__restrict__ vlist,
+                                                         const unsigned *__restrict__ llist,
+                                                         const unsigned *__restrict__ lstart,
+                                                         const unsigned *__restrict__ lbinned, float rad,
+                                                         float *__restrict__ px, float *__restrict__ py,
+                                                         float *__restrict__ pz, float *__restrict__ vx,
+                                                         float *__restrict__ vy, float *__restrict__ vz,
+                                                         unsigned *__restrict__ overflow) {
+  __shared__ unsigned scand[kVlWaves][kVlMax], ssort[kVlWaves][kVlMax];
+  __shared__ unsigned sn[kVlWaves];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int e = blockIdx.x * kVlWaves + wv;
+  if (e >= nV) return;  // the whole wavefront leaves; no workgroup barrier follows
+  const unsigned j = vlist[e];
+  const float x = px[j], y = py[j], z = pz[j];
+  if (lane == 0) sn[wv] = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
   const float reach = 3.f * rad, reach2 = reach * reach;
   int ci = clampi((int)(x * g.inv_dx), 0, g.nx - 1), cj = clampi((int)(y * g.inv_dx), 0, g.ny - 1);
   int ck = clampi((int)(z * g.inv_dx), 0, g.nz - 1);
   int span = (int)ceilf(reach * g.inv_dx);
-  for (int kk = max(ck - span, 0); kk <= min(ck + span, g.nz - 1); kk++)
-    for (int jj = max(cj - span, 0); jj <= min(cj + span, g.ny - 1); jj++)
-      for (int ii = max(ci - span, 0); ii <= min(ci + span, g.nx - 1); ii++) {
-        int cc = (kk * g.ny + jj) * g.nx + ii;
-        for (unsigned b = lstart[cc]; b < lstart[cc + 1]; b++) {
-          unsigned p = llist[lbinned[b]];
-          float ddx = px[p] - x, ddy = py[p] - y, ddz = pz[p] - z;
-          if (ddx * ddx + ddy * ddy + ddz * ddz >= reach2) continue;
-          if (nc == MAXC) { atomicAdd(overflow, 1u); continue; }
-          int k = nc++;
-          while (k > 0 && cand[k - 1] > p) { cand[k] = cand[k - 1]; k--; }
-          cand[k] = p;
-        }
-      }
+  const int ia = max(ci - span, 0), ib = min(ci + span, g.nx - 1);
+  const int ja = max(cj - span, 0), jb = min(cj + span, g.ny - 1);
+  const int ka = max(ck - span, 0), kb = min(ck + span, g.nz - 1);
+  const int ni = ib - ia + 1, nj = jb - ja + 1, nk = kb - ka + 1;
+  for (int cidx = lane; cidx < ni * nj * nk; cidx += 64) {
+    int ii = ia + cidx % ni, jj = ja + (cidx / ni) % nj, kk = ka + cidx / (ni * nj);
+    int cc = (kk * g.ny + jj) * g.nx + ii;
+    for (unsigned b = lstart[cc]; b < lstart[cc + 1]; b++) {
+      unsigned p = llist[lbinned[b]];
+      float ddx = px[p] - x, ddy = py[p] - y, ddz = pz[p] - z;
+      if (ddx * ddx + ddy * ddy + ddz * ddz >= reach2) continue;
+      unsigned slot = atomicAdd(&sn[wv], 1u);
+      if (slot < (unsigned)kVlMax) scand[wv][slot] = p;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const unsigned found = sn[wv];
+  const int nc = found < (unsigned)kVlMax ? (int)found : kVlMax;
+  if (lane == 0 && found > (unsigned)kVlMax) atomicAdd(overflow, found - (unsigned)kVlMax);
+  // rank sort of the (distinct) particle indices
+  if (lane < nc) {
+    unsigned v = scand[wv][lane];
+    int rank = 0;
+    for (int m = 0; m < nc; m++) rank += scand[wv][m] < v ? 1 : 0;
+    ssort[wv][rank] = v;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane != 0) return;
   const float rad2 = rad * rad;
   float jx = x, jy = y, jz = z, ux = vx[j], uy = vy[j], uz = vz[j];
   for (int k = 0; k < nc; k++) {
-    unsigned p = cand[k];
+    unsigned p = ssort[wv][k];
     float wx = px[p], wy = py[p], wz = pz[p];
     // the reference's x window: binary search to x >= wx - rad, stop at x - wx >= rad
     if (jx < wx - rad) continue;
@@ -1181,7 +1213,8 @@ int frame_vapor_collisions(ws_ctx *c) {
     int nL = 0;
     if ((r = subset(c, k->fl, k->ol, k->llist, &nL))) return r;
     if ((r = bin_subset(c, nL, k->llist, k->lcell, k->lrank, k->lstart, k->lbinned))) return r;
-    hipLaunchKernelGGL(k_ct_vapor_liquid, dim3(blocks_for(nV, 256)), dim3(256), 0, c->stream, c->geo, nV, k->vlist,
+    hipLaunchKernelGGL(k_ct_vapor_liquid, dim3((nV + kVlWaves - 1) / kVlWaves), dim3(256), 0, c->stream, c->geo, nV,
+                       k->vlist,
                        k->llist, k->lstart, k->lbinned, rad, c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz,
                        k->counts + 2);
     WS_LAUNCHED();

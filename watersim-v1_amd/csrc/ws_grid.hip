@@ -180,7 +180,15 @@ __global__ __launch_bounds__(256) void k_p2g_tile(Geo g, int tilesX, int tilesY,
   unsigned total = rowPre[nrows];
   // owned face ranges per component: [lo, hi) in i, j, k
   int iHiC = min(i0 + T, g.nx), jHiC = min(j0 + T, g.ny), kHiC = min(k0 + T, g.nz);
-  for (unsigned vI = threadIdx.x; vI < total; vI += blockDim.x) {
+  // The binned particles are in cell order, so neighbouring entries share
+  // their faces. A wavefront takes every 8th entry of a 256-entry chunk
+  // (thread 32a + b takes entry 8b + a) so that its lanes' LDS atomics fall
+  // on different faces instead of queueing on the same few; integer sums do
+  // not depend on the order.
+  const unsigned perm = (threadIdx.x & 31u) * 8u + (threadIdx.x >> 5);
+  for (unsigned base = 0; base < total; base += 256u) {
+    const unsigned vI = base + perm;
+    if (vI >= total) continue;
     int lo = 0, hi = nrows;  // last r with rowPre[r] <= vI
     while (hi - lo > 1) {
       int mid = (lo + hi) >> 1;
