@@ -1222,16 +1222,17 @@ __global__ void k_vort_splat(Geo g, const uint8_t *__restrict__ ct, const float 
 // invalid face takes the mean of its valid neighbours, where a face filled
 // earlier in the same pass counts as valid. A face therefore depends on the
 // final state of its -i/-j/-k neighbours and on the original state of its
-// +i/+j/+k neighbours. Tiles of 32x8x8 faces are processed one tile-diagonal
-// (I+J+K) per launch. Inside a tile one wavefront walks the hyperplanes
-// li+lj+lk: lane (lj, lk) handles face li = step - lj - lk, so the -j / -k
-// neighbours it needs were decided by lanes one step earlier and arrive by
-// lane shuffles; the steps need no barrier. All four waves load the tile
-// (with a one-face halo) into LDS; three of them then leave.
+// +i/+j/+k neighbours. Tiles of 16^3 faces are processed one tile-diagonal
+// (I+J+K) per launch. Inside a tile the workgroup walks the hyperplanes
+// li+lj+lk: lane (lj, lk) of 256 handles face li = step - lj - lk, so the
+// -j / -k neighbours it needs were decided by lanes one step earlier and
+// arrive by lane shuffles (within a wavefront's four k-planes) or through
+// LDS (from the previous wavefront's last plane), one barrier per step. The
+// tile (with a one-face halo) is first loaded into LDS.
 __device__ __forceinline__ void extrap_tile(const Geo &g, int4 t, float *__restrict__ vel,
                                             uint8_t *__restrict__ valid) {
   constexpr int TI = EX_TI, TJ = EX_TJ, TK = EX_TK;
-  static_assert(TJ * TK == 64, "one lane per (j, k) row of a tile");
+  static_assert(TJ == 16 && TJ * TK == 256, "one lane per (j, k) row of a tile, 16-lane rows in j");
   constexpr int HI = TI + 2, HJ = TJ + 2, HK = TK + 2, HN = HI * HJ * HK;
   __shared__ float sv[HK][HJ][HI];
   __shared__ uint8_t sf[HK][HJ][HI];
@@ -1277,18 +1278,23 @@ __device__ __forceinline__ void extrap_tile(const Geo &g, int4 t, float *__restr
     open = __syncthreads_or(open);
     if (!valid || !open) return;
   }
-  if (threadIdx.x >= 64) return;
-  const int lane = threadIdx.x;
-  int lj = 1 + (lane % TJ), lk = 1 + (lane / TJ);
-  int j = j0 + lj, k = k0 + lk;
-  bool row = j < d1 && k < d2;
+  // the -k neighbours of a wavefront's first plane are the previous
+  // wavefront's last plane: it leaves them here, double-buffered by step
+  __shared__ float xv[2][4][TJ];
+  __shared__ int xf[2][4][TJ];
+  const int lane = threadIdx.x, wv = lane >> 6, l64 = lane & 63;
+  const int lj = 1 + (lane % TJ), lk = 1 + (lane / TJ);
+  const int lkw = (lk - 1) & 3;  // plane within the wavefront
+  const int j = j0 + lj, k = k0 + lk;
+  const bool row = j < d1 && k < d2;
   // the LDS copy stays the original state; a face's final state travels in
   // registers: a lane keeps what it decided at the previous step (its -i
   // neighbour now) and takes the -j / -k neighbours from the lanes one row
   // and one plane back, which decided the same li one step earlier (-j by a
-  // DPP row shift, -k by a lane permute, the flags of all lanes by a ballot).
-  // Every LDS read is unconditional (clamped index), so a step has no
-  // branches but the store.
+  // DPP row shift, -k by a lane permute within the wavefront or from the
+  // previous wavefront's plane in LDS; the flags of a wavefront by a
+  // ballot). Every LDS read is unconditional (clamped index); the four
+  // wavefronts step together, one barrier per step.
   float lastV = 0.f;
   int lastF = 0;
   const float vaH = sv[lk][lj][0];
@@ -1301,10 +1307,13 @@ __device__ __forceinline__ void extrap_tile(const Geo &g, int4 t, float *__restr
     int i = i0 + li;
     // lane - 1 (same 16-lane row whenever lj > 1) and lane - TJ
     float vjm = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(lastV), 0x111, 0xf, 0xf, false));
-    float vkm = __shfl_up(lastV, TJ);
+    float vkw = __shfl_up(lastV, TJ);
     unsigned long long fm = __ballot(lastF);
-    int fjm = (int)((fm >> (lane > 0 ? lane - 1 : 0)) & 1ull);
-    int fkm = (int)((fm >> (lane >= TJ ? lane - TJ : 0)) & 1ull);
+    int fjm = (int)((fm >> (l64 > 0 ? l64 - 1 : 0)) & 1ull);
+    int fkw = (int)((fm >> (l64 >= TJ ? l64 - TJ : 0)) & 1ull);
+    const int rb = (step + 1) & 1;  // the buffer written at the previous step
+    float vkm = lkw > 0 ? vkw : xv[rb][wv > 0 ? wv - 1 : 0][lj - 1];
+    int fkm = lkw > 0 ? fkw : xf[rb][wv > 0 ? wv - 1 : 0][lj - 1];
     bool act = row && li >= 1 && li <= TI && i < d0;
     int lc = li < 1 ? 1 : (li > TI ? TI : li);
     int f0 = sf[lk][lj][lc], fb = sf[lk][lj][lc + 1], fd = sf[lk][lj + 1][lc], ff = sf[lk + 1][lj][lc];
@@ -1342,6 +1351,11 @@ __device__ __forceinline__ void extrap_tile(const Geo &g, int4 t, float *__restr
       lastV = fill ? v : v0;
       lastF = fill ? 1 : (f0 ? 1 : 0);
     }
+    if (lkw == 3) {
+      xv[step & 1][wv][lj - 1] = lastV;
+      xf[step & 1][wv][lj - 1] = lastF;
+    }
+    __syncthreads();
   }
 }
 

@@ -23,7 +23,7 @@ constexpr float kFixInvF = 1.0f / 1125899906842624.0f;
 
 // Tile shapes (cells) for P2G and the extrapolation wavefront.
 constexpr int P2G_T = 8;
-constexpr int EX_TI = 32, EX_TJ = 8, EX_TK = 8;
+constexpr int EX_TI = 16, EX_TJ = 16, EX_TK = 16;  // extrapolation tile (256 lanes, one per (j, k))
 constexpr int SOR_TX = 32, SOR_TY = 8, SOR_TZ = 8;
 constexpr int SOR_TB_X = 16, SOR_TB_Y = 8, SOR_TB_Z = 8;  // temporal-blocking tile
 
