@@ -824,11 +824,12 @@ __global__ void k_ct_ice_groups(int n, const unsigned long long *__restrict__ gk
 // wavefront per parcel: the lanes share the cells of the search box and
 // gather the liquid particles within three radii (the parcel moves by at
 // most one radius per contact) into LDS, the list is sorted by particle
-// index (rank of each entry), and the contacts run in that order. Beyond
-// kVlMax candidates (three radii hold about 25 particles at 8 per cell) the
-// overflow counter is raised and the parcel meets only the kVlMax that
-// arrived first: such a pass is reported, not exact.
-constexpr int kVlMax = 64;
+// index (rank of each entry), and the contacts run in that order. Three
+// radii hold about 25 particles at 8 per cell, but compressed water holds
+// more (64 was exceeded in the cfg3 run); beyond kVlMax the overflow counter
+// is raised and one lane gathers the first kVlMax in cell order instead, so
+// the result stays deterministic.
+constexpr int kVlMax = 1024;
 constexpr int kVlWaves = 4;  // parcels per 256-thread workgroup
 
 __global__ __launch_bounds__(256) void k_ct_vapor_liquid(Geo g, int nV, const unsigned *__restrict__ vlist,
@@ -873,10 +874,26 @@ __global__ __launch_bounds__(256) void k_ct_vapor_liquid(Geo g, int nV, const un
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const unsigned found = sn[wv];
   const int nc = found < (unsigned)kVlMax ? (int)found : kVlMax;
-  if (lane == 0 && found > (unsigned)kVlMax) atomicAdd(overflow, found - (unsigned)kVlMax);
+  if (found > (unsigned)kVlMax && lane == 0) {
+    atomicAdd(overflow, found - (unsigned)kVlMax);
+    // the first kVlMax in cell and bin order (arrival order is not fixed)
+    int m = 0;
+    for (int cidx = 0; cidx < ni * nj * nk && m < kVlMax; cidx++) {
+      int ii = ia + cidx % ni, jj = ja + (cidx / ni) % nj, kk = ka + cidx / (ni * nj);
+      int cc = (kk * g.ny + jj) * g.nx + ii;
+      for (unsigned b = lstart[cc]; b < lstart[cc + 1] && m < kVlMax; b++) {
+        unsigned p = llist[lbinned[b]];
+        float ddx = px[p] - x, ddy = py[p] - y, ddz = pz[p] - z;
+        if (ddx * ddx + ddy * ddy + ddz * ddz < reach2) scand[wv][m++] = p;
+      }
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   // rank sort of the (distinct) particle indices
-  if (lane < nc) {
-    unsigned v = scand[wv][lane];
+  for (int q = lane; q < nc; q += 64) {
+    unsigned v = scand[wv][q];
     int rank = 0;
     for (int m = 0; m < nc; m++) rank += scand[wv][m] < v ? 1 : 0;
     ssort[wv][rank] = v;
