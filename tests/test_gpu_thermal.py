@@ -134,3 +134,60 @@ def test_ice_contacts():
     moved = _fe_compare(s, g, "fe1_", "fe2_", ["pvx", "pvy", "pvz"])
     assert moved > 20
     s.close()
+
+
+def test_vapor_liquid_crowded_parcel():
+    """A parcel with 200 liquid particles between 1.5 and 3 contact radii
+    and three inside one radius (more candidates than the 64 the first
+    version of the pass kept): the GPU pass meets them in index order as the
+    oracle's sweep does, raises no overflow, and gives the same bits on a
+    second context."""
+    import cpu_oracle  # noqa: F401  (the oracle library must be built)
+    from scenes import oracle_from_golden, PART_KEYS, EXTRA_KEYS
+    g = golden.load("thermal_32")
+    fe = g.meta["frame_end"]
+    sp = fe["sunPos"]
+    w = g.meta["world"]
+    rad = 0.6 * w["spacing"]
+    n = len(g["fe0_px"])
+    keys = [k for k in PART_KEYS + EXTRA_KEYS if "fe0_" + k in g]
+    parts = {k: g["fe0_" + k].copy() for k in keys}
+    fl = parts["pflag"]
+    vap = np.nonzero(fl == 2)[0]
+    v = int(vap[len(vap) // 2])
+    rng = np.random.default_rng(7)
+    K = 203
+    d = rng.normal(size=(K, 3))
+    r = np.concatenate([rng.uniform(1.5, 3.0, size=200), [0.5, 0.7, 0.9]]) * rad
+    d *= (r / np.linalg.norm(d, axis=1))[:, None]
+    liq = int(np.nonzero(fl == 0)[0][0])
+    for k in keys:
+        extra = np.repeat(parts[k][liq:liq + 1], K)
+        if k in ("px", "py", "pz"):
+            extra = (parts[k][v] + d[:, "xyz".index(k[1])]).astype(parts[k].dtype)
+        parts[k] = np.concatenate([parts[k], extra])
+    N = n + K
+    outs = []
+    for rep in range(2):
+        s = gpu_from_golden(g, tag="fe0_", capacity=N)
+        s.set(nP=N, spinOn=1, hasSun=1, sunX=sp[0], sunY=sp[1], sunZ=sp[2])
+        for k in keys:
+            s.upload(k, parts[k])
+        s.stage("vapor_collisions", 0.0)
+        assert s.get("contactOverflow") == 0
+        outs.append({k: s.download(k) for k in ("px", "py", "pz", "pvx", "pvy", "pvz")})
+        s.close()
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+    st = oracle_from_golden(g, tag="fe0_")
+    st.set_particles(N, **parts)
+    st.set_params(spinOn=1, hasSun=1, sunX=sp[0], sunY=sp[1], sunZ=sp[2])
+    st.call("oc_vapor_collisions")
+    moved = 0
+    for k in outs[0]:
+        want = st[k][:N].astype(np.float64)
+        e = np.abs(outs[0][k] - want).max() / max(1.0, np.abs(want).max())
+        print("  %-4s max rel err vs oracle %.3e" % (k, e))
+        assert e <= 2e-5, (k, e)
+        moved += int(abs(float(outs[0][k][v]) - float(parts[k][v])) > 0)
+    assert moved > 0  # the crowded parcel was pushed
