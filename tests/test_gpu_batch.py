@@ -133,3 +133,30 @@ def test_batch_refuses_timing_contexts():
     b.run(1)
     b.close()
     s.close()
+
+
+def test_batch_with_balls_equals_single_planet():
+    """Balls ride along in a batch: their raster, push-out and integrator run
+    in the planets' own stages, and the planet ends where ws_run_substeps
+    leaves it (balls included), eager and from the graph."""
+    def with_balls(s, n):
+        c = 0.5 * W
+        r = 0.08 * W
+        # x y z vx vy vz r rho submerged: two floating balls at the sea surface
+        s.set_balls(np.array([[c + 0.4 * W, c, c, 0, 0, 0, r, 500.0, 0],
+                              [c, c + 0.4 * W, c, 0.1, 0, 0, r, 700.0, 0]], np.float64))
+        return s
+    ref = with_balls(planet(32, 21), 32)
+    ref.set(sorMode=4, sorThreads=256)
+    bat = with_balls(planet(32, 21), 32)
+    other = planet(48, 22)
+    b = watersim_hip.Batch([bat, other], graph=True, threads=256)
+    for k in (1, 3):
+        b.run(k)
+        ref.run_substeps(k)
+        assert_same(ref, bat, "%d substeps" % k)
+        assert np.array_equal(ref.balls(), bat.balls())
+    print("  balls after 4 substeps: %s" % np.array2string(bat.balls()[:, :3], precision=4))
+    b.close()
+    for s in (ref, bat, other):
+        s.close()
