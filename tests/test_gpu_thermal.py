@@ -107,16 +107,20 @@ def _fe_compare(s, g, a, b, keys, tol=2e-5):
     return moved
 
 
-@pytest.mark.parametrize("serial_max", [48, 0, 1 << 30])
-def test_vapor_collisions(serial_max):
+@pytest.mark.parametrize("serial_max,queue_max,lds_max", [(48, 16000, 16384), (0, 16000, 16384),
+                                                          (1 << 30, 16000, 16384), (0, 0, 16384), (0, 0, 0)])
+def test_vapor_collisions(serial_max, queue_max, lds_max):
     """Parcel-parcel and parcel-liquid contacts on the perturbed frame-end
     state (a chain of 80 touching parcels along the sea surface, real steam
     above it): the census exact, positions, velocities and spins to float
-    rounding of the reference's serial sweep. serial_max picks the path of
-    the contact groups: 0 sends every group through the dependency-ordered
-    parallel sweep, 2^30 every group through the one-thread serial sweep."""
+    rounding of the reference's serial sweep. The settings pick the path of
+    the contact groups: serial_max 0 sends every group to the
+    dependency-ordered sweep, 2^30 every group to the one-thread serial
+    sweep; queue_max 0 moves the dependency-ordered groups from the LDS ready
+    queue to the polling sweep, whose counts lds_max 0 moves from LDS to
+    global memory."""
     g, s = _fe_gpu("fe0_")
-    s.set(vaporSerialMax=serial_max)
+    s.set(vaporSerialMax=serial_max, vaporQueueMax=queue_max, vaporLdsMax=lds_max)
     s.stage("vapor_collisions", 0.0)
     fe = g.meta["frame_end"]
     assert int(s.get("vaporCount")) == fe["vaporCount"]

@@ -910,7 +910,7 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
   SF(heatK) SF(Tamb) SF(shadeCool) SF(sunActivity) SF(evapIntensity) SF(snowT) SF(meltT) SF(cloudP)
   SF(currents) SF(Rlo) SF(Rhi) SF(simTime) SF(heatAcc) SF(netLossCoarse)
   SI(pressureIters) SI(maxPressureIters) SI(residualEvery) SI(ceilReflect) SI(hasSun) SI(measureResidual)
-  SI(adaptiveChecks) SI(spinOn) SI(warmStart) SI(recordSamples) SI(pressureCompare) SI(netTiled) SI(vaporSerialMax)
+  SI(adaptiveChecks) SI(spinOn) SI(warmStart) SI(recordSamples) SI(pressureCompare) SI(netTiled) SI(vaporSerialMax) SI(vaporQueueMax) SI(vaporLdsMax)
 #undef SF
 #undef SI
   if (!strcmp(n, "sorMode")) {

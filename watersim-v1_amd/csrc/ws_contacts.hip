@@ -18,6 +18,8 @@
 // particles near it in index order, as the reference's outer loop does.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
 #include "ws_frame.h"
 
 namespace ws {
@@ -538,7 +540,7 @@ __global__ __launch_bounds__(1024) void k_vc_sweepq(const unsigned *__restrict__
                                                     const unsigned *__restrict__ candP,
                                                     const unsigned *__restrict__ succ,
                                                     const unsigned *__restrict__ depc, float rad, int spin,
-                                                    VaporState P, unsigned *__restrict__ unfinished) {
+                                                    VaporState P, unsigned *__restrict__ unfinished, int qMax) {
   __shared__ unsigned sdep[kVcQ / 2];  // two 16-bit counts per word
   __shared__ unsigned short sq[kVcQ];
   __shared__ unsigned shead, stail, sdone;
@@ -547,7 +549,7 @@ __global__ __launch_bounds__(1024) void k_vc_sweepq(const unsigned *__restrict__
   unsigned nb = *nBig;
   for (unsigned gi = blockIdx.x; gi < nb; gi += gridDim.x) {
     int t0 = (int)big[2 * gi], t1 = (int)big[2 * gi + 1], m = t1 - t0;
-    if (m > kVcQ) continue;  // left to the polling sweep
+    if (m > qMax) continue;  // left to the polling sweep (qMax <= kVcQ)
     if (threadIdx.x == 0) { shead = 0; stail = 0; sdone = 0; }
     for (int q = threadIdx.x; q < m; q += blockDim.x) sq[q] = kVcEmpty;
     for (int w = threadIdx.x; 2 * w < m; w += blockDim.x) {
@@ -642,14 +644,14 @@ __global__ __launch_bounds__(256) void k_vc_sweep(const unsigned *__restrict__ b
                                                   const unsigned *__restrict__ candP,
                                                   const unsigned *__restrict__ ncand, const unsigned *__restrict__ need,
                                                   unsigned *__restrict__ done, float rad, int spin, VaporState P,
-                                                  unsigned *__restrict__ unfinished) {
+                                                  unsigned *__restrict__ unfinished, int qMax, int ldsMax) {
   __shared__ unsigned sdone[kVcLds];
   const float rad2 = rad * rad;
   unsigned nb = *nBig;
   for (unsigned gi = blockIdx.x; gi < nb; gi += gridDim.x) {
     int t0 = (int)big[2 * gi], t1 = (int)big[2 * gi + 1];
-    if (t1 - t0 <= kVcQ) continue;  // run from the ready queue (k_vc_sweepq)
-    if (t1 - t0 <= kVcLds) {
+    if (t1 - t0 <= qMax) continue;  // run from the ready queue (k_vc_sweepq)
+    if (t1 - t0 <= ldsMax) {        // counts in LDS (ldsMax <= kVcLds)
       for (int q = threadIdx.x; q < t1 - t0; q += blockDim.x) sdone[q] = 0;
       __syncthreads();
       vc_sweep_group<true>(t0, t1, sdone, gk, xk, posOf, cand, candT, candP, ncand, need, rad, rad2, spin, P,
@@ -1207,10 +1209,12 @@ int frame_vapor_collisions(ws_ctx *c) {
     if (nBig > 0) {
       int grid = (int)(nBig < 2048 ? nBig : 2048);
       hipLaunchKernelGGL(k_vc_sweepq, dim3(grid), dim3(1024), 0, c->stream, k->big, k->vcCounters, k->stepP, k->stepN,
-                         k->candP, k->succ, k->depc, rad, c->spinOn, P, k->vcCounters + 1);
+                         k->candP, k->succ, k->depc, rad, c->spinOn, P, k->vcCounters + 1,
+                         std::min(std::max(c->vaporQueueMax, 0), kVcQ));
       hipLaunchKernelGGL(k_vc_sweep, dim3(grid), dim3(256), 0, c->stream, k->big, k->vcCounters, k->gk, k->xk,
                          k->posOf, k->cand, k->candT, k->candP, k->ncand, k->need, k->done, rad, c->spinOn, P,
-                         k->vcCounters + 1);
+                         k->vcCounters + 1, std::min(std::max(c->vaporQueueMax, 0), kVcQ),
+                         std::min(std::max(c->vaporLdsMax, 0), kVcLds));
     }
     WS_LAUNCHED();
     c->vaporBigGroups = (long)nBig;

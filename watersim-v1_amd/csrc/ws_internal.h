@@ -150,6 +150,10 @@ struct Ctx {
   Contacts *contacts = nullptr;
   int vaporCount = 0, nightVaporCount = 0;
   int vaporSerialMax = 48;  // larger vapor groups run the dependency-ordered sweep
+  // of those, groups up to vaporQueueMax steps (at most 16000) run from the
+  // LDS ready queue, the rest by the polling sweep, with its counts in LDS up
+  // to vaporLdsMax steps (at most 16384)
+  int vaporQueueMax = 16000, vaporLdsMax = 16384;
   long vaporBigGroups = 0, vaporGroupMax = 0, vaporBigParcels = 0;  // of the last vapor pass
   long contactOverflow = 0;
   // both solves per pressure stage, timed separately (measurement)
