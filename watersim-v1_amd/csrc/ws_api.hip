@@ -4,6 +4,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cfloat>
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -483,8 +484,11 @@ int pressure_iterate(ws_ctx *c) {
   // early if the residual no longer falls (float resolution reached)
   // The batch length follows the observed convergence rate: after each check
   // the sweeps still needed at that rate are predicted, so only a few
-  // residual evaluations run per solve.
+  // residual evaluations run per solve. With adaptive checks the first batch
+  // is three quarters of the previous solve's sweeps (consecutive solves of
+  // a context need similar counts).
   int done = 0, flat = 0, k = c->residualEvery;
+  if (c->adaptiveChecks && c->itersDone > 0) k = std::max(k, (int)(0.75 * c->itersDone));
   double res = 0, best, prev;
   if ((r = residual(c, &res))) return r;
   best = prev = res;

@@ -852,9 +852,10 @@ __global__ void k_residual_final(const double *__restrict__ partials, int n, Dev
 // ------------------------------------------------ on-device pressure solve
 // The whole to-residual solve of st_pressure_solve on the device: the
 // red-black list sweeps (the arithmetic of k_sor_list), the FP64 residual
-// of k_residual_list, and the host loop's stopping rule (checks every
-// `every` sweeps, adaptive check spacing from the observed rate, stop on
-// the tolerance, the sweep cap or twelve checks without a 0.5 % gain). A
+// of k_residual_list, and the host loop's stopping rule (a first batch of
+// three quarters of the previous solve's sweeps, then checks spaced by the
+// observed rate, stop on the tolerance, the sweep cap or twelve checks
+// without a 0.5 % gain). A
 // solve takes G workgroups (G = 1: k_sor_solve, one context; k_sor_solve_batch
 // runs many planets' solves in one launch, G per planet). Workgroup w of G
 // relaxes the w-th slice of each colour list; the G workgroups of a planet
@@ -1015,10 +1016,12 @@ __device__ void sor_solve_part(const SorJob &J, int w) {
     done = (unsigned)max(J.fixedIters, 0);
     if (ok && J.measure) { ok = sor_residual<NT>(J, w, nR, nB, nxy, bar, &res); checks++; }
   } else {
+    const unsigned last = J.ds->sorIters;  // the previous solve's sweeps (rewritten at the end)
     ok = sor_residual<NT>(J, w, nR, nB, nxy, bar, &res);
     checks++;
     double best = res, prev = res;
     int flat = 0, k = J.every;
+    if (J.adaptive && last > 0) k = max(k, (int)(0.75 * last));
     while (ok && res > J.tol && (int)done < J.maxIters && flat < 12) {
       if ((int)done + k > J.maxIters) k = J.maxIters - (int)done;
       for (int s = 0; s < k && ok; s++) ok = sor_sweep<NT>(J, w, nR, nB, nxy, bar);
