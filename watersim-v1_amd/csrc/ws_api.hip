@@ -823,19 +823,25 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
   if (dalloc(c, &c->partials, 2 * c->nPartials + 2048)) { ws_destroy(c); return WS_EHIP; }
   // scan scratch: enough for the larger of the cell and particle scans
   size_t b1 = 0, b2 = 0;
-  hipcub::DeviceScan::ExclusiveSum(nullptr, b1, c->cellCount, c->cellStart, (int)c->nC + 1, c->stream);
-  hipcub::DeviceScan::ExclusiveSum(nullptr, b2, c->vneed, c->vneed, (int)cap, c->stream);
   size_t b3 = 0;
-  hipcub::DeviceScan::ExclusiveSum(nullptr, b3, c->sorOff, c->sorOff, (int)c->nC + 1, c->stream);
+  if (hipcub::DeviceScan::ExclusiveSum(nullptr, b1, c->cellCount, c->cellStart, (int)c->nC + 1, c->stream) ||
+      hipcub::DeviceScan::ExclusiveSum(nullptr, b2, c->vneed, c->vneed, (int)cap, c->stream) ||
+      hipcub::DeviceScan::ExclusiveSum(nullptr, b3, c->sorOff, c->sorOff, (int)c->nC + 1, c->stream)) {
+    ws_destroy(c);
+    return WS_EHIP;
+  }
   c->scanTmpBytes = b1 > b2 ? b1 : b2;
   if (b3 > c->scanTmpBytes) c->scanTmpBytes = b3;
   if (hipMalloc(&c->scanTmp, c->scanTmpBytes) != hipSuccess) { ws_destroy(c); return WS_EHIP; }
-  hipMemcpy(c->thr, thr.data(), thr.size() * sizeof(float), hipMemcpyHostToDevice);
   DevScalars init{};
   init.gRamp = c->gRamp;
-  hipMemcpy(c->ds, &init, sizeof(init), hipMemcpyHostToDevice);
   uint32_t rng[4] = {0x2f6e2b1u, 0x7f4a7c15u, 0x51ab3c77u, 0u};  // _cfS, _bfS, _evS (solver.js:151-153)
-  hipMemcpy(c->rng, rng, sizeof(rng), hipMemcpyHostToDevice);
+  if (hipMemcpy(c->thr, thr.data(), thr.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->ds, &init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->rng, rng, sizeof(rng), hipMemcpyHostToDevice) != hipSuccess) {
+    ws_destroy(c);
+    return WS_EHIP;
+  }
   {
     // affine maps of 1..64 LCG steps: s -> A s + C (mod 2^32)
     uint32_t J[128];
@@ -846,7 +852,7 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
       J[n] = A;
       J[64 + n] = C;
     }
-    hipMemcpy(c->jmp, J, sizeof(J), hipMemcpyHostToDevice);
+    if (hipMemcpy(c->jmp, J, sizeof(J), hipMemcpyHostToDevice) != hipSuccess) { ws_destroy(c); return WS_EHIP; }
   }
   // extrapolation tile schedule, grouped by tile diagonal I+J+K
   {
@@ -870,7 +876,10 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
       c->exDiagOff.push_back((int)flat.size());
     }
     if (hipMalloc(&c->exTiles, flat.size() * sizeof(int4)) != hipSuccess) { ws_destroy(c); return WS_EHIP; }
-    hipMemcpy(c->exTiles, flat.data(), flat.size() * sizeof(int4), hipMemcpyHostToDevice);
+    if (hipMemcpy(c->exTiles, flat.data(), flat.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess) {
+      ws_destroy(c);
+      return WS_EHIP;
+    }
     c->exTilesHost = flat;
   }
   if (hipDeviceSynchronize() != hipSuccess) { ws_destroy(c); return WS_EHIP; }
@@ -880,7 +889,7 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
 
 void ws_destroy(ws_ctx *c) {
   if (!c) return;
-  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   net_free(c);
   contacts_release(c, true);
   balls_release(c);
@@ -893,12 +902,12 @@ void ws_destroy(ws_ctx *c) {
                   c->scanTmp, c->exTiles, c->mtCount, c->mtPos, c->mtNrm, c->mtOff, c->tsolid, c->pDepth,
                   c->pLight, c->pWx, c->pWy, c->pWz, c->pPh, c->evBuf, c->jmp, c->currField};
   for (void *p : ptrs)
-    if (p) hipFree(p);
-  if (c->hs) hipHostFree(c->hs);
-  if (c->renderHost) hipHostFree(c->renderHost);
+    if (p) (void)hipFree(p);
+  if (c->hs) (void)hipHostFree(c->hs);
+  if (c->renderHost) (void)hipHostFree(c->renderHost);
   for (auto &e : c->ev)
-    if (e) hipEventDestroy(e);
-  if (c->stream) hipStreamDestroy(c->stream);
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -1007,7 +1016,7 @@ int ws_upload(ws_ctx *c, const char *name, const void *host, size_t nbytes) {
   if (!strcmp(name, "terrainR")) {
     long n = (long)c->tn * c->tn * c->tn;
     if (n <= 0 || nbytes != (size_t)n * 4) { c->err = "terrainR: set tn first; size must be tn^3 floats"; return WS_EINVAL; }
-    if (c->terrainR) hipFree(c->terrainR);
+    if (c->terrainR) (void)hipFree(c->terrainR);
     c->terrainR = nullptr;
     int r = dalloc(c, &c->terrainR, n);
     if (r) return r;
@@ -1015,7 +1024,7 @@ int ws_upload(ws_ctx *c, const char *name, const void *host, size_t nbytes) {
   if (!strcmp(name, "terrainSolid")) {
     long n = (long)c->tn * c->tn * c->tn;
     if (n <= 0 || nbytes != (size_t)n) { c->err = "terrainSolid: set tn first; size must be tn^3 bytes"; return WS_EINVAL; }
-    if (c->tsolid) hipFree(c->tsolid);
+    if (c->tsolid) (void)hipFree(c->tsolid);
     c->tsolid = nullptr;
     int r = dalloc(c, &c->tsolid, n);
     if (r) return r;

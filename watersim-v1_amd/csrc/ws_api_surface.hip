@@ -49,7 +49,7 @@ namespace {
 template <class T>
 int grow(ws_ctx *c, T **p, long &have, long need) {
   if (have >= need) return WS_OK;
-  if (*p) hipFree(*p);
+  if (*p) (void)hipFree(*p);
   *p = nullptr;
   if (hipMalloc((void **)p, sizeof(T) * (need > 0 ? need : 1)) != hipSuccess) {
     have = 0;
@@ -62,9 +62,9 @@ int grow(ws_ctx *c, T **p, long &have, long need) {
 
 int ensure_scan(ws_ctx *c, long n) {
   size_t b = 0;
-  hipcub::DeviceScan::ExclusiveSum(nullptr, b, (unsigned *)nullptr, (unsigned *)nullptr, (int)n, c->stream);
+  WS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b, (unsigned *)nullptr, (unsigned *)nullptr, (int)n, c->stream));
   if (b <= c->scanTmpBytes) return WS_OK;
-  if (c->scanTmp) hipFree(c->scanTmp);
+  if (c->scanTmp) (void)hipFree(c->scanTmp);
   c->scanTmp = nullptr;
   if (hipMalloc(&c->scanTmp, b) != hipSuccess) { c->err = "hipMalloc scan scratch"; return WS_EHIP; }
   c->scanTmpBytes = b;
@@ -98,8 +98,8 @@ int mt_build_dev(ws_ctx *c, const float *dd, int nx, int ny, int nz, double dx, 
   WS_HIP(hipStreamSynchronize(c->stream));
   long need = pos ? (long)total : 0;
   if (need > c->mtCap) {  // vertex capacity of the device mesh buffers
-    if (c->mtPos) hipFree(c->mtPos);
-    if (c->mtNrm) hipFree(c->mtNrm);
+    if (c->mtPos) (void)hipFree(c->mtPos);
+    if (c->mtNrm) (void)hipFree(c->mtNrm);
     c->mtPos = c->mtNrm = nullptr;
     c->mtCap = 0;
     WS_HIP(hipMalloc((void **)&c->mtPos, sizeof(float) * 3 * need));

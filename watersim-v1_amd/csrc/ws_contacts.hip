@@ -79,11 +79,23 @@ __global__ void k_ct_count(Geo g, int n, const unsigned *__restrict__ list, cons
   cellOf[e] = c;
   rankIn[e] = atomicAdd(&cnt[c], 1u);
 }
-__global__ void k_ct_scatter(int n, const unsigned *__restrict__ cellOf, const unsigned *__restrict__ rankIn,
-                             const unsigned *__restrict__ start, unsigned *__restrict__ binned) {
+// a binned member with its position copied beside it, so that a
+// neighbour search reads one 16-byte record per candidate
+struct __align__(16) Slot {
+  float x, y, z;
+  unsigned o;  // subset position
+};
+
+__global__ void k_ct_scatter(int n, const unsigned *__restrict__ list, const float *__restrict__ px,
+                             const float *__restrict__ py, const float *__restrict__ pz,
+                             const unsigned *__restrict__ cellOf, const unsigned *__restrict__ rankIn,
+                             const unsigned *__restrict__ start, unsigned *__restrict__ binned, Slot *__restrict__ slot) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  binned[start[cellOf[e]] + rankIn[e]] = (unsigned)e;  // subset position
+  unsigned b = start[cellOf[e]] + rankIn[e];
+  binned[b] = (unsigned)e;  // subset position
+  unsigned p = list[e];
+  slot[b] = Slot{px[p], py[p], pz[p], (unsigned)e};
 }
 
 // Parents only ever point to smaller indices, so every walk ends. Reads and
@@ -104,12 +116,14 @@ __device__ __forceinline__ unsigned uf_find(unsigned *parent, unsigned x) {
 }
 
 // union of subset members closer than thr (the smaller index becomes the
-// root); with a mask only masked members take part, the others stay alone
+// root); with a mask only masked members take part, the others stay alone.
+// The cells of one row of the search box are adjacent in the cell order, so
+// their members form one run of slots.
 __global__ void k_ct_union(Geo g, int n, const unsigned *__restrict__ list, const float *__restrict__ px,
                            const float *__restrict__ py, const float *__restrict__ pz,
                            const unsigned *__restrict__ cellOf, const unsigned *__restrict__ start,
-                           const unsigned *__restrict__ binned, float thr2, int span,
-                           const unsigned *__restrict__ mask, unsigned *__restrict__ parent) {
+                           const Slot *__restrict__ slot, float thr2, int span, const unsigned *__restrict__ mask,
+                           unsigned *__restrict__ parent) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   if (mask && !mask[e]) return;
@@ -117,39 +131,41 @@ __global__ void k_ct_union(Geo g, int n, const unsigned *__restrict__ list, cons
   float x = px[p], y = py[p], z = pz[p];
   int c = cellOf[e];
   int ci = c % g.nx, cj = (c / g.nx) % g.ny, ck = c / (g.nx * g.ny);
-  for (int dk = -span; dk <= span; dk++)
-    for (int dj = -span; dj <= span; dj++)
-      for (int di = -span; di <= span; di++) {
-        int ii = ci + di, jj = cj + dj, kk = ck + dk;
-        if (ii < 0 || jj < 0 || kk < 0 || ii >= g.nx || jj >= g.ny || kk >= g.nz) continue;
-        int cc = (kk * g.ny + jj) * g.nx + ii;
-        for (unsigned b = start[cc]; b < start[cc + 1]; b++) {
-          unsigned o = binned[b];
-          if (o <= (unsigned)e) continue;
-          if (mask && !mask[o]) continue;
-          unsigned q = list[o];
-          float ddx = px[q] - x, ddy = py[q] - y, ddz = pz[q] - z;
-          if (ddx * ddx + ddy * ddy + ddz * ddz >= thr2) continue;
-          // link the larger root under the smaller one
-          for (int guard = 0; guard < 1 << 20; guard++) {
-            unsigned ra = uf_find(parent, (unsigned)e), rb = uf_find(parent, o);
-            if (ra == rb) break;
-            unsigned hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
-            if (atomicCAS(&parent[hi], hi, lo) == hi) break;
-          }
+  const int i0 = max(ci - span, 0), i1 = min(ci + span, g.nx - 1);
+  for (int kk = max(ck - span, 0); kk <= min(ck + span, g.nz - 1); kk++)
+    for (int jj = max(cj - span, 0); jj <= min(cj + span, g.ny - 1); jj++) {
+      const int row = (kk * g.ny + jj) * g.nx;
+      for (unsigned b = start[row + i0], be = start[row + i1 + 1]; b < be; b++) {
+        const Slot s = slot[b];
+        unsigned o = s.o;
+        if (o <= (unsigned)e) continue;
+        if (mask && !mask[o]) continue;
+        float ddx = s.x - x, ddy = s.y - y, ddz = s.z - z;
+        if (ddx * ddx + ddy * ddy + ddz * ddz >= thr2) continue;
+        // link the larger root under the smaller one
+        for (int guard = 0; guard < 1 << 20; guard++) {
+          unsigned ra = uf_find(parent, (unsigned)e), rb = uf_find(parent, o);
+          if (ra == rb) break;
+          unsigned hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
+          if (atomicCAS(&parent[hi], hi, lo) == hi) break;
         }
       }
+    }
 }
 
 // component key: (root, position in the x order); members outside the
-// mask get the all-ones key and sort after every group
+// mask get the all-ones key and sort after every group. The key is also
+// written at the member's slot.
 __global__ void k_ct_groupkeys(int n, unsigned *__restrict__ parent, const unsigned *__restrict__ rankOf,
-                               const unsigned *__restrict__ mask, unsigned long long *__restrict__ key) {
+                               const unsigned *__restrict__ mask, const unsigned *__restrict__ cellOf,
+                               const unsigned *__restrict__ rankIn, const unsigned *__restrict__ start,
+                               unsigned long long *__restrict__ key, unsigned long long *__restrict__ slotKey) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  if (mask && !mask[e]) { key[e] = ~0ull; return; }
-  unsigned r = uf_find(parent, (unsigned)e);
-  key[e] = ((unsigned long long)r << 32) | rankOf[e];
+  unsigned long long kv = ~0ull;
+  if (!mask || mask[e]) kv = ((unsigned long long)uf_find(parent, (unsigned)e) << 32) | rankOf[e];
+  key[e] = kv;
+  slotKey[start[cellOf[e]] + rankIn[e]] = kv;
 }
 
 // position of each member in the x order
@@ -305,12 +321,13 @@ constexpr int kVcK = 32;  // partners kept per parcel (more: the group is swept 
 __global__ void k_vc_cand(Geo g, int n, const unsigned *__restrict__ list, const float *__restrict__ px,
                           const float *__restrict__ py, const float *__restrict__ pz,
                           const unsigned *__restrict__ cellOf, const unsigned *__restrict__ start,
-                          const unsigned *__restrict__ binned, int span, const unsigned *__restrict__ rankOf,
+                          const Slot *__restrict__ slot, const unsigned long long *__restrict__ slotKey, int span,
                           const unsigned long long *__restrict__ key, float thr2, unsigned *__restrict__ cand,
                           unsigned *__restrict__ ncand, unsigned *__restrict__ serialOnly) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  unsigned p = list[e], re = rankOf[e], root = (unsigned)(key[e] >> 32);
+  const unsigned long long ke = key[e];
+  unsigned p = list[e], re = (unsigned)ke, root = (unsigned)(ke >> 32);
   float x = px[p], y = py[p], z = pz[p];
   int c = cellOf[e];
   int ci = c % g.nx, cj = (c / g.nx) % g.ny, ck = c / (g.nx * g.ny);
@@ -318,26 +335,24 @@ __global__ void k_vc_cand(Geo g, int n, const unsigned *__restrict__ list, const
   unsigned rk[kVcK];
   int m = 0;
   bool over = false;
-  for (int dk = -span; dk <= span; dk++)
-    for (int dj = -span; dj <= span; dj++)
-      for (int di = -span; di <= span; di++) {
-        int ii = ci + di, jj = cj + dj, kk = ck + dk;
-        if (ii < 0 || jj < 0 || kk < 0 || ii >= g.nx || jj >= g.ny || kk >= g.nz) continue;
-        int cc = (kk * g.ny + jj) * g.nx + ii;
-        for (unsigned b = start[cc]; b < start[cc + 1]; b++) {
-          unsigned o = binned[b];
-          unsigned ro = rankOf[o];
-          if (ro <= re || (unsigned)(key[o] >> 32) != root) continue;
-          unsigned q = list[o];
-          float ddx = px[q] - x, ddy = py[q] - y, ddz = pz[q] - z;
-          if (ddx * ddx + ddy * ddy + ddz * ddz >= thr2) continue;
-          if (m == kVcK) { over = true; continue; }
-          int k = m++;
-          while (k > 0 && rk[k - 1] > ro) { rk[k] = rk[k - 1]; mine[k] = mine[k - 1]; k--; }
-          rk[k] = ro;
-          mine[k] = o;
-        }
+  const int i0 = max(ci - span, 0), i1 = min(ci + span, g.nx - 1);
+  for (int kk = max(ck - span, 0); kk <= min(ck + span, g.nz - 1); kk++)
+    for (int jj = max(cj - span, 0); jj <= min(cj + span, g.ny - 1); jj++) {
+      const int row = (kk * g.ny + jj) * g.nx;
+      for (unsigned b = start[row + i0], be = start[row + i1 + 1]; b < be; b++) {
+        const unsigned long long ko = slotKey[b];
+        unsigned ro = (unsigned)ko;
+        if (ro <= re || (unsigned)(ko >> 32) != root) continue;
+        const Slot s = slot[b];
+        float ddx = s.x - x, ddy = s.y - y, ddz = s.z - z;
+        if (ddx * ddx + ddy * ddy + ddz * ddz >= thr2) continue;
+        if (m == kVcK) { over = true; continue; }
+        int k = m++;
+        while (k > 0 && rk[k - 1] > ro) { rk[k] = rk[k - 1]; mine[k] = mine[k - 1]; k--; }
+        rk[k] = ro;
+        mine[k] = s.o;
       }
+    }
   ncand[e] = (unsigned)m;
   if (over) __hip_atomic_store(&serialOnly[root], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -837,7 +852,7 @@ constexpr int kVlWaves = 4;  // parcels per 256-thread workgroup
 __global__ __launch_bounds__(256) void k_ct_vapor_liquid(Geo g, int nV, const unsigned *__restrict__ vlist,
                                                          const unsigned *__restrict__ llist,
                                                          const unsigned *__restrict__ lstart,
-                                                         const unsigned *__restrict__ lbinned, float rad,
+                                                         const Slot *__restrict__ lslot, float rad,
                                                          float *__restrict__ px, float *__restrict__ py,
                                                          float *__restrict__ pz, float *__restrict__ vx,
                                                          float *__restrict__ vy, float *__restrict__ vz,
@@ -864,11 +879,11 @@ __global__ __launch_bounds__(256) void k_ct_vapor_liquid(Geo g, int nV, const un
     int ii = ia + cidx % ni, jj = ja + (cidx / ni) % nj, kk = ka + cidx / (ni * nj);
     int cc = (kk * g.ny + jj) * g.nx + ii;
     for (unsigned b = lstart[cc]; b < lstart[cc + 1]; b++) {
-      unsigned p = llist[lbinned[b]];
-      float ddx = px[p] - x, ddy = py[p] - y, ddz = pz[p] - z;
+      const Slot s = lslot[b];
+      float ddx = s.x - x, ddy = s.y - y, ddz = s.z - z;
       if (ddx * ddx + ddy * ddy + ddz * ddz >= reach2) continue;
-      unsigned slot = atomicAdd(&sn[wv], 1u);
-      if (slot < (unsigned)kVlMax) scand[wv][slot] = p;
+      unsigned at = atomicAdd(&sn[wv], 1u);
+      if (at < (unsigned)kVlMax) scand[wv][at] = llist[s.o];
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -884,9 +899,9 @@ __global__ __launch_bounds__(256) void k_ct_vapor_liquid(Geo g, int nV, const un
       int ii = ia + cidx % ni, jj = ja + (cidx / ni) % nj, kk = ka + cidx / (ni * nj);
       int cc = (kk * g.ny + jj) * g.nx + ii;
       for (unsigned b = lstart[cc]; b < lstart[cc + 1] && m < kVlMax; b++) {
-        unsigned p = llist[lbinned[b]];
-        float ddx = px[p] - x, ddy = py[p] - y, ddz = pz[p] - z;
-        if (ddx * ddx + ddy * ddy + ddz * ddz < reach2) scand[wv][m++] = p;
+        const Slot s = lslot[b];
+        float ddx = s.x - x, ddy = s.y - y, ddz = s.z - z;
+        if (ddx * ddx + ddy * ddy + ddz * ddz < reach2) scand[wv][m++] = llist[s.o];
       }
     }
   }
@@ -964,7 +979,8 @@ struct Contacts {
            *rankOf = nullptr, *gend = nullptr;
   unsigned *cnt = nullptr, *start = nullptr, *lstart = nullptr, *lbinned = nullptr, *lcell = nullptr,
            *lrank = nullptr, *counts = nullptr;
-  unsigned long long *xk = nullptr, *xk2 = nullptr, *gk = nullptr, *gk2 = nullptr;
+  unsigned long long *xk = nullptr, *xk2 = nullptr, *gk = nullptr, *gk2 = nullptr, *slotKey = nullptr;
+  Slot *slot = nullptr, *lslot = nullptr;
   void *tmp = nullptr;
   size_t tmpBytes = 0;
   // dependency-ordered sweep of large vapor groups, sized by the vapor count
@@ -1003,12 +1019,13 @@ int contacts_ensure(ws_ctx *c) {
       (r = calloc_dev(c, &k->cnt, C)) || (r = calloc_dev(c, &k->start, C)) || (r = calloc_dev(c, &k->lstart, C)) ||
       (r = calloc_dev(c, &k->lbinned, n)) || (r = calloc_dev(c, &k->lcell, n)) || (r = calloc_dev(c, &k->lrank, n)) ||
       (r = calloc_dev(c, &k->counts, 4)) || (r = calloc_dev(c, &k->xk, n)) || (r = calloc_dev(c, &k->xk2, n)) ||
-      (r = calloc_dev(c, &k->gk, n)) || (r = calloc_dev(c, &k->gk2, n)))
+      (r = calloc_dev(c, &k->gk, n)) || (r = calloc_dev(c, &k->gk2, n)) || (r = calloc_dev(c, &k->slotKey, n)) ||
+      (r = calloc_dev(c, &k->slot, n)) || (r = calloc_dev(c, &k->lslot, n)))
     return r;
   size_t a = 0, b = 0, d = 0;
-  hipcub::DeviceScan::ExclusiveSum(nullptr, a, k->fv, k->ov, (int)n, c->stream);
-  hipcub::DeviceScan::ExclusiveSum(nullptr, b, k->cnt, k->start, (int)C, c->stream);
-  hipcub::DeviceRadixSort::SortKeys(nullptr, d, k->xk, k->xk2, (int)n, 0, 64, c->stream);
+  WS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, a, k->fv, k->ov, (int)n, c->stream));
+  WS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b, k->cnt, k->start, (int)C, c->stream));
+  WS_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, d, k->xk, k->xk2, (int)n, 0, 64, c->stream));
   k->tmpBytes = a > b ? a : b;
   if (d > k->tmpBytes) k->tmpBytes = d;
   if ((r = calloc_dev(c, (char **)&k->tmp, k->tmpBytes))) return r;
@@ -1017,7 +1034,7 @@ int contacts_ensure(ws_ctx *c) {
 }
 
 int bin_subset(ws_ctx *c, int n, const unsigned *list, unsigned *cellOf, unsigned *rankIn, unsigned *start,
-               unsigned *binned) {
+               unsigned *binned, Slot *slot) {
   Contacts *k = c->contacts;
   long C = c->nC;
   hipLaunchKernelGGL(k_ct_zero, dim3(blocks_for(C + 1, 256)), dim3(256), 0, c->stream, C + 1, k->cnt);
@@ -1027,8 +1044,8 @@ int bin_subset(ws_ctx *c, int n, const unsigned *list, unsigned *cellOf, unsigne
   WS_LAUNCHED();
   WS_HIP(hipcub::DeviceScan::ExclusiveSum(k->tmp, k->tmpBytes, k->cnt, start, (int)C + 1, c->stream));
   if (n > 0)
-    hipLaunchKernelGGL(k_ct_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, cellOf, rankIn, start,
-                       binned);
+    hipLaunchKernelGGL(k_ct_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, n, list, c->px, c->py, c->pz,
+                       cellOf, rankIn, start, binned, slot);
   WS_LAUNCHED();
   return WS_OK;
 }
@@ -1052,11 +1069,12 @@ int group_subset(ws_ctx *c, int n, const unsigned *list, float thr, const unsign
   hipLaunchKernelGGL(k_ct_rank, dim3(nb), dim3(256), 0, c->stream, n, k->xk, k->posOf, k->rankOf);
   WS_LAUNCHED();
   int r;
-  if ((r = bin_subset(c, n, list, k->cellOf, k->rankIn, k->start, k->binned))) return r;
+  if ((r = bin_subset(c, n, list, k->cellOf, k->rankIn, k->start, k->binned, k->slot))) return r;
   hipLaunchKernelGGL(k_ct_iota, dim3(nb), dim3(256), 0, c->stream, n, k->parent);
   hipLaunchKernelGGL(k_ct_union, dim3(nb), dim3(256), 0, c->stream, c->geo, n, list, c->px, c->py, c->pz, k->cellOf,
-                     k->start, k->binned, thr * thr, cell_span(c, thr), mask, k->parent);
-  hipLaunchKernelGGL(k_ct_groupkeys, dim3(nb), dim3(256), 0, c->stream, n, k->parent, k->rankOf, mask, k->gk2);
+                     k->start, k->slot, thr * thr, cell_span(c, thr), mask, k->parent);
+  hipLaunchKernelGGL(k_ct_groupkeys, dim3(nb), dim3(256), 0, c->stream, n, k->parent, k->rankOf, mask, k->cellOf,
+                     k->rankIn, k->start, k->gk2, k->slotKey);
   WS_LAUNCHED();
   WS_HIP(hipcub::DeviceRadixSort::SortKeys(k->tmp, k->tmpBytes, k->gk2, k->gk, n, 0, 64, c->stream));
   hipLaunchKernelGGL(k_ct_group_ends, dim3(nb), dim3(256), 0, c->stream, n, k->gk, k->gend);
@@ -1087,8 +1105,8 @@ int vapor_ensure(ws_ctx *c, long nV) {
       (r = calloc_dev(c, &k->stepP, n)) || (r = calloc_dev(c, &k->stepN, n)) || (r = calloc_dev(c, &k->succ, np)) || (r = calloc_dev(c, &k->depc, n)))
     return r;
   size_t a = 0, b = 0;
-  hipcub::DeviceScan::ExclusiveSum(nullptr, a, k->ncand, k->pairOff, (int)(n + 1), c->stream);
-  hipcub::DeviceRadixSort::SortKeys(nullptr, b, k->pairKeys2, k->pairKeys, (int)np, 0, 64, c->stream);
+  WS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, a, k->ncand, k->pairOff, (int)(n + 1), c->stream));
+  WS_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, b, k->pairKeys2, k->pairKeys, (int)np, 0, 64, c->stream));
   k->vtmpBytes = a > b ? a : b;
   if ((r = calloc_dev(c, (char **)&k->vtmp, k->vtmpBytes))) return r;
   k->vcap = n;
@@ -1127,7 +1145,7 @@ void contacts_release(ws_ctx *c, bool all) {
                   k->lrank, k->counts, k->xk, k->xk2, k->gk, k->gk2, k->tmp, k->cand, k->ncand, k->pairOff,
                   k->need, k->done, k->serialOnly, k->big, k->vcCounters, k->chainStart, k->pairKeys,
                   k->pairKeys2, k->vtmp, k->candT, k->candP, k->tpos, k->stepP, k->stepN, k->succ,
-                  k->depc};
+                  k->depc, k->slotKey, k->slot, k->lslot};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (all) {
@@ -1172,7 +1190,7 @@ int frame_vapor_collisions(ws_ctx *c) {
     if (nBig > 0) {
       // partners, the steps touching each parcel in sweep order, positions
       hipLaunchKernelGGL(k_vc_cand, dim3(nb), dim3(256), 0, c->stream, c->geo, nV, k->vlist, c->px, c->py, c->pz,
-                         k->cellOf, k->start, k->binned, cell_span(c, 2.0 * rad), k->rankOf, k->gk2,
+                         k->cellOf, k->start, k->slot, k->slotKey, cell_span(c, 2.0 * rad), k->gk2,
                          4.f * rad * rad, k->cand, k->ncand, k->serialOnly);
       hipLaunchKernelGGL(k_vc_pair_count, dim3(blocks_for(nV + 1, 256)), dim3(256), 0, c->stream, nV, k->ncand,
                          k->pairOff);
@@ -1233,10 +1251,10 @@ int frame_vapor_collisions(ws_ctx *c) {
     // liquid partners, binned by cell (they do not move in this pass)
     int nL = 0;
     if ((r = subset(c, k->fl, k->ol, k->llist, &nL))) return r;
-    if ((r = bin_subset(c, nL, k->llist, k->lcell, k->lrank, k->lstart, k->lbinned))) return r;
+    if ((r = bin_subset(c, nL, k->llist, k->lcell, k->lrank, k->lstart, k->lbinned, k->lslot))) return r;
     hipLaunchKernelGGL(k_ct_vapor_liquid, dim3((nV + kVlWaves - 1) / kVlWaves), dim3(256), 0, c->stream, c->geo, nV,
                        k->vlist,
-                       k->llist, k->lstart, k->lbinned, rad, c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz,
+                       k->llist, k->lstart, k->lslot, rad, c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz,
                        k->counts + 2);
     WS_LAUNCHED();
   }

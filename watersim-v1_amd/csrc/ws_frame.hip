@@ -139,7 +139,7 @@ int frame_currents(ws_ctx *c, double dt) {
   Q.shell = (float)shell;
   long gN = (long)Q.gnx * Q.gny * Q.gnz;
   if (c->currN < gN) {
-    if (c->currField) hipFree(c->currField);
+    if (c->currField) (void)hipFree(c->currField);
     c->currField = nullptr;
     WS_HIP(hipMalloc((void **)&c->currField, sizeof(float) * 3 * gN));
     c->currN = gN;
