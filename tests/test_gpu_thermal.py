@@ -195,3 +195,70 @@ def test_vapor_liquid_crowded_parcel():
         assert e <= 2e-5, (k, e)
         moved += int(abs(float(outs[0][k][v]) - float(parts[k][v])) > 0)
     assert moved > 0  # the crowded parcel was pushed
+
+
+LCG_A, LCG_C = 1664525, 1013904223
+
+
+def lcg_back(s, n):
+    """The stream state n draws before s."""
+    inv = pow(LCG_A, -1, 1 << 32)
+    for _ in range(n):
+        s = ((s - LCG_C) * inv) % (1 << 32)
+    return s
+
+
+@pytest.mark.parametrize("dup,rate,reject_at", [(3, 0.02, None), (3, 1.0, None), (1, 0.3, None), (3, 1e9, None),
+                                                 (1, 1e9, 19)])
+def test_evaporation_windows(dup, rate, reject_at):
+    """The chooser tests a window of candidates under several counts of
+    earlier ejections at once; this drives each way a window ends against
+    the oracle's serial walk of the _evS stream: the scene's particles
+    repeated dup times (5151 candidates, more than one window of 4096), the
+    scene's rate times `rate` (0.02: 14 ejections, so whole windows; 1.0 and
+    0.3: about one candidate in ten and one in thirty; 1e9: every candidate
+    ejects, so windows end after their sixteenth ejection), and a start
+    state whose stream puts a value below
+    the 1e-9 cut at the first normal of candidate reject_at (every candidate
+    ejecting takes seven draws, that one eight). Counts, the stream state
+    after the pass and the flags exact; velocities within 1e-5."""
+    import cpu_oracle  # noqa: F401  (the oracle library must be built)
+    from scenes import oracle_from_golden, tick_params, PART_KEYS, EXTRA_KEYS
+    g = golden.load("thermal_32")
+    keys = [k for k in PART_KEYS + EXTRA_KEYS if "t1_" + k in g]
+    parts = {k: np.concatenate([g["t1_" + k]] * dup) for k in keys}
+    N = len(parts["px"])
+    act = g.meta["tick"]["sunActivity"] * rate
+    ev0 = g.meta["tick"]["evS_before"] if reject_at is None else lcg_back(2, 7 * reject_at + 2)
+    dt = g.meta["tick"]["dtTick"]
+    s = gpu_from_golden(g, tag="t1_", capacity=N)
+    s.set(nP=N)
+    for k in keys:
+        s.upload(k, parts[k])
+    s.upload("cellType", g["t0_cellType"])
+    s.upload("dens", g["t0_dens"])
+    for k in "uvw":
+        s.upload(k, g["t0_" + k])
+    rb = g.meta["rng_before"]
+    s.set_rng(rb["cfS"], rb["bfS"], ev0)
+    s.set(sunActivity=act)
+    s.stage("evaporation", dt)
+    st = oracle_from_golden(g, tag="t1_")
+    st.set_particles(N, **parts)
+    st.arrays["cellType"][:] = g["t0_cellType"]
+    st.arrays["dens"][:] = g["t0_dens"]
+    for k in "uvw":
+        st.arrays[k][:] = g["t0_" + k]
+    st.set_params(**tick_params(g))
+    st.set_params(sunActivity=act, evS=ev0)
+    st.call("oc_update_evaporation", dt)
+    n_gpu, n_ref = int(s.get("evapCount")), st.get_i("evapCount")
+    print("  %d particles: evaporated gpu %d oracle %d" % (N, n_gpu, n_ref))
+    assert n_gpu == n_ref and n_ref > 0
+    assert int(s.get_rng()[2]) == st.get_i("evS")
+    assert np.array_equal(s.download("pflag"), st["pflag"][:N])
+    for k in ("pvx", "pvy", "pvz", "pAir"):
+        e = np.abs(s.download(k).astype(np.float64) - st[k][:N].astype(np.float64)).max()
+        print("  %-4s max |err| %.3e" % (k, e))
+        assert e <= 1e-5, (k, e)
+    s.close()

@@ -790,7 +790,7 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
       (r = dalloc(c, &c->vscr, 4 * cap)) || (r = dalloc(c, &c->rng, 4)) || (r = dalloc(c, &c->ds, 1)) ||
       (r = dalloc(c, &c->pLight, cap)) || (r = dalloc(c, &c->pDepth, cap)) || (r = dalloc(c, &c->pWx, cap)) ||
       (r = dalloc(c, &c->pWy, cap)) || (r = dalloc(c, &c->pWz, cap)) || (r = dalloc(c, &c->pPh, cap)) ||
-      (r = dalloc(c, &c->evBuf, 7 * cap + 4)) || (r = dalloc(c, &c->jmp, 128)) ||
+      (r = dalloc(c, &c->evBuf, 7 * cap + 4)) || (r = dalloc(c, &c->jmp, 2 * EV_WIN)) ||
       (r = dalloc(c, &c->mtCount, c->nC + 1))) {
     ws_destroy(c);
     return r;
@@ -843,16 +843,19 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
     return WS_EHIP;
   }
   {
-    // affine maps of 1..64 LCG steps: s -> A s + C (mod 2^32)
-    uint32_t J[128];
+    // affine maps of 1..EV_WIN LCG steps: s -> A s + C (mod 2^32)
+    std::vector<uint32_t> J(2 * EV_WIN);
     uint32_t A = 1u, C = 0u;
-    for (int n = 0; n < 64; n++) {
+    for (int n = 0; n < EV_WIN; n++) {
       A = A * 1664525u;
       C = C * 1664525u + 1013904223u;
       J[n] = A;
-      J[64 + n] = C;
+      J[EV_WIN + n] = C;
     }
-    if (hipMemcpy(c->jmp, J, sizeof(J), hipMemcpyHostToDevice) != hipSuccess) { ws_destroy(c); return WS_EHIP; }
+    if (hipMemcpy(c->jmp, J.data(), J.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+      ws_destroy(c);
+      return WS_EHIP;
+    }
   }
   // extrapolation tile schedule, grouped by tile diagonal I+J+K
   {

@@ -27,6 +27,11 @@ constexpr int EX_TI = 16, EX_TJ = 16, EX_TK = 16;  // extrapolation tile (256 la
 constexpr int SOR_TX = 32, SOR_TY = 8, SOR_TZ = 8;
 constexpr int SOR_TB_X = 16, SOR_TB_Y = 8, SOR_TB_Z = 8;  // temporal-blocking tile
 
+// Evaporation chooser: candidates per window, threads of its one workgroup,
+// and the ejection counts per window tested at once (each ejection moves
+// the stream by six draws)
+constexpr int EV_WIN = 4096, EV_THREADS = 1024, EV_HYP = 16;
+
 // Scalars that live on the device so that substeps can run back to back
 // without host round trips (dt law, gravity ramp, reductions).
 struct DevScalars {
@@ -116,7 +121,7 @@ struct Ctx {
   uint8_t *tsolid = nullptr, *pDepth = nullptr;
   float *pLight = nullptr, *pWx = nullptr, *pWy = nullptr, *pWz = nullptr, *pPh = nullptr;
   unsigned *evBuf = nullptr;    // 6 x capacity: cand, off, thresh, list, thrC, evList | evState
-  uint32_t *jmp = nullptr;      // 2 x 64 jump-ahead coefficients of the LCG
+  uint32_t *jmp = nullptr;      // 2 x EV_WIN jump-ahead coefficients of the LCG (A, then C)
   float *currField = nullptr;   // 3 x lattice
   long currN = 0;
   // live timers (HIP events on the context stream), enabled by "timing"

@@ -221,61 +221,106 @@ __global__ void k_evap_compact(int nP, const unsigned *__restrict__ cand, const 
 
 __device__ __forceinline__ uint32_t lcg1(uint32_t s) { return s * 1664525u + 1013904223u; }
 
-// Walks the candidates in particle order with one wavefront: each lane takes
-// the next candidate and the stream state it would draw if no earlier lane
-// ejects (the jump-ahead maps for 1..64 draws are in jmpA/jmpC). The first
-// ejecting lane is exact; the stream then skips its three Gaussian draws
-// (two draws each, the first repeated while it is below 1e-9, solver.js:3120)
-// and the walk resumes after it. Thresholds stream through an LDS window.
-// Records each ejected particle with the stream state its Gaussian draws
-// start from.
-__global__ __launch_bounds__(64) void k_evap_chooser(const unsigned *__restrict__ list, const unsigned *__restrict__ thrC,
-                                                     const unsigned *__restrict__ off, const unsigned *__restrict__ cand,
-                                                     int nP, const uint32_t *__restrict__ jmpA,
-                                                     const uint32_t *__restrict__ jmpC, uint32_t *__restrict__ rng,
-                                                     unsigned *__restrict__ evList, unsigned *__restrict__ evState,
-                                                     unsigned *__restrict__ evCount) {
-  constexpr int WIN = 4096;
-  __shared__ unsigned sthr[WIN];
-  int lane = threadIdx.x;
-  unsigned nCand = nP > 0 ? off[nP - 1] + cand[nP - 1] : 0;
-  uint32_t s = rng[2];
-  uint32_t A = jmpA[lane], C = jmpC[lane];  // lane+1 draws
-  uint32_t A64 = jmpA[63], C64 = jmpC[63];
-  unsigned k = 0, nev = 0, wbase = 0, wend = 0;
-  while (k < nCand) {
-    if (k + 64u > wend && wend < nCand) {  // refill the window at k
-      wbase = k;
-      wend = min(nCand, wbase + (unsigned)WIN);
-      __syncthreads();
-      for (unsigned e = lane; e < wend - wbase; e += 64) sthr[e] = thrC[wbase + e];
-      __syncthreads();
+// Walks the candidates in particle order through windows of EV_WIN. Without
+// an ejection each candidate takes one draw; an ejection then takes its three
+// Gaussian normals (two draws each, the first repeated while it is below
+// 1e-9, solver.js:3120), almost always six draws. So if m candidates of a
+// window before candidate j ejected, j draws state A_(j+1+6m) s + C_(j+1+6m)
+// from the window's start state s. Every thread tests its candidates of the
+// window under m = 0..EV_HYP-1 at once (bit rows per m in LDS); one
+// wavefront then follows the real sequence: the first ejection under m = 0,
+// the next one after it under m = 1, and so on. The window ends early after
+// the EV_HYP-th ejection or after an ejection whose normals took other than
+// six draws, and the next one starts behind it. Records each ejected
+// particle with the stream state its Gaussian draws start from.
+__global__ __launch_bounds__(EV_THREADS) void k_evap_chooser(const unsigned *__restrict__ list,
+                                                             const unsigned *__restrict__ thrC,
+                                                             const unsigned *__restrict__ off,
+                                                             const unsigned *__restrict__ cand, int nP,
+                                                             const uint32_t *__restrict__ jmpA,
+                                                             const uint32_t *__restrict__ jmpC,
+                                                             uint32_t *__restrict__ rng,
+                                                             unsigned *__restrict__ evList,
+                                                             unsigned *__restrict__ evState,
+                                                             unsigned *__restrict__ evCount) {
+  constexpr int U = EV_WIN / EV_THREADS, WORDS = EV_WIN / 64;
+  static_assert(WORDS == 64, "one bit-row word per lane of the resolving wavefront");
+  __shared__ unsigned long long rows[EV_HYP][WORDS];
+  __shared__ uint32_t sA[EV_WIN], sC[EV_WIN];
+  __shared__ uint32_t shS;
+  __shared__ unsigned shK, shN;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const unsigned nCand = nP > 0 ? off[nP - 1] + cand[nP - 1] : 0;
+  for (int e = tid; e < EV_WIN; e += EV_THREADS) { sA[e] = jmpA[e]; sC[e] = jmpC[e]; }
+  // the thread's candidates of a window: j = u * EV_THREADS + tid
+  uint32_t myA[U], myC[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) { myA[u] = jmpA[u * EV_THREADS + tid]; myC[u] = jmpC[u * EV_THREADS + tid]; }
+  const uint32_t A6 = jmpA[5], C6 = jmpC[5];
+  if (tid == 0) { shS = rng[2]; shK = 0; shN = 0; }
+  __syncthreads();
+  while (true) {
+    const unsigned kw = shK;
+    const uint32_t s = shS;
+    if (kw >= nCand) break;  // uniform: every thread read the same kw after the barrier
+    const unsigned nw = nCand - kw < (unsigned)EV_WIN ? nCand - kw : (unsigned)EV_WIN;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const unsigned j = (unsigned)(u * EV_THREADS + tid);
+      const bool valid = j < nw;
+      const unsigned lim = valid ? thrC[kw + j] : 0u;
+      uint32_t st = myA[u] * s + myC[u];
+      for (int m = 0; m < EV_HYP; m++) {
+        unsigned long long b = __ballot(valid && st <= lim);
+        if (lane == 0) rows[m][u * (EV_THREADS / 64) + wv] = b;
+        st = A6 * st + C6;
+      }
     }
-    unsigned idx = k + lane;
-    bool valid = idx < nCand;
-    uint32_t st = A * s + C;
-    bool ej = valid && st <= sthr[valid ? idx - wbase : 0];
-    unsigned long long m = __ballot(ej);
-    unsigned nvalid = nCand - k < 64u ? nCand - k : 64u;
-    if (!m) {
-      s = nvalid == 64u ? A64 * s + C64 : (uint32_t)__shfl((int)st, (int)nvalid - 1);
-      k += nvalid;
-      continue;
+    __syncthreads();
+    if (wv == 0) {
+      unsigned pos = 0, nev = shN;
+      uint32_t next = 0;
+      bool cut = false;
+      int m = 0;
+      for (; m < EV_HYP; m++) {
+        // first candidate at or after pos that ejects when m ejected before it
+        unsigned long long w = rows[m][lane];
+        unsigned base = (unsigned)lane * 64u;
+        if (base + 64u <= pos) w = 0;
+        else if (base < pos) w &= ~0ull << (pos - base);
+        unsigned long long any = __ballot(w != 0);
+        if (!any) break;
+        int fl = __ffsll((long long)any) - 1;
+        unsigned long long wf = __shfl(w, fl);
+        unsigned j = (unsigned)fl * 64u + (unsigned)(__ffsll((long long)wf) - 1);
+        uint32_t se = sA[j] * s + sC[j];
+        for (int q = 0; q < m; q++) se = A6 * se + C6;
+        uint32_t gs = se;
+        int draws = 0;
+        for (int n = 0; n < 3; n++) {
+          do { gs = lcg1(gs); draws++; } while (gs <= 4u);  // u1 = g / 2^32 < 1e-9
+          gs = lcg1(gs);
+          draws++;
+        }
+        if (lane == 0) { evList[nev] = list[kw + j]; evState[nev] = se; }
+        nev++;
+        pos = j + 1u;
+        if (draws != 6 || m == EV_HYP - 1) {  // the window ends behind this candidate
+          next = gs;
+          cut = true;
+          break;
+        }
+      }
+      if (!cut) {  // all nw candidates with m ejections among them
+        next = sA[nw - 1] * s + sC[nw - 1];
+        for (int q = 0; q < m; q++) next = A6 * next + C6;
+        pos = nw;
+      }
+      if (lane == 0) { shS = next; shK = kw + pos; shN = nev; }
     }
-    int e = __ffsll((long long)m) - 1;
-    uint32_t se = (uint32_t)__shfl((int)st, e);
-    // three Box-Muller normals after the decision draw
-    uint32_t gs = se;
-    for (int n = 0; n < 3; n++) {
-      do { gs = lcg1(gs); } while (gs <= 4u);  // u1 = g / 2^32 < 1e-9
-      gs = lcg1(gs);
-    }
-    if (lane == 0) { evList[nev] = list[k + (unsigned)e]; evState[nev] = se; }
-    nev++;
-    s = gs;
-    k += (unsigned)e + 1u;
+    __syncthreads();
   }
-  if (lane == 0) { rng[2] = s; *evCount = nev; }
+  if (tid == 0) { rng[2] = shS; *evCount = shN; }
 }
 
 // Maxwell-Boltzmann kick, radial mirror and no-slip tangential adoption of
