@@ -84,9 +84,9 @@ int frame_evaporation(ws_ctx *c, double dt) {
   WS_LAUNCHED();
   WS_HIP(hipcub::DeviceScan::ExclusiveSum(c->scanTmp, c->scanTmpBytes, cand, off, c->nP, c->stream));
   hipLaunchKernelGGL(k_evap_compact, dim3(nb), dim3(256), 0, c->stream, c->nP, cand, off, thresh, list, thrC);
-  hipLaunchKernelGGL(k_evap_chooser, dim3(1), dim3(EV_THREADS), 0, c->stream, list, thrC, off, cand, c->nP, c->jmp,
+  hipLaunchKernelGGL(k_evap_chooser, dim3(1), dim3(EV_THREADS), 0, c->stream, thrC, off, cand, c->nP, c->jmp,
                      c->jmp + EV_WIN, c->rng, evList, evState, evCount);
-  hipLaunchKernelGGL(k_evap_apply, dim3(nb), dim3(256), 0, c->stream, c->geo, E, evList, evState, evCount, c->px,
+  hipLaunchKernelGGL(k_evap_apply, dim3(nb), dim3(256), 0, c->stream, c->geo, E, list, evList, evState, evCount, c->px,
                      c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pT, c->pflag, c->pAir, c->vel, c->thr);
   WS_LAUNCHED();
   unsigned n = 0;

@@ -134,10 +134,10 @@ __global__ void k_evap_candidates(Geo g, EvapPar E, int nP, const float *px, con
                                   unsigned *cand, unsigned *thresh);
 __global__ void k_evap_compact(int nP, const unsigned *cand, const unsigned *off, const unsigned *thresh,
                                unsigned *list, unsigned *thrC);
-__global__ void k_evap_chooser(const unsigned *list, const unsigned *thrC, const unsigned *off, const unsigned *cand,
+__global__ void k_evap_chooser(const unsigned *thrC, const unsigned *off, const unsigned *cand,
                                int nP, const uint32_t *jmpA, const uint32_t *jmpC, uint32_t *rng, unsigned *evList,
                                unsigned *evState, unsigned *evCount);
-__global__ void k_evap_apply(Geo g, EvapPar E, const unsigned *evList, const unsigned *evState,
+__global__ void k_evap_apply(Geo g, EvapPar E, const unsigned *list, const unsigned *evList, const unsigned *evState,
                              const unsigned *evCount, float *px, float *py, float *pz, float *pvx, float *pvy,
                              float *pvz, const float *pT, uint8_t *pflag, float *pAir, const float *vel,
                              const float *thr);

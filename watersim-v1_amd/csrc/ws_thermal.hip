@@ -231,10 +231,10 @@ __device__ __forceinline__ uint32_t lcg1(uint32_t s) { return s * 1664525u + 101
 // wavefront then follows the real sequence: the first ejection under m = 0,
 // the next one after it under m = 1, and so on. The window ends early after
 // the EV_HYP-th ejection or after an ejection whose normals took other than
-// six draws, and the next one starts behind it. Records each ejected
-// particle with the stream state its Gaussian draws start from.
-__global__ __launch_bounds__(EV_THREADS) void k_evap_chooser(const unsigned *__restrict__ list,
-                                                             const unsigned *__restrict__ thrC,
+// six draws, and the next one starts behind it. Records the candidate
+// index of each ejection (k_evap_apply maps it to the particle) with the
+// stream state its Gaussian draws start from.
+__global__ __launch_bounds__(EV_THREADS) void k_evap_chooser(const unsigned *__restrict__ thrC,
                                                              const unsigned *__restrict__ off,
                                                              const unsigned *__restrict__ cand, int nP,
                                                              const uint32_t *__restrict__ jmpA,
@@ -244,9 +244,11 @@ __global__ __launch_bounds__(EV_THREADS) void k_evap_chooser(const unsigned *__r
                                                              unsigned *__restrict__ evState,
                                                              unsigned *__restrict__ evCount) {
   constexpr int U = EV_WIN / EV_THREADS, WORDS = EV_WIN / 64;
+  constexpr unsigned STAGE = 4 * EV_WIN;  // thresholds staged in LDS per global load
   static_assert(WORDS == 64, "one bit-row word per lane of the resolving wavefront");
   __shared__ unsigned long long rows[EV_HYP][WORDS];
   __shared__ uint32_t sA[EV_WIN], sC[EV_WIN];
+  __shared__ unsigned sthr[STAGE];
   __shared__ uint32_t shS;
   __shared__ unsigned shK, shN;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -256,20 +258,30 @@ __global__ __launch_bounds__(EV_THREADS) void k_evap_chooser(const unsigned *__r
   uint32_t myA[U], myC[U];
 #pragma unroll
   for (int u = 0; u < U; u++) { myA[u] = jmpA[u * EV_THREADS + tid]; myC[u] = jmpC[u * EV_THREADS + tid]; }
+  // maps of 1, 3, 5 and 6 draws
+  const uint32_t A1 = jmpA[0], C1 = jmpC[0], A3 = jmpA[2], C3 = jmpC[2], A5 = jmpA[4], C5 = jmpC[4];
   const uint32_t A6 = jmpA[5], C6 = jmpC[5];
   if (tid == 0) { shS = rng[2]; shK = 0; shN = 0; }
+  unsigned sb = 0, sEnd = 0;  // staged thresholds: candidates [sb, sEnd)
   __syncthreads();
   while (true) {
     const unsigned kw = shK;
     const uint32_t s = shS;
     if (kw >= nCand) break;  // uniform: every thread read the same kw after the barrier
     const unsigned nw = nCand - kw < (unsigned)EV_WIN ? nCand - kw : (unsigned)EV_WIN;
+    if (kw + nw > sEnd) {  // uniform: restage from kw (no thread still reads the old stage)
+      sb = kw;
+      sEnd = nCand - kw < STAGE ? nCand : kw + STAGE;
+      for (unsigned e = tid; e < sEnd - sb; e += EV_THREADS) sthr[e] = thrC[sb + e];
+      __syncthreads();
+    }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const unsigned j = (unsigned)(u * EV_THREADS + tid);
       const bool valid = j < nw;
-      const unsigned lim = valid ? thrC[kw + j] : 0u;
+      const unsigned lim = valid ? sthr[kw - sb + j] : 0u;
       uint32_t st = myA[u] * s + myC[u];
+#pragma unroll
       for (int m = 0; m < EV_HYP; m++) {
         unsigned long long b = __ballot(valid && st <= lim);
         if (lane == 0) rows[m][u * (EV_THREADS / 64) + wv] = b;
@@ -278,34 +290,47 @@ __global__ __launch_bounds__(EV_THREADS) void k_evap_chooser(const unsigned *__r
     }
     __syncthreads();
     if (wv == 0) {
+      unsigned long long wr[EV_HYP];
+#pragma unroll
+      for (int m = 0; m < EV_HYP; m++) wr[m] = rows[m][lane];
       unsigned pos = 0, nev = shN;
       uint32_t next = 0;
       bool cut = false;
       int m = 0;
-      for (; m < EV_HYP; m++) {
-        // first candidate at or after pos that ejects when m ejected before it
-        unsigned long long w = rows[m][lane];
-        unsigned base = (unsigned)lane * 64u;
+#pragma unroll
+      for (int h = 0; h < EV_HYP; h++) {
+        // first candidate at or after pos that ejects when h ejected before it
+        unsigned long long w = wr[h];
+        const unsigned base = (unsigned)lane * 64u;
         if (base + 64u <= pos) w = 0;
         else if (base < pos) w &= ~0ull << (pos - base);
-        unsigned long long any = __ballot(w != 0);
+        const unsigned long long any = __ballot(w != 0);
         if (!any) break;
-        int fl = __ffsll((long long)any) - 1;
-        unsigned long long wf = __shfl(w, fl);
-        unsigned j = (unsigned)fl * 64u + (unsigned)(__ffsll((long long)wf) - 1);
+        const int fl = __ffsll((long long)any) - 1;
+        const unsigned long long wf =
+            ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(w >> 32), fl) << 32) |
+            (unsigned)__builtin_amdgcn_readlane((int)(unsigned)w, fl);
+        const unsigned j = (unsigned)fl * 64u + (unsigned)(__ffsll((long long)wf) - 1);
         uint32_t se = sA[j] * s + sC[j];
-        for (int q = 0; q < m; q++) se = A6 * se + C6;
-        uint32_t gs = se;
-        int draws = 0;
-        for (int n = 0; n < 3; n++) {
-          do { gs = lcg1(gs); draws++; } while (gs <= 4u);  // u1 = g / 2^32 < 1e-9
-          gs = lcg1(gs);
-          draws++;
+        if (h > 0) se = sA[6 * h - 1] * se + sC[6 * h - 1];  // 6h more draws
+        // three Box-Muller normals: six draws unless a first draw is <= 4
+        // (u1 = g / 2^32 < 1e-9 is drawn again)
+        uint32_t gs;
+        bool six = A1 * se + C1 > 4u && A3 * se + C3 > 4u && A5 * se + C5 > 4u;
+        if (six) {
+          gs = A6 * se + C6;
+        } else {
+          gs = se;
+          for (int n = 0; n < 3; n++) {
+            do { gs = lcg1(gs); } while (gs <= 4u);
+            gs = lcg1(gs);
+          }
         }
-        if (lane == 0) { evList[nev] = list[kw + j]; evState[nev] = se; }
+        if (lane == 0) { evList[nev] = kw + j; evState[nev] = se; }
         nev++;
         pos = j + 1u;
-        if (draws != 6 || m == EV_HYP - 1) {  // the window ends behind this candidate
+        m = h + 1;
+        if (!six || h == EV_HYP - 1) {  // the window ends behind this candidate
           next = gs;
           cut = true;
           break;
@@ -313,7 +338,7 @@ __global__ __launch_bounds__(EV_THREADS) void k_evap_chooser(const unsigned *__r
       }
       if (!cut) {  // all nw candidates with m ejections among them
         next = sA[nw - 1] * s + sC[nw - 1];
-        for (int q = 0; q < m; q++) next = A6 * next + C6;
+        if (m > 0) next = sA[6 * m - 1] * next + sC[6 * m - 1];
         pos = nw;
       }
       if (lane == 0) { shS = next; shK = kw + pos; shN = nev; }
@@ -325,14 +350,15 @@ __global__ __launch_bounds__(EV_THREADS) void k_evap_chooser(const unsigned *__r
 
 // Maxwell-Boltzmann kick, radial mirror and no-slip tangential adoption of
 // the ejected particles (solver.js:3083-3105)
-__global__ void k_evap_apply(Geo g, EvapPar E, const unsigned *__restrict__ evList, const unsigned *__restrict__ evState,
+__global__ void k_evap_apply(Geo g, EvapPar E, const unsigned *__restrict__ list, const unsigned *__restrict__ evList,
+                             const unsigned *__restrict__ evState,
                              const unsigned *__restrict__ evCount, float *__restrict__ px, float *__restrict__ py,
                              float *__restrict__ pz, float *__restrict__ pvx, float *__restrict__ pvy,
                              float *__restrict__ pvz, const float *__restrict__ pT, uint8_t *__restrict__ pflag,
                              float *__restrict__ pAir, const float *__restrict__ vel, const float *__restrict__ thr) {
   unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= *evCount) return;
-  unsigned p = evList[e];
+  unsigned p = list[evList[e]];
   uint32_t gst = evState[e];
   float nrm[3];
   for (int n = 0; n < 3; n++) {
