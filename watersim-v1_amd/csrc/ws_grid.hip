@@ -96,57 +96,27 @@ __device__ int ceiling_demote(const World &w, const Geo &g, float x, float y, fl
   return df_gt0(md) ? 1 : 0;
 }
 
-// rank of the lane's particle within its cell, cnt[c] raised by the cell's
-// count: one atomic per distinct cell of the wavefront (its particles are
-// mostly neighbours in memory and in space), ranks in lane order after the
-// atomic's base. Lanes with c < 0 take no part; every lane must call.
-__device__ __forceinline__ unsigned wave_cell_rank(unsigned *cnt, int c) {
-  const int lane = threadIdx.x & 63;
-  unsigned long long todo = __ballot(c >= 0);
-  unsigned rank = 0;
-  while (todo) {
-    const int leader = __ffsll((long long)todo) - 1;
-    const int lc = __shfl(c, leader);
-    const unsigned long long same = __ballot(c == lc);
-    unsigned base = 0;
-    if (lane == leader) base = atomicAdd(&cnt[lc], (unsigned)__popcll(same));
-    base = (unsigned)__shfl((int)base, leader);
-    if (c == lc) rank = base + (unsigned)__popcll(same & ((1ull << lane) - 1ull));
-    todo &= ~same;
-  }
-  return rank;
-}
-
 // _markFluidCells, solver.js:945-1000, plus the counting-sort histogram of
 // the particles that stay grid-coupled (their bin is the cell they mark).
-__global__ __launch_bounds__(256) void k_mark(Geo g, World w, int nP, const float *__restrict__ px,
-                                              const float *__restrict__ py, const float *__restrict__ pz,
-                                              uint8_t *__restrict__ pflag, uint8_t *__restrict__ ct,
-                                              unsigned *__restrict__ cnt, int *__restrict__ pcell,
-                                              unsigned *__restrict__ prank, DevScalars *s) {
+__global__ void k_mark(Geo g, World w, int nP, const float *__restrict__ px, const float *__restrict__ py,
+                       const float *__restrict__ pz, uint8_t *__restrict__ pflag, uint8_t *__restrict__ ct,
+                       unsigned *__restrict__ cnt, int *__restrict__ pcell, unsigned *__restrict__ prank,
+                       DevScalars *s) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  int c = -1;
-  if (p < nP && pflag[p] == 0) {
-    float x = px[p], y = py[p], z = pz[p];
-    bool demote = sky_demote(w, g, x, y, z, &s->df_refined);
-    int j = 0;
-    if (!demote) {
-      int i = clampi(cell_floor(x, g.inv_dx, w.thr, g.nx), 0, g.nx - 1);
-      j = clampi(cell_floor(y, g.inv_dx, w.thr, g.ny), 0, g.ny - 1);
-      int k = clampi(cell_floor(z, g.inv_dx, w.thr, g.nz), 0, g.nz - 1);
-      demote = j >= g.ny - 2 && ceiling_demote(w, g, x, y, z, &s->df_refined);
-      if (!demote) c = (k * g.ny + j) * g.nx + i;
-    }
-    if (demote) pflag[p] = 1;
-    // AIR -> FLUID on the current types (static rock and ball cells stay
-    // SOLID); concurrent markers of one cell all store the same byte
-    if (c >= 0 && ct[c] == AIR) ct[c] = FLUID;
-  }
-  unsigned r = wave_cell_rank(cnt, c);
-  if (p < nP) {
-    pcell[p] = c;
-    if (c >= 0) prank[p] = r;
-  }
+  if (p >= nP) return;
+  if (pflag[p] != 0) { pcell[p] = -1; return; }
+  float x = px[p], y = py[p], z = pz[p];
+  if (sky_demote(w, g, x, y, z, &s->df_refined)) { pflag[p] = 1; pcell[p] = -1; return; }
+  int i = clampi(cell_floor(x, g.inv_dx, w.thr, g.nx), 0, g.nx - 1);
+  int j = clampi(cell_floor(y, g.inv_dx, w.thr, g.ny), 0, g.ny - 1);
+  int k = clampi(cell_floor(z, g.inv_dx, w.thr, g.nz), 0, g.nz - 1);
+  if (j >= g.ny - 2 && ceiling_demote(w, g, x, y, z, &s->df_refined)) { pflag[p] = 1; pcell[p] = -1; return; }
+  int c = (k * g.ny + j) * g.nx + i;
+  // AIR -> FLUID on the current types (static rock and ball cells stay
+  // SOLID); concurrent markers of one cell all store the same byte
+  if (ct[c] == AIR) ct[c] = FLUID;
+  pcell[p] = c;
+  prank[p] = atomicAdd(&cnt[c], 1u);
 }
 
 __global__ void k_scatter(int nP, const int *__restrict__ pcell, const unsigned *__restrict__ prank,

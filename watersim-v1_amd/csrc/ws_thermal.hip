@@ -297,7 +297,6 @@ __global__ __launch_bounds__(EV_THREADS) void k_evap_chooser(const unsigned *__r
       uint32_t next = 0;
       bool cut = false;
       int m = 0;
-#pragma unroll
       for (int h = 0; h < EV_HYP; h++) {
         // first candidate at or after pos that ejects when h ejected before it
         unsigned long long w = wr[h];
