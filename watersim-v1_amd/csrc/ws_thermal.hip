@@ -246,6 +246,7 @@ __global__ __launch_bounds__(EV_THREADS) void k_evap_chooser(const unsigned *__r
   constexpr int U = EV_WIN / EV_THREADS, WORDS = EV_WIN / 64;
   constexpr unsigned STAGE = 4 * EV_WIN;  // thresholds staged in LDS per global load
   static_assert(WORDS == 64, "one bit-row word per lane of the resolving wavefront");
+  static_assert(EV_HYP <= 64, "one lane per ejection of a window");
   __shared__ unsigned long long rows[EV_HYP][WORDS];
   __shared__ uint32_t sA[EV_WIN], sC[EV_WIN];
   __shared__ unsigned sthr[STAGE];
@@ -293,9 +294,9 @@ __global__ __launch_bounds__(EV_THREADS) void k_evap_chooser(const unsigned *__r
       unsigned long long wr[EV_HYP];
 #pragma unroll
       for (int m = 0; m < EV_HYP; m++) wr[m] = rows[m][lane];
-      unsigned pos = 0, nev = shN;
-      uint32_t next = 0;
-      bool cut = false;
+      // the real sequence by bit scans alone: lane h keeps the candidate of
+      // the window's ejection h
+      unsigned pos = 0, myj = 0;
       int m = 0;
       for (int h = 0; h < EV_HYP; h++) {
         // first candidate at or after pos that ejects when h ejected before it
@@ -310,12 +311,20 @@ __global__ __launch_bounds__(EV_THREADS) void k_evap_chooser(const unsigned *__r
             ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(w >> 32), fl) << 32) |
             (unsigned)__builtin_amdgcn_readlane((int)(unsigned)w, fl);
         const unsigned j = (unsigned)fl * 64u + (unsigned)(__ffsll((long long)wf) - 1);
-        uint32_t se = sA[j] * s + sC[j];
-        if (h > 0) se = sA[6 * h - 1] * se + sC[6 * h - 1];  // 6h more draws
-        // three Box-Muller normals: six draws unless a first draw is <= 4
-        // (u1 = g / 2^32 < 1e-9 is drawn again)
-        uint32_t gs;
-        bool six = A1 * se + C1 > 4u && A3 * se + C3 > 4u && A5 * se + C5 > 4u;
+        if (lane == h) myj = j;
+        pos = j + 1u;
+        m = h + 1;
+      }
+      // each ejection's state (j + 1 + 6h draws from s) and its normals, one
+      // lane each: six draws unless a first draw is <= 4 (u1 = g / 2^32 <
+      // 1e-9 is drawn again)
+      const bool mine = lane < m;
+      uint32_t se = 0, gs = 0;
+      bool six = true;
+      if (mine) {
+        se = sA[myj] * s + sC[myj];
+        if (lane > 0) se = sA[6 * lane - 1] * se + sC[6 * lane - 1];
+        six = A1 * se + C1 > 4u && A3 * se + C3 > 4u && A5 * se + C5 > 4u;
         if (six) {
           gs = A6 * se + C6;
         } else {
@@ -325,22 +334,26 @@ __global__ __launch_bounds__(EV_THREADS) void k_evap_chooser(const unsigned *__r
             gs = lcg1(gs);
           }
         }
-        if (lane == 0) { evList[nev] = kw + j; evState[nev] = se; }
-        nev++;
-        pos = j + 1u;
-        m = h + 1;
-        if (!six || h == EV_HYP - 1) {  // the window ends behind this candidate
-          next = gs;
-          cut = true;
-          break;
-        }
       }
-      if (!cut) {  // all nw candidates with m ejections among them
+      // the window ends behind the first ejection with other than six draws
+      // (later ones assumed six), or behind the EV_HYP-th
+      const unsigned long long bad = __ballot(mine && !six);
+      int keep = m;
+      bool cut = m == EV_HYP;
+      if (bad) { keep = __ffsll((long long)bad); cut = true; }
+      uint32_t next;
+      unsigned end;
+      if (cut) {
+        next = (uint32_t)__builtin_amdgcn_readlane((int)gs, keep - 1);
+        end = (unsigned)__builtin_amdgcn_readlane((int)myj, keep - 1) + 1u;
+      } else {  // all nw candidates with m ejections among them
         next = sA[nw - 1] * s + sC[nw - 1];
         if (m > 0) next = sA[6 * m - 1] * next + sC[6 * m - 1];
-        pos = nw;
+        end = nw;
       }
-      if (lane == 0) { shS = next; shK = kw + pos; shN = nev; }
+      const unsigned nev = shN;
+      if (lane < keep) { evList[nev + lane] = kw + myj; evState[nev + lane] = se; }
+      if (lane == 0) { shS = next; shK = kw + end; shN = nev + (unsigned)keep; }
     }
     __syncthreads();
   }
