@@ -96,7 +96,12 @@ int ws_get_f(ws_ctx *c, const char *name, double *v);
  * "terrainR" (f32, tn^3); with balls also "svx","svy","svz" (f32, C, the
  * solid cell velocities), "ballHist" (u32, 25 per ball: the waterline
  * histogram and its count) and "ballSurf" (f32 per ball, the waterline);
- * after ws_metaballs_build "metaballField" (f32, (m+1)^3).
+ * after ws_metaballs_build "metaballField" (f32, (m+1)^3). Read-only:
+ * "pcell" (i32, nP), the cell each particle was last binned to,
+ * (k*ny+j)*nx+i with i = clamp(floor(x/dx), 0, nx-1) evaluated as the
+ * reference does in double (solver.js:976-978), by the marking pass (-1 for
+ * particles that mark nothing: flags other than 0 and the ones it demotes)
+ * or by splatDensity's binning.
  * nbytes must equal the field's byte size (for particle fields: nP
  * elements). */
 int ws_upload(ws_ctx *c, const char *name, const void *host, size_t nbytes);
@@ -177,7 +182,9 @@ int ws_read_render(ws_ctx *c, unsigned mask, ws_render_view *view);
  * the context's current state it times `sweeps` SOR sweeps of each SOR
  * kernel (sorMode 0 per-colour grid, 1 LDS tile, 2 fluid list, 3 two sweeps
  * per LDS launch, 4 the whole solve in one workgroup of 256, 512 or 1024
- * threads; all give the same bits for the same sweeps) and, with the warm
+ * threads, 5 the whole solve on "flowGroups" workgroups that wait only for
+ * their neighbours between half-sweeps; all give the same bits for the same
+ * sweeps) and, with the warm
  * start on, the untiled and LDS-tiled convolutions, and keeps the fastest
  * ("sorMode", "sorThreads", "netTiled"). The pressure and all counters are
  * left as they were. `report` (may be NULL) receives one line per candidate

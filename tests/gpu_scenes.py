@@ -50,3 +50,29 @@ def rel_err(a, b):
     b = np.asarray(b, np.float64)
     scale = max(np.abs(b).max(), 1e-30)
     return np.abs(a - b).max() / scale
+
+
+ORACLE_PARTS = ["px", "py", "pz", "pvx", "pvy", "pvz", "pT", "pAir", "pflag", "pcool", "pLight", "pDepth", "pWx",
+                "pWy", "pWz", "pPh"]
+
+
+def oracle_from_solver(s, world, bits=32):
+    """The CPU oracle on a GPU context's current state: the world the context
+    was built from, every particle field, q, the gravity ramp, umax and the
+    advection streams, and the context's solver settings."""
+    import cpu_oracle as oracle
+    n = world["nx"]
+    st = oracle.OracleState({"nx": n, "ny": n, "nz": n, "dx": world["dx"], "coreR": world["coreR"],
+                             "oceanR": world["oceanR"], "spacing": world["spacing"],
+                             "particleVolume": world["particleVolume"], "dv": world["dv"], "tn": world["tn"],
+                             "atmosphereH": s.get("atmosphereH")},
+                            bits=bits)
+    st.set_static(world["solid"], world["R"], world["staticTypes"], world["gu"], world["gv"], world["gw"])
+    nP = int(s.get("nP"))
+    st.set_particles(nP, **{k: s.download(k) for k in ORACLE_PARTS})
+    st.arrays["q"][:] = s.download("q")
+    st.set_params(**{k: s.get(k) for k in ("pic", "omega", "gravity", "maxSpeed", "vorticity", "cfl", "viscosity",
+                                             "gRamp", "umax")})
+    cf, bf, ev = s.get_rng()
+    st.set_params(cfS=cf, bfS=bf, evS=ev, Rlo=world["Rlo"], Rhi=world["Rhi"])
+    return st

@@ -54,3 +54,38 @@ def oracle_from_golden(g, tag="in_", bits=32):
     if tag == "in_":
         st.arrays["q"][:] = g["in_q"]
     return st
+
+
+def oracle_step_frame(st, dt_frame, clock, sun_activity, currents, substeps=1):
+    """FluidSolver._stepSync (js/solver.js:2670-2700) over the oracle's stage
+    functions: the 25 Hz heat / evaporation / currents tick (:2672-2681), the
+    CFL substep loop (:2682-2694) and the frame-end passes in their order
+    (:2696-2699). `clock` holds the host-side simTime and heatAcc, as the
+    reference keeps them on the solver object. Returns the substep lengths."""
+    import ctypes
+    clock["simTime"] += dt_frame
+    st.set_params(simTime=clock["simTime"])
+    clock["heatAcc"] += dt_frame
+    if clock["heatAcc"] >= 1.0 / 25:
+        tick = min(clock["heatAcc"], 1.0 / 12)
+        st.call("oc_update_heat", ctypes.c_double(tick))
+        if sun_activity > 0:
+            st.call("oc_update_evaporation", ctypes.c_double(tick))
+        if currents > 0:
+            st.call("oc_update_currents", ctypes.c_double(tick))
+        clock["heatAcc"] = 0.0
+        clock["ticks"] = clock.get("ticks", 0) + 1
+    remaining, n, base = dt_frame, 0, dt_frame / max(1, substeps)
+    dts = []
+    while remaining > 1e-6 and n < 10:
+        # oc_dt_law is min(cfl dx / max(umax, 0.08), 1/100) floored at 1e-5
+        dt = max(min(base, st.call("oc_dt_law")), 1e-5)
+        st.call("oc_substep", ctypes.c_double(dt))
+        dts.append(dt)
+        remaining -= dt
+        n += 1
+    st.call("oc_vapor_collisions")
+    st.call("oc_update_phase", ctypes.c_double(dt_frame))
+    st.call("oc_ice_contacts")
+    st.call("oc_push_surface")
+    return dts

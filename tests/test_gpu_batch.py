@@ -3,7 +3,8 @@
 solver with a host-side SOR loop.
 
 - sorMode 4 runs the sweeps of the fluid-list kernel (sorMode 2) and the
-  host loop's stopping rule on the device. The sweeps give the same bits and
+  host loop's stopping rule on the device (sorMode 5 the same on workgroups
+  synchronised with their neighbours only). The sweeps give the same bits and
   the stopping decisions agree, so whole substeps agree bit for bit: with a
   fixed sweep count, to a residual with checks every 8 sweeps, and with the
   adaptive check spacing (same sweep counts; the residual itself is summed in
@@ -79,6 +80,30 @@ def test_device_solve_split_over_workgroups(groups):
         b.run_substeps(1)
         assert a.get("pressureIters_done") == b.get("pressureIters_done")
         assert_same(a, b, "substep %d" % step)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("groups,threads", [(256, 256), (24, 512)])
+def test_flow_solve_equals_list_sweeps(groups, threads):
+    """The neighbour-synchronised solve (sorMode 5: workgroups own ranges of
+    the lists and wait only for the workgroups within one plane of cells)
+    gives the bits, sweep counts and residuals of the list sweeps; 24
+    workgroups of 512 threads make each range span several planes."""
+    a, b = planet(48, 5), planet(48, 5)
+    a.set(sorMode=2)
+    b.set(sorMode=5, flowGroups=groups, sorThreads=threads)
+    for step in range(3):
+        a.run_substeps(1)
+        b.run_substeps(1)
+        assert a.get("pressureIters_done") == b.get("pressureIters_done")
+        assert abs(a.get("residual") - b.get("residual")) <= 1e-12 * a.get("residual")
+        assert_same(a, b, "substep %d" % step)
+    for s in (a, b):
+        s.set(pressureTol=0, pressureIters=37, measureResidual=1)
+        s.run_substeps(1)
+    assert b.get("pressureIters_done") == 37
+    assert_same(a, b, "fixed sweeps")
     a.close()
     b.close()
 

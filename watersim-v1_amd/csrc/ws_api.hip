@@ -282,13 +282,25 @@ SorJob sor_job(ws_ctx *c) {
 int sor_solve_dev(ws_ctx *c) {
   SorJob J = sor_job(c);
   if (c->timing) WS_HIP(hipEventRecord(c->ev[0], c->stream));
-  if (c->sorGroups > 1) {
+  if (c->sorMode == 5) {
+    // neighbour-synchronised solve: flowGroups workgroups on the first
+    // flowXcds XCDs, all resident at once (at most one per compute unit)
+    const int G = c->flowGroups, X = c->flowXcds;
+    J.G = G;
+    J.arrive = c->sorArrive;
+    J.off = c->sorOff;
+    J.flow = c->sorFlow;
+    hipLaunchKernelGGL(k_flow_arm, dim3(1), dim3(256), 0, c->stream, G, c->sorFlow, c->sorArrive);
+    const int grid = (G / X) * 8 + G % X;
+    if (c->sorThreads == 256)
+      hipLaunchKernelGGL(k_sor_flow<256>, dim3(grid), dim3(256), 0, c->stream, J, X);
+    else if (c->sorThreads == 512)
+      hipLaunchKernelGGL(k_sor_flow<512>, dim3(grid), dim3(512), 0, c->stream, J, X);
+    else
+      hipLaunchKernelGGL(k_sor_flow<1024>, dim3(grid), dim3(1024), 0, c->stream, J, X);
+  } else if (c->sorGroups > 1) {
     // several workgroups meeting at a barrier: all must be resident, so at
     // most one per compute unit (a barrier that times out fails the solve)
-    if (!c->sorArrive) {
-      WS_HIP(hipMalloc((void **)&c->sorArrive, 2 * sizeof(unsigned)));
-      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->sorArrive, 2L);
-    }
     J.G = c->sorGroups;
     J.arrive = c->sorArrive;
     hipLaunchKernelGGL(k_sor_arm, dim3(1), dim3(64), 0, c->stream, 1, c->sorArrive);
@@ -441,7 +453,7 @@ int pressure_setup(ws_ctx *c) {
     hipLaunchKernelGGL(k_sor_compact, dim3(nb), dim3(256), 0, c->stream, c->geo, c->nbm, c->dg, c->rhs, c->pw,
                        c->sorOff, c->sorHalf, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->ds);
     WS_LAUNCHED();
-    if (c->pressureTol <= 0 && c->sorMode != 4) {  // the to-residual path learns the counts with its first check
+    if (c->pressureTol <= 0 && c->sorMode < 4) {  // the to-residual path learns the counts with its first check
       WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
       WS_HIP(hipStreamSynchronize(c->stream));
       take_counts(c);
@@ -468,7 +480,7 @@ int pressure_setup(ws_ctx *c) {
 // the sweeps, to the tolerance or the fixed count
 int pressure_iterate(ws_ctx *c) {
   int r;
-  if (c->sorMode == 4) {
+  if (c->sorMode >= 4) {
     if ((r = sor_solve_dev(c))) return r;
     if (c->recordSamples && (r = net_record_target(c))) return r;
     return WS_OK;
@@ -698,6 +710,7 @@ bool field_of(ws_ctx *c, const char *n, Field &f) {
   PF("advU", c->advU, float) PF("advV", c->advV, float) PF("advW", c->advW, float)
   PF("pLight", c->pLight, float) PF("pDepth", c->pDepth, uint8_t)
   PF("pWx", c->pWx, float) PF("pWy", c->pWy, float) PF("pWz", c->pWz, float) PF("pPh", c->pPh, float)
+  PF("pcell", c->pcell, int)
   GF("terrainSolid", c->tsolid, uint8_t, (long)c->tn * c->tn * c->tn)
   GF("cellType", c->cellType, uint8_t, c->nC) GF("staticTypes", c->staticTypes, uint8_t, c->nC)
   GF("q", c->q, float, c->nC) GF("rhs", c->rhs, float, c->nC)
@@ -812,7 +825,8 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
   }
   if ((r = dalloc(c, &c->sorOff, c->nC + 1)) || (r = dalloc(c, &c->sorCode, c->nC + 4)) ||
       (r = dalloc(c, &c->sorRhs, c->nC + 4)) || (r = dalloc(c, &c->sorPw, c->nC + 4)) ||
-      (r = dalloc(c, &c->sorDg, c->nC + 4)) || (r = dalloc(c, &c->sorChunk, 2 * (c->nC / 2048 + 2)))) {
+      (r = dalloc(c, &c->sorDg, c->nC + 4)) || (r = dalloc(c, &c->sorChunk, 2 * (c->nC / 2048 + 2))) ||
+      (r = dalloc(c, &c->sorArrive, 2)) || (r = dalloc(c, &c->sorFlow, (size_t)kFlowStride * 256))) {
     ws_destroy(c);
     return r;
   }
@@ -898,7 +912,7 @@ void ws_destroy(ws_ctx *c) {
   balls_release(c);
   metaballs_release(c);
   void *ptrs[] = {c->staticTypes, c->gu, c->thr, c->terrainR, c->cellType, c->nbm, c->dg, c->q, c->q2, c->rhs, c->pw,
-                  c->sorActive, c->tbActive, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->sorChunk, c->sorArrive,
+                  c->sorActive, c->tbActive, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->sorChunk, c->sorArrive, c->sorFlow,
                   c->cellCount, c->cellStart, c->vort, c->vel, c->velO, c->wgt, c->scr, c->scr2, c->valid, c->dens,
                   c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pT, c->pAir, c->advU, c->advV, c->advW, c->pflag,
                   c->pcool, c->pcell, c->prank, c->sortedIdx, c->vneed, c->vscr, c->rng, c->ds, c->partials,
@@ -931,8 +945,8 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
 #undef SI
   if (!strcmp(n, "sorMode")) {
     int m = (int)v;
-    if (m < 0 || m > 4 || m != v) { c->err = "sorMode must be 0..4"; return WS_EINVAL; }
-    if (m >= 2 && c->nC >= (1L << 26)) { c->err = "sorMode 2..4 need fewer than 2^26 cells"; return WS_EINVAL; }
+    if (m < 0 || m > 5 || m != v) { c->err = "sorMode must be 0..5"; return WS_EINVAL; }
+    if (m >= 2 && c->nC >= (1L << 26)) { c->err = "sorMode 2..5 need fewer than 2^26 cells"; return WS_EINVAL; }
     c->sorMode = m;
     return WS_OK;
   }
@@ -942,6 +956,20 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
     WS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     if (!(v >= 1 && v <= cus) || v != (int)v) { c->err = "sorGroups must be 1 .. the compute units"; return WS_EINVAL; }
     c->sorGroups = (int)v;
+    return WS_OK;
+  }
+  if (!strcmp(n, "flowGroups") || !strcmp(n, "flowXcds")) {
+    int dev = 0, cus = 0;
+    WS_HIP(hipGetDevice(&dev));
+    WS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const bool g = n[4] == 'G';
+    const int G = g ? (int)v : c->flowGroups, X = g ? c->flowXcds : (int)v;
+    if (v != (int)v || X < 1 || X > 8 || G < 1 || G > 256 || G > cus / 8 * X) {
+      c->err = "flowGroups must be 1..256 and at most the compute units of flowXcds (1..8) XCDs";
+      return WS_EINVAL;
+    }
+    c->flowGroups = G;
+    c->flowXcds = X;
     return WS_OK;
   }
   if (!strcmp(n, "sorThreads")) {
@@ -998,7 +1026,7 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal) GF(residualsTotal) GF(ticksTotal) GF(sorMode) GF(sorCellSweeps)
   GF(cmpMsPlain) GF(cmpMsLearned) GF(cmpSweepsPlain) GF(cmpSweepsLearned) GF(cmpSolves) GF(warmStart) GF(netMs) GF(netCalls)
   GF(vaporCount) GF(nightVaporCount) GF(contactOverflow) GF(nBalls) GF(mbM) GF(vaporBigGroups) GF(vaporGroupMax)
-  GF(vaporBigParcels) GF(netTiled) GF(sorThreads) GF(sorGroups)
+  GF(vaporBigParcels) GF(netTiled) GF(sorThreads) GF(sorGroups) GF(flowGroups) GF(flowXcds)
   GF(heatK) GF(Tamb) GF(shadeCool) GF(sunActivity) GF(evapIntensity) GF(snowT) GF(meltT) GF(cloudP) GF(currents)
   GF(simTime) GF(heatAcc) GF(evapCount) GF(cloudT) GF(rainT) GF(rainLiftAge)
 #undef GF
@@ -1016,6 +1044,7 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
 int ws_upload(ws_ctx *c, const char *name, const void *host, size_t nbytes) {
   if (!c || !name || (!host && nbytes)) return WS_EINVAL;
   Field f;
+  if (!strcmp(name, "pcell")) { c->err = "pcell is read-only (the marking pass writes it)"; return WS_EINVAL; }
   if (!strcmp(name, "terrainR")) {
     long n = (long)c->tn * c->tn * c->tn;
     if (n <= 0 || nbytes != (size_t)n * 4) { c->err = "terrainR: set tn first; size must be tn^3 floats"; return WS_EINVAL; }
@@ -1116,12 +1145,13 @@ int ws_tune(ws_ctx *c, int sweeps, char *report, size_t len) {
     return WS_OK;
   };
   // SOR kernels: per-colour grid, LDS tile, fluid list, two sweeps per
-  // launch, and the one-workgroup solve at three workgroup sizes
+  // launch, the one-workgroup solve at three workgroup sizes and the
+  // neighbour-synchronised solve
   c->warmStart = 0;
   const int keepThreads = c->sorThreads;
   int best = keepMode, bestThreads = keepThreads;
   float bestMs = 1e30f;
-  const int cand[][2] = {{0, 0}, {1, 0}, {2, 0}, {3, 0}, {4, 256}, {4, 512}, {4, 1024}};
+  const int cand[][2] = {{0, 0}, {1, 0}, {2, 0}, {3, 0}, {4, 256}, {4, 512}, {4, 1024}, {5, 256}};
   for (const auto &cm : cand) {
     if (r) break;
     if (cm[0] >= 2 && c->nC >= (1L << 26)) continue;

@@ -1074,6 +1074,192 @@ __global__ void k_sor_arm(int n, unsigned *__restrict__ arrive) {
   if (p < n) arrive[2 * p] = 0u;
 }
 
+// ------------------------------- neighbour-synchronised pressure solve
+// The solve of sor_solve_part (same sweeps, same residual order, same
+// stopping rule, so the same bits and sweep counts) on G resident
+// workgroups that do not meet after every half-sweep. Workgroup w owns the
+// list entries of one contiguous range of cell indices (the red list cut
+// into G equal parts; the black entries of the same cells). A cell's
+// neighbours lie within nx*ny cells of it, so before half-sweep h a
+// workgroup waits only for the workgroups whose ranges come within that
+// distance of its own to have published half-sweep h - 1 (the values it is
+// about to read are then final, and nobody still reads the values it is about
+// to overwrite). Values of q are read and written with agent-scope relaxed
+// atomics (coherent between compute units and XCDs without L2 write-back or
+// invalidation); a workgroup waits for its stores to complete before it
+// publishes its progress. Residual checks remain full barriers. Waits poll a
+// bounded number of times; a timed-out wait fails the solve.
+constexpr int kFlowMaxG = 256;
+
+__device__ __forceinline__ float q_load(const float *q, int c) {
+  return __hip_atomic_load(q + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int NT>
+__device__ __forceinline__ void flow_half(const SorJob &J, long base, long lo, long hi, int nxy) {
+  const int nx = J.g.nx;
+  // the neighbour loads carry no branches (an absent neighbour loads the
+  // cell itself and is not added), so all of a thread's loads are in flight
+  // at once; the sum keeps k_sor_list's order
+  const int step[6] = {-1, 1, -nx, nx, -nxy, nxy};
+  for (long t0 = lo + threadIdx.x; t0 < hi; t0 += (long)NT * kSorU) {
+    unsigned e[kSorU];
+    float rh[kSorU], pw[kSorU], nb[kSorU][6], qc[kSorU];
+#pragma unroll
+    for (int u = 0; u < kSorU; u++) {
+      long t = t0 + (long)u * NT;
+      bool in = t < hi;
+      e[u] = in ? J.code[base + t] : 0u;
+      rh[u] = in ? J.lrhs[base + t] : 0.f;
+      pw[u] = in ? J.lpw[base + t] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < kSorU; u++) {
+      // entries past hi hold code 0: cell 0, no neighbours
+      const int c = (int)(e[u] & 0x3ffffffu);
+      const unsigned m = e[u] >> 26;
+#pragma unroll
+      for (int d = 0; d < 6; d++) nb[u][d] = q_load(J.q, (m >> d) & 1 ? c + step[d] : c);
+      qc[u] = q_load(J.q, c);
+    }
+#pragma unroll
+    for (int u = 0; u < kSorU; u++) {
+      const unsigned m = e[u] >> 26;
+      float a = 0.f;
+#pragma unroll
+      for (int d = 0; d < 6; d++) a = (m >> d) & 1 ? a + nb[u][d] : a;
+      if (t0 + (long)u * NT < hi)
+        __hip_atomic_store(J.q + (e[u] & 0x3ffffffu), sor_relax(qc[u], a, rh[u], pw[u], J.om1), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// publish half-sweep h of workgroup w, then wait until workgroups dlo..dhi
+// have published it too
+__device__ __forceinline__ bool flow_sync(const SorJob &J, int w, int dlo, int dhi, unsigned h) {
+  __shared__ int flow_ok;
+  // this thread's q stores complete (visible at agent scope) before the
+  // workgroup publishes
+  __builtin_amdgcn_s_waitcnt(0);
+  if (threadIdx.x == 0) flow_ok = 1;
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(J.flow + (long)w * kFlowStride, h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x < 64) {
+    bool good = true;
+    for (int v = dlo + (int)threadIdx.x; v <= dhi && good; v += 64) {
+      int polls = 0;
+      while (__hip_atomic_load(J.flow + (long)v * kFlowStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < h) {
+        if (++polls > kSorPolls || __hip_atomic_load(J.arrive + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          good = false;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    if (!good) {
+      __hip_atomic_store(J.arrive + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      flow_ok = 0;
+    }
+  }
+  __syncthreads();
+  return flow_ok != 0;
+}
+
+template <int NT>
+__device__ void sor_flow_part(SorJob J, int w, int G) {
+  J.G = G;
+  const long nR = J.ds->nRed, nB = J.ds->nBlack;
+  const int nxy = J.g.nx * J.g.ny;
+  __shared__ int sclo[kFlowMaxG + 1];
+  __shared__ int sdep[2];
+  for (int v = threadIdx.x; v <= G; v += NT) {
+    long r = nR * v / G;
+    sclo[v] = v == 0 ? 0 : (r < nR ? (int)(J.code[r] & 0x3ffffffu) : J.g.nC);
+  }
+  __syncthreads();
+  const int clo = sclo[w], chi = sclo[w + 1];
+  if (threadIdx.x == 0) {
+    int dlo = w, dhi = w;
+    while (dlo > 0 && sclo[dlo] > clo - nxy) dlo--;      // ranges ending after clo - nxy
+    while (dhi < G - 1 && sclo[dhi + 1] < chi + nxy) dhi++;  // ranges starting before chi + nxy
+    sdep[0] = dlo;
+    sdep[1] = dhi;
+  }
+  __syncthreads();
+  const int dlo = sdep[0], dhi = sdep[1];
+  const long rlo = nR * w / G, rhi = nR * (w + 1) / G;
+  const long blo = (long)(J.off[clo] >> 32), bhi = (long)(J.off[chi] >> 32);
+  unsigned done = 0, checks = 0, bar = 0, h = 0;
+  double res = 0;
+  bool ok = true;
+  auto sweep = [&]() {
+    flow_half<NT>(J, 0, rlo, rhi, nxy);
+    if (!(ok = flow_sync(J, w, dlo, dhi, ++h))) return;
+    flow_half<NT>(J, J.half, blo, bhi, nxy);
+    ok = flow_sync(J, w, dlo, dhi, ++h);
+  };
+  auto check = [&]() {
+    if ((ok = sor_barrier(J, ++bar))) ok = sor_residual<NT>(J, w, nR, nB, nxy, bar, &res);
+    checks++;
+  };
+  if (J.tol <= 0) {
+    for (int s = 0; s < J.fixedIters && ok; s++) sweep();
+    done = (unsigned)max(J.fixedIters, 0);
+    if (ok && J.measure) check();
+  } else {
+    const unsigned last = J.ds->sorIters;
+    check();
+    double best = res, prev = res;
+    int flat = 0, k = J.every;
+    if (J.adaptive && last > 0) k = max(k, (int)(0.75 * last));
+    while (ok && res > J.tol && (int)done < J.maxIters && flat < 12) {
+      if ((int)done + k > J.maxIters) k = J.maxIters - (int)done;
+      for (int s = 0; s < k && ok; s++) sweep();
+      done += (unsigned)k;
+      if (!ok) break;
+      check();
+      if (res < 0.995 * best) { best = res; flat = 0; } else flat++;
+      int kn = J.every;
+      if (J.adaptive && res < prev && res > J.tol) {
+        double rate = pow(res / prev, 1.0 / k);
+        double need = log(J.tol / res) / log(rate);
+        kn = need < 4 ? 4 : (need > 64 ? 64 : (int)ceil(need));
+      }
+      prev = res;
+      k = kn;
+    }
+  }
+  // (ds->sorIters was read by every workgroup before the first check's
+  // barrier, so workgroup 0 may rewrite it now)
+  if (w == 0 && threadIdx.x == 0) {
+    J.ds->sorIters = done;
+    J.ds->sorChecksLast = checks;
+    J.ds->sorRes = ok ? res : -1.0;
+    J.ds->sorSweeps += done;
+    J.ds->sorChecks += checks;
+  }
+}
+
+// G workgroups of one solve. With X < 8 only the blocks of the first X XCDs
+// (blocks are dealt to the 8 XCDs in turn) take part: gridDim.x = 8 G / X.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_sor_flow(SorJob J, int X) {
+  const int b = blockIdx.x;
+  if (b % 8 >= X) return;
+  const int w = (b / 8) * X + b % 8;
+  sor_flow_part<NT>(J, w, (int)gridDim.x / 8 * X + min((int)gridDim.x % 8, X));
+}
+
+// the progress counters and barrier of a flow solve about to start
+__global__ void k_flow_arm(int G, unsigned *__restrict__ flow, unsigned *__restrict__ arrive) {
+  for (int v = threadIdx.x; v < G; v += blockDim.x) flow[(long)v * kFlowStride] = 0u;
+  if (threadIdx.x == 0) arrive[0] = 0u;
+}
+
+template __global__ void k_sor_flow<256>(SorJob, int);
+template __global__ void k_sor_flow<512>(SorJob, int);
+template __global__ void k_sor_flow<1024>(SorJob, int);
 template __global__ void k_sor_solve_multi<256>(SorJob);
 template __global__ void k_sor_solve_multi<512>(SorJob);
 template __global__ void k_sor_solve_multi<1024>(SorJob);

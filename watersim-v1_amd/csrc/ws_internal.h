@@ -68,6 +68,10 @@ struct Geo {
   float cx_lo, cy_lo, cz_lo;  // double centre minus its float rounding
 };
 
+// words between two workgroups' progress counters in the neighbour-
+// synchronised solve (one 128-byte line each)
+constexpr int kFlowStride = 32;
+
 // One pressure solve run to its end by one workgroup (k_sor_solve): the
 // fluid lists of a context, its q, and the stopping rule of the host loop
 struct SorJob {
@@ -84,6 +88,11 @@ struct SorJob {
   double *chunk;     // residual chunk sums, 2 x ceil(nC / 2048)
   unsigned *arrive;  // G > 1: barrier arrivals, then a timed-out flag
   int G;             // workgroups sharing this solve
+  // the neighbour-synchronised solve (k_sor_flow): the lists' scan keys
+  // (red count | black count << 32 before each cell, nC + 1 entries) and one
+  // progress counter per workgroup, kFlowStride words apart
+  const unsigned long long *off;
+  unsigned *flow;
 };
 
 // a planet's face fields for the batched extrapolation wavefront
@@ -200,6 +209,10 @@ struct Ctx {
   double *sorChunk = nullptr;  // residual chunk sums of the on-device solve
   int sorGroups = 1;           // workgroups of one context's on-device solve
   unsigned *sorArrive = nullptr;  // their barrier counter and timed-out flag
+  // 5: the whole solve on flowGroups workgroups that wait only for their
+  // neighbours between half-sweeps (k_sor_flow), on the first flowXcds XCDs
+  int flowGroups = 256, flowXcds = 8;
+  unsigned *sorFlow = nullptr;  // progress counters, kFlowStride words apart
   unsigned long long sorSweepsSeen = 0, sorChecksSeen = 0;  // device totals folded into the host counters
   bool devStatsPending = false;
   unsigned *cellCount = nullptr, *cellStart = nullptr;

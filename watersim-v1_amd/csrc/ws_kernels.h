@@ -82,6 +82,9 @@ __global__ void k_sor_solve_batch(const SorJob *jobs, const int2 *blk);
 template <int NT>
 __global__ void k_sor_solve_multi(SorJob J);
 __global__ void k_sor_arm(int n, unsigned *arrive);
+template <int NT>
+__global__ void k_sor_flow(SorJob J, int X);
+__global__ void k_flow_arm(int G, unsigned *flow, unsigned *arrive);
 __global__ void k_project(Geo g, const uint8_t *ct, const float *phi, float *vel, int do_clamp, float vcap,
                           DevScalars *s);
 __global__ void k_clamp(long F, float *vel, float vcap, DevScalars *s);

@@ -193,6 +193,8 @@ class Solver:
             return self.sizes[name[-1].lower()], np.uint8
         if name == "dens":
             return self.nD, np.float32
+        if name == "pcell":
+            return self.nP, np.int32
         if name == "terrainR":
             tn = int(self.get("tn"))
             return tn ** 3, np.float32
