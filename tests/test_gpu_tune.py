@@ -1,5 +1,6 @@
 """ws_tune (no reference counterpart): on a context's current system it
-times the SOR kernels (the one-workgroup solve at three sizes) and, with the learned warm start on, the
+times the SOR kernels (the one-workgroup solve at three sizes, the
+neighbour-synchronised solve) and, with the learned warm start on, the
 network's two convolution tilings, keeps the fastest, and leaves the
 simulation state as it found it (pressure, velocities, particles and the
 solve counters)."""
@@ -30,12 +31,12 @@ def test_tune_keeps_state_and_picks_a_kernel():
     rep = s.tune(16)
     print("  " + rep.replace("\n", "\n  "))
     modes = [ln for ln in rep.splitlines() if ln.startswith("sorMode ") and "sweeps in" in ln]
-    assert len(modes) == 7  # modes 0-3, and mode 4 at 256, 512 and 1024 threads
+    assert len(modes) == 8  # modes 0-3, mode 4 at 256, 512 and 1024 threads, mode 5
     chosen, threads = int(s.get("sorMode")), int(s.get("sorThreads"))
-    assert 0 <= chosen <= 4 and rep.rstrip().endswith("netTiled %d" % int(s.get("netTiled")))
+    assert 0 <= chosen <= 5 and rep.rstrip().endswith("netTiled %d" % int(s.get("netTiled")))
     assert ("chosen: sorMode %d, sorThreads %d" % (chosen, threads)) in rep
     times = {ln.split(":")[0]: float(ln.split(" in ")[1].split()[0]) for ln in modes}
-    key = "sorMode %d (%d threads)" % (chosen, threads) if chosen == 4 else "sorMode %d" % chosen
+    key = "sorMode %d (%d threads)" % (chosen, threads) if chosen >= 4 else "sorMode %d" % chosen
     assert times[key] == min(times.values())
     for k in keys:
         assert np.array_equal(s.download(k), before[k]), k
