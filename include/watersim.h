@@ -133,6 +133,21 @@ int ws_splat_density(ws_ctx *c);
  * enclosed volume (FP64) into *volume. */
 int ws_mt_build(ws_ctx *c, const float *dens, int nx, int ny, int nz, double dx, double iso,
                 float *pos, float *nrm, long cap, long *count, double *volume);
+/* No counterpart in the reference: ws_mt_build for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The field (dens_dev, (nx+1)(ny+1)(nz+1)
+ * floats, or NULL for the context's own density) and the mesh (pos_dev /
+ * nrm_dev, 3 floats per vertex, up to cap vertices; NULL for count and volume
+ * only) are device buffers of the caller; *count and *volume are host
+ * values, as ws_mt_build returns them, and every output equals
+ * ws_mt_build's on the same field. All scratch sits in a static device array
+ * of the library (about 8.5 MB per GPU, held until the process ends): at most
+ * WS_MT_NOALLOC_MAX_CELLS cells, larger fields are refused with WS_EINVAL.
+ * The call allocates nothing. Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (two threads, two contexts) race. */
+#define WS_MT_NOALLOC_MAX_CELLS (1L << 21)
+int ws_mt_build_noalloc(ws_ctx *c, const float *dens_dev, int nx, int ny, int nz, double dx, double iso,
+                        float *pos_dev, float *nrm_dev, long cap, long *count, double *volume);
 /* autotuneIso(), solver.js:3838-3849: 8 bisections of iso on the device. */
 int ws_autotune_iso(ws_ctx *c, double *iso);
 /* Metaballs.build(solver, tension, detail), surface.js:178-284: the skin of

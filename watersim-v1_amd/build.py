@@ -7,13 +7,14 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = ["ws_grid.hip", "ws_particles.hip", "ws_surface.hip", "ws_api.hip", "ws_api_surface.hip", "ws_thermal.hip", "ws_frame.hip", "ws_learn.hip", "ws_contacts.hip", "ws_balls.hip", "ws_metaballs.hip", "ws_batch.hip", "ws_world.cpp"]
+SRC = ["ws_grid.hip", "ws_particles.hip", "ws_surface.hip", "ws_api.hip", "ws_api_surface.hip", "ws_thermal.hip", "ws_frame.hip", "ws_learn.hip", "ws_contacts.hip", "ws_balls.hip", "ws_metaballs.hip", "ws_batch.hip", "ws_noalloc.hip", "ws_world.cpp"]
 OUT = os.path.join(HERE, "_build", "libwatersim_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # per-file additions: the ball, contact, metaball and surface code evaluate the reference's double
 # formulas operation by operation, so nothing may fuse into an FMA
 FILE_FLAGS = {"ws_balls.hip": ["-ffp-contract=off"], "ws_contacts.hip": ["-ffp-contract=off"],
-              "ws_metaballs.hip": ["-ffp-contract=off"], "ws_surface.hip": ["-ffp-contract=off"]}
+              "ws_metaballs.hip": ["-ffp-contract=off"], "ws_surface.hip": ["-ffp-contract=off"],
+              "ws_noalloc.hip": ["-ffp-contract=off"]}
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
 

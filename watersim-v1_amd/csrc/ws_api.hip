@@ -759,9 +759,13 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
   if (!out) return WS_EINVAL;
   *out = nullptr;
   if (nx < 4 || ny < 4 || nz < 4 || !(dx > 0) || capacity < 0) return WS_EINVAL;
-  int ndev = 0;
+  int ndev = 0, dev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return WS_ENODEV;
+  if (hipGetDevice(&dev) != hipSuccess) return WS_ENODEV;
+  // the device address of this GPU's static scratch (ws_mt_build_noalloc)
+  if (mt_scratch_setup(dev) != WS_OK) return WS_EHIP;
   ws_ctx *c = new ws_ctx();
+  c->device = dev;
   c->nx = nx; c->ny = ny; c->nz = nz; c->dx = dx; c->capacity = capacity;
   // solver.js:84-91
   c->W = nx * dx; c->H = ny * dx; c->D = nz * dx;

@@ -24,6 +24,9 @@ int mt_build_dev(ws_ctx *c, const float *dd, int nx, int ny, int nz, double dx, 
                  long cap, long *count, double *volume);
 void metaballs_release(ws_ctx *c);
 World make_world(ws_ctx *c);
+// the device address of a GPU's static marching-tets scratch, looked up once
+// per device (ws_noalloc.hip)
+int mt_scratch_setup(int device);
 // a device-dt-law substep split around its pressure sweeps (ws_api.hip)
 int batch_substep_pre(ws_ctx *c);
 int batch_substep_post1(ws_ctx *c);

@@ -108,6 +108,7 @@ struct Contacts;  // scratch of the frame-end contact passes (ws_contacts.hip)
 struct Ctx {
   // geometry
   int nx, ny, nz, capacity;
+  int device = 0;  // the GPU current when the context was created
   double dx, W, H, D, cx, cy, cz, domainR;
   long nC, uN, vN, wN, nD;
   Geo geo;

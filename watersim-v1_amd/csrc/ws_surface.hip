@@ -148,12 +148,13 @@ __device__ double mt_cell(int nx, int ny, int nz, double dx, const float *__rest
       }
       vol += (P[0][0] * (P[1][1] * P[2][2] - P[2][1] * P[1][2]) - P[0][1] * (P[1][0] * P[2][2] - P[2][0] * P[1][2]) +
               P[0][2] * (P[1][0] * P[2][1] - P[2][0] * P[1][1])) / 6;
-      if (pos && o + 3 <= cap) {
+      if (pos) {
         for (int vi = 0; vi < 3; vi++)
-          for (int d = 0; d < 3; d++) {
-            pos[(o + vi) * 3 + d] = (float)P[vi][d];
-            nrm[(o + vi) * 3 + d] = (float)N[vi][d];
-          }
+          if (o + vi < cap)
+            for (int d = 0; d < 3; d++) {
+              pos[(o + vi) * 3 + d] = (float)P[vi][d];
+              nrm[(o + vi) * 3 + d] = (float)N[vi][d];
+            }
       }
       o += 3;
     }

@@ -28,7 +28,7 @@ EXPORTS = ["ws_create", "ws_destroy", "ws_last_error", "ws_set_f", "ws_get_f", "
            "ws_net_param_count", "ws_net_set_params", "ws_net_get_params", "ws_net_forward", "ws_net_grad",
            "ws_net_bank", "ws_net_bank_count", "ws_net_train", "ws_set_balls", "ws_get_balls", "ws_metaballs_build", "ws_read_render", "ws_tune",
            "ws_batch_create", "ws_batch_run", "ws_batch_set", "ws_batch_get", "ws_batch_last_error",
-           "ws_batch_destroy"]
+           "ws_batch_destroy", "ws_mt_build_noalloc"]
 
 
 class RenderView(ctypes.Structure):
@@ -88,6 +88,8 @@ def load(path=LIB_PATH):
     L.ws_splat_density.argtypes = [vp]
     L.ws_mt_build.argtypes = [vp, vp, ci, ci, ci, d, d, vp, vp, ctypes.c_long, ctypes.POINTER(ctypes.c_long),
                               ctypes.POINTER(d)]
+    L.ws_mt_build_noalloc.argtypes = [vp, vp, ci, ci, ci, d, d, vp, vp, ctypes.c_long, ctypes.POINTER(ctypes.c_long),
+                                      ctypes.POINTER(d)]
     L.ws_autotune_iso.argtypes = [vp, ctypes.POINTER(d)]
     L.ws_kinetic_energy.argtypes = [vp, ctypes.POINTER(d)]
     L.ws_sync.argtypes = [vp]
@@ -380,6 +382,15 @@ class Solver:
             self._mt_pos = np.zeros(c * 3, np.float32)
             self._mt_nrm = np.zeros(c * 3, np.float32)
         raise RuntimeError("mt_build: vertex count changed between calls")
+
+    def mt_build_noalloc(self, dens_ptr, dims, dx, iso, pos_ptr=None, nrm_ptr=None, cap=0):
+        """ws_mt_build_noalloc on device buffers given as integer addresses
+        (dens_ptr None: the context's density); returns (count, volume)."""
+        nx, ny, nz = dims
+        cnt, vol = ctypes.c_long(), ctypes.c_double()
+        self._check(self.L.ws_mt_build_noalloc(self.ctx, dens_ptr, nx, ny, nz, dx, iso, pos_ptr, nrm_ptr, cap,
+                                               ctypes.byref(cnt), ctypes.byref(vol)), "mt_build_noalloc")
+        return cnt.value, vol.value
 
     def autotune_iso(self):
         out = ctypes.c_double()
