@@ -197,6 +197,64 @@ def test_vapor_liquid_crowded_parcel():
     assert moved > 0  # the crowded parcel was pushed
 
 
+def test_vapor_liquid_chain_walks_a_parcel():
+    """A chain of eight spray particles along y, in index order, each 0.7
+    contact radii beyond the previous one: every contact pushes the parcel
+    (moved to the empty centre of the planet) into reach of the next, 5.4
+    radii in all, more than the three radii gathered around its starting
+    point. The pass gathers again around the parcel once it has moved two
+    radii, so it meets the whole chain as the oracle's sweep does (the chain
+    runs across the x window of the reference's sorted vapor list, which
+    therefore stays valid)."""
+    from scenes import oracle_from_golden, PART_KEYS, EXTRA_KEYS
+    g = golden.load("thermal_32")
+    fe = g.meta["frame_end"]
+    sp = fe["sunPos"]
+    w = g.meta["world"]
+    rad = 0.6 * w["spacing"]
+    c = 0.5 * w["nx"] * w["dx"]
+    n = len(g["fe0_px"])
+    keys = [k for k in PART_KEYS + EXTRA_KEYS if "fe0_" + k in g]
+    parts = {k: g["fe0_" + k].copy() for k in keys}
+    fl = parts["pflag"]
+    v = int(np.nonzero(fl == 2)[0][0])
+    parts["px"][v], parts["py"][v], parts["pz"][v] = c, c, c
+    parts["pvx"][v] = parts["pvy"][v] = parts["pvz"][v] = 0.0
+    K = 8
+    liq = int(np.nonzero(fl == 0)[0][0])
+    for k in keys:
+        extra = np.repeat(parts[k][liq:liq + 1], K)
+        if k == "px" or k == "pz":
+            extra[:] = c
+        elif k == "py":
+            extra = (c + (-0.5 + 0.7 * np.arange(K)) * rad).astype(parts[k].dtype)
+        elif k == "pflag":
+            extra[:] = 1
+        elif k in ("pvx", "pvy", "pvz"):
+            extra[:] = 0.0
+        parts[k] = np.concatenate([parts[k], extra])
+    N = n + K
+    s = gpu_from_golden(g, tag="fe0_", capacity=N)
+    s.set(nP=N, spinOn=1, hasSun=1, sunX=sp[0], sunY=sp[1], sunZ=sp[2])
+    for k in keys:
+        s.upload(k, parts[k])
+    s.stage("vapor_collisions", 0.0)
+    assert s.get("contactOverflow") == 0
+    got = {k: s.download(k) for k in ("px", "py", "pz", "pvx", "pvy", "pvz")}
+    st = oracle_from_golden(g, tag="fe0_")
+    st.set_particles(N, **parts)
+    st.set_params(spinOn=1, hasSun=1, sunX=sp[0], sunY=sp[1], sunZ=sp[2])
+    st.call("oc_vapor_collisions")
+    walk = (float(st["py"][v]) - c) / rad
+    print("  parcel walked %.3f radii (oracle), %.3f (gpu)" % (walk, (float(got["py"][v]) - c) / rad))
+    assert walk > 5.0
+    for k in got:
+        want = st[k][:N].astype(np.float64)
+        e = np.abs(got[k] - want).max() / max(1.0, np.abs(want).max())
+        print("  %-4s max rel err vs oracle %.3e" % (k, e))
+        assert e <= 2e-5, (k, e)
+
+
 LCG_A, LCG_C = 1664525, 1013904223
 
 
@@ -209,7 +267,7 @@ def lcg_back(s, n):
 
 
 @pytest.mark.parametrize("dup,rate,reject_at", [(3, 0.02, None), (3, 1.0, None), (1, 0.3, None), (3, 1e9, None),
-                                                 (1, 1e9, 19)])
+                                                 (1, 1e9, 19), (10, 1.0, None), (10, 1e9, None)])
 def test_evaporation_windows(dup, rate, reject_at):
     """The chooser tests a window of candidates under several counts of
     earlier ejections at once; this drives each way a window ends against
@@ -220,8 +278,10 @@ def test_evaporation_windows(dup, rate, reject_at):
     ejects, so windows end after their sixteenth ejection), and a start
     state whose stream puts a value below
     the 1e-9 cut at the first normal of candidate reject_at (every candidate
-    ejecting takes seven draws, that one eight). Counts, the stream state
-    after the pass and the flags exact; velocities within 1e-5."""
+    ejecting takes seven draws, that one eight). With dup 10 (17 170
+    candidates) a run of candidates crosses the chooser's 16 384-threshold
+    staging in LDS. Counts, the stream state after the pass and the flags
+    exact; velocities within 1e-5."""
     import cpu_oracle  # noqa: F401  (the oracle library must be built)
     from scenes import oracle_from_golden, tick_params, PART_KEYS, EXTRA_KEYS
     g = golden.load("thermal_32")
@@ -261,4 +321,57 @@ def test_evaporation_windows(dup, rate, reject_at):
         e = np.abs(s.download(k).astype(np.float64) - st[k][:N].astype(np.float64)).max()
         print("  %-4s max |err| %.3e" % (k, e))
         assert e <= 1e-5, (k, e)
+    s.close()
+
+
+@pytest.mark.parametrize("offset", [0, 1])
+def test_evaporation_draw_at_threshold(offset):
+    """A candidate whose draw is exactly the largest stream state that ejects
+    (state / 2^32 == floor(prob 2^32) / 2^32 <= prob: ejects) and one state
+    above it (does not): the probability and its threshold are evaluated in
+    double as _evapChunk does (solver.js:3062-3067). One particle the
+    reference ejected in this tick is the only candidate (every other one is
+    taken out of the pass by its flag); GPU and oracle agree on the flag,
+    the count and the stream state after the pass."""
+    import math
+    from scenes import oracle_from_golden, tick_params, PART_KEYS, EXTRA_KEYS
+    g = golden.load("thermal_32")
+    keys = [k for k in PART_KEYS + EXTRA_KEYS if "t1_" + k in g]
+    ejected = np.flatnonzero((g["t1_pflag"] == 0) & (g["t2_pflag"] == 2))
+    p = int(ejected[0])
+    parts = {k: g["t1_" + k].copy() for k in keys}
+    parts["pflag"][:] = 1
+    parts["pflag"][p] = 0
+    t = g.meta["tick"]
+    dt = t["dtTick"]
+    rate = t["sunActivity"] * t["evapIntensity"]
+    freeze = max(t["meltT"], t["snowT"] * 1.05)
+    prob = min(1.0, rate * dt * math.exp(3.5 * (float(parts["pT"][p]) - freeze)))
+    lim = math.floor(prob * 4294967296.0)
+    ev0 = lcg_back(lim + offset, 1)
+    N = len(parts["px"])
+    s = gpu_from_golden(g, tag="t1_")
+    for k in keys:
+        s.upload(k, parts[k])
+    s.upload("cellType", g["t0_cellType"])
+    for k in "uvw":
+        s.upload(k, g["t0_" + k])
+    rb = g.meta["rng_before"]
+    s.set_rng(rb["cfS"], rb["bfS"], ev0)
+    s.stage("evaporation", dt)
+    st = oracle_from_golden(g, tag="t1_")
+    st.set_particles(N, **parts)
+    st.arrays["cellType"][:] = g["t0_cellType"]
+    for k in "uvw":
+        st.arrays[k][:] = g["t0_" + k]
+    st.set_params(**tick_params(g))
+    st.set_params(evS=ev0)
+    st.call("oc_update_evaporation", dt)
+    want = 1 if offset == 0 else 0
+    print("  particle %d, prob %.17g, threshold %d, draw threshold + %d: gpu %d oracle %d"
+          % (p, prob, lim, offset, s.get("evapCount"), st.get_i("evapCount")))
+    assert st.get_i("evapCount") == want
+    assert int(s.get("evapCount")) == want
+    assert int(s.download("pflag")[p]) == int(st["pflag"][p]) == (2 if want else 0)
+    assert int(s.get_rng()[2]) == st.get_i("evS")
     s.close()

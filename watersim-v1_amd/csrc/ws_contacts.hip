@@ -839,15 +839,24 @@ __global__ void k_ct_ice_groups(int n, const unsigned long long *__restrict__ gk
 // vapor x liquid: each parcel meets the liquid particles near it in index
 // order and is pushed out of each it overlaps (the water never moves). One
 // wavefront per parcel: the lanes share the cells of the search box and
-// gather the liquid particles within three radii (the parcel moves by at
-// most one radius per contact) into LDS, the list is sorted by particle
-// index (rank of each entry), and the contacts run in that order. Three
-// radii hold about 25 particles at 8 per cell, but compressed water holds
-// more (64 was exceeded in the cfg3 run); beyond kVlMax the overflow counter
-// is raised and one lane gathers the first kVlMax in cell order instead, so
-// the result stays deterministic.
+// gather the liquid particles within three radii (the parcel moves by less
+// than one radius per contact) into LDS, the list is sorted by particle
+// index (rank of each entry), and the contacts run in that order. Once the
+// parcel has moved two radii from the gather's centre the next contact may
+// lie outside it: the lanes gather again around the parcel's position (the
+// particles after the last one visited) and the walk goes on. Three radii
+// hold about 25 particles at 8 per cell, but compressed water holds more (64
+// was exceeded in a 128^3 / 2M run); beyond kVlMax the overflow counter is
+// raised and one lane gathers the first kVlMax in cell order instead, so the
+// result stays deterministic.
+// Each pair is tested directly (x window, then distance). The reference
+// finds a liquid particle's parcels by a binary search over the vapor list
+// sorted by x before the vapor-vapor pushes (solver.js:3449-3451); a parcel
+// pushed along x past its neighbours in that stale order can be skipped
+// there and is met here.
 constexpr int kVlMax = 1024;
-constexpr int kVlWaves = 4;  // parcels per 256-thread workgroup
+constexpr int kVlWaves = 4;     // parcels per 256-thread workgroup
+constexpr int kVlRounds = 64;   // gathers per parcel before the overflow counter is raised
 
 __global__ __launch_bounds__(256) void k_ct_vapor_liquid(Geo g, int nV, const unsigned *__restrict__ vlist,
                                                          const unsigned *__restrict__ llist,
@@ -858,93 +867,126 @@ __global__ __launch_bounds__(256) void k_ct_vapor_liquid(Geo g, int nV, const un
                                                          float *__restrict__ vy, float *__restrict__ vz,
                                                          unsigned *__restrict__ overflow) {
   __shared__ unsigned scand[kVlWaves][kVlMax], ssort[kVlWaves][kVlMax];
-  __shared__ unsigned sn[kVlWaves];
+  __shared__ unsigned sn[kVlWaves], safter[kVlWaves], sagain[kVlWaves];
+  __shared__ float spos[kVlWaves][3];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int e = blockIdx.x * kVlWaves + wv;
   if (e >= nV) return;  // the whole wavefront leaves; no workgroup barrier follows
   const unsigned j = vlist[e];
-  const float x = px[j], y = py[j], z = pz[j];
-  if (lane == 0) sn[wv] = 0;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
+  const unsigned kNone = 0xffffffffu;
   const float reach = 3.f * rad, reach2 = reach * reach;
-  int ci = clampi((int)(x * g.inv_dx), 0, g.nx - 1), cj = clampi((int)(y * g.inv_dx), 0, g.ny - 1);
-  int ck = clampi((int)(z * g.inv_dx), 0, g.nz - 1);
-  int span = (int)ceilf(reach * g.inv_dx);
-  const int ia = max(ci - span, 0), ib = min(ci + span, g.nx - 1);
-  const int ja = max(cj - span, 0), jb = min(cj + span, g.ny - 1);
-  const int ka = max(ck - span, 0), kb = min(ck + span, g.nz - 1);
-  const int ni = ib - ia + 1, nj = jb - ja + 1, nk = kb - ka + 1;
-  for (int cidx = lane; cidx < ni * nj * nk; cidx += 64) {
-    int ii = ia + cidx % ni, jj = ja + (cidx / ni) % nj, kk = ka + cidx / (ni * nj);
-    int cc = (kk * g.ny + jj) * g.nx + ii;
-    for (unsigned b = lstart[cc]; b < lstart[cc + 1]; b++) {
-      const Slot s = lslot[b];
-      float ddx = s.x - x, ddy = s.y - y, ddz = s.z - z;
-      if (ddx * ddx + ddy * ddy + ddz * ddz >= reach2) continue;
-      unsigned at = atomicAdd(&sn[wv], 1u);
-      if (at < (unsigned)kVlMax) scand[wv][at] = llist[s.o];
-    }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const unsigned found = sn[wv];
-  const int nc = found < (unsigned)kVlMax ? (int)found : kVlMax;
-  if (found > (unsigned)kVlMax && lane == 0) {
-    atomicAdd(overflow, found - (unsigned)kVlMax);
-    // the first kVlMax in cell and bin order (arrival order is not fixed)
-    int m = 0;
-    for (int cidx = 0; cidx < ni * nj * nk && m < kVlMax; cidx++) {
+  // the parcel's state, held by lane 0 across gathers
+  float jx = px[j], jy = py[j], jz = pz[j], ux = vx[j], uy = vy[j], uz = vz[j];
+  if (lane == 0) { safter[wv] = kNone; spos[wv][0] = jx; spos[wv][1] = jy; spos[wv][2] = jz; }
+  for (int round = 0; round < kVlRounds; round++) {
+    if (lane == 0) sn[wv] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // gather around the parcel's position at the start of this round the
+    // liquid particles after the last one already visited (a push moves the
+    // parcel by less than one radius, so three radii hold every contact
+    // until the parcel has moved two radii from this centre)
+    const float x = spos[wv][0], y = spos[wv][1], z = spos[wv][2];
+    const unsigned after = safter[wv];
+    int ci = clampi((int)(x * g.inv_dx), 0, g.nx - 1), cj = clampi((int)(y * g.inv_dx), 0, g.ny - 1);
+    int ck = clampi((int)(z * g.inv_dx), 0, g.nz - 1);
+    int span = (int)ceilf(reach * g.inv_dx);
+    const int ia = max(ci - span, 0), ib = min(ci + span, g.nx - 1);
+    const int ja = max(cj - span, 0), jb = min(cj + span, g.ny - 1);
+    const int ka = max(ck - span, 0), kb = min(ck + span, g.nz - 1);
+    const int ni = ib - ia + 1, nj = jb - ja + 1, nk = kb - ka + 1;
+    for (int cidx = lane; cidx < ni * nj * nk; cidx += 64) {
       int ii = ia + cidx % ni, jj = ja + (cidx / ni) % nj, kk = ka + cidx / (ni * nj);
       int cc = (kk * g.ny + jj) * g.nx + ii;
-      for (unsigned b = lstart[cc]; b < lstart[cc + 1] && m < kVlMax; b++) {
+      for (unsigned b = lstart[cc]; b < lstart[cc + 1]; b++) {
         const Slot s = lslot[b];
+        const unsigned pi = llist[s.o];
+        if (after != kNone && pi <= after) continue;
         float ddx = s.x - x, ddy = s.y - y, ddz = s.z - z;
-        if (ddx * ddx + ddy * ddy + ddz * ddz < reach2) scand[wv][m++] = llist[s.o];
+        if (ddx * ddx + ddy * ddy + ddz * ddz >= reach2) continue;
+        unsigned at = atomicAdd(&sn[wv], 1u);
+        if (at < (unsigned)kVlMax) scand[wv][at] = pi;
       }
     }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // rank sort of the (distinct) particle indices
-  for (int q = lane; q < nc; q += 64) {
-    unsigned v = scand[wv][q];
-    int rank = 0;
-    for (int m = 0; m < nc; m++) rank += scand[wv][m] < v ? 1 : 0;
-    ssort[wv][rank] = v;
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (lane != 0) return;
-  const float rad2 = rad * rad;
-  float jx = x, jy = y, jz = z, ux = vx[j], uy = vy[j], uz = vz[j];
-  for (int k = 0; k < nc; k++) {
-    unsigned p = ssort[wv][k];
-    float wx = px[p], wy = py[p], wz = pz[p];
-    // the reference's x window: binary search to x >= wx - rad, stop at x - wx >= rad
-    if (jx < wx - rad) continue;
-    float ddx = jx - wx;
-    if (ddx >= rad) continue;
-    float ddy = jy - wy, ddz = jz - wz;
-    float dd2 = ddx * ddx + ddy * ddy + ddz * ddz;
-    if (dd2 >= rad2 || dd2 < 1e-12f) continue;
-    float dd = sqrtf(dd2);
-    float nX = ddx / dd, nY = ddy / dd, nZ = ddz / dd;
-    float vR = (ux - vx[p]) * nX + (uy - vy[p]) * nY + (uz - vz[p]) * nZ;
-    if (vR < 0.f) {
-      float vB = vR * 1.4f;
-      ux -= vB * nX; uy -= vB * nY; uz -= vB * nZ;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const unsigned found = sn[wv];
+    const int nc = found < (unsigned)kVlMax ? (int)found : kVlMax;
+    if (found > (unsigned)kVlMax && lane == 0) {
+      atomicAdd(overflow, found - (unsigned)kVlMax);
+      // the first kVlMax in cell and bin order (arrival order is not fixed)
+      int m = 0;
+      for (int cidx = 0; cidx < ni * nj * nk && m < kVlMax; cidx++) {
+        int ii = ia + cidx % ni, jj = ja + (cidx / ni) % nj, kk = ka + cidx / (ni * nj);
+        int cc = (kk * g.ny + jj) * g.nx + ii;
+        for (unsigned b = lstart[cc]; b < lstart[cc + 1] && m < kVlMax; b++) {
+          const Slot s = lslot[b];
+          const unsigned pi = llist[s.o];
+          if (after != kNone && pi <= after) continue;
+          float ddx = s.x - x, ddy = s.y - y, ddz = s.z - z;
+          if (ddx * ddx + ddy * ddy + ddz * ddz < reach2) scand[wv][m++] = pi;
+        }
+      }
     }
-    float ov = rad - dd;
-    jx += nX * ov; jy += nY * ov; jz += nZ * ov;
-    float kk2 = ov * 2.f;
-    ux += nX * kk2; uy += nY * kk2; uz += nZ * kk2;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // rank sort of the (distinct) particle indices
+    for (int q = lane; q < nc; q += 64) {
+      unsigned v = scand[wv][q];
+      int rank = 0;
+      for (int m = 0; m < nc; m++) rank += scand[wv][m] < v ? 1 : 0;
+      ssort[wv][rank] = v;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane == 0) {
+      const float rad2 = rad * rad, drift = reach - rad, drift2 = drift * drift;
+      unsigned again = 0, last = after;
+      for (int k = 0; k < nc; k++) {
+        // moved two radii from this round's centre: the next contacts may lie
+        // outside the gather, so gather again around the current position
+        float mx = jx - x, my = jy - y, mz = jz - z;
+        if (mx * mx + my * my + mz * mz > drift2) { again = 1; break; }
+        unsigned p = ssort[wv][k];
+        last = p;
+        float wx = px[p], wy = py[p], wz = pz[p];
+        // the reference's x window: binary search to x >= wx - rad, stop at x - wx >= rad
+        if (jx < wx - rad) continue;
+        float ddx = jx - wx;
+        if (ddx >= rad) continue;
+        float ddy = jy - wy, ddz = jz - wz;
+        float dd2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        if (dd2 >= rad2 || dd2 < 1e-12f) continue;
+        float dd = sqrtf(dd2);
+        float nX = ddx / dd, nY = ddy / dd, nZ = ddz / dd;
+        float vR = (ux - vx[p]) * nX + (uy - vy[p]) * nY + (uz - vz[p]) * nZ;
+        if (vR < 0.f) {
+          float vB = vR * 1.4f;
+          ux -= vB * nX; uy -= vB * nY; uz -= vB * nZ;
+        }
+        float ov = rad - dd;
+        jx += nX * ov; jy += nY * ov; jz += nZ * ov;
+        float kk2 = ov * 2.f;
+        ux += nX * kk2; uy += nY * kk2; uz += nZ * kk2;
+      }
+      if (again && round + 1 == kVlRounds) atomicAdd(overflow, 1u);
+      sagain[wv] = again;
+      safter[wv] = last;
+      spos[wv][0] = jx; spos[wv][1] = jy; spos[wv][2] = jz;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (!sagain[wv]) break;
   }
-  px[j] = jx; py[j] = jy; pz[j] = jz;
-  vx[j] = ux; vy[j] = uy; vz[j] = uz;
+  if (lane == 0) {
+    px[j] = jx; py[j] = jy; pz[j] = jz;
+    vx[j] = ux; vy[j] = uy; vz[j] = uz;
+  }
 }
 
 __global__ void k_ct_census(Geo g, int nV, const unsigned *__restrict__ vlist, const uint8_t *__restrict__ pflag,

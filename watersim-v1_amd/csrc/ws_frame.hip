@@ -71,9 +71,13 @@ int frame_heat(ws_ctx *c, double dt) {
 int frame_evaporation(ws_ctx *c, double dt) {
   if (c->sunActivity <= 0 || c->evapIntensity <= 0 || c->nP < 1) return WS_OK;
   EvapPar E;
-  E.dt = (float)dt;
-  E.rate = (float)(c->sunActivity * c->evapIntensity);
-  E.freezeT = (float)(c->meltT > c->snowT * 1.05 ? c->meltT : c->snowT * 1.05);
+  E.dt = dt;
+  E.rate = c->sunActivity * c->evapIntensity;
+  E.freezeT = c->meltT > c->snowT * 1.05 ? c->meltT : c->snowT * 1.05;
+  E.dx = c->dx;
+  E.cx = c->cx;
+  E.cy = c->cy;
+  E.cz = c->cz;
   long cap = c->capacity;
   unsigned *cand = c->evBuf, *off = c->evBuf + cap, *thresh = c->evBuf + 2 * cap, *list = c->evBuf + 3 * cap;
   unsigned *thrC = c->evBuf + 4 * cap, *evList = c->evBuf + 5 * cap, *evState = c->evBuf + 6 * cap;

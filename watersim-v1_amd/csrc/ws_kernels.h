@@ -19,8 +19,11 @@ struct ThermalPar {
   float Rlo2, Rhi2, Rhi, coreR2, inv_dv;
   int hasSun, hasTerrain, tn, conduct;
 };
+// evaporation law inputs as the reference holds them (doubles): the
+// candidate tests and the escape threshold are evaluated in double
 struct EvapPar {
-  float dt, rate, freezeT;
+  double dt, rate, freezeT;
+  double dx, cx, cy, cz;
 };
 struct CurrPar {
   int gnx, gny, gnz, kMax;

@@ -177,7 +177,10 @@ __global__ void k_heat_apply(Geo g, ThermalPar T, int nP, const float *__restric
 // Candidates of _evapChunk (solver.js:3041-3069): grid-coupled, above the
 // freezing point, in a FLUID cell with no FLUID cell along the local up.
 // Each candidate draws once from the _evS stream in particle order; the
-// escape threshold is stored as the largest stream state that ejects.
+// escape threshold is stored as the largest stream state that ejects. The
+// freezing gate, the up cell and the probability are evaluated in double
+// from the float fields, as the reference does: a draw ejects iff
+// state / 2^32 <= prob, i.e. state <= floor(prob * 2^32), exact in double.
 __global__ void k_evap_candidates(Geo g, EvapPar E, int nP, const float *__restrict__ px,
                                   const float *__restrict__ py, const float *__restrict__ pz,
                                   const float *__restrict__ pT, const uint8_t *__restrict__ pflag,
@@ -187,25 +190,25 @@ __global__ void k_evap_candidates(Geo g, EvapPar E, int nP, const float *__restr
   if (p >= nP) return;
   cand[p] = 0;
   if (pflag[p] != 0) return;
-  float T = pT[p];
+  const double T = pT[p];
   if (T <= E.freezeT) return;
-  float x = px[p], y = py[p], z = pz[p];
+  const float x = px[p], y = py[p], z = pz[p];
   int i = clampi(cell_floor(x, g.inv_dx, thr, g.nx), 1, g.nx - 2);
   int j = clampi(cell_floor(y, g.inv_dx, thr, g.ny), 1, g.ny - 2);
   int k = clampi(cell_floor(z, g.inv_dx, thr, g.nz), 1, g.nz - 2);
   if (ct[(k * g.ny + j) * g.nx + i] != FLUID) return;
-  float ex = x - g.cx, ey = y - g.cy, ez = z - g.cz;
-  float el = sqrtf(ex * ex + ey * ey + ez * ez);
-  if (!(el != 0.f)) el = 1e-9f;
-  float ux = x + ex / el * g.dx, uy = y + ey / el * g.dx, uz = z + ez / el * g.dx;
-  int ui = (int)(ux * g.inv_dx), uj = (int)(uy * g.inv_dx), uk = (int)(uz * g.inv_dx);
+  const double ex = (double)x - E.cx, ey = (double)y - E.cy, ez = (double)z - E.cz;
+  double el = sqrt(ex * ex + ey * ey + ez * ez);
+  if (!(el != 0.0)) el = 1e-9;
+  // (x + e/|e| dx) / dx truncated toward zero, as `| 0` does
+  const int ui = (int)(((double)x + ex / el * E.dx) / E.dx), uj = (int)(((double)y + ey / el * E.dx) / E.dx);
+  const int uk = (int)(((double)z + ez / el * E.dx) / E.dx);
   if (ui < 0 || uj < 0 || uk < 0 || ui >= g.nx || uj >= g.ny || uk >= g.nz) return;
   if (ct[(uk * g.ny + uj) * g.nx + ui] == FLUID) return;
-  float prob = E.rate * E.dt * expf(3.5f * (T - E.freezeT));
-  if (prob > 1.f) prob = 1.f;
-  // eject iff state / 2^32 <= prob, i.e. state <= floor(prob * 2^32)
-  float lim = prob * 4294967296.0f;
-  thresh[p] = lim >= 4294967040.0f ? 0xffffffffu : (unsigned)lim;
+  double prob = E.rate * E.dt * exp(3.5 * (T - E.freezeT));
+  if (prob > 1.0) prob = 1.0;
+  const double lim = floor(prob * 4294967296.0);
+  thresh[p] = lim >= 4294967295.0 ? 0xffffffffu : (unsigned)lim;
   cand[p] = 1;
 }
 
@@ -381,7 +384,8 @@ __global__ void k_evap_apply(Geo g, EvapPar E, const unsigned *__restrict__ list
     nrm[n] = sqrtf(-2.f * logf(u1)) * cosf(6.283185306f * u2);
   }
   float T = pT[p];
-  float tau = fminf(fmaxf((T - E.freezeT) / (1.15f - E.freezeT), 0.f), 1.f);
+  const float fT = (float)E.freezeT;
+  float tau = fminf(fmaxf((T - fT) / (1.15f - fT), 0.f), 1.f);
   float sig = 0.30f + 0.32f * sqrtf(tau);
   float kx = sig * nrm[0], ky = sig * nrm[1], kz = sig * nrm[2];
   pflag[p] = 2;
