@@ -193,12 +193,71 @@ int st_gravity(ws_ctx *c) {
   return WS_OK;
 }
 
+enum { SPAN_SOR = 0, SPAN_PRESSURE = 1, SPAN_NET = 2 };
+// spans queued before they are read (a read synchronizes with the stream);
+// their events are created when timing is switched on
+constexpr size_t kSpanMax = 512;
+
+int fold_spans(ws_ctx *c);
+
+// the start event of a timed span, recorded on the context stream
+int span_begin(ws_ctx *c, hipEvent_t *a) {
+  if (c->evFree.empty()) {
+    hipEvent_t e;
+    WS_HIP(hipEventCreate(&e));
+    c->evFree.push_back(e);
+  }
+  *a = c->evFree.back();
+  c->evFree.pop_back();
+  WS_HIP(hipEventRecord(*a, c->stream));
+  return WS_OK;
+}
+
+// the end event; the pair is read later by fold_spans
+int span_end(ws_ctx *c, hipEvent_t a, int kind, long launches) {
+  if (c->evFree.empty()) {
+    hipEvent_t e;
+    WS_HIP(hipEventCreate(&e));
+    c->evFree.push_back(e);
+  }
+  hipEvent_t b = c->evFree.back();
+  c->evFree.pop_back();
+  WS_HIP(hipEventRecord(b, c->stream));
+  c->spans.push_back(ws_ctx::Span{a, b, kind, launches});
+  return c->spans.size() >= kSpanMax ? fold_spans(c) : WS_OK;
+}
+
+// reads every queued span into the totals and recycles its events
+int fold_spans(ws_ctx *c) {
+  for (auto &sp : c->spans) {
+    WS_HIP(hipEventSynchronize(sp.b));
+    float ms = 0;
+    WS_HIP(hipEventElapsedTime(&ms, sp.a, sp.b));
+    if (sp.kind == SPAN_SOR) {
+      c->sorMs += ms;
+      c->sorLaunches += sp.launches;
+    } else if (sp.kind == SPAN_PRESSURE) {
+      c->pressureMs += ms;
+      c->pressureSolves++;
+    } else {
+      c->netMs += ms;
+      c->netCalls++;
+    }
+    c->evFree.push_back(sp.a);
+    c->evFree.push_back(sp.b);
+  }
+  c->spans.clear();
+  return WS_OK;
+}
+
 int sor_sweeps(ws_ctx *c, int n) {
   c->tbLaunchesPending = 0;
   int half = (c->nx + 1) / 2;
   long nthr = (long)half * c->ny * c->nz;
   float om1 = (float)(1 - c->omega);
-  if (c->timing && n > 0) WS_HIP(hipEventRecord(c->ev[0], c->stream));
+  hipEvent_t span = nullptr;
+  int r;
+  if (c->timing && n > 0 && (r = span_begin(c, &span))) return r;
   int nT = c->sorTiles[0] * c->sorTiles[1] * c->sorTiles[2];
   for (int it = 0; it < n; it++) {
     if (c->sorMode == 3) {
@@ -233,21 +292,18 @@ int sor_sweeps(ws_ctx *c, int n) {
   }
   WS_LAUNCHED();
   c->sweepsTotal += n;
-  if (c->timing && n > 0) {
-    // the batch holds only back-to-back SOR launches: its span over the
+  c->sorCellSweeps += (c->sorMode >= 2 ? c->nRed + c->nBlack : c->nC) * (long)n;
+  if (span) {
+    // the span holds only back-to-back SOR launches: its length over the
     // launch count is the kernel's average duration (gaps included)
-    WS_HIP(hipEventRecord(c->ev[1], c->stream));
-    WS_HIP(hipEventSynchronize(c->ev[1]));
-    float ms = 0;
-    WS_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    c->sorMs += ms;
+    long launches;
     if (c->sorMode == 3) {
-      c->sorLaunches += c->tbLaunchesPending;
+      launches = c->tbLaunchesPending;
     } else {
       long per = c->sorMode == 1 ? 1L : (c->sorMode == 2 ? (long)(c->nRed > 0) + (c->nBlack > 0) : 2L);
-      c->sorLaunches += per * n;
+      launches = per * n;
     }
-    c->sorCellSweeps += (c->sorMode >= 2 ? c->nRed + c->nBlack : c->nC) * (long)n;
+    if ((r = span_end(c, span, SPAN_SOR, launches))) return r;
   }
   return WS_OK;
 }
@@ -406,17 +462,11 @@ int st_pressure_compare(ws_ctx *c) {
 
 int st_pressure(ws_ctx *c) {
   if (c->pressureCompare && c->timing) return st_pressure_compare(c);
-  if (c->timing) WS_HIP(hipEventRecord(c->ev[2], c->stream));
-  int r = st_pressure_solve(c);
-  if (r) return r;
-  if (c->timing) {
-    WS_HIP(hipEventRecord(c->ev[3], c->stream));
-    WS_HIP(hipEventSynchronize(c->ev[3]));
-    float ms = 0;
-    WS_HIP(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-    c->pressureMs += ms;
-    c->pressureSolves++;
-  }
+  hipEvent_t span = nullptr;
+  int r;
+  if (c->timing && (r = span_begin(c, &span))) return r;
+  if ((r = st_pressure_solve(c))) return r;
+  if (span && (r = span_end(c, span, SPAN_PRESSURE, 0))) return r;
   return WS_OK;
 }
 
@@ -463,16 +513,10 @@ int pressure_setup(ws_ctx *c) {
   // start the sweeps from the network's prediction
   if (c->recordSamples && (r = net_record_inputs(c))) return r;
   if (c->warmStart) {
-    if (c->timing) WS_HIP(hipEventRecord(c->ev[0], c->stream));
+    hipEvent_t span = nullptr;
+    if (c->timing && (r = span_begin(c, &span))) return r;
     if ((r = net_warm_start(c))) return r;
-    if (c->timing) {
-      WS_HIP(hipEventRecord(c->ev[1], c->stream));
-      WS_HIP(hipEventSynchronize(c->ev[1]));
-      float ms = 0;
-      WS_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-      c->netMs += ms;
-      c->netCalls++;
-    }
+    if (span && (r = span_end(c, span, SPAN_NET, 0))) return r;
   }
   return WS_OK;
 }
@@ -928,6 +972,11 @@ void ws_destroy(ws_ctx *c) {
   if (c->renderHost) (void)hipHostFree(c->renderHost);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto &sp : c->spans) {
+    (void)hipEventDestroy(sp.a);
+    (void)hipEventDestroy(sp.b);
+  }
+  for (auto &e : c->evFree) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -991,10 +1040,15 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
   if (!strcmp(n, "hasVapor")) { c->hasVapor = v != 0; return WS_OK; }
   if (!strcmp(n, "timing")) {
     int r = fold_dev_stats(c);  // solves before the reset stay out of the new totals
-    if (r) return r;
+    if (r || (r = fold_spans(c))) return r;
     c->timing = v != 0;
     for (int i = 0; i < 4 && c->timing; i++)
       if (!c->ev[i]) WS_HIP(hipEventCreate(&c->ev[i]));
+    while (c->timing && c->evFree.size() < 2 * kSpanMax + 2) {
+      hipEvent_t e;
+      WS_HIP(hipEventCreate(&e));
+      c->evFree.push_back(e);
+    }
     c->sorMs = c->pressureMs = 0;
     c->sorLaunches = c->pressureSolves = c->sweepsTotal = c->residualsTotal = c->ticksTotal = 0;
     c->sorCellSweeps = 0;
@@ -1013,6 +1067,9 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   if (!strcmp(n, "residual") || !strcmp(n, "sweepsTotal") || !strcmp(n, "residualsTotal") ||
       !strcmp(n, "pressureIters_done"))
     if ((r = fold_dev_stats(c))) return r;
+  if (!strcmp(n, "sorMs") || !strcmp(n, "sorLaunches") || !strcmp(n, "pressureMs") ||
+      !strcmp(n, "pressureSolves") || !strcmp(n, "netMs") || !strcmp(n, "netCalls"))
+    if ((r = fold_spans(c))) return r;
   if (!strcmp(n, "umax") || !strcmp(n, "airborneCount") || !strcmp(n, "gRamp") || !strcmp(n, "dtLast") ||
       !strcmp(n, "dfRefined")) {
     if ((r = pull_scalars(c))) return r;

@@ -137,6 +137,17 @@ struct Ctx {
   // live timers (HIP events on the context stream), enabled by "timing"
   int timing = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // timed spans of the solver's own work (SOR batches, whole pressure solves,
+  // warm-start inferences): event pairs left queued on the stream and read
+  // only when a total is asked for (or the queue is long), so timing adds no
+  // host synchronization to the run
+  struct Span {
+    hipEvent_t a, b;
+    int kind;
+    long launches;
+  };
+  std::vector<Span> spans;
+  std::vector<hipEvent_t> evFree;
   double sorMs = 0, pressureMs = 0;
   long sorLaunches = 0, pressureSolves = 0, sweepsTotal = 0, residualsTotal = 0, ticksTotal = 0;
   long sorCellSweeps = 0;  // cells updated by the timed SOR launches
