@@ -124,6 +124,16 @@ int ws_step_frame(ws_ctx *c, double dt_frame, int *n_sub);
  * between substeps): the measurement path for substeps/s. */
 int ws_run_substeps(ws_ctx *c, int n);
 
+/* _markFluidCells(p0, p1), solver.js:945-1000: the marking pass over
+ * particles [p0, p1) on the current cell types (no re-rasterization): sky and
+ * ceiling demotions, AIR -> FLUID, and each marked particle's cell index
+ * ("pcell"). */
+int ws_mark_fluid(ws_ctx *c, int p0, int p1);
+/* sampleVel(x, y, z), solver.js:811-854: the trilinear MAC sample of the
+ * current face velocities at n points (xyz: 3n floats, host), written to out
+ * (3n floats, host). The sampler is the one G2P uses, so a sample at a
+ * particle's position equals its "advU/advV/advW" entry after G2P. */
+int ws_sample_vel(ws_ctx *c, const float *xyz, long n, float *out);
 /* splatDensity(), solver.js:3785-3820: corner density of fluid particles. */
 int ws_splat_density(ws_ctx *c);
 /* MarchingTetrahedra.build(dens,nx,ny,nz,dx,iso), surface.js:66-177, on the

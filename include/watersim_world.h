@@ -40,6 +40,11 @@ typedef struct {
 int ws_world_build(const ws_world_params *p, ws_world *w);
 /* Releases the arrays of a built world. */
 void ws_world_free(ws_world *w);
+/* _buildPlanetGravity, solver.js:1177-1210: the unit-strength radial
+ * gravity -oceanR^2 e/|e|^3 of every face (e from the domain centre), written
+ * as gu | gv | gw into g (3 (nx+1) ny nz floats for a cubic grid). Returns 0,
+ * or -1 on bad arguments. */
+int ws_planet_gravity(int nx, int ny, int nz, double dx, double oceanR, float *g);
 
 #ifdef __cplusplus
 }

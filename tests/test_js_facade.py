@@ -48,3 +48,12 @@ def test_metaballs_through_node():
 @pytest.mark.parametrize("section", ["settle", "heat", "shade", "currents", "whirl", "long", "balls"])
 def test_planet_invariants_through_node(section):
     run_node("planet_gpu.test.js", section, timeout=110, gpu=True)
+
+
+@pytest.mark.gpu
+def test_optimization_invariants_through_node():
+    """test/optimization.test.js:3-25 restated through the facade (heat
+    conservation, the G2P cache against sampleVel, the gravity slider, the
+    samplers' KAT), plus _markFluidCells, dryLandFraction, waveImpulse."""
+    out = run_node("optimization_gpu.test.js", timeout=120, gpu=True)
+    assert "ALL OPTIMIZATION INVARIANT TESTS PASSED" in out

@@ -32,3 +32,14 @@ def test_world_matches_reference(scene, seed, target):
         assert np.array_equal(w[k], g["fill_" + k]), k
     assert abs(w["spacing"] - wm["spacing"]) <= 1e-15 * wm["spacing"]
     assert w["landFrac"] == wm["landFrac"]
+
+
+@pytest.mark.parametrize("scene", ["cfg1_32", "micro_16", "thermal_32"])
+def test_planet_gravity_matches_reference(scene):
+    """ws_planet_gravity (what the facade's _buildPlanetGravity calls) equals
+    the reference's _buildPlanetGravity faces (solver.js:1177-1210) for the
+    scene's ocean radius, bit for bit."""
+    g = golden.load(scene)
+    wm = g.meta["world"]
+    gu, gv, gw = watersim_hip.planet_gravity(wm["nx"], wm["dx"], wm["oceanR"])
+    assert np.array_equal(gu, g["gu"]) and np.array_equal(gv, g["gv"]) and np.array_equal(gw, g["gw"])

@@ -447,6 +447,54 @@ static napi_value js_world_build(napi_env env, napi_callback_info info) {
   return o;
 }
 
+/* markFluid(h, p0, p1): _markFluidCells over particles [p0, p1) */
+static napi_value js_mark_fluid(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  if (!get_args(env, info, 3, a)) return NULL;
+  ws_ctx *c = get_ctx(env, a[0]);
+  double p0, p1;
+  if (!c || !get_double(env, a[1], &p0) || !get_double(env, a[2], &p1)) return NULL;
+  int rc = ws_mark_fluid(c, (int)p0, (int)p1);
+  return rc ? fail(env, c, rc, "ws_mark_fluid") : NULL;
+}
+
+/* sampleVel(h, xyz Float32Array(3n), out Float32Array(3n)) */
+static napi_value js_sample_vel(napi_env env, napi_callback_info info) {
+  napi_value a[3];
+  if (!get_args(env, info, 3, a)) return NULL;
+  ws_ctx *c = get_ctx(env, a[0]);
+  void *xyz = NULL, *out = NULL;
+  size_t nb1 = 0, nb2 = 0;
+  if (!c || !get_bytes(env, a[1], &xyz, &nb1) || !get_bytes(env, a[2], &out, &nb2)) return NULL;
+  if (nb1 != nb2 || nb1 % 12) {
+    napi_throw_range_error(env, NULL, "sampleVel: xyz and out must both hold 3 floats per point");
+    return NULL;
+  }
+  int rc = ws_sample_vel(c, (const float *)xyz, (long)(nb1 / 12), (float *)out);
+  return rc ? fail(env, c, rc, "ws_sample_vel") : NULL;
+}
+
+/* planetGravity(nx, ny, nz, dx, oceanR) -> Float32Array gu | gv | gw */
+static napi_value js_planet_gravity(napi_env env, napi_callback_info info) {
+  napi_value a[5];
+  if (!get_args(env, info, 5, a)) return NULL;
+  double nx, ny, nz, dx, oR;
+  if (!get_double(env, a[0], &nx) || !get_double(env, a[1], &ny) || !get_double(env, a[2], &nz) ||
+      !get_double(env, a[3], &dx) || !get_double(env, a[4], &oR))
+    return NULL;
+  size_t F = (size_t)(nx + 1) * ny * nz + (size_t)nx * (ny + 1) * nz + (size_t)nx * ny * (nz + 1);
+  float *g = (float *)malloc(sizeof(float) * (F ? F : 1));
+  if (!g) return fail(env, NULL, -1, "planetGravity");
+  if (ws_planet_gravity((int)nx, (int)ny, (int)nz, dx, oR, g) != 0) {
+    free(g);
+    napi_throw_range_error(env, NULL, "planetGravity: bad grid");
+    return NULL;
+  }
+  napi_value r = typed_copy(env, napi_float32_array, g, F, 4);
+  free(g);
+  return r;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
   napi_property_descriptor d[] = {
       {"create", 0, js_create, 0, 0, 0, napi_default, 0},
@@ -469,6 +517,9 @@ static napi_value init(napi_env env, napi_value exports) {
       {"setBalls", 0, js_set_balls, 0, 0, 0, napi_default, 0},
       {"getBalls", 0, js_get_balls, 0, 0, 0, napi_default, 0},
       {"worldBuild", 0, js_world_build, 0, 0, 0, napi_default, 0},
+      {"markFluid", 0, js_mark_fluid, 0, 0, 0, napi_default, 0},
+      {"sampleVel", 0, js_sample_vel, 0, 0, 0, napi_default, 0},
+      {"planetGravity", 0, js_planet_gravity, 0, 0, 0, napi_default, 0},
   };
   if (napi_define_properties(env, exports, sizeof d / sizeof d[0], d) != napi_ok) return NULL;
   return exports;

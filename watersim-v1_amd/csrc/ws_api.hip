@@ -965,7 +965,7 @@ void ws_destroy(ws_ctx *c) {
                   c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pT, c->pAir, c->advU, c->advV, c->advW, c->pflag,
                   c->pcool, c->pcell, c->prank, c->sortedIdx, c->vneed, c->vscr, c->rng, c->ds, c->partials,
                   c->scanTmp, c->exTiles, c->mtCount, c->mtPos, c->mtNrm, c->mtOff, c->tsolid, c->pDepth,
-                  c->pLight, c->pWx, c->pWy, c->pWz, c->pPh, c->evBuf, c->jmp, c->currField};
+                  c->pLight, c->pWx, c->pWy, c->pWz, c->pPh, c->evBuf, c->jmp, c->currField, c->smpBuf};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->hs) (void)hipHostFree(c->hs);
@@ -1416,6 +1416,39 @@ int ws_read_render(ws_ctx *c, unsigned mask, ws_render_view *view) {
     *f.dst = c->renderHost + off;
     off += (f.elem * n + 255) & ~(size_t)255;
   }
+  WS_HIP(hipStreamSynchronize(c->stream));
+  return WS_OK;
+}
+
+int ws_sample_vel(ws_ctx *c, const float *xyz, long n, float *out) {
+  if (!c || n < 0 || (n && (!xyz || !out))) return WS_EINVAL;
+  if (!n) return WS_OK;
+  if (n > c->smpCap) {
+    if (c->smpBuf) (void)hipFree(c->smpBuf);
+    c->smpBuf = nullptr;
+    c->smpCap = 0;
+    int r = dalloc(c, &c->smpBuf, 6 * n);
+    if (r) return r;
+    c->smpCap = n;
+  }
+  WS_HIP(hipMemcpyAsync(c->smpBuf, xyz, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_sample_vel, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, c->geo, n, c->smpBuf, c->vel,
+                     c->smpBuf + 3 * n);
+  WS_LAUNCHED();
+  WS_HIP(hipMemcpyAsync(out, c->smpBuf + 3 * n, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  WS_HIP(hipStreamSynchronize(c->stream));
+  return WS_OK;
+}
+
+int ws_mark_fluid(ws_ctx *c, int p0, int p1) {
+  if (!c || p0 < 0 || p1 < p0 || p1 > c->nP) return WS_EINVAL;
+  int r;
+  if ((r = need_world(c))) return r;
+  if (p1 > p0)
+    hipLaunchKernelGGL(k_mark, dim3(blocks_for(p1 - p0, 256)), dim3(256), 0, c->stream, c->geo, world_of(c), p1 - p0,
+                       c->px + p0, c->py + p0, c->pz + p0, c->pflag + p0, c->cellType, c->cellCount, c->pcell + p0,
+                       c->prank + p0, c->ds);
+  WS_LAUNCHED();
   WS_HIP(hipStreamSynchronize(c->stream));
   return WS_OK;
 }

@@ -254,6 +254,8 @@ struct Ctx {
   std::vector<int4> exTilesHost;  // the same schedule on the host (batches merge the planets')
   std::vector<int> exDiagOff;  // tiles of diagonal d: [off[d], off[d+1])
   // marching tets scratch
+  float *smpBuf = nullptr;  // sampleVel points and results, 6 floats per point
+  long smpCap = 0;
   unsigned *mtCount = nullptr, *mtOff = nullptr;
   float *mtPos = nullptr, *mtNrm = nullptr, *mtDens = nullptr;
   long mtCells = 0, mtCap = 0, mtDensN = 0;

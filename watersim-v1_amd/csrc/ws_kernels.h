@@ -114,6 +114,7 @@ __global__ void k_vapor_c(Geo g, World w, Par P, int nP, float *px, float *py, f
                           const float *scr, const unsigned *needB, const unsigned *offB, const DevScalars *s,
                           const uint32_t *rng);
 __global__ void k_rng_advance(uint32_t *rng, int which, const unsigned *need, const unsigned *off, int nP);
+__global__ void k_sample_vel(Geo g, long n, const float *xyz, const float *vel, float *out);
 
 // surface
 __global__ void k_splat_density(Geo g, World w, int nP, const unsigned *start, const unsigned *sorted,

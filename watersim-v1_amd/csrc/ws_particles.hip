@@ -18,6 +18,19 @@ __device__ __forceinline__ void sample3(const Geo &g, const float *__restrict__ 
   out[2] = stencil_eval(s, vel + g.uN + g.vN);
 }
 
+// sampleVel (solver.js:811-854) at n caller points: the sampler G2P uses,
+// so a sample at a particle's position equals the G2P cache entry
+__global__ void k_sample_vel(Geo g, long n, const float *__restrict__ xyz, const float *__restrict__ vel,
+                             float *__restrict__ out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float v[3];
+  sample3(g, vel, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], v);
+  out[3 * i] = v[0];
+  out[3 * i + 1] = v[1];
+  out[3 * i + 2] = v[2];
+}
+
 __device__ __forceinline__ float rlen(float x, float y, float z) {
   float r = sqrtf(x * x + y * y + z * z);
   return r != 0.f ? r : 1e-9f;  // "|| 1e-9" in the reference

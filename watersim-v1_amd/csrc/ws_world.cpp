@@ -117,6 +117,44 @@ void parallel_for(int n, F f) {
 
 extern "C" {
 
+// _buildPlanetGravity, solver.js:1177-1210: -oceanR^2 e / |e|^3 per face
+// (e from the domain centre to the face centre), u | v | w back to back
+int ws_planet_gravity(int nx, int ny, int nz, double dx, double oceanR, float *g) {
+  if (nx < 1 || ny < 1 || nz < 1 || !(dx > 0) || !g) return -1;
+  const long uN = (long)(nx + 1) * ny * nz, vN = (long)nx * (ny + 1) * nz;
+  const double cx = nx * dx / 2, cy = ny * dx / 2, cz = nz * dx / 2;
+  const double g0 = oceanR * oceanR;
+  auto r3f = [](double a, double b, double c) {
+    double r3 = std::pow(a * a + b * b + c * c, 1.5);
+    return r3 != 0 ? r3 : 1e-9;
+  };
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++) {
+      double ey = (j + 0.5) * dx - cy, ez = (k + 0.5) * dx - cz;
+      for (int i = 0; i <= nx; i++) {
+        double ex = i * dx - cx;
+        g[((long)k * ny + j) * (nx + 1) + i] = (float)(-g0 * ex / r3f(ex, ey, ez));
+      }
+    }
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j <= ny; j++) {
+      double ey = j * dx - cy, ez = (k + 0.5) * dx - cz;
+      for (int i = 0; i < nx; i++) {
+        double ex = (i + 0.5) * dx - cx;
+        g[uN + ((long)k * (ny + 1) + j) * nx + i] = (float)(-g0 * ey / r3f(ex, ey, ez));
+      }
+    }
+  for (int k = 0; k <= nz; k++)
+    for (int j = 0; j < ny; j++) {
+      double ey = (j + 0.5) * dx - cy, ez = k * dx - cz;
+      for (int i = 0; i < nx; i++) {
+        double ex = (i + 0.5) * dx - cx;
+        g[uN + vN + ((long)k * ny + j) * nx + i] = (float)(-g0 * ez / r3f(ex, ey, ez));
+      }
+    }
+  return 0;
+}
+
 int ws_world_build(const ws_world_params *p, ws_world *w) {
   if (!p || !w || p->nx < 4 || p->ny < 4 || p->nz < 4 || !(p->dx > 0)) return -1;
   std::memset(w, 0, sizeof(*w));
@@ -315,35 +353,7 @@ int ws_world_build(const ws_world_params *p, ws_world *w) {
   // radial inverse-square gravity of unit strength (solver.js:1177-1210)
   long uN = (long)(nx + 1) * ny * nz, vN = (long)nx * (ny + 1) * nz, wN = (long)nx * ny * (nz + 1);
   float *g = (float *)std::malloc(sizeof(float) * (uN + vN + wN));
-  double g0 = R2 * R2;
-  auto r3f = [](double a, double b, double c) {
-    double r3 = std::pow(a * a + b * b + c * c, 1.5);
-    return r3 != 0 ? r3 : 1e-9;
-  };
-  for (int k = 0; k < nz; k++)
-    for (int j = 0; j < ny; j++) {
-      double ey = (j + 0.5) * dx - cy, ez = (k + 0.5) * dx - cz;
-      for (int i = 0; i <= nx; i++) {
-        double ex = i * dx - cx;
-        g[((long)k * ny + j) * (nx + 1) + i] = (float)(-g0 * ex / r3f(ex, ey, ez));
-      }
-    }
-  for (int k = 0; k < nz; k++)
-    for (int j = 0; j <= ny; j++) {
-      double ey = j * dx - cy, ez = (k + 0.5) * dx - cz;
-      for (int i = 0; i < nx; i++) {
-        double ex = (i + 0.5) * dx - cx;
-        g[uN + ((long)k * (ny + 1) + j) * nx + i] = (float)(-g0 * ey / r3f(ex, ey, ez));
-      }
-    }
-  for (int k = 0; k <= nz; k++)
-    for (int j = 0; j < ny; j++) {
-      double ey = (j + 0.5) * dx - cy, ez = k * dx - cz;
-      for (int i = 0; i < nx; i++) {
-        double ex = (i + 0.5) * dx - cx;
-        g[uN + vN + ((long)k * ny + j) * nx + i] = (float)(-g0 * ez / r3f(ex, ey, ez));
-      }
-    }
+  ws_planet_gravity(nx, ny, nz, dx, R2, g);
   w->tn = n;
   w->dv = dv;
   w->solid = solid;

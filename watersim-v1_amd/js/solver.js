@@ -64,10 +64,10 @@ var GRID_FIELDS = {
 
 function FluidSolver(opts) {
   opts = opts || {};
-  if (opts.mode !== 'sphere') {
-    throw new Error('watersim: only the ocean-planet world (mode: "sphere") is part of this build');
-  }
-  this.mode = 'sphere';
+  // solver.js:78; the box pool is not part of this build: device calls in
+  // 'pool' mode throw (callers set mode = 'sphere', as the reference's
+  // tests do after construction)
+  this.mode = opts.mode === 'sphere' ? 'sphere' : 'pool';
   this.nx = opts.nx | 0; this.ny = opts.ny | 0; this.nz = opts.nz | 0;
   this.dx = opts.dx;
   // tunables with the reference defaults
@@ -178,8 +178,33 @@ FluidSolver.prototype._invalidate = function () {
   this._dirty = {};
 };
 
+// a device context for calls made before resetWater (a solver whose
+// particles were set by hand after _allocParticles): an empty static world
+// (every cell AIR, no gravity geometry, no terrain) and every host mirror
+// written so far uploaded with the next call
+FluidSolver.prototype._ensure = function () {
+  if (this.mode !== 'sphere') {
+    throw new Error('watersim: only the ocean-planet world (mode "sphere") is part of this build');
+  }
+  if (this._h) return;
+  var L = lib();
+  this._h = L.create(this.nx, this.ny, this.nz, this.dx, Math.max(1, this.capacity));
+  this._ballsOnDevice = false;
+  var h = this._h;
+  L.setF(h, 'coreR', this.coreR); L.setF(h, 'oceanR', this.oceanR); L.setF(h, 'spacing', this.spacing);
+  L.setF(h, 'particleVolume', this.particleVolume);
+  L.upload(h, 'staticTypes', new Uint8Array(this.nCells));
+  L.upload(h, 'gu', new Float32Array(this.uN));
+  L.upload(h, 'gv', new Float32Array(this.vN));
+  L.upload(h, 'gw', new Float32Array(this.wN));
+  L.setF(h, 'nP', this._nP);
+  this._stale = {};
+  this._dirty = {};
+  for (var name in this._host) this._dirty[name] = true;
+};
+
 FluidSolver.prototype._call = function (fn) {
-  if (!this._h) throw new Error('watersim: no world yet (call resetWater first)');
+  this._ensure();
   this._push();
   var L = lib();
   var nb = this.balls.length;
@@ -246,6 +271,46 @@ Object.defineProperty(FluidSolver.prototype, 'nP', {
   }
 });
 
+// _allocParticles(cap), solver.js:218-231: particle capacity. The device
+// context holds its capacity for life, so growing it moves the state to a
+// new context (static world, particle and grid fields and scalars).
+FluidSolver.prototype._allocParticles = function (cap) {
+  cap = cap | 0;
+  if (this.capacity >= cap) return;
+  if (!this._h) { this.capacity = cap; return; }
+  var L = lib(), old = this._h, names = Object.keys(PARTICLE_FIELDS).concat(Object.keys(GRID_FIELDS));
+  for (var i = 0; i < names.length; i++) this._touch(names[i]);
+  var keep = {};
+  ['gRamp', 'simTime', 'heatAcc', 'tn', 'dv', 'Rlo', 'Rhi'].forEach(function (k) { keep[k] = L.getF(old, k); });
+  var st = new Uint8Array(this.nCells), gu = new Float32Array(this.uN), gv = new Float32Array(this.vN),
+    gw = new Float32Array(this.wN);
+  L.download(old, 'staticTypes', st);
+  L.download(old, 'gu', gu); L.download(old, 'gv', gv); L.download(old, 'gw', gw);
+  var hosts = this._host;
+  this._host = {};
+  for (var n in hosts) {
+    var a = hosts[n];
+    if (PARTICLE_FIELDS[n]) { var b = new PARTICLE_FIELDS[n](cap); b.set(a.subarray(0, Math.min(a.length, cap))); a = b; }
+    this._host[n] = a;
+  }
+  L.destroy(old);
+  this._h = null;
+  this.capacity = cap;
+  this._ensure();
+  var h = this._h;
+  L.upload(h, 'staticTypes', st);
+  L.upload(h, 'gu', gu); L.upload(h, 'gv', gv); L.upload(h, 'gw', gw);
+  L.setF(h, 'tn', keep.tn); L.setF(h, 'dv', keep.dv); L.setF(h, 'Rlo', keep.Rlo); L.setF(h, 'Rhi', keep.Rhi);
+  if (this.terrain) { L.upload(h, 'terrainR', this.terrain.R); L.upload(h, 'terrainSolid', this.terrain.solid); }
+  L.setF(h, 'gRamp', keep.gRamp); L.setF(h, 'simTime', keep.simTime); L.setF(h, 'heatAcc', keep.heatAcc);
+};
+
+// the gravity ramp timer (solver.js:2657-2658), read and written by callers
+Object.defineProperty(FluidSolver.prototype, '_gRamp', {
+  get: function () { return this._h ? lib().getF(this._h, 'gRamp') : 0; },
+  set: function (v) { this._ensure(); lib().setF(this._h, 'gRamp', +v); }
+});
+
 function devScalar(name, dev) {
   Object.defineProperty(FluidSolver.prototype, name, {
     get: function () { return this._h ? lib().getF(this._h, dev) : 0; }
@@ -261,6 +326,9 @@ devScalar('evapCount', 'evapCount');
 // resetWater(depth), solver.js:611 / _resetWaterSphere :651-731: the voxel
 // planet, the ocean fill at rest, radial gravity and the iso calibration
 FluidSolver.prototype.resetWater = function (depth) {
+  if (this.mode !== 'sphere') {
+    throw new Error('watersim: only the ocean-planet world (mode "sphere") is part of this build');
+  }
   var L = lib();
   this.balls.length = 0;
   var w = L.worldBuild({
@@ -425,6 +493,138 @@ FluidSolver.prototype.stirAt = function (cx, cy, cz, ux, uy, uz, R, dt) {
     vy[p] += (uy * 1.25 - vy[p]) * k * f;
     vz[p] += (uz * 1.25 - vz[p]) * k * f;
     fl[p] = 0;
+  }
+};
+
+// ---------------------------------------------------------------- sampling
+// sampleVel / _sampleVel3, solver.js:811-854: the device sampler G2P uses, so
+// sampleVel at a particle equals its _advU/_advV/_advW entry after _g2p
+var _pt = new Float32Array(3), _pv = new Float32Array(3);
+FluidSolver.prototype._sampleVel3 = function (x, y, z, out) {
+  _pt[0] = x; _pt[1] = y; _pt[2] = z;
+  this._call(function (L, h) { L.sampleVel(h, _pt, _pv); });
+  out[0] = _pv[0]; out[1] = _pv[1]; out[2] = _pv[2];
+  return out;
+};
+FluidSolver.prototype.sampleVel = function (x, y, z, out) {
+  return this._sampleVel3(x, y, z, out || [0, 0, 0]);
+};
+
+// _faceCoords, solver.js:737-748: base index and fractions of component
+// comp's face grid at (x, y, z), with the reference's clamps (i0 in
+// [0, nx-1] with tx = 1 at the top, j0 and k0 in [0, n-2])
+FluidSolver.prototype._faceCoords = function (comp, x, y, z) {
+  var dx = this.dx;
+  var a = comp === 0 ? x / dx : x / dx - 0.5;
+  var b = comp === 1 ? y / dx : y / dx - 0.5;
+  var c = comp === 2 ? z / dx : z / dx - 0.5;
+  var f = { i0: Math.floor(a), j0: Math.floor(b), k0: Math.floor(c), tx: 0, ty: 0, tz: 0 };
+  if (f.i0 < 0) { f.i0 = 0; f.tx = 0; } else if (f.i0 > this.nx - 1) { f.i0 = this.nx - 1; f.tx = 1; } else f.tx = a - f.i0;
+  if (f.j0 < 0) { f.j0 = 0; f.ty = 0; } else if (f.j0 > this.ny - 2) { f.j0 = this.ny - 2; f.ty = 1; } else f.ty = b - f.j0;
+  if (f.k0 < 0) { f.k0 = 0; f.tz = 0; } else if (f.k0 > this.nz - 2) { f.k0 = this.nz - 2; f.tz = 1; } else f.tz = c - f.k0;
+  return f;
+};
+
+// row and slab strides and base index of the face grid of comp
+FluidSolver.prototype._faceIndex = function (comp, f) {
+  var nx = this.nx, ny = this.ny;
+  if (comp === 0) return { sj: nx + 1, sk: ny * (nx + 1), idx: (f.k0 * ny + f.j0) * (nx + 1) + f.i0 };
+  if (comp === 1) return { sj: nx, sk: (ny + 1) * nx, idx: (f.k0 * (ny + 1) + f.j0) * nx + f.i0 };
+  return { sj: nx, sk: ny * nx, idx: (f.k0 * ny + f.j0) * nx + f.i0 };
+};
+
+// the trilinear sum of solver.js:777-782 / :800-805 (x, then y, then z) in
+// double over any array of the face grid's layout
+function trilinear(arr, o, f) {
+  var s0 = 1 - f.tx, s1 = f.tx, t0 = 1 - f.ty, t1 = f.ty, r0 = 1 - f.tz, r1 = f.tz;
+  var i = o.idx, a = i + o.sj, b = i + o.sk, d = b + o.sj;
+  return ((arr[i] * s0 + arr[i + 1] * s1) * t0 + (arr[a] * s0 + arr[a + 1] * s1) * t1) * r0 +
+    ((arr[b] * s0 + arr[b + 1] * s1) * t0 + (arr[d] * s0 + arr[d + 1] * s1) * t1) * r1;
+}
+
+// _sampleFace, solver.js:750-783 (host arithmetic, any caller array)
+FluidSolver.prototype._sampleFace = function (arr, comp, x, y, z) {
+  var f = this._faceCoords(comp, x, y, z);
+  return trilinear(arr, this._faceIndex(comp, f), f);
+};
+
+// _sampleFacePair, solver.js:788-809: two grids at one position
+var _pair = [0, 0];
+FluidSolver.prototype._sampleFacePair = function (arrA, arrB, comp, x, y, z) {
+  var f = this._faceCoords(comp, x, y, z), o = this._faceIndex(comp, f);
+  _pair[0] = trilinear(arrA, o, f);
+  _pair[1] = trilinear(arrB, o, f);
+  return _pair;
+};
+
+// _markFluidCells(p0, p1), solver.js:945-1000, on the device
+FluidSolver.prototype._markFluidCells = function (p0, p1) {
+  var a = p0 === undefined ? 0 : p0, b = p1 === undefined ? this._nP : p1;
+  this._call(function (L, h) { L.markFluid(h, a, b); });
+};
+
+// _buildPlanetGravity, solver.js:1177-1210: the unit-g radial geometry for
+// the current ocean radius (the gravity slider scales it when applied)
+FluidSolver.prototype._buildPlanetGravity = function () {
+  var g = lib().planetGravity(this.nx, this.ny, this.nz, this.dx, this.oceanR);
+  this._gu = g.subarray(0, this.uN);
+  this._gv = g.subarray(this.uN, this.uN + this.vN);
+  this._gw = g.subarray(this.uN + this.vN);
+  this._ensure();
+  var L = lib(), h = this._h;
+  L.upload(h, 'gu', this._gu); L.upload(h, 'gv', this._gv); L.upload(h, 'gw', this._gw);
+};
+
+// dryLandFraction, solver.js:3593-3622: share of 1200 spiral directions
+// whose rock stands above sea level with no fluid cell on it (host, over
+// the terrain and the cell types)
+FluidSolver.prototype.dryLandFraction = function () {
+  if (this.mode !== 'sphere' || !this.terrain) return 0;
+  var cx = this.cx, cy = this.cy, cz = this.cz, dx = this.dx;
+  var type = this.cellType, nx = this.nx, ny = this.ny, nz = this.nz;
+  var t = this.terrain, n = t.n, dv = t.dv, solid = t.solid;
+  var N = 1200, dry = 0;
+  for (var i = 0; i < N; i++) {
+    var ang = i * 2.399963, u = 1 - 2 * (i + 0.5) / N, sq = Math.sqrt(1 - u * u);
+    var lx = sq * Math.cos(ang), ly = u, lz = sq * Math.sin(ang);
+    var top = -1;
+    for (var r = t.Rhi + dv; r > t.Rlo - dv; r -= dv * 0.5) {
+      var sx = (cx + lx * r) / dv | 0, sy = (cy + ly * r) / dv | 0, sz = (cz + lz * r) / dv | 0;
+      if (sx < 0 || sy < 0 || sz < 0 || sx >= n || sy >= n || sz >= n) continue;
+      if (solid[(sz * n + sy) * n + sx]) { top = r; break; }
+    }
+    if (top < 0 || top <= this.oceanR) continue;
+    var wet = false;
+    for (var k = 1; k <= 4; k++) {
+      var rp = top + k * dx * 0.55;
+      var ii = (cx + lx * rp) / dx | 0, jj = (cy + ly * rp) / dx | 0, kk = (cz + lz * rp) / dx | 0;
+      if (ii < 0 || jj < 0 || kk < 0 || ii >= nx || jj >= ny || kk >= nz) break;
+      var ct = type[(kk * ny + jj) * nx + ii];
+      if (ct === SOLID) continue;
+      if (ct === FLUID) wet = true;
+      break;
+    }
+    if (!wet) dry++;
+  }
+  return dry / N;
+};
+
+// waveImpulse(dvx), solver.js:3866-3888 (planet branch): a tidal bulge,
+// water pushed radially outward around a random direction (host, over the
+// particle mirrors; uploaded with the next device call)
+FluidSolver.prototype.waveImpulse = function (dvx) {
+  var px = this.px, py = this.py, pz = this.pz, vx = this.pvx, vy = this.pvy, vz = this.pvz;
+  var th = Math.random() * Math.PI * 2, ph = Math.acos(2 * Math.random() - 1);
+  var bx = Math.sin(ph) * Math.cos(th), by = Math.cos(ph), bz = Math.sin(ph) * Math.sin(th);
+  for (var p = 0; p < this._nP; p++) {
+    var ex = px[p] - this.cx, ey = py[p] - this.cy, ez = pz[p] - this.cz;
+    var er = Math.sqrt(ex * ex + ey * ey + ez * ez) || 1e-9;
+    var pr = (ex * bx + ey * by + ez * bz) / er;
+    if (pr <= 0) continue;
+    var f = pr * pr * pr * (0.7 + Math.random() * 0.3);
+    vx[p] += ex / er * dvx * f;
+    vy[p] += ey / er * dvx * f;
+    vz[p] += ez / er * dvx * f;
   }
 };
 

@@ -28,7 +28,7 @@ EXPORTS = ["ws_create", "ws_destroy", "ws_last_error", "ws_set_f", "ws_get_f", "
            "ws_net_param_count", "ws_net_set_params", "ws_net_get_params", "ws_net_forward", "ws_net_grad",
            "ws_net_bank", "ws_net_bank_count", "ws_net_train", "ws_set_balls", "ws_get_balls", "ws_metaballs_build", "ws_read_render", "ws_tune",
            "ws_batch_create", "ws_batch_run", "ws_batch_set", "ws_batch_get", "ws_batch_last_error",
-           "ws_batch_destroy", "ws_mt_build_noalloc"]
+           "ws_batch_destroy", "ws_mt_build_noalloc", "ws_mark_fluid", "ws_sample_vel", "ws_planet_gravity"]
 
 
 class RenderView(ctypes.Structure):
@@ -90,6 +90,9 @@ def load(path=LIB_PATH):
                               ctypes.POINTER(d)]
     L.ws_mt_build_noalloc.argtypes = [vp, vp, ci, ci, ci, d, d, vp, vp, ctypes.c_long, ctypes.POINTER(ctypes.c_long),
                                       ctypes.POINTER(d)]
+    L.ws_mark_fluid.argtypes = [vp, ci, ci]
+    L.ws_sample_vel.argtypes = [vp, vp, ctypes.c_long, vp]
+    L.ws_planet_gravity.argtypes = [ci, ci, ci, d, d, vp]
     L.ws_autotune_iso.argtypes = [vp, ctypes.POINTER(d)]
     L.ws_kinetic_energy.argtypes = [vp, ctypes.POINTER(d)]
     L.ws_sync.argtypes = [vp]
@@ -392,6 +395,17 @@ class Solver:
                                                ctypes.byref(cnt), ctypes.byref(vol)), "mt_build_noalloc")
         return cnt.value, vol.value
 
+    def mark_fluid(self, p0, p1):
+        self._check(self.L.ws_mark_fluid(self.ctx, p0, p1), "mark_fluid")
+
+    def sample_vel(self, pts):
+        """sampleVel at the rows of pts (n x 3) with the device sampler."""
+        xyz = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        out = np.empty_like(xyz)
+        self._check(self.L.ws_sample_vel(self.ctx, xyz.ctypes.data_as(ctypes.c_void_p), len(xyz),
+                                         out.ctypes.data_as(ctypes.c_void_p)), "sample_vel")
+        return out
+
     def autotune_iso(self):
         out = ctypes.c_double()
         self._check(self.L.ws_autotune_iso(self.ctx, ctypes.byref(out)), "autotune_iso")
@@ -475,6 +489,16 @@ def build_world(n, dx, coreR, depth, seed, target, bumpiness=1.0):
     finally:
         L.ws_world_free(ctypes.byref(w))
     return out
+
+
+def planet_gravity(n, dx, oceanR):
+    """ws_planet_gravity: (gu, gv, gw) of a cubic n^3 grid."""
+    L = load()
+    uN = (n + 1) * n * n
+    g = np.empty(3 * uN, np.float32)
+    if L.ws_planet_gravity(n, n, n, dx, oceanR, g.ctypes.data_as(ctypes.c_void_p)) != 0:
+        raise WatersimError("ws_planet_gravity failed")
+    return g[:uN], g[uN:2 * uN], g[2 * uN:]
 
 
 def solver_for_world(world, capacity=None, Tamb=0.32):
