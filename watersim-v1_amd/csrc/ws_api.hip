@@ -608,8 +608,8 @@ int st_extrapolate(ws_ctx *c) {
   for (size_t d = 0; d + 1 < c->exDiagOff.size(); d++) {
     int n = c->exDiagOff[d + 1] - c->exDiagOff[d];
     if (n > 0)
-      hipLaunchKernelGGL(k_extrap_diag, dim3(n), dim3(256), 0, c->stream, c->geo, c->exTiles, c->exDiagOff[d],
-                         c->vel, c->valid);
+      hipLaunchKernelGGL(k_extrap_diag, dim3(n), dim3(EX_THREADS), 0, c->stream, c->geo, c->exDiag[d], c->vel,
+                         c->valid);
   }
   WS_LAUNCHED();
   return WS_OK;
@@ -936,9 +936,19 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
     }
     std::vector<int4> flat;
     c->exDiagOff.push_back(0);
-    for (auto &v : byDiag) {
+    for (size_t d = 0; d < byDiag.size(); d++) {
+      const auto &v = byDiag[d];
       flat.insert(flat.end(), v.begin(), v.end());
       c->exDiagOff.push_back((int)flat.size());
+      ExDiag D{};
+      D.d = (int)d;
+      for (int comp = 0; comp < 3; comp++) {
+        D.ti[comp] = (dims[comp][0] + EX_TI - 1) / EX_TI;
+        D.tj[comp] = (dims[comp][1] + EX_TJ - 1) / EX_TJ;
+        D.tk[comp] = (dims[comp][2] + EX_TK - 1) / EX_TK;
+      }
+      for (const int4 &t : v) D.cnt[t.x]++;
+      c->exDiag.push_back(D);
     }
     if (hipMalloc(&c->exTiles, flat.size() * sizeof(int4)) != hipSuccess) { ws_destroy(c); return WS_EHIP; }
     if (hipMemcpy(c->exTiles, flat.data(), flat.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess) {

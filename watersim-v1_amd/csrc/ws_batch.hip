@@ -158,7 +158,7 @@ int enqueue_substep(ws_batch *b) {
   for (size_t d = 0; d + 1 < b->exDiagOff.size(); d++) {
     int nt = b->exDiagOff[d + 1] - b->exDiagOff[d];
     if (nt > 0)
-      hipLaunchKernelGGL(k_extrap_diag_batch, dim3(nt), dim3(256), 0, b->stream, b->exPlanets, b->exTiles,
+      hipLaunchKernelGGL(k_extrap_diag_batch, dim3(nt), dim3(EX_THREADS), 0, b->stream, b->exPlanets, b->exTiles,
                          b->exPlanetOf, b->exDiagOff[d]);
   }
   BH(hipGetLastError());

@@ -1411,153 +1411,250 @@ __global__ void k_vort_splat(Geo g, const uint8_t *__restrict__ ct, const float 
 // invalid face takes the mean of its valid neighbours, where a face filled
 // earlier in the same pass counts as valid. A face therefore depends on the
 // final state of its -i/-j/-k neighbours and on the original state of its
-// +i/+j/+k neighbours. Tiles of 16^3 faces are processed one tile-diagonal
-// (I+J+K) per launch. Inside a tile the workgroup walks the hyperplanes
-// li+lj+lk: lane (lj, lk) of 256 handles face li = step - lj - lk, so the
-// -j / -k neighbours it needs were decided by lanes one step earlier and
-// arrive by lane shuffles (within a wavefront's four k-planes) or through
-// LDS (from the previous wavefront's last plane), one barrier per step. The
-// tile (with a one-face halo) is first loaded into LDS.
+// +i/+j/+k neighbours. Tiles of 64 x 8 x 8 faces are processed one tile
+// diagonal (I+J+K) per launch; the tile and a one-face halo (the -i/-j/-k
+// side already final from earlier diagonals, the +i/+j/+k side still
+// original) are loaded into LDS.
+//
+// Validity does not depend on the values: a face ends valid iff some face
+// y <= x (componentwise, inside the tile, or a final halo face just below
+// it) was valid or had a +i/+j/+k neighbour that was valid originally. So
+// the final flags of the whole tile are a 3-D prefix OR, computed up front
+// from 64-bit row masks (a ballot and a lowest-set-bit fill along i, an OR
+// down j in registers and down k through LDS). With every flag known, the
+// value of a filled face is an affine function of its -i, -j and -k
+// neighbours' final values, x = r (x(-i) + x(-j) + x(-k) [those valid] +
+// the sum of the rest), r the reciprocal of the count; every coefficient
+// is computed before the recurrence starts.
+//
+// The recurrence runs over rows along i by diagonals lj + lk, one wavefront
+// per k-plane and one lane per face: at step s wavefront lk takes row
+// lj = s - lk, whose -j row it finished itself one step earlier (kept in a
+// register) and whose -k row the wavefront below finished one step earlier
+// (handed over in LDS). Along the row the -i dependence is an inclusive scan
+// of the affine maps x -> A x + B across the wavefront (DPP row shifts and
+// row broadcasts); the multiplicative half of that scan is value-free and
+// precomputed, so a step is two FMAs, six DPP FMAs, an LDS read and write,
+// and a barrier. The flags are those of the serial flood exactly; the values
+// differ from its sequential double sums by rounding (a few float ulps).
+__device__ __forceinline__ float ex_recip(int c) {
+  // 1/c, correctly rounded, for the neighbour counts 1..6
+  float r = 1.f;
+  r = c == 2 ? 0.5f : r;
+  r = c == 3 ? 1.f / 3.f : r;
+  r = c == 4 ? 0.25f : r;
+  r = c == 5 ? 0.2f : r;
+  r = c == 6 ? 1.f / 6.f : r;
+  return r;
+}
+
+// lane-shifted copy of v (DPP): lanes with no source lane get `old`
+#define EX_DPP(v, CTRL, RMASK, OLD) \
+  __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(OLD), __float_as_int(v), CTRL, RMASK, 0xf, false))
+// B += a * (B of the source lane), one DPP FMA; a lane with no source lane,
+// or in a row the mask leaves out, is not written (keeps B). The s_nop
+// gives the two wait states a DPP read needs after B's last VALU write.
+#define EX_FMAC_DPP(B, a, CTRL) \
+  asm("s_nop 1\n\tv_fmac_f32_dpp %0, %1, %2 " CTRL " bank_mask:0xf" : "+v"(B) : "v"(B), "v"(a))
+
 __device__ __forceinline__ void extrap_tile(const Geo &g, int4 t, float *__restrict__ vel,
                                             uint8_t *__restrict__ valid) {
   constexpr int TI = EX_TI, TJ = EX_TJ, TK = EX_TK;
-  static_assert(TJ == 16 && TJ * TK == 256, "one lane per (j, k) row of a tile, 16-lane rows in j");
+  static_assert(TI == 64 && EX_THREADS == 64 * TK, "one lane per face of a row, one wavefront per k-plane");
   constexpr int HI = TI + 2, HJ = TJ + 2, HK = TK + 2, HN = HI * HJ * HK;
   __shared__ float sv[HK][HJ][HI];
   __shared__ uint8_t sf[HK][HJ][HI];
-  int comp = t.x, I = t.y, J = t.z, K = t.w;
-  int d0 = comp == 0 ? g.nx + 1 : g.nx, d1 = comp == 1 ? g.ny + 1 : g.ny, d2 = comp == 2 ? g.nz + 1 : g.nz;
-  long base = comp == 0 ? 0 : (comp == 1 ? g.uN : (long)g.uN + g.vN);
-  float *A = vel + base;
-  uint8_t *V = valid + base;
-  int i0 = I * TI - 1, j0 = J * TJ - 1, k0 = K * TK - 1;
-  constexpr int B = (HN + 255) / 256;
+  __shared__ unsigned long long vmj[TK][TJ];  // row masks, prefix OR along i and j
+  __shared__ unsigned long long vmk[TK][TJ];  // ... and along k: the final flags
+  __shared__ float xfin[TK][2][64];           // a finished row, for the wavefront above
+  const int comp = t.x, I = t.y, J = t.z, K = t.w;
+  const int d0 = comp == 0 ? g.nx + 1 : g.nx, d1 = comp == 1 ? g.ny + 1 : g.ny, d2 = comp == 2 ? g.nz + 1 : g.nz;
+  const long base = comp == 0 ? 0 : (comp == 1 ? g.uN : (long)g.uN + g.vN);
+  float *Av = vel + base;
+  uint8_t *Vf = valid + base;
+  const int i0 = I * TI - 1, j0 = J * TJ - 1, k0 = K * TK - 1;
   {
-    float val[B];
-    uint8_t fl[B];
+    // every load of the tile first (one round trip), then the LDS stores
+    constexpr int NB = (HN + EX_THREADS - 1) / EX_THREADS;
+    float val[NB];
+    int fl[NB];
 #pragma unroll
-    for (int b = 0; b < B; b++) {
-      int e = b * 256 + threadIdx.x;
-      int li = e % HI, lj = (e / HI) % HJ, lk = e / (HI * HJ);
-      int i = i0 + li, j = j0 + lj, k = k0 + lk;
-      bool in = e < HN && i >= 0 && i < d0 && j >= 0 && j < d1 && k >= 0 && k < d2;
-      long f = in ? ((long)k * d1 + j) * d0 + i : 0;
-      val[b] = in ? A[f] : 0.f;
-      fl[b] = in ? V[f] : 0;
+    for (int b = 0; b < NB; b++) {
+      const int e = b * EX_THREADS + threadIdx.x;
+      const int li = e % HI, lj = (e / HI) % HJ, lk = e / (HI * HJ);
+      const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+      const bool in = e < HN && i >= 0 && i < d0 && j >= 0 && j < d1 && k >= 0 && k < d2;
+      // unconditional loads (face 0 stands in for a halo face outside the
+      // grid) so that the compiler issues them together
+      const long f = in ? ((long)k * d1 + j) * d0 + i : 0;
+      const float v = Av[f];
+      const int q = Vf[f];
+      val[b] = in ? v : 0.f;
+      fl[b] = in ? q : 0;
     }
-    int valid = 0, open = 0;
+    int any = 0, open = 0;
 #pragma unroll
-    for (int b = 0; b < B; b++) {
-      int e = b * 256 + threadIdx.x;
+    for (int b = 0; b < NB; b++) {
+      const int e = b * EX_THREADS + threadIdx.x;
       if (e < HN) {
+        const int li = e % HI, lj = (e / HI) % HJ, lk = e / (HI * HJ);
+        const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+        const bool in = i >= 0 && i < d0 && j >= 0 && j < d1 && k >= 0 && k < d2;
+        const bool own = li >= 1 && li <= TI && lj >= 1 && lj <= TJ && lk >= 1 && lk <= TK;
+        const int f = fl[b] ? 1 : 0;
         (&sv[0][0][0])[e] = val[b];
-        (&sf[0][0][0])[e] = fl[b] ? 1 : 0;
-        int li = e % HI, lj = (e / HI) % HJ, lk = e / (HI * HJ);
-        int i = i0 + li, j = j0 + lj, k = k0 + lk;
-        bool in = i >= 0 && i < d0 && j >= 0 && j < d1 && k >= 0 && k < d2;
-        bool own = li >= 1 && li <= TI && lj >= 1 && lj <= TJ && lk >= 1 && lk <= TK;
-        valid |= (in && fl[b]) ? 1 : 0;
-        open |= (in && own && !fl[b]) ? 1 : 0;
+        (&sf[0][0][0])[e] = (uint8_t)f;
+        any |= f;
+        open |= (in && own && !f) ? 1 : 0;
       }
     }
     // nothing to fill (every owned face valid), or nothing to fill from (no
-    // valid face in the tile or its halo, whose -i/-j/-k side already holds
-    // the earlier diagonals' results): the flood leaves the tile as it is
-    valid = __syncthreads_or(valid);
+    // valid face in the tile or its halo): the flood leaves the tile as it is
+    any = __syncthreads_or(any);
     open = __syncthreads_or(open);
-    if (!valid || !open) return;
+    if (!any || !open) return;
   }
-  // the -k neighbours of a wavefront's first plane are the previous
-  // wavefront's last plane: it leaves them here, double-buffered by step
-  __shared__ float xv[2][4][TJ];
-  __shared__ int xf[2][4][TJ];
-  const int lane = threadIdx.x, wv = lane >> 6, l64 = lane & 63;
-  const int lj = 1 + (lane % TJ), lk = 1 + (lane / TJ);
-  const int lkw = (lk - 1) & 3;  // plane within the wavefront
-  const int j = j0 + lj, k = k0 + lk;
-  const bool row = j < d1 && k < d2;
-  // the LDS copy stays the original state; a face's final state travels in
-  // registers: a lane keeps what it decided at the previous step (its -i
-  // neighbour now) and takes the -j / -k neighbours from the lanes one row
-  // and one plane back, which decided the same li one step earlier (-j by a
-  // DPP row shift, -k by a lane permute within the wavefront or from the
-  // previous wavefront's plane in LDS; the flags of a wavefront by a
-  // ballot). Every LDS read is unconditional (clamped index); the four
-  // wavefronts step together, one barrier per step.
-  float lastV = 0.f;
-  int lastF = 0;
-  const float vaH = sv[lk][lj][0];
-  const int faH = sf[lk][lj][0];
-  const bool hasJ = lj > 1, hasK = lk > 1;
-  const bool jlo = j > 0, jhi = j < d1 - 1, klo = k > 0, khi = k < d2 - 1;
-  const long rowBase = ((long)k * d1 + j) * d0;
-  for (int step = 0; step < TI + TJ + TK - 2; step++) {
-    int li = 1 + step - (lj - 1) - (lk - 1);
-    int i = i0 + li;
-    // lane - 1 (same 16-lane row whenever lj > 1) and lane - TJ
-    float vjm = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(lastV), 0x111, 0xf, 0xf, false));
-    float vkw = __shfl_up(lastV, TJ);
-    unsigned long long fm = __ballot(lastF);
-    int fjm = (int)((fm >> (l64 > 0 ? l64 - 1 : 0)) & 1ull);
-    int fkw = (int)((fm >> (l64 >= TJ ? l64 - TJ : 0)) & 1ull);
-    const int rb = (step + 1) & 1;  // the buffer written at the previous step
-    float vkm = lkw > 0 ? vkw : xv[rb][wv > 0 ? wv - 1 : 0][lj - 1];
-    int fkm = lkw > 0 ? fkw : xf[rb][wv > 0 ? wv - 1 : 0][lj - 1];
-    bool act = row && li >= 1 && li <= TI && i < d0;
-    int lc = li < 1 ? 1 : (li > TI ? TI : li);
-    int f0 = sf[lk][lj][lc], fb = sf[lk][lj][lc + 1], fd = sf[lk][lj + 1][lc], ff = sf[lk + 1][lj][lc];
-    float v0 = sv[lk][lj][lc], vb = sv[lk][lj][lc + 1], vd = sv[lk][lj + 1][lc], vf = sv[lk + 1][lj][lc];
-    float vcH = sv[lk][0][lc], veH = sv[0][lj][lc];
-    int fcH = sf[lk][0][lc], feH = sf[0][lj][lc];
-    float va = li > 1 ? lastV : vaH;
-    int fa = li > 1 ? lastF : faH;
-    float vc = hasJ ? vjm : vcH;
-    int fc = hasJ ? fjm : fcH;
-    float ve = hasK ? vkm : veH;
-    int fe = hasK ? fkm : feH;
-    bool ua = i > 0 && fa, ub = i < d0 - 1 && fb, uc = jlo && fc;
-    bool ud = jhi && fd, ue = klo && fe, uf = khi && ff;
-    float sum = 0.f;
-    sum = ua ? sum + va : sum;
-    sum = ub ? sum + vb : sum;
-    sum = uc ? sum + vc : sum;
-    sum = ud ? sum + vd : sum;
-    sum = ue ? sum + ve : sum;
-    sum = uf ? sum + vf : sum;
-    int cnt = (int)ua + (int)ub + (int)uc + (int)ud + (int)ue + (int)uf;
-    bool fill = act && !f0 && cnt > 0;
-    // the mean through the correctly rounded reciprocal of the count (the
-    // product is within 1.5 ulp of the quotient; the reference divides in
-    // double and rounds, so neither is exact here). The reciprocal is
-    // computed, not looked up: an indexed constant array would be a global
-    // load on every step of the dependency chain.
-    float v = sum * (1.f / (float)(cnt > 0 ? cnt : 1));
-    if (fill) {
-      A[rowBase + i] = v;
-      V[rowBase + i] = 1;
+  const int lane = threadIdx.x & 63;
+  const int lk = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // this wavefront's k-plane
+  const int i = i0 + 1 + lane, k = k0 + 1 + lk;
+  const int si = lane + 1, sk = lk + 1;
+  const int nI = d0 - (i0 + 1);  // faces of the tile's rows inside the grid, >= 1
+  const unsigned long long inMask = nI >= 64 ? ~0ull : ((1ull << nI) - 1ull);
+
+  // (1) the final flags. A seed is a face inside the grid that is valid or
+  // has a valid +i/+j/+k neighbour, or lies next to a valid -i/-j/-k halo face
+  {
+    unsigned long long pj = 0;
+#pragma unroll
+    for (int lj = 0; lj < TJ; lj++) {
+      const int sj = lj + 1;
+      const bool in = k < d2 && j0 + 1 + lj < d1 && lane < nI;
+      int s = sf[sk][sj][si] | sf[sk][sj][si + 1] | sf[sk][sj + 1][si] | sf[sk + 1][sj][si];
+      s |= lane == 0 ? sf[sk][sj][0] : 0;
+      s |= lj == 0 ? sf[sk][0][si] : 0;
+      s |= lk == 0 ? sf[0][sj][si] : 0;
+      const unsigned long long m = __ballot(in && s);
+      pj |= m ? (~0ull << __builtin_ctzll(m)) : 0ull;
+      if (lane == 0) vmj[lk][lj] = pj;
     }
-    if (act) {
-      lastV = fill ? v : v0;
-      lastF = fill ? 1 : (f0 ? 1 : 0);
+  }
+  __syncthreads();
+  if (threadIdx.x < TK * TJ) {
+    const int kk = threadIdx.x / TJ, jj = threadIdx.x % TJ;
+    unsigned long long a = 0;
+    for (int q = 0; q <= kk; q++) a |= vmj[q][jj];
+    const bool rin = k0 + 1 + kk < d2 && j0 + 1 + jj < d1;
+    vmk[kk][jj] = rin ? (a & inMask) : 0ull;
+  }
+  __syncthreads();
+
+  // (2) per face: x = A x(-i) + Pj x(-j) + Pk x(-k) + Q, and the scan
+  // multipliers of A. Faces that stay as they are (valid, or left invalid)
+  // have A = Pj = Pk = 0 and Q their value (0 for a face left invalid, so
+  // no value of an invalid face reaches a product)
+  float Q[TJ], Pj[TJ], Pk[TJ], As[TJ][6];
+  unsigned fillBits = 0;
+  {
+    unsigned long long prevRow = 0;
+#pragma unroll
+    for (int lj = 0; lj < TJ; lj++) {
+      const int sj = lj + 1;
+      const unsigned long long Vm = vmk[lk][lj];
+      const unsigned long long Vkm = lk > 0 ? vmk[lk > 0 ? lk - 1 : 0][lj] : 0ull;
+      const bool V = (Vm >> lane) & 1ull;
+      const int f0 = sf[sk][sj][si];
+      const int fb = sf[sk][sj][si + 1], fd = sf[sk][sj + 1][si], ff = sf[sk + 1][sj][si];
+      const float v0 = sv[sk][sj][si], vb = sv[sk][sj][si + 1], vd = sv[sk][sj + 1][si], vf = sv[sk + 1][sj][si];
+      const int fh = sf[sk][sj][0], fjh = sf[sk][0][si], fkh = sf[0][sj][si];
+      const float vh = sv[sk][sj][0], vjh = sv[sk][0][si], vkh = sv[0][sj][si];
+      // the -i / -j / -k neighbours' final flags
+      const int ua = lane > 0 ? (int)((Vm >> (lane > 0 ? lane - 1 : 0)) & 1ull) : fh;
+      const int uc = lj > 0 ? (int)((prevRow >> lane) & 1ull) : fjh;
+      const int ue = lk > 0 ? (int)((Vkm >> lane) & 1ull) : fkh;
+      const int cnt = ua + fb + uc + fd + ue + ff;
+      const bool fill = V && !f0;
+      const float r = ex_recip(cnt);
+      // the neighbours whose values are known now, in the reference's order
+      // (-i, +i, -j, +j, -k, +k); the halo faces among them are final
+      float R = 0.f;
+      R = (lane == 0 && ua) ? R + vh : R;
+      R = fb ? R + vb : R;
+      R = (lj == 0 && uc) ? R + vjh : R;
+      R = fd ? R + vd : R;
+      R = (lk == 0 && ue) ? R + vkh : R;
+      R = ff ? R + vf : R;
+      Q[lj] = fill ? R * r : (V ? v0 : 0.f);
+      Pj[lj] = (fill && lj > 0 && uc) ? r : 0.f;
+      Pk[lj] = (fill && lk > 0 && ue) ? r : 0.f;
+      float A = (fill && lane > 0 && ua) ? r : 0.f;
+      As[lj][0] = A;
+      A *= EX_DPP(A, 0x111, 0xf, 1.f);  // row_shr:1
+      As[lj][1] = A;
+      A *= EX_DPP(A, 0x112, 0xf, 1.f);  // row_shr:2
+      As[lj][2] = A;
+      A *= EX_DPP(A, 0x114, 0xf, 1.f);  // row_shr:4
+      As[lj][3] = A;
+      A *= EX_DPP(A, 0x118, 0xf, 1.f);  // row_shr:8
+      As[lj][4] = A;
+      A *= EX_DPP(A, 0x142, 0xa, 1.f);  // row_bcast:15 into rows 1 and 3
+      As[lj][5] = A;
+      if (fill) {
+        fillBits |= 1u << lj;
+        Vf[((long)k * d1 + (j0 + 1 + lj)) * d0 + i] = 1;
+      }
+      prevRow = Vm;
     }
-    if (lkw == 3) {
-      xv[step & 1][wv][lj - 1] = lastV;
-      xf[step & 1][wv][lj - 1] = lastF;
-    }
+  }
+
+  // (3) the recurrence: TJ + TK - 1 steps, one barrier each; wavefront lk
+  // is busy at steps lk .. lk + TJ - 1
+  for (int q = 0; q < lk; q++) __syncthreads();
+  float xj = 0.f;
+#pragma unroll
+  for (int lj = 0; lj < TJ; lj++) {
+    const float xk = lk > 0 ? xfin[lk > 0 ? lk - 1 : 0][lj & 1][lane] : 0.f;
+    float B = fmaf(Pk[lj], xk, fmaf(Pj[lj], xj, Q[lj]));
+    EX_FMAC_DPP(B, As[lj][0], "row_shr:1 row_mask:0xf");
+    EX_FMAC_DPP(B, As[lj][1], "row_shr:2 row_mask:0xf");
+    EX_FMAC_DPP(B, As[lj][2], "row_shr:4 row_mask:0xf");
+    EX_FMAC_DPP(B, As[lj][3], "row_shr:8 row_mask:0xf");
+    EX_FMAC_DPP(B, As[lj][4], "row_bcast:15 row_mask:0xa");  // into rows 1 and 3
+    EX_FMAC_DPP(B, As[lj][5], "row_bcast:31 row_mask:0xc");  // into rows 2 and 3
+    if (lk < TK - 1) xfin[lk][lj & 1][lane] = B;
+    if (fillBits & (1u << lj)) Av[((long)k * d1 + (j0 + 1 + lj)) * d0 + i] = B;
+    xj = B;
     __syncthreads();
   }
+  for (int q = lk; q < TK - 1; q++) __syncthreads();
 }
+#undef EX_DPP
+#undef EX_FMAC_DPP
 
 // one launch per tile diagonal of one context
-__global__ __launch_bounds__(256) void k_extrap_diag(Geo g, const int4 *__restrict__ tiles, int off,
-                                                     float *__restrict__ vel, uint8_t *__restrict__ valid) {
-  extrap_tile(g, tiles[off + blockIdx.x], vel, valid);
+// the b-th tile of a diagonal (scalar arithmetic: no table load ahead of
+// the tile's own loads)
+__device__ __forceinline__ int4 ex_tile_of(const ExDiag &D, int b) {
+  int comp = 0;
+  while (comp < 2 && b >= D.cnt[comp]) b -= D.cnt[comp++];
+  const int ti = D.ti[comp], tj = D.tj[comp], tk = D.tk[comp];
+  for (int K = 0; K < tk; K++) {
+    const int jlo = max(0, D.d - K - (ti - 1)), jhi = min(tj - 1, D.d - K);
+    const int n = jhi >= jlo ? jhi - jlo + 1 : 0;
+    if (b < n) return make_int4(comp, D.d - K - (jlo + b), jlo + b, K);
+    b -= n;
+  }
+  return make_int4(comp, 0, 0, 0);  // not reached: b < the diagonal's count
+}
+
+__global__ __launch_bounds__(EX_THREADS) void k_extrap_diag(Geo g, ExDiag D, float *__restrict__ vel,
+                                                     uint8_t *__restrict__ valid) {
+  extrap_tile(g, ex_tile_of(D, __builtin_amdgcn_readfirstlane(blockIdx.x)), vel, valid);
 }
 
 // one launch per tile diagonal of many planets: block b takes tile b of the
 // merged schedule, on planet planetOf[b] (planets are independent, and a
 // planet's tiles of one diagonal depend only on its earlier diagonals)
-__global__ __launch_bounds__(256) void k_extrap_diag_batch(const ExPlanet *__restrict__ planets,
+__global__ __launch_bounds__(EX_THREADS) void k_extrap_diag_batch(const ExPlanet *__restrict__ planets,
                                                            const int4 *__restrict__ tiles,
                                                            const int *__restrict__ planetOf, int off) {
   const int b = off + blockIdx.x;

@@ -23,7 +23,10 @@ constexpr float kFixInvF = 1.0f / 1125899906842624.0f;
 
 // Tile shapes (cells) for P2G and the extrapolation wavefront.
 constexpr int P2G_T = 8;
-constexpr int EX_TI = 16, EX_TJ = 16, EX_TK = 16;  // extrapolation tile (256 lanes, one per (j, k))
+// extrapolation tile: 64 faces along i (one lane each), 8 x 8 rows in j and
+// k; one wavefront per k-plane of the tile
+constexpr int EX_TI = 64, EX_TJ = 8, EX_TK = 8;
+constexpr int EX_THREADS = 64 * EX_TK;
 constexpr int SOR_TX = 32, SOR_TY = 8, SOR_TZ = 8;
 constexpr int SOR_TB_X = 16, SOR_TB_Y = 8, SOR_TB_Z = 8;  // temporal-blocking tile
 
@@ -93,6 +96,15 @@ struct SorJob {
   // progress counter per workgroup, kFlowStride words apart
   const unsigned long long *off;
   unsigned *flow;
+};
+
+// one tile diagonal d = I + J + K of the extrapolation wavefront: its tiles
+// are those of component 0, then 1, then 2 (cnt of each), each in the order
+// K, then J ascending (I = d - J - K)
+struct ExDiag {
+  int d;
+  int cnt[3];
+  int ti[3], tj[3], tk[3];  // tiles per axis of each component
 };
 
 // a planet's face fields for the batched extrapolation wavefront
@@ -253,6 +265,7 @@ struct Ctx {
   int4 *exTiles = nullptr;
   std::vector<int4> exTilesHost;  // the same schedule on the host (batches merge the planets')
   std::vector<int> exDiagOff;  // tiles of diagonal d: [off[d], off[d+1])
+  std::vector<ExDiag> exDiag;  // the same diagonals, for tiles found from the block index
   // marching tets scratch
   float *smpBuf = nullptr;  // sampleVel points and results, 6 floats per point
   long smpCap = 0;

@@ -96,7 +96,7 @@ __global__ void k_vort_omega(Geo g, const float *vc, float *om, float inv2);
 __global__ void k_vort_force(Geo g, const uint8_t *ct, const float *om, float *fc, float eps,
                              const DevScalars *s);
 __global__ void k_vort_splat(Geo g, const uint8_t *ct, const float *fc, float *vel);
-__global__ void k_extrap_diag(Geo g, const int4 *tiles, int off, float *vel, uint8_t *valid);
+__global__ void k_extrap_diag(Geo g, ExDiag D, float *vel, uint8_t *valid);
 __global__ void k_extrap_diag_batch(const ExPlanet *planets, const int4 *tiles, const int *planetOf, int off);
 
 __global__ void k_g2p_advect(Geo g, World w, Par P, int nP, int mode, const float *vel, const float *velO,
