@@ -16,6 +16,8 @@ CPU oracle, plus the particle-to-cell index at float boundaries.
   both sides of every cell boundary along each axis (the smallest float
   whose double quotient floors to i, and the float just below it), where a
   float division would round the wrong way.
+- cfg3 with the temporally blocked SOR (two sweeps per launch) gives the
+  bits of the fluid-list sweeps.
 - cfg4: the shipped network weights on the held-out benchmark seed: the
   learned start reaches residual 1e-4 in fewer sweeps than the previous
   pressure on the same systems; the time ratio is printed.
@@ -175,6 +177,27 @@ def test_learned_warm_start_gain():
     assert ns >= 6
     assert g["cmpSweepsLearned"] < g["cmpSweepsPlain"]
     s.close()
+
+
+def test_temporal_blocking_equals_list_sweeps_at_128():
+    """The two-sweeps-per-launch temporally blocked SOR (sorMode 3: 32x16x16
+    tiles, per-tile phase lists) against the fluid-list kernel on the cfg3
+    scene at its own size: every substep's solve to 1e-4 (an odd sweep
+    count ends with a one-sweep launch), the same bits in q and every
+    face and particle."""
+    cfg = bench.CONFIGS["cfg3"]
+    out = []
+    for mode in (2, 3):
+        s, world = bench.make_solver(cfg)
+        s.set(sorMode=mode)
+        s.run_substeps(4)
+        out.append({k: s.download(k).copy() for k in ("q", "u", "v", "w", "px", "pvy")})
+        out[-1]["sweeps"] = s.get("sweepsTotal")
+        s.close()
+    a, b = out
+    assert a["sweeps"] == b["sweeps"] and a["sweeps"] > 0
+    for k in ("q", "u", "v", "w", "px", "pvy"):
+        assert np.array_equal(a[k], b[k]), k
 
 
 def _planet(n, seed):

@@ -263,12 +263,14 @@ int sor_sweeps(ws_ctx *c, int n) {
     if (c->sorMode == 3) {
       int nTB = c->tbTiles[0] * c->tbTiles[1] * c->tbTiles[2];
       if (it + 1 < n) {
-        hipLaunchKernelGGL(k_sor_tb<2>, dim3(nTB), dim3(256), 0, c->stream, c->geo, c->tbTiles[0], c->tbTiles[1],
-                           c->tbActive, c->q, c->q2, c->rhs, c->pw, c->nbm, om1);
+        hipLaunchKernelGGL(k_sor_tb<2>, dim3(nTB), dim3(SOR_TB_THREADS), 0, c->stream, c->geo, c->tbTiles[0],
+                           c->tbTiles[1], c->tbActive, c->q, c->q2, c->tbPk, c->tbRhs, c->tbCnt, (float)c->omega,
+                           om1);
         it++;
       } else {
-        hipLaunchKernelGGL(k_sor_tb<1>, dim3(nTB), dim3(256), 0, c->stream, c->geo, c->tbTiles[0], c->tbTiles[1],
-                           c->tbActive, c->q, c->q2, c->rhs, c->pw, c->nbm, om1);
+        hipLaunchKernelGGL(k_sor_tb<1>, dim3(nTB), dim3(SOR_TB_THREADS), 0, c->stream, c->geo, c->tbTiles[0],
+                           c->tbTiles[1], c->tbActive, c->q, c->q2, c->tbPk, c->tbRhs, c->tbCnt, (float)c->omega,
+                           om1);
       }
       float *t = c->q; c->q = c->q2; c->q2 = t;
       c->tbLaunchesPending++;
@@ -489,8 +491,16 @@ int pressure_setup(ws_ctx *c) {
   }
   if (c->sorMode == 3) {
     int nTB = c->tbTiles[0] * c->tbTiles[1] * c->tbTiles[2];
+    if (!c->tbPk) {
+      int r;
+      if ((r = dalloc(c, &c->tbPk, (size_t)nTB * TB_CAP)) || (r = dalloc(c, &c->tbRhs, (size_t)nTB * TB_CAP)) ||
+          (r = dalloc(c, &c->tbCnt, (size_t)nTB * 4)))
+        return r;
+    }
     hipLaunchKernelGGL(k_sor_tb_mark, dim3(nTB), dim3(256), 0, c->stream, c->geo, c->tbTiles[0], c->tbTiles[1], nTB,
                        c->nbm, c->tbActive);
+    hipLaunchKernelGGL(k_sor_tb_build, dim3(nTB), dim3(SOR_TB_THREADS), 0, c->stream, c->geo, c->tbTiles[0],
+                       c->tbTiles[1], c->tbActive, c->nbm, c->dg, c->rhs, c->tbPk, c->tbRhs, c->tbCnt);
   }
   WS_LAUNCHED();
   int r;
@@ -970,7 +980,7 @@ void ws_destroy(ws_ctx *c) {
   balls_release(c);
   metaballs_release(c);
   void *ptrs[] = {c->staticTypes, c->gu, c->thr, c->terrainR, c->cellType, c->nbm, c->dg, c->q, c->q2, c->rhs, c->pw,
-                  c->sorActive, c->tbActive, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->sorChunk, c->sorArrive, c->sorFlow,
+                  c->sorActive, c->tbActive, c->tbPk, c->tbRhs, c->tbCnt, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->sorChunk, c->sorArrive, c->sorFlow,
                   c->cellCount, c->cellStart, c->vort, c->vel, c->velO, c->wgt, c->scr, c->scr2, c->valid, c->dens,
                   c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pT, c->pAir, c->advU, c->advV, c->advW, c->pflag,
                   c->pcool, c->pcell, c->prank, c->sortedIdx, c->vneed, c->vscr, c->rng, c->ds, c->partials,

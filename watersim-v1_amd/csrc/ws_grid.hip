@@ -682,110 +682,214 @@ __global__ void k_sor_tiles_mark(Geo g, int tilesX, int tilesY, int nTiles, cons
   if (threadIdx.x == 0) active[b] = (uint8_t)any;
 }
 
-// S red-black sweeps per launch over a 16x8x8 cell tile (temporal
-// blocking). The block loads q with a halo of 2S cells and the coefficients
-// with a halo of 2S-1 into LDS, then runs the 2S colour phases on shrinking
-// regions: the red phase of sweep t covers the tile plus a ring of
-// 2(S-1-t)+1 cells, the black phase a ring of 2(S-1-t). Every ring cell is
-// recomputed from exactly the inputs its owning tile uses, so the tile ends
-// with the values of S global sweeps, bit for bit. The tile is written to
-// the other q buffer (every workgroup reads the previous state). Tiles
-// without a fluid cell return at once.
+// S red-black sweeps per launch over a 32x16x16 cell tile (temporal
+// blocking). The workgroup loads q with a halo of 4 cells into LDS and runs
+// the colour phases on shrinking regions: the red phase of sweep t covers
+// the tile plus a ring of 2(S-1-t)+1 cells, the black phase a ring of
+// 2(S-1-t). Every ring cell is recomputed from exactly the inputs its owning
+// tile uses, so the tile ends with the values of S global sweeps, bit for
+// bit, and is written to the other q buffer (every workgroup reads the
+// previous state). Tiles without a fluid cell return at once.
+// The fluid cells of each phase are listed per tile once per solve
+// (k_sor_tb_build) with their LDS index, neighbour mask, diagonal count and
+// rhs, so a launch streams its lists: each thread takes list entries
+// t, t + 512, ...; a phase's entries are loaded during the phase before,
+// and a phase does all of a thread's LDS reads before its writes (cells of
+// one colour never neighbour each other).
+namespace {
+constexpr int TB_QX = SOR_TB_X + 2 * TB_H, TB_QY = SOR_TB_Y + 2 * TB_H, TB_QZ = SOR_TB_Z + 2 * TB_H;
+constexpr int TB_QN = TB_QX * TB_QY * TB_QZ;
+static_assert(TB_QN < (1 << 15), "LDS index in 15 bits");
+
+template <int PH>
+struct TbPhase {
+  static constexpr int NT = SOR_TB_THREADS, U = (tb_items(PH) + NT - 1) / NT;
+  unsigned pk[U];  // LDS index (15 bits) | neighbour mask << 15 (6) | diagonal count << 21 (3)
+  float rh[U];
+  int n;
+  __device__ __forceinline__ void load(const unsigned *__restrict__ lpk, const float *__restrict__ lr,
+                                       const int *__restrict__ cnt, int tile) {
+    n = cnt[4 * tile + PH];
+    const long base = (long)tile * TB_CAP + tb_off(PH);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int e = u * NT + (int)threadIdx.x;
+      const long at = base + (e < n ? e : 0);
+      pk[u] = lpk[at];
+      rh[u] = lr[at];
+    }
+  }
+  __device__ __forceinline__ void relax(float *sq, const float *spw, float om1) {
+    float nv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (u * NT >= n) break;  // uniform: no entry of this slot in the workgroup
+      const int e = u * NT + (int)threadIdx.x;
+      const unsigned w = pk[u];
+      // an empty slot reads a harmless interior cell
+      const int qi = e < n ? (int)(w & 0x7fffu) : (TB_H * TB_QY + TB_H) * TB_QX + TB_H;
+      const float c0 = sq[qi];
+      const float a = sq[qi - 1], b = sq[qi + 1], cc = sq[qi - TB_QX], dd = sq[qi + TB_QX];
+      const float ee = sq[qi - TB_QX * TB_QY], ff = sq[qi + TB_QX * TB_QY];
+      const float pw = spw[(w >> 21) & 7u];
+      // the sum of k_sor_list (an absent neighbour adds +0: the sum never
+      // holds -0, so the bits are those of skipping it)
+      const unsigned m = w >> 15;
+      float sum = 0.f;
+      sum += (m & 1u) ? a : 0.f;
+      sum += (m & 2u) ? b : 0.f;
+      sum += (m & 4u) ? cc : 0.f;
+      sum += (m & 8u) ? dd : 0.f;
+      sum += (m & 16u) ? ee : 0.f;
+      sum += (m & 32u) ? ff : 0.f;
+      nv[u] = sor_relax(c0, sum, rh[u], pw, om1);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (u * NT >= n) break;
+      if (u * NT + (int)threadIdx.x < n) sq[pk[u] & 0x7fffu] = nv[u];
+    }
+  }
+};
+}  // namespace
+
 template <int S>
-__global__ __launch_bounds__(256) void k_sor_tb(Geo g, int tilesX, int tilesY, const uint8_t *__restrict__ active,
-                                                const float *__restrict__ qin, float *__restrict__ qout,
-                                                const float *__restrict__ rhs, const float *__restrict__ pw,
-                                                const uint8_t *__restrict__ nbm, float om1) {
-  constexpr int TX = SOR_TB_X, TY = SOR_TB_Y, TZ = SOR_TB_Z, H = 2 * S, HC = 2 * S - 1;
-  constexpr int QX = TX + 2 * H, QY = TY + 2 * H, QZ = TZ + 2 * H, QN = QX * QY * QZ;
-  constexpr int CX = TX + 2 * HC, CY = TY + 2 * HC, CZ = TZ + 2 * HC, CN = CX * CY * CZ;
+__global__ __launch_bounds__(SOR_TB_THREADS) void k_sor_tb(Geo g, int tilesX, int tilesY,
+                                                           const uint8_t *__restrict__ active,
+                                                           const float *__restrict__ qin, float *__restrict__ qout,
+                                                           const unsigned *__restrict__ lpk,
+                                                           const float *__restrict__ lr, const int *__restrict__ cnt,
+                                                           float omega, float om1) {
+  static_assert(S == 1 || S == 2, "one or two sweeps per launch");
+  constexpr int NT = SOR_TB_THREADS, TX = SOR_TB_X, TY = SOR_TB_Y, TZ = SOR_TB_Z, H = TB_H;
+  constexpr int QX = TB_QX, QY = TB_QY, QN = TB_QN;
   __shared__ float sq[QN];
-  __shared__ float srhs[CN];
-  __shared__ float spw[CN];
-  __shared__ uint8_t snb[CN];
-  int b = xcd_block(blockIdx.x, gridDim.x);
+  __shared__ float spw[8];
+  const int b = xcd_block(blockIdx.x, gridDim.x);
   if (!active[b]) return;
-  int tx = b % tilesX, ty = (b / tilesX) % tilesY, tz = b / (tilesX * tilesY);
-  int i0 = tx * TX, j0 = ty * TY, k0 = tz * TZ;
+  const int tx = b % tilesX, ty = (b / tilesX) % tilesY, tz = b / (tilesX * tilesY);
+  const int i0 = tx * TX, j0 = ty * TY, k0 = tz * TZ;
   const int nxy = g.nx * g.ny;
-  constexpr int BQ = (QN + 255) / 256, BC = (CN + 255) / 256;
+  // the relaxation weights omega / diag as k_assemble rounds them
+  if (threadIdx.x < 8) spw[threadIdx.x] = threadIdx.x >= 1 && threadIdx.x <= 6 ? omega / (float)threadIdx.x : 0.f;
+  // a one-sweep launch runs the last two phases of a two-sweep one
+  TbPhase<S == 2 ? 0 : 2> pa;
+  TbPhase<S == 2 ? 1 : 3> pb;
+  TbPhase<2> pc;
+  TbPhase<3> pd;
   {
+    constexpr int BQ = (QN + NT - 1) / NT;
     float v[BQ];
 #pragma unroll
     for (int u = 0; u < BQ; u++) {
-      int e = u * 256 + threadIdx.x;
-      int li = e % QX, lj = (e / QX) % QY, lk = e / (QX * QY);
-      int i = i0 - H + li, j = j0 - H + lj, k = k0 - H + lk;
-      bool in = e < QN && i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
-      v[u] = in ? qin[k * nxy + j * g.nx + i] : 0.f;
+      const int e = u * NT + (int)threadIdx.x;
+      const int li = e % QX, lj = (e / QX) % QY, lk = e / (QX * QY);
+      const int i = i0 - H + li, j = j0 - H + lj, k = k0 - H + lk;
+      const bool in = e < QN && i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
+      const float x = qin[in ? k * nxy + j * g.nx + i : 0];
+      v[u] = in ? x : 0.f;
     }
+    pa.load(lpk, lr, cnt, b);
 #pragma unroll
     for (int u = 0; u < BQ; u++) {
-      int e = u * 256 + threadIdx.x;
+      const int e = u * NT + (int)threadIdx.x;
       if (e < QN) sq[e] = v[u];
     }
   }
-  {
-    float r[BC], w[BC];
-    uint8_t m[BC];
-#pragma unroll
-    for (int u = 0; u < BC; u++) {
-      int e = u * 256 + threadIdx.x;
-      int li = e % CX, lj = (e / CX) % CY, lk = e / (CX * CY);
-      int i = i0 - HC + li, j = j0 - HC + lj, k = k0 - HC + lk;
-      bool in = e < CN && i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
-      int c = in ? k * nxy + j * g.nx + i : 0;
-      m[u] = in ? nbm[c] : 0;
-      r[u] = in ? rhs[c] : 0.f;
-      w[u] = in ? pw[c] : 0.f;
-    }
-#pragma unroll
-    for (int u = 0; u < BC; u++) {
-      int e = u * 256 + threadIdx.x;
-      if (e < CN) { srhs[e] = r[u]; spw[e] = w[u]; snb[e] = m[u]; }
-    }
-  }
   __syncthreads();
-#pragma unroll
-  for (int ph = 0; ph < 2 * S; ph++) {
-    int color = ph & 1;                     // red (even i+j+k) first
-    int ring = 2 * S - 1 - ph;              // cells around the tile updated in this phase
-    int lo = HC - ring;                     // region start in coefficient coordinates
-    int nX = TX + 2 * ring, nY = TY + 2 * ring, nZ = TZ + 2 * ring;
-    int hx = (nX + 1) >> 1;                 // cells of one colour per row (upper bound)
-    for (int e = threadIdx.x; e < hx * nY * nZ; e += 256) {
-      int cj = lo + (e / hx) % nY, ck = lo + e / (hx * nY);
-      // the first column of this row with the phase's colour
-      int gi0 = i0 - HC + lo, gj = j0 - HC + cj, gk = k0 - HC + ck;
-      int first = ((gi0 + gj + gk) & 1) == color ? 0 : 1;
-      int ci = lo + first + 2 * (e % hx);
-      if (ci >= lo + nX) continue;
-      int cc = (ck * CY + cj) * CX + ci;
-      unsigned msk = snb[cc];
-      if (!(msk & 0x40)) continue;
-      int qi = ((ck + 1) * QY + (cj + 1)) * QX + (ci + 1);
-      float sum = 0.f;
-      if (msk & 1) sum += sq[qi - 1];
-      if (msk & 2) sum += sq[qi + 1];
-      if (msk & 4) sum += sq[qi - QX];
-      if (msk & 8) sum += sq[qi + QX];
-      if (msk & 16) sum += sq[qi - QX * QY];
-      if (msk & 32) sum += sq[qi + QX * QY];
-      sq[qi] = sor_relax(sq[qi], sum, srhs[cc], spw[cc], om1);
-    }
+  // each phase's list entries are loaded before the previous phase's
+  // relaxation, which hides them
+  pb.load(lpk, lr, cnt, b);
+  pa.relax(sq, spw, om1);
+  __syncthreads();
+  if constexpr (S == 2) pc.load(lpk, lr, cnt, b);
+  pb.relax(sq, spw, om1);
+  __syncthreads();
+  if constexpr (S == 2) {
+    pd.load(lpk, lr, cnt, b);
+    pc.relax(sq, spw, om1);
+    __syncthreads();
+    pd.relax(sq, spw, om1);
     __syncthreads();
   }
-  for (int e = threadIdx.x; e < TX * TY * TZ; e += 256) {
-    int li = e % TX, lj = (e / TX) % TY, lk = e / (TX * TY);
-    int i = i0 + li, j = j0 + lj, k = k0 + lk;
+  for (int e = threadIdx.x; e < TX * TY * TZ; e += NT) {
+    const int li = e % TX, lj = (e / TX) % TY, lk = e / (TX * TY);
+    const int i = i0 + li, j = j0 + lj, k = k0 + lk;
     if (i < g.nx && j < g.ny && k < g.nz)
       qout[k * nxy + j * g.nx + i] = sq[((lk + H) * QY + (lj + H)) * QX + (li + H)];
   }
 }
-template __global__ void k_sor_tb<1>(Geo, int, int, const uint8_t *, const float *, float *, const float *,
-                                     const float *, const uint8_t *, float);
-template __global__ void k_sor_tb<2>(Geo, int, int, const uint8_t *, const float *, float *, const float *,
-                                     const float *, const uint8_t *, float);
+template __global__ void k_sor_tb<1>(Geo, int, int, const uint8_t *, const float *, float *, const unsigned *,
+                                     const float *, const int *, float, float);
+template __global__ void k_sor_tb<2>(Geo, int, int, const uint8_t *, const float *, float *, const unsigned *,
+                                     const float *, const int *, float, float);
+
+// The per-tile phase lists of the temporal-blocking sweep, built once per
+// solve: phase ph takes the fluid cells of its colour in the tile plus a
+// ring of 3 - ph cells, in row order (a stream compaction: a ballot per
+// wavefront, the wavefront totals scanned in LDS).
+__global__ __launch_bounds__(SOR_TB_THREADS) void k_sor_tb_build(Geo g, int tilesX, int tilesY,
+                                                                 const uint8_t *__restrict__ active,
+                                                                 const uint8_t *__restrict__ nbm,
+                                                                 const uint8_t *__restrict__ dg,
+                                                                 const float *__restrict__ rhs,
+                                                                 unsigned *__restrict__ lpk, float *__restrict__ lr,
+                                                                 int *__restrict__ cnt) {
+  constexpr int NT = SOR_TB_THREADS, NW = NT / 64, H = TB_H;
+  __shared__ int wsum[NW];
+  __shared__ int run;
+  const int b = blockIdx.x;
+  if (!active[b]) {
+    if (threadIdx.x < 4) cnt[4 * b + threadIdx.x] = 0;
+    return;
+  }
+  const int tx = b % tilesX, ty = (b / tilesX) % tilesY, tz = b / (tilesX * tilesY);
+  const int i0 = tx * SOR_TB_X, j0 = ty * SOR_TB_Y, k0 = tz * SOR_TB_Z;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int ph = 0; ph < 4; ph++) {
+    const int ring = tb_ring(ph), color = ph & 1;
+    const int nX = SOR_TB_X + 2 * ring, nY = SOR_TB_Y + 2 * ring, nZ = SOR_TB_Z + 2 * ring;
+    const int hx = (nX + 1) / 2, items = hx * nY * nZ;
+    const long base = (long)b * TB_CAP + tb_off(ph);
+    if (threadIdx.x == 0) run = 0;
+    __syncthreads();
+    for (int e0 = 0; e0 < items; e0 += NT) {
+      const int e = e0 + threadIdx.x;
+      const int x2 = e % hx, row = e / hx;
+      const int ly = row % nY - ring, lz = row / nY - ring;
+      const int gj = j0 + ly, gk = k0 + lz;
+      const int first = ((i0 - ring + gj + gk) & 1) == color ? 0 : 1;
+      const int lx = -ring + first + 2 * x2;
+      const int gi = i0 + lx;
+      const bool in = e < items && lx < SOR_TB_X + ring && gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny &&
+                      gk >= 0 && gk < g.nz;
+      const int c = in ? (gk * g.ny + gj) * g.nx + gi : 0;
+      const unsigned m = in ? nbm[c] : 0u;
+      const bool fl = (m & 0x40u) != 0;
+      const unsigned long long bal = __ballot(fl);
+      const int before = __popcll(bal & ((1ull << lane) - 1ull));
+      if (lane == 0) wsum[wv] = __popcll(bal);
+      __syncthreads();
+      int off = run;
+      for (int w = 0; w < wv; w++) off += wsum[w];
+      if (fl) {
+        const long at = base + off + before;
+        lpk[at] = (unsigned)(((lz + H) * TB_QY + (ly + H)) * TB_QX + (lx + H)) | ((m & 63u) << 15) |
+                  ((unsigned)dg[c] << 21);
+        lr[at] = rhs[c];
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < NW; w++) t += wsum[w];
+        run += t;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) cnt[4 * b + ph] = run;
+    __syncthreads();
+  }
+}
 
 // per temporal-blocking tile: does it hold a fluid cell
 __global__ void k_sor_tb_mark(Geo g, int tilesX, int tilesY, int nTiles, const uint8_t *__restrict__ nbm,

@@ -28,7 +28,19 @@ constexpr int P2G_T = 8;
 constexpr int EX_TI = 64, EX_TJ = 8, EX_TK = 8;
 constexpr int EX_THREADS = 64 * EX_TK;
 constexpr int SOR_TX = 32, SOR_TY = 8, SOR_TZ = 8;
-constexpr int SOR_TB_X = 16, SOR_TB_Y = 8, SOR_TB_Z = 8;  // temporal-blocking tile
+constexpr int SOR_TB_X = 32, SOR_TB_Y = 16, SOR_TB_Z = 16;  // temporal-blocking tile
+constexpr int SOR_TB_THREADS = 512;
+// The four colour phases of a two-sweep temporal-blocking launch: phase ph
+// relaxes one colour (red first) of the tile plus a ring of 3 - ph cells.
+// Their cells are listed per tile, at most tb_items(ph) each, at offset
+// tb_off(ph) of the tile's TB_CAP entries.
+__host__ __device__ constexpr int tb_ring(int ph) { return 3 - ph; }
+__host__ __device__ constexpr int tb_items(int ph) {
+  return ((SOR_TB_X + 2 * tb_ring(ph) + 1) / 2) * (SOR_TB_Y + 2 * tb_ring(ph)) * (SOR_TB_Z + 2 * tb_ring(ph));
+}
+__host__ __device__ constexpr int tb_off(int ph) { return ph <= 0 ? 0 : tb_off(ph - 1) + tb_items(ph - 1); }
+constexpr int TB_CAP = tb_off(4);
+constexpr int TB_H = 4;  // q halo of the tile in LDS
 
 // Evaporation chooser: candidates per window, threads of its one workgroup,
 // and the ejection counts per window tested at once (each ejection moves
@@ -212,6 +224,9 @@ struct Ctx {
   float *q = nullptr, *q2 = nullptr, *rhs = nullptr, *pw = nullptr;
   uint8_t *sorActive = nullptr;  // per SOR tile: holds a fluid cell
   uint8_t *tbActive = nullptr;   // per temporal-blocking tile: holds a fluid cell
+  unsigned *tbPk = nullptr;      // per tile and phase: listed cells (LDS index, mask, diagonal)
+  float *tbRhs = nullptr;        // ... and their rhs
+  int *tbCnt = nullptr;          // ... and their counts (4 per tile)
   int tbTiles[3] = {0, 0, 0};
   // fluid cells per colour, in cell order (red from 0, black from sorHalf):
   // code = cell | neighbour mask << 26, with rhs, omega/diag and diag per entry

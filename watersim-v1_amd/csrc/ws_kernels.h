@@ -75,7 +75,9 @@ __global__ void k_residual_list(Geo g, const DevScalars *ds, long half, const un
                                 const uint8_t *ldg, const float *q, double *partials);
 template <int S>
 __global__ void k_sor_tb(Geo g, int tilesX, int tilesY, const uint8_t *active, const float *qin, float *qout,
-                         const float *rhs, const float *pw, const uint8_t *nbm, float om1);
+                         const unsigned *pk, const float *lrhs, const int *cnt, float omega, float om1);
+__global__ void k_sor_tb_build(Geo g, int tilesX, int tilesY, const uint8_t *active, const uint8_t *nbm,
+                               const uint8_t *dg, const float *rhs, unsigned *pk, float *lrhs, int *cnt);
 __global__ void k_sor_tb_mark(Geo g, int tilesX, int tilesY, int nTiles, const uint8_t *nbm, uint8_t *active);
 __global__ void k_residual_final(const double *partials, int n, DevScalars *s);
 template <int NT>
