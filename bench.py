@@ -32,7 +32,7 @@ CONFIGS = {
     "cfg1": dict(n=32, target=10000, seed=20260114, vorticity=0.0, tol=0.0, iters=40, physics=False, tune=True),
     "cfg2": dict(n=64, target=232000, seed=20260115, vorticity=0.05, tol=1e-4, iters=0, physics=False,
                  frames=True, currents=0.8, curl_init=0.5, surface_every=10, tune=True),
-    "cfg3": dict(n=128, target=2117000, seed=20260116, vorticity=0.05, tol=1e-4, iters=0, physics=True),
+    "cfg3": dict(n=128, target=2117000, seed=20260116, vorticity=0.05, tol=1e-4, iters=0, physics=True, tune=True),
     # the learned warm start on the cfg3 scene (a seed held out of training)
     "cfg4": dict(n=128, target=2117000, seed=20260116, vorticity=0.05, tol=1e-4, iters=0, physics=True,
                  learned=True),
@@ -310,7 +310,8 @@ def main():
     s.sync()
     tune_report = None
     if cfg.get("tune"):
-        # the autotuner picks the SOR kernel for this grid (outside the timed region)
+        # the autotuner picks the SOR kernel and the P2G bin for this grid
+        # (outside the timed region)
         tune_report = s.tune(64)
         log("tune: %s" % tune_report.replace("\n", "; "))
     # HIP events around the solver's SOR batches and pressure solves stay
@@ -542,16 +543,17 @@ def batched_main(a, cfg):
     log("%d planets, %d particles (%d .. %d)" % (P, sum(nPs), min(nPs), max(nPs)))
     batch = watersim_hip.Batch(planets, graph=True, threads=1024)
     batch.run(a.warmup)
-    # autotuner: convolution tiling per grid size (one planet of each size),
-    # then the solve's workgroup size for the whole batch
-    tiling = {}
+    # autotuner: convolution tiling and P2G bin per grid size (one planet of
+    # each size), then the solve's workgroup size for the whole batch
+    tiling, bins = {}, {}
     for n in sorted(set(sizes)):
         s = planets[sizes.index(n)]
         rep = s.tune(64)
         tiling[n] = int(s.get("netTiled"))
+        bins[n] = int(s.get("p2gBin"))
         log("tune %d^3: %s" % (n, rep.replace("\n", "; ")))
     for i, s in enumerate(planets):
-        s.set(netTiled=tiling[sizes[i]])
+        s.set(netTiled=tiling[sizes[i]], p2gBin=bins[sizes[i]])
     trial = {}
     for th in (256, 512, 1024):
         batch.set(threads=th)
@@ -587,7 +589,7 @@ def batched_main(a, cfg):
     for n in sorted(set(sizes)):
         idx = [i for i in range(P) if sizes[i] == n]
         per_size["%d^3" % n] = {"planets": len(idx), "particles_mean": float(np.mean([nPs[i] for i in idx])),
-                                "netTiled": tiling[n]}
+                                "netTiled": tiling[n], "p2gBin": bins[n]}
     out = {
         "metric": "aggregate particle-updates/s, %d batched planets (32^3..96^3, learned warm start + SOR to 1e-4)" % P,
         "value": updates / el,
@@ -600,8 +602,8 @@ def batched_main(a, cfg):
                                "liquid cell, own terrain seed and gravity each, learned warm start + red-black SOR "
                                "to 1e-4, vorticity %g; one batch: every planet's stages on its own stream, all "
                                "pressure solves in one launch (workgroups per planet by size), the batched substep "
-                               "replayed from a HIP graph; autotuned convolution tiling per size and solve "
-                               "workgroup size" % (P, list(cfg["sizes"]), cfg["vorticity"]),
+                               "replayed from a HIP graph; autotuned convolution tiling and P2G bin per size and "
+                               "solve workgroup size" % (P, list(cfg["sizes"]), cfg["vorticity"]),
                    "particles_total": sum(nPs), "per_size": per_size,
                    "planet_substep_ms": 1e3 * el / a.steps,
                    "solve_threads": threads,

@@ -81,7 +81,10 @@ const char *ws_last_error(const ws_ctx *c);
  * (solver.js:56-75, 81-87, 137-193, 203-210), e.g. "pic", "omega",
  * "pressureIters", "gravity", "maxSpeed", "viscosity", "vorticity", "cfl",
  * "coreR", "oceanR", "spacing", "particleVolume", "atmosphereH", "gRamp",
- * "nP", "dv", "tn", "pressureTol", "maxPressureIters", "iso".
+ * "nP", "dv", "tn", "pressureTol", "maxPressureIters", "iso"; and the
+ * kernel choices of this library (no reference counterpart; each gives the
+ * same results): "sorMode", "sorThreads", "netTiled", "p2gBin" (0..3: P2G
+ * bins of 8x8x8, 4x4x4, 16x8x8, 8x8x4 cells).
  * ws_get_f also reads results: "umax", "dtLast", "residual",
  * "pressureIters_done", "airborneCount", "nFluid", "dfRefined". */
 int ws_set_f(ws_ctx *c, const char *name, double v);
@@ -210,9 +213,11 @@ int ws_read_render(ws_ctx *c, unsigned mask, ws_render_view *view);
  * threads, 5 the whole solve on "flowGroups" workgroups that wait only for
  * their neighbours between half-sweeps; all give the same bits for the same
  * sweeps) and, with the warm
- * start on, the untiled and LDS-tiled convolutions, and keeps the fastest
- * ("sorMode", "sorThreads", "netTiled"). The pressure and all counters are
- * left as they were. `report` (may be NULL) receives one line per candidate
+ * start on, the untiled and LDS-tiled convolutions, and the P2G bins
+ * (8x8x8, 4x4x4, 16x8x8, 8x8x4 cells per tile; the sums are the same for
+ * every bin), and keeps the fastest ("sorMode", "sorThreads", "netTiled",
+ * "p2gBin"). The pressure, the faces and all counters are left as they
+ * were. `report` (may be NULL) receives one line per candidate
  * and the choice. */
 int ws_tune(ws_ctx *c, int sweeps, char *report, size_t len);
 

@@ -1,9 +1,9 @@
 """ws_tune (no reference counterpart): on a context's current system it
 times the SOR kernels (the one-workgroup solve at three sizes, the
-neighbour-synchronised solve) and, with the learned warm start on, the
-network's two convolution tilings, keeps the fastest, and leaves the
-simulation state as it found it (pressure, velocities, particles and the
-solve counters)."""
+neighbour-synchronised solve), the P2G bins and, with the learned warm
+start on, the network's two convolution tilings, keeps the fastest, and
+leaves the simulation state as it found it (pressure, velocities,
+particles and the solve counters)."""
 import numpy as np
 import pytest
 
@@ -33,8 +33,12 @@ def test_tune_keeps_state_and_picks_a_kernel():
     modes = [ln for ln in rep.splitlines() if ln.startswith("sorMode ") and "sweeps in" in ln]
     assert len(modes) == 8  # modes 0-3, mode 4 at 256, 512 and 1024 threads, mode 5
     chosen, threads = int(s.get("sorMode")), int(s.get("sorThreads"))
-    assert 0 <= chosen <= 5 and rep.rstrip().endswith("netTiled %d" % int(s.get("netTiled")))
+    assert 0 <= chosen <= 5 and rep.rstrip().endswith("netTiled %d, p2gBin %d" % (int(s.get("netTiled")),
+                                                                                  int(s.get("p2gBin"))))
     assert ("chosen: sorMode %d, sorThreads %d" % (chosen, threads)) in rep
+    bins = {int(ln.split()[1]): float(ln.split(": ")[1].split()[0]) for ln in rep.splitlines()
+            if ln.startswith("p2gBin ") and ln.rstrip().endswith(" ms")}
+    assert sorted(bins) == [0, 1, 2, 3] and bins[int(s.get("p2gBin"))] == min(bins.values())
     times = {ln.split(":")[0]: float(ln.split(" in ")[1].split()[0]) for ln in modes}
     key = "sorMode %d (%d threads)" % (chosen, threads) if chosen >= 4 else "sorMode %d" % chosen
     assert times[key] == min(times.values())

@@ -138,10 +138,21 @@ int st_rasterize(ws_ctx *c) {
   return WS_OK;
 }
 
+template <int TX, int TY, int TZ>
+static void p2g_launch(ws_ctx *c) {
+  int tx = (c->nx + TX - 1) / TX, ty = (c->ny + TY - 1) / TY, tz = (c->nz + TZ - 1) / TZ;
+  hipLaunchKernelGGL((k_p2g_tile<TX, TY, TZ>), dim3(tx * ty * tz), dim3(256), 0, c->stream, c->geo, tx, ty,
+                     c->cellStart, c->sortedIdx, c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->vel, c->wgt);
+}
+
 int st_p2g(ws_ctx *c) {
-  int tx = (c->nx + P2G_T - 1) / P2G_T, ty = (c->ny + P2G_T - 1) / P2G_T, tz = (c->nz + P2G_T - 1) / P2G_T;
-  hipLaunchKernelGGL(k_p2g_tile, dim3(tx * ty * tz), dim3(256), 0, c->stream, c->geo, tx, ty, c->cellStart,
-                     c->sortedIdx, c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->vel, c->wgt);
+  static_assert(P2G_NBINS == 4, "one launch per bin shape");
+  switch (c->p2gBin) {
+    case 1: p2g_launch<P2G_BINS[1][0], P2G_BINS[1][1], P2G_BINS[1][2]>(c); break;
+    case 2: p2g_launch<P2G_BINS[2][0], P2G_BINS[2][1], P2G_BINS[2][2]>(c); break;
+    case 3: p2g_launch<P2G_BINS[3][0], P2G_BINS[3][1], P2G_BINS[3][2]>(c); break;
+    default: p2g_launch<P2G_BINS[0][0], P2G_BINS[0][1], P2G_BINS[0][2]>(c); break;
+  }
   WS_LAUNCHED();
   return WS_OK;
 }
@@ -1045,6 +1056,11 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
     c->flowXcds = X;
     return WS_OK;
   }
+  if (!strcmp(n, "p2gBin")) {
+    if (v != (int)v || v < 0 || v >= P2G_NBINS) { c->err = "p2gBin must be 0..3 (8x8x8, 4x4x4, 16x8x8, 8x8x4)"; return WS_EINVAL; }
+    c->p2gBin = (int)v;
+    return WS_OK;
+  }
   if (!strcmp(n, "sorThreads")) {
     if (v != 256 && v != 512 && v != 1024) { c->err = "sorThreads must be 256, 512 or 1024"; return WS_EINVAL; }
     c->sorThreads = (int)v;
@@ -1107,7 +1123,7 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal) GF(residualsTotal) GF(ticksTotal) GF(sorMode) GF(sorCellSweeps)
   GF(cmpMsPlain) GF(cmpMsLearned) GF(cmpSweepsPlain) GF(cmpSweepsLearned) GF(cmpSolves) GF(warmStart) GF(netMs) GF(netCalls)
   GF(vaporCount) GF(nightVaporCount) GF(contactOverflow) GF(nBalls) GF(mbM) GF(vaporBigGroups) GF(vaporGroupMax)
-  GF(vaporBigParcels) GF(netTiled) GF(sorThreads) GF(sorGroups) GF(flowGroups) GF(flowXcds)
+  GF(vaporBigParcels) GF(netTiled) GF(p2gBin) GF(sorThreads) GF(sorGroups) GF(flowGroups) GF(flowXcds)
   GF(heatK) GF(Tamb) GF(shadeCool) GF(sunActivity) GF(evapIntensity) GF(snowT) GF(meltT) GF(cloudP) GF(currents)
   GF(simTime) GF(heatAcc) GF(evapCount) GF(cloudT) GF(rainT) GF(rainLiftAge)
 #undef GF
@@ -1268,6 +1284,55 @@ int ws_tune(ws_ctx *c, int sweeps, char *report, size_t len) {
     }
   }
   c->netTiled = bestTiled;
+  // P2G bins: every bin gives the same sums; the faces and weights are kept
+  // aside and put back
+  int bestBin = c->p2gBin;
+  if (!r && c->nP > 0) {
+    const long nf = nfaces(c);
+    float *keepVel = nullptr, *keepWgt = nullptr;
+    if (hipMalloc((void **)&keepVel, sizeof(float) * nf) != hipSuccess ||
+        hipMalloc((void **)&keepWgt, sizeof(float) * nf) != hipSuccess) {
+      c->err = "ws_tune: no memory for the P2G trials";
+      r = WS_EHIP;
+    }
+    if (!r && (hipMemcpyAsync(keepVel, c->vel, sizeof(float) * nf, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+               hipMemcpyAsync(keepWgt, c->wgt, sizeof(float) * nf, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)) {
+      c->err = "ws_tune: copy failed";
+      r = WS_EHIP;
+    }
+    const int keepBin = c->p2gBin;
+    float bt = 1e30f;
+    for (int bin = 0; bin < P2G_NBINS && !r; bin++) {
+      c->p2gBin = bin;
+      float ms = 0;
+      for (int rep2 = 0; rep2 < 2 && !r; rep2++) {
+        if (hipEventRecord(c->ev[2], c->stream) != hipSuccess) { r = WS_EHIP; break; }
+        if ((r = st_p2g(c))) break;
+        if (hipEventRecord(c->ev[3], c->stream) != hipSuccess || hipEventSynchronize(c->ev[3]) != hipSuccess ||
+            hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) != hipSuccess) {
+          c->err = "ws_tune: P2G timing failed";
+          r = WS_EHIP;
+        }
+      }
+      if (r) break;
+      char line[128];
+      snprintf(line, sizeof line, "p2gBin %d (%dx%dx%d): %.3f ms\n", bin, P2G_BINS[bin][0], P2G_BINS[bin][1],
+               P2G_BINS[bin][2], ms);
+      rep += line;
+      if (ms < bt) { bt = ms; bestBin = bin; }
+    }
+    c->p2gBin = r ? keepBin : bestBin;
+    if (keepVel && keepWgt && !r &&
+        (hipMemcpyAsync(c->vel, keepVel, sizeof(float) * nf, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+         hipMemcpyAsync(c->wgt, keepWgt, sizeof(float) * nf, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+         hipStreamSynchronize(c->stream) != hipSuccess)) {
+      c->err = "ws_tune: restoring the faces failed";
+      r = WS_EHIP;
+    }
+    (void)hipStreamSynchronize(c->stream);
+    if (keepVel) (void)hipFree(keepVel);
+    if (keepWgt) (void)hipFree(keepWgt);
+  }
   // put everything back
   hipError_t e1 = hipMemcpyAsync(c->q, saved, sizeof(float) * c->nC, hipMemcpyDeviceToDevice, c->stream);
   hipError_t e2 = hipStreamSynchronize(c->stream);
@@ -1286,8 +1351,8 @@ int ws_tune(ws_ctx *c, int sweeps, char *report, size_t len) {
   if (r) return r;
   if (e1 != hipSuccess || e2 != hipSuccess) { c->err = "ws_tune: restoring the pressure failed"; return WS_EHIP; }
   if (report && len) {
-    snprintf(report, len, "%schosen: sorMode %d, sorThreads %d, netTiled %d", rep.c_str(), c->sorMode, c->sorThreads,
-             c->netTiled);
+    snprintf(report, len, "%schosen: sorMode %d, sorThreads %d, netTiled %d, p2gBin %d", rep.c_str(), c->sorMode,
+             c->sorThreads, c->netTiled, c->p2gBin);
   }
   return WS_OK;
 }

@@ -137,30 +137,32 @@ __global__ void k_scatter(int nP, const int *__restrict__ pcell, const unsigned 
 // exact sums by float rounding only.
 __device__ __forceinline__ long long fix(float v) { return (long long)__builtin_rintf(v * kFixScaleF); }
 
+template <int TX, int TY, int TZ>
 __global__ __launch_bounds__(256) void k_p2g_tile(Geo g, int tilesX, int tilesY, const unsigned *__restrict__ start,
                                                   const unsigned *__restrict__ sorted, const float *__restrict__ px,
                                                   const float *__restrict__ py, const float *__restrict__ pz,
                                                   const float *__restrict__ pvx, const float *__restrict__ pvy,
                                                   const float *__restrict__ pvz, float *__restrict__ vel,
                                                   float *__restrict__ wgt) {
-  constexpr int T = P2G_T;
-  constexpr int BOX = (T + 1) * T * T;  // owned faces of one component (+1 plane for the last tile)
+  // owned faces of one component (+1 plane along its axis for the last tile)
+  constexpr int BOX = cmax(cmax((TX + 1) * TY * TZ, TX * (TY + 1) * TZ), TX * TY * (TZ + 1));
+  static_assert((TY + 2) * (TZ + 2) <= 256, "one thread per particle row of the tile");
   __shared__ unsigned long long accM[3][BOX];
   __shared__ unsigned long long accW[3][BOX];
-  __shared__ unsigned rowBeg[(T + 2) * (T + 2)];
-  __shared__ unsigned rowPre[(T + 2) * (T + 2) + 1];
+  __shared__ unsigned rowBeg[(TY + 2) * (TZ + 2)];
+  __shared__ unsigned rowPre[(TY + 2) * (TZ + 2) + 1];
 
   int b = xcd_block(blockIdx.x, gridDim.x);
   int tx = b % tilesX, ty = (b / tilesX) % tilesY, tz = b / (tilesX * tilesY);
-  int i0 = tx * T, j0 = ty * T, k0 = tz * T;
+  int i0 = tx * TX, j0 = ty * TY, k0 = tz * TZ;
   for (int e = threadIdx.x; e < 3 * BOX; e += blockDim.x) {
     (&accM[0][0])[e] = 0ull;
     (&accW[0][0])[e] = 0ull;
   }
   // particle rows of the tile + halo: cells [i0-1, i0+T] x [j0-1, j0+T] x [k0-1, k0+T]
-  int ci0 = max(i0 - 1, 0), ci1 = min(i0 + T, g.nx - 1);
-  int cj0 = max(j0 - 1, 0), cj1 = min(j0 + T, g.ny - 1);
-  int ck0 = max(k0 - 1, 0), ck1 = min(k0 + T, g.nz - 1);
+  int ci0 = max(i0 - 1, 0), ci1 = min(i0 + TX, g.nx - 1);
+  int cj0 = max(j0 - 1, 0), cj1 = min(j0 + TY, g.ny - 1);
+  int ck0 = max(k0 - 1, 0), ck1 = min(k0 + TZ, g.nz - 1);
   int nrj = cj1 - cj0 + 1, nrows = nrj * (ck1 - ck0 + 1);
   if ((int)threadIdx.x < nrows) {
     int r = threadIdx.x;
@@ -179,7 +181,7 @@ __global__ __launch_bounds__(256) void k_p2g_tile(Geo g, int tilesX, int tilesY,
   __syncthreads();
   unsigned total = rowPre[nrows];
   // owned face ranges per component: [lo, hi) in i, j, k
-  int iHiC = min(i0 + T, g.nx), jHiC = min(j0 + T, g.ny), kHiC = min(k0 + T, g.nz);
+  int iHiC = min(i0 + TX, g.nx), jHiC = min(j0 + TY, g.ny), kHiC = min(k0 + TZ, g.nz);
   // The binned particles are in cell order, so neighbouring entries share
   // their faces. A wavefront takes every 8th entry of a 256-entry chunk
   // (thread 32a + b takes entry 8b + a) so that its lanes' LDS atomics fall
@@ -209,9 +211,9 @@ __global__ __launch_bounds__(256) void k_p2g_tile(Geo g, int tilesX, int tilesY,
       float wA = s0 * t0, wB = s1 * t0, wC = s0 * t1, wD = s1 * t1;
       int d0 = comp == 0 ? g.nx + 1 : g.nx, d1 = comp == 1 ? g.ny + 1 : g.ny;
       // owned ranges of this component
-      int oi1 = comp == 0 ? (i0 + T >= g.nx ? g.nx + 1 : i0 + T) : iHiC;
-      int oj1 = comp == 1 ? (j0 + T >= g.ny ? g.ny + 1 : j0 + T) : jHiC;
-      int ok1 = comp == 2 ? (k0 + T >= g.nz ? g.nz + 1 : k0 + T) : kHiC;
+      int oi1 = comp == 0 ? (i0 + TX >= g.nx ? g.nx + 1 : i0 + TX) : iHiC;
+      int oj1 = comp == 1 ? (j0 + TY >= g.ny ? g.ny + 1 : j0 + TY) : jHiC;
+      int ok1 = comp == 2 ? (k0 + TZ >= g.nz ? g.nz + 1 : k0 + TZ) : kHiC;
 #pragma unroll
       for (int e = 0; e < 8; e++) {
         int di = e & 1, dj = (e >> 1) & 1, dk = (e >> 2) & 1;
@@ -221,7 +223,7 @@ __global__ __launch_bounds__(256) void k_p2g_tile(Geo g, int tilesX, int tilesY,
         // reference's idx+1 at the +x edge, solver.js:765,772)
         if (ii >= d0) { ii -= d0; jj += 1; if (jj >= d1) { jj = 0; kk += 1; } }
         if (ii < i0 || ii >= oi1 || jj < j0 || jj >= oj1 || kk < k0 || kk >= ok1) continue;
-        int l = ((kk - k0) * (T + (comp == 1)) + (jj - j0)) * (T + (comp == 0)) + (ii - i0);
+        int l = ((kk - k0) * (TY + (comp == 1)) + (jj - j0)) * (TX + (comp == 0)) + (ii - i0);
         atomicAdd(&accM[comp][l], (unsigned long long)fix(wt * vc[comp]));
         atomicAdd(&accW[comp][l], (unsigned long long)fix(wt));
       }
@@ -231,14 +233,14 @@ __global__ __launch_bounds__(256) void k_p2g_tile(Geo g, int tilesX, int tilesY,
   // normalise and write the owned faces
   for (int comp = 0; comp < 3; comp++) {
     int d0 = comp == 0 ? g.nx + 1 : g.nx, d1 = comp == 1 ? g.ny + 1 : g.ny;
-    int oi1 = comp == 0 ? (i0 + T >= g.nx ? g.nx + 1 : i0 + T) : iHiC;
-    int oj1 = comp == 1 ? (j0 + T >= g.ny ? g.ny + 1 : j0 + T) : jHiC;
-    int ok1 = comp == 2 ? (k0 + T >= g.nz ? g.nz + 1 : k0 + T) : kHiC;
+    int oi1 = comp == 0 ? (i0 + TX >= g.nx ? g.nx + 1 : i0 + TX) : iHiC;
+    int oj1 = comp == 1 ? (j0 + TY >= g.ny ? g.ny + 1 : j0 + TY) : jHiC;
+    int ok1 = comp == 2 ? (k0 + TZ >= g.nz ? g.nz + 1 : k0 + TZ) : kHiC;
     int ni = oi1 - i0, nj = oj1 - j0, nk = ok1 - k0;
     long base = comp == 0 ? 0 : (comp == 1 ? g.uN : (long)g.uN + g.vN);
     for (int e = threadIdx.x; e < ni * nj * nk; e += blockDim.x) {
       int li = e % ni, lj = (e / ni) % nj, lk = e / (ni * nj);
-      int l = (lk * (T + (comp == 1)) + lj) * (T + (comp == 0)) + li;
+      int l = (lk * (TY + (comp == 1)) + lj) * (TX + (comp == 0)) + li;
       float W = (float)(long long)accW[comp][l] * kFixInvF;
       float M = (float)(long long)accM[comp][l] * kFixInvF;
       long f = base + ((long)(k0 + lk) * d1 + (j0 + lj)) * d0 + (i0 + li);
@@ -247,6 +249,15 @@ __global__ __launch_bounds__(256) void k_p2g_tile(Geo g, int tilesX, int tilesY,
     }
   }
 }
+#define WS_P2G_INST(X, Y, Z)                                                                                   \
+  template __global__ void k_p2g_tile<X, Y, Z>(Geo, int, int, const unsigned *, const unsigned *, const float *, \
+                                               const float *, const float *, const float *, const float *,      \
+                                               const float *, float *, float *);
+WS_P2G_INST(8, 8, 8)
+WS_P2G_INST(4, 4, 4)
+WS_P2G_INST(16, 8, 8)
+WS_P2G_INST(8, 8, 4)
+#undef WS_P2G_INST
 
 // ------------------------------------------------- boundary conditions
 // Face classification shared by BC, gravity, projection and extrapolation.

@@ -22,7 +22,11 @@ constexpr float kFixScaleF = 1125899906842624.0f;
 constexpr float kFixInvF = 1.0f / 1125899906842624.0f;
 
 // Tile shapes (cells) for P2G and the extrapolation wavefront.
-constexpr int P2G_T = 8;
+// P2G bins (cells per tile along i, j, k), chosen by the context's p2gBin
+// (the autotuner times them; every bin gives the same sums)
+constexpr int P2G_BINS[4][3] = {{8, 8, 8}, {4, 4, 4}, {16, 8, 8}, {8, 8, 4}};
+constexpr int P2G_NBINS = 4;
+__host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // extrapolation tile: 64 faces along i (one lane each), 8 x 8 rows in j and
 // k; one wavefront per k-plane of the tile
 constexpr int EX_TI = 64, EX_TJ = 8, EX_TK = 8;
@@ -245,6 +249,7 @@ struct Ctx {
   // residual checks on the device, no host round trips
   int sorMode = 2;
   int sorThreads = 1024;
+  int p2gBin = 0;  // index into P2G_BINS
   double *sorChunk = nullptr;  // residual chunk sums of the on-device solve
   int sorGroups = 1;           // workgroups of one context's on-device solve
   unsigned *sorArrive = nullptr;  // their barrier counter and timed-out flag

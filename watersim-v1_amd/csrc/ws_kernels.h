@@ -45,6 +45,7 @@ __global__ void k_mark(Geo g, World w, int nP, const float *px, const float *py,
                        uint8_t *ct, unsigned *cnt, int *pcell, unsigned *prank, DevScalars *s);
 __global__ void k_scatter(int nP, const int *pcell, const unsigned *prank, const unsigned *start,
                           unsigned *sorted);
+template <int TX, int TY, int TZ>
 __global__ void k_p2g_tile(Geo g, int tilesX, int tilesY, const unsigned *start, const unsigned *sorted,
                            const float *px, const float *py, const float *pz, const float *pvx, const float *pvy,
                            const float *pvz, float *vel, float *wgt);
