@@ -44,28 +44,34 @@ WEIGHTS = os.path.join(ROOT, "watersim-v1_amd", "weights", "warmstart_unet.f32")
 BF16_PEAK = 2.5e15  # MI355X dense BF16 matrix FLOP/s
 
 
-def net_flops(n):
+def net_flops(n, coarse=False):
     """Multiply-adds x 2 of one inference of the warm-start network on an
-    n^3 grid: 3x3x3 convolutions 5->16->16 (n^3), 16->32->32 ((n/2)^3),
-    32->64->64 ((n/4)^3), 96->32->32 ((n/2)^3), 48->16->16 (n^3), head."""
-    V = n ** 3
+    n^3 solver grid (the network's grid m = n, or n/2 when it runs on the
+    coarsened grid): 3x3x3 convolutions 5->16->16 (m^3), 16->32->32
+    ((m/2)^3), 32->64->64 ((m/4)^3), 96->32->32 ((m/2)^3), 48->16->16 (m^3),
+    head."""
+    V = (n // 2 if coarse else n) ** 3
     full = 16 * 5 + 16 * 16 + 48 * 16 + 16 * 16
     half = 16 * 32 + 32 * 32 + 96 * 32 + 32 * 32
     quarter = 32 * 64 + 64 * 64
     return 2 * 27 * (V * full + V / 8 * half + V / 64 * quarter) + 2 * 16 * V
 
 
-def net_bytes(n):
-    """Algorithmic HBM bytes of one inference on an n^3 grid: every layer
-    reads its bf16 input once and writes its bf16 output once (the 27 taps
-    reuse the input through LDS), pooling and up-sampling read and write
-    their tensors once, plus the solver-side inputs (residual, scales,
-    5-channel input tensor) and the final q update. Per full-resolution
-    voxel: 500 B at n^3 (convolutions 0, 1, 8, 9, head, pooling read, skip
-    concatenation), 1024 B at (n/2)^3, 640 B at (n/4)^3, 56 B of inputs and
-    update."""
+def net_bytes(n, coarse=False):
+    """Algorithmic HBM bytes of one inference on an n^3 solver grid: every
+    layer reads its bf16 input once and writes its bf16 output once (the 27
+    taps reuse the input through LDS), pooling and up-sampling read and write
+    their tensors once. Per network voxel (grid m = n, or n/2 coarsened):
+    500 B at m^3 (convolutions 0, 1, 8, 9, head, pooling read, skip
+    concatenation), 1024 B at (m/2)^3, 640 B at (m/4)^3. The solver side at
+    full resolution: 56 B per cell (residual, input tensor, update), or on
+    the coarsened grid 32 B per cell (residual; the block averages read the
+    cell types, r0 and q; the update reads the types and updates q) and the
+    16-byte input voxel."""
     V = n ** 3
-    return V * 500 + V / 8 * 1024 + V / 64 * 640 + V * 56
+    m = (n // 2 if coarse else n) ** 3
+    side = V * 32 + m * 16 if coarse else V * 56
+    return m * 500 + m / 8 * 1024 + m / 64 * 640 + side
 
 
 FRAME = 1.0 / 60
@@ -428,9 +434,13 @@ def learned_compare(s, world, cfg, weights, n, done):
     """n substeps in the library's compare mode: every pressure solve runs
     twice from the same assembled system, plain SOR from the previous
     pressure and SOR from the network's prediction, both to the tolerance.
-    Returns the per-solve means."""
-    s.set_net_params(np.fromfile(weights, dtype=np.float32))
-    s.set(timing=1, pressureCompare=1)
+    Returns the per-solve means. One untimed substep in compare mode comes
+    first: the network's first call allocates its buffers."""
+    s.load_net_weights(weights)
+    s.set(pressureCompare=1)
+    done += advance(s, world, cfg, 1, done)
+    s.sync()
+    s.set(timing=1)
     advance(s, world, cfg, n, done)
     s.sync()
     g = {k: s.get(k) for k in ("cmpMsPlain", "cmpMsLearned", "cmpSweepsPlain", "cmpSweepsLearned", "cmpSolves",
@@ -456,7 +466,7 @@ def learned_main(a, cfg):
         sys.exit(2)
     s, world = make_solver(cfg)
     n, nP = cfg["n"], int(s.get("nP"))
-    s.set_net_params(np.fromfile(a.weights, dtype=np.float32))
+    s.load_net_weights(a.weights)
     done = advance(s, world, cfg, a.warmup, 0)
     s.sync()
     s.set(timing=1, pressureCompare=1)
@@ -470,8 +480,9 @@ def learned_main(a, cfg):
     ns = max(g["cmpSolves"], 1)
     plain, learned = g["cmpMsPlain"] / ns, g["cmpMsLearned"] / ns
     infer = g["netMs"] / max(g["netCalls"], 1)
-    flops = net_flops(n)
-    nbytes = net_bytes(n)
+    coarse = bool(s.get("netCoarse"))
+    flops = net_flops(n, coarse)
+    nbytes = net_bytes(n, coarse)
     tflops = flops / (infer / 1e3) / 1e12 if infer > 0 else 0.0
     # the binding roof at full speed is HBM: the bytes take longer at 8 TB/s
     # than the MFMA work at 2.5 PFLOP/s
@@ -487,20 +498,57 @@ def learned_main(a, cfg):
         "data": "synthetic (seeded planet built by the library's world builder; world seed held out of training)",
         "config": {"workload": "cfg4: the cfg3 full-physics scene, %d^3, %d particles; every pressure solve run "
                                "twice from the same system: plain red-black SOR from the previous pressure, and "
-                               "SOR from the network's prediction, both to relative residual 1e-4" % (n, nP),
+                               "SOR from the network's prediction, both to relative residual 1e-4; the "
+                               "network runs on the %s grid" % (n, nP, "coarsened (%d^3)" % (n // 2) if coarse
+                                                                else "full"),
+                   "net_coarse": int(coarse),
                    "pressure_ms_plain": plain, "pressure_ms_learned": learned,
                    "sweeps_plain": g["cmpSweepsPlain"] / ns, "sweeps_learned": g["cmpSweepsLearned"] / ns,
                    "inference_ms": infer, "solves": int(ns)},
         "roofline": {"bound": "hbm" if nbytes / HBM_PEAK >= flops / BF16_PEAK else "mfma",
                      "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / (HBM_PEAK / 1e9), "traffic": None,
-                     "kernel": "warm-start network inference (k_conv3l BF16 MFMA implicit GEMMs + elementwise)",
+                     "kernel": "warm-start network inference (BF16 MFMA implicit-GEMM convolutions, LDS-tiled or "
+                               "split-K by level size, + elementwise)",
                      "avg_launch_us": 1e3 * infer, "algorithmic_bytes_per_launch": nbytes,
                      "algorithmic_flops_per_launch": flops, "mfma_tflops": tflops,
                      "mfma_frac": tflops / (BF16_PEAK / 1e12)},
     }
     s.close()
     print(json.dumps(out), flush=True)
+
+
+def make_planets(cfg, weights, only=None):
+    """The batched-planets workload: cfg["planets"] planets whose grid sides
+    are drawn from cfg["sizes"] (seeded), each with its own terrain seed and
+    gravity (0.8..1.2 g), about 8 particles per liquid cell, the learned warm
+    start and SOR to the tolerance. only: indices of the planets to build
+    (all by default). Returns the solvers and every planet's grid side."""
+    from concurrent.futures import ThreadPoolExecutor
+    import watersim_hip
+    rng = np.random.default_rng(cfg["seed"])
+    P = cfg["planets"]
+    sizes = [int(x) for x in rng.choice(cfg["sizes"], size=P)]
+    gravs = [float(9.8 * rng.uniform(0.8, 1.2)) for _ in range(P)]
+    idx = list(range(P)) if only is None else list(only)
+    t = time.time()
+
+    def build(i):
+        n = sizes[i]
+        return watersim_hip.build_world(n, W_DOMAIN / n, 0.25 * W_DOMAIN, 0.15 * W_DOMAIN, cfg["seed"] + 1 + i,
+                                        int(cfg["per_cell"] * n ** 3))
+    with ThreadPoolExecutor(8) as ex:
+        worlds = list(ex.map(build, idx))
+    log("%d worlds built in %.1fs" % (len(idx), time.time() - t))
+    planets = []
+    for i, w in zip(idx, worlds):
+        s = watersim_hip.solver_for_world(w)
+        s.set(pic=0.05, omega=1.7, gravity=gravs[i], maxSpeed=3.2, vorticity=cfg["vorticity"], cfl=3.0,
+              pressureTol=cfg["tol"], maxPressureIters=4000, residualEvery=8)
+        s.load_net_weights(weights)
+        s.set(warmStart=1)
+        planets.append(s)
+    return planets, sizes
 
 
 def batched_main(a, cfg):
@@ -512,33 +560,11 @@ def batched_main(a, cfg):
     graph and replayed. The autotuner picks the convolution tiling per grid size (on
     one planet of each size) and the solve's workgroup size for the batch.
     value = particle updates per second summed over the planets."""
-    from concurrent.futures import ThreadPoolExecutor
     import watersim_hip
-    rng = np.random.default_rng(cfg["seed"])
-    P = cfg["planets"]
-    sizes = [int(x) for x in rng.choice(cfg["sizes"], size=P)]
-    gravs = [float(9.8 * rng.uniform(0.8, 1.2)) for _ in range(P)]
-    weights = np.fromfile(a.weights, dtype=np.float32) if os.path.exists(a.weights) else None
-    if weights is None:
+    if not os.path.exists(a.weights):
         raise SystemExit("cfg5 needs the trained warm-start weights (%s)" % a.weights)
-    t = time.time()
-
-    def build(i):
-        n = sizes[i]
-        return watersim_hip.build_world(n, W_DOMAIN / n, 0.25 * W_DOMAIN, 0.15 * W_DOMAIN, cfg["seed"] + 1 + i,
-                                        int(cfg["per_cell"] * n ** 3))
-    with ThreadPoolExecutor(8) as ex:
-        worlds = list(ex.map(build, range(P)))
-    log("%d worlds built in %.1fs" % (P, time.time() - t))
-    planets = []
-    for i, w in enumerate(worlds):
-        s = watersim_hip.solver_for_world(w)
-        s.set(pic=0.05, omega=1.7, gravity=gravs[i], maxSpeed=3.2, vorticity=cfg["vorticity"], cfl=3.0,
-              pressureTol=cfg["tol"], maxPressureIters=4000, residualEvery=8)
-        s.set_net_params(weights)
-        s.set(warmStart=1)
-        planets.append(s)
-    del worlds
+    planets, sizes = make_planets(cfg, a.weights)
+    P = len(planets)
     nPs = [int(s.get("nP")) for s in planets]
     log("%d planets, %d particles (%d .. %d)" % (P, sum(nPs), min(nPs), max(nPs)))
     batch = watersim_hip.Batch(planets, graph=True, threads=1024)

@@ -16,6 +16,9 @@ solver with a host-side SOR loop.
   the list, not by the split). The graph path is called several
   times, with downloads and synchronizations between the calls, and every
   call's results are checked.
+- The batched-planets workload itself, 64 planets of 32^3..96^3, runs from
+  its graph with every planet at the tolerance, and one planet of each size
+  equals its own single-planet run.
 """
 import os
 
@@ -37,7 +40,7 @@ def planet(n, seed, warm=False):
     s = watersim_hip.solver_for_world(w)
     s.set(pressureTol=1e-4, maxPressureIters=4000, residualEvery=8, vorticity=0.05, cfl=3.0, maxSpeed=3.2)
     if warm:
-        s.set_net_params(np.fromfile(WEIGHTS, dtype=np.float32))
+        s.load_net_weights(WEIGHTS)
         s.set(warmStart=1)
     return s
 
@@ -145,6 +148,42 @@ def test_batch_equals_single_planets(graph, cells_per_group):
             call, k, len(bat), [int(s.get("sweepsTotal")) for s in bat], int(b.get("groups"))))
     b.close()
     for s in ref + bat:
+        s.close()
+
+
+def test_batch_of_64_planets_as_benchmarked():
+    """The batched-planets workload at its own size (the benchmark's draw of
+    64 planets with grid sides 32, 48, 64, 96, about 8 particles per liquid
+    cell, own terrain seed and gravity, learned warm start, SOR to 1e-4),
+    two substeps replayed from the batch's HIP graph. Every planet ends at
+    the tolerance with finite state inside the domain, and the first planet
+    of each size equals its own run through ws_run_substeps (sorMode 4) bit
+    for bit."""
+    if not os.path.exists(WEIGHTS):
+        pytest.skip("no trained warm-start weights")
+    import bench
+    cfg = bench.CONFIGS["cfg5"]
+    planets, sizes = bench.make_planets(cfg, WEIGHTS)
+    assert len(planets) == 64 and sorted(set(sizes)) == [32, 48, 64, 96]
+    firsts = [sizes.index(n) for n in (32, 48, 64, 96)]
+    ref, _ = bench.make_planets(cfg, WEIGHTS, only=firsts)
+    for s in ref:
+        s.set(sorMode=4, sorThreads=1024)
+    b = watersim_hip.Batch(planets, graph=True, threads=1024)
+    b.run(2)
+    for s in ref:
+        s.run_substeps(2)
+    for s, n in zip(planets, sizes):
+        assert s.get("residual") <= 1e-4
+        p = s.download("px")
+        assert np.all(np.isfinite(p)) and p.min() >= 0 and p.max() <= W
+        assert np.isfinite(s.download("pvy")).all()
+    for r, i in zip(ref, firsts):
+        assert_same(r, planets[i], "planet %d (%d^3)" % (i, sizes[i]))
+    print("  64 planets, %d particles; sweeps of the checked planets %s" % (
+        sum(int(s.get("nP")) for s in planets), [int(planets[i].get("sweepsTotal")) for i in firsts]))
+    b.close()
+    for s in planets + ref:
         s.close()
 
 

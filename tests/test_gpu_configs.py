@@ -162,7 +162,7 @@ def test_learned_warm_start_gain():
         pytest.skip("no trained warm-start weights")
     cfg = dict(bench.CONFIGS["cfg4"])
     s, world = bench.make_solver(cfg)
-    s.set_net_params(np.fromfile(WEIGHTS, dtype=np.float32))
+    s.load_net_weights(WEIGHTS)
     bench.advance(s, world, cfg, 6, 0)
     s.sync()
     s.set(timing=1, pressureCompare=1)
@@ -206,7 +206,7 @@ def _planet(n, seed):
     s = watersim_hip.solver_for_world(w)
     s.set(pressureTol=1e-4, maxPressureIters=4000, residualEvery=8, vorticity=0.05, cfl=3.0, maxSpeed=3.2,
           pic=0.05, omega=1.7, gravity=9.8)
-    s.set_net_params(np.fromfile(WEIGHTS, dtype=np.float32))
+    s.load_net_weights(WEIGHTS)
     s.set(warmStart=1)
     return s, w
 

@@ -1024,7 +1024,7 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
   SF(heatK) SF(Tamb) SF(shadeCool) SF(sunActivity) SF(evapIntensity) SF(snowT) SF(meltT) SF(cloudP)
   SF(currents) SF(Rlo) SF(Rhi) SF(simTime) SF(heatAcc) SF(netLossCoarse)
   SI(pressureIters) SI(maxPressureIters) SI(residualEvery) SI(ceilReflect) SI(hasSun) SI(measureResidual)
-  SI(adaptiveChecks) SI(spinOn) SI(warmStart) SI(recordSamples) SI(pressureCompare) SI(netTiled) SI(vaporSerialMax) SI(vaporQueueMax) SI(vaporLdsMax)
+  SI(adaptiveChecks) SI(spinOn) SI(warmStart) SI(recordSamples) SI(pressureCompare) SI(netTiled) SI(netCoarse) SI(vaporSerialMax) SI(vaporQueueMax) SI(vaporLdsMax)
 #undef SF
 #undef SI
   if (!strcmp(n, "sorMode")) {
@@ -1123,7 +1123,7 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal) GF(residualsTotal) GF(ticksTotal) GF(sorMode) GF(sorCellSweeps)
   GF(cmpMsPlain) GF(cmpMsLearned) GF(cmpSweepsPlain) GF(cmpSweepsLearned) GF(cmpSolves) GF(warmStart) GF(netMs) GF(netCalls)
   GF(vaporCount) GF(nightVaporCount) GF(contactOverflow) GF(nBalls) GF(mbM) GF(vaporBigGroups) GF(vaporGroupMax)
-  GF(vaporBigParcels) GF(netTiled) GF(p2gBin) GF(sorThreads) GF(sorGroups) GF(flowGroups) GF(flowXcds)
+  GF(vaporBigParcels) GF(netTiled) GF(netCoarse) GF(p2gBin) GF(sorThreads) GF(sorGroups) GF(flowGroups) GF(flowXcds)
   GF(heatK) GF(Tamb) GF(shadeCool) GF(sunActivity) GF(evapIntensity) GF(snowT) GF(meltT) GF(cloudP) GF(currents)
   GF(simTime) GF(heatAcc) GF(evapCount) GF(cloudT) GF(rainT) GF(rainLiftAge)
 #undef GF

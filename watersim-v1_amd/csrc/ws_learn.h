@@ -49,6 +49,7 @@ struct Net {
   double *rpart = nullptr;
   // recorded (input, target) samples of the context's grid
   int bankCap = 0, bankCount = 0, bankNext = 0;
+  int bankCoarse = 0;  // the samples are of the coarsened grid
   bf16_t *bankX = nullptr;
   float *bankT = nullptr;
   std::vector<void *> owned;

@@ -80,6 +80,72 @@ __global__ __launch_bounds__(256) void k_conv3(NetDims d, int Cin, int K, int fl
   }
 }
 
+// The same product for small grids, where one wave per 16 voxels leaves
+// most of the GPU idle: the S waves of a block share 16 voxels, wave w takes
+// the steps of 32 K values w, w + S, ..., and the S partial sums are added
+// in wave order through LDS (a fixed order: the result does not depend on
+// timing).
+template <int NT, int OUT, int S>
+__global__ __launch_bounds__(64 * S) void k_conv3s(NetDims d, int Cin, int K, int flip,
+                                                   const bf16_t *__restrict__ in, const bf16_t *__restrict__ W,
+                                                   const float *__restrict__ bias, void *__restrict__ out) {
+  __shared__ f4 part[S - 1][NT][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long V = (long)d.Z * d.Y * d.X;
+  const long v0 = (long)blockIdx.x * 16;
+  const long va = v0 + (lane & 15);
+  const bool okA = va < V;
+  const int x = (int)(va % d.X), y = (int)((va / d.X) % d.Y), z = (int)(va / ((long)d.X * d.Y));
+  const int kq = 8 * (lane >> 4);
+  f4 acc[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; nt++) acc[nt] = f4{0.f, 0.f, 0.f, 0.f};
+  const bf16_t *wrow = W + (long)(lane & 15) * K;
+#pragma unroll 2
+  for (int kc = 32 * wv; kc < K; kc += 32 * S) {
+    const int kk = kc + kq;
+    const int tap = kk / Cin, ci = kk - tap * Cin;
+    bf8 a = {};
+    if (okA && tap < 27) {
+      int dz, dy, dx;
+      tap_offset(tap, flip, dz, dy, dx);
+      const int zz = z + dz, yy = y + dy, xx = x + dx;
+      if (zz >= 0 && zz < d.Z && yy >= 0 && yy < d.Y && xx >= 0 && xx < d.X)
+        a = *(const bf8 *)(in + (((long)zz * d.Y + yy) * d.X + xx) * Cin + ci);
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) {
+      const bf8 b = *(const bf8 *)(wrow + (long)nt * 16 * K + kk);
+      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nt], 0, 0, 0);
+    }
+  }
+  if (wv > 0) {
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) part[wv - 1][nt][lane] = acc[nt];
+  }
+  __syncthreads();
+  if (wv != 0) return;
+  for (int w = 1; w < S; w++) {
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) acc[nt] += part[w - 1][nt][lane];
+  }
+  constexpr int Cout = NT * 16;
+#pragma unroll
+  for (int nt = 0; nt < NT; nt++) {
+    const int n = nt * 16 + (lane & 15);
+    const float bn = bias ? bias[n] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const long v = v0 + 4 * (lane >> 4) + i;
+      if (v >= V) continue;
+      float val = acc[nt][i] + bn;
+      if (OUT == 0) val = fmaxf(val, 0.f);
+      if (OUT == 2) ((float *)out)[v * Cout + n] = val;
+      else ((bf16_t *)out)[v * Cout + n] = f2bf(val);
+    }
+  }
+}
+
 // The same convolution with the input tile staged in LDS. A block computes
 // a 16 x 8 x (MT/2) voxel tile: each of its four waves owns MT rows of 16
 // voxels along x and all output channels. The input region (tile plus a
@@ -293,35 +359,52 @@ __global__ void k_pack_weights(int Cout, int Cin, int Cin_p, int K, int Kd, cons
 
 // 2x2x2 average pooling (average in f32, stored bf16)
 __global__ void k_pool(NetDims dc, int C, const bf16_t *__restrict__ fine, bf16_t *__restrict__ coarse) {
+  // one thread per coarse voxel and group of 8 channels (C is a multiple of 8)
   long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long Vc = (long)dc.Z * dc.Y * dc.X;
-  if (e >= Vc * C) return;
-  int c = (int)(e % C);
-  long v = e / C;
+  const int G = C >> 3;
+  if (e >= Vc * G) return;
+  const int grp = (int)(e % G);
+  long v = e / G;
   int x = (int)(v % dc.X), y = (int)((v / dc.X) % dc.Y), z = (int)(v / ((long)dc.X * dc.Y));
   int FX = 2 * dc.X, FY = 2 * dc.Y;
-  float s = 0.f;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   for (int q = 0; q < 8; q++) {
     int zz = 2 * z + (q >> 2), yy = 2 * y + ((q >> 1) & 1), xx = 2 * x + (q & 1);
-    s += bf2f(fine[(((long)zz * FY + yy) * FX + xx) * C + c]);
+    const uint4 w = *(const uint4 *)(fine + (((long)zz * FY + yy) * FX + xx) * C + 8 * grp);
+    const unsigned u[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      s[2 * i] += bf2f((bf16_t)(u[i] & 0xffffu));
+      s[2 * i + 1] += bf2f((bf16_t)(u[i] >> 16));
+    }
   }
-  coarse[e] = f2bf(s * 0.125f);
+  unsigned o[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) o[i] = (unsigned)f2bf(s[2 * i] * 0.125f) | ((unsigned)f2bf(s[2 * i + 1] * 0.125f) << 16);
+  *(uint4 *)(coarse + v * C + 8 * grp) = uint4{o[0], o[1], o[2], o[3]};
 }
 
 // nearest upsampling x2 of the coarse tensor concatenated with the skip:
-// out[v][0..Cu) = coarse[parent(v)], out[v][Cu..Cu+Cs) = skip[v]
+// out[v][0..Cu) = coarse[parent(v)], out[v][Cu..Cu+Cs) = skip[v]; one
+// thread per fine voxel and group of 8 channels (Cu, Cs multiples of 8)
 __global__ void k_up_concat(NetDims df, int Cu, int Cs, const bf16_t *__restrict__ coarse,
                             const bf16_t *__restrict__ skip, bf16_t *__restrict__ out) {
   long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long Vf = (long)df.Z * df.Y * df.X;
-  int C = Cu + Cs;
-  if (e >= Vf * C) return;
-  int c = (int)(e % C);
-  long v = e / C;
-  if (c >= Cu) { out[e] = skip[v * Cs + (c - Cu)]; return; }
-  int x = (int)(v % df.X), y = (int)((v / df.X) % df.Y), z = (int)(v / ((long)df.X * df.Y));
-  long p = ((long)(z >> 1) * (df.Y >> 1) + (y >> 1)) * (df.X >> 1) + (x >> 1);
-  out[e] = coarse[p * Cu + c];
+  const int C = Cu + Cs, G = C >> 3;
+  if (e >= Vf * G) return;
+  const int grp = (int)(e % G), c = 8 * grp;
+  long v = e / G;
+  uint4 w;
+  if (c >= Cu) {
+    w = *(const uint4 *)(skip + v * Cs + (c - Cu));
+  } else {
+    int x = (int)(v % df.X), y = (int)((v / df.X) % df.Y), z = (int)(v / ((long)df.X * df.Y));
+    long p = ((long)(z >> 1) * (df.Y >> 1) + (y >> 1)) * (df.X >> 1) + (x >> 1);
+    w = *(const uint4 *)(coarse + p * Cu + c);
+  }
+  *(uint4 *)(out + v * C + c) = w;
 }
 
 // gz = bf16(g * [y > 0]), g a channel slice [off, off + C) of a row of Cs floats
@@ -515,10 +598,11 @@ __global__ void k_pack_input(long V, const float *__restrict__ in5, bf16_t *__re
   x[e] = c < 5 ? f2bf(in5[v * 5 + c]) : (bf16_t)0;
 }
 
-// a crop of a bank sample: x [S^3][8], target, mask (fluid channel)
+// a crop of a bank sample: x [S^3][8], target, mask (fluid channel above
+// mth: a fluid cell, or on the coarsened grid a block with a fluid cell)
 __global__ void k_crop(NetDims full, NetDims cd, int oz, int oy, int ox, const bf16_t *__restrict__ sx,
                        const float *__restrict__ st, bf16_t *__restrict__ x, float *__restrict__ t,
-                       uint8_t *__restrict__ m) {
+                       uint8_t *__restrict__ m, float mth) {
   long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long Vc = (long)cd.Z * cd.Y * cd.X;
   if (v >= Vc) return;
@@ -527,7 +611,7 @@ __global__ void k_crop(NetDims full, NetDims cd, int oz, int oy, int ox, const b
   const uint4 *src = (const uint4 *)(sx + f * 8);
   *(uint4 *)(x + v * 8) = *src;
   t[v] = st[f];
-  m[v] = bf2f(sx[f * 8 + 2]) > 0.5f ? 1 : 0;
+  m[v] = bf2f(sx[f * 8 + 2]) > mth ? 1 : 0;
 }
 
 // ---- the solver side: inputs from the pressure system, the correction
@@ -599,6 +683,61 @@ __global__ void k_ws_inputs(long C, const uint8_t *__restrict__ ct, const float 
                  f2bf(t == SOLID ? 1.f : 0.f), fl ? f2bf(q[c] / sq) : (bf16_t)0, 0, 0, 0};
   for (int i = 0; i < 8; i++) x[c * 8 + i] = v[i];
 }
+// the five input channels of the grid coarsened 2x per axis: each coarse
+// cell averages its eight cells' full-resolution channels (r0/s and q/sq on
+// fluid cells, 0 elsewhere; the type one-hots, so types become fractions)
+__global__ void k_ws_inputs_coarse(NetDims cd, const uint8_t *__restrict__ ct, const float *__restrict__ r0,
+                                   const float *__restrict__ q, const float *__restrict__ scales,
+                                   bf16_t *__restrict__ x) {
+  long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long V = (long)cd.Z * cd.Y * cd.X;
+  if (v >= V) return;
+  const int X = 2 * cd.X, XY = 4 * cd.X * cd.Y;
+  const int cx = (int)(v % cd.X), cy = (int)((v / cd.X) % cd.Y), cz = (int)(v / ((long)cd.X * cd.Y));
+  const float s = scales[0], sq = scales[1];
+  float a[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int o = 0; o < 8; o++) {
+    long c = (long)(2 * cz + (o >> 2)) * XY + (long)(2 * cy + ((o >> 1) & 1)) * X + (2 * cx + (o & 1));
+    const uint8_t t = ct[c];
+    const bool fl = t == FLUID;
+    a[0] += fl ? r0[c] / s : 0.f;
+    a[1] += t == AIR ? 1.f : 0.f;
+    a[2] += fl ? 1.f : 0.f;
+    a[3] += t == SOLID ? 1.f : 0.f;
+    a[4] += fl ? q[c] / sq : 0.f;
+  }
+  for (int i = 0; i < 8; i++) x[v * 8 + i] = i < 5 ? f2bf(a[i] * 0.125f) : (bf16_t)0;
+}
+// q += s y(parent) on fluid cells: the coarse prediction, constant over
+// each 2x2x2 block
+__global__ void k_ws_apply_coarse(NetDims cd, const uint8_t *__restrict__ ct, const float *__restrict__ y,
+                                  const float *__restrict__ scales, float *__restrict__ q) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long C = 8L * cd.Z * cd.Y * cd.X;
+  if (c >= C || ct[c] != FLUID) return;
+  const int X = 2 * cd.X, Y = 2 * cd.Y;
+  const int x = (int)(c % X), yy = (int)((c / X) % Y), z = (int)(c / ((long)X * Y));
+  q[c] = q[c] + scales[0] * y[((long)(z >> 1) * cd.Y + (yy >> 1)) * cd.X + (x >> 1)];
+}
+// training target on the coarsened grid: the mean of (q* - q_prev) / s over
+// each block's fluid cells (0 for a block without one), the value that a
+// block-constant correction fits best in least squares
+__global__ void k_ws_target_coarse(NetDims cd, const uint8_t *__restrict__ ct, const float *__restrict__ q,
+                                   const float *__restrict__ qprev, const float *__restrict__ scales,
+                                   float *__restrict__ t) {
+  long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long V = (long)cd.Z * cd.Y * cd.X;
+  if (v >= V) return;
+  const int X = 2 * cd.X, XY = 4 * cd.X * cd.Y;
+  const int cx = (int)(v % cd.X), cy = (int)((v / cd.X) % cd.Y), cz = (int)(v / ((long)cd.X * cd.Y));
+  float sum = 0.f;
+  int n = 0;
+  for (int o = 0; o < 8; o++) {
+    long c = (long)(2 * cz + (o >> 2)) * XY + (long)(2 * cy + ((o >> 1) & 1)) * X + (2 * cx + (o & 1));
+    if (ct[c] == FLUID) { sum += q[c] - qprev[c]; n++; }
+  }
+  t[v] = n ? sum / (float)n / scales[0] : 0.f;
+}
 // q += s y on fluid cells (the prediction of the correction)
 __global__ void k_ws_apply(long C, const uint8_t *__restrict__ ct, const float *__restrict__ y,
                            const float *__restrict__ scales, float *__restrict__ q) {
@@ -621,6 +760,11 @@ __global__ void k_ws_target(long C, const uint8_t *__restrict__ ct, const float 
                                             void *);
 CONV(1, 0) CONV(2, 0) CONV(4, 0) CONV(1, 2) CONV(2, 2) CONV(3, 2) CONV(4, 2) CONV(6, 2)
 #undef CONV
+#define CONVS(NT, OUT)                                                                                     \
+  template __global__ void k_conv3s<NT, OUT, 4>(NetDims, int, int, int, const bf16_t *, const bf16_t *,     \
+                                                const float *, void *);
+CONVS(1, 0) CONVS(2, 0) CONVS(4, 0) CONVS(1, 2) CONVS(2, 2) CONVS(3, 2) CONVS(4, 2) CONVS(6, 2)
+#undef CONVS
 #define CONVL(NT, OUT, MT)                                                                                    \
   template __global__ void k_conv3l<NT, OUT, MT>(NetDims, int, int, int, const bf16_t *, const bf16_t *,     \
                                                  const float *, void *);
@@ -672,6 +816,12 @@ void conv(ws_ctx *c, NetDims d, int cin, int K, int flip, const bf16_t *in, cons
   }
   constexpr int MT = NT >= 3 ? 4 : 8;
   int tx = (d.X + 15) / 16, ty = (d.Y + 7) / 8, tz = (d.Z + MT / 2 - 1) / (MT / 2);
+  if ((long)tx * ty * tz < 256) {
+    // too few LDS tiles to fill the GPU: four waves per 16 voxels, K split
+    hipLaunchKernelGGL((k_conv3s<NT, OUT, 4>), dim3(blocks_for(V, 16)), dim3(256), 0, c->stream, d, cin, K, flip, in,
+                       W, b, out);
+    return;
+  }
   hipLaunchKernelGGL((k_conv3l<NT, OUT, MT>), dim3(tx * ty * tz), dim3(256), 0, c->stream, d, cin, K, flip, in, W, b,
                      out);
 }
@@ -828,17 +978,17 @@ int net_forward(ws_ctx *c, NetDims d) {
   long V = vox(d), Vh = vox(h), Vq = vox(q);
   conv_fwd(c, 0, d, n->x0, n->a0);
   conv_fwd(c, 1, d, n->a0, n->s1);
-  hipLaunchKernelGGL(k_pool, dim3(blocks_for(Vh * 16, 256)), dim3(256), 0, c->stream, h, 16, n->s1, n->p1);
+  hipLaunchKernelGGL(k_pool, dim3(blocks_for(Vh * 2, 256)), dim3(256), 0, c->stream, h, 16, n->s1, n->p1);
   conv_fwd(c, 2, h, n->p1, n->a2);
   conv_fwd(c, 3, h, n->a2, n->s2);
-  hipLaunchKernelGGL(k_pool, dim3(blocks_for(Vq * 32, 256)), dim3(256), 0, c->stream, q, 32, n->s2, n->p2);
+  hipLaunchKernelGGL(k_pool, dim3(blocks_for(Vq * 4, 256)), dim3(256), 0, c->stream, q, 32, n->s2, n->p2);
   conv_fwd(c, 4, q, n->p2, n->a4);
   conv_fwd(c, 5, q, n->a4, n->a5);
-  hipLaunchKernelGGL(k_up_concat, dim3(blocks_for(Vh * 96, 256)), dim3(256), 0, c->stream, h, 64, 32, n->a5, n->s2,
+  hipLaunchKernelGGL(k_up_concat, dim3(blocks_for(Vh * 12, 256)), dim3(256), 0, c->stream, h, 64, 32, n->a5, n->s2,
                      n->u1);
   conv_fwd(c, 6, h, n->u1, n->a6);
   conv_fwd(c, 7, h, n->a6, n->a7);
-  hipLaunchKernelGGL(k_up_concat, dim3(blocks_for(V * 48, 256)), dim3(256), 0, c->stream, d, 32, 16, n->a7, n->s1,
+  hipLaunchKernelGGL(k_up_concat, dim3(blocks_for(V * 6, 256)), dim3(256), 0, c->stream, d, 32, 16, n->a7, n->s1,
                      n->u2);
   conv_fwd(c, 8, d, n->u2, n->a8);
   conv_fwd(c, 9, d, n->a8, n->a9);
@@ -949,6 +1099,11 @@ static int net_loss(ws_ctx *c, NetDims d, double *loss) {
 
 // ---- the solver side
 static NetDims grid_dims(ws_ctx *c) { return NetDims{c->nz, c->ny, c->nx}; }
+// the grid the network sees: the solver's, or with netCoarse the solver's
+// coarsened 2x per axis
+static NetDims net_grid(ws_ctx *c, int coarse) {
+  return coarse ? NetDims{c->nz / 2, c->ny / 2, c->nx / 2} : grid_dims(c);
+}
 
 static int solver_inputs(ws_ctx *c) {
   Net *n = c->net;
@@ -957,15 +1112,26 @@ static int solver_inputs(ws_ctx *c) {
   hipLaunchKernelGGL(k_ws_residual, dim3(1024), dim3(256), 0, c->stream, d, c->cellType, c->nbm, c->dg, c->rhs, c->q,
                      n->r0, n->rpart);
   hipLaunchKernelGGL(k_ws_scales, dim3(1), dim3(256), 0, c->stream, 1024, n->rpart, n->scales);
-  hipLaunchKernelGGL(k_ws_inputs, dim3(blocks_for(C, 256)), dim3(256), 0, c->stream, C, c->cellType, n->r0, c->q,
-                     n->scales, n->x0);
+  if (c->netCoarse) {
+    NetDims cd = net_grid(c, 1);
+    hipLaunchKernelGGL(k_ws_inputs_coarse, dim3(blocks_for(vox(cd), 256)), dim3(256), 0, c->stream, cd, c->cellType,
+                       n->r0, c->q, n->scales, n->x0);
+  } else {
+    hipLaunchKernelGGL(k_ws_inputs, dim3(blocks_for(C, 256)), dim3(256), 0, c->stream, C, c->cellType, n->r0, c->q,
+                       n->scales, n->x0);
+  }
   WS_LAUNCHED();
   return WS_OK;
 }
 
 static int solver_side(ws_ctx *c) {
   int r;
-  if ((r = net_ensure(c)) || (r = dims_ok(c, grid_dims(c))) || (r = net_buffers(c, c->nC))) return r;
+  if (c->netCoarse && ((c->nx | c->ny | c->nz) & 1)) {
+    c->err = "netCoarse: the grid sides must be even";
+    return WS_EINVAL;
+  }
+  NetDims d = net_grid(c, c->netCoarse);
+  if ((r = net_ensure(c)) || (r = dims_ok(c, d)) || (r = net_buffers(c, vox(d)))) return r;
   Net *n = c->net;
   if (!n->r0 && ((r = nalloc(c, n, &n->r0, c->nC)) || (r = nalloc(c, n, &n->qprev, c->nC)))) return r;
   return WS_OK;
@@ -973,10 +1139,14 @@ static int solver_side(ws_ctx *c) {
 
 int net_warm_start(ws_ctx *c) {
   int r;
-  if ((r = solver_side(c)) || (r = solver_inputs(c)) || (r = net_forward(c, grid_dims(c)))) return r;
+  if ((r = solver_side(c)) || (r = solver_inputs(c)) || (r = net_forward(c, net_grid(c, c->netCoarse)))) return r;
   Net *n = c->net;
-  hipLaunchKernelGGL(k_ws_apply, dim3(blocks_for(c->nC, 256)), dim3(256), 0, c->stream, c->nC, c->cellType, n->out,
-                     n->scales, c->q);
+  if (c->netCoarse)
+    hipLaunchKernelGGL(k_ws_apply_coarse, dim3(blocks_for(c->nC, 256)), dim3(256), 0, c->stream, net_grid(c, 1),
+                       c->cellType, n->out, n->scales, c->q);
+  else
+    hipLaunchKernelGGL(k_ws_apply, dim3(blocks_for(c->nC, 256)), dim3(256), 0, c->stream, c->nC, c->cellType, n->out,
+                       n->scales, c->q);
   WS_LAUNCHED();
   return WS_OK;
 }
@@ -986,19 +1156,29 @@ int net_record_inputs(ws_ctx *c) {
   if ((r = solver_side(c))) return r;
   Net *n = c->net;
   if (n->bankCap < 1) { c->err = "recordSamples: no sample bank (ws_net_bank)"; return WS_ESTATE; }
+  if (n->bankCoarse != c->netCoarse) {
+    c->err = "recordSamples: the sample bank was made for the other netCoarse";
+    return WS_ESTATE;
+  }
   if ((r = solver_inputs(c))) return r;
-  long C = c->nC;
-  WS_HIP(hipMemcpyAsync(n->bankX + (long)n->bankNext * C * 8, n->x0, sizeof(bf16_t) * C * 8, hipMemcpyDeviceToDevice,
+  const long V = vox(net_grid(c, c->netCoarse));
+  WS_HIP(hipMemcpyAsync(n->bankX + (long)n->bankNext * V * 8, n->x0, sizeof(bf16_t) * V * 8, hipMemcpyDeviceToDevice,
                         c->stream));
-  WS_HIP(hipMemcpyAsync(n->qprev, c->q, sizeof(float) * C, hipMemcpyDeviceToDevice, c->stream));
+  WS_HIP(hipMemcpyAsync(n->qprev, c->q, sizeof(float) * c->nC, hipMemcpyDeviceToDevice, c->stream));
   return WS_OK;
 }
 
 int net_record_target(ws_ctx *c) {
   Net *n = c->net;
   long C = c->nC;
-  hipLaunchKernelGGL(k_ws_target, dim3(blocks_for(C, 256)), dim3(256), 0, c->stream, C, c->cellType, c->q, n->qprev,
-                     n->scales, n->bankT + (long)n->bankNext * C);
+  if (n->bankCoarse) {
+    NetDims cd = net_grid(c, 1);
+    hipLaunchKernelGGL(k_ws_target_coarse, dim3(blocks_for(vox(cd), 256)), dim3(256), 0, c->stream, cd, c->cellType,
+                       c->q, n->qprev, n->scales, n->bankT + (long)n->bankNext * vox(cd));
+  } else {
+    hipLaunchKernelGGL(k_ws_target, dim3(blocks_for(C, 256)), dim3(256), 0, c->stream, C, c->cellType, c->q, n->qprev,
+                       n->scales, n->bankT + (long)n->bankNext * C);
+  }
   WS_LAUNCHED();
   n->bankNext = (n->bankNext + 1) % n->bankCap;
   if (n->bankCount < n->bankCap) n->bankCount++;
@@ -1093,10 +1273,11 @@ int ws_net_bank(ws_ctx *c, int capacity) {
   n->bankX = nullptr;
   n->bankT = nullptr;
   n->bankCap = n->bankCount = n->bankNext = 0;
-  if ((r = nalloc(c, n, &n->bankX, (size_t)capacity * c->nC * 8)) ||
-      (r = nalloc(c, n, &n->bankT, (size_t)capacity * c->nC)))
+  const long V = vox(net_grid(c, c->netCoarse));
+  if ((r = nalloc(c, n, &n->bankX, (size_t)capacity * V * 8)) || (r = nalloc(c, n, &n->bankT, (size_t)capacity * V)))
     return r;
   n->bankCap = capacity;
+  n->bankCoarse = c->netCoarse;
   return WS_OK;
 }
 
@@ -1110,10 +1291,14 @@ int ws_net_train(ws_ctx *c, int steps, int batch, int crop, double lr, unsigned 
   if (!c || steps < 0 || batch < 1 || crop < 4 || (crop & 3)) return WS_EINVAL;
   Net *n = c->net;
   if (!n || n->bankCount < 1) { c->err = "ws_net_train: no recorded samples"; return WS_ESTATE; }
-  if (crop > c->nx || crop > c->ny || crop > c->nz) { c->err = "ws_net_train: crop larger than the grid"; return WS_EINVAL; }
-  NetDims cd{crop, crop, crop}, full = grid_dims(c);
+  NetDims cd{crop, crop, crop}, full = net_grid(c, n->bankCoarse);
+  if (crop > full.X || crop > full.Y || crop > full.Z) {
+    c->err = "ws_net_train: crop larger than the network's grid";
+    return WS_EINVAL;
+  }
+  const long Vs = vox(full);  // one bank sample
   int r;
-  if ((r = net_buffers(c, c->nC > vox(cd) ? c->nC : vox(cd)))) return r;
+  if ((r = net_buffers(c, Vs > vox(cd) ? Vs : vox(cd)))) return r;
   long Vc = vox(cd);
   uint32_t s = seed * 2654435761u + 12345u;
   auto rnd = [&](int m) { s = s * 1664525u + 1013904223u; return (int)((s >> 8) % (uint32_t)m); };
@@ -1126,7 +1311,8 @@ int ws_net_train(ws_ctx *c, int steps, int batch, int crop, double lr, unsigned 
       int oz = 4 * rnd((full.Z - crop) / 4 + 1), oy = 4 * rnd((full.Y - crop) / 4 + 1);
       int ox = 4 * rnd((full.X - crop) / 4 + 1);
       hipLaunchKernelGGL(k_crop, dim3(blocks_for(Vc, 256)), dim3(256), 0, c->stream, full, cd, oz, oy, ox,
-                         n->bankX + (long)smp * c->nC * 8, n->bankT + (long)smp * c->nC, n->x0, n->tgt, n->mask);
+                         n->bankX + (long)smp * Vs * 8, n->bankT + (long)smp * Vs, n->x0, n->tgt, n->mask,
+                         n->bankCoarse ? 0.0625f : 0.5f);
       WS_LAUNCHED();
       double l = 0;
       if ((r = net_forward(c, cd)) || (r = net_loss(c, cd, &l)) || (r = net_backward(c, cd, bi > 0))) return r;

@@ -5,7 +5,8 @@ Pairs come from plain runs of the planet substep: for each training seed a
 world is built, the solver runs `--warmup` substeps, then `--record`
 substeps whose pressure solves go to a tight residual (`--tight`) with
 recording on, so every solve stores (inputs, converged correction) in the
-device sample bank. Training (Adam, random crops of the recorded grids) is
+device sample bank. With --coarse-grid 1 (the default) the network sees the
+grid coarsened 2x per axis and learns the block mean of the correction. Training (Adam, random crops of the recorded grids) is
 interleaved with recording. The weights are written as raw float32 in the
 flat order of include/watersim_learn.h.
 
@@ -49,6 +50,9 @@ def main():
     ap.add_argument("--final", type=int, default=600, help="updates after the last seed")
     ap.add_argument("--coarse", type=float, default=1.0,
                     help="weight of the loss term on the 4x4x4-averaged error (the smooth modes SOR is slow on)")
+    ap.add_argument("--coarse-grid", type=int, default=1,
+                    help="1: the network sees the grid coarsened 2x per axis and predicts a correction constant "
+                         "over 2x2x2 blocks (netCoarse)")
     ap.add_argument("--init", default=None, help="start from these weights")
     ap.add_argument("--out", required=True)
     a = ap.parse_args()
@@ -68,6 +72,7 @@ def main():
             w0 = np.fromfile(a.init, dtype=np.float32) if a.init else init_weights(
                 np.random.default_rng(a.seed0), len(s.net_params()))
             s.set_net_params(w0)
+            s.set(netCoarse=a.coarse_grid)
             s.net_bank(a.bank)
             s.set(netLossCoarse=a.coarse)
         watersim_hip.load_world(s, world)
@@ -92,7 +97,9 @@ def main():
     w = s.net_params()
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     w.astype(np.float32).tofile(a.out)
-    meta = {"pairs": pairs, "seeds": [a.seed0 + k for k in range(a.seeds)], "grid": a.n, "crop": a.crop,
+    meta = {"pairs": pairs, "net_coarse": a.coarse_grid, "seeds": [a.seed0 + k for k in range(a.seeds)],
+            "grid": a.n, "crop": a.crop, "bank": a.bank, "train_every": a.train_every, "burst": a.burst,
+            "final": a.final, "record_per_seed": a.record, "warmup": a.warmup,
             "batch": a.batch, "lr": a.lr, "tight_residual": a.tight, "loss_history": history,
             "seconds": time.time() - t0}
     with open(os.path.splitext(a.out)[0] + ".json", "w") as f:

@@ -300,6 +300,20 @@ class Solver:
         self._check(self.L.ws_net_set_params(self.ctx, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(w)),
                     "net_set_params")
 
+    def load_net_weights(self, path):
+        """Trained weights as written by train_warmstart.py: the flat float32
+        vector and, beside it, its .json record, whose "net_coarse" says on
+        which grid the network was trained (1: the solver's grid coarsened
+        2x per axis); the context is set to match."""
+        self.set_net_params(np.fromfile(path, dtype=np.float32))
+        meta = os.path.splitext(path)[0] + ".json"
+        coarse = 0
+        if os.path.exists(meta):
+            import json
+            with open(meta) as f:
+                coarse = int(json.load(f).get("net_coarse", 0))
+        self.set(netCoarse=coarse)
+
     def net_forward(self, inp):
         """inp [Z, Y, X, 5] -> out [Z, Y, X]"""
         inp = np.ascontiguousarray(inp, np.float32)
