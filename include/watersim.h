@@ -84,7 +84,8 @@ const char *ws_last_error(const ws_ctx *c);
  * "nP", "dv", "tn", "pressureTol", "maxPressureIters", "iso"; and the
  * kernel choices of this library (no reference counterpart; each gives the
  * same results): "sorMode", "sorThreads", "netTiled", "p2gBin" (0..3: P2G
- * bins of 8x8x8, 4x4x4, 16x8x8, 8x8x4 cells).
+ * bins of 8x8x8, 4x4x4, 16x8x8, 8x8x4 cells); the warm-start switches
+ * "warmStart", "recordSamples", "netCoarse" (include/watersim_learn.h).
  * ws_get_f also reads results: "umax", "dtLast", "residual",
  * "pressureIters_done", "airborneCount", "nFluid", "dfRefined". */
 int ws_set_f(ws_ctx *c, const char *name, double v);

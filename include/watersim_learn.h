@@ -23,6 +23,14 @@
  *   "recordSamples" 1: every pressure solve stores (inputs, target) in the
  *                   sample bank, the target being the converged correction
  *                   (q_solved - q_previous) / scale
+ *   "netCoarse"     1: the network sees the solver's grid coarsened 2x per
+ *                   axis (even sides): each input channel averaged over a
+ *                   2x2x2 block (the one-hots become type fractions), and
+ *                   its output is added to q constant over each block's
+ *                   fluid cells; recorded targets are the block means of
+ *                   the correction over fluid cells. A sample bank holds
+ *                   samples of the netCoarse it was made with. The shipped
+ *                   weights record theirs in their .json ("net_coarse").
  */
 #ifndef WATERSIM_LEARN_H
 #define WATERSIM_LEARN_H
@@ -47,11 +55,12 @@ int ws_net_forward(ws_ctx *c, int nz, int ny, int nx, const float *in5, float *o
 int ws_net_grad(ws_ctx *c, int nz, int ny, int nx, const float *in5, const float *target, const uint8_t *mask,
                 float *grad, double *loss);
 /* Allocates a device bank for `capacity` recorded samples of the context's
- * grid (recording overwrites the oldest once full). */
+ * grid, or of its coarsened grid with netCoarse 1 (recording overwrites the
+ * oldest once full). */
 int ws_net_bank(ws_ctx *c, int capacity);
 int ws_net_bank_count(ws_ctx *c, int *count);
-/* Adam training on random crops (crop^3 cells, crop a multiple of 4) of
- * recorded samples: `steps` updates of `batch` crops each; *loss is the
+/* Adam training on random crops (crop^3 cells of the bank's grid, crop a
+ * multiple of 4) of recorded samples: `steps` updates of `batch` crops each; *loss is the
  * mean loss of the last update's batch. */
 int ws_net_train(ws_ctx *c, int steps, int batch, int crop, double lr, unsigned seed, double *loss);
 
