@@ -470,15 +470,9 @@ __global__ __launch_bounds__(256) void k_sor(Geo g, float *__restrict__ q, const
   int c = rowi * g.nx + i;
   unsigned m = nbm[c];
   if (!(m & 0x40)) return;
-  int nxy = g.nx * g.ny;
-  float sum = 0.f;
-  if (m & 1) sum += q[c - 1];
-  if (m & 2) sum += q[c + 1];
-  if (m & 4) sum += q[c - g.nx];
-  if (m & 8) sum += q[c + g.nx];
-  if (m & 16) sum += q[c - nxy];
-  if (m & 32) sum += q[c + nxy];
-  q[c] = sor_relax(q[c], sum, rhs[c], pw[c], om1);
+  const float rh = rhs[c], w = pw[c], qc = q[c];
+  const float sum = nb_sum(q, c, m, g.nx, g.nx * g.ny);
+  q[c] = sor_relax(qc, sum, rh, w, om1);
 }
 
 // One full red-black sweep per launch, tiled through LDS. A block owns a
@@ -625,18 +619,15 @@ __global__ __launch_bounds__(256) void k_sor_list(Geo g, int n, const unsigned *
   int b = xcd_block(blockIdx.x, gridDim.x);
   int t = b * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  unsigned e = code[t];
-  int c = (int)(e & 0x3ffffffu);
-  unsigned m = e >> 26;
-  int nxy = g.nx * g.ny;
-  float sum = 0.f;
-  if (m & 1) sum += q[c - 1];
-  if (m & 2) sum += q[c + 1];
-  if (m & 4) sum += q[c - g.nx];
-  if (m & 8) sum += q[c + g.nx];
-  if (m & 16) sum += q[c - nxy];
-  if (m & 32) sum += q[c + nxy];
-  q[c] = sor_relax(q[c], sum, lrhs[t], lpw[t], om1);
+  // the code, rhs and weight in one round trip, then the cell and its six
+  // neighbours in one more
+  const unsigned e = code[t];
+  const float rh = lrhs[t], w = lpw[t];
+  const int c = (int)(e & 0x3ffffffu);
+  const unsigned m = e >> 26;
+  const float qc = q[c];
+  const float sum = nb_sum(q, c, m, g.nx, g.nx * g.ny);
+  q[c] = sor_relax(qc, sum, rh, w, om1);
 }
 
 // ||rhs - A q||^2 and ||rhs||^2 over the two lists (red, then black), FP64
@@ -655,13 +646,7 @@ __global__ __launch_bounds__(256) void k_residual_list(Geo g, const DevScalars *
     unsigned e = code[pos];
     int c = (int)(e & 0x3ffffffu);
     unsigned m = e >> 26;
-    float sum = 0.f;
-    if (m & 1) sum += q[c - 1];
-    if (m & 2) sum += q[c + 1];
-    if (m & 4) sum += q[c - g.nx];
-    if (m & 8) sum += q[c + g.nx];
-    if (m & 16) sum += q[c - nxy];
-    if (m & 32) sum += q[c + nxy];
+    const float sum = nb_sum(q, c, m, g.nx, nxy);
     double rh = (double)lrhs[pos];
     double r = rh - ((double)ldg[pos] * (double)q[c] - (double)sum);
     rr += r * r;
@@ -928,13 +913,7 @@ __global__ __launch_bounds__(256) void k_residual(Geo g, const float *__restrict
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.nC; c += gridDim.x * blockDim.x) {
     unsigned m = nbm[c];
     if (!(m & 0x40)) continue;
-    float sum = 0.f;
-    if (m & 1) sum += q[c - 1];
-    if (m & 2) sum += q[c + 1];
-    if (m & 4) sum += q[c - g.nx];
-    if (m & 8) sum += q[c + g.nx];
-    if (m & 16) sum += q[c - nxy];
-    if (m & 32) sum += q[c + nxy];
+    const float sum = nb_sum(q, c, m, g.nx, nxy);
     double r = (double)rhs[c] - ((double)dg[c] * (double)q[c] - (double)sum);
     rr += r * r;
     bb += (double)rhs[c] * (double)rhs[c];

@@ -383,6 +383,25 @@ __device__ __forceinline__ float stencil_eval(const Stencil &st, const float *__
 // Logical block index with consecutive chunks of the grid on one XCD: the
 // hardware deals workgroups round-robin over the 8 XCDs, so block b runs on
 // XCD b%8; remapping gives each XCD (and its L2) a contiguous slab.
+// The sum of the fluid neighbours' q in the order -i, +i, -j, +j, -k, +k
+// (mask bits 0..5), every load issued unconditionally so that all six are in
+// flight together: an absent neighbour loads the cell itself and adds +0.
+// The sum starts at +0 and so never holds -0, and adding +0 to it leaves its
+// bits unchanged: the result is that of skipping the absent neighbours.
+__device__ __forceinline__ float nb_sum(const float *__restrict__ q, int c, unsigned m, int nx, int nxy) {
+  const float a = q[(m & 1u) ? c - 1 : c], b = q[(m & 2u) ? c + 1 : c];
+  const float cc = q[(m & 4u) ? c - nx : c], d = q[(m & 8u) ? c + nx : c];
+  const float e = q[(m & 16u) ? c - nxy : c], f = q[(m & 32u) ? c + nxy : c];
+  float sum = 0.f;
+  sum += (m & 1u) ? a : 0.f;
+  sum += (m & 2u) ? b : 0.f;
+  sum += (m & 4u) ? cc : 0.f;
+  sum += (m & 8u) ? d : 0.f;
+  sum += (m & 16u) ? e : 0.f;
+  sum += (m & 32u) ? f : 0.f;
+  return sum;
+}
+
 __device__ __forceinline__ int xcd_block(int b, int nb) {
   const int X = 8;
   int per = nb / X, rem = nb % X, xcd = b % X, slot = b / X;

@@ -627,13 +627,7 @@ __global__ void k_ws_residual(NetDims d, const uint8_t *__restrict__ ct, const u
   for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < C; c += (long)gridDim.x * blockDim.x) {
     unsigned mk = nbm[c];
     if (!(mk & 0x40)) { r0[c] = 0.f; continue; }
-    float sum = 0.f;
-    if (mk & 1) sum += q[c - 1];
-    if (mk & 2) sum += q[c + 1];
-    if (mk & 4) sum += q[c - nx];
-    if (mk & 8) sum += q[c + nx];
-    if (mk & 16) sum += q[c - nxy];
-    if (mk & 32) sum += q[c + nxy];
+    const float sum = nb_sum(q, c, mk, nx, nxy);
     float r = rhs[c] - ((float)dg[c] * q[c] - sum);
     r0[c] = r;
     a += (double)r * r;
