@@ -330,7 +330,11 @@ def test_p2g_bins_agree(scene):
         s.close()
     for b in range(1, 4):
         for k in out[0]:
-            assert np.array_equal(out[b][k], out[0][k]), (b, k)
+            d = np.nonzero(out[b][k] != out[0][k])[0]
+            if len(d):
+                print("  bin %d %s: %d faces differ, first %s: %s vs %s (golden %s)" % (
+                    b, k, len(d), d[:6], out[b][k][d[:6]], out[0][k][d[:6]], g["p2g_" + k][d[:6]]))
+            assert len(d) == 0, (b, k)
     from gpu_scenes import watersim_hip
     s = load_gpu(g)
     with pytest.raises(watersim_hip.WatersimError):

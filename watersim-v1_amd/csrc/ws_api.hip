@@ -40,16 +40,28 @@ std::vector<float> floor_thresholds(double dx, int n) {
   return t;
 }
 
+__global__ void k_zero_words(unsigned *p, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// Device memory of n elements, zeroed on the context's stream, so that the
+// zeros land before any later work of that stream. (A hipMemset would run on
+// the null stream, which the context's non-blocking stream does not wait
+// for: a kernel or copy queued right after could be overwritten by it.)
 template <class T>
 int dalloc(ws_ctx *c, T **p, size_t n) {
-  hipError_t e = hipMalloc((void **)p, sizeof(T) * (n ? n : 1));
+  const size_t bytes = sizeof(T) * (n ? n : 1), words = (bytes + 3) / 4;
+  hipError_t e = hipMalloc((void **)p, words * 4);
   if (e != hipSuccess) {
     c->err = std::string("hipMalloc: ") + hipGetErrorString(e);
     return WS_EHIP;
   }
-  e = hipMemset(*p, 0, sizeof(T) * (n ? n : 1));
+  hipLaunchKernelGGL(k_zero_words, dim3(blocks_for((long)words, 256)), dim3(256), 0, c->stream, (unsigned *)*p,
+                     (long)words);
+  e = hipGetLastError();
   if (e != hipSuccess) {
-    c->err = std::string("hipMemset: ") + hipGetErrorString(e);
+    c->err = std::string("zeroing a new buffer: ") + hipGetErrorString(e);
     return WS_EHIP;
   }
   return WS_OK;
@@ -916,6 +928,9 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
   c->scanTmpBytes = b1 > b2 ? b1 : b2;
   if (b3 > c->scanTmpBytes) c->scanTmpBytes = b3;
   if (hipMalloc(&c->scanTmp, c->scanTmpBytes) != hipSuccess) { ws_destroy(c); return WS_EHIP; }
+  // the zeroing of every buffer above (on the context's stream) is done
+  // before the host copies below (on the null stream) write into some of them
+  if (hipStreamSynchronize(c->stream) != hipSuccess) { ws_destroy(c); return WS_EHIP; }
   DevScalars init{};
   init.gRamp = c->gRamp;
   uint32_t rng[4] = {0x2f6e2b1u, 0x7f4a7c15u, 0x51ab3c77u, 0u};  // _cfS, _bfS, _evS (solver.js:151-153)
@@ -1130,7 +1145,9 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   if (!strcmp(n, "pressureIters_done")) { *v = c->itersDone; return WS_OK; }
   if (!strcmp(n, "rngCfS") || !strcmp(n, "rngBfS") || !strcmp(n, "rngEvS")) {
     uint32_t rng[4];
-    WS_HIP(hipMemcpy(rng, c->rng, sizeof(rng), hipMemcpyDeviceToHost));
+    // on the context's stream: ordered after its queued substeps
+    WS_HIP(hipMemcpyAsync(rng, c->rng, sizeof(rng), hipMemcpyDeviceToHost, c->stream));
+    WS_HIP(hipStreamSynchronize(c->stream));
     *v = n[4] == 'C' ? rng[0] : (n[4] == 'B' ? rng[1] : rng[2]);
     return WS_OK;
   }
@@ -1160,7 +1177,8 @@ int ws_upload(ws_ctx *c, const char *name, const void *host, size_t nbytes) {
   }
   if (!strcmp(name, "rng")) {
     if (nbytes != 12) return WS_EINVAL;
-    WS_HIP(hipMemcpy(c->rng, host, 12, hipMemcpyHostToDevice));
+    WS_HIP(hipMemcpyAsync(c->rng, host, 12, hipMemcpyHostToDevice, c->stream));
+    WS_HIP(hipStreamSynchronize(c->stream));
     return WS_OK;
   }
   if (!field_of(c, name, f)) { c->err = std::string("unknown field ") + name; return WS_EINVAL; }
@@ -1185,7 +1203,8 @@ int ws_download(ws_ctx *c, const char *name, void *host, size_t nbytes) {
   if (!c || !name || (!host && nbytes)) return WS_EINVAL;
   if (!strcmp(name, "rng")) {
     if (nbytes != 12) return WS_EINVAL;
-    WS_HIP(hipMemcpy(host, c->rng, 12, hipMemcpyDeviceToHost));
+    WS_HIP(hipMemcpyAsync(host, c->rng, 12, hipMemcpyDeviceToHost, c->stream));
+    WS_HIP(hipStreamSynchronize(c->stream));
     return WS_OK;
   }
   Field f;

@@ -383,27 +383,42 @@ __global__ void k_visc_iter(Geo g, const uint8_t *__restrict__ ct, const float *
   if (f >= F) return;
   FaceInfo fi;
   face_of(g, f, fi);
-  float u0 = old[f];
-  if (fi.cl < 0 || fi.cr < 0 || ct[fi.cl] != FLUID || ct[fi.cr] != FLUID) { out[f] = u0; return; }
   int d0 = fi.comp == 0 ? g.nx + 1 : g.nx, d1 = fi.comp == 1 ? g.ny + 1 : g.ny, d2 = fi.comp == 2 ? g.nz + 1 : g.nz;
   long s1 = d0, s2 = (long)d0 * d1;
   int nxy = g.nx * g.ny;
-  // is the lower neighbour (same component) also a fluid-fluid face?
-  auto ffLower = [&](int dir) -> bool {
-    // the lower neighbour along the face normal is a domain plane at index 0
-    if (dir == fi.comp && (dir == 0 ? fi.i : dir == 1 ? fi.j : fi.k) - 1 == 0) return false;
-    // otherwise its cells are this face's cells shifted by -1 along dir
-    int step = dir == 0 ? 1 : (dir == 1 ? g.nx : nxy);
-    return ct[fi.cl - step] == FLUID && ct[fi.cr - step] == FLUID;
-  };
+  // Every load of the face is issued before any is used (indices that do
+  // not exist are replaced by the face itself or cell 0), so they are in
+  // flight together; the selections below reproduce the branches exactly.
+  const bool ffValid = fi.cl >= 0 && fi.cr >= 0;
+  const int cl = ffValid ? fi.cl : 0, cr = ffValid ? fi.cr : 0;
+  const bool hasI0 = fi.i - 1 >= 0, hasI1 = fi.i + 1 < d0, hasJ0 = fi.j - 1 >= 0, hasJ1 = fi.j + 1 < d1;
+  const bool hasK0 = fi.k - 1 >= 0, hasK1 = fi.k + 1 < d2;
+  // the lower neighbour along dir is also a fluid-fluid face: its cells are
+  // this face's cells shifted by -1 along dir, except that the lower
+  // neighbour along the face normal at index 1 is a domain plane
+  const bool lowI = ffValid && hasI0 && !(fi.comp == 0 && fi.i - 1 == 0);
+  const bool lowJ = ffValid && hasJ0 && !(fi.comp == 1 && fi.j - 1 == 0);
+  const bool lowK = ffValid && hasK0 && !(fi.comp == 2 && fi.k - 1 == 0);
+  const float u0 = old[f];
+  const uint8_t tl = ct[cl], tr = ct[cr];
+  const uint8_t tlI = ct[lowI ? cl - 1 : 0], trI = ct[lowI ? cr - 1 : 0];
+  const uint8_t tlJ = ct[lowJ ? cl - g.nx : 0], trJ = ct[lowJ ? cr - g.nx : 0];
+  const uint8_t tlK = ct[lowK ? cl - nxy : 0], trK = ct[lowK ? cr - nxy : 0];
+  const float oI0 = old[hasI0 ? f - 1 : f], oI1 = old[hasI1 ? f + 1 : f];
+  const float oJ0 = old[hasJ0 ? f - s1 : f], oJ1 = old[hasJ1 ? f + s1 : f];
+  const float oK0 = old[hasK0 ? f - s2 : f], oK1 = old[hasK1 ? f + s2 : f];
+  const float cI0 = cur[hasI0 ? f - 1 : f], cJ0 = cur[hasJ0 ? f - s1 : f], cK0 = cur[hasK0 ? f - s2 : f];
+  if (!ffValid || tl != FLUID || tr != FLUID) { out[f] = u0; return; }
+  const bool ffI = lowI && tlI == FLUID && trI == FLUID;
+  const bool ffJ = lowJ && tlJ == FLUID && trJ == FLUID;
+  const bool ffK = lowK && tlK == FLUID && trK == FLUID;
   float val = u0 * 6.f;
-  // -i
-  if (fi.i - 1 >= 0) val -= (!first && ffLower(0)) ? cur[f - 1] : old[f - 1]; else val -= u0;
-  if (fi.i + 1 < d0) val -= old[f + 1]; else val -= u0;
-  if (fi.j - 1 >= 0) val -= (!first && ffLower(1)) ? cur[f - s1] : old[f - s1]; else val -= u0;
-  if (fi.j + 1 < d1) val -= old[f + s1]; else val -= u0;
-  if (fi.k - 1 >= 0) val -= (!first && ffLower(2)) ? cur[f - s2] : old[f - s2]; else val -= u0;
-  if (fi.k + 1 < d2) val -= old[f + s2]; else val -= u0;
+  val -= hasI0 ? ((!first && ffI) ? cI0 : oI0) : u0;
+  val -= hasI1 ? oI1 : u0;
+  val -= hasJ0 ? ((!first && ffJ) ? cJ0 : oJ0) : u0;
+  val -= hasJ1 ? oJ1 : u0;
+  val -= hasK0 ? ((!first && ffK) ? cK0 : oK0) : u0;
+  val -= hasK1 ? oK1 : u0;
   out[f] = u0 + coef * val;
 }
 
@@ -997,26 +1012,20 @@ __device__ __forceinline__ void sor_half(const SorJob &J, long base, long lo, lo
   for (long t0 = lo + threadIdx.x; t0 < hi; t0 += (long)NT * kSorU) {
     unsigned e[kSorU];
     float rh[kSorU], pw[kSorU], sum[kSorU], qc[kSorU];
+    // every load unconditional, so that a thread's loads are in flight
+    // together: a slot past hi reads entry lo (its result is not written)
 #pragma unroll
     for (int u = 0; u < kSorU; u++) {
-      long t = t0 + (long)u * NT;
-      bool in = t < hi;
-      e[u] = in ? J.code[base + t] : 0u;
-      rh[u] = in ? J.lrhs[base + t] : 0.f;
-      pw[u] = in ? J.lpw[base + t] : 0.f;
+      const long t = t0 + (long)u * NT;
+      const long at = base + (t < hi ? t : lo);
+      e[u] = J.code[at];
+      rh[u] = J.lrhs[at];
+      pw[u] = J.lpw[at];
     }
 #pragma unroll
     for (int u = 0; u < kSorU; u++) {
-      int c = (int)(e[u] & 0x3ffffffu);
-      unsigned m = e[u] >> 26;
-      float a = 0.f;
-      if (m & 1) a += J.q[c - 1];
-      if (m & 2) a += J.q[c + 1];
-      if (m & 4) a += J.q[c - nx];
-      if (m & 8) a += J.q[c + nx];
-      if (m & 16) a += J.q[c - nxy];
-      if (m & 32) a += J.q[c + nxy];
-      sum[u] = a;
+      const int c = (int)(e[u] & 0x3ffffffu);
+      sum[u] = nb_sum(J.q, c, e[u] >> 26, nx, nxy);
       qc[u] = J.q[c];
     }
 #pragma unroll
@@ -1052,14 +1061,7 @@ __device__ bool sor_residual(const SorJob &J, int w, long nR, long nB, int nxy, 
         long pos = t < nR ? t : J.half + (t - nR);
         unsigned e = J.code[pos];
         int c = (int)(e & 0x3ffffffu);
-        unsigned m = e >> 26;
-        float a = 0.f;
-        if (m & 1) a += J.q[c - 1];
-        if (m & 2) a += J.q[c + 1];
-        if (m & 4) a += J.q[c - nx];
-        if (m & 8) a += J.q[c + nx];
-        if (m & 16) a += J.q[c - nxy];
-        if (m & 32) a += J.q[c + nxy];
+        const float a = nb_sum(J.q, c, e >> 26, nx, nxy);
         double rh = (double)J.lrhs[pos];
         double r = rh - ((double)J.ldg[pos] * (double)J.q[c] - (double)a);
         rr += r * r;

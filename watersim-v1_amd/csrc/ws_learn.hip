@@ -589,6 +589,11 @@ __global__ void k_adam(long n, float *__restrict__ w, const float *__restrict__ 
   w[e] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
 }
 
+__global__ void k_fill_f32(long n, float *__restrict__ p, float v) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) p[e] = v;
+}
+
 // host-supplied input channels (f32 [V][5]) -> bf16 [V][8]
 __global__ void k_pack_input(long V, const float *__restrict__ in5, bf16_t *__restrict__ x) {
   long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -903,9 +908,11 @@ int net_ensure(ws_ctx *c) {
       (r = nalloc(c, n, &n->m2, off)) || (r = nalloc(c, n, &n->sums, 4096)) || (r = nalloc(c, n, &n->scales, 4)) ||
       (r = nalloc(c, n, &n->rpart, 3 * 1024)))
     return r;
-  WS_HIP(hipMemset(n->w, 0, sizeof(float) * off));
-  WS_HIP(hipMemset(n->m1, 0, sizeof(float) * off));
-  WS_HIP(hipMemset(n->m2, 0, sizeof(float) * off));
+  // zeroed on the context's stream (a null-stream memset would not be
+  // ordered before the stream's later kernels)
+  for (float *z : {n->w, n->m1, n->m2})
+    hipLaunchKernelGGL(k_fill_f32, dim3(blocks_for(off, 256)), dim3(256), 0, c->stream, off, z, 0.f);
+  WS_LAUNCHED();
   for (int l = 0; l < kNetLayers - 1; l++) {
     NetLayer &L = n->L[l];
     if ((r = nalloc(c, n, &L.wf, (size_t)L.cout * L.K))) return r;
@@ -1202,7 +1209,10 @@ int ws_net_set_params(ws_ctx *c, const float *w, long n) {
   int r = net_ensure(c);
   if (r) return r;
   if (n != c->net->nparam) { c->err = "ws_net_set_params: wrong parameter count"; return WS_EINVAL; }
-  WS_HIP(hipMemcpy(c->net->w, w, sizeof(float) * n, hipMemcpyHostToDevice));
+  // on the context's stream, after the zeroing of a new network and any
+  // queued training
+  WS_HIP(hipMemcpyAsync(c->net->w, w, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+  WS_HIP(hipStreamSynchronize(c->stream));
   c->net->packed = false;
   return WS_OK;
 }
@@ -1212,7 +1222,8 @@ int ws_net_get_params(ws_ctx *c, float *w, long n) {
   int r = net_ensure(c);
   if (r) return r;
   if (n != c->net->nparam) { c->err = "ws_net_get_params: wrong parameter count"; return WS_EINVAL; }
-  WS_HIP(hipMemcpy(w, c->net->w, sizeof(float) * n, hipMemcpyDeviceToHost));
+  WS_HIP(hipMemcpyAsync(w, c->net->w, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+  WS_HIP(hipStreamSynchronize(c->stream));
   return WS_OK;
 }
 
