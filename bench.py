@@ -284,6 +284,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain solve pass (cfg3)")
     ap.add_argument("--weights", default=WEIGHTS, help="warm-start network weights (cfg4)")
+    ap.add_argument("--sor-mode", type=int, default=None,
+                    help="use this SOR kernel instead of the autotuner's choice (a profiler changes the timings the "
+                         "autotuner compares)")
+    ap.add_argument("--p2g-bin", type=int, default=None, help="use this P2G bin instead of the autotuner's choice")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times and exit: the "
                          "reference kernel whose FETCH_SIZE / WRITE_SIZE counters scale the HBM traffic")
@@ -320,6 +324,12 @@ def main():
         # (outside the timed region)
         tune_report = s.tune(64)
         log("tune: %s" % tune_report.replace("\n", "; "))
+    if a.sor_mode is not None:
+        s.set(sorMode=a.sor_mode)
+        tune_report = (tune_report + "; " if tune_report else "") + "sorMode %d set by --sor-mode" % a.sor_mode
+    if a.p2g_bin is not None:
+        s.set(p2gBin=a.p2g_bin)
+        tune_report = (tune_report + "; " if tune_report else "") + "p2gBin %d set by --p2g-bin" % a.p2g_bin
     # HIP events around the solver's SOR batches and pressure solves stay
     # queued on its stream and are read after the timed region (no added
     # host synchronization)
@@ -343,7 +353,7 @@ def main():
     launches = max(g["sorLaunches"], 1)
     sor_avg_s = (g["sorMs"] / 1e3) / launches
     cell_sweeps = g["sorCellSweeps"]
-    if mode in (2, 4):
+    if mode in (2, 4, 5):
         sor_bytes = 13.0 * g["sorCellSweeps"] / launches
     else:
         # the kernels over the whole grid count every cell; the algorithmic
@@ -354,8 +364,9 @@ def main():
         sor_bytes = 13.0 * per_launch
         cell_sweeps = g["sweepsTotal"] * nfl
     achieved = sor_bytes / sor_avg_s / 1e9 if g["sorLaunches"] else 0.0
-    kname = {4: "k_sor_solve", 3: "k_sor_tb", 2: "k_sor_list", 1: "k_sor_tile", 0: "k_sor"}[mode]
-    kdesc = {4: " (a whole pressure solve in one workgroup: its sweeps and residual checks)",
+    kname = {5: "k_sor_flow", 4: "k_sor_solve", 3: "k_sor_tb", 2: "k_sor_list", 1: "k_sor_tile", 0: "k_sor"}[mode]
+    kdesc = {5: " (a whole pressure solve on workgroups that wait only for their neighbours)",
+             4: " (a whole pressure solve in one workgroup: its sweeps and residual checks)",
              3: " (two red-black sweeps of the fluid cells per launch, LDS tiles)",
              2: " (the fluid cells of one colour)", 1: " (one full red-black sweep, LDS tile)",
              0: " (one colour of the whole grid; algorithmic bytes count its fluid cells)"}[mode]

@@ -314,7 +314,7 @@ __global__ void k_ct_vapor_groups(int n, const unsigned long long *__restrict__ 
 // touch its members), each thread taking every blockDim-th step in order
 // and polling for readiness a bounded number of times; the earliest pending
 // step is always ready, so every poll round completes at least one step.
-constexpr int kVcK = 32;  // partners kept per parcel (more: the group is swept serially)
+constexpr int kVcK = 63;  // partners kept per parcel (more: the group is swept serially); the wavefront step holds kVcK + 1 successors in its 64 lanes
 
 // later partners of each parcel: same group, later x rank, closer than thr
 // at the start of the pass, sorted by rank
