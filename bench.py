@@ -353,7 +353,7 @@ def main():
     launches = max(g["sorLaunches"], 1)
     sor_avg_s = (g["sorMs"] / 1e3) / launches
     cell_sweeps = g["sorCellSweeps"]
-    if mode in (2, 4, 5):
+    if mode in (2, 4, 5, 6):
         sor_bytes = 13.0 * g["sorCellSweeps"] / launches
     else:
         # the kernels over the whole grid count every cell; the algorithmic
@@ -364,8 +364,10 @@ def main():
         sor_bytes = 13.0 * per_launch
         cell_sweeps = g["sweepsTotal"] * nfl
     achieved = sor_bytes / sor_avg_s / 1e9 if g["sorLaunches"] else 0.0
-    kname = {5: "k_sor_flow", 4: "k_sor_solve", 3: "k_sor_tb", 2: "k_sor_list", 1: "k_sor_tile", 0: "k_sor"}[mode]
-    kdesc = {5: " (a whole pressure solve on workgroups that wait only for their neighbours)",
+    kname = {6: "k_sor_xcd", 5: "k_sor_flow", 4: "k_sor_solve", 3: "k_sor_tb", 2: "k_sor_list", 1: "k_sor_tile",
+             0: "k_sor"}[mode]
+    kdesc = {6: " (the red-black sweeps between two residual checks, one launch on the workgroups of one XCD)",
+             5: " (a whole pressure solve on workgroups that wait only for their neighbours)",
              4: " (a whole pressure solve in one workgroup: its sweeps and residual checks)",
              3: " (two red-black sweeps of the fluid cells per launch, LDS tiles)",
              2: " (the fluid cells of one colour)", 1: " (one full red-black sweep, LDS tile)",

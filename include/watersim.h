@@ -84,7 +84,11 @@ const char *ws_last_error(const ws_ctx *c);
  * "nP", "dv", "tn", "pressureTol", "maxPressureIters", "iso"; and the
  * kernel choices of this library (no reference counterpart; each gives the
  * same results): "sorMode", "sorThreads", "netTiled", "p2gBin" (0..3: P2G
- * bins of 8x8x8, 4x4x4, 16x8x8, 8x8x4 cells); the warm-start switches
+ * bins of 8x8x8, 4x4x4, 16x8x8, 8x8x4 cells), "xcdGroups" (workgroups of
+ * sorMode 6, 0 = one per compute unit of an XCD), "xcdUnits" (its cells per
+ * thread and colour, 1..3, 0 = chosen from the list lengths); the counter
+ * "xcdFallbacks" (sweep batches that sorMode 6 handed to the list kernel);
+ * the warm-start switches
  * "warmStart", "recordSamples", "netCoarse" (include/watersim_learn.h).
  * ws_get_f also reads results: "umax", "dtLast", "residual",
  * "pressureIters_done", "airborneCount", "nFluid", "dfRefined". */
@@ -212,8 +216,12 @@ int ws_read_render(ws_ctx *c, unsigned mask, ws_render_view *view);
  * kernel (sorMode 0 per-colour grid, 1 LDS tile, 2 fluid list, 3 two sweeps
  * per LDS launch, 4 the whole solve in one workgroup of 256, 512 or 1024
  * threads, 5 the whole solve on "flowGroups" workgroups that wait only for
- * their neighbours between half-sweeps; all give the same bits for the same
- * sweeps) and, with the warm
+ * their neighbours between half-sweeps, 6 the sweeps between two residual
+ * checks in one launch on the workgroups of one XCD, each holding its range
+ * of cells in LDS; all give the same bits for the same sweeps; a sorMode 6
+ * launch that finds its workgroups off one XCD or its ranges too large
+ * returns before writing, and the same sweeps run on the list kernel) and,
+ * with the warm
  * start on, the untiled and LDS-tiled convolutions, and the P2G bins
  * (8x8x8, 4x4x4, 16x8x8, 8x8x4 cells per tile; the sums are the same for
  * every bin), and keeps the fastest ("sorMode", "sorThreads", "netTiled",

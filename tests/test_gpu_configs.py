@@ -200,6 +200,57 @@ def test_temporal_blocking_equals_list_sweeps_at_128():
         assert np.array_equal(a[k], b[k]), k
 
 
+@pytest.mark.parametrize("groups,units", [(0, 0), (31, 0), (32, 3)])
+def test_xcd_sweeps_equal_list_sweeps_at_128(groups, units):
+    """The XCD-resident SOR (sorMode 6: every batch of sweeps between two
+    residual checks in one launch on the workgroups of one XCD, q of each
+    workgroup's cell range in LDS, neighbour ranges exchanged through L2)
+    against the fluid-list kernel on the cfg3 scene at its own size, for
+    several workgroup counts (0: one per compute unit of the XCD): every
+    substep's solve to 1e-4, the same sweep counts and the same bits in q,
+    every face and every particle. No launch may decline."""
+    cfg = bench.CONFIGS["cfg3"]
+    out = []
+    for mode in (2, 6):
+        s, world = bench.make_solver(cfg)
+        s.set(sorMode=mode, xcdGroups=groups, xcdUnits=units)
+        s.run_substeps(4)
+        out.append({k: s.download(k).copy() for k in ("q", "u", "v", "w", "px", "pvy")})
+        out[-1]["sweeps"] = s.get("sweepsTotal")
+        out[-1]["fallbacks"] = s.get("xcdFallbacks")
+        out[-1]["mode"] = s.get("sorMode")
+        s.close()
+    a, b = out
+    assert b["fallbacks"] == 0 and b["mode"] == 6
+    assert a["sweeps"] == b["sweeps"] and a["sweeps"] > 0
+    for k in ("q", "u", "v", "w", "px", "pvy"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_xcd_sweeps_decline_to_list_kernel():
+    """A launch whose cell ranges do not fit its workgroups' LDS (forced from
+    the host: one cell per thread and colour at 128^3) declines before it
+    writes anything; the host sees it at the residual check, runs the same
+    sweeps with the list kernel and, the size being pinned by the setting,
+    keeps that kernel. The results are the list kernel's bits."""
+    cfg = bench.CONFIGS["cfg3"]
+    out = []
+    for mode in (2, 6):
+        s, world = bench.make_solver(cfg)
+        s.set(sorMode=mode, xcdUnits=1)
+        s.run_substeps(2)
+        out.append({k: s.download(k).copy() for k in ("q", "u", "px")})
+        out[-1]["sweeps"] = s.get("sweepsTotal")
+        out[-1]["fallbacks"] = s.get("xcdFallbacks")
+        out[-1]["mode"] = s.get("sorMode")
+        s.close()
+    a, b = out
+    assert b["fallbacks"] == 1 and b["mode"] == 2
+    assert a["sweeps"] == b["sweeps"]
+    for k in ("q", "u", "px"):
+        assert np.array_equal(a[k], b[k]), k
+
+
 def _planet(n, seed):
     w = watersim_hip.build_world(n, bench.W_DOMAIN / n, 0.25 * bench.W_DOMAIN, 0.15 * bench.W_DOMAIN,
                                  20260500 + seed, int(0.8 * n ** 3))

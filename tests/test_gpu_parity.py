@@ -342,16 +342,40 @@ def test_p2g_bins_agree(scene):
     s.close()
 
 
+def test_xcd_sweeps_after_flow_solves():
+    """The XCD-resident sweeps (sorMode 6) on a context whose earlier solves
+    ran the neighbour-synchronised solve (sorMode 5, as the autotuner runs
+    them one after the other): the two kernels keep separate arrival
+    counters, so the later launches neither time out nor decline, and the
+    result equals the list kernel's after the same solves."""
+    g = golden.load("thermal_32")
+    qs = {}
+    for modes in ((2, 2, 2), (5, 6, 6)):
+        s = load_gpu(g)
+        s.set(pressureIters=37)
+        s.stage("rasterize")
+        for k in "uvw":
+            s.upload(k, g["grav_" + k])
+        s.upload("q", g["in_q"])
+        for m in modes:
+            s.set(sorMode=m)
+            s.stage("pressure")
+        qs[modes] = s.download("q")
+        assert s.get("xcdFallbacks") == 0
+        s.close()
+    assert np.array_equal(qs[(2, 2, 2)], qs[(5, 6, 6)])
+
+
 @pytest.mark.parametrize("scene", SCENES)
 def test_sor_kernels_agree(scene):
-    """The four SOR kernels perform the same red-black relaxation with the
-    same roundings: the whole-grid per-colour kernel, the fluid-list kernel,
-    the one-sweep LDS tile kernel and the two-sweeps-per-launch temporally
-    blocked kernel (37 sweeps: eighteen double launches and a single one)
-    give the same bits."""
+    """The SOR kernels perform the same red-black relaxation with the same
+    roundings: the whole-grid per-colour kernel, the fluid-list kernel, the
+    one-sweep LDS tile kernel, the two-sweeps-per-launch temporally blocked
+    kernel (37 sweeps: eighteen double launches and a single one) and the
+    XCD-resident kernel (all 37 sweeps in one launch) give the same bits."""
     g = golden.load(scene)
     qs = {}
-    for mode in (0, 1, 2, 3):
+    for mode in (0, 1, 2, 3, 6):
         s = load_gpu(g)
         s.set(sorMode=mode, pressureIters=37)
         s.stage("rasterize")
@@ -360,8 +384,10 @@ def test_sor_kernels_agree(scene):
         s.upload("q", g["in_q"])
         s.stage("pressure")
         qs[mode] = s.download("q")
+        if mode == 6:
+            assert s.get("xcdFallbacks") == 0
         s.close()
-    for mode in (1, 2, 3):
+    for mode in (1, 2, 3, 6):
         diff = int(np.sum(qs[mode] != qs[0]))
         report("sor mode %d cells differing from per-colour" % mode, diff)
         assert diff == 0, mode

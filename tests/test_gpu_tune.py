@@ -31,16 +31,16 @@ def test_tune_keeps_state_and_picks_a_kernel():
     rep = s.tune(16)
     print("  " + rep.replace("\n", "\n  "))
     modes = [ln for ln in rep.splitlines() if ln.startswith("sorMode ") and "sweeps in" in ln]
-    assert len(modes) == 8  # modes 0-3, mode 4 at 256, 512 and 1024 threads, mode 5
+    assert len(modes) == 9  # modes 0-3, mode 4 at 256, 512 and 1024 threads, modes 5 and 6
     chosen, threads = int(s.get("sorMode")), int(s.get("sorThreads"))
-    assert 0 <= chosen <= 5 and rep.rstrip().endswith("netTiled %d, p2gBin %d" % (int(s.get("netTiled")),
+    assert 0 <= chosen <= 6 and rep.rstrip().endswith("netTiled %d, p2gBin %d" % (int(s.get("netTiled")),
                                                                                   int(s.get("p2gBin"))))
     assert ("chosen: sorMode %d, sorThreads %d" % (chosen, threads)) in rep
     bins = {int(ln.split()[1]): float(ln.split(": ")[1].split()[0]) for ln in rep.splitlines()
             if ln.startswith("p2gBin ") and ln.rstrip().endswith(" ms")}
     assert sorted(bins) == [0, 1, 2, 3] and bins[int(s.get("p2gBin"))] == min(bins.values())
     times = {ln.split(":")[0]: float(ln.split(" in ")[1].split()[0]) for ln in modes}
-    key = "sorMode %d (%d threads)" % (chosen, threads) if chosen >= 4 else "sorMode %d" % chosen
+    key = "sorMode %d (%d threads)" % (chosen, threads) if chosen in (4, 5) else "sorMode %d" % chosen
     assert times[key] == min(times.values())
     for k in keys:
         assert np.array_equal(s.download(k), before[k]), k

@@ -273,8 +273,95 @@ int fold_spans(ws_ctx *c) {
   return WS_OK;
 }
 
+// sorMode 4 and 5 run the whole solve, stopping rule included, on the device
+static inline bool dev_solve(const ws_ctx *c) { return c->sorMode == 4 || c->sorMode == 5; }
+
+// the XCD-resident launch of n sweeps (sorMode 6); false when the lists
+// do not fit any workgroup size it is built for (the caller then runs the
+// list kernels)
+bool sor_xcd_launch(ws_ctx *c, int n, int *err) {
+  *err = WS_OK;
+  int G = c->xcdGroups;
+  if (G <= 0) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+      *err = WS_EHIP;
+      return false;
+    }
+    G = std::min(kXcdMaxG, std::max(1, cus / 8));
+  }
+  // a workgroup holds U x 1024 cells of each colour; ranges are cut from
+  // the red list, and the black cells of a range differ from its red ones
+  // by a few percent, so the choice keeps a margin (a range that does not
+  // fit makes the launch decline)
+  const long per = (std::max(c->nRed, c->nBlack) + G - 1) / G;
+  const long want = per + per / 32 + 32;
+  int U = c->xcdUnits ? c->xcdUnits : (want <= 1024 ? 1 : want <= 2048 ? 2 : want <= 3072 ? 3 : 0);
+  if (!c->xcdUnits && U) U = std::max(U, c->xcdMinUnits);
+  if (!U || U > 3) return false;
+  if (!c->xcdX) {
+    // exchange words are written by their launch before they are read (no
+    // clearing); the arrival counters start at zero, below every epoch
+    if (hipMalloc((void **)&c->xcdX, sizeof(unsigned long long) * (size_t)c->nC) != hipSuccess ||
+        hipMalloc((void **)&c->xcdFlow, sizeof(unsigned) * kFlowStride * kXcdMaxG) != hipSuccess) {
+      *err = WS_EHIP;
+      return false;
+    }
+    c->xcdEpoch = 0;
+  }
+  if (c->xcdEpoch == 0 || c->xcdEpoch > 0xf0000000u - 2u * (unsigned)n) {
+    hipLaunchKernelGGL(k_xcd_arm, dim3(1), dim3(256), 0, c->stream, kXcdMaxG, c->xcdFlow);
+    c->xcdEpoch = 1;
+  }
+  XcdSweep J;
+  J.g = c->geo;
+  J.ds = c->ds;
+  J.code = c->sorCode;
+  J.lrhs = c->sorRhs;
+  J.lpw = c->sorPw;
+  J.off = c->sorOff;
+  J.q = c->q;
+  J.half = c->sorHalf;
+  J.om1 = (float)(1 - c->omega);
+  J.G = G;
+  J.sweeps = n;
+  J.epoch = c->xcdEpoch;
+  J.flow = c->xcdFlow;
+  J.xq = c->xcdX;
+  c->xcdEpoch += 2u * (unsigned)n + 2u;
+  const dim3 grid(8 * (G - 1) + 1);
+  if (U == 1) hipLaunchKernelGGL((k_sor_xcd<1024, 1, 1024>), grid, dim3(1024), 0, c->stream, J);
+  else if (U == 2) hipLaunchKernelGGL((k_sor_xcd<1024, 2, 2048>), grid, dim3(1024), 0, c->stream, J);
+  else hipLaunchKernelGGL((k_sor_xcd<1024, 3, 3072>), grid, dim3(1024), 0, c->stream, J);
+  if (hipGetLastError() != hipSuccess) {
+    *err = WS_EHIP;
+    return false;
+  }
+  c->xcdPending = n;
+  c->xcdPendingUnits = U;
+  return true;
+}
+
 int sor_sweeps(ws_ctx *c, int n) {
   c->tbLaunchesPending = 0;
+  if (c->sorMode == 6 && n > 0) {
+    hipEvent_t span = nullptr;
+    int r;
+    if (c->timing && (r = span_begin(c, &span))) return r;
+    if (sor_xcd_launch(c, n, &r)) {
+      c->sweepsTotal += n;
+      c->sorCellSweeps += (c->nRed + c->nBlack) * (long)n;
+      if (span && (r = span_end(c, span, SPAN_SOR, 1))) return r;
+      return WS_OK;
+    }
+    if (r) return r;
+    // lists too long for the XCD kernel: this context sweeps with the list
+    // kernel from now on
+    c->sorMode = 2;
+    c->xcdFallbacks++;
+    if (span && (r = span_end(c, span, SPAN_SOR, 0))) return r;
+  }
   int half = (c->nx + 1) / 2;
   long nthr = (long)half * c->ny * c->nz;
   float om1 = (float)(1 - c->omega);
@@ -335,6 +422,9 @@ int sor_sweeps(ws_ctx *c, int n) {
 
 int fold_dev_stats(ws_ctx *c);
 void take_counts(ws_ctx *c);
+int xcd_confirm(ws_ctx *c, int *redo);
+int xcd_settle(ws_ctx *c);
+int sor_sweeps_list(ws_ctx *c, int n);
 
 // the on-device solve of one context (sorMode 4)
 SorJob sor_job(ws_ctx *c) {
@@ -453,8 +543,65 @@ int residual(ws_ctx *c, double *res) {
   WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
   WS_HIP(hipStreamSynchronize(c->stream));
   take_counts(c);
+  int redo = 0, r = xcd_confirm(c, &redo);
+  if (r) return r;
+  if (redo) {
+    // the XCD launch declined and q is as it was: the list kernel runs
+    // the same sweeps, then the residual is taken again
+    c->residualsTotal--;
+    if ((r = sor_sweeps_list(c, redo))) return r;
+    return residual(c, res);
+  }
   *res = c->hs->bb > 0 ? std::sqrt(c->hs->rr) / std::sqrt(c->hs->bb) : 0.0;
   return WS_OK;
+}
+
+// After a read of the device scalars: did the last XCD-resident launch run?
+// A declined launch (q untouched) switches the context to the list kernel
+// and returns its sweeps in *redo; a wait that gave up fails the solve.
+int xcd_confirm(ws_ctx *c, int *redo) {
+  *redo = 0;
+  if (!c->xcdPending) return WS_OK;
+  const int n = c->xcdPending;
+  c->xcdPending = 0;
+  if (c->hs->xcdTimedOut) {
+    c->err = "XCD-resident pressure sweeps: a workgroup's wait for its neighbours timed out";
+    return WS_ESTATE;
+  }
+  if (c->hs->xcdDeclined != c->xcdDeclinedSeen) {
+    const bool big = c->hs->xcdTooBig != c->xcdTooBigSeen;
+    c->xcdDeclinedSeen = c->hs->xcdDeclined;
+    c->xcdTooBigSeen = c->hs->xcdTooBig;
+    if (big && !c->xcdUnits && c->xcdPendingUnits < 3)
+      c->xcdMinUnits = c->xcdPendingUnits + 1;  // later launches hold more cells per thread
+    else
+      c->sorMode = 2;  // placement, or no larger size: the list kernel from now on
+    c->xcdFallbacks++;
+    c->sweepsTotal -= n;
+    c->sorCellSweeps -= (c->nRed + c->nBlack) * (long)n;
+    *redo = n;
+  }
+  return WS_OK;
+}
+
+// the device scalars read now, to confirm a pending XCD-resident launch
+// (paths that sweep without a residual check afterwards)
+int xcd_settle(ws_ctx *c) {
+  if (!c->xcdPending) return WS_OK;
+  WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+  WS_HIP(hipStreamSynchronize(c->stream));
+  int redo = 0, r = xcd_confirm(c, &redo);
+  if (r) return r;
+  return redo ? sor_sweeps_list(c, redo) : WS_OK;
+}
+
+// n sweeps with the list kernel whatever the mode (a declined XCD launch)
+int sor_sweeps_list(ws_ctx *c, int n) {
+  const int keep = c->sorMode;
+  c->sorMode = 2;
+  const int r = sor_sweeps(c, n);
+  c->sorMode = keep;
+  return r;
 }
 
 int st_pressure_solve(ws_ctx *c);
@@ -536,7 +683,7 @@ int pressure_setup(ws_ctx *c) {
     hipLaunchKernelGGL(k_sor_compact, dim3(nb), dim3(256), 0, c->stream, c->geo, c->nbm, c->dg, c->rhs, c->pw,
                        c->sorOff, c->sorHalf, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->ds);
     WS_LAUNCHED();
-    if (c->pressureTol <= 0 && c->sorMode < 4) {  // the to-residual path learns the counts with its first check
+    if (c->pressureTol <= 0 && !dev_solve(c)) {  // the to-residual path learns the counts with its first check
       WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
       WS_HIP(hipStreamSynchronize(c->stream));
       take_counts(c);
@@ -557,7 +704,7 @@ int pressure_setup(ws_ctx *c) {
 // the sweeps, to the tolerance or the fixed count
 int pressure_iterate(ws_ctx *c) {
   int r;
-  if (c->sorMode >= 4) {
+  if (dev_solve(c)) {
     if ((r = sor_solve_dev(c))) return r;
     if (c->recordSamples && (r = net_record_target(c))) return r;
     return WS_OK;
@@ -565,7 +712,11 @@ int pressure_iterate(ws_ctx *c) {
   if (c->pressureTol <= 0) {
     if ((r = sor_sweeps(c, c->pressureIters))) return r;
     c->itersDone = c->pressureIters;
-    if (c->measureResidual && (r = residual(c, &c->residual))) return r;
+    if (c->measureResidual) {
+      if ((r = residual(c, &c->residual))) return r;
+    } else if ((r = xcd_settle(c))) {
+      return r;
+    }
     if (c->recordSamples && (r = net_record_target(c))) return r;
     return WS_OK;
   }
@@ -1006,7 +1157,7 @@ void ws_destroy(ws_ctx *c) {
   balls_release(c);
   metaballs_release(c);
   void *ptrs[] = {c->staticTypes, c->gu, c->thr, c->terrainR, c->cellType, c->nbm, c->dg, c->q, c->q2, c->rhs, c->pw,
-                  c->sorActive, c->tbActive, c->tbPk, c->tbRhs, c->tbCnt, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->sorChunk, c->sorArrive, c->sorFlow,
+                  c->sorActive, c->tbActive, c->tbPk, c->tbRhs, c->tbCnt, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->sorChunk, c->sorArrive, c->sorFlow, c->xcdX, c->xcdFlow,
                   c->cellCount, c->cellStart, c->vort, c->vel, c->velO, c->wgt, c->scr, c->scr2, c->valid, c->dens,
                   c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pT, c->pAir, c->advU, c->advV, c->advW, c->pflag,
                   c->pcool, c->pcell, c->prank, c->sortedIdx, c->vneed, c->vscr, c->rng, c->ds, c->partials,
@@ -1044,8 +1195,8 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
 #undef SI
   if (!strcmp(n, "sorMode")) {
     int m = (int)v;
-    if (m < 0 || m > 5 || m != v) { c->err = "sorMode must be 0..5"; return WS_EINVAL; }
-    if (m >= 2 && c->nC >= (1L << 26)) { c->err = "sorMode 2..5 need fewer than 2^26 cells"; return WS_EINVAL; }
+    if (m < 0 || m > 6 || m != v) { c->err = "sorMode must be 0..6"; return WS_EINVAL; }
+    if (m >= 2 && c->nC >= (1L << 26)) { c->err = "sorMode 2..6 need fewer than 2^26 cells"; return WS_EINVAL; }
     c->sorMode = m;
     return WS_OK;
   }
@@ -1055,6 +1206,23 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
     WS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     if (!(v >= 1 && v <= cus) || v != (int)v) { c->err = "sorGroups must be 1 .. the compute units"; return WS_EINVAL; }
     c->sorGroups = (int)v;
+    return WS_OK;
+  }
+  if (!strcmp(n, "xcdGroups")) {
+    int dev = 0, cus = 0;
+    WS_HIP(hipGetDevice(&dev));
+    WS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (v != (int)v || v < 0 || v > std::min(kXcdMaxG, cus / 8)) {
+      c->err = "xcdGroups must be 0 (one per compute unit of an XCD) .. the compute units of one XCD";
+      return WS_EINVAL;
+    }
+    c->xcdGroups = (int)v;
+    return WS_OK;
+  }
+  if (!strcmp(n, "xcdUnits")) {
+    if (v != (int)v || v < 0 || v > 3) { c->err = "xcdUnits must be 0 (chosen from the list lengths) .. 3"; return WS_EINVAL; }
+    c->xcdUnits = (int)v;
+    c->xcdMinUnits = 1;
     return WS_OK;
   }
   if (!strcmp(n, "flowGroups") || !strcmp(n, "flowXcds")) {
@@ -1135,7 +1303,7 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   GF(cfl) GF(pic) GF(gravity) GF(viscosity) GF(omega) GF(maxSpeed) GF(vorticity) GF(coreR) GF(oceanR)
   GF(atmosphereH) GF(spacing) GF(particleVolume) GF(iso) GF(substeps) GF(pressureTol) GF(pressureIters)
   GF(maxPressureIters) GF(nP) GF(capacity) GF(residual) GF(domainR) GF(dx) GF(tn) GF(dv)
-  GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal) GF(residualsTotal) GF(ticksTotal) GF(sorMode) GF(sorCellSweeps)
+  GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal) GF(residualsTotal) GF(ticksTotal) GF(sorMode) GF(sorCellSweeps) GF(xcdFallbacks) GF(xcdGroups) GF(xcdUnits)
   GF(cmpMsPlain) GF(cmpMsLearned) GF(cmpSweepsPlain) GF(cmpSweepsLearned) GF(cmpSolves) GF(warmStart) GF(netMs) GF(netCalls)
   GF(vaporCount) GF(nightVaporCount) GF(contactOverflow) GF(nBalls) GF(mbM) GF(vaporBigGroups) GF(vaporGroupMax)
   GF(vaporBigParcels) GF(netTiled) GF(netCoarse) GF(p2gBin) GF(sorThreads) GF(sorGroups) GF(flowGroups) GF(flowXcds)
@@ -1267,16 +1435,23 @@ int ws_tune(ws_ctx *c, int sweeps, char *report, size_t len) {
   const int keepThreads = c->sorThreads;
   int best = keepMode, bestThreads = keepThreads;
   float bestMs = 1e30f;
-  const int cand[][2] = {{0, 0}, {1, 0}, {2, 0}, {3, 0}, {4, 256}, {4, 512}, {4, 1024}, {5, 256}};
+  const int cand[][2] = {{0, 0}, {1, 0}, {2, 0}, {3, 0}, {4, 256}, {4, 512}, {4, 1024}, {5, 256}, {6, 0}};
   for (const auto &cm : cand) {
     if (r) break;
     if (cm[0] >= 2 && c->nC >= (1L << 26)) continue;
     c->sorMode = cm[0];
     if (cm[1]) c->sorThreads = cm[1];
+    const long fallbacks = c->xcdFallbacks;
     float ms = 0;
     for (int rep2 = 0; rep2 < 2 && !r; rep2++) r = trial(&ms);  // the second run is timed
     if (r) break;
     char line[128];
+    if (cm[0] == 6 && c->xcdFallbacks != fallbacks) {
+      // the XCD-resident sweeps declined (placement or list length): not a candidate
+      snprintf(line, sizeof line, "sorMode 6: declined, not a candidate\n");
+      rep += line;
+      continue;
+    }
     if (cm[1])
       snprintf(line, sizeof line, "sorMode %d (%d threads): %d sweeps in %.3f ms\n", cm[0], cm[1], sweeps, ms);
     else

@@ -456,19 +456,6 @@ __global__ void k_assemble(Geo g, const uint8_t *__restrict__ ct, const float *_
   dg[c] = (uint8_t)diag;
 }
 
-// The relaxation of solver.js:1402, q(1 - omega) + omega/diag (rhs + sum),
-// with every rounding explicit so that all SOR kernels produce the same
-// bits for the same inputs. The operators are written here under the
-// pragma (HIP's __fmul_rn / __fadd_rn are plain operators defined elsewhere,
-// which the compiler may contract into an FMA differently in each kernel).
-__device__ __forceinline__ float sor_relax(float q, float sum, float rhs, float pw, float om1) {
-#pragma clang fp contract(off)
-  float keep = q * om1;
-  float src = rhs + sum;
-  float upd = pw * src;
-  return keep + upd;
-}
-
 // One colour of red-black SOR (the relaxation of solver.js:1402): cells with
 // (i+j+k)&1 == color. Each thread updates one cell of that colour.
 __global__ __launch_bounds__(256) void k_sor(Geo g, float *__restrict__ q, const float *__restrict__ rhs,

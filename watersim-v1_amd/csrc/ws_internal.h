@@ -75,6 +75,11 @@ struct DevScalars {
   unsigned sorIters, sorChecksLast;
   double sorRes;
   unsigned long long sorSweeps, sorChecks;
+  // XCD-resident sweeps (k_sor_xcd): launches that declined (workgroups
+  // not on one XCD, or a range larger than its LDS; q untouched), and a
+  // sticky flag set when one of their waits gave up, and the declines
+  // whose cause was a range or halo larger than the workgroup holds
+  unsigned xcdDeclined, xcdTimedOut, xcdTooBig;
 };
 
 // Geometry and per-substep constants passed by value to kernels.
@@ -90,6 +95,25 @@ struct Geo {
 // words between two workgroups' progress counters in the neighbour-
 // synchronised solve (one 128-byte line each)
 constexpr int kFlowStride = 32;
+
+// n red-black sweeps of the fluid lists in one launch on the workgroups of
+// one XCD (k_sor_xcd); progress counters and placement reports share the
+// flow buffer (word 0 and word 1 of each workgroup's line)
+constexpr int kXcdMaxG = 64;
+struct XcdSweep {
+  Geo g;
+  DevScalars *ds;
+  const unsigned *code;
+  const float *lrhs, *lpw;
+  const unsigned long long *off;  // the lists' scan keys (nC + 1 entries)
+  float *q;
+  long half;
+  float om1;
+  int G, sweeps;
+  unsigned epoch;  // every tag of this launch is at least epoch; earlier launches' are below it
+  unsigned *flow;  // per workgroup: arrival counter (word 0) and placement report (word 1)
+  unsigned long long *xq;  // per cell: a boundary cell's value and half-sweep tag
+};
 
 // One pressure solve run to its end by one workgroup (k_sor_solve): the
 // fluid lists of a context, its q, and the stopping rule of the host loop
@@ -258,6 +282,24 @@ struct Ctx {
   // neighbours between half-sweeps (k_sor_flow), on the first flowXcds XCDs
   int flowGroups = 256, flowXcds = 8;
   unsigned *sorFlow = nullptr;  // progress counters, kFlowStride words apart
+  // 6: n sweeps per launch on xcdGroups workgroups of one XCD
+  // (k_sor_xcd; 0 = one per compute unit of an XCD). A launch that
+  // declines leaves q untouched; the host sees it at its next read of the
+  // device scalars and runs the sweeps with k_sor_list instead. A range too
+  // large for its workgroup raises the cells per thread of later launches;
+  // any other decline, or ranges too large at the most cells per thread,
+  // keeps the context on sorMode 2 from then on (xcdFallbacks counts each
+  // batch run by the list kernel instead)
+  int xcdGroups = 0;
+  int xcdUnits = 0;             // cells per thread and colour (1..3); 0: chosen from the list lengths
+  unsigned xcdEpoch = 1;        // the next launch's counter and tag base
+  unsigned long long *xcdX = nullptr;  // the tagged exchange array (nC words, allocated on first use)
+  unsigned *xcdFlow = nullptr;  // arrival counters and placement reports (only k_sor_xcd writes them)
+  unsigned xcdDeclinedSeen = 0, xcdTooBigSeen = 0;
+  int xcdPending = 0;           // sweeps of the last launch not yet confirmed
+  int xcdPendingUnits = 0;      // its cells per thread
+  int xcdMinUnits = 1;          // raised when a launch's ranges did not fit
+  long xcdFallbacks = 0;
   unsigned long long sorSweepsSeen = 0, sorChecksSeen = 0;  // device totals folded into the host counters
   bool devStatsPending = false;
   unsigned *cellCount = nullptr, *cellStart = nullptr;
@@ -400,6 +442,19 @@ __device__ __forceinline__ float nb_sum(const float *__restrict__ q, int c, unsi
   sum += (m & 16u) ? e : 0.f;
   sum += (m & 32u) ? f : 0.f;
   return sum;
+}
+
+// The relaxation of solver.js:1402, q(1 - omega) + omega/diag (rhs + sum),
+// with every rounding explicit so that all SOR kernels produce the same
+// bits for the same inputs. The operators are written here under the
+// pragma (HIP's __fmul_rn / __fadd_rn are plain operators defined elsewhere,
+// which the compiler may contract into an FMA differently in each kernel).
+__device__ __forceinline__ float sor_relax(float q, float sum, float rhs, float pw, float om1) {
+#pragma clang fp contract(off)
+  float keep = q * om1;
+  float src = rhs + sum;
+  float upd = pw * src;
+  return keep + upd;
 }
 
 __device__ __forceinline__ int xcd_block(int b, int nb) {

@@ -91,6 +91,9 @@ __global__ void k_sor_arm(int n, unsigned *arrive);
 template <int NT>
 __global__ void k_sor_flow(SorJob J, int X);
 __global__ void k_flow_arm(int G, unsigned *flow, unsigned *arrive);
+template <int NT, int U, int H>
+__global__ void k_sor_xcd(XcdSweep J);
+__global__ void k_xcd_arm(int G, unsigned *flow);
 __global__ void k_project(Geo g, const uint8_t *ct, const float *phi, float *vel, int do_clamp, float vcap,
                           DevScalars *s);
 __global__ void k_clamp(long F, float *vel, float vcap, DevScalars *s);
