@@ -106,6 +106,51 @@ def test_full_size_substep_vs_oracle(name):
     s.close()
 
 
+@pytest.mark.skip(reason="the row-pipelined extrapolation is switched off: its first GPU run (a 64^3 "
+                         "substep) stalled and no cause was found by reading the code")
+def test_row_extrapolation_at_128():
+    """The row-pipelined extrapolation (exRows 1: flags as a 3-D prefix OR
+    of the seeds, values in one launch of slabs x chunks that hand rows over
+    through one XCD's L2) against the tile-diagonal kernel (exRows 0) and
+    the oracle's serial flood (js/solver.js:1562-1636) on the cfg3 scene at
+    its own size, three substeps in: every final flag exact, values within
+    2e-5 of the field scale (the reference sums in double, these kernels
+    scan affine maps in float)."""
+    cfg = bench.CONFIGS["cfg3"]
+    s, world = bench.make_solver(cfg)
+    s.run_substeps(3)
+    keys = ("u", "v", "w", "uW", "vW", "wW")
+    keep = {k: s.download(k) for k in keys}
+    res = {}
+    for rows in (1, 0, 1):
+        for k in keys:
+            s.upload(k, keep[k])
+        s.set(exRows=rows)
+        s.stage("extrapolate")
+        out = {k: s.download(k) for k in ("u", "v", "w", "validU", "validV", "validW")}
+        if rows in res:  # the second row-pipelined run repeats the first bit for bit
+            for k in out:
+                assert np.array_equal(out[k], res[rows][k]), ("repeat", k)
+        res[rows] = out
+    o = oracle_from_solver(s, world)
+    o.arrays["cellType"][:] = s.download("cellType")
+    for k in keys:
+        o.arrays[k][:] = keep[k]
+    o.call("oc_extrapolate")
+    filled = 0
+    for k, vk in (("u", "validU"), ("v", "validV"), ("w", "validW")):
+        assert np.array_equal(res[1][vk], o[vk]), vk
+        assert np.array_equal(res[0][vk], o[vk]), vk
+        scale = float(np.abs(o[k]).max())
+        for rows in (1, 0):
+            e = float(np.abs(res[rows][k].astype(np.float64) - o[k]).max()) / scale
+            print("  %s exRows %d: max err %.3e of the field scale" % (k, rows, e))
+            assert e <= 2e-5, (k, rows, e)
+        filled += int(np.sum(res[1][vk] != (keep[k + "W"] > 1e-8)))
+    print("  flags exact; %d faces valid without a P2G weight" % filled)
+    s.close()
+
+
 def test_cell_index_at_float_boundaries():
     """Particles on both sides of every cell boundary of each axis, the other
     two coordinates at the domain centre. The planet's demotion rules are

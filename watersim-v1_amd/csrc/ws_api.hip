@@ -423,6 +423,7 @@ int sor_sweeps(ws_ctx *c, int n) {
 int fold_dev_stats(ws_ctx *c);
 void take_counts(ws_ctx *c);
 int xcd_confirm(ws_ctx *c, int *redo);
+int device_failures(ws_ctx *c);
 int xcd_settle(ws_ctx *c);
 int sor_sweeps_list(ws_ctx *c, int n);
 
@@ -543,6 +544,7 @@ int residual(ws_ctx *c, double *res) {
   WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
   WS_HIP(hipStreamSynchronize(c->stream));
   take_counts(c);
+  if (int e = device_failures(c)) return e;
   int redo = 0, r = xcd_confirm(c, &redo);
   if (r) return r;
   if (redo) {
@@ -788,7 +790,66 @@ int st_clamp(ws_ctx *c) {
   return WS_OK;
 }
 
+// the row-pipelined extrapolation: flags (two launches), values (one),
+// the check that every job ran
+int extrapolate_rows(ws_ctx *c) {
+  const int D1 = c->ny + 1, D2 = c->nz + 1, W = (c->nx + 1 + 63) / 64;
+  if (!c->exFin) {
+    c->exS = (D2 + EXR_TK - 1) / EXR_TK;
+    c->exC = W;
+    const size_t nxk = (size_t)3 * c->exS * c->exC * D1 * 64, nxi = (size_t)3 * c->exS * c->exC * EXR_TK * D1;
+    WS_HIP(hipMalloc((void **)&c->exFin, sizeof(unsigned long long) * (size_t)3 * D2 * D1 * W));
+    WS_HIP(hipMalloc((void **)&c->exXk, sizeof(unsigned long long) * nxk));
+    WS_HIP(hipMalloc((void **)&c->exXi, sizeof(unsigned long long) * nxi));
+    WS_HIP(hipMalloc((void **)&c->exCnt, sizeof(unsigned) * 8));
+    // tags 0: below every launch epoch (the clears run on the context's
+    // stream, ahead of the first launch)
+    hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for((long)nxk * 2, 256)), dim3(256), 0, c->stream,
+                       (unsigned *)c->exXk, (long)nxk * 2);
+    hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for((long)nxi * 2, 256)), dim3(256), 0, c->stream,
+                       (unsigned *)c->exXi, (long)nxi * 2);
+    c->exEpoch = 0;
+  }
+  if (++c->exEpoch == 0) {
+    const size_t nxk = (size_t)3 * c->exS * c->exC * D1 * 64, nxi = (size_t)3 * c->exS * c->exC * EXR_TK * D1;
+    hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for((long)nxk * 2, 256)), dim3(256), 0, c->stream,
+                       (unsigned *)c->exXk, (long)nxk * 2);
+    hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for((long)nxi * 2, 256)), dim3(256), 0, c->stream,
+                       (unsigned *)c->exXi, (long)nxi * 2);
+    c->exEpoch = 1;
+  }
+  ExRun R;
+  R.ticket = c->exCnt;
+  R.done = c->exCnt + 4;
+  R.xk = c->exXk;
+  R.xi = c->exXi;
+  R.S = c->exS;
+  R.C = c->exC;
+  R.epoch = c->exEpoch;
+  hipLaunchKernelGGL(k_exf_rows, dim3(3 * D2), dim3(256), 0, c->stream, c->geo, c->valid, c->exFin, R);
+  hipLaunchKernelGGL(k_exf_k, dim3((3 * D1 * W + 3) / 4), dim3(256), 0, c->stream, c->geo, c->exFin);
+  // the most jobs of any component, on each of the 8 XCDs (components run on
+  // XCDs 0, 1, 2; the other blocks return at once)
+  const int jobs = c->exS * c->exC;
+  hipLaunchKernelGGL(k_extrap_rows, dim3(8 * jobs), dim3(EXR_THREADS), 0, c->stream, c->geo, c->vel, c->valid,
+                     c->exFin, R, c->ds);
+  hipLaunchKernelGGL(k_extrap_check, dim3(1), dim3(64), 0, c->stream, c->geo, R, c->ds);
+  WS_LAUNCHED();
+  return WS_OK;
+}
+
+// failures the device reports in its scalars (read after each copy of them)
+int device_failures(ws_ctx *c) {
+  if (c->hs->exFailed) {
+    c->err = c->hs->exFailed & 2 ? "extrapolation: a component's jobs were not all run (no workgroup on its XCD)"
+                                 : "extrapolation: a wait for a neighbouring slab or chunk timed out";
+    return WS_ESTATE;
+  }
+  return WS_OK;
+}
+
 int st_extrapolate(ws_ctx *c) {
+  if (c->exRows) return extrapolate_rows(c);
   for (size_t d = 0; d + 1 < c->exDiagOff.size(); d++) {
     int n = c->exDiagOff[d + 1] - c->exDiagOff[d];
     if (n > 0)
@@ -905,6 +966,7 @@ int substep_post2(ws_ctx *c) {
 int pull_scalars(ws_ctx *c) {
   WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
   WS_HIP(hipStreamSynchronize(c->stream));
+  if (int r = device_failures(c)) return r;
   c->umax = __builtin_bit_cast(float, c->hs->umax_bits);
   c->airborne = (int)c->hs->airborne;
   c->gRamp = c->hs->gRamp;
@@ -1157,7 +1219,7 @@ void ws_destroy(ws_ctx *c) {
   balls_release(c);
   metaballs_release(c);
   void *ptrs[] = {c->staticTypes, c->gu, c->thr, c->terrainR, c->cellType, c->nbm, c->dg, c->q, c->q2, c->rhs, c->pw,
-                  c->sorActive, c->tbActive, c->tbPk, c->tbRhs, c->tbCnt, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->sorChunk, c->sorArrive, c->sorFlow, c->xcdX, c->xcdFlow,
+                  c->sorActive, c->tbActive, c->tbPk, c->tbRhs, c->tbCnt, c->sorOff, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->sorChunk, c->sorArrive, c->sorFlow, c->xcdX, c->xcdFlow, c->exFin, c->exXk, c->exXi, c->exCnt,
                   c->cellCount, c->cellStart, c->vort, c->vel, c->velO, c->wgt, c->scr, c->scr2, c->valid, c->dens,
                   c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pT, c->pAir, c->advU, c->advV, c->advW, c->pflag,
                   c->pcool, c->pcell, c->prank, c->sortedIdx, c->vneed, c->vscr, c->rng, c->ds, c->partials,
@@ -1217,6 +1279,15 @@ int ws_set_f(ws_ctx *c, const char *n, double v) {
       return WS_EINVAL;
     }
     c->xcdGroups = (int)v;
+    return WS_OK;
+  }
+  if (!strcmp(n, "exRows")) {
+    if (v != 0 && v != 1) { c->err = "exRows must be 0 (tile diagonals) or 1 (row pipeline)"; return WS_EINVAL; }
+    if (v == 1) {
+      c->err = "exRows 1 (the row-pipelined extrapolation) is switched off: its first GPU run stalled, cause not found";
+      return WS_EINVAL;
+    }
+    c->exRows = (int)v;
     return WS_OK;
   }
   if (!strcmp(n, "xcdUnits")) {
@@ -1303,7 +1374,7 @@ int ws_get_f(ws_ctx *c, const char *n, double *v) {
   GF(cfl) GF(pic) GF(gravity) GF(viscosity) GF(omega) GF(maxSpeed) GF(vorticity) GF(coreR) GF(oceanR)
   GF(atmosphereH) GF(spacing) GF(particleVolume) GF(iso) GF(substeps) GF(pressureTol) GF(pressureIters)
   GF(maxPressureIters) GF(nP) GF(capacity) GF(residual) GF(domainR) GF(dx) GF(tn) GF(dv)
-  GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal) GF(residualsTotal) GF(ticksTotal) GF(sorMode) GF(sorCellSweeps) GF(xcdFallbacks) GF(xcdGroups) GF(xcdUnits)
+  GF(sorMs) GF(pressureMs) GF(sorLaunches) GF(pressureSolves) GF(sweepsTotal) GF(residualsTotal) GF(ticksTotal) GF(sorMode) GF(sorCellSweeps) GF(xcdFallbacks) GF(xcdGroups) GF(xcdUnits) GF(exRows)
   GF(cmpMsPlain) GF(cmpMsLearned) GF(cmpSweepsPlain) GF(cmpSweepsLearned) GF(cmpSolves) GF(warmStart) GF(netMs) GF(netCalls)
   GF(vaporCount) GF(nightVaporCount) GF(contactOverflow) GF(nBalls) GF(mbM) GF(vaporBigGroups) GF(vaporGroupMax)
   GF(vaporBigParcels) GF(netTiled) GF(netCoarse) GF(p2gBin) GF(sorThreads) GF(sorGroups) GF(flowGroups) GF(flowXcds)

@@ -31,6 +31,9 @@ __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // k; one wavefront per k-plane of the tile
 constexpr int EX_TI = 64, EX_TJ = 8, EX_TK = 8;
 constexpr int EX_THREADS = 64 * EX_TK;
+// the row-pipelined extrapolation: k-planes per workgroup (one wavefront each)
+constexpr int EXR_TK = 8;
+constexpr int EXR_THREADS = 64 * EXR_TK;
 constexpr int SOR_TX = 32, SOR_TY = 8, SOR_TZ = 8;
 constexpr int SOR_TB_X = 32, SOR_TB_Y = 16, SOR_TB_Z = 16;  // temporal-blocking tile
 constexpr int SOR_TB_THREADS = 512;
@@ -80,6 +83,9 @@ struct DevScalars {
   // sticky flag set when one of their waits gave up, and the declines
   // whose cause was a range or halo larger than the workgroup holds
   unsigned xcdDeclined, xcdTimedOut, xcdTooBig;
+  // the row-pipelined extrapolation: 1 a poll gave up, 2 a component's jobs
+  // were not all done (sticky until the host reads and clears it)
+  unsigned exFailed;
 };
 
 // Geometry and per-substep constants passed by value to kernels.
@@ -136,6 +142,18 @@ struct SorJob {
   // progress counter per workgroup, kFlowStride words apart
   const unsigned long long *off;
   unsigned *flow;
+};
+
+// one launch of the row-pipelined extrapolation (k_extrap_rows): per
+// component a ticket counter and a count of finished jobs, the exchange
+// arrays (per component, slab and chunk: the top plane's rows, 64 words each,
+// and the last face of each plane's rows), their dimensions (the most slabs
+// and chunks of any component) and the launch's tag
+struct ExRun {
+  unsigned *ticket, *done;
+  unsigned long long *xk, *xi;
+  int S, C;
+  unsigned epoch;
 };
 
 // one tile diagonal d = I + J + K of the extrapolation wavefront: its tiles
@@ -326,6 +344,16 @@ struct Ctx {
   size_t scanTmpBytes = 0;
   // extrapolation tile schedule
   int4 *exTiles = nullptr;
+  // the row-pipelined extrapolation (exRows 1; 0, the default: one launch
+  // per tile diagonal). Switched off: its first GPU run stalled with no
+  // cause found; see the note in ws_extrap.hip. Final-flag words, exchange
+  // arrays (cleared once, tags are launch epochs from 1), tickets and job
+  // counts
+  int exRows = 0;
+  unsigned long long *exFin = nullptr, *exXk = nullptr, *exXi = nullptr;
+  unsigned *exCnt = nullptr;
+  int exS = 0, exC = 0;
+  unsigned exEpoch = 0;
   std::vector<int4> exTilesHost;  // the same schedule on the host (batches merge the planets')
   std::vector<int> exDiagOff;  // tiles of diagonal d: [off[d], off[d+1])
   std::vector<ExDiag> exDiag;  // the same diagonals, for tiles found from the block index

@@ -93,6 +93,11 @@ __global__ void k_sor_flow(SorJob J, int X);
 __global__ void k_flow_arm(int G, unsigned *flow, unsigned *arrive);
 template <int NT, int U, int H>
 __global__ void k_sor_xcd(XcdSweep J);
+__global__ void k_exf_rows(Geo g, const uint8_t *valid, unsigned long long *fin, ExRun R);
+__global__ void k_exf_k(Geo g, unsigned long long *fin);
+__global__ void k_extrap_rows(Geo g, float *vel, uint8_t *valid, const unsigned long long *fin, ExRun R,
+                              DevScalars *ds);
+__global__ void k_extrap_check(Geo g, ExRun R, DevScalars *ds);
 __global__ void k_xcd_arm(int G, unsigned *flow);
 __global__ void k_project(Geo g, const uint8_t *ct, const float *phi, float *vel, int do_clamp, float vcap,
                           DevScalars *s);
