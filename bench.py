@@ -593,16 +593,20 @@ def batched_main(a, cfg):
         log("tune %d^3: %s" % (n, rep.replace("\n", "; ")))
     for i, s in enumerate(planets):
         s.set(netTiled=tiling[sizes[i]], p2gBin=bins[sizes[i]])
+    # the solve's workgroup size, and each planet's workgroups on one XCD
+    # (L2-local barriers) or dealt over all XCDs (agent-scope barriers)
     trial = {}
-    for th in (256, 512, 1024):
-        batch.set(threads=th)
-        t1 = time.perf_counter()
-        batch.run(10)
-        trial[th] = (time.perf_counter() - t1) / 10
-        log("batch with %d-thread solves: %.3f ms per batched substep" % (th, 1e3 * trial[th]))
-    threads = min(trial, key=trial.get)
-    batch.set(threads=threads)
-    log("timing %d batched substeps (HIP graph), %d-thread solves" % (a.steps, threads))
+    for xl in (1, 0):
+        for th in (256, 512, 1024):
+            batch.set(threads=th, xcdLocal=xl)
+            t1 = time.perf_counter()
+            batch.run(10)
+            trial[(th, xl)] = (time.perf_counter() - t1) / 10
+            log("batch with %d-thread solves, xcdLocal %d (%d planets XCD-local): %.3f ms per batched substep" % (
+                th, xl, int(batch.get("xcdLocalPlanets")), 1e3 * trial[(th, xl)]))
+    threads, xcd_local = min(trial, key=trial.get)
+    batch.set(threads=threads, xcdLocal=xcd_local)
+    log("timing %d batched substeps (HIP graph), %d-thread solves, xcdLocal %d" % (a.steps, threads, xcd_local))
     t0 = time.perf_counter()
     done = 0
     while done < a.steps:
@@ -618,6 +622,7 @@ def batched_main(a, cfg):
     batch.set(timing=1)
     batch.run(4)
     g = {k: batch.get(k) for k in ("solveMs", "solveLaunches", "cellSweeps", "sweeps")}
+    local_planets = int(batch.get("xcdLocalPlanets"))
     batch.set(timing=0)
     launches = max(g["solveLaunches"], 1)
     solve_s = g["solveMs"] / 1e3 / launches
@@ -645,8 +650,8 @@ def batched_main(a, cfg):
                                "solve workgroup size" % (P, list(cfg["sizes"]), cfg["vorticity"]),
                    "particles_total": sum(nPs), "per_size": per_size,
                    "planet_substep_ms": 1e3 * el / a.steps,
-                   "solve_threads": threads,
-                   "batch_trial_ms": {str(k): 1e3 * v for k, v in trial.items()},
+                   "solve_threads": threads, "xcd_local": xcd_local, "xcd_local_planets": local_planets,
+                   "batch_trial_ms": {"%d threads, xcdLocal %d" % k: 1e3 * v for k, v in trial.items()},
                    "joint_solve_ms": 1e3 * solve_s, "sweeps_per_planet_substep": sweeps_per},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / (HBM_PEAK / 1e9), "traffic": None,

@@ -123,10 +123,17 @@ def test_device_solve_fixed_sweeps():
     b.close()
 
 
-@pytest.mark.parametrize("graph,cells_per_group", [(False, 110000), (True, 110000), (True, 4000)])
-def test_batch_equals_single_planets(graph, cells_per_group):
+@pytest.mark.parametrize("graph,cells_per_group,xcd_local,max_groups,local_planets", [
+    (False, 110000, 1, 256, 1), (True, 110000, 1, 256, 1), (True, 4000, 1, 256, 3), (True, 4000, 0, 256, 0),
+    (True, 4000, 1, 96, 2), (True, 4000, 1, 40, 0)])
+def test_batch_equals_single_planets(graph, cells_per_group, xcd_local, max_groups, local_planets):
     """cells_per_group 110000 gives the 48^3 planet two workgroups; 4000
-    splits every solve over 8 to 28 workgroups."""
+    splits the solves over 9, 28 and 9 workgroups. With xcd_local each
+    planet's workgroups are laid out on one XCD where they fit (32 slots per
+    XCD at max_groups 256: all three planets; 12 at 96: the 9-workgroup
+    planets only; 5 at 40, where the 48^3 planet is cut to 22: none) and take
+    the L2-local barriers; the others take the agent-scope ones. Either way
+    the bits are those of ws_run_substeps."""
     if not os.path.exists(WEIGHTS):
         pytest.skip("no trained warm-start weights")
     spec = [(32, 11, False), (48, 12, True), (32, 13, False)]
@@ -135,7 +142,7 @@ def test_batch_equals_single_planets(graph, cells_per_group):
     for s in ref:
         s.set(sorMode=4, sorThreads=512)
     b = watersim_hip.Batch(bat, graph=graph, threads=512)
-    b.set(cellsPerGroup=cells_per_group)
+    b.set(cellsPerGroup=cells_per_group, xcdLocal=xcd_local, maxGroups=max_groups)
     for call, k in enumerate((1, 2, 3)):
         b.run(k)
         for s in ref:
@@ -144,8 +151,10 @@ def test_batch_equals_single_planets(graph, cells_per_group):
             assert_same(r, s, "call %d planet %d" % (call, i))
             assert r.get("sweepsTotal") == s.get("sweepsTotal")
             assert r.get("residual") == s.get("residual")
-        print("  call %d (%d substeps): %d planets equal, sweeps %s, %d workgroups" % (
-            call, k, len(bat), [int(s.get("sweepsTotal")) for s in bat], int(b.get("groups"))))
+        print("  call %d (%d substeps): %d planets equal, sweeps %s, %d blocks, %d planets XCD-local" % (
+            call, k, len(bat), [int(s.get("sweepsTotal")) for s in bat], int(b.get("groups")),
+            int(b.get("xcdLocalPlanets"))))
+        assert int(b.get("xcdLocalPlanets")) == local_planets
     b.close()
     for s in ref + bat:
         s.close()
@@ -180,8 +189,9 @@ def test_batch_of_64_planets_as_benchmarked():
         assert np.isfinite(s.download("pvy")).all()
     for r, i in zip(ref, firsts):
         assert_same(r, planets[i], "planet %d (%d^3)" % (i, sizes[i]))
-    print("  64 planets, %d particles; sweeps of the checked planets %s" % (
-        sum(int(s.get("nP")) for s in planets), [int(planets[i].get("sweepsTotal")) for i in firsts]))
+    print("  64 planets, %d particles; sweeps of the checked planets %s; %d planets XCD-local" % (
+        sum(int(s.get("nP")) for s in planets), [int(planets[i].get("sweepsTotal")) for i in firsts],
+        int(b.get("xcdLocalPlanets"))))
     b.close()
     for s in planets + ref:
         s.close()

@@ -7,8 +7,10 @@
 // diagonal). A substep costs no host round trip. With a graph, the batched
 // substep (fork to the planets' streams, the joint solve, join) is captured
 // once per run and replayed for every substep.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -33,6 +35,12 @@ struct ws_batch {
   std::vector<int> G;
   int2 *blk = nullptr;
   unsigned *arrive = nullptr;
+  // xcdLocal: lay each planet's workgroups out on one XCD where they fit
+  // (blocks b with b % 8 equal), so that its barriers stay in that XCD's L2;
+  // place holds the planets' placement words (G + 1 each, at placeOff)
+  int xcdLocal = 1;
+  unsigned *place = nullptr;
+  std::vector<int> placeOff;
   // the planets' extrapolation schedules merged by tile diagonal: one launch
   // per diagonal covers every planet's tiles of that diagonal
   ExPlanet *exPlanets = nullptr;
@@ -95,6 +103,51 @@ void launch_solves(ws_batch *b) {
     hipLaunchKernelGGL(k_sor_solve_batch<1024>, dim3(b->nBlocks), dim3(1024), 0, b->stream, b->jobs, b->blk);
 }
 
+// the blocks of the joint solve with each planet's workgroups on one XCD:
+// workgroups are dealt to the 8 XCDs in turn (block b to XCD b % 8), so slot
+// s of XCD x is block 8 s + x. Planets go to the XCD with the most free slots
+// that holds them, the largest first; a planet that fits no XCD whole takes
+// free slots wherever they are (its workgroups then find, on the device,
+// that they do not share an XCD). Returns the table (x < 0: an empty slot)
+// and marks which planets were laid out whole on one XCD.
+std::vector<int2> xcd_layout(const std::vector<int> &G, int slots, std::vector<char> &whole) {
+  const int n = (int)G.size();
+  std::vector<int> order(n), fill(8, 0), xcd(n, -1);
+  for (int i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return G[a] > G[b]; });
+  whole.assign(n, 0);
+  for (int i : order) {
+    int best = -1;
+    for (int x = 0; x < 8; x++)
+      if (fill[x] + G[i] <= slots && (best < 0 || fill[x] < fill[best])) best = x;
+    if (best >= 0) {
+      xcd[i] = best;
+      fill[best] += G[i];
+      whole[i] = 1;
+    }
+  }
+  int rows = 0;
+  for (int x = 0; x < 8; x++) rows = std::max(rows, fill[x]);
+  // the planets that fit nowhere whole, spread over the free slots
+  int spill = 0;
+  for (int i = 0; i < n; i++)
+    if (!whole[i]) spill += G[i];
+  while (8 * rows - std::accumulate(fill.begin(), fill.end(), 0) < spill) rows++;
+  std::vector<int2> hb(8 * (size_t)rows, make_int2(-1, -1));
+  std::vector<int> next(8, 0);
+  for (int i : order)
+    if (whole[i])
+      for (int w = 0; w < G[i]; w++) hb[8 * (size_t)next[xcd[i]]++ + xcd[i]] = make_int2(i, w);
+  size_t free_slot = 0;
+  for (int i : order)
+    if (!whole[i])
+      for (int w = 0; w < G[i]; w++) {
+        while (hb[free_slot].x >= 0) free_slot++;
+        hb[free_slot] = make_int2(i, w);
+      }
+  return hb;
+}
+
 // workgroups per planet from its cell count, the block table, the jobs
 int plan_solves(ws_batch *b) {
   int n = (int)b->ctx.size();
@@ -114,19 +167,29 @@ int plan_solves(ws_batch *b) {
     total--;
   }
   std::vector<int2> hb;
-  for (int i = 0; i < n; i++)
-    for (int w = 0; w < b->G[i]; w++) hb.push_back(make_int2(i, w));
-  if ((int)hb.size() > b->nBlocks) {
-    if (b->blk) (void)hipFree(b->blk);
-    b->blk = nullptr;
-    BH(hipMalloc((void **)&b->blk, sizeof(int2) * b->maxGroups));
+  std::vector<char> whole(n, 0);
+  if (b->xcdLocal && b->maxGroups >= 8) {
+    hb = xcd_layout(b->G, b->maxGroups / 8, whole);
+  } else {
+    for (int i = 0; i < n; i++)
+      for (int w = 0; w < b->G[i]; w++) hb.push_back(make_int2(i, w));
   }
+  if (!b->blk) BH(hipMalloc((void **)&b->blk, sizeof(int2) * 256));
+  if (hb.size() > 256) return fail(b, "the joint solve's layout exceeds 256 blocks", WS_EINVAL);
   b->nBlocks = (int)hb.size();
   BH(hipMemcpy(b->blk, hb.data(), sizeof(int2) * hb.size(), hipMemcpyHostToDevice));
-  for (int i = 0; i < n; i++) {
+  if (!b->place) BH(hipMalloc((void **)&b->place, sizeof(unsigned) * (256 + 256)));
+  BH(hipMemset(b->place, 0, sizeof(unsigned) * (256 + 256)));
+  b->placeOff.assign(n, -1);
+  for (int i = 0, at = 0; i < n; i++) {
     b->hjobs[i] = batch_sor_job(b->ctx[i]);
     b->hjobs[i].G = b->G[i];
     b->hjobs[i].arrive = b->arrive + 2 * i;
+    if (whole[i] && b->G[i] > 1) {
+      b->placeOff[i] = at;
+      b->hjobs[i].place = b->place + at;
+      at += b->G[i] + 1;
+    }
   }
   BH(hipMemcpy(b->jobs, b->hjobs.data(), sizeof(SorJob) * n, hipMemcpyHostToDevice));
   return WS_OK;
@@ -357,6 +420,10 @@ int ws_batch_set(ws_batch *b, const char *name, double v) {
     b->threads = (int)v;
     return WS_OK;
   }
+  if (!strcmp(name, "xcdLocal")) {
+    b->xcdLocal = v != 0;
+    return WS_OK;
+  }
   if (!strcmp(name, "cellsPerGroup")) {
     if (!(v >= 1)) return fail(b, "cellsPerGroup must be at least 1", WS_EINVAL);
     b->cellsPerGroup = v;
@@ -386,6 +453,19 @@ int ws_batch_get(ws_batch *b, const char *name, double *v) {
   else if (!strcmp(name, "cellsPerGroup")) *v = b->cellsPerGroup;
   else if (!strcmp(name, "maxGroups")) *v = b->maxGroups;
   else if (!strcmp(name, "groups")) *v = b->nBlocks;
+  else if (!strcmp(name, "xcdLocal")) *v = b->xcdLocal;
+  else if (!strcmp(name, "xcdLocalPlanets")) {
+    // planets whose last joint solve ran with its workgroups on one XCD
+    int k = 0;
+    BH(hipStreamSynchronize(b->stream));
+    for (size_t i = 0; i < b->placeOff.size(); i++) {
+      if (b->placeOff[i] < 0) continue;
+      unsigned u = 0;
+      BH(hipMemcpy(&u, b->place + b->placeOff[i] + b->G[i], sizeof(unsigned), hipMemcpyDeviceToHost));
+      k += u != 0;
+    }
+    *v = k;
+  }
   else return fail(b, std::string("unknown batch value ") + name, WS_EINVAL);
   return WS_OK;
 }
@@ -412,6 +492,7 @@ void ws_batch_destroy(ws_batch *b) {
   if (b->jobs) (void)hipFree(b->jobs);
   if (b->blk) (void)hipFree(b->blk);
   if (b->arrive) (void)hipFree(b->arrive);
+  if (b->place) (void)hipFree(b->place);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }

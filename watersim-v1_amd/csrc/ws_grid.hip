@@ -969,17 +969,53 @@ constexpr int kSorU = 8;
 constexpr int kSorChunk = 2048;
 constexpr int kSorPolls = 1 << 20;
 
-// the G workgroups of a planet wait for each other (barrier number k)
+// q as the solve reads it. L (the planet's workgroups share one XCD's L2):
+// an agent-scope load, which skips the compute unit's L1 (it may hold q of
+// an earlier half-sweep) and reads that L2, where every workgroup of the
+// planet has stored its values.
+template <bool L>
+__device__ __forceinline__ float ldq(const float *p) {
+  if constexpr (L) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else return *p;
+}
+
+// nb_sum with the loads of ldq<L> (same order of additions, same bits)
+template <bool L>
+__device__ __forceinline__ float nb_sum_l(const float *q, int c, unsigned m, int nx, int nxy) {
+  if constexpr (!L) return nb_sum(q, c, m, nx, nxy);
+  const float a = ldq<L>(q + ((m & 1u) ? c - 1 : c)), b = ldq<L>(q + ((m & 2u) ? c + 1 : c));
+  const float cc = ldq<L>(q + ((m & 4u) ? c - nx : c)), d = ldq<L>(q + ((m & 8u) ? c + nx : c));
+  const float e = ldq<L>(q + ((m & 16u) ? c - nxy : c)), f = ldq<L>(q + ((m & 32u) ? c + nxy : c));
+  float sum = 0.f;
+  sum += (m & 1u) ? a : 0.f;
+  sum += (m & 2u) ? b : 0.f;
+  sum += (m & 4u) ? cc : 0.f;
+  sum += (m & 8u) ? d : 0.f;
+  sum += (m & 16u) ? e : 0.f;
+  sum += (m & 32u) ? f : 0.f;
+  return sum;
+}
+
+// the G workgroups of a planet wait for each other (barrier number k).
+// Across XCDs (L false) the arrival is an agent-scope release and the wait
+// an acquire: the XCD's L2 writes its dirty lines back and drops its copies.
+// Within one XCD (L true) every thread waits for its own stores to reach the
+// shared L2 and the counter is relaxed; the loads after the barrier read
+// that L2 (ldq<true>), so nothing is written back or dropped.
+template <bool L>
 __device__ __forceinline__ bool sor_barrier(const SorJob &J, unsigned k) {
+  if constexpr (L) __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   __shared__ int ok;
   if (threadIdx.x == 0) {
     ok = 1;
     if (J.G > 1) {
-      __hip_atomic_fetch_add(J.arrive, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      constexpr int order = L ? __ATOMIC_RELAXED : __ATOMIC_RELEASE;
+      constexpr int wait = L ? __ATOMIC_RELAXED : __ATOMIC_ACQUIRE;
+      __hip_atomic_fetch_add(J.arrive, 1u, order, __HIP_MEMORY_SCOPE_AGENT);
       const unsigned want = k * (unsigned)J.G;
       int polls = 0;
-      while (__hip_atomic_load(J.arrive, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
+      while (__hip_atomic_load(J.arrive, wait, __HIP_MEMORY_SCOPE_AGENT) < want) {
         if (__hip_atomic_load(J.arrive + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++polls > kSorPolls) {
           __hip_atomic_store(J.arrive + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           ok = 0;
@@ -993,7 +1029,7 @@ __device__ __forceinline__ bool sor_barrier(const SorJob &J, unsigned k) {
   return ok != 0;
 }
 
-template <int NT>
+template <int NT, bool L>
 __device__ __forceinline__ void sor_half(const SorJob &J, long base, long lo, long hi, int nxy) {
   const int nx = J.g.nx;
   for (long t0 = lo + threadIdx.x; t0 < hi; t0 += (long)NT * kSorU) {
@@ -1012,8 +1048,8 @@ __device__ __forceinline__ void sor_half(const SorJob &J, long base, long lo, lo
 #pragma unroll
     for (int u = 0; u < kSorU; u++) {
       const int c = (int)(e[u] & 0x3ffffffu);
-      sum[u] = nb_sum(J.q, c, e[u] >> 26, nx, nxy);
-      qc[u] = J.q[c];
+      sum[u] = nb_sum_l<L>(J.q, c, e[u] >> 26, nx, nxy);
+      qc[u] = ldq<L>(J.q + c);
     }
 #pragma unroll
     for (int u = 0; u < kSorU; u++)
@@ -1022,17 +1058,17 @@ __device__ __forceinline__ void sor_half(const SorJob &J, long base, long lo, lo
 }
 
 // one sweep (red, then black) by workgroup w of the planet's G
-template <int NT>
+template <int NT, bool L>
 __device__ __forceinline__ bool sor_sweep(const SorJob &J, int w, long nR, long nB, int nxy, unsigned &bar) {
-  sor_half<NT>(J, 0, nR * w / J.G, nR * (w + 1) / J.G, nxy);
-  if (!sor_barrier(J, ++bar)) return false;
-  sor_half<NT>(J, J.half, nB * w / J.G, nB * (w + 1) / J.G, nxy);
-  return sor_barrier(J, ++bar);
+  sor_half<NT, L>(J, 0, nR * w / J.G, nR * (w + 1) / J.G, nxy);
+  if (!sor_barrier<L>(J, ++bar)) return false;
+  sor_half<NT, L>(J, J.half, nB * w / J.G, nB * (w + 1) / J.G, nxy);
+  return sor_barrier<L>(J, ++bar);
 }
 
 // sqrt(rr) / sqrt(bb) over both lists; every thread of every workgroup of
 // the planet returns the same value (false: a barrier timed out)
-template <int NT>
+template <int NT, bool L>
 __device__ bool sor_residual(const SorJob &J, int w, long nR, long nB, int nxy, unsigned &bar, double *res) {
   __shared__ double wr[NT / 64], wb[NT / 64], tot;
   const int nx = J.g.nx;
@@ -1048,9 +1084,9 @@ __device__ bool sor_residual(const SorJob &J, int w, long nR, long nB, int nxy, 
         long pos = t < nR ? t : J.half + (t - nR);
         unsigned e = J.code[pos];
         int c = (int)(e & 0x3ffffffu);
-        const float a = nb_sum(J.q, c, e >> 26, nx, nxy);
+        const float a = nb_sum_l<L>(J.q, c, e >> 26, nx, nxy);
         double rh = (double)J.lrhs[pos];
-        double r = rh - ((double)J.ldg[pos] * (double)J.q[c] - (double)a);
+        double r = rh - ((double)J.ldg[pos] * (double)ldq<L>(J.q + c) - (double)a);
         rr += r * r;
         bb += rh * rh;
       }
@@ -1071,7 +1107,7 @@ __device__ bool sor_residual(const SorJob &J, int w, long nR, long nB, int nxy, 
     }
     __syncthreads();
   }
-  if (!sor_barrier(J, ++bar)) return false;
+  if (!sor_barrier<L>(J, ++bar)) return false;
   if (threadIdx.x == 0) {
     double a = 0, b = 0;
     for (long ch = 0; ch < nChunks; ch++) {
@@ -1084,33 +1120,35 @@ __device__ bool sor_residual(const SorJob &J, int w, long nR, long nB, int nxy, 
   *res = tot;
   // the chunk sums are rewritten by the next evaluation only after every
   // workgroup has read them
-  return sor_barrier(J, ++bar);
+  return sor_barrier<L>(J, ++bar);
 }
 
-template <int NT>
-__device__ void sor_solve_part(const SorJob &J, int w) {
+// the solve by workgroup w of G; bar: barriers the workgroups have already
+// passed in this launch
+template <int NT, bool L>
+__device__ void sor_solve_part(const SorJob &J, int w, unsigned bar) {
   const long nR = J.ds->nRed, nB = J.ds->nBlack;
   const int nxy = J.g.nx * J.g.ny;
-  unsigned done = 0, checks = 0, bar = 0;
+  unsigned done = 0, checks = 0;
   double res = 0;
   bool ok = true;
   if (J.tol <= 0) {
-    for (int s = 0; s < J.fixedIters && ok; s++) ok = sor_sweep<NT>(J, w, nR, nB, nxy, bar);
+    for (int s = 0; s < J.fixedIters && ok; s++) ok = sor_sweep<NT, L>(J, w, nR, nB, nxy, bar);
     done = (unsigned)max(J.fixedIters, 0);
-    if (ok && J.measure) { ok = sor_residual<NT>(J, w, nR, nB, nxy, bar, &res); checks++; }
+    if (ok && J.measure) { ok = sor_residual<NT, L>(J, w, nR, nB, nxy, bar, &res); checks++; }
   } else {
     const unsigned last = J.ds->sorIters;  // the previous solve's sweeps (rewritten at the end)
-    ok = sor_residual<NT>(J, w, nR, nB, nxy, bar, &res);
+    ok = sor_residual<NT, L>(J, w, nR, nB, nxy, bar, &res);
     checks++;
     double best = res, prev = res;
     int flat = 0, k = J.every;
     if (J.adaptive && last > 0) k = max(k, (int)(0.75 * last));
     while (ok && res > J.tol && (int)done < J.maxIters && flat < 12) {
       if ((int)done + k > J.maxIters) k = J.maxIters - (int)done;
-      for (int s = 0; s < k && ok; s++) ok = sor_sweep<NT>(J, w, nR, nB, nxy, bar);
+      for (int s = 0; s < k && ok; s++) ok = sor_sweep<NT, L>(J, w, nR, nB, nxy, bar);
       done += (unsigned)k;
       if (!ok) break;
-      ok = sor_residual<NT>(J, w, nR, nB, nxy, bar, &res);
+      ok = sor_residual<NT, L>(J, w, nR, nB, nxy, bar, &res);
       checks++;
       if (res < 0.995 * best) { best = res; flat = 0; } else flat++;
       int kn = J.every;
@@ -1134,21 +1172,47 @@ __device__ void sor_solve_part(const SorJob &J, int w) {
 
 template <int NT>
 __global__ __launch_bounds__(NT) void k_sor_solve(SorJob J) {
-  sor_solve_part<NT>(J, 0);
+  sor_solve_part<NT, false>(J, 0, 0);
 }
 
-// block b works on planet blk[b].x as its workgroup blk[b].y
+// block b works on planet blk[b].x as its workgroup blk[b].y (x < 0: an
+// empty slot of the layout). A planet whose workgroups were laid out on one
+// XCD (J.place) checks that they run there: each workgroup writes its XCD id
+// (hardware register XCC_ID) before the first barrier and reads the others'
+// after it, so all G decide alike; they then take the L2-local barriers, and
+// the others the agent-scope ones. Both give the same bits.
 template <int NT>
 __global__ __launch_bounds__(NT) void k_sor_solve_batch(const SorJob *__restrict__ jobs, const int2 *__restrict__ blk) {
   const int2 pw = blk[blockIdx.x];
+  if (pw.x < 0) return;
   const SorJob J = jobs[pw.x];
-  sor_solve_part<NT>(J, pw.y);
+  if (!J.place || J.G < 2) {
+    sor_solve_part<NT, false>(J, pw.y, 0);
+    return;
+  }
+  __shared__ int local;
+  if (threadIdx.x == 0) {
+    const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);  // HW_REG_XCC_ID, bits 3:0
+    __hip_atomic_store(J.place + pw.y, xcc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  const bool ok = sor_barrier<false>(J, 1);
+  if (threadIdx.x == 0) {
+    int same = 1;
+    const unsigned x0 = __hip_atomic_load(J.place, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int v = 1; v < J.G; v++)
+      same &= __hip_atomic_load(J.place + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == x0;
+    local = ok && same;
+    if (pw.y == 0) __hip_atomic_store(J.place + J.G, (unsigned)local, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  if (local) sor_solve_part<NT, true>(J, pw.y, 1);
+  else sor_solve_part<NT, false>(J, pw.y, 1);
 }
 
 // one context's solve over gridDim.x workgroups (J.G == gridDim.x)
 template <int NT>
 __global__ __launch_bounds__(NT) void k_sor_solve_multi(SorJob J) {
-  sor_solve_part<NT>(J, blockIdx.x);
+  sor_solve_part<NT, false>(J, blockIdx.x, 0);
 }
 
 // the barrier counters of solves about to start (flags stay set)
@@ -1283,7 +1347,7 @@ __device__ void sor_flow_part(SorJob J, int w, int G) {
     ok = flow_sync(J, w, dlo, dhi, ++h);
   };
   auto check = [&]() {
-    if ((ok = sor_barrier(J, ++bar))) ok = sor_residual<NT>(J, w, nR, nB, nxy, bar, &res);
+    if ((ok = sor_barrier<false>(J, ++bar))) ok = sor_residual<NT, false>(J, w, nR, nB, nxy, bar, &res);
     checks++;
   };
   if (J.tol <= 0) {

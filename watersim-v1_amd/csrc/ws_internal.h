@@ -137,6 +137,10 @@ struct SorJob {
   double *chunk;     // residual chunk sums, 2 x ceil(nC / 2048)
   unsigned *arrive;  // G > 1: barrier arrivals, then a timed-out flag
   int G;             // workgroups sharing this solve
+  // batched solve, G > 1: G + 1 words, non-null when the G workgroups were
+  // laid out on one XCD (each writes its XCD id to word w; word G records
+  // whether they met there and took the L2-local barriers)
+  unsigned *place = nullptr;
   // the neighbour-synchronised solve (k_sor_flow): the lists' scan keys
   // (red count | black count << 32 before each cell, nC + 1 entries) and one
   // progress counter per workgroup, kFlowStride words apart
