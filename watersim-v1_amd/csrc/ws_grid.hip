@@ -128,13 +128,32 @@ __global__ void k_scatter(int nP, const int *__restrict__ pcell, const unsigned 
 }
 
 // -------------------------------------------------------------------- P2G
-// _p2gInto + _p2gNormalizeFace, solver.js:1035-1120. One workgroup per 8^3
-// cell tile owns the tile's faces; it visits the particles binned in the
-// tile and its one-cell halo (their trilinear footprint reaches one cell) and
+// _p2gInto + _p2gNormalizeFace, solver.js:1035-1120. One workgroup per cell
+// tile owns the tile's faces; it visits the particles binned in the tile and
+// its one-cell halo (their trilinear footprint reaches one cell) and
 // accumulates momentum and weight in LDS as 64-bit fixed point. Integer
 // addition is exact and order independent, so the result is deterministic
 // without float atomics; the reference's sequential f32 sums differ from the
 // exact sums by float rounding only.
+//
+// The kernel is bound by the vector ALU (three components x eight corners
+// per visit, two fixed-point conversions per owned corner) and by the latency
+// of each visit's chain (find the row, read the particle index, read the
+// particle). So:
+// - a tile whose region holds no particle (two loads of the bin starts)
+//   writes its zero faces and leaves;
+// - each particle row of the region is cut into three segments (the -x halo
+//   cell, the tile's cells, the +x halo cell), and the segments are visited
+//   grouped by which side of the tile they lie on along x, y and z: the
+//   lanes of a wavefront then own the same corners, and a corner no lane
+//   owns is skipped (a halo particle owns half of its corners or fewer along
+//   each halo axis);
+// - per particle the two offsets of each axis (0 along the component's own
+//   axis, 0.5 along the others) are computed once, and per component a
+//   two-bit mask per axis says which of the stencil's two faces the tile
+//   owns;
+// - the next visit's particle loads and the one after's index load are in
+//   flight while a visit is accumulated.
 __device__ __forceinline__ long long fix(float v) { return (long long)__builtin_rintf(v * kFixScaleF); }
 
 template <int TX, int TY, int TZ>
@@ -146,104 +165,193 @@ __global__ __launch_bounds__(256) void k_p2g_tile(Geo g, int tilesX, int tilesY,
                                                   float *__restrict__ wgt) {
   // owned faces of one component (+1 plane along its axis for the last tile)
   constexpr int BOX = cmax(cmax((TX + 1) * TY * TZ, TX * (TY + 1) * TZ), TX * TY * (TZ + 1));
-  static_assert((TY + 2) * (TZ + 2) <= 256, "one thread per particle row of the tile");
+  constexpr int NSEG = 3 * (TY + 2) * (TZ + 2);
+  static_assert(NSEG <= 512, "two segments per thread");
   __shared__ unsigned long long accM[3][BOX];
   __shared__ unsigned long long accW[3][BOX];
-  __shared__ unsigned rowBeg[(TY + 2) * (TZ + 2)];
-  __shared__ unsigned rowPre[(TY + 2) * (TZ + 2) + 1];
+  __shared__ unsigned segBeg[NSEG], segPre[NSEG + 1];
+  __shared__ unsigned clsCount[27], clsStart[28];
+  __shared__ unsigned wsum[4];
 
-  int b = xcd_block(blockIdx.x, gridDim.x);
-  int tx = b % tilesX, ty = (b / tilesX) % tilesY, tz = b / (tilesX * tilesY);
-  int i0 = tx * TX, j0 = ty * TY, k0 = tz * TZ;
-  for (int e = threadIdx.x; e < 3 * BOX; e += blockDim.x) {
-    (&accM[0][0])[e] = 0ull;
-    (&accW[0][0])[e] = 0ull;
-  }
+  const int b = blockIdx.x;
+  const int tx = b % tilesX, ty = (b / tilesX) % tilesY, tz = b / (tilesX * tilesY);
+  const int i0 = tx * TX, j0 = ty * TY, k0 = tz * TZ;
   // particle rows of the tile + halo: cells [i0-1, i0+T] x [j0-1, j0+T] x [k0-1, k0+T]
-  int ci0 = max(i0 - 1, 0), ci1 = min(i0 + TX, g.nx - 1);
-  int cj0 = max(j0 - 1, 0), cj1 = min(j0 + TY, g.ny - 1);
-  int ck0 = max(k0 - 1, 0), ck1 = min(k0 + TZ, g.nz - 1);
-  int nrj = cj1 - cj0 + 1, nrows = nrj * (ck1 - ck0 + 1);
-  if ((int)threadIdx.x < nrows) {
-    int r = threadIdx.x;
-    int cj = cj0 + r % nrj, ck = ck0 + r / nrj;
-    int row = (ck * g.ny + cj) * g.nx;
-    unsigned a = start[row + ci0], e = start[row + ci1 + 1];
-    rowBeg[r] = a;
-    rowPre[r + 1] = e - a;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned acc = 0;
-    rowPre[0] = 0;
-    for (int r = 1; r <= nrows; r++) { acc += rowPre[r]; rowPre[r] = acc; }
-  }
-  __syncthreads();
-  unsigned total = rowPre[nrows];
-  // owned face ranges per component: [lo, hi) in i, j, k
-  int iHiC = min(i0 + TX, g.nx), jHiC = min(j0 + TY, g.ny), kHiC = min(k0 + TZ, g.nz);
-  // The binned particles are in cell order, so neighbouring entries share
-  // their faces. A wavefront takes every 8th entry of a 256-entry chunk
-  // (thread 32a + b takes entry 8b + a) so that its lanes' LDS atomics fall
-  // on different faces instead of queueing on the same few; integer sums do
-  // not depend on the order.
-  const unsigned perm = (threadIdx.x & 31u) * 8u + (threadIdx.x >> 5);
-  for (unsigned base = 0; base < total; base += 256u) {
-    const unsigned vI = base + perm;
-    if (vI >= total) continue;
-    int lo = 0, hi = nrows;  // last r with rowPre[r] <= vI
-    while (hi - lo > 1) {
-      int mid = (lo + hi) >> 1;
-      if (rowPre[mid] <= vI) lo = mid; else hi = mid;
+  const int ci0 = max(i0 - 1, 0), ci1 = min(i0 + TX, g.nx - 1);
+  const int cj0 = max(j0 - 1, 0), cj1 = min(j0 + TY, g.ny - 1);
+  const int ck0 = max(k0 - 1, 0), ck1 = min(k0 + TZ, g.nz - 1);
+  // owned face ranges: [i0, hi) per component and axis
+  const int iHiC = min(i0 + TX, g.nx), jHiC = min(j0 + TY, g.ny), kHiC = min(k0 + TZ, g.nz);
+  const int uHi = i0 + TX >= g.nx ? g.nx + 1 : i0 + TX, vHi = j0 + TY >= g.ny ? g.ny + 1 : j0 + TY,
+            wHi = k0 + TZ >= g.nz ? g.nz + 1 : k0 + TZ;
+  // no particle binned anywhere from the region's first cell to its last
+  const bool empty = start[(ck0 * g.ny + cj0) * g.nx + ci0] == start[(ck1 * g.ny + cj1) * g.nx + ci1 + 1];
+  if (!empty) {
+    for (int e = threadIdx.x; e < 3 * BOX; e += blockDim.x) {
+      (&accM[0][0])[e] = 0ull;
+      (&accW[0][0])[e] = 0ull;
     }
-    unsigned p = sorted[rowBeg[lo] + (vI - rowPre[lo])];
-    float x = px[p], y = py[p], z = pz[p];
-    float vc[3] = {pvx[p], pvy[p], pvz[p]};
+    if (threadIdx.x < 27) clsCount[threadIdx.x] = 0;
+    const int nrj = cj1 - cj0 + 1, nseg = 3 * nrj * (ck1 - ck0 + 1);
+    __syncthreads();
+    // segment s: row s / 3, part s % 3; its class (side along x, y, z: 0
+    // below the tile, 1 inside, 2 above) and its slot among its class
+    unsigned sb[2], sl[2], sc[2], ss[2];
 #pragma unroll
-    for (int comp = 0; comp < 3; comp++) {
-      float oa = comp == 0 ? 0.f : 0.5f, ob = comp == 1 ? 0.f : 0.5f, oc = comp == 2 ? 0.f : 0.5f;
-      int fi = (int)floorf(x * g.inv_dx - oa), fj = (int)floorf(y * g.inv_dx - ob), fk = (int)floorf(z * g.inv_dx - oc);
-      float txf, tyf, tzf;
-      if (fi < 0) { fi = 0; txf = 0.f; } else if (fi > g.nx - 1) { fi = g.nx - 1; txf = 1.f; } else txf = axis_frac(g, x, oa, fi);
-      if (fj < 0) { fj = 0; tyf = 0.f; } else if (fj > g.ny - 2) { fj = g.ny - 2; tyf = 1.f; } else tyf = axis_frac(g, y, ob, fj);
-      if (fk < 0) { fk = 0; tzf = 0.f; } else if (fk > g.nz - 2) { fk = g.nz - 2; tzf = 1.f; } else tzf = axis_frac(g, z, oc, fk);
-      float s0 = 1.f - txf, s1 = txf, t0 = 1.f - tyf, t1 = tyf, r0 = 1.f - tzf, r1 = tzf;
-      float wA = s0 * t0, wB = s1 * t0, wC = s0 * t1, wD = s1 * t1;
-      int d0 = comp == 0 ? g.nx + 1 : g.nx, d1 = comp == 1 ? g.ny + 1 : g.ny;
-      // owned ranges of this component
-      int oi1 = comp == 0 ? (i0 + TX >= g.nx ? g.nx + 1 : i0 + TX) : iHiC;
-      int oj1 = comp == 1 ? (j0 + TY >= g.ny ? g.ny + 1 : j0 + TY) : jHiC;
-      int ok1 = comp == 2 ? (k0 + TZ >= g.nz ? g.nz + 1 : k0 + TZ) : kHiC;
-#pragma unroll
-      for (int e = 0; e < 8; e++) {
-        int di = e & 1, dj = (e >> 1) & 1, dk = (e >> 2) & 1;
-        float wt = (di ? (dj ? wD : wB) : (dj ? wC : wA)) * (dk ? r1 : r0);
-        int ii = fi + di, jj = fj + dj, kk = fk + dk;
-        // v/w stencils clamped at i0 = nx-1 read the next row (the
-        // reference's idx+1 at the +x edge, solver.js:765,772)
-        if (ii >= d0) { ii -= d0; jj += 1; if (jj >= d1) { jj = 0; kk += 1; } }
-        if (ii < i0 || ii >= oi1 || jj < j0 || jj >= oj1 || kk < k0 || kk >= ok1) continue;
-        int l = ((kk - k0) * (TY + (comp == 1)) + (jj - j0)) * (TX + (comp == 0)) + (ii - i0);
-        atomicAdd(&accM[comp][l], (unsigned long long)fix(wt * vc[comp]));
-        atomicAdd(&accW[comp][l], (unsigned long long)fix(wt));
+    for (int h = 0; h < 2; h++) {
+      const int s = threadIdx.x + 256 * h;
+      sl[h] = 0;
+      sb[h] = sc[h] = ss[h] = 0;
+      if (s < nseg) {
+        const int r = s / 3, part = s % 3;
+        const int cj = cj0 + r % nrj, ck = ck0 + r / nrj;
+        const int row = (ck * g.ny + cj) * g.nx;
+        const int a = part == 0 ? ci0 : (part == 1 ? i0 : iHiC), e = part == 0 ? i0 : (part == 1 ? iHiC : ci1 + 1);
+        if (e > a) {
+          sb[h] = start[row + a];
+          sl[h] = start[row + e] - sb[h];
+        }
+        const int jc = cj < j0 ? 0 : (cj >= j0 + TY ? 2 : 1), kc = ck < k0 ? 0 : (ck >= k0 + TZ ? 2 : 1);
+        sc[h] = part * 9 + jc * 3 + kc;
+        if (sl[h]) ss[h] = atomicAdd(&clsCount[sc[h]], 1u);
       }
     }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      unsigned v = threadIdx.x < 27 ? clsCount[threadIdx.x] : 0u;
+      for (int o = 1; o < 64; o <<= 1) {
+        const unsigned t = __shfl_up(v, o);
+        if ((int)threadIdx.x >= o) v += t;
+      }
+      if (threadIdx.x < 27) clsStart[threadIdx.x + 1] = v;
+      if (threadIdx.x == 0) clsStart[0] = 0;
+    }
+    __syncthreads();
+    const int nzs = clsStart[27];  // non-empty segments, in class order
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+      if (sl[h]) {
+        const unsigned pos = clsStart[sc[h]] + ss[h];
+        segBeg[pos] = sb[h];
+        segPre[pos + 1] = sl[h];
+      }
+    __syncthreads();
+    {  // inclusive scan of the segment lengths, two per thread
+      const int e0 = 2 * threadIdx.x + 1, e1 = e0 + 1;
+      const unsigned a0 = e0 <= nzs ? segPre[e0] : 0u, a1 = e1 <= nzs ? segPre[e1] : 0u;
+      unsigned v = a0 + a1;
+      for (int o = 1; o < 64; o <<= 1) {
+        const unsigned t = __shfl_up(v, o);
+        if ((int)(threadIdx.x & 63) >= o) v += t;
+      }
+      if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = v;
+      __syncthreads();
+      for (int w = 0; w < (int)(threadIdx.x >> 6); w++) v += wsum[w];
+      if (e0 <= nzs) segPre[e0] = v - a1;
+      if (e1 <= nzs) segPre[e1] = v;
+      if (threadIdx.x == 0) segPre[0] = 0;
+    }
+    __syncthreads();
+    const unsigned total = segPre[nzs];
+    auto locate = [&](unsigned v) -> unsigned {
+      int lo = 0, hi = nzs;  // last segment starting at or before v
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (segPre[mid] <= v) lo = mid; else hi = mid;
+      }
+      return sorted[segBeg[lo] + (v - segPre[lo])];
+    };
+    // The binned particles are in cell order, so neighbouring entries share
+    // their faces. A wavefront takes every 8th entry of a 256-entry chunk
+    // (thread 32a + b takes entry 8b + a) so that its lanes' LDS atomics fall
+    // on different faces instead of queueing on the same few.
+    const unsigned perm = (threadIdx.x & 31u) * 8u + (threadIdx.x >> 5);
+    unsigned vA = perm;
+    const unsigned pA = vA < total ? locate(vA) : 0u;
+    unsigned pB = vA + 256u < total ? locate(vA + 256u) : 0u;
+    float xA = px[pA], yA = py[pA], zA = pz[pA], uA = pvx[pA], vA_ = pvy[pA], wA_ = pvz[pA];
+    while (vA < total) {
+      // entry 0 stands in for a visit past the end (its loads are harmless)
+      const float xB = px[pB], yB = py[pB], zB = pz[pB], uB = pvx[pB], vB_ = pvy[pB], wB_ = pvz[pB];
+      const unsigned pC = vA + 512u < total ? locate(vA + 512u) : 0u;
+      const float vc[3] = {uA, vA_, wA_};
+      // per axis, offset 0 (the component along the axis) and 0.5 (the others)
+      int fx[2], fy[2], fz[2];
+      float tx_[2], ty_[2], tz_[2];
+#pragma unroll
+      for (int o = 0; o < 2; o++) {
+        const float off = o ? 0.5f : 0.f;
+        int f = (int)floorf(xA * g.inv_dx - off);
+        if (f < 0) { f = 0; tx_[o] = 0.f; } else if (f > g.nx - 1) { f = g.nx - 1; tx_[o] = 1.f; } else tx_[o] = axis_frac(g, xA, off, f);
+        fx[o] = f;
+        f = (int)floorf(yA * g.inv_dx - off);
+        if (f < 0) { f = 0; ty_[o] = 0.f; } else if (f > g.ny - 2) { f = g.ny - 2; ty_[o] = 1.f; } else ty_[o] = axis_frac(g, yA, off, f);
+        fy[o] = f;
+        f = (int)floorf(zA * g.inv_dx - off);
+        if (f < 0) { f = 0; tz_[o] = 0.f; } else if (f > g.nz - 2) { f = g.nz - 2; tz_[o] = 1.f; } else tz_[o] = axis_frac(g, zA, off, f);
+        fz[o] = f;
+      }
+#pragma unroll
+      for (int comp = 0; comp < 3; comp++) {
+        const int ox = comp == 0 ? 0 : 1, oy = comp == 1 ? 0 : 1, oz = comp == 2 ? 0 : 1;
+        const int fi = fx[ox], fj = fy[oy], fk = fz[oz];
+        const int hx = comp == 0 ? uHi : iHiC, hy = comp == 1 ? vHi : jHiC, hz = comp == 2 ? wHi : kHiC;
+        const float s0 = 1.f - tx_[ox], s1 = tx_[ox], t0 = 1.f - ty_[oy], t1 = ty_[oy], r0 = 1.f - tz_[oz], r1 = tz_[oz];
+        const float wa = s0 * t0, wb = s1 * t0, wc = s0 * t1, wd = s1 * t1;
+        if (comp != 0 && fi == g.nx - 1) {
+          // v/w stencils clamped at i0 = nx-1 read the next row (the
+          // reference's idx+1 at the +x edge, solver.js:765,772)
+          const int d0 = g.nx, d1 = comp == 1 ? g.ny + 1 : g.ny;
+#pragma unroll
+          for (int e = 0; e < 8; e++) {
+            const int di = e & 1, dj = (e >> 1) & 1, dk = (e >> 2) & 1;
+            const float wt = (di ? (dj ? wd : wb) : (dj ? wc : wa)) * (dk ? r1 : r0);
+            int ii = fi + di, jj = fj + dj, kk = fk + dk;
+            if (ii >= d0) { ii -= d0; jj += 1; if (jj >= d1) { jj = 0; kk += 1; } }
+            if (ii < i0 || ii >= hx || jj < j0 || jj >= hy || kk < k0 || kk >= hz) continue;
+            const int l = ((kk - k0) * (TY + (comp == 1)) + (jj - j0)) * (TX + (comp == 0)) + (ii - i0);
+            atomicAdd(&accM[comp][l], (unsigned long long)fix(wt * vc[comp]));
+            atomicAdd(&accW[comp][l], (unsigned long long)fix(wt));
+          }
+          continue;
+        }
+        const unsigned mx = (unsigned)(fi >= i0 && fi < hx) | (unsigned)(fi + 1 >= i0 && fi + 1 < hx) << 1;
+        const unsigned my = (unsigned)(fj >= j0 && fj < hy) | (unsigned)(fj + 1 >= j0 && fj + 1 < hy) << 1;
+        const unsigned mz = (unsigned)(fk >= k0 && fk < hz) | (unsigned)(fk + 1 >= k0 && fk + 1 < hz) << 1;
+        if (!(mx && my && mz)) continue;
+        const int SX = TX + (comp == 0), SY = TY + (comp == 1);
+        const int lb = ((fk - k0) * SY + (fj - j0)) * SX + (fi - i0);
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          const int di = e & 1, dj = (e >> 1) & 1, dk = (e >> 2) & 1;
+          if (!((mx >> di) & (my >> dj) & (mz >> dk) & 1u)) continue;
+          const float wt = (di ? (dj ? wd : wb) : (dj ? wc : wa)) * (dk ? r1 : r0);
+          const int l = lb + (dk * SY + dj) * SX + di;
+          atomicAdd(&accM[comp][l], (unsigned long long)fix(wt * vc[comp]));
+          atomicAdd(&accW[comp][l], (unsigned long long)fix(wt));
+        }
+      }
+      xA = xB; yA = yB; zA = zB; uA = uB; vA_ = vB_; wA_ = wB_;
+      pB = pC;
+      vA += 256u;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   // normalise and write the owned faces
   for (int comp = 0; comp < 3; comp++) {
-    int d0 = comp == 0 ? g.nx + 1 : g.nx, d1 = comp == 1 ? g.ny + 1 : g.ny;
-    int oi1 = comp == 0 ? (i0 + TX >= g.nx ? g.nx + 1 : i0 + TX) : iHiC;
-    int oj1 = comp == 1 ? (j0 + TY >= g.ny ? g.ny + 1 : j0 + TY) : jHiC;
-    int ok1 = comp == 2 ? (k0 + TZ >= g.nz ? g.nz + 1 : k0 + TZ) : kHiC;
-    int ni = oi1 - i0, nj = oj1 - j0, nk = ok1 - k0;
-    long base = comp == 0 ? 0 : (comp == 1 ? g.uN : (long)g.uN + g.vN);
+    const int d0 = comp == 0 ? g.nx + 1 : g.nx, d1 = comp == 1 ? g.ny + 1 : g.ny;
+    const int oi1 = comp == 0 ? uHi : iHiC, oj1 = comp == 1 ? vHi : jHiC, ok1 = comp == 2 ? wHi : kHiC;
+    const int ni = oi1 - i0, nj = oj1 - j0, nk = ok1 - k0;
+    const long base = comp == 0 ? 0 : (comp == 1 ? g.uN : (long)g.uN + g.vN);
     for (int e = threadIdx.x; e < ni * nj * nk; e += blockDim.x) {
-      int li = e % ni, lj = (e / ni) % nj, lk = e / (ni * nj);
-      int l = (lk * (TY + (comp == 1)) + lj) * (TX + (comp == 0)) + li;
-      float W = (float)(long long)accW[comp][l] * kFixInvF;
-      float M = (float)(long long)accM[comp][l] * kFixInvF;
-      long f = base + ((long)(k0 + lk) * d1 + (j0 + lj)) * d0 + (i0 + li);
+      const int li = e % ni, lj = (e / ni) % nj, lk = e / (ni * nj);
+      const long f = base + ((long)(k0 + lk) * d1 + (j0 + lj)) * d0 + (i0 + li);
+      float W = 0.f, M = 0.f;
+      if (!empty) {
+        const int l = (lk * (TY + (comp == 1)) + lj) * (TX + (comp == 0)) + li;
+        W = (float)(long long)accW[comp][l] * kFixInvF;
+        M = (float)(long long)accM[comp][l] * kFixInvF;
+      }
       vel[f] = W > 1e-8f ? M / W : M;
       wgt[f] = W;
     }
