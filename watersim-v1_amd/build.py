@@ -7,7 +7,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = ["ws_grid.hip", "ws_particles.hip", "ws_surface.hip", "ws_api.hip", "ws_api_surface.hip", "ws_thermal.hip", "ws_frame.hip", "ws_learn.hip", "ws_contacts.hip", "ws_balls.hip", "ws_metaballs.hip", "ws_batch.hip", "ws_noalloc.hip", "ws_sor_xcd.hip", "ws_extrap.hip", "ws_world.cpp"]
+SRC = ["ws_grid.hip", "ws_particles.hip", "ws_surface.hip", "ws_api.hip", "ws_api_surface.hip", "ws_thermal.hip", "ws_frame.hip", "ws_learn.hip", "ws_contacts.hip", "ws_balls.hip", "ws_metaballs.hip", "ws_batch.hip", "ws_noalloc.hip", "ws_sor_xcd.hip", "ws_extrap.hip", "ws_scan.hip", "ws_world.cpp"]
 OUT = os.path.join(HERE, "_build", "libwatersim_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # per-file additions: the ball, contact, metaball and surface code evaluate the reference's double

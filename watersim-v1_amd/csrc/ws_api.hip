@@ -141,8 +141,7 @@ int st_rasterize(ws_ctx *c) {
     hipLaunchKernelGGL(k_mark, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->geo, world_of(c), c->nP,
                        c->px, c->py, c->pz, c->pflag, c->cellType, c->cellCount, c->pcell, c->prank, c->ds);
   WS_LAUNCHED();
-  WS_HIP(hipcub::DeviceScan::ExclusiveSum(c->scanTmp, c->scanTmpBytes, c->cellCount, c->cellStart, nC + 1,
-                                          c->stream));
+  if (int r = scan_excl(c, c->cellCount, c->cellStart, nC + 1)) return r;
   if (c->nP > 0)
     hipLaunchKernelGGL(k_scatter, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->nP, c->pcell, c->prank,
                        c->cellStart, c->sortedIdx);
@@ -680,8 +679,7 @@ int pressure_setup(ws_ctx *c) {
     int nb = blocks_for(c->nC + 1, 256);
     hipLaunchKernelGGL(k_sor_count, dim3(nb), dim3(256), 0, c->stream, c->geo, c->nbm, c->sorOff);
     WS_LAUNCHED();
-    WS_HIP(hipcub::DeviceScan::ExclusiveSum(c->scanTmp, c->scanTmpBytes, c->sorOff, c->sorOff, (int)c->nC + 1,
-                                            c->stream));
+    if (int r = scan_excl(c, c->sorOff, c->sorOff, (long)c->nC + 1)) return r;
     hipLaunchKernelGGL(k_sor_compact, dim3(nb), dim3(256), 0, c->stream, c->geo, c->nbm, c->dg, c->rhs, c->pw,
                        c->sorOff, c->sorHalf, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->ds);
     WS_LAUNCHED();
@@ -840,6 +838,10 @@ int extrapolate_rows(ws_ctx *c) {
 
 // failures the device reports in its scalars (read after each copy of them)
 int device_failures(ws_ctx *c) {
+  if (c->hs->scanFailed) {
+    c->err = "a prefix sum's wait for an earlier tile timed out";
+    return WS_ESTATE;
+  }
   if (c->hs->exFailed) {
     c->err = c->hs->exFailed & 2 ? "extrapolation: a component's jobs were not all run (no workgroup on its XCD)"
                                  : "extrapolation: a wait for a neighbouring slab or chunk timed out";
@@ -870,12 +872,12 @@ int st_vapor(ws_ctx *c) {
   hipLaunchKernelGGL(k_vapor_a, dim3(nb), dim3(256), 0, c->stream, c->geo, w, P, c->nP, c->px, c->py, c->pz, c->pvx,
                      c->pvy, c->pvz, c->pflag, c->pAir, c->vscr, needA, c->ds);
   WS_LAUNCHED();
-  WS_HIP(hipcub::DeviceScan::ExclusiveSum(c->scanTmp, c->scanTmpBytes, needA, offA, c->nP, c->stream));
+  if (int r = scan_excl(c, needA, offA, c->nP)) return r;
   hipLaunchKernelGGL(k_vapor_b, dim3(nb), dim3(256), 0, c->stream, c->geo, P, c->nP, c->px, c->py, c->pz, c->pvx,
                      c->pvy, c->pvz, c->pflag, c->vscr, needA, offA, needB, c->ds, c->rng);
   hipLaunchKernelGGL(k_rng_advance, dim3(1), dim3(64), 0, c->stream, c->rng, 0, needA, offA, c->nP);
   WS_LAUNCHED();
-  WS_HIP(hipcub::DeviceScan::ExclusiveSum(c->scanTmp, c->scanTmpBytes, needB, offB, c->nP, c->stream));
+  if (int r = scan_excl(c, needB, offB, c->nP)) return r;
   hipLaunchKernelGGL(k_vapor_c, dim3(nb), dim3(256), 0, c->stream, c->geo, w, P, c->nP, c->px, c->py, c->pz, c->pvx,
                      c->pvy, c->pvz, c->pflag, c->pAir, c->pT, c->cellType, c->vscr, needB, offB, c->ds, c->rng);
   hipLaunchKernelGGL(k_rng_advance, dim3(1), dim3(64), 0, c->stream, c->rng, 1, needB, offB, c->nP);
@@ -1141,6 +1143,15 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
   c->scanTmpBytes = b1 > b2 ? b1 : b2;
   if (b3 > c->scanTmpBytes) c->scanTmpBytes = b3;
   if (hipMalloc(&c->scanTmp, c->scanTmpBytes) != hipSuccess) { ws_destroy(c); return WS_EHIP; }
+  c->scanTiles = scan_tiles_for(std::max((long)c->nC + 1, (long)cap));
+  const unsigned epoch1 = 1;  // status words start at tag 0, below every epoch
+  if (hipMalloc((void **)&c->scanStatus, sizeof(unsigned long long) * 2 * c->scanTiles) != hipSuccess ||
+      hipMalloc((void **)&c->scanCtl, sizeof(unsigned) * 4) != hipSuccess ||
+      hipMemset(c->scanStatus, 0, sizeof(unsigned long long) * 2 * c->scanTiles) != hipSuccess ||
+      hipMemcpy(c->scanCtl, &epoch1, sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess) {
+    ws_destroy(c);
+    return WS_EHIP;
+  }
   // the zeroing of every buffer above (on the context's stream) is done
   // before the host copies below (on the null stream) write into some of them
   if (hipStreamSynchronize(c->stream) != hipSuccess) { ws_destroy(c); return WS_EHIP; }
@@ -1227,6 +1238,8 @@ void ws_destroy(ws_ctx *c) {
                   c->pLight, c->pWx, c->pWy, c->pWz, c->pPh, c->evBuf, c->jmp, c->currField, c->smpBuf};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  if (c->scanStatus) (void)hipFree(c->scanStatus);
+  if (c->scanCtl) (void)hipFree(c->scanCtl);
   if (c->hs) (void)hipHostFree(c->hs);
   if (c->renderHost) (void)hipHostFree(c->renderHost);
   for (auto &e : c->ev)

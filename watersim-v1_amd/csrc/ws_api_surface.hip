@@ -141,8 +141,7 @@ int ws_splat_density(ws_ctx *c) {
     hipLaunchKernelGGL(k_bin_particles, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->geo, c->thr, c->nP,
                        c->px, c->py, c->pz, c->pflag, c->cellCount, c->pcell, c->prank);
   WS_LAUNCHED();
-  WS_HIP(hipcub::DeviceScan::ExclusiveSum(c->scanTmp, c->scanTmpBytes, c->cellCount, c->cellStart, nC + 1,
-                                          c->stream));
+  if (int r = scan_excl(c, c->cellCount, c->cellStart, (long)nC + 1)) return r;
   if (c->nP > 0)
     hipLaunchKernelGGL(k_bin_scatter, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->nP, c->pcell,
                        c->prank, c->cellStart, c->sortedIdx);

@@ -469,8 +469,7 @@ int balls_push(ws_ctx *c, bool rebin) {
   hipLaunchKernelGGL(k_rebin, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->geo, make_world(c), c->nP,
                      c->px, c->py, c->pz, c->pcell, c->prank, c->cellCount);
   WS_LAUNCHED();
-  WS_HIP(hipcub::DeviceScan::ExclusiveSum(c->scanTmp, c->scanTmpBytes, c->cellCount, c->cellStart, (int)c->nC + 1,
-                                          c->stream));
+  if (int r = scan_excl(c, c->cellCount, c->cellStart, (long)c->nC + 1)) return r;
   hipLaunchKernelGGL(k_scatter, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->nP, c->pcell, c->prank,
                      c->cellStart, c->sortedIdx);
   WS_LAUNCHED();

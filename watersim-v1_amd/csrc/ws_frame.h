@@ -34,4 +34,9 @@ int batch_substep_post2(ws_ctx *c);
 ExPlanet batch_ex_planet(ws_ctx *c);
 SorJob batch_sor_job(ws_ctx *c);
 void batch_solved(ws_ctx *c);
+// exclusive prefix sums on the context's stream, one launch each (ws_scan.hip;
+// 64-bit entries below 2^62); in may equal out
+int scan_excl(ws_ctx *c, const unsigned *in, unsigned *out, long n);
+int scan_excl(ws_ctx *c, const unsigned long long *in, unsigned long long *out, long n);
+long scan_tiles_for(long n);
 }  // namespace ws

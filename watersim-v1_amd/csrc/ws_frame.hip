@@ -86,7 +86,7 @@ int frame_evaporation(ws_ctx *c, double dt) {
   hipLaunchKernelGGL(k_evap_candidates, dim3(nb), dim3(256), 0, c->stream, c->geo, E, c->nP, c->px, c->py, c->pz,
                      c->pT, c->pflag, c->cellType, c->thr, cand, thresh);
   WS_LAUNCHED();
-  WS_HIP(hipcub::DeviceScan::ExclusiveSum(c->scanTmp, c->scanTmpBytes, cand, off, c->nP, c->stream));
+  if (int r = scan_excl(c, cand, off, c->nP)) return r;
   hipLaunchKernelGGL(k_evap_compact, dim3(nb), dim3(256), 0, c->stream, c->nP, cand, off, thresh, list, thrC);
   hipLaunchKernelGGL(k_evap_chooser, dim3(1), dim3(EV_THREADS), 0, c->stream, thrC, off, cand, c->nP, c->jmp,
                      c->jmp + EV_WIN, c->rng, evList, evState, evCount);

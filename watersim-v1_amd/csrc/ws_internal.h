@@ -86,6 +86,7 @@ struct DevScalars {
   // the row-pipelined extrapolation: 1 a poll gave up, 2 a component's jobs
   // were not all done (sticky until the host reads and clears it)
   unsigned exFailed;
+  unsigned scanFailed;  // a single-launch scan gave up waiting for an earlier tile
 };
 
 // Geometry and per-substep constants passed by value to kernels.
@@ -346,6 +347,11 @@ struct Ctx {
   int nPartials = 0;
   void *scanTmp = nullptr;
   size_t scanTmpBytes = 0;
+  // the single-launch scans (ws_scan.hip): status words (two per tile of the
+  // smallest tile size) and the launch epoch
+  unsigned long long *scanStatus = nullptr;
+  unsigned *scanCtl = nullptr;
+  long scanTiles = 0;
   // extrapolation tile schedule
   int4 *exTiles = nullptr;
   // the row-pipelined extrapolation (exRows 1; 0, the default: one launch
