@@ -610,11 +610,16 @@ __global__ __launch_bounds__(1024) void k_vc_sweepq(const unsigned *__restrict__
       q = __builtin_amdgcn_readfirstlane(q);
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
       const long t = t0 + (long)q;
+      // the step's lists are read whole (they lie inside the step's rows of
+      // candP and succ) together with its parcel and partner count, so the
+      // step's metadata costs one memory round trip; the count then masks them
+      const unsigned jpAll = lane < (unsigned)kVcK ? candP[t * kVcK + lane] : 0u;
+      const unsigned sAll = succ[t * (kVcK + 1) + lane];
       unsigned pi = __builtin_amdgcn_readfirstlane(stepP[t]);
       unsigned nc = __builtin_amdgcn_readfirstlane(stepN[t]);
       bool mine = lane < nc;
-      unsigned jp = mine ? candP[t * kVcK + lane] : 0u;
-      unsigned sT = lane <= nc ? succ[t * (kVcK + 1) + lane] : 0xffffffffu;
+      unsigned jp = mine ? jpAll : 0u;
+      unsigned sT = lane <= nc ? sAll : 0xffffffffu;
       Parcel I = parcel_load(P, pi, spin);
       Parcel J{};
       if (mine) J = parcel_load(P, jp, spin);
