@@ -266,6 +266,17 @@ __device__ __forceinline__ bool vapor_pair(Parcel &I, Parcel &J, float rad, floa
   return true;
 }
 
+// vapor_pair's first two tests alone: 2 the pass over this parcel's partners
+// stops here (the reference's early exit), 1 the pair touches, 0 neither
+// (the pair leaves both parcels as they are). The same expressions as
+// vapor_pair, so the decisions agree with it.
+__device__ __forceinline__ int vapor_pair_test(const Parcel &I, const Parcel &J, float rad, float rad2) {
+  if (J.x - I.x >= rad) return 2;
+  float ddx = J.x - I.x, ddy = J.y - I.y, ddz = J.z - I.z;
+  float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+  return d2 >= rad2 ? 0 : 1;
+}
+
 // the sweep of a small group, one group per thread; groups larger than
 // serialMax (and not marked for the serial path) go to k_vc_sweep
 __global__ void k_ct_vapor_groups(int n, const unsigned long long *__restrict__ gk,
@@ -623,19 +634,31 @@ __global__ __launch_bounds__(1024) void k_vc_sweepq(const unsigned *__restrict__
       Parcel I = parcel_load(P, pi, spin);
       Parcel J{};
       if (mine) J = parcel_load(P, jp, spin);
+      // The pairs are resolved in partner order, each on the parcel as the
+      // pairs before it left it. A pair that neither touches nor stops the
+      // pass changes nothing, so every lane tests its partner against the
+      // current parcel at once and only the first that touches (or stops)
+      // is resolved in turn; the lanes after it are tested again.
       bool wrote = false;
-      for (unsigned k = 0; k < nc; k++) {
+      for (unsigned k = 0; k < nc;) {
+        const bool act = lane >= k && lane < nc;
+        const int tst = act ? vapor_pair_test(I, J, rad, rad2) : 0;
+        const unsigned long long hit = __ballot(tst != 0);
+        if (!hit) break;
+        const unsigned f = (unsigned)__builtin_ctzll(hit);
+        if (__builtin_amdgcn_readlane(tst, (int)f) == 2) break;
         Parcel K;
-        K.x = bcast(J.x, k); K.y = bcast(J.y, k); K.z = bcast(J.z, k);
-        K.vx = bcast(J.vx, k); K.vy = bcast(J.vy, k); K.vz = bcast(J.vz, k);
-        K.wx = bcast(J.wx, k); K.wy = bcast(J.wy, k); K.wz = bcast(J.wz, k);
+        K.x = bcast(J.x, f); K.y = bcast(J.y, f); K.z = bcast(J.z, f);
+        K.vx = bcast(J.vx, f); K.vy = bcast(J.vy, f); K.vz = bcast(J.vz, f);
+        K.wx = bcast(J.wx, f); K.wy = bcast(J.wy, f); K.wz = bcast(J.wz, f);
         bool touched;
         bool go = vapor_pair(I, K, rad, rad2, spin, &touched);
-        if (touched && lane == k) {
+        if (touched && lane == f) {
           J = K;
           wrote = true;
         }
         if (!go) break;
+        k = f + 1;
       }
       if (wrote) parcel_store(P, jp, J, spin);
       if (lane == 0) parcel_store(P, pi, I, spin);
