@@ -277,20 +277,22 @@ __device__ __forceinline__ int vapor_pair_test(const Parcel &I, const Parcel &J,
   return d2 >= rad2 ? 0 : 1;
 }
 
-// the sweep of a small group, one group per thread; groups larger than
-// serialMax (and not marked for the serial path) go to k_vc_sweep
-__global__ void k_ct_vapor_groups(int n, const unsigned long long *__restrict__ gk,
-                                  const unsigned long long *__restrict__ xk, const unsigned *__restrict__ ends,
-                                  float rad, int spin, VaporState P, int serialMax,
-                                  const unsigned *__restrict__ serialOnly) {
-  int t0 = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t0 >= n) return;
-  if (t0 > 0 && (gk[t0] >> 32) == (gk[t0 - 1] >> 32)) return;  // not the first member of its group
-  unsigned root = (unsigned)(gk[t0] >> 32);
-  int t1 = (int)ends[root];
-  if (t1 - t0 < 2) return;
-  if (t1 - t0 > serialMax && !serialOnly[root]) return;
-  const float rad2 = rad * rad;
+__device__ __forceinline__ float bcast(float v, unsigned k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)k));
+}
+
+__device__ __forceinline__ Parcel parcel_bcast(const Parcel &J, unsigned k) {
+  Parcel K;
+  K.x = bcast(J.x, k); K.y = bcast(J.y, k); K.z = bcast(J.z, k);
+  K.vx = bcast(J.vx, k); K.vy = bcast(J.vy, k); K.vz = bcast(J.vz, k);
+  K.wx = bcast(J.wx, k); K.wy = bcast(J.wy, k); K.wz = bcast(J.wz, k);
+  return K;
+}
+
+// the reference's serial sweep of one group by one thread
+__device__ void vapor_group_serial(int t0, int t1, const unsigned long long *__restrict__ gk,
+                                   const unsigned long long *__restrict__ xk, float rad, float rad2, int spin,
+                                   const VaporState &P) {
   for (int a = t0; a < t1; a++) {
     unsigned i = member(gk, xk, a);
     Parcel I = parcel_load(P, i, spin);
@@ -310,6 +312,66 @@ __global__ void k_ct_vapor_groups(int n, const unsigned long long *__restrict__ 
       if (!go) break;
     }
     parcel_store(P, i, I, spin);
+  }
+}
+
+// The sweeps of the small groups (groups larger than serialMax and not
+// marked for the serial path go to the dependency-ordered sweep). A
+// wavefront takes the groups that start among its 64 positions, one at a
+// time: lane l holds member l's state in registers for the whole sweep. Step
+// a broadcasts member a, tests every later member against it at once (a pair
+// that neither touches nor stops the pass changes nothing) and resolves the
+// touching pairs in member order, each on the parcel as the pairs before it
+// left it: the reference's serial sweep, with the states never leaving the
+// registers. A group of more than 64 members runs the serial sweep on one
+// lane.
+__global__ __launch_bounds__(256) void k_ct_vapor_groups(int n, const unsigned long long *__restrict__ gk,
+                                                         const unsigned long long *__restrict__ xk,
+                                                         const unsigned *__restrict__ ends, float rad, int spin,
+                                                         VaporState P, int serialMax,
+                                                         const unsigned *__restrict__ serialOnly) {
+  const unsigned lane = threadIdx.x & 63;
+  const int base = (int)((blockIdx.x * blockDim.x + threadIdx.x) & ~63u);
+  const int t = base + (int)lane;
+  // the first member of a group (its key's root differs from the previous)
+  const bool first = t < n && (t == 0 || (gk[t] >> 32) != (gk[t - 1] >> 32));
+  unsigned long long starts = __ballot(first);
+  const float rad2 = rad * rad;
+  while (starts) {
+    const int t0 = base + __builtin_ctzll(starts);
+    starts &= starts - 1;
+    const unsigned root = (unsigned)(gk[t0] >> 32);
+    const int t1 = (int)ends[root];
+    const int m = t1 - t0;
+    if (m < 2) continue;
+    if (m > serialMax && !serialOnly[root]) continue;
+    if (m > 64) {
+      if (lane == 0) vapor_group_serial(t0, t1, gk, xk, rad, rad2, spin, P);
+      continue;
+    }
+    const bool in = (int)lane < m;
+    const unsigned me = in ? member(gk, xk, t0 + (int)lane) : 0u;
+    Parcel S{};
+    if (in) S = parcel_load(P, me, spin);
+    for (int a = 0; a + 1 < m; a++) {
+      Parcel I = parcel_bcast(S, (unsigned)a);
+      for (int k = a + 1; k < m;) {
+        const bool act = (int)lane >= k && in;
+        const int tst = act ? vapor_pair_test(I, S, rad, rad2) : 0;
+        const unsigned long long hit = __ballot(tst != 0);
+        if (!hit) break;
+        const unsigned f = (unsigned)__builtin_ctzll(hit);
+        if (__builtin_amdgcn_readlane(tst, (int)f) == 2) break;
+        Parcel K = parcel_bcast(S, f);
+        bool touched;
+        bool go = vapor_pair(I, K, rad, rad2, spin, &touched);
+        if (touched && lane == f) S = K;
+        if (!go) break;
+        k = (int)f + 1;
+      }
+      if ((int)lane == a) S = I;
+    }
+    if (in) parcel_store(P, me, S, spin);
   }
 }
 
@@ -556,10 +618,6 @@ constexpr int kVcQ = 16000;  // steps of a group kept in LDS
 constexpr unsigned short kVcEmpty = 0xffffu;
 constexpr int kVcIdle = 1 << 22;  // polls without progress before a group is given up
 
-__device__ __forceinline__ float bcast(float v, unsigned k) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)k));
-}
-
 __global__ __launch_bounds__(1024) void k_vc_sweepq(const unsigned *__restrict__ big, const unsigned *__restrict__ nBig,
                                                     const unsigned *__restrict__ stepP,
                                                     const unsigned *__restrict__ stepN,
@@ -647,10 +705,7 @@ __global__ __launch_bounds__(1024) void k_vc_sweepq(const unsigned *__restrict__
         if (!hit) break;
         const unsigned f = (unsigned)__builtin_ctzll(hit);
         if (__builtin_amdgcn_readlane(tst, (int)f) == 2) break;
-        Parcel K;
-        K.x = bcast(J.x, f); K.y = bcast(J.y, f); K.z = bcast(J.z, f);
-        K.vx = bcast(J.vx, f); K.vy = bcast(J.vy, f); K.vz = bcast(J.vz, f);
-        K.wx = bcast(J.wx, f); K.wy = bcast(J.wy, f); K.wz = bcast(J.wz, f);
+        Parcel K = parcel_bcast(J, f);
         bool touched;
         bool go = vapor_pair(I, K, rad, rad2, spin, &touched);
         if (touched && lane == f) {
