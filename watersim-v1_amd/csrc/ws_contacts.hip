@@ -389,45 +389,101 @@ __global__ __launch_bounds__(256) void k_ct_vapor_groups(int n, const unsigned l
 // step is always ready, so every poll round completes at least one step.
 constexpr int kVcK = 63;  // partners kept per parcel (more: the group is swept serially); the wavefront step holds kVcK + 1 successors in its 64 lanes
 
+// per group root: whether the group is larger than serialMax (only those
+// groups' steps can run in dependency order)
+__global__ void k_vc_mark_big(int n, const unsigned long long *__restrict__ gk, const unsigned *__restrict__ ends,
+                              int serialMax, unsigned *__restrict__ gbig) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  if (t > 0 && (gk[t] >> 32) == (gk[t - 1] >> 32)) return;
+  const unsigned root = (unsigned)(gk[t] >> 32);
+  gbig[root] = (int)ends[root] - t > serialMax;
+}
+
 // later partners of each parcel: same group, later x rank, closer than thr
-// at the start of the pass, sorted by rank
-__global__ void k_vc_cand(Geo g, int n, const unsigned *__restrict__ list, const float *__restrict__ px,
-                          const float *__restrict__ py, const float *__restrict__ pz,
-                          const unsigned *__restrict__ cellOf, const unsigned *__restrict__ start,
-                          const Slot *__restrict__ slot, const unsigned long long *__restrict__ slotKey, int span,
-                          const unsigned long long *__restrict__ key, float thr2, unsigned *__restrict__ cand,
-                          unsigned *__restrict__ ncand, unsigned *__restrict__ serialOnly) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+// at the start of the pass, sorted by rank (at most kVcK; a parcel with more
+// marks its group for the serial sweep). Parcels of groups no larger than
+// serialMax get none: their groups are swept whole by one wavefront, and
+// their steps then only take part in the pair keys as themselves.
+// One wavefront per parcel: the lanes test the slots of the neighbouring
+// cells 64 at a time, the partners found are gathered in LDS in the order
+// found, and a bitonic sort across the lanes puts them in rank order (ranks
+// are distinct).
+__global__ __launch_bounds__(256) void k_vc_cand(Geo g, int n, const unsigned *__restrict__ list,
+                                                 const unsigned *__restrict__ gbig, const float *__restrict__ px,
+                                                 const float *__restrict__ py, const float *__restrict__ pz,
+                                                 const unsigned *__restrict__ cellOf, const unsigned *__restrict__ start,
+                                                 const Slot *__restrict__ slot,
+                                                 const unsigned long long *__restrict__ slotKey, int span,
+                                                 const unsigned long long *__restrict__ key, float thr2,
+                                                 unsigned *__restrict__ cand, unsigned *__restrict__ ncand,
+                                                 unsigned *__restrict__ serialOnly) {
+  __shared__ unsigned sRank[4][64], sO[4][64];
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int e = (int)(blockIdx.x * 4 + wave);
   if (e >= n) return;
   const unsigned long long ke = key[e];
-  unsigned p = list[e], re = (unsigned)ke, root = (unsigned)(ke >> 32);
-  float x = px[p], y = py[p], z = pz[p];
-  int c = cellOf[e];
-  int ci = c % g.nx, cj = (c / g.nx) % g.ny, ck = c / (g.nx * g.ny);
-  unsigned *mine = cand + (long)e * kVcK;
-  unsigned rk[kVcK];
-  int m = 0;
-  bool over = false;
+  const unsigned p = list[e], re = (unsigned)ke, root = (unsigned)(ke >> 32);
+  if (!gbig[root]) {
+    if (lane == 0) ncand[e] = 0u;
+    return;
+  }
+  const float x = px[p], y = py[p], z = pz[p];
+  const int c = cellOf[e];
+  const int ci = c % g.nx, cj = (c / g.nx) % g.ny, ck = c / (g.nx * g.ny);
+  unsigned m = 0;  // partners found (wave-uniform)
   const int i0 = max(ci - span, 0), i1 = min(ci + span, g.nx - 1);
   for (int kk = max(ck - span, 0); kk <= min(ck + span, g.nz - 1); kk++)
     for (int jj = max(cj - span, 0); jj <= min(cj + span, g.ny - 1); jj++) {
       const int row = (kk * g.ny + jj) * g.nx;
-      for (unsigned b = start[row + i0], be = start[row + i1 + 1]; b < be; b++) {
-        const unsigned long long ko = slotKey[b];
-        unsigned ro = (unsigned)ko;
-        if (ro <= re || (unsigned)(ko >> 32) != root) continue;
-        const Slot s = slot[b];
-        float ddx = s.x - x, ddy = s.y - y, ddz = s.z - z;
-        if (ddx * ddx + ddy * ddy + ddz * ddz >= thr2) continue;
-        if (m == kVcK) { over = true; continue; }
-        int k = m++;
-        while (k > 0 && rk[k - 1] > ro) { rk[k] = rk[k - 1]; mine[k] = mine[k - 1]; k--; }
-        rk[k] = ro;
-        mine[k] = s.o;
+      const unsigned b0 = start[row + i0], be = start[row + i1 + 1];
+      for (unsigned b1 = b0; b1 < be; b1 += 64) {
+        const unsigned b = b1 + lane;
+        bool ok = false;
+        unsigned ro = 0, so = 0;
+        if (b < be) {
+          const unsigned long long ko = slotKey[b];
+          ro = (unsigned)ko;
+          if (ro > re && (unsigned)(ko >> 32) == root) {
+            const Slot sl = slot[b];
+            float ddx = sl.x - x, ddy = sl.y - y, ddz = sl.z - z;
+            ok = !(ddx * ddx + ddy * ddy + ddz * ddz >= thr2);
+            so = sl.o;
+          }
+        }
+        const unsigned long long bal = __ballot(ok);
+        const unsigned at = m + (unsigned)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+        if (ok && at < 64) {
+          sRank[wave][at] = ro;
+          sO[wave][at] = so;
+        }
+        m += (unsigned)__popcll(bal);
       }
     }
-  ncand[e] = (unsigned)m;
-  if (over) __hip_atomic_store(&serialOnly[root], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __builtin_amdgcn_wave_barrier();
+  const unsigned kept = m < (unsigned)kVcK ? m : (unsigned)kVcK;
+  unsigned rk = lane < kept ? sRank[wave][lane] : 0xffffffffu;
+  unsigned o = lane < kept ? sO[wave][lane] : 0u;
+  // bitonic sort of the 64 lanes by rank, ascending
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const unsigned prk = __shfl_xor(rk, j, 64), po = __shfl_xor(o, j, 64);
+      const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+      const bool take = lower == up ? prk < rk : prk > rk;
+      if (take) {
+        rk = prk;
+        o = po;
+      }
+    }
+  }
+  if (lane < kept) cand[(long)e * kVcK + lane] = o;
+  if (lane == 0) {
+    ncand[e] = kept;
+    if (m > (unsigned)kVcK) __hip_atomic_store(&serialOnly[root], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 __global__ void k_vc_pair_count(int n, const unsigned *__restrict__ ncand, unsigned *__restrict__ cnt) {
@@ -1113,6 +1169,7 @@ struct Contacts {
   unsigned *cand = nullptr, *ncand = nullptr, *pairOff = nullptr, *need = nullptr, *done = nullptr,
            *serialOnly = nullptr, *big = nullptr, *vcCounters = nullptr, *chainStart = nullptr, *candT = nullptr,
            *candP = nullptr, *tpos = nullptr, *stepP = nullptr, *stepN = nullptr, *succ = nullptr, *depc = nullptr;
+  unsigned *gbig = nullptr;  // per group root: 1 if the group is larger than serialMax
   unsigned long long *pairKeys = nullptr, *pairKeys2 = nullptr;
   void *vtmp = nullptr;
   size_t vtmpBytes = 0;
@@ -1214,7 +1271,7 @@ int vapor_ensure(ws_ctx *c, long nV) {
   long n = nV + nV / 4 + 1024;
   void *old[] = {k->cand, k->ncand, k->pairOff, k->need, k->done, k->serialOnly, k->big, k->vcCounters,
                  k->chainStart, k->pairKeys, k->pairKeys2, k->vtmp, k->candT, k->candP, k->tpos, k->stepP,
-                 k->stepN, k->succ, k->depc};
+                 k->stepN, k->succ, k->depc, k->gbig};
   for (void *p : old)
     if (p) (void)hipFree(p);
   k->vcap = 0;
@@ -1227,7 +1284,8 @@ int vapor_ensure(ws_ctx *c, long nV) {
       (r = calloc_dev(c, &k->chainStart, n)) || (r = calloc_dev(c, &k->pairKeys, np)) ||
       (r = calloc_dev(c, &k->pairKeys2, np)) || (r = calloc_dev(c, &k->candT, n * kVcK)) ||
       (r = calloc_dev(c, &k->candP, n * kVcK)) || (r = calloc_dev(c, &k->tpos, n)) ||
-      (r = calloc_dev(c, &k->stepP, n)) || (r = calloc_dev(c, &k->stepN, n)) || (r = calloc_dev(c, &k->succ, np)) || (r = calloc_dev(c, &k->depc, n)))
+      (r = calloc_dev(c, &k->stepP, n)) || (r = calloc_dev(c, &k->stepN, n)) || (r = calloc_dev(c, &k->succ, np)) || (r = calloc_dev(c, &k->depc, n)) ||
+      (r = calloc_dev(c, &k->gbig, n)))
     return r;
   size_t a = 0, b = 0;
   WS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, a, k->ncand, k->pairOff, (int)(n + 1), c->stream));
@@ -1314,7 +1372,9 @@ int frame_vapor_collisions(ws_ctx *c) {
     WS_HIP(hipStreamSynchronize(c->stream));
     if (nBig > 0) {
       // partners, the steps touching each parcel in sweep order, positions
-      hipLaunchKernelGGL(k_vc_cand, dim3(nb), dim3(256), 0, c->stream, c->geo, nV, k->vlist, c->px, c->py, c->pz,
+      hipLaunchKernelGGL(k_vc_mark_big, dim3(nb), dim3(256), 0, c->stream, nV, k->gk, k->gend, serialMax, k->gbig);
+      hipLaunchKernelGGL(k_vc_cand, dim3((nV + 3) / 4), dim3(256), 0, c->stream, c->geo, nV, k->vlist, k->gbig, c->px,
+                         c->py, c->pz,
                          k->cellOf, k->start, k->slot, k->slotKey, cell_span(c, 2.0 * rad), k->gk2,
                          4.f * rad * rad, k->cand, k->ncand, k->serialOnly);
       hipLaunchKernelGGL(k_vc_pair_count, dim3(blocks_for(nV + 1, 256)), dim3(256), 0, c->stream, nV, k->ncand,
