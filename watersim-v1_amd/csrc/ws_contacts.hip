@@ -119,25 +119,34 @@ __device__ __forceinline__ unsigned uf_find(unsigned *parent, unsigned x) {
 // root); with a mask only masked members take part, the others stay alone.
 // The cells of one row of the search box are adjacent in the cell order, so
 // their members form one run of slots.
-__global__ void k_ct_union(Geo g, int n, const unsigned *__restrict__ list, const float *__restrict__ px,
-                           const float *__restrict__ py, const float *__restrict__ pz,
-                           const unsigned *__restrict__ cellOf, const unsigned *__restrict__ start,
-                           const Slot *__restrict__ slot, float thr2, int span, const unsigned *__restrict__ mask,
-                           unsigned *__restrict__ parent) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+// one wavefront per member: the lanes test the slots of the neighbouring
+// cells 64 at a time and link each member found closer than the threshold.
+// Links always put the larger root under the smaller, so every component
+// ends with its smallest member as root whatever the order of the links.
+__global__ __launch_bounds__(256) void k_ct_union(Geo g, int n, const unsigned *__restrict__ list,
+                                                  const float *__restrict__ px, const float *__restrict__ py,
+                                                  const float *__restrict__ pz, const unsigned *__restrict__ cellOf,
+                                                  const unsigned *__restrict__ start, const Slot *__restrict__ slot,
+                                                  float thr2, int span, const unsigned *__restrict__ mask,
+                                                  unsigned *__restrict__ parent) {
+  const unsigned lane = threadIdx.x & 63;
+  const int e = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (e >= n) return;
   if (mask && !mask[e]) return;
-  unsigned p = list[e];
-  float x = px[p], y = py[p], z = pz[p];
-  int c = cellOf[e];
-  int ci = c % g.nx, cj = (c / g.nx) % g.ny, ck = c / (g.nx * g.ny);
+  const unsigned p = list[e];
+  const float x = px[p], y = py[p], z = pz[p];
+  const int c = cellOf[e];
+  const int ci = c % g.nx, cj = (c / g.nx) % g.ny, ck = c / (g.nx * g.ny);
   const int i0 = max(ci - span, 0), i1 = min(ci + span, g.nx - 1);
   for (int kk = max(ck - span, 0); kk <= min(ck + span, g.nz - 1); kk++)
     for (int jj = max(cj - span, 0); jj <= min(cj + span, g.ny - 1); jj++) {
       const int row = (kk * g.ny + jj) * g.nx;
-      for (unsigned b = start[row + i0], be = start[row + i1 + 1]; b < be; b++) {
+      const unsigned b0 = start[row + i0], be = start[row + i1 + 1];
+      for (unsigned b1 = b0; b1 < be; b1 += 64) {
+        const unsigned b = b1 + lane;
+        if (b >= be) continue;
         const Slot s = slot[b];
-        unsigned o = s.o;
+        const unsigned o = s.o;
         if (o <= (unsigned)e) continue;
         if (mask && !mask[o]) continue;
         float ddx = s.x - x, ddy = s.y - y, ddz = s.z - z;
@@ -1253,7 +1262,8 @@ int group_subset(ws_ctx *c, int n, const unsigned *list, float thr, const unsign
   int r;
   if ((r = bin_subset(c, n, list, k->cellOf, k->rankIn, k->start, k->binned, k->slot))) return r;
   hipLaunchKernelGGL(k_ct_iota, dim3(nb), dim3(256), 0, c->stream, n, k->parent);
-  hipLaunchKernelGGL(k_ct_union, dim3(nb), dim3(256), 0, c->stream, c->geo, n, list, c->px, c->py, c->pz, k->cellOf,
+  hipLaunchKernelGGL(k_ct_union, dim3((n + 3) / 4), dim3(256), 0, c->stream, c->geo, n, list, c->px, c->py, c->pz,
+                     k->cellOf,
                      k->start, k->slot, thr * thr, cell_span(c, thr), mask, k->parent);
   hipLaunchKernelGGL(k_ct_groupkeys, dim3(nb), dim3(256), 0, c->stream, n, k->parent, k->rankOf, mask, k->cellOf,
                      k->rankIn, k->start, k->gk2, k->slotKey);
