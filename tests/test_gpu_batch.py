@@ -19,6 +19,8 @@ solver with a host-side SOR loop.
 - The batched-planets workload itself, 64 planets of 32^3..96^3, runs from
   its graph with every planet at the tolerance, and one planet of each size
   equals its own single-planet run.
+- The warm start's forward pass, batched over the planets, gives each
+  planet the bits of its own pass in all three forms of the convolution.
 """
 import os
 
@@ -156,6 +158,48 @@ def test_batch_equals_single_planets(graph, cells_per_group, xcd_local, max_grou
             int(b.get("xcdLocalPlanets"))))
         assert int(b.get("xcdLocalPlanets")) == local_planets
     b.close()
+    for s in ref + bat:
+        s.close()
+
+
+@pytest.mark.parametrize("net_batch", [1, 0])
+def test_batched_network_pass_equals_planets_own(net_batch):
+    """The warm start's forward pass and correction run as one launch per
+    network step over every planet with the warm start on (netBatch 1) or
+    as each planet's own launches (0). The planets cover the three forms of
+    the convolution (a 32^3 grid coarsened to 16^3: four waves per 16 voxels;
+    64^3 at full resolution: LDS-tiled at its finest level; 48^3 with
+    netTiled 0: untiled) and both corrections (coarsened and full), beside a
+    planet without the warm start. Either way every planet's state equals
+    its own ws_run_substeps run bit for bit, eagerly and from the graph."""
+    if not os.path.exists(WEIGHTS):
+        pytest.skip("no trained warm-start weights")
+    spec = [(32, 21, True, dict(netCoarse=1)), (64, 22, True, dict(netCoarse=0)),
+            (48, 23, True, dict(netCoarse=0, netTiled=0)), (32, 24, False, {})]
+
+    def make():
+        out = []
+        for n, sd, wm, extra in spec:
+            s = planet(n, sd, wm)
+            if extra:
+                s.set(**extra)
+            out.append(s)
+        return out
+    ref, bat = make(), make()
+    for s in ref:
+        s.set(sorMode=4, sorThreads=512)
+    for graph in (False, True):
+        b = watersim_hip.Batch(bat, graph=graph, threads=512)
+        b.set(netBatch=net_batch)
+        assert int(b.get("netBatch")) == net_batch
+        b.run(2)
+        for s in ref:
+            s.run_substeps(2)
+        for i, (r, s) in enumerate(zip(ref, bat)):
+            assert_same(r, s, "graph %d planet %d" % (graph, i))
+            assert r.get("sweepsTotal") == s.get("sweepsTotal")
+        b.close()
+    print("  sweeps %s" % [int(s.get("sweepsTotal")) for s in bat])
     for s in ref + bat:
         s.close()
 

@@ -695,7 +695,7 @@ int pressure_setup(ws_ctx *c) {
   if (c->warmStart) {
     hipEvent_t span = nullptr;
     if (c->timing && (r = span_begin(c, &span))) return r;
-    if ((r = net_warm_start(c))) return r;
+    if ((r = c->netDeferred ? net_warm_inputs(c) : net_warm_start(c))) return r;
     if (span && (r = span_end(c, span, SPAN_NET, 0))) return r;
   }
   return WS_OK;

@@ -1,10 +1,11 @@
 // Batched planets: many contexts advanced together by device-dt-law
-// substeps. Each planet's stages are queued on its own stream, except two
-// that run for all planets at once on the batch stream: the pressure solve
-// (one launch of k_sor_solve_batch: every planet's whole solve, a few
-// workgroups each, the residual checks on the device) and the extrapolation
-// wavefront (one launch per tile diagonal over every planet's tiles of that
-// diagonal). A substep costs no host round trip. With a graph, the batched
+// substeps. Each planet's stages are queued on its own stream, except three
+// that run for all planets at once on the batch stream: the learned warm
+// start's forward pass and correction (one launch per step of the network
+// over every planet's grid), the pressure solve (one launch of
+// k_sor_solve_batch: every planet's whole solve, a few workgroups each, the
+// residual checks on the device) and the extrapolation wavefront (one launch
+// per tile diagonal over every planet's tiles of that diagonal). A substep costs no host round trip. With a graph, the batched
 // substep (fork to the planets' streams, the joint solve, join) is captured
 // once per run and replayed for every substep.
 #include <algorithm>
@@ -15,6 +16,7 @@
 #include <vector>
 
 #include "ws_frame.h"
+#include "ws_learn.h"
 
 using namespace ws;
 
@@ -40,6 +42,10 @@ struct ws_batch {
   // place holds the planets' placement words (G + 1 each, at placeOff)
   int xcdLocal = 1;
   unsigned *place = nullptr;
+  // netBatch: the warm start's forward passes run as one launch per step
+  // over all planets (else each planet runs its own on its stream)
+  int netBatch = 1;
+  NetBatch *net = nullptr;
   std::vector<int> placeOff;
   // the planets' extrapolation schedules merged by tile diagonal: one launch
   // per diagonal covers every planet's tiles of that diagonal
@@ -206,6 +212,8 @@ int enqueue_substep(ws_batch *b) {
     BH(hipEventRecord(b->pre[i], c->stream));
     BH(hipStreamWaitEvent(b->stream, b->pre[i], 0));
   }
+  net_batch_forward(b->net, b->stream);
+  BH(hipGetLastError());
   if (b->timing) BH(hipEventRecord(b->t0, b->stream));
   launch_solves(b);
   BH(hipGetLastError());
@@ -302,55 +310,9 @@ int finish(ws_batch *b) {
   return WS_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int ws_batch_create(ws_batch **out, ws_ctx *const *ctx, int n, int use_graph, int threads) {
-  if (!out || !ctx || n < 1 || (threads != 256 && threads != 512 && threads != 1024)) return WS_EINVAL;
-  *out = nullptr;
-  for (int i = 0; i < n; i++)
-    if (!ctx[i]) return WS_EINVAL;
-  ws_batch *b = new ws_batch();
-  b->ctx.assign(ctx, ctx + n);
-  b->useGraph = use_graph != 0;
-  b->threads = threads;
-  b->pre.assign(n, nullptr);
-  b->post.assign(n, nullptr);
-  b->mid.assign(n, nullptr);
-  b->hjobs.resize(n);
-  b->hs.resize(n);
-  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->solved, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreate(&b->t0);
-  if (e == hipSuccess) e = hipEventCreate(&b->t1);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->extrapolated, hipEventDisableTiming);
-  for (int i = 0; i < n && e == hipSuccess; i++) {
-    e = hipEventCreateWithFlags(&b->pre[i], hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->post[i], hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->mid[i], hipEventDisableTiming);
-  }
-  // one resident 1024-thread workgroup per compute unit
-  int dev = 0, cus = 0;
-  if (e == hipSuccess) e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e == hipSuccess && cus > 0 && cus < b->maxGroups) b->maxGroups = cus;
-  if (e == hipSuccess) e = hipMalloc((void **)&b->jobs, sizeof(SorJob) * n);
-  if (e == hipSuccess) e = hipMalloc((void **)&b->arrive, sizeof(unsigned) * 2 * n);
-  if (e == hipSuccess) e = hipMemset(b->arrive, 0, sizeof(unsigned) * 2 * n);
-  if (e != hipSuccess || plan_extrapolation(b) != WS_OK) {
-    ws_batch_destroy(b);
-    return WS_EHIP;
-  }
-  *out = b;
-  return WS_OK;
-}
-
-int ws_batch_run(ws_batch *b, int substeps) {
-  if (!b || substeps < 0) return WS_EINVAL;
+// the substeps of a run: eagerly, timed, or captured once and replayed
+int run_substeps(ws_batch *b, int substeps) {
   int r;
-  if ((r = check_planets(b))) return r;
   // the solves' descriptions as the contexts stand now (tolerances, caps,
   // omega); every planet runs the one-workgroup solve
   for (size_t i = 0; i < b->ctx.size(); i++) {
@@ -402,6 +364,69 @@ int ws_batch_run(ws_batch *b, int substeps) {
   }
   return finish(b);
 }
+}  // namespace
+
+extern "C" {
+
+int ws_batch_create(ws_batch **out, ws_ctx *const *ctx, int n, int use_graph, int threads) {
+  if (!out || !ctx || n < 1 || (threads != 256 && threads != 512 && threads != 1024)) return WS_EINVAL;
+  *out = nullptr;
+  for (int i = 0; i < n; i++)
+    if (!ctx[i]) return WS_EINVAL;
+  ws_batch *b = new ws_batch();
+  b->ctx.assign(ctx, ctx + n);
+  b->useGraph = use_graph != 0;
+  b->threads = threads;
+  b->pre.assign(n, nullptr);
+  b->post.assign(n, nullptr);
+  b->mid.assign(n, nullptr);
+  b->hjobs.resize(n);
+  b->hs.resize(n);
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->solved, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreate(&b->t0);
+  if (e == hipSuccess) e = hipEventCreate(&b->t1);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->extrapolated, hipEventDisableTiming);
+  for (int i = 0; i < n && e == hipSuccess; i++) {
+    e = hipEventCreateWithFlags(&b->pre[i], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->post[i], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->mid[i], hipEventDisableTiming);
+  }
+  // one resident 1024-thread workgroup per compute unit
+  int dev = 0, cus = 0;
+  if (e == hipSuccess) e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess && cus > 0 && cus < b->maxGroups) b->maxGroups = cus;
+  if (e == hipSuccess) e = hipMalloc((void **)&b->jobs, sizeof(SorJob) * n);
+  if (e == hipSuccess) e = hipMalloc((void **)&b->arrive, sizeof(unsigned) * 2 * n);
+  if (e == hipSuccess) e = hipMemset(b->arrive, 0, sizeof(unsigned) * 2 * n);
+  if (e != hipSuccess || plan_extrapolation(b) != WS_OK) {
+    ws_batch_destroy(b);
+    return WS_EHIP;
+  }
+  *out = b;
+  return WS_OK;
+}
+
+int ws_batch_run(ws_batch *b, int substeps) {
+  if (!b || substeps < 0) return WS_EINVAL;
+  int r;
+  if ((r = check_planets(b))) return r;
+  // the warm starts' forward passes: batched, or each planet's own
+  net_batch_free(b->net);
+  b->net = nullptr;
+  if (b->netBatch) {
+    int bad = -1;
+    if ((r = net_batch_plan(b->ctx.data(), (int)b->ctx.size(), &b->net, &bad)))
+      return bad >= 0 ? planet_fail(b, bad, r) : fail(b, "warm-start network: batched pass", r);
+  }
+  for (ws_ctx *c : b->ctx) c->netDeferred = b->net && c->warmStart;
+  r = run_substeps(b, substeps);
+  for (ws_ctx *c : b->ctx) c->netDeferred = 0;
+  return r;
+}
+
 
 int ws_batch_set(ws_batch *b, const char *name, double v) {
   if (!b || !name) return WS_EINVAL;
@@ -422,6 +447,10 @@ int ws_batch_set(ws_batch *b, const char *name, double v) {
   }
   if (!strcmp(name, "xcdLocal")) {
     b->xcdLocal = v != 0;
+    return WS_OK;
+  }
+  if (!strcmp(name, "netBatch")) {
+    b->netBatch = v != 0;
     return WS_OK;
   }
   if (!strcmp(name, "cellsPerGroup")) {
@@ -454,6 +483,7 @@ int ws_batch_get(ws_batch *b, const char *name, double *v) {
   else if (!strcmp(name, "maxGroups")) *v = b->maxGroups;
   else if (!strcmp(name, "groups")) *v = b->nBlocks;
   else if (!strcmp(name, "xcdLocal")) *v = b->xcdLocal;
+  else if (!strcmp(name, "netBatch")) *v = b->netBatch;
   else if (!strcmp(name, "xcdLocalPlanets")) {
     // planets whose last joint solve ran with its workgroups on one XCD
     int k = 0;
@@ -493,6 +523,7 @@ void ws_batch_destroy(ws_batch *b) {
   if (b->blk) (void)hipFree(b->blk);
   if (b->arrive) (void)hipFree(b->arrive);
   if (b->place) (void)hipFree(b->place);
+  net_batch_free(b->net);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }

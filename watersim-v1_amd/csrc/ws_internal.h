@@ -238,6 +238,9 @@ struct Ctx {
   int warmStart = 0, recordSamples = 0;
   int netTiled = 1;  // convolutions staged through LDS (0: the untiled kernel)
   int netCoarse = 0;  // the network runs on the grid coarsened 2x per axis
+  // set by the batched driver: the forward pass and the correction run for
+  // all planets at once after the planets' solve set-up
+  int netDeferred = 0;
   double netLossCoarse = 0;  // weight of the 4x4x4-averaged error term of the training loss
   // balls (moving solids, solver.js:115, 918-938, 1005-1026, 2294-2564):
   // nine doubles per ball on the device (x y z vx vy vz r rho submerged),

@@ -63,6 +63,16 @@ int net_pack(ws_ctx *c);
 void net_free(ws_ctx *c);
 // the warm start inside the pressure solve: q += s * net(r0, types, q)
 int net_warm_start(ws_ctx *c);
+// many planets' forward passes and corrections at once, one launch per
+// step: the planets' solves call net_warm_inputs in place of net_warm_start,
+// then net_batch_forward runs on a stream ordered after all of them. A plan
+// holds the planets' buffers (valid while their grids and networks stand);
+// *bad names the planet of an error
+struct NetBatch;
+int net_warm_inputs(ws_ctx *c);
+int net_batch_plan(ws_ctx *const *cs, int n, NetBatch **nb, int *bad);
+void net_batch_forward(const NetBatch *nb, hipStream_t s);
+void net_batch_free(NetBatch *nb);
 // recording: inputs before the solve, the target after it
 int net_record_inputs(ws_ctx *c);
 int net_record_target(ws_ctx *c);

@@ -31,13 +31,14 @@ __device__ __forceinline__ void tap_offset(int tap, int flip, int &dz, int &dy, 
 // in[v + off(tap)][ci] (zero outside the grid; off negated when flip).
 // OUT 0: bf16 with ReLU, 1: bf16, 2: f32. Cin is a multiple of 8 so the
 // eight K values of a lane lie in one tap; K is a multiple of 32.
+// (block blk of the product: 64 voxels, 16 per wave)
 template <int NT, int OUT>
-__global__ __launch_bounds__(256) void k_conv3(NetDims d, int Cin, int K, int flip, const bf16_t *__restrict__ in,
-                                               const bf16_t *__restrict__ W, const float *__restrict__ bias,
-                                               void *__restrict__ out) {
+__device__ __forceinline__ void conv3_blk(NetDims d, int Cin, int K, int flip, const bf16_t *__restrict__ in,
+                                          const bf16_t *__restrict__ W, const float *__restrict__ bias,
+                                          void *__restrict__ out, long blk) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long V = (long)d.Z * d.Y * d.X;
-  const long v0 = ((long)blockIdx.x * 4 + wv) * 16;
+  const long v0 = (blk * 4 + wv) * 16;
   const long va = v0 + (lane & 15);
   const bool okA = va < V;
   const int x = (int)(va % d.X), y = (int)((va / d.X) % d.Y), z = (int)(va / ((long)d.X * d.Y));
@@ -80,19 +81,27 @@ __global__ __launch_bounds__(256) void k_conv3(NetDims d, int Cin, int K, int fl
   }
 }
 
+template <int NT, int OUT>
+__global__ __launch_bounds__(256) void k_conv3(NetDims d, int Cin, int K, int flip, const bf16_t *__restrict__ in,
+                                               const bf16_t *__restrict__ W, const float *__restrict__ bias,
+                                               void *__restrict__ out) {
+  conv3_blk<NT, OUT>(d, Cin, K, flip, in, W, bias, out, blockIdx.x);
+}
+
 // The same product for small grids, where one wave per 16 voxels leaves
 // most of the GPU idle: the S waves of a block share 16 voxels, wave w takes
 // the steps of 32 K values w, w + S, ..., and the S partial sums are added
 // in wave order through LDS (a fixed order: the result does not depend on
 // timing).
+// (block blk: voxels 16 blk to 16 blk + 15)
 template <int NT, int OUT, int S>
-__global__ __launch_bounds__(64 * S) void k_conv3s(NetDims d, int Cin, int K, int flip,
-                                                   const bf16_t *__restrict__ in, const bf16_t *__restrict__ W,
-                                                   const float *__restrict__ bias, void *__restrict__ out) {
+__device__ __forceinline__ void conv3s_blk(NetDims d, int Cin, int K, int flip, const bf16_t *__restrict__ in,
+                                           const bf16_t *__restrict__ W, const float *__restrict__ bias,
+                                           void *__restrict__ out, long blk) {
   __shared__ f4 part[S - 1][NT][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long V = (long)d.Z * d.Y * d.X;
-  const long v0 = (long)blockIdx.x * 16;
+  const long v0 = blk * 16;
   const long va = v0 + (lane & 15);
   const bool okA = va < V;
   const int x = (int)(va % d.X), y = (int)((va / d.X) % d.Y), z = (int)(va / ((long)d.X * d.Y));
@@ -146,6 +155,13 @@ __global__ __launch_bounds__(64 * S) void k_conv3s(NetDims d, int Cin, int K, in
   }
 }
 
+template <int NT, int OUT, int S>
+__global__ __launch_bounds__(64 * S) void k_conv3s(NetDims d, int Cin, int K, int flip,
+                                                   const bf16_t *__restrict__ in, const bf16_t *__restrict__ W,
+                                                   const float *__restrict__ bias, void *__restrict__ out) {
+  conv3s_blk<NT, OUT, S>(d, Cin, K, flip, in, W, bias, out, blockIdx.x);
+}
+
 // The same convolution with the input tile staged in LDS. A block computes
 // a 16 x 8 x (MT/2) voxel tile: each of its four waves owns MT rows of 16
 // voxels along x and all output channels. The input region (tile plus a
@@ -153,15 +169,16 @@ __global__ __launch_bounds__(64 * S) void k_conv3s(NetDims d, int Cin, int K, in
 // (CH = 16, or 8 for an 8-channel input), so every input element comes from
 // L2 once per block and the 27 taps read it from LDS. One MFMA step covers
 // 32 K values: two taps of 16 channels (four taps of 8 when CH = 8).
+// (tile blk of the grid, x fastest)
 template <int NT, int OUT, int MT>
-__global__ __launch_bounds__(256) void k_conv3l(NetDims d, int Cin, int K, int flip, const bf16_t *__restrict__ in,
-                                                const bf16_t *__restrict__ W, const float *__restrict__ bias,
-                                                void *__restrict__ out) {
+__device__ __forceinline__ void conv3l_blk(NetDims d, int Cin, int K, int flip, const bf16_t *__restrict__ in,
+                                           const bf16_t *__restrict__ W, const float *__restrict__ bias,
+                                           void *__restrict__ out, long blk) {
   constexpr int TX = 16, TY = 8, TZ = MT / 2, RX = TX + 2, RY = TY + 2, RZ = TZ + 2, RN = RX * RY * RZ;
   __shared__ uint4 reg[RN * 2];  // RN voxels x 16 channels of bf16 (32 bytes each)
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int tilesX = (d.X + TX - 1) / TX, tilesY = (d.Y + TY - 1) / TY;
-  const int b = blockIdx.x;
+  const int b = (int)blk;
   const int x0 = (b % tilesX) * TX, y0 = ((b / tilesX) % tilesY) * TY, z0 = (b / (tilesX * tilesY)) * TZ;
   const int CH = Cin < 16 ? 8 : 16;  // channels per staged chunk
   const int tpl = CH == 16 ? 2 : 4;  // taps per MFMA step
@@ -234,6 +251,13 @@ __global__ __launch_bounds__(256) void k_conv3l(NetDims d, int Cin, int K, int f
       }
     }
   }
+}
+
+template <int NT, int OUT, int MT>
+__global__ __launch_bounds__(256) void k_conv3l(NetDims d, int Cin, int K, int flip, const bf16_t *__restrict__ in,
+                                                const bf16_t *__restrict__ W, const float *__restrict__ bias,
+                                                void *__restrict__ out) {
+  conv3l_blk<NT, OUT, MT>(d, Cin, K, flip, in, W, bias, out, blockIdx.x);
 }
 
 // Weight gradient partials: part[chunk][co][k] = sum over the chunk's
@@ -358,9 +382,9 @@ __global__ void k_pack_weights(int Cout, int Cin, int Cin_p, int K, int Kd, cons
 }
 
 // 2x2x2 average pooling (average in f32, stored bf16)
-__global__ void k_pool(NetDims dc, int C, const bf16_t *__restrict__ fine, bf16_t *__restrict__ coarse) {
-  // one thread per coarse voxel and group of 8 channels (C is a multiple of 8)
-  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// (element e: a coarse voxel and group of 8 channels; C is a multiple of 8)
+__device__ __forceinline__ void pool_at(NetDims dc, int C, const bf16_t *__restrict__ fine, bf16_t *__restrict__ coarse,
+                                        long e) {
   long Vc = (long)dc.Z * dc.Y * dc.X;
   const int G = C >> 3;
   if (e >= Vc * G) return;
@@ -384,13 +408,15 @@ __global__ void k_pool(NetDims dc, int C, const bf16_t *__restrict__ fine, bf16_
   for (int i = 0; i < 4; i++) o[i] = (unsigned)f2bf(s[2 * i] * 0.125f) | ((unsigned)f2bf(s[2 * i + 1] * 0.125f) << 16);
   *(uint4 *)(coarse + v * C + 8 * grp) = uint4{o[0], o[1], o[2], o[3]};
 }
+__global__ void k_pool(NetDims dc, int C, const bf16_t *__restrict__ fine, bf16_t *__restrict__ coarse) {
+  pool_at(dc, C, fine, coarse, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // nearest upsampling x2 of the coarse tensor concatenated with the skip:
 // out[v][0..Cu) = coarse[parent(v)], out[v][Cu..Cu+Cs) = skip[v]; one
-// thread per fine voxel and group of 8 channels (Cu, Cs multiples of 8)
-__global__ void k_up_concat(NetDims df, int Cu, int Cs, const bf16_t *__restrict__ coarse,
-                            const bf16_t *__restrict__ skip, bf16_t *__restrict__ out) {
-  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// element per fine voxel and group of 8 channels (Cu, Cs multiples of 8)
+__device__ __forceinline__ void up_concat_at(NetDims df, int Cu, int Cs, const bf16_t *__restrict__ coarse,
+                                             const bf16_t *__restrict__ skip, bf16_t *__restrict__ out, long e) {
   long Vf = (long)df.Z * df.Y * df.X;
   const int C = Cu + Cs, G = C >> 3;
   if (e >= Vf * G) return;
@@ -405,6 +431,10 @@ __global__ void k_up_concat(NetDims df, int Cu, int Cs, const bf16_t *__restrict
     w = *(const uint4 *)(coarse + p * Cu + c);
   }
   *(uint4 *)(out + v * C + c) = w;
+}
+__global__ void k_up_concat(NetDims df, int Cu, int Cs, const bf16_t *__restrict__ coarse,
+                            const bf16_t *__restrict__ skip, bf16_t *__restrict__ out) {
+  up_concat_at(df, Cu, Cs, coarse, skip, out, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // gz = bf16(g * [y > 0]), g a channel slice [off, off + C) of a row of Cs floats
@@ -454,12 +484,15 @@ __global__ void k_poolback_add_relu(NetDims df, int C, const float *__restrict__
 }
 
 // 1x1x1 head: out[v] = sum_c w[c] a[v][c] + b
-__global__ void k_head_fwd(long V, const bf16_t *__restrict__ a, const float *__restrict__ w, float *__restrict__ out) {
-  long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void head_at(long V, const bf16_t *__restrict__ a, const float *__restrict__ w,
+                                        float *__restrict__ out, long v) {
   if (v >= V) return;
   float s = 0.f;
   for (int c = 0; c < 16; c++) s += bf2f(f2bf(w[c])) * bf2f(a[v * 16 + c]);
   out[v] = s + w[16];
+}
+__global__ void k_head_fwd(long V, const bf16_t *__restrict__ a, const float *__restrict__ w, float *__restrict__ out) {
+  head_at(V, a, w, out, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // head backward: gz9[v][c] = bf16(gout[v] w[c] [a9 > 0]); per-block partial
@@ -709,14 +742,17 @@ __global__ void k_ws_inputs_coarse(NetDims cd, const uint8_t *__restrict__ ct, c
 }
 // q += s y(parent) on fluid cells: the coarse prediction, constant over
 // each 2x2x2 block
-__global__ void k_ws_apply_coarse(NetDims cd, const uint8_t *__restrict__ ct, const float *__restrict__ y,
-                                  const float *__restrict__ scales, float *__restrict__ q) {
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void apply_coarse_at(NetDims cd, const uint8_t *__restrict__ ct, const float *__restrict__ y,
+                                                const float *__restrict__ scales, float *__restrict__ q, long c) {
   const long C = 8L * cd.Z * cd.Y * cd.X;
   if (c >= C || ct[c] != FLUID) return;
   const int X = 2 * cd.X, Y = 2 * cd.Y;
   const int x = (int)(c % X), yy = (int)((c / X) % Y), z = (int)(c / ((long)X * Y));
   q[c] = q[c] + scales[0] * y[((long)(z >> 1) * cd.Y + (yy >> 1)) * cd.X + (x >> 1)];
+}
+__global__ void k_ws_apply_coarse(NetDims cd, const uint8_t *__restrict__ ct, const float *__restrict__ y,
+                                  const float *__restrict__ scales, float *__restrict__ q) {
+  apply_coarse_at(cd, ct, y, scales, q, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 // training target on the coarsened grid: the mean of (q* - q_prev) / s over
 // each block's fluid cells (0 for a block without one), the value that a
@@ -738,11 +774,14 @@ __global__ void k_ws_target_coarse(NetDims cd, const uint8_t *__restrict__ ct, c
   t[v] = n ? sum / (float)n / scales[0] : 0.f;
 }
 // q += s y on fluid cells (the prediction of the correction)
-__global__ void k_ws_apply(long C, const uint8_t *__restrict__ ct, const float *__restrict__ y,
-                           const float *__restrict__ scales, float *__restrict__ q) {
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void apply_at(long C, const uint8_t *__restrict__ ct, const float *__restrict__ y,
+                                         const float *__restrict__ scales, float *__restrict__ q, long c) {
   if (c >= C) return;
   if (ct[c] == FLUID) q[c] = q[c] + scales[0] * y[c];
+}
+__global__ void k_ws_apply(long C, const uint8_t *__restrict__ ct, const float *__restrict__ y,
+                           const float *__restrict__ scales, float *__restrict__ q) {
+  apply_at(C, ct, y, scales, q, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 // training target of a recorded sample: (q* - q_prev) / s on fluid cells
 __global__ void k_ws_target(long C, const uint8_t *__restrict__ ct, const float *__restrict__ q,
@@ -751,6 +790,67 @@ __global__ void k_ws_target(long C, const uint8_t *__restrict__ ct, const float 
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C) return;
   t[c] = ct[c] == FLUID ? (q[c] - qprev[c]) / scales[0] : 0.f;
+}
+
+// ---- the forward pass of many planets at once
+// Each step of the forward pass (a convolution, a pooling, an upsampling,
+// the head, the correction) runs as one launch over every planet: planet j
+// owns the launch's blocks [starts[j], starts[j + 1]) and its block k there
+// does what block k of its own launch would do, so the results equal the
+// planets' separate passes bit for bit.
+struct NetJob {
+  NetDims d;  // the grid of the step (for the correction: the network's grid)
+  const void *in, *in2;  // input, and the skip tensor of an upsampling
+  const bf16_t *W;
+  const float *b;  // bias; the head's weights
+  void *out;       // output; the pressure of a correction
+  const uint8_t *ct;
+  const float *scales;
+};
+
+// the planet that owns block b: the last j with starts[j] <= b (starts
+// ascending), as a wave-uniform value
+__device__ __forceinline__ int net_job_of(const long *__restrict__ starts, int nj, long b) {
+  const int lane = threadIdx.x & 63;
+  int k = 0;
+  for (int base = 0; base < nj; base += 64) {
+    const int j = base + lane;
+    k += __popcll(__ballot(j < nj && starts[j] <= b));
+  }
+  return __builtin_amdgcn_readfirstlane(k - 1);
+}
+
+// a convolution layer over many planets (forward, bf16 + ReLU); V selects the
+// form each planet's own pass uses: 0 untiled, 1 small-grid, 2 LDS-tiled
+template <int NT, int V>
+__global__ __launch_bounds__(256) void k_conv_batch(const NetJob *__restrict__ J, const long *__restrict__ starts,
+                                                    int nj, int Cin, int K) {
+  const int j = net_job_of(starts, nj, blockIdx.x);
+  const NetJob q = J[j];
+  const long blk = (long)blockIdx.x - starts[j];
+  const bf16_t *in = (const bf16_t *)q.in;
+  if constexpr (V == 0) conv3_blk<NT, 0>(q.d, Cin, K, 0, in, q.W, q.b, q.out, blk);
+  else if constexpr (V == 1) conv3s_blk<NT, 0, 4>(q.d, Cin, K, 0, in, q.W, q.b, q.out, blk);
+  else conv3l_blk<NT, 0, NT >= 3 ? 4 : 8>(q.d, Cin, K, 0, in, q.W, q.b, q.out, blk);
+}
+
+// the element-wise steps over many planets, 256 elements per block:
+// 0 pooling (a channels), 1 upsampling with concatenation (a coarse, a2 skip
+// channels), 2 the head, 3 the correction from a coarsened grid, 4 the
+// correction
+__global__ __launch_bounds__(256) void k_net_elem_batch(const NetJob *__restrict__ J, const long *__restrict__ starts,
+                                                        int nj, int kind, int a, int a2) {
+  const int j = net_job_of(starts, nj, blockIdx.x);
+  const NetJob q = J[j];
+  const long e = ((long)blockIdx.x - starts[j]) * 256 + threadIdx.x;
+  const long V = (long)q.d.Z * q.d.Y * q.d.X;
+  switch (kind) {
+    case 0: pool_at(q.d, a, (const bf16_t *)q.in, (bf16_t *)q.out, e); break;
+    case 1: up_concat_at(q.d, a, a2, (const bf16_t *)q.in, (const bf16_t *)q.in2, (bf16_t *)q.out, e); break;
+    case 2: head_at(V, (const bf16_t *)q.in, q.b, (float *)q.out, e); break;
+    case 3: apply_coarse_at(q.d, q.ct, (const float *)q.in, q.scales, (float *)q.out, e); break;
+    default: apply_at(V, q.ct, (const float *)q.in, q.scales, (float *)q.out, e); break;
+  }
 }
 
 // explicit instantiations
@@ -804,25 +904,34 @@ void release(Net *n, void *p) {
 long vox(NetDims d) { return (long)d.Z * d.Y * d.X; }
 NetDims half(NetDims d) { return NetDims{d.Z / 2, d.Y / 2, d.X / 2}; }
 
+// the form of a product with NT x 16 output channels on grid d: 0 untiled,
+// 1 four waves per 16 voxels with K split (too few LDS tiles to fill the
+// GPU), 2 LDS-tiled; and its blocks
+int conv_form(const ws_ctx *c, NetDims d, int NT) {
+  if (!c->netTiled) return 0;
+  const int MT = NT >= 3 ? 4 : 8;
+  const long tiles = (long)((d.X + 15) / 16) * ((d.Y + 7) / 8) * ((d.Z + MT / 2 - 1) / (MT / 2));
+  return tiles < 256 ? 1 : 2;
+}
+long conv_blocks(int form, NetDims d, int NT) {
+  const long V = vox(d);
+  if (form == 0) return blocks_for(V, 64);
+  if (form == 1) return blocks_for(V, 16);
+  const int MT = NT >= 3 ? 4 : 8;
+  return (long)((d.X + 15) / 16) * ((d.Y + 7) / 8) * ((d.Z + MT / 2 - 1) / (MT / 2));
+}
+
 template <int NT, int OUT>
 void conv(ws_ctx *c, NetDims d, int cin, int K, int flip, const bf16_t *in, const bf16_t *W, const float *b, void *out) {
-  long V = vox(d);
-  if (!c->netTiled) {  // the untiled form of the same product
-
-    hipLaunchKernelGGL((k_conv3<NT, OUT>), dim3(blocks_for(V, 64)), dim3(256), 0, c->stream, d, cin, K, flip, in, W,
-                       b, out);
-    return;
-  }
+  const int form = conv_form(c, d, NT);
+  const dim3 grid(conv_blocks(form, d, NT));
   constexpr int MT = NT >= 3 ? 4 : 8;
-  int tx = (d.X + 15) / 16, ty = (d.Y + 7) / 8, tz = (d.Z + MT / 2 - 1) / (MT / 2);
-  if ((long)tx * ty * tz < 256) {
-    // too few LDS tiles to fill the GPU: four waves per 16 voxels, K split
-    hipLaunchKernelGGL((k_conv3s<NT, OUT, 4>), dim3(blocks_for(V, 16)), dim3(256), 0, c->stream, d, cin, K, flip, in,
-                       W, b, out);
-    return;
-  }
-  hipLaunchKernelGGL((k_conv3l<NT, OUT, MT>), dim3(tx * ty * tz), dim3(256), 0, c->stream, d, cin, K, flip, in, W, b,
-                     out);
+  if (form == 0)
+    hipLaunchKernelGGL((k_conv3<NT, OUT>), grid, dim3(256), 0, c->stream, d, cin, K, flip, in, W, b, out);
+  else if (form == 1)
+    hipLaunchKernelGGL((k_conv3s<NT, OUT, 4>), grid, dim3(256), 0, c->stream, d, cin, K, flip, in, W, b, out);
+  else
+    hipLaunchKernelGGL((k_conv3l<NT, OUT, MT>), grid, dim3(256), 0, c->stream, d, cin, K, flip, in, W, b, out);
 }
 
 // forward conv of layer l (bf16 + ReLU)
@@ -1150,6 +1259,169 @@ int net_warm_start(ws_ctx *c) {
                        n->scales, c->q);
   WS_LAUNCHED();
   return WS_OK;
+}
+
+// the warm start of a solve whose forward pass and correction run for many
+// planets at once (net_batch_forward): the network's inputs only
+int net_warm_inputs(ws_ctx *c) {
+  int r;
+  if ((r = solver_side(c))) return r;
+  if (!c->net->packed && (r = net_pack(c))) return r;
+  return solver_inputs(c);
+}
+
+struct NetBatch {
+  struct Step {
+    int kind;  // -1 a convolution (form, nt: its product), else k_net_elem_batch's kind
+    int form, nt, a, a2;
+    int nj, off;  // planets, and their first entry in jobs / starts
+    long blocks;
+  };
+  std::vector<Step> steps;
+  NetJob *jobs = nullptr;
+  long *starts = nullptr;
+};
+
+void net_batch_free(NetBatch *nb) {
+  if (!nb) return;
+  if (nb->jobs) (void)hipFree(nb->jobs);
+  if (nb->starts) (void)hipFree(nb->starts);
+  delete nb;
+}
+
+// the steps of the forward pass (the order of net_forward) and the
+// correction, each over the planets with the warm start on
+int net_batch_plan(ws_ctx *const *cs, int n, NetBatch **out, int *bad) {
+  *out = nullptr;
+  std::vector<ws_ctx *> P;
+  for (int i = 0; i < n; i++) {
+    if (!cs[i]->warmStart) continue;
+    *bad = i;
+    int r;
+    if ((r = solver_side(cs[i]))) return r;
+    if (!cs[i]->net->packed && (r = net_pack(cs[i]))) return r;
+    P.push_back(cs[i]);
+  }
+  *bad = -1;
+  if (P.empty()) return WS_OK;
+  NetBatch *nb = new NetBatch();
+  std::vector<NetJob> hj;
+  std::vector<long> hs;
+  auto add = [&](NetBatch::Step st, const std::vector<NetJob> &jobs, const std::vector<long> &blocks) {
+    if (jobs.empty()) return;
+    st.nj = (int)jobs.size();
+    st.off = (int)hj.size();
+    st.blocks = 0;
+    for (size_t j = 0; j < jobs.size(); j++) {
+      hj.push_back(jobs[j]);
+      hs.push_back(st.blocks);
+      st.blocks += blocks[j];
+    }
+    nb->steps.push_back(st);
+  };
+  // level 0, 1, 2: the network's grid, halved, quartered
+  auto dims = [&](ws_ctx *c, int lev) {
+    NetDims d = net_grid(c, c->netCoarse);
+    for (int k = 0; k < lev; k++) d = half(d);
+    return d;
+  };
+  auto layer = [&](int l, int lev, bf16_t *Net::*in, bf16_t *Net::*o) {
+    const int nt = P[0]->net->L[l].cout / 16;
+    for (int form = 0; form < 3; form++) {
+      std::vector<NetJob> jobs;
+      std::vector<long> blocks;
+      for (ws_ctx *c : P) {
+        const NetDims d = dims(c, lev);
+        if (conv_form(c, d, nt) != form) continue;
+        Net *m = c->net;
+        const NetLayer &L = m->L[l];
+        jobs.push_back(NetJob{d, m->*in, nullptr, L.wf, m->w + L.boff, m->*o, nullptr, nullptr});
+        blocks.push_back(conv_blocks(form, d, nt));
+      }
+      add(NetBatch::Step{-1, form, nt, P[0]->net->L[l].cinp, P[0]->net->L[l].K, 0, 0, 0}, jobs, blocks);
+    }
+  };
+  auto elem = [&](int kind, int lev, int a, int a2, auto job, auto count) {
+    std::vector<NetJob> jobs;
+    std::vector<long> blocks;
+    for (ws_ctx *c : P) {
+      jobs.push_back(job(c, dims(c, lev)));
+      blocks.push_back(blocks_for(count(c, dims(c, lev)), 256));
+    }
+    add(NetBatch::Step{kind, 0, 0, a, a2, 0, 0, 0}, jobs, blocks);
+  };
+  auto pool = [&](int lev, int C, bf16_t *Net::*in, bf16_t *Net::*o) {
+    elem(0, lev, C, 0, [&](ws_ctx *c, NetDims d) { return NetJob{d, c->net->*in, nullptr, nullptr, nullptr, c->net->*o, nullptr, nullptr}; },
+         [&](ws_ctx *, NetDims d) { return vox(d) * C / 8; });
+  };
+  auto up = [&](int lev, int Cu, int Cs, bf16_t *Net::*in, bf16_t *Net::*skip, bf16_t *Net::*o) {
+    elem(1, lev, Cu, Cs, [&](ws_ctx *c, NetDims d) { return NetJob{d, c->net->*in, c->net->*skip, nullptr, nullptr, c->net->*o, nullptr, nullptr}; },
+         [&](ws_ctx *, NetDims d) { return vox(d) * (Cu + Cs) / 8; });
+  };
+  layer(0, 0, &Net::x0, &Net::a0);
+  layer(1, 0, &Net::a0, &Net::s1);
+  pool(1, 16, &Net::s1, &Net::p1);
+  layer(2, 1, &Net::p1, &Net::a2);
+  layer(3, 1, &Net::a2, &Net::s2);
+  pool(2, 32, &Net::s2, &Net::p2);
+  layer(4, 2, &Net::p2, &Net::a4);
+  layer(5, 2, &Net::a4, &Net::a5);
+  up(1, 64, 32, &Net::a5, &Net::s2, &Net::u1);
+  layer(6, 1, &Net::u1, &Net::a6);
+  layer(7, 1, &Net::a6, &Net::a7);
+  up(0, 32, 16, &Net::a7, &Net::s1, &Net::u2);
+  layer(8, 0, &Net::u2, &Net::a8);
+  layer(9, 0, &Net::a8, &Net::a9);
+  elem(2, 0, 0, 0,
+       [&](ws_ctx *c, NetDims d) {
+         Net *m = c->net;
+         return NetJob{d, m->a9, nullptr, nullptr, m->w + m->L[10].woff, m->out, nullptr, nullptr};
+       },
+       [&](ws_ctx *, NetDims d) { return vox(d); });
+  // the correction: from the coarsened grid or the solver's own
+  for (int coarse = 1; coarse >= 0; coarse--) {
+    std::vector<NetJob> jobs;
+    std::vector<long> blocks;
+    for (ws_ctx *c : P) {
+      if ((c->netCoarse != 0) != (coarse != 0)) continue;
+      Net *m = c->net;
+      jobs.push_back(NetJob{net_grid(c, coarse), m->out, nullptr, nullptr, nullptr, c->q, c->cellType, m->scales});
+      blocks.push_back(blocks_for(c->nC, 256));
+    }
+    add(NetBatch::Step{coarse ? 3 : 4, 0, 0, 0, 0, 0, 0, 0}, jobs, blocks);
+  }
+  hipError_t e = hipMalloc((void **)&nb->jobs, sizeof(NetJob) * hj.size());
+  if (e == hipSuccess) e = hipMalloc((void **)&nb->starts, sizeof(long) * hs.size());
+  if (e == hipSuccess) e = hipMemcpy(nb->jobs, hj.data(), sizeof(NetJob) * hj.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(nb->starts, hs.data(), sizeof(long) * hs.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    P[0]->err = std::string("warm-start network, batched pass: ") + hipGetErrorString(e);
+    *bad = 0;
+    for (int i = 0; i < n; i++)
+      if (cs[i] == P[0]) *bad = i;
+    net_batch_free(nb);
+    return WS_EHIP;
+  }
+  *out = nb;
+  return WS_OK;
+}
+
+void net_batch_forward(const NetBatch *nb, hipStream_t s) {
+  if (!nb) return;
+  for (const NetBatch::Step &st : nb->steps) {
+    const NetJob *J = nb->jobs + st.off;
+    const long *S = nb->starts + st.off;
+    const dim3 grid(st.blocks), blk(256);
+    if (st.kind >= 0) {
+      hipLaunchKernelGGL(k_net_elem_batch, grid, blk, 0, s, J, S, st.nj, st.kind, st.a, st.a2);
+      continue;
+    }
+#define NB_CONV(NT, F) \
+  if (st.nt == NT && st.form == F) hipLaunchKernelGGL((k_conv_batch<NT, F>), grid, blk, 0, s, J, S, st.nj, st.a, st.a2);
+    NB_CONV(1, 0) NB_CONV(1, 1) NB_CONV(1, 2) NB_CONV(2, 0) NB_CONV(2, 1) NB_CONV(2, 2) NB_CONV(4, 0) NB_CONV(4, 1)
+    NB_CONV(4, 2)
+#undef NB_CONV
+  }
 }
 
 int net_record_inputs(ws_ctx *c) {
