@@ -566,12 +566,14 @@ def make_planets(cfg, weights, only=None):
 
 def batched_main(a, cfg):
     """Batched planets: independent planets of mixed sizes advanced together
-    by the library's batch (ws_batch_*): each planet's stages on its own
-    stream, all pressure solves (learned warm start, then SOR to 1e-4 with
-    the residual checks on the device) in one launch (a few workgroups per
-    planet, by size), and the whole batched substep captured into a HIP
-    graph and replayed. The autotuner picks the convolution tiling per grid size (on
-    one planet of each size) and the solve's workgroup size for the batch.
+    by the library's batch (ws_batch_*) on one stream: each stage is one
+    launch over all planets (the planets' own stages merged from their
+    recorded launches, the warm start's network steps, all pressure solves
+    to 1e-4 with the residual checks on the device, the extrapolation by
+    tile diagonal), and the whole batched substep is captured into a HIP
+    graph and replayed. The autotuner picks the P2G bin per grid size (on
+    one planet of each size), then the convolution form and the solve's
+    workgroup size for the batch as a whole.
     value = particle updates per second summed over the planets."""
     import watersim_hip
     if not os.path.exists(a.weights):
@@ -593,6 +595,20 @@ def batched_main(a, cfg):
         log("tune %d^3: %s" % (n, rep.replace("\n", "; ")))
     for i, s in enumerate(planets):
         s.set(netTiled=tiling[sizes[i]], p2gBin=bins[sizes[i]])
+    # the convolutions' form for the batch as a whole: each size's own choice
+    # (a single planet's grid may be too small to fill the GPU with LDS
+    # tiles), or LDS tiles on every grid (the planets fill it together)
+    net_trial = {}
+    for form in ("per size", 2):
+        for i, s in enumerate(planets):
+            s.set(netTiled=tiling[sizes[i]] if form == "per size" else form)
+        t1 = time.perf_counter()
+        batch.run(10)
+        net_trial[form] = (time.perf_counter() - t1) / 10
+        log("batch with netTiled %s: %.3f ms per batched substep" % (form, 1e3 * net_trial[form]))
+    net_form = min(net_trial, key=net_trial.get)
+    for i, s in enumerate(planets):
+        s.set(netTiled=tiling[sizes[i]] if net_form == "per size" else net_form)
     # the solve's workgroup size, and each planet's workgroups on one XCD
     # (L2-local barriers) or dealt over all XCDs (agent-scope barriers)
     trial = {}
@@ -633,7 +649,8 @@ def batched_main(a, cfg):
     for n in sorted(set(sizes)):
         idx = [i for i in range(P) if sizes[i] == n]
         per_size["%d^3" % n] = {"planets": len(idx), "particles_mean": float(np.mean([nPs[i] for i in idx])),
-                                "netTiled": tiling[n], "p2gBin": bins[n]}
+                                "netTiled": tiling[n] if net_form == "per size" else net_form,
+                                "p2gBin": bins[n]}
     out = {
         "metric": "aggregate particle-updates/s, %d batched planets (32^3..96^3, learned warm start + SOR to 1e-4)" % P,
         "value": updates / el,
@@ -644,14 +661,18 @@ def batched_main(a, cfg):
                                                                                             cfg["seed"] + P),
         "config": {"workload": "cfg5: %d independent planets, grid sizes drawn from %s, about 8 particles per "
                                "liquid cell, own terrain seed and gravity each, learned warm start + red-black SOR "
-                               "to 1e-4, vorticity %g; one batch: every planet's stages on its own stream, all "
-                               "pressure solves in one launch (workgroups per planet by size), the batched substep "
-                               "replayed from a HIP graph; autotuned convolution tiling and P2G bin per size and "
-                               "solve workgroup size" % (P, list(cfg["sizes"]), cfg["vorticity"]),
+                               "to 1e-4, vorticity %g; one batch on one stream: every stage one launch over all "
+                               "planets (the planets' own stages merged from their recorded launches, the "
+                               "network's steps, all pressure solves with workgroups per planet by size, the "
+                               "extrapolation by tile diagonal), the batched substep replayed from a HIP graph; "
+                               "autotuned P2G bin per size, convolution form and solve workgroup size for the "
+                               "batch" % (P, list(cfg["sizes"]), cfg["vorticity"]),
                    "particles_total": sum(nPs), "per_size": per_size,
                    "planet_substep_ms": 1e3 * el / a.steps,
                    "solve_threads": threads, "xcd_local": xcd_local, "xcd_local_planets": local_planets,
                    "batch_trial_ms": {"%d threads, xcdLocal %d" % k: 1e3 * v for k, v in trial.items()},
+                   "net_form_trial_ms": {"netTiled %s" % k: 1e3 * v for k, v in net_trial.items()},
+                   "stage_launches": int(batch.get("stageLaunches")),
                    "joint_solve_ms": 1e3 * solve_s, "sweeps_per_planet_substep": sweeps_per},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / (HBM_PEAK / 1e9), "traffic": None,

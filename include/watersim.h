@@ -83,7 +83,11 @@ const char *ws_last_error(const ws_ctx *c);
  * "coreR", "oceanR", "spacing", "particleVolume", "atmosphereH", "gRamp",
  * "nP", "dv", "tn", "pressureTol", "maxPressureIters", "iso"; and the
  * kernel choices of this library (no reference counterpart; each gives the
- * same results): "sorMode", "sorThreads", "netTiled", "p2gBin" (0..3: P2G
+ * same results, except that the network's forms round its prediction
+ * differently): "sorMode", "sorThreads", "netTiled" (the warm start's
+ * convolutions: 0 untiled, 1 LDS-tiled where the grid has enough tiles to
+ * fill the GPU and four waves per 16 voxels elsewhere, 2 LDS-tiled on every
+ * grid, for a batch whose planets fill the GPU together), "p2gBin" (0..3: P2G
  * bins of 8x8x8, 4x4x4, 16x8x8, 8x8x4 cells), "xcdGroups" (workgroups of
  * sorMode 6, 0 = one per compute unit of an XCD), "xcdUnits" (its cells per
  * thread and colour, 1..3, 0 = chosen from the list lengths); the counter

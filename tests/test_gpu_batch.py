@@ -162,19 +162,23 @@ def test_batch_equals_single_planets(graph, cells_per_group, xcd_local, max_grou
         s.close()
 
 
-@pytest.mark.parametrize("net_batch", [1, 0])
-def test_batched_network_pass_equals_planets_own(net_batch):
+@pytest.mark.parametrize("net_batch,stage_batch", [(1, 1), (1, 0), (0, 0)])
+def test_batched_network_pass_equals_planets_own(net_batch, stage_batch):
     """The warm start's forward pass and correction run as one launch per
     network step over every planet with the warm start on (netBatch 1) or
     as each planet's own launches (0). The planets cover the three forms of
-    the convolution (a 32^3 grid coarsened to 16^3: four waves per 16 voxels;
-    64^3 at full resolution: LDS-tiled at its finest level; 48^3 with
-    netTiled 0: untiled) and both corrections (coarsened and full), beside a
-    planet without the warm start. Either way every planet's state equals
-    its own ws_run_substeps run bit for bit, eagerly and from the graph."""
+    the convolution (64^3 at full resolution: LDS-tiled at its finest level,
+    four waves per 16 voxels at the coarser ones; a 32^3 grid coarsened to
+    16^3 with netTiled 2: LDS-tiled down to 4^3; 48^3 with netTiled 0:
+    untiled) and both corrections (coarsened and full), beside a
+    planet without the warm start. The planets' own stages run merged into
+    one launch per stage (stageBatch 1, which batches the network's pass
+    too; the planets differ in grid size, so in viscosity iterations, and in
+    the network's input stage) or on their own streams (0). Either way every planet's state equals its own
+    ws_run_substeps run bit for bit, eagerly and from the graph."""
     if not os.path.exists(WEIGHTS):
         pytest.skip("no trained warm-start weights")
-    spec = [(32, 21, True, dict(netCoarse=1)), (64, 22, True, dict(netCoarse=0)),
+    spec = [(32, 21, True, dict(netCoarse=1, netTiled=2)), (64, 22, True, dict(netCoarse=0)),
             (48, 23, True, dict(netCoarse=0, netTiled=0)), (32, 24, False, {})]
 
     def make():
@@ -190,16 +194,18 @@ def test_batched_network_pass_equals_planets_own(net_batch):
         s.set(sorMode=4, sorThreads=512)
     for graph in (False, True):
         b = watersim_hip.Batch(bat, graph=graph, threads=512)
-        b.set(netBatch=net_batch)
+        b.set(netBatch=net_batch, stageBatch=stage_batch)
         assert int(b.get("netBatch")) == net_batch
         b.run(2)
+        launches = int(b.get("stageLaunches"))
+        assert (launches > 0) == bool(stage_batch)
         for s in ref:
             s.run_substeps(2)
         for i, (r, s) in enumerate(zip(ref, bat)):
             assert_same(r, s, "graph %d planet %d" % (graph, i))
             assert r.get("sweepsTotal") == s.get("sweepsTotal")
         b.close()
-    print("  sweeps %s" % [int(s.get("sweepsTotal")) for s in bat])
+    print("  sweeps %s, %d stage launches per batched substep" % ([int(s.get("sweepsTotal")) for s in bat], launches))
     for s in ref + bat:
         s.close()
 

@@ -121,9 +121,13 @@ __global__ void k_set_dt(DevScalars *s, double dt) {
   if (threadIdx.x == 0) { s->dt = (float)dt; s->dtD = dt; }
 }
 
-__global__ void k_reset_counters(DevScalars *s) {
+__device__ __forceinline__ void k_reset_counters_body(long bx, long nbx, DevScalars *s) {
   if (threadIdx.x == 0) { s->umax_bits = 0; s->airborne = 0; }
 }
+__global__ void k_reset_counters(DevScalars *s) {
+  k_reset_counters_body(blockIdx.x, gridDim.x, s);
+}
+WS_PLANETS_OPS(k_reset_counters_ops, k_reset_counters, k_reset_counters_body)
 
 __global__ void k_zero_u32(unsigned *p, long n) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -133,36 +137,37 @@ __global__ void k_zero_u32(unsigned *p, long n) {
 // ------------------------------------------------------------------ stages
 int st_rasterize(ws_ctx *c) {
   int nC = (int)c->nC;
-  hipLaunchKernelGGL(k_init_cells, dim3(blocks_for(nC + 1, 256)), dim3(256), 0, c->stream, c->staticTypes, c->cellType,
-                     c->cellCount, nC);
+  stage_launch(c, k_init_cells_ops, k_init_cells, blocks_for(nC + 1, 256), 256, c->staticTypes, c->cellType,
+               c->cellCount, nC);
   int r = balls_raster(c);
   if (r) return r;
   if (c->nP > 0)
-    hipLaunchKernelGGL(k_mark, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->geo, world_of(c), c->nP,
-                       c->px, c->py, c->pz, c->pflag, c->cellType, c->cellCount, c->pcell, c->prank, c->ds);
+    stage_launch(c, k_mark_ops, k_mark, blocks_for(c->nP, 256), 256, c->geo, world_of(c), c->nP, c->px, c->py, c->pz,
+                 c->pflag, c->cellType, c->cellCount, c->pcell, c->prank, c->ds);
   WS_LAUNCHED();
   if (int r = scan_excl(c, c->cellCount, c->cellStart, nC + 1)) return r;
   if (c->nP > 0)
-    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->nP, c->pcell, c->prank,
-                       c->cellStart, c->sortedIdx);
+    stage_launch(c, k_scatter_ops, k_scatter, blocks_for(c->nP, 256), 256, c->nP, c->pcell, c->prank, c->cellStart,
+                 c->sortedIdx);
   WS_LAUNCHED();
   return WS_OK;
 }
 
-template <int TX, int TY, int TZ>
-static void p2g_launch(ws_ctx *c) {
+template <int B>
+static void p2g_launch(ws_ctx *c, const StageOps &ops) {
+  constexpr int TX = P2G_BINS[B][0], TY = P2G_BINS[B][1], TZ = P2G_BINS[B][2];
   int tx = (c->nx + TX - 1) / TX, ty = (c->ny + TY - 1) / TY, tz = (c->nz + TZ - 1) / TZ;
-  hipLaunchKernelGGL((k_p2g_tile<TX, TY, TZ>), dim3(tx * ty * tz), dim3(256), 0, c->stream, c->geo, tx, ty,
-                     c->cellStart, c->sortedIdx, c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->vel, c->wgt);
+  stage_launch(c, ops, k_p2g_tile<TX, TY, TZ>, (long)tx * ty * tz, 256, c->geo, tx, ty, c->cellStart, c->sortedIdx,
+               c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->vel, c->wgt);
 }
 
 int st_p2g(ws_ctx *c) {
   static_assert(P2G_NBINS == 4, "one launch per bin shape");
   switch (c->p2gBin) {
-    case 1: p2g_launch<P2G_BINS[1][0], P2G_BINS[1][1], P2G_BINS[1][2]>(c); break;
-    case 2: p2g_launch<P2G_BINS[2][0], P2G_BINS[2][1], P2G_BINS[2][2]>(c); break;
-    case 3: p2g_launch<P2G_BINS[3][0], P2G_BINS[3][1], P2G_BINS[3][2]>(c); break;
-    default: p2g_launch<P2G_BINS[0][0], P2G_BINS[0][1], P2G_BINS[0][2]>(c); break;
+    case 1: p2g_launch<1>(c, k_p2g_tile_ops1); break;
+    case 2: p2g_launch<2>(c, k_p2g_tile_ops2); break;
+    case 3: p2g_launch<3>(c, k_p2g_tile_ops3); break;
+    default: p2g_launch<0>(c, k_p2g_tile_ops0); break;
   }
   WS_LAUNCHED();
   return WS_OK;
@@ -171,8 +176,8 @@ int st_p2g(ws_ctx *c) {
 long nfaces(ws_ctx *c) { return c->uN + c->vN + c->wN; }
 
 int st_bc(ws_ctx *c, float *snapshot, int fuse_gravity) {
-  hipLaunchKernelGGL(k_bc_snapshot, dim3(blocks_for(nfaces(c), 256)), dim3(256), 0, c->stream, c->geo, c->cellType,
-                     c->vel, snapshot, c->wgt, c->valid, fuse_gravity, c->gu, c->ds, 1, c->nBalls ? c->sv : nullptr);
+  stage_launch(c, k_bc_snapshot_ops, k_bc_snapshot, blocks_for(nfaces(c), 256), 256, c->geo, c->cellType, c->vel,
+               snapshot, c->wgt, c->valid, fuse_gravity, c->gu, c->ds, 1, c->nBalls ? c->sv : nullptr);
   WS_LAUNCHED();
   return WS_OK;
 }
@@ -200,8 +205,8 @@ int st_viscosity(ws_ctx *c, double dt_bound, const float *old) {
   const float *cur = old;
   for (int it = 0; it < m; it++) {
     float *out = (it & 1) ? c->scr2 : c->vel;
-    hipLaunchKernelGGL(k_visc_iter, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->geo, c->cellType, old, cur,
-                       out, (float)(nu / (c->dx * c->dx)), c->ds, it == 0 ? 1 : 0);
+    stage_launch(c, k_visc_iter_ops, k_visc_iter, blocks_for(F, 256), 256, c->geo, c->cellType, old, cur, out,
+                 (float)(nu / (c->dx * c->dx)), c->ds, it == 0 ? 1 : 0);
     cur = out;
   }
   WS_LAUNCHED();
@@ -209,8 +214,8 @@ int st_viscosity(ws_ctx *c, double dt_bound, const float *old) {
 }
 
 int st_gravity(ws_ctx *c) {
-  hipLaunchKernelGGL(k_gravity_bc, dim3(blocks_for(nfaces(c), 256)), dim3(256), 0, c->stream, c->geo, c->cellType,
-                     c->vel, c->gu, c->ds, c->nBalls ? c->sv : nullptr);
+  stage_launch(c, k_gravity_bc_ops, k_gravity_bc, blocks_for(nfaces(c), 256), 256, c->geo, c->cellType, c->vel, c->gu,
+               c->ds, c->nBalls ? c->sv : nullptr);
   WS_LAUNCHED();
   return WS_OK;
 }
@@ -653,8 +658,8 @@ int st_pressure_solve(ws_ctx *c) {
 
 // the system, its fluid lists and the starting guess (warm start)
 int pressure_setup(ws_ctx *c) {
-  hipLaunchKernelGGL(k_assemble, dim3(blocks_for(c->nC, 256)), dim3(256), 0, c->stream, c->geo, c->cellType, c->vel,
-                     c->q, c->q2, c->rhs, c->pw, c->nbm, c->dg, (float)c->omega);
+  stage_launch(c, k_assemble_ops, k_assemble, blocks_for(c->nC, 256), 256, c->geo, c->cellType, c->vel, c->q, c->q2,
+               c->rhs, c->pw, c->nbm, c->dg, (float)c->omega);
   if (c->sorMode == 1) {
     int nT = c->sorTiles[0] * c->sorTiles[1] * c->sorTiles[2];
     hipLaunchKernelGGL(k_sor_tiles_mark, dim3(nT), dim3(256), 0, c->stream, c->geo, c->sorTiles[0], c->sorTiles[1],
@@ -677,11 +682,11 @@ int pressure_setup(ws_ctx *c) {
   int r;
   if (c->sorMode >= 2) {
     int nb = blocks_for(c->nC + 1, 256);
-    hipLaunchKernelGGL(k_sor_count, dim3(nb), dim3(256), 0, c->stream, c->geo, c->nbm, c->sorOff);
+    stage_launch(c, k_sor_count_ops, k_sor_count, nb, 256, c->geo, c->nbm, c->sorOff);
     WS_LAUNCHED();
     if (int r = scan_excl(c, c->sorOff, c->sorOff, (long)c->nC + 1)) return r;
-    hipLaunchKernelGGL(k_sor_compact, dim3(nb), dim3(256), 0, c->stream, c->geo, c->nbm, c->dg, c->rhs, c->pw,
-                       c->sorOff, c->sorHalf, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->ds);
+    stage_launch(c, k_sor_compact_ops, k_sor_compact, nb, 256, c->geo, c->nbm, c->dg, c->rhs, c->pw, c->sorOff,
+                 c->sorHalf, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->ds);
     WS_LAUNCHED();
     if (c->pressureTol <= 0 && !dev_solve(c)) {  // the to-residual path learns the counts with its first check
       WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
@@ -760,8 +765,8 @@ int reduce_grid(long F) {
 }
 
 int st_project(ws_ctx *c, int do_clamp) {
-  hipLaunchKernelGGL(k_project, dim3(reduce_grid(nfaces(c))), dim3(256), 0, c->stream, c->geo, c->cellType, c->q,
-                     c->vel, do_clamp, (float)(c->maxSpeed * 1.1), c->ds);
+  stage_launch(c, k_project_ops, k_project, reduce_grid(nfaces(c)), 256, c->geo, c->cellType, c->q, c->vel, do_clamp,
+               (float)(c->maxSpeed * 1.1), c->ds);
   WS_LAUNCHED();
   return WS_OK;
 }
@@ -770,20 +775,18 @@ int st_vorticity(ws_ctx *c) {
   if (!(c->vorticity > 0)) return WS_OK;
   int nC = (int)c->nC;
   float *vc = c->vort, *om = c->vort + 3 * c->nC, *fc = c->vort + 7 * c->nC;
-  hipLaunchKernelGGL(k_vort_center, dim3(blocks_for(nC, 256)), dim3(256), 0, c->stream, c->geo, c->vel, vc);
-  hipLaunchKernelGGL(k_vort_omega, dim3(blocks_for(nC, 256)), dim3(256), 0, c->stream, c->geo, vc, om,
-                     (float)(0.5 / c->dx));
-  hipLaunchKernelGGL(k_vort_force, dim3(blocks_for(nC, 256)), dim3(256), 0, c->stream, c->geo, c->cellType, om, fc,
-                     (float)c->vorticity, c->ds);
-  hipLaunchKernelGGL(k_vort_splat, dim3(blocks_for(nfaces(c), 256)), dim3(256), 0, c->stream, c->geo, c->cellType,
-                     fc, c->vel);
+  stage_launch(c, k_vort_center_ops, k_vort_center, blocks_for(nC, 256), 256, c->geo, c->vel, vc);
+  stage_launch(c, k_vort_omega_ops, k_vort_omega, blocks_for(nC, 256), 256, c->geo, vc, om, (float)(0.5 / c->dx));
+  stage_launch(c, k_vort_force_ops, k_vort_force, blocks_for(nC, 256), 256, c->geo, c->cellType, om, fc,
+               (float)c->vorticity, c->ds);
+  stage_launch(c, k_vort_splat_ops, k_vort_splat, blocks_for(nfaces(c), 256), 256, c->geo, c->cellType, fc, c->vel);
   WS_LAUNCHED();
   return WS_OK;
 }
 
 int st_clamp(ws_ctx *c) {
-  hipLaunchKernelGGL(k_clamp, dim3(reduce_grid(nfaces(c))), dim3(256), 0, c->stream, nfaces(c), c->vel,
-                     (float)(c->maxSpeed * 1.1), c->ds);
+  stage_launch(c, k_clamp_ops, k_clamp, reduce_grid(nfaces(c)), 256, nfaces(c), c->vel, (float)(c->maxSpeed * 1.1),
+               c->ds);
   WS_LAUNCHED();
   return WS_OK;
 }
@@ -887,9 +890,9 @@ int st_vapor(ws_ctx *c) {
 
 int st_particles(ws_ctx *c, int mode) {
   if (c->nP <= 0) return WS_OK;
-  hipLaunchKernelGGL(k_g2p_advect, dim3(blocks_for(c->nP, 256)), dim3(256), 0, c->stream, c->geo, world_of(c),
-                     make_par(c), c->nP, mode, c->vel, c->velO, c->cellType, c->px, c->py, c->pz, c->pvx, c->pvy,
-                     c->pvz, c->pflag, c->pcool, c->pAir, c->advU, c->advV, c->advW, c->ds);
+  stage_launch(c, k_g2p_advect_ops, k_g2p_advect, blocks_for(c->nP, 256), 256, c->geo, world_of(c), make_par(c), c->nP,
+               mode, c->vel, c->velO, c->cellType, c->px, c->py, c->pz, c->pvx, c->pvy, c->pvz, c->pflag, c->pcool,
+               c->pAir, c->advU, c->advV, c->advW, c->ds);
   WS_LAUNCHED();
   if (mode & 2) {
     int r = st_vapor(c);
@@ -916,9 +919,9 @@ int substep_pre(ws_ctx *c, double dt, int use_law) {
   int r;
   if ((r = need_world(c))) return r;
   double base = c->dtFrameBase > 0 ? c->dtFrameBase : 0.01;
-  hipLaunchKernelGGL(k_begin_substep, dim3(1), dim3(64), 0, c->stream, c->ds, dt, use_law, c->cfl * c->dx, base,
-                     (float)c->gravity);
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, c->stream, c->ds);
+  stage_launch(c, k_begin_substep_ops, k_begin_substep, 1, 64, c->ds, dt, use_law, c->cfl * c->dx, base,
+               (float)c->gravity);
+  stage_launch(c, k_reset_counters_ops, k_reset_counters, 1, 64, c->ds);
   WS_LAUNCHED();
   bool visc = c->viscosity > 1e-6;
   if ((r = st_rasterize(c))) return r;
@@ -960,7 +963,7 @@ int substep_post2(ws_ctx *c) {
   int r;
   if ((r = st_particles(c, 3))) return r;
   if ((r = balls_update(c))) return r;
-  hipLaunchKernelGGL(k_end_substep, dim3(1), dim3(64), 0, c->stream, c->ds);
+  stage_launch(c, k_end_substep_ops, k_end_substep, 1, 64, c->ds);
   WS_LAUNCHED();
   return WS_OK;
 }
@@ -1706,15 +1709,14 @@ int ws_stage(ws_ctx *c, int stage, double dt) {
       r = st_viscosity(c, dt, c->scr);
       break;
     case WS_STAGE_GRAVITY:
-      hipLaunchKernelGGL(k_begin_substep, dim3(1), dim3(64), 0, c->stream, c->ds, dt, 0, 0.0, 0.0,
-                         (float)c->gravity);
+      stage_launch(c, k_begin_substep_ops, k_begin_substep, 1, 64, c->ds, dt, 0, 0.0, 0.0, (float)c->gravity);
       r = st_gravity(c);
       break;
     case WS_STAGE_PRESSURE: r = st_pressure(c); break;
     case WS_STAGE_PROJECT: r = st_project(c, 0); break;
     case WS_STAGE_VORTICITY: r = st_vorticity(c); break;
     case WS_STAGE_CLAMP:
-      hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, c->stream, c->ds);
+      stage_launch(c, k_reset_counters_ops, k_reset_counters, 1, 64, c->ds);
       r = st_clamp(c);
       break;
     case WS_STAGE_EXTRAPOLATE:
@@ -1723,7 +1725,7 @@ int ws_stage(ws_ctx *c, int stage, double dt) {
       r = st_extrapolate(c);
       break;
     case WS_STAGE_G2P:
-      hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, c->stream, c->ds);
+      stage_launch(c, k_reset_counters_ops, k_reset_counters, 1, 64, c->ds);
       r = st_particles(c, 1);
       break;
     case WS_STAGE_ADVECT: r = st_particles(c, 2); break;
@@ -1808,9 +1810,8 @@ int ws_mark_fluid(ws_ctx *c, int p0, int p1) {
   int r;
   if ((r = need_world(c))) return r;
   if (p1 > p0)
-    hipLaunchKernelGGL(k_mark, dim3(blocks_for(p1 - p0, 256)), dim3(256), 0, c->stream, c->geo, world_of(c), p1 - p0,
-                       c->px + p0, c->py + p0, c->pz + p0, c->pflag + p0, c->cellType, c->cellCount, c->pcell + p0,
-                       c->prank + p0, c->ds);
+    stage_launch(c, k_mark_ops, k_mark, blocks_for(p1 - p0, 256), 256, c->geo, world_of(c), p1 - p0, c->px + p0,
+                 c->py + p0, c->pz + p0, c->pflag + p0, c->cellType, c->cellCount, c->pcell + p0, c->prank + p0, c->ds);
   WS_LAUNCHED();
   WS_HIP(hipStreamSynchronize(c->stream));
   return WS_OK;

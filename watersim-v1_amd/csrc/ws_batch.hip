@@ -1,13 +1,21 @@
 // Batched planets: many contexts advanced together by device-dt-law
-// substeps. Each planet's stages are queued on its own stream, except three
-// that run for all planets at once on the batch stream: the learned warm
-// start's forward pass and correction (one launch per step of the network
-// over every planet's grid), the pressure solve (one launch of
-// k_sor_solve_batch: every planet's whole solve, a few workgroups each, the
-// residual checks on the device) and the extrapolation wavefront (one launch
-// per tile diagonal over every planet's tiles of that diagonal). A substep costs no host round trip. With a graph, the batched
-// substep (fork to the planets' streams, the joint solve, join) is captured
-// once per run and replayed for every substep.
+// substeps, every stage as one launch over all planets on the batch stream:
+// - the planets' own stages (binning, P2G, boundary conditions, viscosity,
+//   the system's assembly, projection, confinement, G2P and advection): each
+//   planet's substep is recorded once per run (its launches, with their
+//   blocks and arguments, instead of launching them), the recordings are
+//   merged in order and each merged step replays as one k_planets launch
+//   that gives every planet its own blocks;
+// - the learned warm start's forward pass and correction (one launch per
+//   step of the network over every planet's grid);
+// - the pressure solve (one launch of k_sor_solve_batch: every planet's
+//   whole solve, a few workgroups each, the residual checks on the device);
+// - the extrapolation wavefront (one launch per tile diagonal over every
+//   planet's tiles of that diagonal).
+// With stageBatch off, each planet's own stages are queued on its own stream
+// instead (fork and join around the batched parts). A substep costs no host
+// round trip. With a graph, the batched substep is captured once per run and
+// replayed for every substep.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -43,9 +51,23 @@ struct ws_batch {
   int xcdLocal = 1;
   unsigned *place = nullptr;
   // netBatch: the warm start's forward passes run as one launch per step
-  // over all planets (else each planet runs its own on its stream)
+  // over all planets (else each planet runs its own on its stream; with
+  // stageBatch on they are batched either way)
   int netBatch = 1;
   NetBatch *net = nullptr;
+  // stageBatch: the planets' own stages as merged launches over all planets
+  // (else on the planets' streams); a stage of the merged sequence is one
+  // k_planets launch of the planets whose next recorded launch it is
+  struct Stage {
+    const StageOps *ops;
+    int threads, nj;
+    long blocks;
+    size_t args, starts;  // offsets into stageMem
+  };
+  int stageBatch = 1;
+  bool staged = false;
+  std::vector<Stage> stages[3];  // before the solve, before and after the extrapolation
+  unsigned char *stageMem = nullptr;
   std::vector<int> placeOff;
   // the planets' extrapolation schedules merged by tile diagonal: one launch
   // per diagonal covers every planet's tiles of that diagonal
@@ -201,8 +223,40 @@ int plan_solves(ws_batch *b) {
   return WS_OK;
 }
 
+void launch_stages(ws_batch *b, int part) {
+  for (const ws_batch::Stage &st : b->stages[part])
+    st.ops->planets(b->stageMem + st.args, (const long *)(b->stageMem + st.starts), st.nj, st.blocks, st.threads,
+                    b->stream);
+}
+
+void launch_extrapolation(ws_batch *b) {
+  for (size_t d = 0; d + 1 < b->exDiagOff.size(); d++) {
+    int nt = b->exDiagOff[d + 1] - b->exDiagOff[d];
+    if (nt > 0)
+      hipLaunchKernelGGL(k_extrap_diag_batch, dim3(nt), dim3(EX_THREADS), 0, b->stream, b->exPlanets, b->exTiles,
+                         b->exPlanetOf, b->exDiagOff[d]);
+  }
+}
+
+// one batched substep with every stage on the batch stream
+int enqueue_staged(ws_batch *b) {
+  launch_stages(b, 0);
+  net_batch_forward(b->net, b->stream);
+  BH(hipGetLastError());
+  if (b->timing) BH(hipEventRecord(b->t0, b->stream));
+  launch_solves(b);
+  BH(hipGetLastError());
+  if (b->timing) BH(hipEventRecord(b->t1, b->stream));
+  launch_stages(b, 1);
+  launch_extrapolation(b);
+  launch_stages(b, 2);
+  BH(hipGetLastError());
+  return WS_OK;
+}
+
 // one batched substep
 int enqueue_substep(ws_batch *b) {
+  if (b->staged) return enqueue_staged(b);
   int n = (int)b->ctx.size(), r;
   BH(hipEventRecord(b->fork, b->stream));
   for (int i = 0; i < n; i++) {
@@ -226,12 +280,7 @@ int enqueue_substep(ws_batch *b) {
     BH(hipEventRecord(b->mid[i], c->stream));
     BH(hipStreamWaitEvent(b->stream, b->mid[i], 0));
   }
-  for (size_t d = 0; d + 1 < b->exDiagOff.size(); d++) {
-    int nt = b->exDiagOff[d + 1] - b->exDiagOff[d];
-    if (nt > 0)
-      hipLaunchKernelGGL(k_extrap_diag_batch, dim3(nt), dim3(EX_THREADS), 0, b->stream, b->exPlanets, b->exTiles,
-                         b->exPlanetOf, b->exDiagOff[d]);
-  }
+  launch_extrapolation(b);
   BH(hipGetLastError());
   BH(hipEventRecord(b->extrapolated, b->stream));
   for (int i = 0; i < n; i++) {
@@ -307,6 +356,108 @@ int finish(ws_batch *b) {
       BH(hipMemset(b->arrive, 0, sizeof(unsigned) * 2 * n));
       return fail(b, "planet " + std::to_string(i) + ": the joint pressure solve's barrier timed out", WS_ESTATE);
     }
+  return WS_OK;
+}
+
+// The planets' substeps recorded (pre: through the solve's set-up; post1:
+// projection to clamp; post2: G2P to the substep's end) and merged into one
+// sequence of stages per part. Each planet keeps its own order; a merged
+// stage takes every planet whose next launch is of that kernel and block
+// size, chosen among the kernels at the planets' heads: one that no other
+// planet still has further on (else it would split a later stage), the one
+// most planets are at. Buffers a recording creates are cleared on the
+// planets' streams, which are drained before the batch stream runs.
+int plan_stages(ws_batch *b) {
+  for (auto &v : b->stages) v.clear();
+  if (b->stageMem) (void)hipFree(b->stageMem);
+  b->stageMem = nullptr;
+  b->staged = false;
+  if (!b->stageBatch) return WS_OK;
+  const int n = (int)b->ctx.size();
+  // planets with vapor parcels or balls take their own streams: those
+  // passes are not recorded
+  for (ws_ctx *c : b->ctx)
+    if (c->hasVapor || c->nBalls || c->hadSolidVelocity) return WS_OK;
+  std::vector<Recorder> rec((size_t)3 * n);
+  for (int i = 0; i < n; i++) {
+    ws_ctx *c = b->ctx[i];
+    c->rec = &rec[3 * i];
+    int r = batch_substep_pre(c);
+    if (!r) {
+      c->rec = &rec[3 * i + 1];
+      r = batch_substep_post1(c);
+    }
+    if (!r) {
+      c->rec = &rec[3 * i + 2];
+      r = batch_substep_post2(c);
+    }
+    c->rec = nullptr;
+    if (r) return planet_fail(b, i, r);
+  }
+  std::vector<unsigned char> mem;
+  auto align = [&](size_t a) { mem.resize((mem.size() + a - 1) / a * a); };
+  for (int part = 0; part < 3; part++) {
+    std::vector<size_t> at(n, 0);
+    auto seq = [&](int i) -> const std::vector<Recorded> & { return rec[3 * i + part].seq; };
+    auto same = [](const Recorded &x, const Recorded &y) { return x.ops == y.ops && x.threads == y.threads; };
+    for (;;) {
+      int pick = -1, pickCount = 0;
+      bool pickFree = false;
+      for (int i = 0; i < n; i++) {
+        if (at[i] >= seq(i).size()) continue;
+        const Recorded &h = seq(i)[at[i]];
+        int count = 0;
+        bool free = true;
+        for (int k = 0; k < n; k++) {
+          if (at[k] >= seq(k).size()) continue;
+          if (same(seq(k)[at[k]], h)) {
+            count++;
+            continue;
+          }
+          for (size_t q = at[k] + 1; q < seq(k).size() && free; q++)
+            if (same(seq(k)[q], h)) free = false;
+        }
+        if (pick < 0 || (free && !pickFree) || (free == pickFree && count > pickCount)) {
+          pick = i;
+          pickCount = count;
+          pickFree = free;
+        }
+      }
+      if (pick < 0) break;
+      const Recorded head = seq(pick)[at[pick]];
+      std::vector<int> who;
+      for (int i = 0; i < n; i++)
+        if (at[i] < seq(i).size() && same(seq(i)[at[i]], head)) who.push_back(i);
+      ws_batch::Stage st{head.ops, head.threads, (int)who.size(), 0, 0, 0};
+      align(256);
+      st.args = mem.size();
+      for (int i : who) {
+        const Recorded &x = seq(i)[at[i]];
+        if (x.args.size() != head.ops->argSize)
+          return fail(b, std::string("stage ") + head.ops->name + ": argument size mismatch", WS_EINVAL);
+        mem.insert(mem.end(), x.args.begin(), x.args.end());
+      }
+      align(8);
+      st.starts = mem.size();
+      long total = 0;
+      auto put = [&](long v) {
+        const unsigned char *p = (const unsigned char *)&v;
+        mem.insert(mem.end(), p, p + sizeof v);
+      };
+      for (int i : who) {
+        put(total);
+        total += seq(i)[at[i]].blocks;
+        at[i]++;
+      }
+      put(total);
+      st.blocks = total;
+      if (total > 0) b->stages[part].push_back(st);
+    }
+  }
+  BH(hipMalloc((void **)&b->stageMem, mem.size() ? mem.size() : 1));
+  BH(hipMemcpy(b->stageMem, mem.data(), mem.size(), hipMemcpyHostToDevice));
+  for (ws_ctx *c : b->ctx) BH(hipStreamSynchronize(c->stream));
+  b->staged = true;
   return WS_OK;
 }
 
@@ -416,13 +567,20 @@ int ws_batch_run(ws_batch *b, int substeps) {
   // the warm starts' forward passes: batched, or each planet's own
   net_batch_free(b->net);
   b->net = nullptr;
-  if (b->netBatch) {
+  // (merged stages need the batched pass: a planet's own pass is not recorded)
+  if (b->netBatch || b->stageBatch) {
     int bad = -1;
     if ((r = net_batch_plan(b->ctx.data(), (int)b->ctx.size(), &b->net, &bad)))
       return bad >= 0 ? planet_fail(b, bad, r) : fail(b, "warm-start network: batched pass", r);
   }
   for (ws_ctx *c : b->ctx) c->netDeferred = b->net && c->warmStart;
-  r = run_substeps(b, substeps);
+  // every planet runs the one-workgroup solve; the recordings are of that
+  for (ws_ctx *c : b->ctx) {
+    c->sorMode = 4;
+    c->sorThreads = b->threads;
+  }
+  r = plan_stages(b);
+  if (!r) r = run_substeps(b, substeps);
   for (ws_ctx *c : b->ctx) c->netDeferred = 0;
   return r;
 }
@@ -451,6 +609,10 @@ int ws_batch_set(ws_batch *b, const char *name, double v) {
   }
   if (!strcmp(name, "netBatch")) {
     b->netBatch = v != 0;
+    return WS_OK;
+  }
+  if (!strcmp(name, "stageBatch")) {
+    b->stageBatch = v != 0;
     return WS_OK;
   }
   if (!strcmp(name, "cellsPerGroup")) {
@@ -484,6 +646,11 @@ int ws_batch_get(ws_batch *b, const char *name, double *v) {
   else if (!strcmp(name, "groups")) *v = b->nBlocks;
   else if (!strcmp(name, "xcdLocal")) *v = b->xcdLocal;
   else if (!strcmp(name, "netBatch")) *v = b->netBatch;
+  else if (!strcmp(name, "stageBatch")) *v = b->stageBatch;
+  else if (!strcmp(name, "stageLaunches")) {
+    // launches of the planets' own stages per batched substep
+    *v = b->staged ? (double)(b->stages[0].size() + b->stages[1].size() + b->stages[2].size()) : 0.0;
+  }
   else if (!strcmp(name, "xcdLocalPlanets")) {
     // planets whose last joint solve ran with its workgroups on one XCD
     int k = 0;
@@ -524,6 +691,7 @@ void ws_batch_destroy(ws_batch *b) {
   if (b->arrive) (void)hipFree(b->arrive);
   if (b->place) (void)hipFree(b->place);
   net_batch_free(b->net);
+  if (b->stageMem) (void)hipFree(b->stageMem);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }

@@ -9,8 +9,8 @@ namespace ws {
 // ---------------------------------------------------------------- scalars
 // dt and the gravity ramp of this substep (solver.js:2656-2662); mode 1 takes
 // dt from the CFL law of _stepSync (solver.js:2689-2690) on the device.
-__global__ void k_begin_substep(DevScalars *s, double dt_host, int use_law, double cfl_dx, double dt_base,
-                                float gravity) {
+__device__ __forceinline__ void k_begin_substep_body(long bx, long nbx, DevScalars *s, double dt_host, int use_law, double cfl_dx, double dt_base,
+                                                     float gravity) {
   if (threadIdx.x != 0) return;
   // the length is a double, as in the reference; the float copy feeds the
   // float kernels
@@ -30,20 +30,34 @@ __global__ void k_begin_substep(DevScalars *s, double dt_host, int use_law, doub
   s->dtD = dt;
   s->simTime += dt;
 }
+__global__ void k_begin_substep(DevScalars *s, double dt_host, int use_law, double cfl_dx, double dt_base,
+                                float gravity) {
+  k_begin_substep_body(blockIdx.x, gridDim.x, s, dt_host, use_law, cfl_dx, dt_base, gravity);
+}
+WS_PLANETS_OPS(k_begin_substep_ops, k_begin_substep, k_begin_substep_body)
 
-__global__ void k_end_substep(DevScalars *s) {
+__device__ __forceinline__ void k_end_substep_body(long bx, long nbx, DevScalars *s) {
   if (threadIdx.x == 0) s->dtLast = s->dt;
 }
+__global__ void k_end_substep(DevScalars *s) {
+  k_end_substep_body(blockIdx.x, gridDim.x, s);
+}
+WS_PLANETS_OPS(k_end_substep_ops, k_end_substep, k_end_substep_body)
 
 // ------------------------------------------------------------- rasterize
 // _rasterizeStatic (cached copy, solver.js:868-870) and zeroed bin counts
-__global__ void k_init_cells(const uint8_t *__restrict__ st, uint8_t *__restrict__ ct, unsigned *__restrict__ cnt,
-                             int nC) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_init_cells_body(long bx, long nbx, const uint8_t *__restrict__ st, uint8_t *__restrict__ ct, unsigned *__restrict__ cnt,
+                                                  int nC) {
+  int c = bx * blockDim.x + threadIdx.x;
   if (c >= nC) return;
   ct[c] = st[c];
   cnt[c] = 0;
 }
+__global__ void k_init_cells(const uint8_t *__restrict__ st, uint8_t *__restrict__ ct, unsigned *__restrict__ cnt,
+                             int nC) {
+  k_init_cells_body(blockIdx.x, gridDim.x, st, ct, cnt, nC);
+}
+WS_PLANETS_OPS(k_init_cells_ops, k_init_cells, k_init_cells_body)
 
 // "sea in the sky" test of _markFluidCells (solver.js:964-975) for a float
 // position: float filter, float-float when the float margin is inside its
@@ -98,11 +112,11 @@ __device__ int ceiling_demote(const World &w, const Geo &g, float x, float y, fl
 
 // _markFluidCells, solver.js:945-1000, plus the counting-sort histogram of
 // the particles that stay grid-coupled (their bin is the cell they mark).
-__global__ void k_mark(Geo g, World w, int nP, const float *__restrict__ px, const float *__restrict__ py,
-                       const float *__restrict__ pz, uint8_t *__restrict__ pflag, uint8_t *__restrict__ ct,
-                       unsigned *__restrict__ cnt, int *__restrict__ pcell, unsigned *__restrict__ prank,
-                       DevScalars *s) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_mark_body(long bx, long nbx, Geo g, World w, int nP, const float *__restrict__ px, const float *__restrict__ py,
+                                            const float *__restrict__ pz, uint8_t *__restrict__ pflag, uint8_t *__restrict__ ct,
+                                            unsigned *__restrict__ cnt, int *__restrict__ pcell, unsigned *__restrict__ prank,
+                                            DevScalars *s) {
+  int p = bx * blockDim.x + threadIdx.x;
   if (p >= nP) return;
   if (pflag[p] != 0) { pcell[p] = -1; return; }
   float x = px[p], y = py[p], z = pz[p];
@@ -118,14 +132,26 @@ __global__ void k_mark(Geo g, World w, int nP, const float *__restrict__ px, con
   pcell[p] = c;
   prank[p] = atomicAdd(&cnt[c], 1u);
 }
+__global__ void k_mark(Geo g, World w, int nP, const float *__restrict__ px, const float *__restrict__ py,
+                       const float *__restrict__ pz, uint8_t *__restrict__ pflag, uint8_t *__restrict__ ct,
+                       unsigned *__restrict__ cnt, int *__restrict__ pcell, unsigned *__restrict__ prank,
+                       DevScalars *s) {
+  k_mark_body(blockIdx.x, gridDim.x, g, w, nP, px, py, pz, pflag, ct, cnt, pcell, prank, s);
+}
+WS_PLANETS_OPS(k_mark_ops, k_mark, k_mark_body)
 
-__global__ void k_scatter(int nP, const int *__restrict__ pcell, const unsigned *__restrict__ prank,
-                          const unsigned *__restrict__ start, unsigned *__restrict__ sorted) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_scatter_body(long bx, long nbx, int nP, const int *__restrict__ pcell, const unsigned *__restrict__ prank,
+                                               const unsigned *__restrict__ start, unsigned *__restrict__ sorted) {
+  int p = bx * blockDim.x + threadIdx.x;
   if (p >= nP) return;
   int c = pcell[p];
   if (c >= 0) sorted[start[c] + prank[p]] = (unsigned)p;
 }
+__global__ void k_scatter(int nP, const int *__restrict__ pcell, const unsigned *__restrict__ prank,
+                          const unsigned *__restrict__ start, unsigned *__restrict__ sorted) {
+  k_scatter_body(blockIdx.x, gridDim.x, nP, pcell, prank, start, sorted);
+}
+WS_PLANETS_OPS(k_scatter_ops, k_scatter, k_scatter_body)
 
 // -------------------------------------------------------------------- P2G
 // _p2gInto + _p2gNormalizeFace, solver.js:1035-1120. One workgroup per cell
@@ -157,12 +183,12 @@ __global__ void k_scatter(int nP, const int *__restrict__ pcell, const unsigned 
 __device__ __forceinline__ long long fix(float v) { return (long long)__builtin_rintf(v * kFixScaleF); }
 
 template <int TX, int TY, int TZ>
-__global__ __launch_bounds__(256) void k_p2g_tile(Geo g, int tilesX, int tilesY, const unsigned *__restrict__ start,
-                                                  const unsigned *__restrict__ sorted, const float *__restrict__ px,
-                                                  const float *__restrict__ py, const float *__restrict__ pz,
-                                                  const float *__restrict__ pvx, const float *__restrict__ pvy,
-                                                  const float *__restrict__ pvz, float *__restrict__ vel,
-                                                  float *__restrict__ wgt) {
+__device__ __forceinline__ void k_p2g_tile_body(long bx, long, Geo g, int tilesX, int tilesY,
+                                                const unsigned *__restrict__ start, const unsigned *__restrict__ sorted,
+                                                const float *__restrict__ px, const float *__restrict__ py,
+                                                const float *__restrict__ pz, const float *__restrict__ pvx,
+                                                const float *__restrict__ pvy, const float *__restrict__ pvz,
+                                                float *__restrict__ vel, float *__restrict__ wgt) {
   // owned faces of one component (+1 plane along its axis for the last tile)
   constexpr int BOX = cmax(cmax((TX + 1) * TY * TZ, TX * (TY + 1) * TZ), TX * TY * (TZ + 1));
   constexpr int NSEG = 3 * (TY + 2) * (TZ + 2);
@@ -173,7 +199,7 @@ __global__ __launch_bounds__(256) void k_p2g_tile(Geo g, int tilesX, int tilesY,
   __shared__ unsigned clsCount[27], clsStart[28];
   __shared__ unsigned wsum[4];
 
-  const int b = blockIdx.x;
+  const int b = (int)bx;
   const int tx = b % tilesX, ty = (b / tilesX) % tilesY, tz = b / (tilesX * tilesY);
   const int i0 = tx * TX, j0 = ty * TY, k0 = tz * TZ;
   // particle rows of the tile + halo: cells [i0-1, i0+T] x [j0-1, j0+T] x [k0-1, k0+T]
@@ -357,6 +383,16 @@ __global__ __launch_bounds__(256) void k_p2g_tile(Geo g, int tilesX, int tilesY,
     }
   }
 }
+template <int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void k_p2g_tile(Geo g, int tilesX, int tilesY, const unsigned *__restrict__ start,
+                                                  const unsigned *__restrict__ sorted, const float *__restrict__ px,
+                                                  const float *__restrict__ py, const float *__restrict__ pz,
+                                                  const float *__restrict__ pvx, const float *__restrict__ pvy,
+                                                  const float *__restrict__ pvz, float *__restrict__ vel,
+                                                  float *__restrict__ wgt) {
+  k_p2g_tile_body<TX, TY, TZ>(blockIdx.x, gridDim.x, g, tilesX, tilesY, start, sorted, px, py, pz, pvx, pvy, pvz, vel,
+                              wgt);
+}
 #define WS_P2G_INST(X, Y, Z)                                                                                   \
   template __global__ void k_p2g_tile<X, Y, Z>(Geo, int, int, const unsigned *, const unsigned *, const float *, \
                                                const float *, const float *, const float *, const float *,      \
@@ -366,6 +402,14 @@ WS_P2G_INST(4, 4, 4)
 WS_P2G_INST(16, 8, 8)
 WS_P2G_INST(8, 8, 4)
 #undef WS_P2G_INST
+WS_PLANETS_OPS(k_p2g_tile_ops0, (k_p2g_tile<P2G_BINS[0][0], P2G_BINS[0][1], P2G_BINS[0][2]>),
+               (k_p2g_tile_body<P2G_BINS[0][0], P2G_BINS[0][1], P2G_BINS[0][2]>))
+WS_PLANETS_OPS(k_p2g_tile_ops1, (k_p2g_tile<P2G_BINS[1][0], P2G_BINS[1][1], P2G_BINS[1][2]>),
+               (k_p2g_tile_body<P2G_BINS[1][0], P2G_BINS[1][1], P2G_BINS[1][2]>))
+WS_PLANETS_OPS(k_p2g_tile_ops2, (k_p2g_tile<P2G_BINS[2][0], P2G_BINS[2][1], P2G_BINS[2][2]>),
+               (k_p2g_tile_body<P2G_BINS[2][0], P2G_BINS[2][1], P2G_BINS[2][2]>))
+WS_PLANETS_OPS(k_p2g_tile_ops3, (k_p2g_tile<P2G_BINS[3][0], P2G_BINS[3][1], P2G_BINS[3][2]>),
+               (k_p2g_tile_body<P2G_BINS[3][0], P2G_BINS[3][1], P2G_BINS[3][2]>))
 
 // ------------------------------------------------- boundary conditions
 // Face classification shared by BC, gravity, projection and extrapolation.
@@ -428,12 +472,12 @@ __device__ __forceinline__ float solid_face_value(const Geo &g, const FaceInfo &
   return tl == SOLID ? a[fi.cl] : a[fi.cr];
 }
 
-__global__ void k_bc_snapshot(Geo g, const uint8_t *__restrict__ ct, float *__restrict__ vel,
-                              float *__restrict__ velO, const float *__restrict__ wgt,
-                              uint8_t *__restrict__ valid, int fuse_gravity, const float *__restrict__ gvec,
-                              const DevScalars *__restrict__ s, int write_valid, const float *__restrict__ sv) {
+__device__ __forceinline__ void k_bc_snapshot_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct, float *__restrict__ vel,
+                                                   float *__restrict__ velO, const float *__restrict__ wgt,
+                                                   uint8_t *__restrict__ valid, int fuse_gravity, const float *__restrict__ gvec,
+                                                   const DevScalars *__restrict__ s, int write_valid, const float *__restrict__ sv) {
   long F = (long)g.uN + g.vN + g.wN;
-  long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long f = (long)bx * blockDim.x + threadIdx.x;
   if (f >= F) return;
   FaceInfo fi;
   face_of(g, f, fi);
@@ -445,6 +489,13 @@ __global__ void k_bc_snapshot(Geo g, const uint8_t *__restrict__ ct, float *__re
   if (fuse_gravity && !bc && s->gs > 0.f && (tl == FLUID || tr == FLUID)) u = u + gvec[f] * s->dt * s->gs;
   vel[f] = u;
 }
+__global__ void k_bc_snapshot(Geo g, const uint8_t *__restrict__ ct, float *__restrict__ vel,
+                              float *__restrict__ velO, const float *__restrict__ wgt,
+                              uint8_t *__restrict__ valid, int fuse_gravity, const float *__restrict__ gvec,
+                              const DevScalars *__restrict__ s, int write_valid, const float *__restrict__ sv) {
+  k_bc_snapshot_body(blockIdx.x, gridDim.x, g, ct, vel, velO, wgt, valid, fuse_gravity, gvec, s, write_valid, sv);
+}
+WS_PLANETS_OPS(k_bc_snapshot_ops, k_bc_snapshot, k_bc_snapshot_body)
 
 // extrapolation validity alone (solver.js:1578-1611), for the stand-alone
 // _extrapolate stage; the substep computes it inside k_bc_snapshot
@@ -460,11 +511,11 @@ __global__ void k_valid(Geo g, const uint8_t *__restrict__ ct, const float *__re
 }
 
 // radial gravity + _applyBC after viscosity
-__global__ void k_gravity_bc(Geo g, const uint8_t *__restrict__ ct, float *__restrict__ vel,
-                             const float *__restrict__ gvec, const DevScalars *__restrict__ s,
-                             const float *__restrict__ sv) {
+__device__ __forceinline__ void k_gravity_bc_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct, float *__restrict__ vel,
+                                                  const float *__restrict__ gvec, const DevScalars *__restrict__ s,
+                                                  const float *__restrict__ sv) {
   long F = (long)g.uN + g.vN + g.wN;
-  long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long f = (long)bx * blockDim.x + threadIdx.x;
   if (f >= F) return;
   FaceInfo fi;
   face_of(g, f, fi);
@@ -473,6 +524,12 @@ __global__ void k_gravity_bc(Geo g, const uint8_t *__restrict__ ct, float *__res
   if (bc) { vel[f] = solid_face_value(g, fi, tl, sv); return; }
   if (s->gs > 0.f && (tl == FLUID || tr == FLUID)) vel[f] = vel[f] + gvec[f] * s->dt * s->gs;
 }
+__global__ void k_gravity_bc(Geo g, const uint8_t *__restrict__ ct, float *__restrict__ vel,
+                             const float *__restrict__ gvec, const DevScalars *__restrict__ s,
+                             const float *__restrict__ sv) {
+  k_gravity_bc_body(blockIdx.x, gridDim.x, g, ct, vel, gvec, s, sv);
+}
+WS_PLANETS_OPS(k_gravity_bc_ops, k_gravity_bc, k_gravity_bc_body)
 
 // ------------------------------------------------------------ viscosity
 // _viscosity, solver.js:1259-1304, is one in-place lexicographic sweep over
@@ -482,12 +539,12 @@ __global__ void k_gravity_bc(Geo g, const uint8_t *__restrict__ ct, float *__res
 // x(m+1) replaces the lower neighbours by x(m). The error shrinks by 3*coef
 // per iteration, so a few iterations reach float resolution (the host picks
 // the count from coef).
-__global__ void k_visc_iter(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ old,
-                            const float *__restrict__ cur, float *__restrict__ out, float nu_dx2,
-                            const DevScalars *__restrict__ s, int first) {
+__device__ __forceinline__ void k_visc_iter_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ old,
+                                                 const float *__restrict__ cur, float *__restrict__ out, float nu_dx2,
+                                                 const DevScalars *__restrict__ s, int first) {
   const float coef = fminf(nu_dx2 * s->dt, 0.24f);  // solver.js:1264
   long F = (long)g.uN + g.vN + g.wN;
-  long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long f = (long)bx * blockDim.x + threadIdx.x;
   if (f >= F) return;
   FaceInfo fi;
   face_of(g, f, fi);
@@ -529,16 +586,22 @@ __global__ void k_visc_iter(Geo g, const uint8_t *__restrict__ ct, const float *
   val -= hasK1 ? oK1 : u0;
   out[f] = u0 + coef * val;
 }
+__global__ void k_visc_iter(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ old,
+                            const float *__restrict__ cur, float *__restrict__ out, float nu_dx2,
+                            const DevScalars *__restrict__ s, int first) {
+  k_visc_iter_body(blockIdx.x, gridDim.x, g, ct, old, cur, out, nu_dx2, s, first);
+}
+WS_PLANETS_OPS(k_visc_iter_ops, k_visc_iter, k_visc_iter_body)
 
 // ------------------------------------------------------ pressure system
 // _pressureSolve assembly, solver.js:1322-1394: rhs = -dx^2 div, diag =
 // fluid + air neighbours, relaxation weight omega/diag (stored float as the
 // reference's pW), fluid-neighbour mask; non-fluid cells hold q = 0.
-__global__ void k_assemble(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ vel,
-                           float *__restrict__ q, float *__restrict__ q2, float *__restrict__ rhs,
-                           float *__restrict__ pw, uint8_t *__restrict__ nbm, uint8_t *__restrict__ dg,
-                           float omega) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_assemble_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ vel,
+                                                float *__restrict__ q, float *__restrict__ q2, float *__restrict__ rhs,
+                                                float *__restrict__ pw, uint8_t *__restrict__ nbm, uint8_t *__restrict__ dg,
+                                                float omega) {
+  int c = bx * blockDim.x + threadIdx.x;
   if (c >= g.nC) return;
   if (ct[c] != FLUID) { q[c] = 0.f; q2[c] = 0.f; pw[c] = 0.f; nbm[c] = 0; dg[c] = 0; return; }
   int i = c % g.nx, j = (c / g.nx) % g.ny, k = c / (g.nx * g.ny);
@@ -563,6 +626,13 @@ __global__ void k_assemble(Geo g, const uint8_t *__restrict__ ct, const float *_
   nbm[c] = (uint8_t)m;
   dg[c] = (uint8_t)diag;
 }
+__global__ void k_assemble(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ vel,
+                           float *__restrict__ q, float *__restrict__ q2, float *__restrict__ rhs,
+                           float *__restrict__ pw, uint8_t *__restrict__ nbm, uint8_t *__restrict__ dg,
+                           float omega) {
+  k_assemble_body(blockIdx.x, gridDim.x, g, ct, vel, q, q2, rhs, pw, nbm, dg, omega);
+}
+WS_PLANETS_OPS(k_assemble_ops, k_assemble, k_assemble_body)
 
 // One colour of red-black SOR (the relaxation of solver.js:1402): cells with
 // (i+j+k)&1 == color. Each thread updates one cell of that colour.
@@ -686,8 +756,8 @@ __global__ __launch_bounds__(256) void k_sor_tile(Geo g, int tilesX, int tilesY,
 
 // scan keys: 1 for a red fluid cell, 2^32 for a black one (one 64-bit
 // exclusive scan then gives both list positions)
-__global__ void k_sor_count(Geo g, const uint8_t *__restrict__ nbm, unsigned long long *__restrict__ key) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_sor_count_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ nbm, unsigned long long *__restrict__ key) {
+  int c = bx * blockDim.x + threadIdx.x;
   if (c > g.nC) return;
   unsigned long long v = 0;
   if (c < g.nC && (nbm[c] & 0x40)) {
@@ -696,13 +766,17 @@ __global__ void k_sor_count(Geo g, const uint8_t *__restrict__ nbm, unsigned lon
   }
   key[c] = v;
 }
+__global__ void k_sor_count(Geo g, const uint8_t *__restrict__ nbm, unsigned long long *__restrict__ key) {
+  k_sor_count_body(blockIdx.x, gridDim.x, g, nbm, key);
+}
+WS_PLANETS_OPS(k_sor_count_ops, k_sor_count, k_sor_count_body)
 
-__global__ void k_sor_compact(Geo g, const uint8_t *__restrict__ nbm, const uint8_t *__restrict__ dg,
-                              const float *__restrict__ rhs, const float *__restrict__ pw,
-                              const unsigned long long *__restrict__ off, long half, unsigned *__restrict__ code,
-                              float *__restrict__ lrhs, float *__restrict__ lpw, uint8_t *__restrict__ ldg,
-                              DevScalars *ds) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_sor_compact_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ nbm, const uint8_t *__restrict__ dg,
+                                                   const float *__restrict__ rhs, const float *__restrict__ pw,
+                                                   const unsigned long long *__restrict__ off, long half, unsigned *__restrict__ code,
+                                                   float *__restrict__ lrhs, float *__restrict__ lpw, uint8_t *__restrict__ ldg,
+                                                   DevScalars *ds) {
+  int c = bx * blockDim.x + threadIdx.x;
   if (c > g.nC) return;
   if (c == g.nC) {
     unsigned long long t = off[c];
@@ -720,6 +794,14 @@ __global__ void k_sor_compact(Geo g, const uint8_t *__restrict__ nbm, const uint
   lpw[pos] = pw[c];
   ldg[pos] = dg[c];
 }
+__global__ void k_sor_compact(Geo g, const uint8_t *__restrict__ nbm, const uint8_t *__restrict__ dg,
+                              const float *__restrict__ rhs, const float *__restrict__ pw,
+                              const unsigned long long *__restrict__ off, long half, unsigned *__restrict__ code,
+                              float *__restrict__ lrhs, float *__restrict__ lpw, uint8_t *__restrict__ ldg,
+                              DevScalars *ds) {
+  k_sor_compact_body(blockIdx.x, gridDim.x, g, nbm, dg, rhs, pw, off, half, code, lrhs, lpw, ldg, ds);
+}
+WS_PLANETS_OPS(k_sor_compact_ops, k_sor_compact, k_sor_compact_body)
 
 // one colour: q[c] = q[c](1 - omega) + omega/diag (rhs + sum of fluid
 // neighbours), the arithmetic of k_sor
@@ -1550,11 +1632,11 @@ __device__ __forceinline__ unsigned abs_bits(float u) {
   return (av == av) ? __float_as_uint(av) : 0u;
 }
 
-__global__ __launch_bounds__(256) void k_project(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ phi,
-                                                 float *__restrict__ vel, int do_clamp, float vcap, DevScalars *s) {
+__device__ __forceinline__ void k_project_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ phi,
+                                               float *__restrict__ vel, int do_clamp, float vcap, DevScalars *s) {
   long F = (long)g.uN + g.vN + g.wN;
   unsigned bits = 0;
-  for (long f = (long)blockIdx.x * blockDim.x + threadIdx.x; f < F; f += (long)gridDim.x * blockDim.x) {
+  for (long f = (long)bx * blockDim.x + threadIdx.x; f < F; f += (long)nbx * blockDim.x) {
     FaceInfo fi;
     face_of(g, f, fi);
     float u = vel[f];
@@ -1573,10 +1655,15 @@ __global__ __launch_bounds__(256) void k_project(Geo g, const uint8_t *__restric
   }
   if (do_clamp) umax_reduce(bits, &s->umax_bits);
 }
+__global__ __launch_bounds__(256) void k_project(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ phi,
+                                                 float *__restrict__ vel, int do_clamp, float vcap, DevScalars *s) {
+  k_project_body(blockIdx.x, gridDim.x, g, ct, phi, vel, do_clamp, vcap, s);
+}
+WS_PLANETS_OPS(k_project_ops, k_project, k_project_body)
 
-__global__ __launch_bounds__(256) void k_clamp(long F, float *__restrict__ vel, float vcap, DevScalars *s) {
+__device__ __forceinline__ void k_clamp_body(long bx, long nbx, long F, float *__restrict__ vel, float vcap, DevScalars *s) {
   unsigned bits = 0;
-  for (long f = (long)blockIdx.x * blockDim.x + threadIdx.x; f < F; f += (long)gridDim.x * blockDim.x) {
+  for (long f = (long)bx * blockDim.x + threadIdx.x; f < F; f += (long)nbx * blockDim.x) {
     float u = vel[f];
     if (u > vcap) u = vcap; else if (u < -vcap) u = -vcap;
     vel[f] = u;
@@ -1584,13 +1671,17 @@ __global__ __launch_bounds__(256) void k_clamp(long F, float *__restrict__ vel, 
   }
   umax_reduce(bits, &s->umax_bits);
 }
+__global__ __launch_bounds__(256) void k_clamp(long F, float *__restrict__ vel, float vcap, DevScalars *s) {
+  k_clamp_body(blockIdx.x, gridDim.x, F, vel, vcap, s);
+}
+WS_PLANETS_OPS(k_clamp_ops, k_clamp, k_clamp_body)
 
 // ------------------------------------------------ vorticity confinement
 // _vorticityConfinement, solver.js:1454-1559, in three cell passes and one
 // face pass. scratch layout: vc[3C] centred velocity, om[4C] (x,y,z,|w|),
 // fc[3C] velocity deltas of interior fluid cells.
-__global__ void k_vort_center(Geo g, const float *__restrict__ vel, float *__restrict__ vc) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_vort_center_body(long bx, long nbx, Geo g, const float *__restrict__ vel, float *__restrict__ vc) {
+  int c = bx * blockDim.x + threadIdx.x;
   if (c >= g.nC) return;
   int i = c % g.nx, j = (c / g.nx) % g.ny, k = c / (g.nx * g.ny);
   const float *u = vel, *v = vel + g.uN, *w = vel + g.uN + g.vN;
@@ -1599,9 +1690,13 @@ __global__ void k_vort_center(Geo g, const float *__restrict__ vel, float *__res
   vc[g.nC + c] = 0.5f * (v[rv] + v[rv + g.nx]);
   vc[2 * g.nC + c] = 0.5f * (w[c] + w[c + g.nx * g.ny]);
 }
+__global__ void k_vort_center(Geo g, const float *__restrict__ vel, float *__restrict__ vc) {
+  k_vort_center_body(blockIdx.x, gridDim.x, g, vel, vc);
+}
+WS_PLANETS_OPS(k_vort_center_ops, k_vort_center, k_vort_center_body)
 
-__global__ void k_vort_omega(Geo g, const float *__restrict__ vc, float *__restrict__ om, float inv2) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_vort_omega_body(long bx, long nbx, Geo g, const float *__restrict__ vc, float *__restrict__ om, float inv2) {
+  int c = bx * blockDim.x + threadIdx.x;
   if (c >= g.nC) return;
   int i = c % g.nx, j = (c / g.nx) % g.ny, k = c / (g.nx * g.ny), nxy = g.nx * g.ny;
   int im = i > 0 ? c - 1 : c, ip = i < g.nx - 1 ? c + 1 : c;
@@ -1616,10 +1711,14 @@ __global__ void k_vort_omega(Geo g, const float *__restrict__ vc, float *__restr
   om[2 * g.nC + c] = wz;
   om[3 * g.nC + c] = sqrtf(wx * wx + wy * wy + wz * wz);
 }
+__global__ void k_vort_omega(Geo g, const float *__restrict__ vc, float *__restrict__ om, float inv2) {
+  k_vort_omega_body(blockIdx.x, gridDim.x, g, vc, om, inv2);
+}
+WS_PLANETS_OPS(k_vort_omega_ops, k_vort_omega, k_vort_omega_body)
 
-__global__ void k_vort_force(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ om,
-                             float *__restrict__ fc, float eps, const DevScalars *__restrict__ s) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_vort_force_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ om,
+                                                  float *__restrict__ fc, float eps, const DevScalars *__restrict__ s) {
+  int c = bx * blockDim.x + threadIdx.x;
   if (c >= g.nC) return;
   if (ct[c] != FLUID) return;
   int i = c % g.nx, j = (c / g.nx) % g.ny, k = c / (g.nx * g.ny), nxy = g.nx * g.ny;
@@ -1649,11 +1748,16 @@ __global__ void k_vort_force(Geo g, const uint8_t *__restrict__ ct, const float 
   fc[g.nC + c] = fy;
   fc[2 * g.nC + c] = fz;
 }
+__global__ void k_vort_force(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ om,
+                             float *__restrict__ fc, float eps, const DevScalars *__restrict__ s) {
+  k_vort_force_body(blockIdx.x, gridDim.x, g, ct, om, fc, eps, s);
+}
+WS_PLANETS_OPS(k_vort_force_ops, k_vort_force, k_vort_force_body)
 
-__global__ void k_vort_splat(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ fc,
-                             float *__restrict__ vel) {
+__device__ __forceinline__ void k_vort_splat_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ fc,
+                                                  float *__restrict__ vel) {
   long F = (long)g.uN + g.vN + g.wN;
-  long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long f = (long)bx * blockDim.x + threadIdx.x;
   if (f >= F) return;
   FaceInfo fi;
   face_of(g, f, fi);
@@ -1661,6 +1765,11 @@ __global__ void k_vort_splat(Geo g, const uint8_t *__restrict__ ct, const float 
   const float *fcc = fc + (long)fi.comp * g.nC;
   vel[f] = vel[f] + 0.5f * (fcc[fi.cl] + fcc[fi.cr]);
 }
+__global__ void k_vort_splat(Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ fc,
+                             float *__restrict__ vel) {
+  k_vort_splat_body(blockIdx.x, gridDim.x, g, ct, fc, vel);
+}
+WS_PLANETS_OPS(k_vort_splat_ops, k_vort_splat, k_vort_splat_body)
 
 // _extrapolate's flood (solver.js:1613-1634): in lexicographic order, an
 // invalid face takes the mean of its valid neighbours, where a face filled

@@ -179,6 +179,7 @@ struct ExPlanet {
 
 struct Net;       // the learned warm start (ws_learn.h)
 struct Contacts;  // scratch of the frame-end contact passes (ws_contacts.hip)
+struct Recorder;  // a recording of stage launches (ws_stage.h)
 
 struct Ctx {
   // geometry
@@ -269,6 +270,8 @@ struct Ctx {
   long netCalls = 0;
 
   hipStream_t stream = nullptr;
+  // while set, stage launches are recorded here instead of launched
+  Recorder *rec = nullptr;
   // static world
   uint8_t *staticTypes = nullptr;
   float *gu = nullptr, *gv = nullptr, *gw = nullptr;

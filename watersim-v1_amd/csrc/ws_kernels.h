@@ -1,6 +1,7 @@
 // Kernel declarations and the per-call parameter block of the substep.
 #pragma once
 #include "ws_internal.h"
+#include "ws_stage.h"
 #include "ws_world.h"
 
 namespace ws {
@@ -126,6 +127,13 @@ __global__ void k_vapor_c(Geo g, World w, Par P, int nP, float *px, float *py, f
                           const uint32_t *rng);
 __global__ void k_rng_advance(uint32_t *rng, int which, const unsigned *need, const unsigned *off, int nP);
 __global__ void k_sample_vel(Geo g, long n, const float *xyz, const float *vel, float *out);
+
+// the substep's stages as launches over many planets (ws_stage.h); the
+// P2G ops are indexed like P2G_BINS
+extern StageOps k_begin_substep_ops, k_end_substep_ops, k_init_cells_ops, k_mark_ops, k_scatter_ops,
+    k_bc_snapshot_ops, k_gravity_bc_ops, k_visc_iter_ops, k_assemble_ops, k_sor_count_ops, k_sor_compact_ops,
+    k_project_ops, k_clamp_ops, k_vort_center_ops, k_vort_omega_ops, k_vort_force_ops, k_vort_splat_ops,
+    k_g2p_advect_ops, k_p2g_tile_ops0, k_p2g_tile_ops1, k_p2g_tile_ops2, k_p2g_tile_ops3;
 
 // surface
 __global__ void k_splat_density(Geo g, World w, int nP, const unsigned *start, const unsigned *sorted,

@@ -655,14 +655,14 @@ __global__ void k_crop(NetDims full, NetDims cd, int oz, int oy, int ox, const b
 // ---- the solver side: inputs from the pressure system, the correction
 // initial residual r0 = rhs - A q over fluid cells and FP64 sums for the
 // two scales (sum r0^2, sum q^2, fluid count)
-__global__ void k_ws_residual(NetDims d, const uint8_t *__restrict__ ct, const uint8_t *__restrict__ nbm,
-                              const uint8_t *__restrict__ dg, const float *__restrict__ rhs,
-                              const float *__restrict__ q, float *__restrict__ r0, double *__restrict__ part) {
+__device__ __forceinline__ void k_ws_residual_body(long bx, long nbx, NetDims d, const uint8_t *__restrict__ ct, const uint8_t *__restrict__ nbm,
+                                                   const uint8_t *__restrict__ dg, const float *__restrict__ rhs,
+                                                   const float *__restrict__ q, float *__restrict__ r0, double *__restrict__ part) {
   __shared__ double sa[256], sb[256], sc[256];
   double a = 0, b = 0, n = 0;
   long C = (long)d.Z * d.Y * d.X;
   int nx = d.X, nxy = d.X * d.Y;
-  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < C; c += (long)gridDim.x * blockDim.x) {
+  for (long c = (long)bx * blockDim.x + threadIdx.x; c < C; c += (long)nbx * blockDim.x) {
     unsigned mk = nbm[c];
     if (!(mk & 0x40)) { r0[c] = 0.f; continue; }
     const float sum = nb_sum(q, c, mk, nx, nxy);
@@ -680,12 +680,18 @@ __global__ void k_ws_residual(NetDims d, const uint8_t *__restrict__ ct, const u
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) { part[3 * blockIdx.x] = sa[0]; part[3 * blockIdx.x + 1] = sb[0]; part[3 * blockIdx.x + 2] = sc[0]; }
+  if (threadIdx.x == 0) { part[3 * bx] = sa[0]; part[3 * bx + 1] = sb[0]; part[3 * bx + 2] = sc[0]; }
 }
+__global__ void k_ws_residual(NetDims d, const uint8_t *__restrict__ ct, const uint8_t *__restrict__ nbm,
+                              const uint8_t *__restrict__ dg, const float *__restrict__ rhs,
+                              const float *__restrict__ q, float *__restrict__ r0, double *__restrict__ part) {
+  k_ws_residual_body(blockIdx.x, gridDim.x, d, ct, nbm, dg, rhs, q, r0, part);
+}
+WS_PLANETS_OPS(k_ws_residual_ops, k_ws_residual, k_ws_residual_body)
 // the block partials summed in a fixed order by 256 threads (thread t takes
 // partials t, t + 256, ... in turn, then a pairwise tree); nb <= 1024
-__global__ __launch_bounds__(256) void k_ws_scales(int nb, const double *__restrict__ part,
-                                                   float *__restrict__ scales) {
+__device__ __forceinline__ void k_ws_scales_body(long bx, long nbx, int nb, const double *__restrict__ part,
+                                                 float *__restrict__ scales) {
   __shared__ double sa[256], sb[256], sn[256];
   double a = 0, b = 0, n = 0;
   for (int i = threadIdx.x; i < nb; i += 256) { a += part[3 * i]; b += part[3 * i + 1]; n += part[3 * i + 2]; }
@@ -703,10 +709,15 @@ __global__ __launch_bounds__(256) void k_ws_scales(int nb, const double *__restr
   scales[0] = s > 0 ? (float)s : 1.f;
   scales[1] = sq > 0 ? (float)sq : 1.f;
 }
+__global__ __launch_bounds__(256) void k_ws_scales(int nb, const double *__restrict__ part,
+                                                   float *__restrict__ scales) {
+  k_ws_scales_body(blockIdx.x, gridDim.x, nb, part, scales);
+}
+WS_PLANETS_OPS(k_ws_scales_ops, k_ws_scales, k_ws_scales_body)
 // the five input channels [r0/s, air, fluid, solid, q/sq] as bf16 [V][8]
-__global__ void k_ws_inputs(long C, const uint8_t *__restrict__ ct, const float *__restrict__ r0,
-                            const float *__restrict__ q, const float *__restrict__ scales, bf16_t *__restrict__ x) {
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_ws_inputs_body(long bx, long nbx, long C, const uint8_t *__restrict__ ct, const float *__restrict__ r0,
+                                                 const float *__restrict__ q, const float *__restrict__ scales, bf16_t *__restrict__ x) {
+  long c = (long)bx * blockDim.x + threadIdx.x;
   if (c >= C) return;
   uint8_t t = ct[c];
   float s = scales[0], sq = scales[1];
@@ -715,13 +726,18 @@ __global__ void k_ws_inputs(long C, const uint8_t *__restrict__ ct, const float 
                  f2bf(t == SOLID ? 1.f : 0.f), fl ? f2bf(q[c] / sq) : (bf16_t)0, 0, 0, 0};
   for (int i = 0; i < 8; i++) x[c * 8 + i] = v[i];
 }
+__global__ void k_ws_inputs(long C, const uint8_t *__restrict__ ct, const float *__restrict__ r0,
+                            const float *__restrict__ q, const float *__restrict__ scales, bf16_t *__restrict__ x) {
+  k_ws_inputs_body(blockIdx.x, gridDim.x, C, ct, r0, q, scales, x);
+}
+WS_PLANETS_OPS(k_ws_inputs_ops, k_ws_inputs, k_ws_inputs_body)
 // the five input channels of the grid coarsened 2x per axis: each coarse
 // cell averages its eight cells' full-resolution channels (r0/s and q/sq on
 // fluid cells, 0 elsewhere; the type one-hots, so types become fractions)
-__global__ void k_ws_inputs_coarse(NetDims cd, const uint8_t *__restrict__ ct, const float *__restrict__ r0,
-                                   const float *__restrict__ q, const float *__restrict__ scales,
-                                   bf16_t *__restrict__ x) {
-  long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_ws_inputs_coarse_body(long bx, long nbx, NetDims cd, const uint8_t *__restrict__ ct, const float *__restrict__ r0,
+                                                        const float *__restrict__ q, const float *__restrict__ scales,
+                                                        bf16_t *__restrict__ x) {
+  long v = (long)bx * blockDim.x + threadIdx.x;
   long V = (long)cd.Z * cd.Y * cd.X;
   if (v >= V) return;
   const int X = 2 * cd.X, XY = 4 * cd.X * cd.Y;
@@ -740,6 +756,12 @@ __global__ void k_ws_inputs_coarse(NetDims cd, const uint8_t *__restrict__ ct, c
   }
   for (int i = 0; i < 8; i++) x[v * 8 + i] = i < 5 ? f2bf(a[i] * 0.125f) : (bf16_t)0;
 }
+__global__ void k_ws_inputs_coarse(NetDims cd, const uint8_t *__restrict__ ct, const float *__restrict__ r0,
+                                   const float *__restrict__ q, const float *__restrict__ scales,
+                                   bf16_t *__restrict__ x) {
+  k_ws_inputs_coarse_body(blockIdx.x, gridDim.x, cd, ct, r0, q, scales, x);
+}
+WS_PLANETS_OPS(k_ws_inputs_coarse_ops, k_ws_inputs_coarse, k_ws_inputs_coarse_body)
 // q += s y(parent) on fluid cells: the coarse prediction, constant over
 // each 2x2x2 block
 __device__ __forceinline__ void apply_coarse_at(NetDims cd, const uint8_t *__restrict__ ct, const float *__restrict__ y,
@@ -905,10 +927,11 @@ long vox(NetDims d) { return (long)d.Z * d.Y * d.X; }
 NetDims half(NetDims d) { return NetDims{d.Z / 2, d.Y / 2, d.X / 2}; }
 
 // the form of a product with NT x 16 output channels on grid d: 0 untiled,
-// 1 four waves per 16 voxels with K split (too few LDS tiles to fill the
-// GPU), 2 LDS-tiled; and its blocks
+// 1 four waves per 16 voxels with K split (netTiled 1, on grids with too few
+// LDS tiles to fill the GPU alone), 2 LDS-tiled; and its blocks
 int conv_form(const ws_ctx *c, NetDims d, int NT) {
   if (!c->netTiled) return 0;
+  if (c->netTiled == 2) return 2;
   const int MT = NT >= 3 ? 4 : 8;
   const long tiles = (long)((d.X + 15) / 16) * ((d.Y + 7) / 8) * ((d.Z + MT / 2 - 1) / (MT / 2));
   return tiles < 256 ? 1 : 2;
@@ -1219,16 +1242,16 @@ static int solver_inputs(ws_ctx *c) {
   Net *n = c->net;
   NetDims d = grid_dims(c);
   long C = c->nC;
-  hipLaunchKernelGGL(k_ws_residual, dim3(1024), dim3(256), 0, c->stream, d, c->cellType, c->nbm, c->dg, c->rhs, c->q,
-                     n->r0, n->rpart);
-  hipLaunchKernelGGL(k_ws_scales, dim3(1), dim3(256), 0, c->stream, 1024, n->rpart, n->scales);
+  stage_launch(c, k_ws_residual_ops, k_ws_residual, 1024, 256, d, c->cellType, c->nbm, c->dg, c->rhs, c->q, n->r0,
+               n->rpart);
+  stage_launch(c, k_ws_scales_ops, k_ws_scales, 1, 256, 1024, n->rpart, n->scales);
   if (c->netCoarse) {
     NetDims cd = net_grid(c, 1);
-    hipLaunchKernelGGL(k_ws_inputs_coarse, dim3(blocks_for(vox(cd), 256)), dim3(256), 0, c->stream, cd, c->cellType,
-                       n->r0, c->q, n->scales, n->x0);
+    stage_launch(c, k_ws_inputs_coarse_ops, k_ws_inputs_coarse, blocks_for(vox(cd), 256), 256, cd, c->cellType, n->r0,
+                 c->q, n->scales, n->x0);
   } else {
-    hipLaunchKernelGGL(k_ws_inputs, dim3(blocks_for(C, 256)), dim3(256), 0, c->stream, C, c->cellType, n->r0, c->q,
-                       n->scales, n->x0);
+    stage_launch(c, k_ws_inputs_ops, k_ws_inputs, blocks_for(C, 256), 256, C, c->cellType, n->r0, c->q, n->scales,
+                 n->x0);
   }
   WS_LAUNCHED();
   return WS_OK;

@@ -256,16 +256,16 @@ __device__ void advect_one(const Geo &g, const World &w, const Par &P, const flo
 }
 
 // mode: 1 = G2P only (writes adv), 2 = advect only (reads adv), 3 = fused
-__global__ __launch_bounds__(256) void k_g2p_advect(Geo g, World w, Par P, int nP, int mode,
-                                                    const float *__restrict__ vel, const float *__restrict__ velO,
-                                                    const uint8_t *__restrict__ ct, float *__restrict__ px,
-                                                    float *__restrict__ py, float *__restrict__ pz,
-                                                    float *__restrict__ pvx, float *__restrict__ pvy,
-                                                    float *__restrict__ pvz, uint8_t *__restrict__ pflag,
-                                                    uint8_t *__restrict__ pcool, float *__restrict__ pAir,
-                                                    float *__restrict__ advU, float *__restrict__ advV,
-                                                    float *__restrict__ advW, DevScalars *s) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_g2p_advect_body(long bx, long nbx, Geo g, World w, Par P, int nP, int mode,
+                                                  const float *__restrict__ vel, const float *__restrict__ velO,
+                                                  const uint8_t *__restrict__ ct, float *__restrict__ px,
+                                                  float *__restrict__ py, float *__restrict__ pz,
+                                                  float *__restrict__ pvx, float *__restrict__ pvy,
+                                                  float *__restrict__ pvz, uint8_t *__restrict__ pflag,
+                                                  uint8_t *__restrict__ pcool, float *__restrict__ pAir,
+                                                  float *__restrict__ advU, float *__restrict__ advV,
+                                                  float *__restrict__ advW, DevScalars *s) {
+  int p = bx * blockDim.x + threadIdx.x;
   unsigned airborne = 0;
   if (p < nP) {
     float dt = s->dt;
@@ -302,6 +302,18 @@ __global__ __launch_bounds__(256) void k_g2p_advect(Geo g, World w, Par P, int n
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&s->airborne, (unsigned)__popcll(m));
   }
 }
+__global__ __launch_bounds__(256) void k_g2p_advect(Geo g, World w, Par P, int nP, int mode,
+                                                    const float *__restrict__ vel, const float *__restrict__ velO,
+                                                    const uint8_t *__restrict__ ct, float *__restrict__ px,
+                                                    float *__restrict__ py, float *__restrict__ pz,
+                                                    float *__restrict__ pvx, float *__restrict__ pvy,
+                                                    float *__restrict__ pvz, uint8_t *__restrict__ pflag,
+                                                    uint8_t *__restrict__ pcool, float *__restrict__ pAir,
+                                                    float *__restrict__ advU, float *__restrict__ advV,
+                                                    float *__restrict__ advW, DevScalars *s) {
+  k_g2p_advect_body(blockIdx.x, gridDim.x, g, w, P, nP, mode, vel, velO, ct, px, py, pz, pvx, pvy, pvz, pflag, pcool, pAir, advU, advV, advW, s);
+}
+WS_PLANETS_OPS(k_g2p_advect_ops, k_g2p_advect, k_g2p_advect_body)
 
 // ---------------------------------------------------------------- vapor
 // Vapor and cloud parcels, solver.js:1978-2174. The reference draws from two

@@ -77,9 +77,9 @@ __device__ __forceinline__ T scan_total(const unsigned long long *status, long q
 // IPT entries per thread, NT threads: a tile of NT * IPT entries. ctl[0] is
 // the epoch (never 0: freshly cleared words carry tag 0).
 template <typename T, int IPT>
-__global__ __launch_bounds__(kScanNT) void k_scan_excl(const T *in, T *out, long n,
-                                                       unsigned long long *__restrict__ status,
-                                                       unsigned *__restrict__ ctl, unsigned *__restrict__ failed) {
+__device__ __forceinline__ void k_scan_excl_body(long bx, long nbx, const T *in, T *out, long n,
+                                                 unsigned long long *__restrict__ status, unsigned *__restrict__ ctl,
+                                                 unsigned *__restrict__ failed) {
   constexpr int TILE = kScanNT * IPT;
   // entry e of the tile at e + e / IPT: thread t's IPT entries are
   // contiguous and consecutive threads' runs start IPT + 1 words apart
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kScanNT) void k_scan_excl(const T *in, T *out, long
   __shared__ T wtot[kScanNT / 64];
   __shared__ T tilePrefix;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long tile = blockIdx.x, nTiles = gridDim.x;
+  const long tile = bx, nTiles = nbx;
   const long base = tile * TILE;
   const unsigned E = __hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
@@ -159,6 +159,15 @@ __global__ __launch_bounds__(kScanNT) void k_scan_excl(const T *in, T *out, long
 }
 
 template <typename T, int IPT>
+__global__ __launch_bounds__(kScanNT) void k_scan_excl(const T *in, T *out, long n,
+                                                       unsigned long long *__restrict__ status,
+                                                       unsigned *__restrict__ ctl, unsigned *__restrict__ failed) {
+  k_scan_excl_body<T, IPT>(blockIdx.x, gridDim.x, in, out, n, status, ctl, failed);
+}
+WS_PLANETS_OPS(k_scan_u32_ops, (k_scan_excl<unsigned, 16>), (k_scan_excl_body<unsigned, 16>))
+WS_PLANETS_OPS(k_scan_u64_ops, (k_scan_excl<unsigned long long, 8>), (k_scan_excl_body<unsigned long long, 8>))
+
+template <typename T, int IPT>
 int launch_scan(ws_ctx *c, const T *in, T *out, long n) {
   if (n <= 0) return WS_OK;
   constexpr long TILE = (long)kScanNT * IPT;
@@ -167,8 +176,9 @@ int launch_scan(ws_ctx *c, const T *in, T *out, long n) {
     c->err = "scan of " + std::to_string(n) + " entries exceeds the scan state";
     return WS_EINVAL;
   }
-  hipLaunchKernelGGL((k_scan_excl<T, IPT>), dim3(tiles), dim3(kScanNT), 0, c->stream, in, out, n, c->scanStatus,
-                     c->scanCtl, &c->ds->scanFailed);
+  const StageOps &ops = sizeof(T) == 4 ? k_scan_u32_ops : k_scan_u64_ops;
+  stage_launch(c, ops, k_scan_excl<T, IPT>, tiles, kScanNT, in, out, n, c->scanStatus, c->scanCtl,
+               &c->ds->scanFailed);
   return WS_OK;
 }
 
