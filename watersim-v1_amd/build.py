@@ -12,11 +12,13 @@ OUT = os.path.join(HERE, "_build", "libwatersim_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # per-file additions: the ball, contact, metaball and surface code evaluate the reference's double
 # formulas operation by operation, so nothing may fuse into an FMA
-FILE_FLAGS = {"ws_balls.hip": ["-ffp-contract=off"], "ws_contacts.hip": ["-ffp-contract=off"],
-              "ws_metaballs.hip": ["-ffp-contract=off"], "ws_surface.hip": ["-ffp-contract=off"],
-              "ws_noalloc.hip": ["-ffp-contract=off"]}
+FILE_FLAGS = {}
+# -ffp-contract=off: no multiply-add is fused unless the source asks for it,
+# so the bits of a computation follow from its source alone, whichever
+# kernel it is inlined into (a stage's body runs in its own kernel and in
+# the kernel that runs it for many planets at once)
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
-         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics", "-ffp-contract=off"]
 
 
 def build(verbose=False):
