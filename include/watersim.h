@@ -236,12 +236,16 @@ int ws_tune(ws_ctx *c, int sweeps, char *report, size_t len);
 
 /* Batched planets (no counterpart in the reference, which runs one solver
  * per page). A batch advances n contexts together, each by whole substeps
- * with the device dt law, as ws_run_substeps does for one context: every
- * planet's stages are queued on its own stream, the pressure solves of all
- * planets run in one launch (one workgroup per planet, residual checks on
- * the device, sorMode 4 arithmetic), and with use_graph the batched substep
- * is captured once per ws_batch_run call into a HIP graph that is replayed
- * for each substep. Each planet's results equal those of ws_run_substeps
+ * with the device dt law, as ws_run_substeps does for one context. Every
+ * stage is one launch over all planets on the batch's stream: the planets'
+ * own stages (each planet's substep is recorded once per ws_batch_run call,
+ * with the settings it has then, and the recordings merged stage by stage),
+ * the learned warm start's forward pass, the pressure solves (workgroups
+ * per planet by size, residual checks on the device, sorMode 4 arithmetic)
+ * and the extrapolation; with use_graph the batched substep is captured
+ * once per call into a HIP graph that is replayed for each substep. (With
+ * "stageBatch" 0, or when a planet has vapor parcels or balls, the planets'
+ * own stages are queued on their own streams instead.) Each planet's results equal those of ws_run_substeps
  * with sorMode 4 and the same sorThreads (the batch's); a planet's solve may
  * span several workgroups, which changes no bits. Between calls the
  * contexts may be read, written and reconfigured as usual; while
@@ -257,8 +261,15 @@ int ws_batch_run(ws_batch *b, int substeps);
  * "maxGroups" (1..256: workgroups of the joint solve in all; they must all be
  * resident at once, so the launch runs alone on the GPU between the planets'
  * stages), "timing" (0/1: the following runs go eagerly, timing the joint
- * solve launch with HIP events; setting it resets the totals). Values: those,
- * "groups" (workgroups of the last joint solve), and the totals of the timed
+ * solve launch with HIP events; setting it resets the totals), "xcdLocal"
+ * (0/1: lay each planet's solve workgroups out on one XCD where they fit,
+ * so that its barriers stay in that XCD's L2), "netBatch" (0/1: the warm
+ * start's forward passes as one launch per network step over all planets;
+ * implied by stageBatch), "stageBatch" (0/1: the planets' own stages merged
+ * into one launch per stage). Values: those, "groups" (workgroups of the
+ * last joint solve), "xcdLocalPlanets" (planets whose last solve ran on one
+ * XCD), "stageLaunches" (merged stage launches per batched substep, 0 when
+ * the stages run per planet), and the totals of the timed
  * runs: "solveMs", "solveLaunches", "sweeps" (summed over planets),
  * "cellSweeps" (fluid cells relaxed, summed over planets and sweeps). */
 int ws_batch_set(ws_batch *b, const char *name, double v);
