@@ -572,8 +572,8 @@ def batched_main(a, cfg):
     to 1e-4 with the residual checks on the device, the extrapolation by
     tile diagonal), and the whole batched substep is captured into a HIP
     graph and replayed. The autotuner picks the P2G bin per grid size (on
-    one planet of each size), then the convolution form and the solve's
-    workgroup size for the batch as a whole.
+    one planet of each size), then the convolution form, the P2G bin and
+    the solve's workgroup size for the batch as a whole.
     value = particle updates per second summed over the planets."""
     import watersim_hip
     if not os.path.exists(a.weights):
@@ -609,6 +609,21 @@ def batched_main(a, cfg):
     net_form = min(net_trial, key=net_trial.get)
     for i, s in enumerate(planets):
         s.set(netTiled=tiling[sizes[i]] if net_form == "per size" else net_form)
+    # the P2G bin for the batch as a whole: each size's own choice (small
+    # bins keep one planet's GPU busy), or one bin for every planet (larger
+    # bins visit each particle from fewer tiles; the planets fill the GPU
+    # together)
+    bin_trial = {}
+    for choice in ["per size"] + list(range(4)):
+        for i, s in enumerate(planets):
+            s.set(p2gBin=bins[sizes[i]] if choice == "per size" else choice)
+        t1 = time.perf_counter()
+        batch.run(10)
+        bin_trial[choice] = (time.perf_counter() - t1) / 10
+        log("batch with p2gBin %s: %.3f ms per batched substep" % (choice, 1e3 * bin_trial[choice]))
+    bin_choice = min(bin_trial, key=bin_trial.get)
+    for i, s in enumerate(planets):
+        s.set(p2gBin=bins[sizes[i]] if bin_choice == "per size" else bin_choice)
     # the solve's workgroup size, and each planet's workgroups on one XCD
     # (L2-local barriers) or dealt over all XCDs (agent-scope barriers)
     trial = {}
@@ -650,7 +665,7 @@ def batched_main(a, cfg):
         idx = [i for i in range(P) if sizes[i] == n]
         per_size["%d^3" % n] = {"planets": len(idx), "particles_mean": float(np.mean([nPs[i] for i in idx])),
                                 "netTiled": tiling[n] if net_form == "per size" else net_form,
-                                "p2gBin": bins[n]}
+                                "p2gBin": bins[n] if bin_choice == "per size" else bin_choice}
     out = {
         "metric": "aggregate particle-updates/s, %d batched planets (32^3..96^3, learned warm start + SOR to 1e-4)" % P,
         "value": updates / el,
@@ -665,13 +680,14 @@ def batched_main(a, cfg):
                                "planets (the planets' own stages merged from their recorded launches, the "
                                "network's steps, all pressure solves with workgroups per planet by size, the "
                                "extrapolation by tile diagonal), the batched substep replayed from a HIP graph; "
-                               "autotuned P2G bin per size, convolution form and solve workgroup size for the "
-                               "batch" % (P, list(cfg["sizes"]), cfg["vorticity"]),
+                               "convolution form, P2G bin and solve workgroup size autotuned for the batch"
+                               % (P, list(cfg["sizes"]), cfg["vorticity"]),
                    "particles_total": sum(nPs), "per_size": per_size,
                    "planet_substep_ms": 1e3 * el / a.steps,
                    "solve_threads": threads, "xcd_local": xcd_local, "xcd_local_planets": local_planets,
                    "batch_trial_ms": {"%d threads, xcdLocal %d" % k: 1e3 * v for k, v in trial.items()},
                    "net_form_trial_ms": {"netTiled %s" % k: 1e3 * v for k, v in net_trial.items()},
+                   "p2g_bin_trial_ms": {"p2gBin %s" % k: 1e3 * v for k, v in bin_trial.items()},
                    "stage_launches": int(batch.get("stageLaunches")),
                    "joint_solve_ms": 1e3 * solve_s, "sweeps_per_planet_substep": sweeps_per},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
