@@ -4,6 +4,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cfloat>
+#include <chrono>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -132,6 +133,44 @@ WS_PLANETS_OPS(k_reset_counters_ops, k_reset_counters, k_reset_counters_body)
 __global__ void k_zero_u32(unsigned *p, long n) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
+}
+
+// The device scalars published into the pinned host mirror by a kernel: it
+// copies them, makes its writes visible system-wide, then stores the call's
+// sequence number into a pinned word that the host polls. This replaces the
+// runtime's copy and blocking wait (a copy launch, then a wake-up of tens of
+// microseconds) on the round trips of the to-residual solve and of every
+// substep's end. A wait longer than 2 ms falls back to blocking on the
+// stream, which also reports a failed launch.
+__global__ void k_publish_scalars(const DevScalars *__restrict__ ds, DevScalars *hs, unsigned *seqWord,
+                                  unsigned seq) {
+  static_assert(sizeof(DevScalars) % 4 == 0, "whole words");
+  constexpr int W = (int)(sizeof(DevScalars) / 4);
+  const unsigned *src = (const unsigned *)ds;
+  unsigned *dst = (unsigned *)hs;
+  for (int i = threadIdx.x; i < W; i += blockDim.x) dst[i] = src[i];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(seqWord, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+int read_scalars(ws_ctx *c) {
+  unsigned seq = ++c->hsPub;
+  if (seq == 0) seq = ++c->hsPub;
+  hipLaunchKernelGGL(k_publish_scalars, dim3(1), dim3(256), 0, c->stream, c->ds, c->hs, c->hsSeq, seq);
+  WS_HIP(hipGetLastError());
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned spins = 0;; spins++) {
+    if (__atomic_load_n(c->hsSeq, __ATOMIC_ACQUIRE) == seq) return WS_OK;
+    if ((spins & 255u) == 255u && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) break;
+    __builtin_ia32_pause();
+  }
+  WS_HIP(hipStreamSynchronize(c->stream));
+  if (__atomic_load_n(c->hsSeq, __ATOMIC_ACQUIRE) != seq) {
+    c->err = "the device scalars were not published";
+    return WS_EHIP;
+  }
+  return WS_OK;
 }
 
 // ------------------------------------------------------------------ stages
@@ -512,8 +551,7 @@ int sor_solve_dev(ws_ctx *c) {
 // counters (and the last solve's sweeps and residual)
 int fold_dev_stats(ws_ctx *c) {
   if (!c->devStatsPending) return WS_OK;
-  WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
-  WS_HIP(hipStreamSynchronize(c->stream));
+  if (int r = read_scalars(c)) return r;
   c->devStatsPending = false;
   if (c->hs->sorRes < 0) {
     c->err = "on-device pressure solve: its workgroups' barrier timed out (not all resident)";
@@ -545,8 +583,7 @@ int residual(ws_ctx *c, double *res) {
                        c->partials);
   hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, c->stream, c->partials, c->nPartials, c->ds);
   WS_LAUNCHED();
-  WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
-  WS_HIP(hipStreamSynchronize(c->stream));
+  if (int r = read_scalars(c)) return r;
   take_counts(c);
   if (int e = device_failures(c)) return e;
   int redo = 0, r = xcd_confirm(c, &redo);
@@ -594,8 +631,7 @@ int xcd_confirm(ws_ctx *c, int *redo) {
 // (paths that sweep without a residual check afterwards)
 int xcd_settle(ws_ctx *c) {
   if (!c->xcdPending) return WS_OK;
-  WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
-  WS_HIP(hipStreamSynchronize(c->stream));
+  if (int r = read_scalars(c)) return r;
   int redo = 0, r = xcd_confirm(c, &redo);
   if (r) return r;
   return redo ? sor_sweeps_list(c, redo) : WS_OK;
@@ -689,8 +725,7 @@ int pressure_setup(ws_ctx *c) {
                  c->sorHalf, c->sorCode, c->sorRhs, c->sorPw, c->sorDg, c->ds);
     WS_LAUNCHED();
     if (c->pressureTol <= 0 && !dev_solve(c)) {  // the to-residual path learns the counts with its first check
-      WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
-      WS_HIP(hipStreamSynchronize(c->stream));
+      if ((r = read_scalars(c))) return r;
       take_counts(c);
     }
   }
@@ -969,8 +1004,7 @@ int substep_post2(ws_ctx *c) {
 }
 
 int pull_scalars(ws_ctx *c) {
-  WS_HIP(hipMemcpyAsync(c->hs, c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
-  WS_HIP(hipStreamSynchronize(c->stream));
+  if (int r = read_scalars(c)) return r;
   if (int r = device_failures(c)) return r;
   c->umax = __builtin_bit_cast(float, c->hs->umax_bits);
   c->airborne = (int)c->hs->airborne;
@@ -1131,7 +1165,13 @@ int ws_create(ws_ctx **out, int nx, int ny, int nz, double dx, int capacity) {
   }
   c->gv = c->gu + c->uN;
   c->gw = c->gu + c->uN + c->vN;
-  if (hipHostMalloc((void **)&c->hs, sizeof(DevScalars), 0) != hipSuccess) { ws_destroy(c); return WS_EHIP; }
+  // coherent: k_publish_scalars writes both while the host polls
+  if (hipHostMalloc((void **)&c->hs, sizeof(DevScalars), hipHostMallocCoherent) != hipSuccess ||
+      hipHostMalloc((void **)&c->hsSeq, 64, hipHostMallocCoherent) != hipSuccess) {
+    ws_destroy(c);
+    return WS_EHIP;
+  }
+  *c->hsSeq = 0;
   c->nPartials = 1024;
   if (dalloc(c, &c->partials, 2 * c->nPartials + 2048)) { ws_destroy(c); return WS_EHIP; }
   // scan scratch: enough for the larger of the cell and particle scans
@@ -1244,6 +1284,7 @@ void ws_destroy(ws_ctx *c) {
   if (c->scanStatus) (void)hipFree(c->scanStatus);
   if (c->scanCtl) (void)hipFree(c->scanCtl);
   if (c->hs) (void)hipHostFree(c->hs);
+  if (c->hsSeq) (void)hipHostFree(c->hsSeq);
   if (c->renderHost) (void)hipHostFree(c->renderHost);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);

@@ -349,6 +349,10 @@ struct Ctx {
   // reductions
   DevScalars *ds = nullptr;     // device
   DevScalars *hs = nullptr;     // pinned host mirror
+  // pinned host word: the sequence number of the last scalars published
+  // into hs by k_publish_scalars (hsPub: the last one requested)
+  unsigned *hsSeq = nullptr;
+  unsigned hsPub = 0;
   double *partials = nullptr;   // residual block partials
   int nPartials = 0;
   void *scanTmp = nullptr;
