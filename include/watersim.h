@@ -226,7 +226,8 @@ int ws_read_render(ws_ctx *c, unsigned mask, ws_render_view *view);
  * launch that finds its workgroups off one XCD or its ranges too large
  * returns before writing, and the same sweeps run on the list kernel) and,
  * with the warm
- * start on, the untiled and LDS-tiled convolutions, and the P2G bins
+ * start on, the three forms of the convolutions ("netTiled" 0, 1, 2), and
+ * the P2G bins
  * (8x8x8, 4x4x4, 16x8x8, 8x8x4 cells per tile; the sums are the same for
  * every bin), and keeps the fastest ("sorMode", "sorThreads", "netTiled",
  * "p2gBin"). The pressure, the faces and all counters are left as they

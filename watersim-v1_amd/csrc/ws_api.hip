@@ -1589,11 +1589,12 @@ int ws_tune(ws_ctx *c, int sweeps, char *report, size_t len) {
   }
   c->sorMode = best;
   c->sorThreads = bestThreads;
-  // the warm start's convolutions: untiled or LDS-tiled
+  // the warm start's convolutions: untiled, LDS-tiled where the grid fills
+  // the GPU (split K elsewhere), or LDS-tiled on every level
   int bestTiled = keepTiled;
   if (!r && keepWarm && c->net) {
     float bt = 1e30f;
-    for (int tiled = 0; tiled <= 1 && !r; tiled++) {
+    for (int tiled = 0; tiled <= 2 && !r; tiled++) {
       c->netTiled = tiled;
       c->warmStart = 1;
       float ms = 0;
