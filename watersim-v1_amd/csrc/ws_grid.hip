@@ -9,8 +9,8 @@ namespace ws {
 // ---------------------------------------------------------------- scalars
 // dt and the gravity ramp of this substep (solver.js:2656-2662); mode 1 takes
 // dt from the CFL law of _stepSync (solver.js:2689-2690) on the device.
-__device__ __forceinline__ void k_begin_substep_body(long bx, long nbx, DevScalars *s, double dt_host, int use_law, double cfl_dx, double dt_base,
-                                                     float gravity) {
+__device__ __forceinline__ void k_begin_substep_body(long bx, long nbx, DevScalars *s, double dt_host, int use_law,
+                                                     double cfl_dx, double dt_base, float gravity) {
   if (threadIdx.x != 0) return;
   // the length is a double, as in the reference; the float copy feeds the
   // float kernels
@@ -46,8 +46,8 @@ WS_PLANETS_OPS(k_end_substep_ops, k_end_substep, k_end_substep_body)
 
 // ------------------------------------------------------------- rasterize
 // _rasterizeStatic (cached copy, solver.js:868-870) and zeroed bin counts
-__device__ __forceinline__ void k_init_cells_body(long bx, long nbx, const uint8_t *__restrict__ st, uint8_t *__restrict__ ct, unsigned *__restrict__ cnt,
-                                                  int nC) {
+__device__ __forceinline__ void k_init_cells_body(long bx, long nbx, const uint8_t *__restrict__ st,
+                                                  uint8_t *__restrict__ ct, unsigned *__restrict__ cnt, int nC) {
   int c = bx * blockDim.x + threadIdx.x;
   if (c >= nC) return;
   ct[c] = st[c];
@@ -112,10 +112,11 @@ __device__ int ceiling_demote(const World &w, const Geo &g, float x, float y, fl
 
 // _markFluidCells, solver.js:945-1000, plus the counting-sort histogram of
 // the particles that stay grid-coupled (their bin is the cell they mark).
-__device__ __forceinline__ void k_mark_body(long bx, long nbx, Geo g, World w, int nP, const float *__restrict__ px, const float *__restrict__ py,
-                                            const float *__restrict__ pz, uint8_t *__restrict__ pflag, uint8_t *__restrict__ ct,
-                                            unsigned *__restrict__ cnt, int *__restrict__ pcell, unsigned *__restrict__ prank,
-                                            DevScalars *s) {
+__device__ __forceinline__ void k_mark_body(long bx, long nbx, Geo g, World w, int nP, const float *__restrict__ px,
+                                            const float *__restrict__ py, const float *__restrict__ pz,
+                                            uint8_t *__restrict__ pflag, uint8_t *__restrict__ ct,
+                                            unsigned *__restrict__ cnt, int *__restrict__ pcell,
+                                            unsigned *__restrict__ prank, DevScalars *s) {
   int p = bx * blockDim.x + threadIdx.x;
   if (p >= nP) return;
   if (pflag[p] != 0) { pcell[p] = -1; return; }
@@ -140,8 +141,9 @@ __global__ void k_mark(Geo g, World w, int nP, const float *__restrict__ px, con
 }
 WS_PLANETS_OPS(k_mark_ops, k_mark, k_mark_body)
 
-__device__ __forceinline__ void k_scatter_body(long bx, long nbx, int nP, const int *__restrict__ pcell, const unsigned *__restrict__ prank,
-                                               const unsigned *__restrict__ start, unsigned *__restrict__ sorted) {
+__device__ __forceinline__ void k_scatter_body(long bx, long nbx, int nP, const int *__restrict__ pcell,
+                                               const unsigned *__restrict__ prank, const unsigned *__restrict__ start,
+                                               unsigned *__restrict__ sorted) {
   int p = bx * blockDim.x + threadIdx.x;
   if (p >= nP) return;
   int c = pcell[p];
@@ -472,10 +474,12 @@ __device__ __forceinline__ float solid_face_value(const Geo &g, const FaceInfo &
   return tl == SOLID ? a[fi.cl] : a[fi.cr];
 }
 
-__device__ __forceinline__ void k_bc_snapshot_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct, float *__restrict__ vel,
-                                                   float *__restrict__ velO, const float *__restrict__ wgt,
-                                                   uint8_t *__restrict__ valid, int fuse_gravity, const float *__restrict__ gvec,
-                                                   const DevScalars *__restrict__ s, int write_valid, const float *__restrict__ sv) {
+__device__ __forceinline__ void k_bc_snapshot_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct,
+                                                   float *__restrict__ vel, float *__restrict__ velO,
+                                                   const float *__restrict__ wgt, uint8_t *__restrict__ valid,
+                                                   int fuse_gravity, const float *__restrict__ gvec,
+                                                   const DevScalars *__restrict__ s, int write_valid,
+                                                   const float *__restrict__ sv) {
   long F = (long)g.uN + g.vN + g.wN;
   long f = (long)bx * blockDim.x + threadIdx.x;
   if (f >= F) return;
@@ -511,9 +515,9 @@ __global__ void k_valid(Geo g, const uint8_t *__restrict__ ct, const float *__re
 }
 
 // radial gravity + _applyBC after viscosity
-__device__ __forceinline__ void k_gravity_bc_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct, float *__restrict__ vel,
-                                                  const float *__restrict__ gvec, const DevScalars *__restrict__ s,
-                                                  const float *__restrict__ sv) {
+__device__ __forceinline__ void k_gravity_bc_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct,
+                                                  float *__restrict__ vel, const float *__restrict__ gvec,
+                                                  const DevScalars *__restrict__ s, const float *__restrict__ sv) {
   long F = (long)g.uN + g.vN + g.wN;
   long f = (long)bx * blockDim.x + threadIdx.x;
   if (f >= F) return;
@@ -539,8 +543,9 @@ WS_PLANETS_OPS(k_gravity_bc_ops, k_gravity_bc, k_gravity_bc_body)
 // x(m+1) replaces the lower neighbours by x(m). The error shrinks by 3*coef
 // per iteration, so a few iterations reach float resolution (the host picks
 // the count from coef).
-__device__ __forceinline__ void k_visc_iter_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ old,
-                                                 const float *__restrict__ cur, float *__restrict__ out, float nu_dx2,
+__device__ __forceinline__ void k_visc_iter_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct,
+                                                 const float *__restrict__ old, const float *__restrict__ cur,
+                                                 float *__restrict__ out, float nu_dx2,
                                                  const DevScalars *__restrict__ s, int first) {
   const float coef = fminf(nu_dx2 * s->dt, 0.24f);  // solver.js:1264
   long F = (long)g.uN + g.vN + g.wN;
@@ -597,10 +602,10 @@ WS_PLANETS_OPS(k_visc_iter_ops, k_visc_iter, k_visc_iter_body)
 // _pressureSolve assembly, solver.js:1322-1394: rhs = -dx^2 div, diag =
 // fluid + air neighbours, relaxation weight omega/diag (stored float as the
 // reference's pW), fluid-neighbour mask; non-fluid cells hold q = 0.
-__device__ __forceinline__ void k_assemble_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ vel,
-                                                float *__restrict__ q, float *__restrict__ q2, float *__restrict__ rhs,
-                                                float *__restrict__ pw, uint8_t *__restrict__ nbm, uint8_t *__restrict__ dg,
-                                                float omega) {
+__device__ __forceinline__ void k_assemble_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct,
+                                                const float *__restrict__ vel, float *__restrict__ q,
+                                                float *__restrict__ q2, float *__restrict__ rhs, float *__restrict__ pw,
+                                                uint8_t *__restrict__ nbm, uint8_t *__restrict__ dg, float omega) {
   int c = bx * blockDim.x + threadIdx.x;
   if (c >= g.nC) return;
   if (ct[c] != FLUID) { q[c] = 0.f; q2[c] = 0.f; pw[c] = 0.f; nbm[c] = 0; dg[c] = 0; return; }
@@ -756,7 +761,8 @@ __global__ __launch_bounds__(256) void k_sor_tile(Geo g, int tilesX, int tilesY,
 
 // scan keys: 1 for a red fluid cell, 2^32 for a black one (one 64-bit
 // exclusive scan then gives both list positions)
-__device__ __forceinline__ void k_sor_count_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ nbm, unsigned long long *__restrict__ key) {
+__device__ __forceinline__ void k_sor_count_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ nbm,
+                                                 unsigned long long *__restrict__ key) {
   int c = bx * blockDim.x + threadIdx.x;
   if (c > g.nC) return;
   unsigned long long v = 0;
@@ -771,11 +777,12 @@ __global__ void k_sor_count(Geo g, const uint8_t *__restrict__ nbm, unsigned lon
 }
 WS_PLANETS_OPS(k_sor_count_ops, k_sor_count, k_sor_count_body)
 
-__device__ __forceinline__ void k_sor_compact_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ nbm, const uint8_t *__restrict__ dg,
-                                                   const float *__restrict__ rhs, const float *__restrict__ pw,
-                                                   const unsigned long long *__restrict__ off, long half, unsigned *__restrict__ code,
-                                                   float *__restrict__ lrhs, float *__restrict__ lpw, uint8_t *__restrict__ ldg,
-                                                   DevScalars *ds) {
+__device__ __forceinline__ void k_sor_compact_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ nbm,
+                                                   const uint8_t *__restrict__ dg, const float *__restrict__ rhs,
+                                                   const float *__restrict__ pw,
+                                                   const unsigned long long *__restrict__ off, long half,
+                                                   unsigned *__restrict__ code, float *__restrict__ lrhs,
+                                                   float *__restrict__ lpw, uint8_t *__restrict__ ldg, DevScalars *ds) {
   int c = bx * blockDim.x + threadIdx.x;
   if (c > g.nC) return;
   if (c == g.nC) {
@@ -1632,8 +1639,9 @@ __device__ __forceinline__ unsigned abs_bits(float u) {
   return (av == av) ? __float_as_uint(av) : 0u;
 }
 
-__device__ __forceinline__ void k_project_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ phi,
-                                               float *__restrict__ vel, int do_clamp, float vcap, DevScalars *s) {
+__device__ __forceinline__ void k_project_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct,
+                                               const float *__restrict__ phi, float *__restrict__ vel, int do_clamp,
+                                               float vcap, DevScalars *s) {
   long F = (long)g.uN + g.vN + g.wN;
   unsigned bits = 0;
   for (long f = (long)bx * blockDim.x + threadIdx.x; f < F; f += (long)nbx * blockDim.x) {
@@ -1661,7 +1669,8 @@ __global__ __launch_bounds__(256) void k_project(Geo g, const uint8_t *__restric
 }
 WS_PLANETS_OPS(k_project_ops, k_project, k_project_body)
 
-__device__ __forceinline__ void k_clamp_body(long bx, long nbx, long F, float *__restrict__ vel, float vcap, DevScalars *s) {
+__device__ __forceinline__ void k_clamp_body(long bx, long nbx, long F, float *__restrict__ vel, float vcap,
+                                             DevScalars *s) {
   unsigned bits = 0;
   for (long f = (long)bx * blockDim.x + threadIdx.x; f < F; f += (long)nbx * blockDim.x) {
     float u = vel[f];
@@ -1680,7 +1689,8 @@ WS_PLANETS_OPS(k_clamp_ops, k_clamp, k_clamp_body)
 // _vorticityConfinement, solver.js:1454-1559, in three cell passes and one
 // face pass. scratch layout: vc[3C] centred velocity, om[4C] (x,y,z,|w|),
 // fc[3C] velocity deltas of interior fluid cells.
-__device__ __forceinline__ void k_vort_center_body(long bx, long nbx, Geo g, const float *__restrict__ vel, float *__restrict__ vc) {
+__device__ __forceinline__ void k_vort_center_body(long bx, long nbx, Geo g, const float *__restrict__ vel,
+                                                   float *__restrict__ vc) {
   int c = bx * blockDim.x + threadIdx.x;
   if (c >= g.nC) return;
   int i = c % g.nx, j = (c / g.nx) % g.ny, k = c / (g.nx * g.ny);
@@ -1695,7 +1705,8 @@ __global__ void k_vort_center(Geo g, const float *__restrict__ vel, float *__res
 }
 WS_PLANETS_OPS(k_vort_center_ops, k_vort_center, k_vort_center_body)
 
-__device__ __forceinline__ void k_vort_omega_body(long bx, long nbx, Geo g, const float *__restrict__ vc, float *__restrict__ om, float inv2) {
+__device__ __forceinline__ void k_vort_omega_body(long bx, long nbx, Geo g, const float *__restrict__ vc,
+                                                  float *__restrict__ om, float inv2) {
   int c = bx * blockDim.x + threadIdx.x;
   if (c >= g.nC) return;
   int i = c % g.nx, j = (c / g.nx) % g.ny, k = c / (g.nx * g.ny), nxy = g.nx * g.ny;
@@ -1716,8 +1727,9 @@ __global__ void k_vort_omega(Geo g, const float *__restrict__ vc, float *__restr
 }
 WS_PLANETS_OPS(k_vort_omega_ops, k_vort_omega, k_vort_omega_body)
 
-__device__ __forceinline__ void k_vort_force_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ om,
-                                                  float *__restrict__ fc, float eps, const DevScalars *__restrict__ s) {
+__device__ __forceinline__ void k_vort_force_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct,
+                                                  const float *__restrict__ om, float *__restrict__ fc, float eps,
+                                                  const DevScalars *__restrict__ s) {
   int c = bx * blockDim.x + threadIdx.x;
   if (c >= g.nC) return;
   if (ct[c] != FLUID) return;
@@ -1754,8 +1766,8 @@ __global__ void k_vort_force(Geo g, const uint8_t *__restrict__ ct, const float 
 }
 WS_PLANETS_OPS(k_vort_force_ops, k_vort_force, k_vort_force_body)
 
-__device__ __forceinline__ void k_vort_splat_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct, const float *__restrict__ fc,
-                                                  float *__restrict__ vel) {
+__device__ __forceinline__ void k_vort_splat_body(long bx, long nbx, Geo g, const uint8_t *__restrict__ ct,
+                                                  const float *__restrict__ fc, float *__restrict__ vel) {
   long F = (long)g.uN + g.vN + g.wN;
   long f = (long)bx * blockDim.x + threadIdx.x;
   if (f >= F) return;

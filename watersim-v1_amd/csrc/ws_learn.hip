@@ -655,9 +655,10 @@ __global__ void k_crop(NetDims full, NetDims cd, int oz, int oy, int ox, const b
 // ---- the solver side: inputs from the pressure system, the correction
 // initial residual r0 = rhs - A q over fluid cells and FP64 sums for the
 // two scales (sum r0^2, sum q^2, fluid count)
-__device__ __forceinline__ void k_ws_residual_body(long bx, long nbx, NetDims d, const uint8_t *__restrict__ ct, const uint8_t *__restrict__ nbm,
-                                                   const uint8_t *__restrict__ dg, const float *__restrict__ rhs,
-                                                   const float *__restrict__ q, float *__restrict__ r0, double *__restrict__ part) {
+__device__ __forceinline__ void k_ws_residual_body(long bx, long nbx, NetDims d, const uint8_t *__restrict__ ct,
+                                                   const uint8_t *__restrict__ nbm, const uint8_t *__restrict__ dg,
+                                                   const float *__restrict__ rhs, const float *__restrict__ q,
+                                                   float *__restrict__ r0, double *__restrict__ part) {
   __shared__ double sa[256], sb[256], sc[256];
   double a = 0, b = 0, n = 0;
   long C = (long)d.Z * d.Y * d.X;
@@ -715,8 +716,9 @@ __global__ __launch_bounds__(256) void k_ws_scales(int nb, const double *__restr
 }
 WS_PLANETS_OPS(k_ws_scales_ops, k_ws_scales, k_ws_scales_body)
 // the five input channels [r0/s, air, fluid, solid, q/sq] as bf16 [V][8]
-__device__ __forceinline__ void k_ws_inputs_body(long bx, long nbx, long C, const uint8_t *__restrict__ ct, const float *__restrict__ r0,
-                                                 const float *__restrict__ q, const float *__restrict__ scales, bf16_t *__restrict__ x) {
+__device__ __forceinline__ void k_ws_inputs_body(long bx, long nbx, long C, const uint8_t *__restrict__ ct,
+                                                 const float *__restrict__ r0, const float *__restrict__ q,
+                                                 const float *__restrict__ scales, bf16_t *__restrict__ x) {
   long c = (long)bx * blockDim.x + threadIdx.x;
   if (c >= C) return;
   uint8_t t = ct[c];
@@ -734,9 +736,9 @@ WS_PLANETS_OPS(k_ws_inputs_ops, k_ws_inputs, k_ws_inputs_body)
 // the five input channels of the grid coarsened 2x per axis: each coarse
 // cell averages its eight cells' full-resolution channels (r0/s and q/sq on
 // fluid cells, 0 elsewhere; the type one-hots, so types become fractions)
-__device__ __forceinline__ void k_ws_inputs_coarse_body(long bx, long nbx, NetDims cd, const uint8_t *__restrict__ ct, const float *__restrict__ r0,
-                                                        const float *__restrict__ q, const float *__restrict__ scales,
-                                                        bf16_t *__restrict__ x) {
+__device__ __forceinline__ void k_ws_inputs_coarse_body(long bx, long nbx, NetDims cd, const uint8_t *__restrict__ ct,
+                                                        const float *__restrict__ r0, const float *__restrict__ q,
+                                                        const float *__restrict__ scales, bf16_t *__restrict__ x) {
   long v = (long)bx * blockDim.x + threadIdx.x;
   long V = (long)cd.Z * cd.Y * cd.X;
   if (v >= V) return;

@@ -311,7 +311,8 @@ __global__ __launch_bounds__(256) void k_g2p_advect(Geo g, World w, Par P, int n
                                                     uint8_t *__restrict__ pcool, float *__restrict__ pAir,
                                                     float *__restrict__ advU, float *__restrict__ advV,
                                                     float *__restrict__ advW, DevScalars *s) {
-  k_g2p_advect_body(blockIdx.x, gridDim.x, g, w, P, nP, mode, vel, velO, ct, px, py, pz, pvx, pvy, pvz, pflag, pcool, pAir, advU, advV, advW, s);
+  k_g2p_advect_body(blockIdx.x, gridDim.x, g, w, P, nP, mode, vel, velO, ct, px, py, pz, pvx, pvy, pvz, pflag, pcool,
+                    pAir, advU, advV, advW, s);
 }
 WS_PLANETS_OPS(k_g2p_advect_ops, k_g2p_advect, k_g2p_advect_body)
 
