@@ -421,24 +421,26 @@ struct FaceInfo {
   bool plane;  // domain boundary plane
 };
 
+// (face indices are below 2^31, so the decoding divides in 32-bit unsigned
+// arithmetic: a 64-bit division is a long software routine on the GPU)
 __device__ __forceinline__ bool face_of(const Geo &g, long f, FaceInfo &fi) {
+  const unsigned nx = (unsigned)g.nx, ny = (unsigned)g.ny;
   if (f < g.uN) {
+    const unsigned u = (unsigned)f, r = u / (nx + 1u);
     fi.comp = 0;
-    fi.i = (int)(f % (g.nx + 1));
-    long r = f / (g.nx + 1);
-    fi.j = (int)(r % g.ny);
-    fi.k = (int)(r / g.ny);
+    fi.i = (int)(u - r * (nx + 1u));
+    fi.j = (int)(r % ny);
+    fi.k = (int)(r / ny);
     fi.plane = fi.i == 0 || fi.i == g.nx;
     int c = (fi.k * g.ny + fi.j) * g.nx + fi.i;
     fi.cl = fi.i > 0 ? c - 1 : -1;
     fi.cr = fi.i < g.nx ? c : -1;
   } else if (f < (long)g.uN + g.vN) {
-    f -= g.uN;
+    const unsigned u = (unsigned)(f - g.uN), r = u / nx;
     fi.comp = 1;
-    fi.i = (int)(f % g.nx);
-    long r = f / g.nx;
-    fi.j = (int)(r % (g.ny + 1));
-    fi.k = (int)(r / (g.ny + 1));
+    fi.i = (int)(u - r * nx);
+    fi.j = (int)(r % (ny + 1u));
+    fi.k = (int)(r / (ny + 1u));
     fi.plane = fi.j == 0 || fi.j == g.ny;
     int c = (fi.k * g.ny + fi.j) * g.nx + fi.i;
     fi.cl = fi.j > 0 ? c - g.nx : -1;
@@ -446,11 +448,11 @@ __device__ __forceinline__ bool face_of(const Geo &g, long f, FaceInfo &fi) {
   } else {
     f -= (long)g.uN + g.vN;
     if (f >= g.wN) return false;
+    const unsigned u = (unsigned)f, r = u / nx;
     fi.comp = 2;
-    fi.i = (int)(f % g.nx);
-    long r = f / g.nx;
-    fi.j = (int)(r % g.ny);
-    fi.k = (int)(r / g.ny);
+    fi.i = (int)(u - r * nx);
+    fi.j = (int)(r % ny);
+    fi.k = (int)(r / ny);
     fi.plane = fi.k == 0 || fi.k == g.nz;
     int c = (fi.k * g.ny + fi.j) * g.nx + fi.i;
     fi.cl = fi.k > 0 ? c - g.nx * g.ny : -1;
