@@ -154,11 +154,22 @@ __global__ void k_publish_scalars(const DevScalars *__restrict__ ds, DevScalars 
   if (threadIdx.x == 0) __hip_atomic_store(seqWord, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-int read_scalars(ws_ctx *c) {
+unsigned next_publication(ws_ctx *c) {
   unsigned seq = ++c->hsPub;
-  if (seq == 0) seq = ++c->hsPub;
+  return seq ? seq : ++c->hsPub;
+}
+
+int wait_scalars(ws_ctx *c, unsigned seq);
+
+int read_scalars(ws_ctx *c) {
+  const unsigned seq = next_publication(c);
   hipLaunchKernelGGL(k_publish_scalars, dim3(1), dim3(256), 0, c->stream, c->ds, c->hs, c->hsSeq, seq);
   WS_HIP(hipGetLastError());
+  return wait_scalars(c, seq);
+}
+
+// the wait for publication seq (queued on the context's stream)
+int wait_scalars(ws_ctx *c, unsigned seq) {
   const auto t0 = std::chrono::steady_clock::now();
   for (unsigned spins = 0;; spins++) {
     if (__atomic_load_n(c->hsSeq, __ATOMIC_ACQUIRE) == seq) return WS_OK;
@@ -581,9 +592,12 @@ int residual(ws_ctx *c, double *res) {
   else
     hipLaunchKernelGGL(k_residual, dim3(c->nPartials), dim3(256), 0, c->stream, c->geo, c->q, c->rhs, c->nbm, c->dg,
                        c->partials);
-  hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, c->stream, c->partials, c->nPartials, c->ds);
+  // the final sum publishes the scalars to the host mirror itself
+  const unsigned seq = next_publication(c);
+  hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, c->stream, c->partials, c->nPartials, c->ds, c->hs,
+                     c->hsSeq, seq);
   WS_LAUNCHED();
-  if (int r = read_scalars(c)) return r;
+  if (int r = wait_scalars(c, seq)) return r;
   take_counts(c);
   if (int e = device_failures(c)) return e;
   int redo = 0, r = xcd_confirm(c, &redo);

@@ -1130,7 +1130,11 @@ __global__ __launch_bounds__(256) void k_residual(Geo g, const float *__restrict
 }
 
 // fixed-order sum of the block partials (deterministic)
-__global__ void k_residual_final(const double *__restrict__ partials, int n, DevScalars *s) {
+// With hs set, the scalars are also published to the pinned host mirror
+// hs (as k_publish_scalars does): the other words copied from s, the two
+// sums from LDS (each word written once), then the sequence word.
+__global__ void k_residual_final(const double *__restrict__ partials, int n, DevScalars *s, DevScalars *hs,
+                                 unsigned *seqWord, unsigned seq) {
   __shared__ double sr[256], sb[256];
   double rr = 0, bb = 0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) { rr += partials[2 * i]; bb += partials[2 * i + 1]; }
@@ -1141,7 +1145,25 @@ __global__ void k_residual_final(const double *__restrict__ partials, int n, Dev
     if (threadIdx.x < o) { sr[threadIdx.x] += sr[threadIdx.x + o]; sb[threadIdx.x] += sb[threadIdx.x + o]; }
     __syncthreads();
   }
-  if (threadIdx.x == 0) { s->rr = sr[0]; s->bb = sb[0]; }
+  if (!hs) {
+    if (threadIdx.x == 0) { s->rr = sr[0]; s->bb = sb[0]; }
+    return;
+  }
+  constexpr int W = (int)(sizeof(DevScalars) / 4);
+  constexpr int RR = (int)(offsetof(DevScalars, rr) / 4), BB = (int)(offsetof(DevScalars, bb) / 4);
+  const unsigned *src = (const unsigned *)s;
+  unsigned *dst = (unsigned *)hs;
+  for (int i = threadIdx.x; i < W; i += blockDim.x)
+    if ((i < RR || i >= RR + 2) && (i < BB || i >= BB + 2)) dst[i] = src[i];
+  if (threadIdx.x == 0) {
+    s->rr = sr[0];
+    s->bb = sb[0];
+    hs->rr = sr[0];
+    hs->bb = sb[0];
+  }
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(seqWord, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // ------------------------------------------------ on-device pressure solve

@@ -81,7 +81,8 @@ __global__ void k_sor_tb(Geo g, int tilesX, int tilesY, const uint8_t *active, c
 __global__ void k_sor_tb_build(Geo g, int tilesX, int tilesY, const uint8_t *active, const uint8_t *nbm,
                                const uint8_t *dg, const float *rhs, unsigned *pk, float *lrhs, int *cnt);
 __global__ void k_sor_tb_mark(Geo g, int tilesX, int tilesY, int nTiles, const uint8_t *nbm, uint8_t *active);
-__global__ void k_residual_final(const double *partials, int n, DevScalars *s);
+__global__ void k_residual_final(const double *partials, int n, DevScalars *s, DevScalars *hs, unsigned *seqWord,
+                                 unsigned seq);
 template <int NT>
 __global__ void k_sor_solve(SorJob J);
 template <int NT>
